@@ -1,0 +1,207 @@
+/*
+ * kdh.h -- C ABI of the MI355X scan-alignment library (libkdhip.so).
+ *
+ * The library replaces the GPU side of the reference's aligner plugins
+ * ICPCUDA3D / CTICPCUDA3D (rvp-group/kd_slam2).  A C++ adapter that mirrors the
+ * reference's TreeAlignerBase_ / ICP_ / CTICP_ interface (registered as the
+ * configurable types ICPHIP3D / CTICPHIP3D) calls these entry points; see
+ * kd_slam2_amd/adapter/.  Only plain pointers and sizes cross this boundary.
+ *
+ * Conventions
+ *  - Every call returns KDH_OK (0) or a negative KDH_ERR_* code and never
+ *    exits the process (the reference's CUDA_CHECK exits,
+ *    kd_slam2/src/lib/kd_slam/cuda/cuda_common.h:26-35).  kdh_last_error()
+ *    returns the message of the last failing call on that context.
+ *  - Poses cross the boundary as 3x4 row-major [R | t] arrays (float for the
+ *    FP32 per-iteration cache the reference forms in updateCache, double for
+ *    the FP64 state the device loop keeps).
+ *  - Quadratic forms come back as full symmetric row-major FP64 matrices:
+ *    6x6 for ICP, 12x12 [H_pp H_pv; H_pv^T H_vv] for CT-ICP, as assembled by
+ *    the reference (icp/icp_ws_item_.h:36-44, cticp/cticp_cpu_impl_.h:84-93).
+ *  - Calls on one context are serialised by the context; contexts are
+ *    independent.  One context owns one HIP stream on one device.
+ */
+#ifndef KDH_H_
+#define KDH_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define KDH_OK 0
+#define KDH_ERR_INVALID (-1)   /* bad argument or misuse */
+#define KDH_ERR_HIP (-2)       /* HIP runtime error */
+#define KDH_ERR_NOMEM (-3)     /* device or host allocation failed */
+#define KDH_ERR_BUDGET (-4)    /* request exceeds the serving memory budget */
+#define KDH_ERR_NOT_FOUND (-5) /* unknown request / handle */
+#define KDH_ERR_CANCELLED (-6) /* request cancelled before it completed */
+
+typedef struct kdh_ctx kdh_ctx;
+typedef struct kdh_tree kdh_tree;
+typedef struct kdh_server kdh_server;
+
+/* Memory layout of the reference's host node Node_<Point3fTraits>
+ * (kd_slam2/src/lib/kd_slam/tree/node_.h:12-43: TimestampInfo base, then
+ * _mean, _direction, _idx_start, _idx_end, _idx_left, _idx_right); 64 bytes.
+ * A TreeCPU_ node array can be passed to kdh_tree_upload as is. */
+typedef struct kdh_node3f {
+  double dts_mean;
+  double dts_covariance;
+  float mean[3];
+  float direction[3];
+  uint64_t idx_start;
+  uint64_t idx_end;
+  int32_t idx_left;
+  int32_t idx_right;
+} kdh_node3f;
+
+/* ICPParams_ / CTICPParams_ (icp/tree_aligner_base_.h:12-21,
+ * cticp/cticp_.h:14-18) plus the score termination criterion
+ * (icp/icp_stats_.h:85-102) and the Jacobian re-centring switch. */
+typedef struct kdh_params {
+  float kernel_threshold;      /* 0.1 */
+  float damping;               /* 1 */
+  float max_error;             /* 1 */
+  float min_normal_cos;        /* cos(25 deg) */
+  int32_t min_good_measurements; /* 10 */
+  int32_t max_iterations;      /* 20 */
+  float velocity_damping;      /* 100 (CT only) */
+  int32_t use_term_crit;       /* 0: run max_iterations */
+  float term_d_pert;           /* 1e-3 */
+  float term_d_score;          /* 1e-3 */
+  int32_t recenter;            /* 1: re-centred Jacobian (FP32 terms about the
+                                  scan centroid), 0: reference formula */
+} kdh_params;
+
+/* ICPStats_ (icp/icp_stats_.h:10-20). */
+typedef struct kdh_stats {
+  int32_t num_inliers;
+  int32_t num_outliers;
+  int32_t num_bad;
+  int32_t num_err_normal;
+  int32_t num_err_fn;
+  int32_t pad;
+  double chi2;
+  double chi2_kernelized;
+  double pert_pose_norm;
+  double pert_vel_norm;
+} kdh_stats;
+
+/* Pose prior of TreeAlignerBase_::setPosePrior (tree_aligner_base_.h:127-133). */
+typedef struct kdh_prior {
+  double X_ref[12];
+  double Z[12];
+  double info[36];
+} kdh_prior;
+
+/* Fills p with the reference defaults. */
+void kdh_params_default(kdh_params* p);
+
+/* ---- context ------------------------------------------------------------
+ * Replaces the CUDA library's load-time device set-up
+ * (kd_slam2/src/lib/kd_slam_cuda_init.cu:3-6). */
+int kdh_ctx_create(int device, kdh_ctx** out);
+int kdh_ctx_destroy(kdh_ctx* ctx);
+const char* kdh_last_error(const kdh_ctx* ctx);
+
+/* ---- trees --------------------------------------------------------------
+ * Replaces Tree_CUDAfromCPU_ (kd_slam2/src/lib/kd_slam/tree/tree_cuda_impl_.h:198-222)
+ * and the TreeCUDA_ destructor (:31-40).  nodes/leaf_indices are the
+ * TreeCPU_ arrays _nodes_ptr / _leaves_indices_ptr (tree_.h:27-33).  The tree
+ * is re-laid out for the device (32-byte node records, packed leaf records)
+ * and stays resident until kdh_tree_free. */
+int kdh_tree_upload(kdh_ctx* ctx, const kdh_node3f* nodes, int64_t n_nodes,
+                    const uint64_t* leaf_indices, int64_t n_leaves, kdh_tree** out);
+int kdh_tree_free(kdh_ctx* ctx, kdh_tree* tree);
+int kdh_tree_info(const kdh_tree* tree, int64_t* n_nodes, int64_t* n_leaves, int64_t* device_bytes);
+
+/* ---- one quadratic form (the plain per-call path) -------------------------
+ * Replaces ICP_CUDA_::_buildQuadraticForm (icp/icp_cuda_impl_.cuh:95-132):
+ * for every leaf of `moving`, descend `fixed`, apply the gates, and reduce
+ * the point-to-plane normal equations.  Xm, Xf, Xmf are the FP32
+ * QuadraticTermCache (icp/icp_.h:97-102) the caller formed in updateCache
+ * (icp/icp_impl_.h:16-21).  stats_mode as in quadraticTerm (outliers get
+ * weight 0).  H: 36 doubles, b: 6 doubles. */
+int kdh_icp_build(kdh_ctx* ctx, const kdh_tree* fixed, const kdh_tree* moving,
+                  const float Xm[12], const float Xf[12], const float Xmf[12],
+                  const kdh_params* params, int stats_mode,
+                  double H[36], double b[6], kdh_stats* stats);
+
+/* Replaces CTICP_CUDA_::_buildQuadraticForm (cticp/cticp_cuda_impl_.cuh:163-223):
+ * as kdh_icp_build with per-leaf deskew by the moving velocities v ([t; w],
+ * cticp/cticp_impl_.h:118-121) and the 12-DoF pose + velocity system.
+ * H: 144 doubles, b: 12 doubles. */
+int kdh_cticp_build(kdh_ctx* ctx, const kdh_tree* fixed, const kdh_tree* moving,
+                    const float Xm[12], const float Xf[12], const float Xmf[12],
+                    const float v[6], const kdh_params* params, int stats_mode,
+                    double H[144], double b[12], kdh_stats* stats);
+
+/* Parity hook: the fixed-tree leaf index each moving leaf descends to
+ * (-1 = num_bad) and the number of internal nodes visited, at the given
+ * cache.  The reference has no counterpart; it exposes the discrete part of
+ * quadraticTerm (icp_impl_.h:113-121, cticp_impl_.h:121-131) for checking.
+ * leaf_out / depth_out hold n_leaves(moving) int32 each (depth_out may be NULL). */
+int kdh_correspondences(kdh_ctx* ctx, const kdh_tree* fixed, const kdh_tree* moving, int ct,
+                        const float Xm[12], const float Xf[12], const float Xmf[12],
+                        const float v[6], const kdh_params* params,
+                        int32_t* leaf_out, int32_t* depth_out);
+
+/* ---- whole alignment on the device ----------------------------------------
+ * TreeAlignerBase_::compute (icp/tree_aligner_base_.h:100-115) with the
+ * Gauss-Newton loop, the FP64 damped solve and the SE(3) update resident on
+ * the device and the iterations replayed from a HIP graph.  X0/v0: initial
+ * moving pose (3x4) and velocities (CT), Xf: fixed pose.  prior may be NULL.
+ * stats_log receives one entry per executed iteration (params->max_iterations
+ * entries, or 1 in stats mode); *result is compute()'s return value (i on
+ * early stop, -i on a failed round, max_iterations otherwise).
+ * cache_log (optional, 3 x 12 floats per iteration) receives the FP32
+ * Xm, Xf, Xmf each iteration used, for parity checks. */
+int kdh_align(kdh_ctx* ctx, const kdh_tree* fixed, const kdh_tree* moving, int ct,
+              const double X0[12], const double v0[6], const double Xf[12],
+              const kdh_params* params, const kdh_prior* prior, int stats_mode,
+              double X_out[12], double v_out[6], kdh_stats* stats_log, int* n_log,
+              int* result, float* cache_log);
+
+/* ---- many alignments in one launch -------------------------------------- */
+typedef struct kdh_pair {
+  const kdh_tree* fixed;
+  const kdh_tree* moving;
+  int32_t ct;
+  int32_t pad;
+  double X0[12];
+  double v0[6];
+  double Xf[12];
+  kdh_params params;
+  const kdh_prior* prior;      /* may be NULL */
+} kdh_pair;
+
+typedef struct kdh_pair_result {
+  double X[12];
+  double v[6];
+  kdh_stats last;
+  int32_t iterations;
+  int32_t result;
+} kdh_pair_result;
+
+/* Runs every pair to completion: one persistent launch per Gauss-Newton
+ * iteration over the (pair, tile) list, captured as a HIP graph.  Each
+ * pair's result is bit-identical to running it alone through kdh_batch_align
+ * with n_pairs = 1. */
+int kdh_batch_align(kdh_ctx* ctx, int n_pairs, const kdh_pair* pairs, kdh_pair_result* results);
+
+/* Timing hook for benchmarks: average duration in microseconds of the tile
+ * kernel launches of the last kdh_align / kdh_batch_align call on ctx, from
+ * HIP events recorded on the context stream. */
+int kdh_last_kernel_time(kdh_ctx* ctx, double* avg_us, int* n_launches);
+
+/* Tuning knobs (project autotuner): workgroups per launch (0 = one per tile)
+ * and LDS-staged top-of-tree nodes (0/256/512/1024/2048). */
+int kdh_set_launch_config(kdh_ctx* ctx, int grid, int lds_nodes);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* KDH_H_ */

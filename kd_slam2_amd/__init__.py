@@ -1,0 +1,17 @@
+"""kd_slam2_amd: MI355X-native scan-alignment hot path of KD-SLAM.
+
+The GPU path lives in libkdhip.so (kd_slam2_amd/csrc, C ABI in include/kdh.h).
+This package holds the host-side mirror of the reference aligner interface
+(api.py), the seeded synthetic workloads (synth.py) and the build helper.
+"""
+from .api import (  # noqa: F401
+    CTICPHIP3D,
+    Context,
+    DeviceTree,
+    HostTree,
+    ICPHIP3D,
+    TreeGenerator3D,
+    batch_align,
+    default_params,
+    update_cache_f32,
+)

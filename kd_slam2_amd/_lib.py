@@ -1,0 +1,211 @@
+"""ctypes bindings of libkdhip.so (declarations in include/kdh.h, include/kdh_host.h).
+
+The library is built in-tree by ``kd_slam2_amd.build.build()``.  Loading fails
+loudly when it is missing: there is no CPU fallback for the device path.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libkdhip.so")
+
+KDH_OK = 0
+ERRORS = {
+    -1: "KDH_ERR_INVALID",
+    -2: "KDH_ERR_HIP",
+    -3: "KDH_ERR_NOMEM",
+    -4: "KDH_ERR_BUDGET",
+    -5: "KDH_ERR_NOT_FOUND",
+    -6: "KDH_ERR_CANCELLED",
+}
+
+
+class Node3f(C.Structure):
+    """Node_<Point3fTraits> layout (kd_slam2/src/lib/kd_slam/tree/node_.h:12-43)."""
+
+    _fields_ = [
+        ("dts_mean", C.c_double),
+        ("dts_covariance", C.c_double),
+        ("mean", C.c_float * 3),
+        ("direction", C.c_float * 3),
+        ("idx_start", C.c_uint64),
+        ("idx_end", C.c_uint64),
+        ("idx_left", C.c_int32),
+        ("idx_right", C.c_int32),
+    ]
+
+
+NODE_DTYPE = np.dtype(
+    [
+        ("dts_mean", "<f8"),
+        ("dts_covariance", "<f8"),
+        ("mean", "<f4", (3,)),
+        ("direction", "<f4", (3,)),
+        ("idx_start", "<u8"),
+        ("idx_end", "<u8"),
+        ("idx_left", "<i4"),
+        ("idx_right", "<i4"),
+    ]
+)
+assert NODE_DTYPE.itemsize == 64 and C.sizeof(Node3f) == 64
+
+
+class Params(C.Structure):
+    _fields_ = [
+        ("kernel_threshold", C.c_float),
+        ("damping", C.c_float),
+        ("max_error", C.c_float),
+        ("min_normal_cos", C.c_float),
+        ("min_good_measurements", C.c_int32),
+        ("max_iterations", C.c_int32),
+        ("velocity_damping", C.c_float),
+        ("use_term_crit", C.c_int32),
+        ("term_d_pert", C.c_float),
+        ("term_d_score", C.c_float),
+        ("recenter", C.c_int32),
+    ]
+
+
+class Stats(C.Structure):
+    _fields_ = [
+        ("num_inliers", C.c_int32),
+        ("num_outliers", C.c_int32),
+        ("num_bad", C.c_int32),
+        ("num_err_normal", C.c_int32),
+        ("num_err_fn", C.c_int32),
+        ("pad", C.c_int32),
+        ("chi2", C.c_double),
+        ("chi2_kernelized", C.c_double),
+        ("pert_pose_norm", C.c_double),
+        ("pert_vel_norm", C.c_double),
+    ]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_ if f != "pad"}
+
+
+class Prior(C.Structure):
+    _fields_ = [("X_ref", C.c_double * 12), ("Z", C.c_double * 12), ("info", C.c_double * 36)]
+
+
+class Pair(C.Structure):
+    _fields_ = [
+        ("fixed", C.c_void_p),
+        ("moving", C.c_void_p),
+        ("ct", C.c_int32),
+        ("pad", C.c_int32),
+        ("X0", C.c_double * 12),
+        ("v0", C.c_double * 6),
+        ("Xf", C.c_double * 12),
+        ("params", Params),
+        ("prior", C.c_void_p),
+    ]
+
+
+class PairResult(C.Structure):
+    _fields_ = [
+        ("X", C.c_double * 12),
+        ("v", C.c_double * 6),
+        ("last", Stats),
+        ("iterations", C.c_int32),
+        ("result", C.c_int32),
+    ]
+
+
+class TreeGenParams(C.Structure):
+    _fields_ = [
+        ("propagate_normal", C.c_int32),
+        ("tree_min_good_leaves", C.c_int32),
+        ("leaf_min_points", C.c_int32),
+        ("leaf_max_points", C.c_int32),
+        ("leaf_density", C.c_float),
+        ("leaf_min_normal_incidence", C.c_float),
+        ("leaf_eigenvalue_threshold", C.c_float),
+        ("leaf_normal_eigenratio", C.c_float),
+        ("parallel_level", C.c_int32),
+    ]
+
+
+_lib = None
+
+P = C.c_void_p
+F32P = C.POINTER(C.c_float)
+F64P = C.POINTER(C.c_double)
+I32P = C.POINTER(C.c_int32)
+
+_SIGS = {
+    "kdh_params_default": (None, [C.POINTER(Params)]),
+    "kdh_ctx_create": (C.c_int, [C.c_int, C.POINTER(P)]),
+    "kdh_ctx_destroy": (C.c_int, [P]),
+    "kdh_last_error": (C.c_char_p, [P]),
+    "kdh_tree_upload": (C.c_int, [P, P, C.c_int64, P, C.c_int64, C.POINTER(P)]),
+    "kdh_tree_free": (C.c_int, [P, P]),
+    "kdh_tree_info": (C.c_int, [P, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+    "kdh_icp_build": (C.c_int, [P, P, P, F32P, F32P, F32P, C.POINTER(Params), C.c_int, F64P, F64P, C.POINTER(Stats)]),
+    "kdh_cticp_build": (
+        C.c_int,
+        [P, P, P, F32P, F32P, F32P, F32P, C.POINTER(Params), C.c_int, F64P, F64P, C.POINTER(Stats)],
+    ),
+    "kdh_correspondences": (C.c_int, [P, P, P, C.c_int, F32P, F32P, F32P, F32P, C.POINTER(Params), I32P, I32P]),
+    "kdh_align": (
+        C.c_int,
+        [P, P, P, C.c_int, F64P, F64P, F64P, C.POINTER(Params), C.POINTER(Prior), C.c_int, F64P, F64P,
+         C.POINTER(Stats), C.POINTER(C.c_int), C.POINTER(C.c_int), F32P],
+    ),
+    "kdh_batch_align": (C.c_int, [P, C.c_int, C.POINTER(Pair), C.POINTER(PairResult)]),
+    "kdh_last_kernel_time": (C.c_int, [P, C.POINTER(C.c_double), C.POINTER(C.c_int)]),
+    "kdh_set_launch_config": (C.c_int, [P, C.c_int, C.c_int]),
+    "kdh_tree_gen_params_default": (None, [C.POINTER(TreeGenParams)]),
+    "kdh_host_tree_build": (C.c_int, [F32P, F64P, C.c_int64, C.POINTER(TreeGenParams), C.POINTER(P)]),
+    "kdh_host_tree_info": (C.c_int, [P, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_double)]),
+    "kdh_host_tree_nodes": (P, [P]),
+    "kdh_host_tree_leaves": (P, [P]),
+    "kdh_host_tree_points": (P, [P]),
+    "kdh_host_tree_free": (C.c_int, [P]),
+}
+
+# Entry points added after the first release of the ABI are bound when present.
+_OPTIONAL = set()
+
+
+def lib():
+    """Load libkdhip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"{LIB_PATH} is missing: build it with kd_slam2_amd.build.build() "
+                "(there is no CPU fallback for the HIP path)"
+            )
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            if name in _OPTIONAL and not hasattr(L, name):
+                continue
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def f32p(a):
+    return a.ctypes.data_as(F32P)
+
+
+def f64p(a):
+    return a.ctypes.data_as(F64P)
+
+
+def i32p(a):
+    return a.ctypes.data_as(I32P)
+
+
+def check(rc, ctx=None, what=""):
+    if rc != KDH_OK:
+        msg = ""
+        if ctx is not None:
+            m = lib().kdh_last_error(ctx)
+            msg = m.decode() if m else ""
+        raise RuntimeError(f"{what} failed: {ERRORS.get(rc, rc)}: {msg}")

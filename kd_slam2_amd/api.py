@@ -1,0 +1,462 @@
+"""Python host-side mirror of the reference aligner interface over the C ABI.
+
+Names, parameters and return conventions follow the reference types
+``TreeGenerator3D``, ``ICPCUDA3D`` / ``CTICPCUDA3D`` (TreeAlignerBase_,
+kd_slam2/src/lib/kd_slam/icp/tree_aligner_base_.h:26-161; ICP_,
+icp/icp_.h:53-161; CTICP_, cticp/cticp_.h:20-261), exposed here as
+``ICPHIP3D`` / ``CTICPHIP3D``.  Every computation runs in libkdhip.so on the
+GPU; this module only marshals arrays.
+"""
+import ctypes as C
+import math
+
+import numpy as np
+
+from . import _lib as L
+
+# ---------------------------------------------------------------------------
+# Trees
+
+
+class HostTree:
+    """A built PCA KD-tree: Node_ array (NODE_DTYPE), leaf-index list, the
+    builder's permuted points and the global start stamp (Tree_ fields
+    _nodes_ptr, _leaves_indices_ptr, _points_ptr, _ts_start; tree/tree_.h:27-37)."""
+
+    def __init__(self, nodes, leaves, points=None, ts_start=0.0):
+        self.nodes = np.ascontiguousarray(nodes, dtype=L.NODE_DTYPE)
+        self.leaves = np.ascontiguousarray(leaves, dtype=np.uint64)
+        self.points = points
+        self.ts_start = ts_start
+
+    @property
+    def num_nodes(self):
+        return len(self.nodes)
+
+    @property
+    def num_leaves(self):
+        return len(self.leaves)
+
+
+class TreeGenerator3D:
+    """TreeGenerator_<NodePoint3f> (tree/tree_generator_.h:11-47)."""
+
+    def __init__(self, **kw):
+        p = L.TreeGenParams()
+        L.lib().kdh_tree_gen_params_default(C.byref(p))
+        self._p = p
+        for k, v in kw.items():
+            self.set(k, v)
+
+    def set(self, name, value):
+        name = name[len("param_"):] if name.startswith("param_") else name
+        if not hasattr(self._p, name):
+            raise AttributeError(name)
+        setattr(self._p, name, value)
+
+    def makeTree(self, points, stamps=None):
+        """Returns a HostTree, or None where the reference returns nullptr."""
+        pts = np.ascontiguousarray(points, dtype=np.float32).reshape(-1, 3)
+        st = None if stamps is None else np.ascontiguousarray(stamps, dtype=np.float64)
+        h = C.c_void_p()
+        rc = L.lib().kdh_host_tree_build(
+            L.f32p(pts), None if st is None else L.f64p(st), len(pts), C.byref(self._p), C.byref(h)
+        )
+        L.check(rc, what="kdh_host_tree_build")
+        if not h.value:
+            return None
+        try:
+            nn, nl, ts = C.c_int64(), C.c_int64(), C.c_double()
+            L.check(L.lib().kdh_host_tree_info(h, C.byref(nn), C.byref(nl), C.byref(ts)), what="tree info")
+            nodes = np.ctypeslib.as_array(
+                C.cast(L.lib().kdh_host_tree_nodes(h), C.POINTER(C.c_uint8)), shape=(nn.value * 64,)
+            ).view(L.NODE_DTYPE).copy()
+            leaves = np.ctypeslib.as_array(
+                C.cast(L.lib().kdh_host_tree_leaves(h), C.POINTER(C.c_uint64)), shape=(nl.value,)
+            ).copy()
+            pp = np.ctypeslib.as_array(
+                C.cast(L.lib().kdh_host_tree_points(h), C.POINTER(C.c_float)), shape=(len(pts) * 3,)
+            ).copy().reshape(-1, 3)
+        finally:
+            L.lib().kdh_host_tree_free(h)
+        return HostTree(nodes, leaves, pp, ts.value)
+
+
+# ---------------------------------------------------------------------------
+# Device context and trees
+
+
+class Context:
+    """One kdh_ctx: a HIP stream on one device plus the aligner workspace."""
+
+    def __init__(self, device=0):
+        h = C.c_void_p()
+        rc = L.lib().kdh_ctx_create(device, C.byref(h))
+        if rc != L.KDH_OK:
+            raise RuntimeError(f"kdh_ctx_create({device}) failed: {L.ERRORS.get(rc, rc)}")
+        self.h = h
+        self.device = device
+
+    def close(self):
+        if self.h:
+            L.lib().kdh_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def check(self, rc, what):
+        L.check(rc, self.h, what)
+
+    def set_launch_config(self, grid=0, lds_nodes=1024):
+        self.check(L.lib().kdh_set_launch_config(self.h, grid, lds_nodes), "kdh_set_launch_config")
+
+    def last_kernel_time(self):
+        us, n = C.c_double(), C.c_int()
+        self.check(L.lib().kdh_last_kernel_time(self.h, C.byref(us), C.byref(n)), "kdh_last_kernel_time")
+        return us.value, n.value
+
+
+class DeviceTree:
+    """A tree resident in HBM (kdh_tree_upload); freed with close()."""
+
+    def __init__(self, ctx, host_tree):
+        self.ctx = ctx
+        self.host = host_tree
+        h = C.c_void_p()
+        nodes = host_tree.nodes
+        leaves = host_tree.leaves
+        rc = L.lib().kdh_tree_upload(
+            ctx.h, nodes.ctypes.data_as(C.c_void_p), len(nodes), leaves.ctypes.data_as(C.c_void_p), len(leaves),
+            C.byref(h),
+        )
+        ctx.check(rc, "kdh_tree_upload")
+        self.h = h
+
+    @property
+    def num_leaves(self):
+        return self.host.num_leaves
+
+    def device_bytes(self):
+        nb = C.c_int64()
+        L.lib().kdh_tree_info(self.h, None, None, C.byref(nb))
+        return nb.value
+
+    def close(self):
+        if self.h and self.ctx.h:
+            L.lib().kdh_tree_free(self.ctx.h, self.h)
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ---------------------------------------------------------------------------
+# Aligners
+
+
+def default_params():
+    p = L.Params()
+    L.lib().kdh_params_default(C.byref(p))
+    return p
+
+
+def iso34_f32(X):
+    return np.ascontiguousarray(np.asarray(X, dtype=np.float64)[:3, :4], dtype=np.float32).reshape(12)
+
+
+def update_cache_f32(Xm, Xf):
+    """updateCache (icp/icp_impl_.h:16-21) in FP32 with the reference's
+    evaluation order: Xmf = Xf^-1 * Xm."""
+    m = np.asarray(Xm, dtype=np.float64)[:3, :4].astype(np.float32)
+    f = np.asarray(Xf, dtype=np.float64)[:3, :4].astype(np.float32)
+    Rf, tf = f[:, :3], f[:, 3]
+    Rm, tm = m[:, :3], m[:, 3]
+    Ri = Rf.T.copy()
+    ti = np.zeros(3, dtype=np.float32)
+    for r in range(3):
+        ti[r] = -(((Ri[r, 0] * tf[0]) + Ri[r, 1] * tf[1]) + Ri[r, 2] * tf[2])
+    R = np.zeros((3, 3), dtype=np.float32)
+    t = np.zeros(3, dtype=np.float32)
+    for r in range(3):
+        for c in range(3):
+            R[r, c] = ((Ri[r, 0] * Rm[0, c]) + Ri[r, 1] * Rm[1, c]) + Ri[r, 2] * Rm[2, c]
+        t[r] = (((Ri[r, 0] * tm[0]) + Ri[r, 1] * tm[1]) + Ri[r, 2] * tm[2]) + ti[r]
+    out = np.zeros((3, 4), dtype=np.float32)
+    out[:, :3] = R
+    out[:, 3] = t
+    return m.reshape(12).copy(), f.reshape(12).copy(), out.reshape(12)
+
+
+class _AlignerBase:
+    """TreeAlignerBase_ state and parameters shared by ICPHIP3D / CTICPHIP3D."""
+
+    CT = 0
+    DIM = 6
+
+    def __init__(self, ctx, **params):
+        self.ctx = ctx
+        self.params = default_params()
+        for k, v in params.items():
+            self.set(k, v)
+        self._fixed = None
+        self._moving = None
+        self.X = np.eye(4)
+        self.Xf = np.eye(4)
+        self.v = np.zeros(6)
+        self.prior = None
+        self.stats = None
+        self.stats_log = []
+        self.H = np.zeros((self.DIM, self.DIM))
+        self.b = np.zeros(self.DIM)
+        self.dx = np.zeros(self.DIM)
+
+    # ---- parameters (PARAM, tree_aligner_base_.h:39-47) ----
+    def set(self, name, value):
+        name = name[len("param_"):] if name.startswith("param_") else name
+        if not hasattr(self.params, name):
+            raise AttributeError(name)
+        setattr(self.params, name, value)
+
+    # ---- inputs / state ----
+    def setFixed(self, tree):
+        self._fixed = tree
+
+    def setMoving(self, tree):
+        self._moving = tree
+
+    def setX(self, X):
+        self.X = np.array(X, dtype=np.float64)
+
+    def getX(self):
+        return self.X.copy()
+
+    def setFixedX(self, X):
+        self.Xf = np.array(X, dtype=np.float64)
+
+    def setPosePrior(self, X_ref=None, Z=None, info=None):
+        if info is None or not np.any(info):
+            self.prior = None
+            return
+        pr = L.Prior()
+        pr.X_ref[:] = list(np.asarray(X_ref if X_ref is not None else np.eye(4))[:3, :4].reshape(12))
+        pr.Z[:] = list(np.asarray(Z if Z is not None else np.eye(4))[:3, :4].reshape(12))
+        pr.info[:] = list(np.asarray(info, dtype=np.float64).reshape(36))
+        self.prior = pr
+
+    def isGPU(self):
+        return True
+
+    # ---- the plain per-call path: one kernel build per iteration, host solve
+    def buildQuadraticForm(self, stats_mode=False):
+        Xm, Xf, Xmf = update_cache_f32(self.X, self.Xf)
+        H = np.zeros(self.DIM * self.DIM)
+        b = np.zeros(self.DIM)
+        st = L.Stats()
+        if self.CT:
+            v = np.ascontiguousarray(self.v, dtype=np.float32)
+            rc = L.lib().kdh_cticp_build(self.ctx.h, self._fixed.h, self._moving.h, L.f32p(Xm), L.f32p(Xf),
+                                         L.f32p(Xmf), L.f32p(v), C.byref(self.params), int(stats_mode), L.f64p(H),
+                                         L.f64p(b), C.byref(st))
+        else:
+            rc = L.lib().kdh_icp_build(self.ctx.h, self._fixed.h, self._moving.h, L.f32p(Xm), L.f32p(Xf),
+                                       L.f32p(Xmf), C.byref(self.params), int(stats_mode), L.f64p(H), L.f64p(b),
+                                       C.byref(st))
+        self.ctx.check(rc, "build")
+        self.H = H.reshape(self.DIM, self.DIM)
+        self.b = b
+        self.stats = st.as_dict()
+        self.stats["pert_pose_norm"] = 0.0
+        self.stats["pert_vel_norm"] = 0.0
+
+    def oneRound(self, stats_mode=False):
+        """ICP_::oneRound / CTICP_::oneRound with an FP64 solve on the host."""
+        from .synth import expmap
+
+        self.buildQuadraticForm(stats_mode)
+        H = self.H.copy()
+        b = self.b.copy()
+        if self.prior is not None:
+            Hp, bp = _prior_form(self.prior, self.X)
+            H[:6, :6] += Hp
+            b[:6] += bp
+        good = self.stats["num_outliers"] + self.stats["num_inliers"]
+        if good < self.params.min_good_measurements:
+            return False
+        damp = np.full(self.DIM, float(self.params.damping))
+        if self.CT:
+            damp[6:] = float(self.params.velocity_damping)
+        self.dx = np.linalg.solve(H + np.diag(damp), -b)
+        self.stats["pert_pose_norm"] = float(np.linalg.norm(self.dx[:6]))
+        if self.CT:
+            self.stats["pert_vel_norm"] = float(np.linalg.norm(self.dx[6:]))
+        if stats_mode:
+            return True
+        self.X = expmap(self.dx[:6]) @ self.X
+        if self.CT:
+            self.v = self.v + self.dx[6:]
+        return True
+
+    def compute(self, stats_mode=False):
+        """TreeAlignerBase_::compute (tree_aligner_base_.h:100-115), host loop."""
+        self.stats_log = []
+        max_iter = 1 if stats_mode else self.params.max_iterations
+        prev_score = 0.0
+        for i in range(max_iter):
+            ok = self.oneRound(stats_mode)
+            self.stats_log.append(dict(self.stats))
+            if not ok:
+                return -i
+            if self.params.use_term_crit:
+                s = self.stats
+                nl = s["num_inliers"] + s["num_outliers"] + s["num_bad"] + s["num_err_normal"] + s["num_err_fn"]
+                ratio = s["num_inliers"] / nl if nl else 0.0
+                score = ratio / (s["chi2_kernelized"] / max(s["num_inliers"], 1e-30) + 1e-6)
+                if i > 0:
+                    d = score - prev_score
+                    if s["pert_pose_norm"] < self.params.term_d_pert and d > 0 and \
+                            d / (prev_score + 1e-6) < self.params.term_d_score:
+                        return i
+                prev_score = score
+        return self.params.max_iterations
+
+    # ---- the device-resident loop ----
+    def computeOnDevice(self, stats_mode=False, cache_log=False):
+        """compute() with the whole Gauss-Newton loop on the GPU (kdh_align)."""
+        X0 = np.ascontiguousarray(self.X[:3, :4].reshape(12))
+        Xf = np.ascontiguousarray(self.Xf[:3, :4].reshape(12))
+        v0 = np.ascontiguousarray(self.v, dtype=np.float64)
+        Xo = np.zeros(12)
+        vo = np.zeros(6)
+        n = max(1, int(self.params.max_iterations))
+        log = (L.Stats * n)()
+        nlog, res = C.c_int(), C.c_int()
+        clog = np.zeros((n, 36), dtype=np.float32) if cache_log else None
+        rc = L.lib().kdh_align(self.ctx.h, self._fixed.h, self._moving.h, self.CT, L.f64p(X0), L.f64p(v0),
+                               L.f64p(Xf), C.byref(self.params),
+                               C.byref(self.prior) if self.prior is not None else None, int(stats_mode),
+                               L.f64p(Xo), L.f64p(vo), log, C.byref(nlog), C.byref(res),
+                               L.f32p(clog) if clog is not None else None)
+        self.ctx.check(rc, "kdh_align")
+        self.X = np.eye(4)
+        self.X[:3, :4] = Xo.reshape(3, 4)
+        if self.CT:
+            self.v = vo.copy()
+        self.stats_log = [log[i].as_dict() for i in range(nlog.value)]
+        self.stats = self.stats_log[-1] if self.stats_log else None
+        self.cache_log = clog[: nlog.value] if clog is not None else None
+        return res.value
+
+    def correspondences(self, Xm34=None, Xf34=None, Xmf34=None, v=None):
+        """Per moving leaf: fixed leaf index (-1 = num_bad) and descent depth at
+        the given FP32 cache (default: the cache of the current state)."""
+        if Xm34 is None:
+            Xm34, Xf34, Xmf34 = update_cache_f32(self.X, self.Xf)
+        n = self._moving.num_leaves
+        leaf = np.zeros(max(n, 1), dtype=np.int32)
+        depth = np.zeros(max(n, 1), dtype=np.int32)
+        vv = np.ascontiguousarray(self.v if v is None else v, dtype=np.float32)
+        rc = L.lib().kdh_correspondences(self.ctx.h, self._fixed.h, self._moving.h, self.CT,
+                                         L.f32p(np.ascontiguousarray(Xm34, dtype=np.float32)),
+                                         L.f32p(np.ascontiguousarray(Xf34, dtype=np.float32)),
+                                         L.f32p(np.ascontiguousarray(Xmf34, dtype=np.float32)), L.f32p(vv),
+                                         C.byref(self.params), L.i32p(leaf), L.i32p(depth))
+        self.ctx.check(rc, "kdh_correspondences")
+        return leaf[:n], depth[:n]
+
+
+def _prior_form(prior, X):
+    """Pose prior H/b in FP64 (tree_aligner_base_.h:139-152)."""
+    from .synth import from34
+
+    Xr = from34(np.array(prior.X_ref[:]))
+    Z = from34(np.array(prior.Z[:]))
+    info = np.array(prior.info[:]).reshape(6, 6)
+    ZXi = np.linalg.inv(Xr @ Z)
+
+    def err(Xc):
+        P = ZXi @ Xc
+        R = P[:3, :3]
+        c = max(-1.0, min(1.0, 0.5 * (np.trace(R) - 1)))
+        a = math.acos(c)
+        ax = np.array([R[2, 1] - R[1, 2], R[0, 2] - R[2, 0], R[1, 0] - R[0, 1]])
+        w = ax * 0.5 if a < 1e-7 else ax * (a / (2 * math.sin(a)))
+        return np.concatenate([P[:3, 3], w])
+
+    from .synth import expmap
+
+    e = err(X)
+    J = np.zeros((6, 6))
+    h = 1e-6
+    for k in range(6):
+        d = np.zeros(6)
+        d[k] = h
+        J[:, k] = (err(expmap(d) @ X) - err(expmap(-d) @ X)) / (2 * h)
+    return J.T @ info @ J, J.T @ info @ e
+
+
+class ICPHIP3D(_AlignerBase):
+    """Rigid point-to-plane ICP on the GPU (drop-in for ICPCPU3D / ICPCUDA3D)."""
+
+    CT = 0
+    DIM = 6
+
+
+class CTICPHIP3D(_AlignerBase):
+    """Continuous-time ICP (pose + velocity) on the GPU (drop-in for
+    CTICPCPU3D / CTICPCUDA3D)."""
+
+    CT = 1
+    DIM = 12
+
+    def setVelocities(self, v):
+        self.v = np.array(v, dtype=np.float64)
+
+    def velocities(self):
+        return self.v.copy()
+
+    def resetMovingState(self):
+        self.v = np.zeros(6)
+
+    def setFixedX(self, X):
+        self.Xf = np.array(X, dtype=np.float64)
+
+
+def batch_align(ctx, pairs):
+    """Run many alignments in one batched launch (kdh_batch_align).
+
+    pairs: list of dicts with keys fixed, moving (DeviceTree), ct, X0, Xf, v0,
+    params (L.Params).  Returns a list of dicts (X, v, iterations, result, last)."""
+    n = len(pairs)
+    arr = (L.Pair * n)()
+    keep = []
+    for i, p in enumerate(pairs):
+        a = arr[i]
+        a.fixed = p["fixed"].h.value
+        a.moving = p["moving"].h.value
+        a.ct = int(p.get("ct", 0))
+        a.X0[:] = list(np.asarray(p["X0"])[:3, :4].reshape(12))
+        a.v0[:] = list(np.asarray(p.get("v0", np.zeros(6)), dtype=np.float64))
+        a.Xf[:] = list(np.asarray(p.get("Xf", np.eye(4)))[:3, :4].reshape(12))
+        a.params = p["params"]
+        pr = p.get("prior")
+        if pr is not None:
+            keep.append(pr)
+            a.prior = C.cast(C.pointer(pr), C.c_void_p).value
+        else:
+            a.prior = None
+    res = (L.PairResult * n)()
+    ctx.check(L.lib().kdh_batch_align(ctx.h, n, arr, res), "kdh_batch_align")
+    out = []
+    for i in range(n):
+        r = res[i]
+        X = np.eye(4)
+        X[:3, :4] = np.array(r.X[:]).reshape(3, 4)
+        out.append(dict(X=X, v=np.array(r.v[:]), iterations=r.iterations, result=r.result, last=r.last.as_dict()))
+    return out

@@ -1,0 +1,148 @@
+// Many alignments in one launch (kdh_batch_align).
+//
+// All pairs of a call share one tile list per aligner kind (rigid / CT); each
+// Gauss-Newton iteration is one persistent launch per kind over that list,
+// and the iteration is captured once as a HIP graph and replayed until every
+// pair has finished.  Per-pair results do not depend on the other pairs: the
+// tile decomposition, the per-tile reduction and the FP64 tile-order sum of a
+// pair are the same alone and in any mix.
+//
+// The reference runs alignments one after another on the host
+// (loop-closure candidates, slam/slam_proc_loops_impl_.h:95-115; bundle
+// factors, map/multiview_icp_factor_impl_.h:10-62); this is the batched form
+// of that loop.
+#include <hip/hip_runtime.h>
+
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "../../include/kdh.h"
+#include "kdh_capi_internal.h"
+#include "kdh_kernels.h"
+#include "kdh_solve.h"
+
+using namespace kdh;
+
+#define KDH_HIP_TRY(ctx, expr)                                 \
+  do {                                                         \
+    hipError_t _e = (expr);                                    \
+    if (_e != hipSuccess) return kdh_hip_fail(ctx, _e, #expr); \
+  } while (0)
+
+namespace {
+
+struct Group {
+  int ct = 0;
+  int n_tiles = 0;
+  size_t partial_off = 0;  // floats
+  size_t tile_off = 0;     // ints
+};
+
+}  // namespace
+
+int kdh_batch_align(kdh_ctx* ctx, int n_pairs, const kdh_pair* pairs, kdh_pair_result* results) {
+  if (!ctx) return KDH_ERR_INVALID;
+  if (n_pairs < 0 || (n_pairs > 0 && (!pairs || !results))) return kdh_fail(ctx, KDH_ERR_INVALID, "bad arguments");
+  if (n_pairs == 0) return KDH_OK;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  KDH_HIP_TRY(ctx, hipSetDevice(ctx->device));
+  int rc = kdh_ensure_pairs(ctx, (size_t)n_pairs);
+  if (rc) return rc;
+  Group g[2];
+  g[0].ct = 0;
+  g[1].ct = 1;
+  int max_iter = 0;
+  for (int i = 0; i < n_pairs; ++i) {
+    const kdh_pair& pr = pairs[i];
+    PairState& ps = ctx->h_pairs[i];
+    rc = kdh_fill_pair(ctx, ps, pr.fixed, pr.moving, pr.ct, pr.params, 0, 0);
+    if (rc) return kdh_fail(ctx, rc, "pair " + std::to_string(i) + ": " + ctx->err);
+    memcpy(ps.X, pr.X0, sizeof(ps.X));
+    memcpy(ps.Xf, pr.Xf, sizeof(ps.Xf));
+    for (int k = 0; k < 6; ++k) ps.v[k] = pr.ct ? pr.v0[k] : 0.0;
+    kdh_setup_prior(ps, pr.prior);
+    kdh_cache_from_state(ps);
+    Group& G = g[ps.ct];
+    ps.tile_first = G.n_tiles;
+    G.n_tiles += ps.n_tiles;
+    if (ps.n_tiles == 0) {
+      // no moving leaves: the first round fails the min_good gate
+      ps.iter = 1;
+      ps.status = ps.sp.min_good_measurements > 0 ? 2 : 1;
+    }
+    max_iter = std::max(max_iter, ps.sp.max_iterations);
+  }
+  g[0].partial_off = 0;
+  g[1].partial_off = (size_t)g[0].n_tiles * slots(0);
+  g[0].tile_off = 0;
+  g[1].tile_off = (size_t)g[0].n_tiles;
+  const size_t n_partials = g[1].partial_off + (size_t)g[1].n_tiles * slots(1);
+  const size_t n_tiles_all = (size_t)g[0].n_tiles + (size_t)g[1].n_tiles;
+  rc = kdh_ensure_partials(ctx, std::max<size_t>(n_partials, 1));
+  if (rc) return rc;
+  rc = kdh_ensure_tile_pair(ctx, std::max<size_t>(n_tiles_all, 1));
+  if (rc) return rc;
+  std::vector<int> tp(std::max<size_t>(n_tiles_all, 1), 0);
+  for (int i = 0; i < n_pairs; ++i) {
+    const PairState& ps = ctx->h_pairs[i];
+    const Group& G = g[ps.ct];
+    for (int t = 0; t < ps.n_tiles; ++t) tp[G.tile_off + ps.tile_first + t] = i;
+  }
+  KDH_HIP_TRY(ctx, hipMemcpyAsync(ctx->d_tile_pair, tp.data(), tp.size() * sizeof(int), hipMemcpyHostToDevice,
+                                  ctx->stream));
+  KDH_HIP_TRY(ctx, hipMemcpyAsync(ctx->d_pairs, ctx->h_pairs, sizeof(PairState) * n_pairs, hipMemcpyHostToDevice,
+                                  ctx->stream));
+
+  // Capture one iteration (one launch per non-empty group) as a graph.
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  KDH_HIP_TRY(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+  int lrc = KDH_OK;
+  for (int k = 0; k < 2 && lrc == KDH_OK; ++k) {
+    if (g[k].n_tiles == 0) continue;
+    lrc = kdh_launch(ctx, ctx->d_pairs, ctx->d_tile_pair + g[k].tile_off, g[k].n_tiles, g[k].ct, nullptr, nullptr,
+                     ctx->d_partials + g[k].partial_off);
+  }
+  hipError_t ce = hipStreamEndCapture(ctx->stream, &graph);
+  if (lrc != KDH_OK) {
+    if (graph) hipGraphDestroy(graph);
+    return lrc;
+  }
+  if (ce != hipSuccess) return kdh_hip_fail(ctx, ce, "hipStreamEndCapture");
+  hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  hipGraphDestroy(graph);
+  if (ie != hipSuccess) return kdh_hip_fail(ctx, ie, "hipGraphInstantiate");
+  const int iters = std::min(max_iter, kMaxIter);
+  hipError_t le = hipEventRecord(ctx->ev[0], ctx->stream);
+  for (int it = 0; it < iters && le == hipSuccess; ++it) {
+    le = hipGraphLaunch(exec, ctx->stream);
+    if (le == hipSuccess) le = hipEventRecord(ctx->ev[it + 1], ctx->stream);
+  }
+  if (le == hipSuccess)
+    le = hipMemcpyAsync(ctx->h_pairs, ctx->d_pairs, sizeof(PairState) * n_pairs, hipMemcpyDeviceToHost, ctx->stream);
+  if (le == hipSuccess) le = hipStreamSynchronize(ctx->stream);
+  hipGraphExecDestroy(exec);
+  if (le != hipSuccess) return kdh_hip_fail(ctx, le, "batch replay");
+  double tot = 0.0;
+  for (int it = 0; it < iters; ++it) {
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, ctx->ev[it], ctx->ev[it + 1]) == hipSuccess) tot += ms;
+  }
+  ctx->last_kernel_us = iters ? 1000.0 * tot / iters : 0.0;
+  ctx->last_launches = iters;
+  for (int i = 0; i < n_pairs; ++i) {
+    const PairState& ps = ctx->h_pairs[i];
+    kdh_pair_result& r = results[i];
+    memcpy(r.X, ps.X, sizeof(r.X));
+    memcpy(r.v, ps.v, sizeof(r.v));
+    r.iterations = ps.iter;
+    if (ps.iter > 0)
+      memcpy(&r.last, &ps.stats_log[ps.iter - 1], sizeof(kdh_stats));
+    else
+      memset(&r.last, 0, sizeof(kdh_stats));
+    r.result = kdh_result_code(ps, pairs[i].params.max_iterations);
+  }
+  return KDH_OK;
+}

@@ -1,0 +1,510 @@
+// C ABI of the scan-alignment library (declarations and reference citations in
+// include/kdh.h).  Host-side orchestration: contexts, device-resident trees,
+// the per-call quadratic-form build, the device-resident Gauss-Newton loop and
+// the batched launch.  No call exits the process; every HIP error becomes a
+// negative return code and a message in kdh_last_error().
+#include <hip/hip_runtime.h>
+
+#include <stddef.h>
+#include <string.h>
+
+#include <algorithm>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/kdh.h"
+#include "kdh_capi_internal.h"
+#include "kdh_kernels.h"
+#include "kdh_solve.h"
+
+using namespace kdh;
+
+// ---------------------------------------------------------------------------
+
+void kdh_params_default(kdh_params* p) {
+  if (!p) return;
+  p->kernel_threshold = 0.1f;
+  p->damping = 1.0f;
+  p->max_error = 1.0f;
+  p->min_normal_cos = (float)cos(25. / 180. * M_PI);
+  p->min_good_measurements = 10;
+  p->max_iterations = 20;
+  p->velocity_damping = 100.0f;
+  p->use_term_crit = 0;
+  p->term_d_pert = 1e-3f;
+  p->term_d_score = 1e-3f;
+  p->recenter = 1;
+}
+
+int kdh_fail(kdh_ctx* ctx, int code, const std::string& msg) {
+  if (ctx) ctx->err = msg;
+  return code;
+}
+
+int kdh_hip_fail(kdh_ctx* ctx, hipError_t e, const char* where) {
+  std::string m = std::string(where) + ": " + hipGetErrorString(e);
+  return kdh_fail(ctx, KDH_ERR_HIP, m);
+}
+
+#define KDH_HIP_TRY(ctx, expr)                              \
+  do {                                                      \
+    hipError_t _e = (expr);                                 \
+    if (_e != hipSuccess) return kdh_hip_fail(ctx, _e, #expr); \
+  } while (0)
+
+int kdh_ctx_create(int device, kdh_ctx** out) {
+  if (!out) return KDH_ERR_INVALID;
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return KDH_ERR_HIP;
+  if (device < 0 || device >= n) return KDH_ERR_INVALID;
+  if (hipSetDevice(device) != hipSuccess) return KDH_ERR_HIP;
+  kdh_ctx* c = new kdh_ctx();
+  c->device = device;
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return KDH_ERR_HIP;
+  }
+  if (hipHostMalloc((void**)&c->h_pairs, sizeof(PairState) * 1, hipHostMallocDefault) != hipSuccess) {
+    hipStreamDestroy(c->stream);
+    delete c;
+    return KDH_ERR_NOMEM;
+  }
+  c->h_pairs_cap = 1;
+  for (int i = 0; i < kMaxIter + 1; ++i) {
+    if (hipEventCreate(&c->ev[i]) != hipSuccess) {
+      delete c;
+      return KDH_ERR_HIP;
+    }
+  }
+  *out = c;
+  return KDH_OK;
+}
+
+int kdh_ctx_destroy(kdh_ctx* ctx) {
+  if (!ctx) return KDH_ERR_INVALID;
+  hipSetDevice(ctx->device);
+  hipStreamSynchronize(ctx->stream);
+  if (ctx->graph_exec) hipGraphExecDestroy(ctx->graph_exec);
+  if (ctx->d_pairs) hipFree(ctx->d_pairs);
+  if (ctx->d_partials) hipFree(ctx->d_partials);
+  if (ctx->d_tile_pair) hipFree(ctx->d_tile_pair);
+  if (ctx->d_corr) hipFree(ctx->d_corr);
+  if (ctx->d_depth) hipFree(ctx->d_depth);
+  if (ctx->h_pairs) hipHostFree(ctx->h_pairs);
+  for (int i = 0; i < kMaxIter + 1; ++i)
+    if (ctx->ev[i]) hipEventDestroy(ctx->ev[i]);
+  hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return KDH_OK;
+}
+
+const char* kdh_last_error(const kdh_ctx* ctx) {
+  if (!ctx) return "null context";
+  return ctx->err.c_str();
+}
+
+int kdh_set_launch_config(kdh_ctx* ctx, int grid, int lds_nodes) {
+  if (!ctx) return KDH_ERR_INVALID;
+  if (grid < 0) return kdh_fail(ctx, KDH_ERR_INVALID, "grid must be >= 0");
+  if (lds_nodes != 0 && lds_nodes != 256 && lds_nodes != 512 && lds_nodes != 1024 && lds_nodes != 2048)
+    return kdh_fail(ctx, KDH_ERR_INVALID, "lds_nodes must be one of 0, 256, 512, 1024, 2048");
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  ctx->grid_override = grid;
+  ctx->lds_nodes = lds_nodes;
+  return KDH_OK;
+}
+
+// ---- trees -----------------------------------------------------------------
+
+static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+int kdh_tree_upload(kdh_ctx* ctx, const kdh_node3f* nodes, int64_t n_nodes, const uint64_t* leaf_indices,
+                    int64_t n_leaves, kdh_tree** out) {
+  if (!ctx || !out) return KDH_ERR_INVALID;
+  *out = nullptr;
+  if (n_nodes < 0 || n_leaves < 0 || (n_nodes > 0 && !nodes) || (n_leaves > 0 && !leaf_indices))
+    return kdh_fail(ctx, KDH_ERR_INVALID, "kdh_tree_upload: bad sizes or null arrays");
+  if (n_nodes > (int64_t)0x3fffffff || n_leaves > n_nodes)
+    return kdh_fail(ctx, KDH_ERR_INVALID, "kdh_tree_upload: tree too large or more leaves than nodes");
+  // Validate links and leaf indices on the host: the kernels index without checks.
+  for (int64_t i = 0; i < n_nodes; ++i) {
+    const int32_t l = nodes[i].idx_left, r = nodes[i].idx_right;
+    if (l >= (int32_t)n_nodes || r >= (int32_t)n_nodes || l < -1 || r < -1)
+      return kdh_fail(ctx, KDH_ERR_INVALID, "kdh_tree_upload: child link out of range at node " + std::to_string(i));
+    if ((l >= 0 && l <= i) || (r >= 0 && r <= i))
+      return kdh_fail(ctx, KDH_ERR_INVALID,
+                      "kdh_tree_upload: child link does not point forward (tree not in BFS order) at node " +
+                          std::to_string(i));
+  }
+  for (int64_t i = 0; i < n_leaves; ++i) {
+    if (leaf_indices[i] >= (uint64_t)n_nodes)
+      return kdh_fail(ctx, KDH_ERR_INVALID, "kdh_tree_upload: leaf index out of range");
+  }
+  const size_t node_bytes = align256((size_t)std::max<int64_t>(n_nodes, 1) * 32);
+  const size_t leaf_bytes = align256((size_t)std::max<int64_t>(n_leaves, 1) * 32);
+  std::vector<float> staging((node_bytes + leaf_bytes) / 4, 0.0f);
+  float* sn = staging.data();
+  for (int64_t i = 0; i < n_nodes; ++i) {
+    const kdh_node3f& n = nodes[i];
+    float* o = sn + i * 8;
+    o[0] = n.mean[0]; o[1] = n.mean[1]; o[2] = n.mean[2]; o[3] = n.direction[0];
+    o[4] = n.direction[1]; o[5] = n.direction[2];
+    memcpy(&o[6], &n.idx_left, 4);
+    memcpy(&o[7], &n.idx_right, 4);
+  }
+  float* sl = staging.data() + node_bytes / 4;
+  for (int64_t i = 0; i < n_leaves; ++i) {
+    const kdh_node3f& n = nodes[leaf_indices[i]];
+    float* o = sl + i * 8;
+    o[0] = n.mean[0]; o[1] = n.mean[1]; o[2] = n.mean[2]; o[3] = n.direction[0];
+    o[4] = n.direction[1]; o[5] = n.direction[2];
+    o[6] = (float)n.dts_mean;  // narrowed as `Scalar dt = _dts_mean` (cticp_impl_.h:158)
+    int32_t idx = (int32_t)leaf_indices[i];
+    memcpy(&o[7], &idx, 4);
+  }
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  KDH_HIP_TRY(ctx, hipSetDevice(ctx->device));
+  void* d = nullptr;
+  hipError_t e = hipMalloc(&d, node_bytes + leaf_bytes);
+  if (e != hipSuccess) return kdh_fail(ctx, KDH_ERR_NOMEM, std::string("kdh_tree_upload: hipMalloc: ") + hipGetErrorString(e));
+  e = hipMemcpyAsync(d, staging.data(), node_bytes + leaf_bytes, hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  if (e != hipSuccess) {
+    hipFree(d);
+    return kdh_hip_fail(ctx, e, "kdh_tree_upload: copy");
+  }
+  kdh_tree* t = new kdh_tree();
+  t->device = ctx->device;
+  t->dmem = d;
+  t->bytes = node_bytes + leaf_bytes;
+  t->dt.nodes = (const float4*)d;
+  t->dt.leaves = (const float4*)((char*)d + node_bytes);
+  t->dt.n_nodes = (int)n_nodes;
+  t->dt.n_leaves = (int)n_leaves;
+  t->root_mean = n_nodes > 0 ? v3(nodes[0].mean[0], nodes[0].mean[1], nodes[0].mean[2]) : v3(0.0f, 0.0f, 0.0f);
+  *out = t;
+  return KDH_OK;
+}
+
+int kdh_tree_free(kdh_ctx* ctx, kdh_tree* tree) {
+  if (!ctx || !tree) return KDH_ERR_INVALID;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  hipSetDevice(tree->device);
+  hipStreamSynchronize(ctx->stream);
+  hipError_t e = hipFree(tree->dmem);
+  delete tree;
+  if (e != hipSuccess) return kdh_hip_fail(ctx, e, "kdh_tree_free");
+  return KDH_OK;
+}
+
+int kdh_tree_info(const kdh_tree* tree, int64_t* n_nodes, int64_t* n_leaves, int64_t* device_bytes) {
+  if (!tree) return KDH_ERR_INVALID;
+  if (n_nodes) *n_nodes = tree->dt.n_nodes;
+  if (n_leaves) *n_leaves = tree->dt.n_leaves;
+  if (device_bytes) *device_bytes = (int64_t)tree->bytes;
+  return KDH_OK;
+}
+
+// ---- workspace ---------------------------------------------------------------
+
+template <typename T>
+static hipError_t grow(T** p, size_t* cap, size_t need) {
+  if (need <= *cap && *p) return hipSuccess;
+  size_t nc = std::max<size_t>(need, 2 * *cap);
+  if (*p) {
+    hipError_t e = hipFree(*p);
+    if (e != hipSuccess) return e;
+    *p = nullptr;
+  }
+  hipError_t e = hipMalloc((void**)p, nc * sizeof(T));
+  if (e == hipSuccess) *cap = nc;
+  else *cap = 0;
+  return e;
+}
+
+int kdh_ensure_pairs(kdh_ctx* ctx, size_t n_pairs) {
+  KDH_HIP_TRY(ctx, grow(&ctx->d_pairs, &ctx->pairs_cap, n_pairs));
+  if (n_pairs > ctx->h_pairs_cap) {
+    if (ctx->h_pairs) hipHostFree(ctx->h_pairs);
+    ctx->h_pairs = nullptr;
+    size_t nc = std::max(n_pairs, 2 * ctx->h_pairs_cap);
+    hipError_t e = hipHostMalloc((void**)&ctx->h_pairs, nc * sizeof(PairState), hipHostMallocDefault);
+    if (e != hipSuccess) {
+      ctx->h_pairs_cap = 0;
+      return kdh_fail(ctx, KDH_ERR_NOMEM, "pinned pair buffer");
+    }
+    ctx->h_pairs_cap = nc;
+  }
+  return KDH_OK;
+}
+
+int kdh_ensure_partials(kdh_ctx* ctx, size_t n_floats) {
+  KDH_HIP_TRY(ctx, grow(&ctx->d_partials, &ctx->partials_cap, n_floats));
+  return KDH_OK;
+}
+
+int kdh_ensure_tile_pair(kdh_ctx* ctx, size_t n) {
+  KDH_HIP_TRY(ctx, grow(&ctx->d_tile_pair, &ctx->tile_cap, n));
+  return KDH_OK;
+}
+
+// ---- pair set-up ---------------------------------------------------------------
+
+static TermParams term_params(const kdh_params& p, int stats_mode) {
+  TermParams t;
+  t.kernel_threshold = p.kernel_threshold;
+  t.max_error = p.max_error;
+  t.min_normal_cos = p.min_normal_cos;
+  t.stats_mode = stats_mode ? 1 : 0;
+  return t;
+}
+
+static SolveParams solve_params(const kdh_params& p) {
+  SolveParams s;
+  s.damping = p.damping;
+  s.velocity_damping = p.velocity_damping;
+  s.min_good_measurements = p.min_good_measurements;
+  s.max_iterations = p.max_iterations;
+  s.use_term_crit = p.use_term_crit;
+  s.d_pert = p.term_d_pert;
+  s.d_score = p.term_d_score;
+  s.recenter = p.recenter;
+  return s;
+}
+
+static int n_tiles_of(int n_leaves) { return (n_leaves + kTile - 1) / kTile; }
+
+// Initial FP32 cache from FP64 state: the same formula the device applies
+// after every update (kdh_kernels.hip, finish_pair).
+void kdh_cache_from_state(PairState& ps) {
+  IsoD Xc = iso_from_34<double>(ps.X);
+  IsoD Xf = iso_from_34<double>(ps.Xf);
+  ps.cache.Xm = iso_cast<float>(Xc);
+  ps.cache.Xf = iso_cast<float>(Xf);
+  ps.cache.Xmf = iso_cast<float>(compose(inverse(Xf), Xc));
+  for (int i = 0; i < 6; ++i) ps.cache.v[i] = (float)ps.v[i];
+  ps.cache.c = ps.sp.recenter ? apply_iso(ps.cache.Xm, ps.moving_root_mean) : v3(0.0f, 0.0f, 0.0f);
+}
+
+int kdh_fill_pair(kdh_ctx* ctx, PairState& ps, const kdh_tree* fixed, const kdh_tree* moving, int ct,
+                  const kdh_params& p, int stats_mode, int mode) {
+  if (!fixed || !moving) return kdh_fail(ctx, KDH_ERR_INVALID, "null tree");
+  if (fixed->device != ctx->device || moving->device != ctx->device)
+    return kdh_fail(ctx, KDH_ERR_INVALID, "tree lives on another device");
+  if (mode == 0 && (p.max_iterations < 1 || p.max_iterations > kMaxIter))
+    return kdh_fail(ctx, KDH_ERR_INVALID, "max_iterations must be in [1, 64] for the device loop");
+  memset(&ps, 0, offsetof(PairState, stats_log));
+  ps.fixed = fixed->dt;
+  ps.moving = moving->dt;
+  ps.tp = term_params(p, stats_mode);
+  ps.sp = solve_params(p);
+  if (stats_mode) ps.sp.max_iterations = 1;
+  ps.ct = ct ? 1 : 0;
+  ps.mode = mode;
+  ps.n_tiles = n_tiles_of(moving->dt.n_leaves);
+  ps.moving_root_mean = moving->root_mean;
+  ps.status = 0;
+  ps.iter = 0;
+  ps.tiles_done = 0;
+  return KDH_OK;
+}
+
+int kdh_launch(kdh_ctx* ctx, PairState* d_pairs, const int* d_tile_pair, int n_tiles, int ct, int* corr,
+               int* depth, float* partials) {
+  LaunchArgs a;
+  a.pairs = d_pairs;
+  a.tile_pair = d_tile_pair;
+  a.n_tiles_total = n_tiles;
+  a.partials = partials ? partials : ctx->d_partials;
+  a.xcd_remap = ctx->xcd_remap;
+  a.corr_out = corr;
+  a.depth_out = depth;
+  a.grid = ctx->grid_override > 0 ? std::min(ctx->grid_override, n_tiles) : n_tiles;
+  a.ct = ct;
+  int lds = ctx->lds_nodes;
+  if (ct && lds > 1024) lds = 1024;
+  a.lds_nodes = lds;
+  if (n_tiles <= 0) return KDH_OK;
+  hipError_t e = launch_tiles(a, ctx->stream);
+  if (e != hipSuccess) return kdh_hip_fail(ctx, e, "tile kernel launch");
+  return KDH_OK;
+}
+
+// ---- per-call build ---------------------------------------------------------------
+
+static int build_common(kdh_ctx* ctx, const kdh_tree* fixed, const kdh_tree* moving, int ct, const float Xm[12],
+                        const float Xf[12], const float Xmf[12], const float v[6], const kdh_params* params,
+                        int stats_mode, double* H, double* b, kdh_stats* stats, int32_t* leaf_out,
+                        int32_t* depth_out) {
+  if (!ctx) return KDH_ERR_INVALID;
+  if (!Xm || !Xf || !Xmf || !params || (ct && !v)) return kdh_fail(ctx, KDH_ERR_INVALID, "null argument");
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  KDH_HIP_TRY(ctx, hipSetDevice(ctx->device));
+  int rc = kdh_ensure_pairs(ctx, 1);
+  if (rc) return rc;
+  PairState& ps = ctx->h_pairs[0];
+  rc = kdh_fill_pair(ctx, ps, fixed, moving, ct, *params, stats_mode, 1);
+  if (rc) return rc;
+  ps.cache.Xm = iso_from_34<float>(Xm);
+  ps.cache.Xf = iso_from_34<float>(Xf);
+  ps.cache.Xmf = iso_from_34<float>(Xmf);
+  for (int i = 0; i < 6; ++i) ps.cache.v[i] = ct ? v[i] : 0.0f;
+  ps.cache.c = params->recenter ? apply_iso(ps.cache.Xm, moving->root_mean) : v3(0.0f, 0.0f, 0.0f);
+  const int nt = ps.n_tiles;
+  rc = kdh_ensure_partials(ctx, (size_t)std::max(nt, 1) * slots(ct));
+  if (rc) return rc;
+  const size_t nl = (size_t)moving->dt.n_leaves;
+  if (leaf_out) {
+    KDH_HIP_TRY(ctx, grow(&ctx->d_corr, &ctx->corr_cap, std::max<size_t>(nl, 1)));
+    KDH_HIP_TRY(ctx, grow(&ctx->d_depth, &ctx->depth_cap, std::max<size_t>(nl, 1)));
+  }
+  KDH_HIP_TRY(ctx, hipMemcpyAsync(ctx->d_pairs, &ps, offsetof(PairState, stats_log), hipMemcpyHostToDevice,
+                                  ctx->stream));
+  rc = kdh_launch(ctx, ctx->d_pairs, nullptr, nt, ct, leaf_out ? ctx->d_corr : nullptr,
+                  leaf_out ? ctx->d_depth : nullptr, nullptr);
+  if (rc) return rc;
+  if (nt > 0) {
+    KDH_HIP_TRY(ctx, hipMemcpyAsync(&ps.stats_log[0], &ctx->d_pairs->stats_log[0], sizeof(DevStats),
+                                    hipMemcpyDeviceToHost, ctx->stream));
+    KDH_HIP_TRY(ctx, hipMemcpyAsync(ps.H, ctx->d_pairs->H, sizeof(ps.H) + sizeof(ps.b), hipMemcpyDeviceToHost,
+                                    ctx->stream));
+  }
+  std::vector<int32_t> hc, hd;
+  if (leaf_out && nl > 0) {
+    KDH_HIP_TRY(ctx, hipMemcpyAsync(leaf_out, ctx->d_corr, nl * 4, hipMemcpyDeviceToHost, ctx->stream));
+    if (depth_out)
+      KDH_HIP_TRY(ctx, hipMemcpyAsync(depth_out, ctx->d_depth, nl * 4, hipMemcpyDeviceToHost, ctx->stream));
+  }
+  KDH_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  const int D = ct ? 12 : 6;
+  if (nt == 0) {
+    // empty moving tree: nothing to reduce
+    memset(ps.H, 0, sizeof(ps.H));
+    memset(ps.b, 0, sizeof(ps.b));
+    memset(&ps.stats_log[0], 0, sizeof(DevStats));
+  }
+  if (H) memcpy(H, ps.H, sizeof(double) * D * D);
+  if (b) memcpy(b, ps.b, sizeof(double) * D);
+  if (stats) memcpy(stats, &ps.stats_log[0], sizeof(kdh_stats));
+  return KDH_OK;
+}
+
+int kdh_icp_build(kdh_ctx* ctx, const kdh_tree* fixed, const kdh_tree* moving, const float Xm[12], const float Xf[12],
+                  const float Xmf[12], const kdh_params* params, int stats_mode, double H[36], double b[6],
+                  kdh_stats* stats) {
+  return build_common(ctx, fixed, moving, 0, Xm, Xf, Xmf, nullptr, params, stats_mode, H, b, stats, nullptr,
+                      nullptr);
+}
+
+int kdh_cticp_build(kdh_ctx* ctx, const kdh_tree* fixed, const kdh_tree* moving, const float Xm[12],
+                    const float Xf[12], const float Xmf[12], const float v[6], const kdh_params* params,
+                    int stats_mode, double H[144], double b[12], kdh_stats* stats) {
+  return build_common(ctx, fixed, moving, 1, Xm, Xf, Xmf, v, params, stats_mode, H, b, stats, nullptr, nullptr);
+}
+
+int kdh_correspondences(kdh_ctx* ctx, const kdh_tree* fixed, const kdh_tree* moving, int ct, const float Xm[12],
+                        const float Xf[12], const float Xmf[12], const float v[6], const kdh_params* params,
+                        int32_t* leaf_out, int32_t* depth_out) {
+  if (!leaf_out) return kdh_fail(ctx, KDH_ERR_INVALID, "leaf_out is null");
+  float zero[6] = {0, 0, 0, 0, 0, 0};
+  return build_common(ctx, fixed, moving, ct, Xm, Xf, Xmf, v ? v : zero, params, 0, nullptr, nullptr, nullptr,
+                      leaf_out, depth_out);
+}
+
+// ---- whole alignment on the device ---------------------------------------------------
+
+// compute()'s return value (tree_aligner_base_.h:100-115): -i when round i
+// failed the min_good gate, i when the criterion stopped after round i,
+// otherwise the configured max_iterations (also in stats mode).
+int kdh_result_code(const PairState& ps, int max_iterations_param) {
+  if (ps.status == 2) return -(ps.iter - 1);
+  if (ps.status == 3) return ps.iter - 1;
+  return max_iterations_param;
+}
+
+int kdh_setup_prior(PairState& ps, const kdh_prior* prior) {
+  ps.has_prior = 0.0;
+  if (!prior) return KDH_OK;
+  bool nz = false;
+  for (int i = 0; i < 36; ++i) nz = nz || prior->info[i] != 0.0;
+  if (!nz) return KDH_OK;
+  IsoD Xr = iso_from_34<double>(prior->X_ref);
+  IsoD Z = iso_from_34<double>(prior->Z);
+  IsoD ZXi = inverse(compose(Xr, Z));
+  iso_to_34(ZXi, ps.prior_ZXi);
+  memcpy(ps.prior_info, prior->info, sizeof(ps.prior_info));
+  ps.has_prior = 1.0;
+  return KDH_OK;
+}
+
+int kdh_align(kdh_ctx* ctx, const kdh_tree* fixed, const kdh_tree* moving, int ct, const double X0[12],
+              const double v0[6], const double Xf[12], const kdh_params* params, const kdh_prior* prior,
+              int stats_mode, double X_out[12], double v_out[6], kdh_stats* stats_log, int* n_log, int* result,
+              float* cache_log) {
+  if (!ctx) return KDH_ERR_INVALID;
+  if (!X0 || !Xf || !params) return kdh_fail(ctx, KDH_ERR_INVALID, "null argument");
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  KDH_HIP_TRY(ctx, hipSetDevice(ctx->device));
+  int rc = kdh_ensure_pairs(ctx, 1);
+  if (rc) return rc;
+  PairState& ps = ctx->h_pairs[0];
+  rc = kdh_fill_pair(ctx, ps, fixed, moving, ct, *params, stats_mode, 0);
+  if (rc) return rc;
+  memcpy(ps.X, X0, sizeof(ps.X));
+  memcpy(ps.Xf, Xf, sizeof(ps.Xf));
+  for (int i = 0; i < 6; ++i) ps.v[i] = (ct && v0) ? v0[i] : 0.0;
+  kdh_setup_prior(ps, prior);
+  kdh_cache_from_state(ps);
+  const int nt = ps.n_tiles;
+  const int iters = ps.sp.max_iterations;
+  rc = kdh_ensure_partials(ctx, (size_t)std::max(nt, 1) * slots(ct));
+  if (rc) return rc;
+  KDH_HIP_TRY(ctx, hipMemcpyAsync(ctx->d_pairs, &ps, sizeof(PairState), hipMemcpyHostToDevice, ctx->stream));
+  if (nt == 0) {
+    // No moving leaves: every round fails the min_good gate, as the reference does.
+    KDH_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    memset(&ps.stats_log[0], 0, sizeof(DevStats));
+    ps.iter = 1;
+    ps.status = params->min_good_measurements > 0 ? 2 : 1;
+  } else {
+    KDH_HIP_TRY(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
+    for (int it = 0; it < iters; ++it) {
+      if (cache_log) {
+        KDH_HIP_TRY(ctx, hipMemcpyAsync(&ps.cache, &ctx->d_pairs->cache, sizeof(TermCache), hipMemcpyDeviceToHost,
+                                        ctx->stream));
+        KDH_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+        iso_to_34(ps.cache.Xm, cache_log + it * 36);
+        iso_to_34(ps.cache.Xf, cache_log + it * 36 + 12);
+        iso_to_34(ps.cache.Xmf, cache_log + it * 36 + 24);
+      }
+      rc = kdh_launch(ctx, ctx->d_pairs, nullptr, nt, ct, nullptr, nullptr, nullptr);
+      if (rc) return rc;
+      KDH_HIP_TRY(ctx, hipEventRecord(ctx->ev[it + 1], ctx->stream));
+    }
+    KDH_HIP_TRY(ctx, hipMemcpyAsync(&ps, ctx->d_pairs, sizeof(PairState), hipMemcpyDeviceToHost, ctx->stream));
+    KDH_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    double tot = 0.0;
+    for (int it = 0; it < iters; ++it) {
+      float ms = 0.0f;
+      if (hipEventElapsedTime(&ms, ctx->ev[it], ctx->ev[it + 1]) == hipSuccess) tot += ms;
+    }
+    ctx->last_kernel_us = iters ? 1000.0 * tot / iters : 0.0;
+    ctx->last_launches = iters;
+  }
+  if (X_out) memcpy(X_out, ps.X, sizeof(ps.X));
+  if (v_out) memcpy(v_out, ps.v, sizeof(ps.v));
+  const int n = ps.iter;
+  if (stats_log) memcpy(stats_log, ps.stats_log, sizeof(DevStats) * n);
+  if (n_log) *n_log = n;
+  if (result) *result = kdh_result_code(ps, params->max_iterations);
+  return KDH_OK;
+}
+
+int kdh_last_kernel_time(kdh_ctx* ctx, double* avg_us, int* n_launches) {
+  if (!ctx) return KDH_ERR_INVALID;
+  if (avg_us) *avg_us = ctx->last_kernel_us;
+  if (n_launches) *n_launches = ctx->last_launches;
+  return KDH_OK;
+}

@@ -1,0 +1,60 @@
+// Opaque C-ABI handles and helpers shared by the host-side translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <mutex>
+#include <string>
+
+#include "../../include/kdh.h"
+#include "kdh_internal.h"
+
+struct kdh_tree {
+  int device = 0;
+  void* dmem = nullptr;
+  size_t bytes = 0;
+  kdh::DevTree dt{};
+  kdh::V3f root_mean{};
+};
+
+struct kdh_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::mutex mu;
+  std::string err;
+  // workspace (grown x2 on demand, never shrunk)
+  kdh::PairState* d_pairs = nullptr;
+  size_t pairs_cap = 0;
+  kdh::PairState* h_pairs = nullptr;  // pinned mirror
+  size_t h_pairs_cap = 0;
+  float* d_partials = nullptr;
+  size_t partials_cap = 0;
+  int* d_tile_pair = nullptr;
+  size_t tile_cap = 0;
+  int* d_corr = nullptr;
+  size_t corr_cap = 0;
+  int* d_depth = nullptr;
+  size_t depth_cap = 0;
+  hipGraphExec_t graph_exec = nullptr;
+  // launch configuration (autotuner)
+  int grid_override = 0;
+  int lds_nodes = 1024;
+  int xcd_remap = 1;
+  // timing of the last device loop
+  hipEvent_t ev[kdh::kMaxIter + 1] = {};
+  double last_kernel_us = 0.0;
+  int last_launches = 0;
+};
+
+int kdh_fail(kdh_ctx* ctx, int code, const std::string& msg);
+int kdh_hip_fail(kdh_ctx* ctx, hipError_t e, const char* where);
+int kdh_ensure_pairs(kdh_ctx* ctx, size_t n_pairs);
+int kdh_ensure_partials(kdh_ctx* ctx, size_t n_floats);
+int kdh_ensure_tile_pair(kdh_ctx* ctx, size_t n);
+int kdh_fill_pair(kdh_ctx* ctx, kdh::PairState& ps, const kdh_tree* fixed, const kdh_tree* moving, int ct,
+                  const kdh_params& p, int stats_mode, int mode);
+void kdh_cache_from_state(kdh::PairState& ps);
+int kdh_setup_prior(kdh::PairState& ps, const kdh_prior* prior);
+int kdh_launch(kdh_ctx* ctx, kdh::PairState* d_pairs, const int* d_tile_pair, int n_tiles, int ct, int* corr,
+               int* depth, float* partials);
+int kdh_result_code(const kdh::PairState& ps, int max_iterations_param);

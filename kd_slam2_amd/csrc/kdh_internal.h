@@ -1,0 +1,84 @@
+// Device-side data structures shared by the kernels and the C-ABI layer.
+//
+// HBM layout of a tree (one allocation per tree, 256-byte aligned sections):
+//   nodes  : 2 x float4 per node, BFS order as produced by the reference's
+//            prune + breadthFirstPermutation (tree_cpu_.h:173-207,
+//            tree_cpu_impl_.h:137-158):
+//              a = (mean.x, mean.y, mean.z, dir.x)
+//              b = (dir.y, dir.z, left, right)      (links as int32 bits)
+//            32 bytes per node, one 32-byte sector per visit.
+//   leaves : 2 x float4 per leaf of _leaves_indices order (the query set when
+//            the tree is the moving cloud):
+//              a = (mean.x, mean.y, mean.z, dir.x)
+//              b = (dir.y, dir.z, (float)_dts_mean, node index bits)
+// The 64-byte host node (Node_<Point3fTraits>, node_.h:18-43) also carries
+// _idx_start/_idx_end and _dts_covariance, which the alignment never reads.
+#pragma once
+
+#include "kdh_math.h"
+#include "kdh_term.h"
+
+namespace kdh {
+
+constexpr int kTile = 256;           // moving leaves per tile (one workgroup pass)
+constexpr int kMaxIter = 64;         // longest Gauss-Newton loop kept on device
+
+struct DevTree {
+  const float4* nodes;
+  const float4* leaves;
+  int n_nodes;
+  int n_leaves;
+};
+
+// Per-iteration statistics, ICPStats_ (icp_stats_.h:10-20) in FP64 sums.
+struct DevStats {
+  int num_inliers, num_outliers, num_bad, num_err_normal, num_err_fn;
+  int pad;
+  double chi2, chi2_kernelized, pert_pose_norm, pert_vel_norm;
+};
+
+// Solver settings of one alignment (ICPParams_ / CTICPParams_,
+// tree_aligner_base_.h:12-21, cticp_.h:14-18, and the score termination
+// criterion icp_stats_.h:85-102).
+struct SolveParams {
+  double damping;
+  double velocity_damping;
+  int min_good_measurements;
+  int max_iterations;
+  int use_term_crit;
+  float d_pert;
+  float d_score;
+  int recenter;
+};
+
+// Everything the device needs to run one alignment, and its evolving state.
+struct PairState {
+  // inputs
+  DevTree fixed;
+  DevTree moving;
+  TermParams tp;
+  SolveParams sp;
+  int ct;               // 0 rigid ICP, 1 CT-ICP
+  int tile_first;       // first tile of this pair in the launch's tile list
+  int n_tiles;
+  int mode;             // 0: solve + update (align), 1: build only (write H/b)
+  double Xf[12];        // fixed pose, 3x4 row-major
+  double has_prior;
+  double prior_ZXi[12]; // (X_ref * Z)^-1
+  double prior_info[36];
+  V3f moving_root_mean; // re-centring anchor in the moving frame
+  // state
+  double X[12];         // moving pose
+  double v[6];          // moving velocities (CT)
+  TermCache cache;      // FP32 inputs of the current iteration
+  int iter;             // completed iterations
+  int status;           // 0 running, 1 finished (max_iterations or criterion), -k failed at k
+  float prev_score;
+  unsigned tiles_done;  // last-block detection counter (reset by the last block)
+  // outputs
+  DevStats stats_log[kMaxIter];
+  double H[144];        // last H (n x n row-major, n = 6 or 12)
+  double b[12];
+};
+
+}  // namespace kdh

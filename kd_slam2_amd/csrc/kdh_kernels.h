@@ -1,0 +1,26 @@
+// Launch interface of the tile kernel (kdh_kernels.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "kdh_internal.h"
+
+namespace kdh {
+
+struct LaunchArgs {
+  PairState* pairs;       // device array of alignments
+  const int* tile_pair;   // device array: tile -> pair (nullptr: all tiles belong to pair 0)
+  int n_tiles_total;
+  float* partials;        // n_tiles_total x slots(ct) floats
+  int* corr_out;          // optional, single-pair launches: fixed leaf per moving leaf
+  int* depth_out;         // optional, single-pair launches: descent depth per moving leaf
+  int grid;               // workgroups (tiles are strided over them)
+  int ct;
+  int lds_nodes;          // top-of-tree nodes staged in LDS: 0, 256, 512, 1024, 2048 (ICP)
+  int xcd_remap;          // 1: give each XCD a contiguous range of tiles (needs grid % 8 == 0)
+};
+
+hipError_t launch_tiles(const LaunchArgs& a, hipStream_t s);
+int slots(int ct);
+
+}  // namespace kdh

@@ -1,0 +1,295 @@
+// Per-moving-leaf quadratic term of the point-to-plane aligners, for the
+// device kernels.
+//
+// Follows ICP_::quadraticTerm (kd_slam2/src/lib/kd_slam/icp/icp_impl_.h:103-152)
+// and CTICP_::quadraticTerm (kd_slam2/src/lib/kd_slam/cticp/cticp_impl_.h:107-181)
+// with the 3D traits of d3/icp_.h:41-60 and d3/cticp_impl_.h:11-39, 76-85.
+//
+// Split of responsibilities:
+//  * Everything that decides a discrete outcome -- the leaf warp (CT), the
+//    query point p_mif = Xmf*mean, every whichSide test of the descent, the
+//    normal-cosine gate, the residual e and its gate, chi2 > kernel_threshold
+//    -- is evaluated with the reference formula and operation order, with no
+//    contraction, so it matches the host restatement bit for bit.
+//  * The Jacobian is re-centred: the rotational column uses (p - c) instead of
+//    p, with c a per-iteration centroid chosen by the host, so FP32 terms stay
+//    small when the scene is far from the origin.  The host (or the solve
+//    stage) maps H_c, b_c back with H = M^T H_c M, b = M^T b_c,
+//    M = [[I, -[c]x], [0, I]], in FP64.  c = 0 reproduces the reference
+//    Jacobian exactly.
+#pragma once
+
+#include "kdh_math.h"
+#include "kdh_sincos.h"
+
+#pragma clang fp contract(off)
+
+namespace kdh {
+
+// Accumulator slots per leaf.
+//   ICP: 21 upper-triangular J_r J_c g, 6 J_r e g, chi2, chi2*g, 5 one-hot counts
+//   CT : 27 pose diag + 27 velocity diag + 36 cross (row-major J_pose_r J_vel_c g)
+//        + chi2, chi2*g, 5 counts
+enum : int {
+  kIcpDiag = 27,
+  kIcpSlots = 27 + 2 + 5,      // 34
+  kCtDiagPose = 0,
+  kCtDiagVel = 27,
+  kCtCross = 54,
+  kCtSlots = 90 + 2 + 5,       // 97
+};
+
+// Outcome of one leaf, as the one-hot counters of ICPStats_ (icp_stats_.h:10-20).
+enum : int { kInlier = 0, kOutlier = 1, kBad = 2, kErrNormal = 3, kErrFn = 4 };
+
+struct TermParams {
+  float kernel_threshold;
+  float max_error;
+  float min_normal_cos;
+  int stats_mode;
+};
+
+// Per-iteration inputs of the term (QuadraticTermCache, icp_.h:97-102, plus
+// the CT velocity and the re-centring point).
+struct TermCache {
+  IsoF Xm;
+  IsoF Xf;
+  IsoF Xmf;
+  float v[6];   // moving velocities [t; w] (CT only)
+  V3f c;        // re-centring point for the Jacobian
+};
+
+// Greedy descent (icp_impl_.h:116-121, node_.h:54-63).  Node record layout:
+//   a = (mean.x, mean.y, mean.z, dir.x), b = (dir.y, dir.z, left, right)
+// with the child links stored as int bits.  Returns the leaf index or -1.
+template <typename NodeFetch>
+KDH_HD int descend(const NodeFetch& fetch, V3f q, int* depth_out) {
+  int idx = 0;
+  int depth = 0;
+  float4 a, b;
+  fetch(0, a, b);
+  for (;;) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    int l = __float_as_int(b.z);
+    int r = __float_as_int(b.w);
+#else
+    int l, r;
+    memcpy(&l, &b.z, 4);
+    memcpy(&r, &b.w, 4);
+#endif
+    if (l < 0 && r < 0) break;
+    float dx = q.x - a.x;
+    float dy = q.y - a.y;
+    float dz = q.z - a.z;
+    float s = (a.w * dx + b.x * dy) + b.y * dz;
+    idx = s > 0.0f ? l : r;
+    ++depth;
+    if (idx < 0) break;
+    fetch(idx, a, b);
+  }
+  if (depth_out) *depth_out = depth;
+  return idx;
+}
+
+// rodrigues(w) (geometry_3d_impl_.h:22-46): identity below 1e-9, otherwise
+// R = I + sin(a) S + (1 - cos(a)) S^2 with S = skew(w / |w|).
+struct Rodrigues {
+  M3f R;
+  M3f S;
+  V3f n;
+  float angle;
+  bool small;
+};
+
+KDH_HD Rodrigues rodrigues_f(V3f w) {
+  Rodrigues o;
+  o.angle = sqrtf(sqnorm(w));
+  if (o.angle < 1e-9f) {
+    o.small = true;
+    o.R = m3_identity<float>();
+    o.S = m3_zero<float>();
+    o.n = v3(0.0f, 0.0f, 0.0f);
+    return o;
+  }
+  o.small = false;
+  float inv = 1.0f / o.angle;
+  o.n = w * inv;
+  o.S = skew(o.n);
+  M3f S2 = mul(o.S, o.S);
+  float s, c;
+  sincos_shared(o.angle, &s, &c);
+  float omc = 1.0f - c;
+  M3f I = m3_identity<float>();
+  for (int i = 0; i < 9; ++i) o.R.m[i] = (I.m[i] + s * o.S.m[i]) + omc * S2.m[i];
+  return o;
+}
+
+// d(R(w) u)/dw (Gallego & Yezzi; geometry_3d_impl_.h:143-174), for the
+// velocity Jacobian only (tolerance path).
+KDH_HD M3f dRxp_f(const Rodrigues& rd, V3f u) {
+  if (rd.small) return skew(-u);  // -skew(u)
+  // -R * skew(u) * (n n^T + (R^T - I) S / angle)
+  M3f nn;
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) nn.m[r * 3 + c] = rd.n[r] * rd.n[c];
+  M3f RtmI = transpose(rd.R);
+  RtmI.m[0] -= 1.0f; RtmI.m[4] -= 1.0f; RtmI.m[8] -= 1.0f;
+  M3f B = mul(RtmI, rd.S);
+  float ia = 1.0f / rd.angle;
+  for (int i = 0; i < 9; ++i) B.m[i] = nn.m[i] + B.m[i] * ia;
+  M3f A = mul(rd.R, skew(u));
+  M3f out = mul(A, B);
+  for (int i = 0; i < 9; ++i) out.m[i] = -out.m[i];
+  return out;
+}
+
+}  // namespace kdh
+
+namespace kdh {
+
+// One moving leaf as the kernels see it: (mean, dir) in the moving tree frame
+// and, for CT, the leaf's mean stamp offset narrowed to float exactly as the
+// reference's `const Scalar dt = moving_leaf._dts_mean` (cticp_impl_.h:158).
+struct LeafIn {
+  V3f mean;
+  V3f dir;
+  float dts;
+};
+
+// Rigid ICP term.  Writes kIcpSlots accumulator values into acc (zeros for a
+// rejected leaf, with the one-hot outcome counter set) and returns the fixed
+// leaf index reached by the descent (-1 for num_bad).
+template <typename NodeFetch>
+KDH_HD int icp_term(const NodeFetch& fetch, const TermCache& C, const TermParams& P,
+                    const LeafIn& L, float* acc, int* depth_out) {
+#pragma unroll
+  for (int i = 0; i < kIcpSlots; ++i) acc[i] = 0.0f;
+  const V3f p_mif = apply_iso(C.Xmf, L.mean);
+  const int leaf = descend(fetch, p_mif, depth_out);
+  if (leaf < 0) {
+    acc[29 + kBad] = 1.0f;
+    return -1;
+  }
+  float4 a, b;
+  fetch(leaf, a, b);
+  const V3f pf = v3(a.x, a.y, a.z);
+  const V3f nf = v3(a.w, b.x, b.y);
+  const V3f p_m = apply_iso(C.Xm, L.mean);
+  const V3f n_m = apply_rot(C.Xm, L.dir);
+  const V3f p_f = apply_iso(C.Xf, pf);
+  const V3f n_f = apply_rot(C.Xf, nf);
+  // errorAndJacobian (d3/icp_.h:41-60)
+  if (dot(n_f, n_m) < P.min_normal_cos) {
+    acc[29 + kErrNormal] = 1.0f;
+    return leaf;
+  }
+  const V3f n = n_m + n_f;
+  const float e = dot(p_m - p_f, n);
+  if (fabsf(e) > P.max_error) {
+    acc[29 + kErrFn] = 1.0f;
+    return leaf;
+  }
+  const V3f jr = cross(p_m - C.c, n_f) + cross(p_f - C.c, n_m);
+  const float J[6] = {n.x, n.y, n.z, jr.x, jr.y, jr.z};
+  const float chi2 = e * e;
+  float gamma = 1.0f;
+  if (chi2 > P.kernel_threshold) {
+    acc[29 + kOutlier] = 1.0f;
+    gamma = P.stats_mode ? 0.0f : sqrtf(P.kernel_threshold / chi2);
+  } else {
+    acc[29 + kInlier] = 1.0f;
+  }
+  // ICPWsItemDiag_::pack (icp_ws_item_.h:26-34)
+  int k = 0;
+#pragma unroll
+  for (int r = 0; r < 6; ++r)
+#pragma unroll
+    for (int c = r; c < 6; ++c, ++k) acc[k] = (J[r] * J[c]) * gamma;
+#pragma unroll
+  for (int r = 0; r < 6; ++r, ++k) acc[k] = (J[r] * e) * gamma;
+  acc[27] = chi2;
+  acc[28] = chi2 * gamma;
+  return leaf;
+}
+
+// Continuous-time term (pose + velocity, 12 DoF).
+template <typename NodeFetch>
+KDH_HD int ct_term(const NodeFetch& fetch, const TermCache& C, const TermParams& P,
+                   const LeafIn& L, float* acc, int* depth_out) {
+#pragma unroll
+  for (int i = 0; i < kCtSlots; ++i) acc[i] = 0.0f;
+  // applyVelocities on the leaf (d3/cticp_impl_.h:76-85)
+  const float dt = L.dts;
+  const V3f t_delta = v3(C.v[0] * dt, C.v[1] * dt, C.v[2] * dt);
+  const V3f r_delta = v3(C.v[3] * dt, C.v[4] * dt, C.v[5] * dt);
+  const Rodrigues rd = rodrigues_f(r_delta);
+  const V3f wmean = mul(rd.R, L.mean) + t_delta;
+  const V3f wdir = mul(rd.R, L.dir);
+  const V3f p_mif = apply_iso(C.Xmf, wmean);
+  const int leaf = descend(fetch, p_mif, depth_out);
+  if (leaf < 0) {
+    acc[92 + kBad] = 1.0f;
+    return -1;
+  }
+  float4 a, b;
+  fetch(leaf, a, b);
+  const V3f pf = v3(a.x, a.y, a.z);
+  const V3f nf = v3(a.w, b.x, b.y);
+  const V3f p_m = apply_iso(C.Xm, wmean);
+  const V3f n_m = apply_rot(C.Xm, wdir);
+  const V3f p_f = apply_iso(C.Xf, pf);
+  const V3f n_f = apply_rot(C.Xf, nf);
+  if (dot(n_f, n_m) < P.min_normal_cos) {
+    acc[92 + kErrNormal] = 1.0f;
+    return leaf;
+  }
+  const V3f n = n_m + n_f;
+  const V3f dp = p_m - p_f;
+  const float e = dot(dp, n);
+  if (fabsf(e) > P.max_error) {
+    acc[92 + kErrFn] = 1.0f;
+    return leaf;
+  }
+  const V3f jr = cross(p_m - C.c, n_f) + cross(p_f - C.c, n_m);
+  const float Jp[6] = {n.x, n.y, n.z, jr.x, jr.y, jr.z};
+  // errorAndJacobianVelocityA (d3/cticp_impl_.h:11-39) at the unwarped leaf:
+  //   J_t = dt * n^T Ra ; J_w = dt * (n^T Ra dR_p + dp^T Ra dR_n)
+  const M3f& Ra = C.Xm.R;
+  const V3f nR = mul_t(Ra, n);    // (n^T Ra)^T
+  const V3f dpR = mul_t(Ra, dp);  // (dp^T Ra)^T
+  const M3f dRp = dRxp_f(rd, L.mean);
+  const M3f dRn = dRxp_f(rd, L.dir);
+  const V3f jw = mul_t(dRp, nR) + mul_t(dRn, dpR);
+  const float Jv[6] = {nR.x * dt, nR.y * dt, nR.z * dt, jw.x * dt, jw.y * dt, jw.z * dt};
+  const float chi2 = e * e;
+  float gamma = 1.0f;
+  if (chi2 > P.kernel_threshold) {
+    acc[92 + kOutlier] = 1.0f;
+    gamma = P.stats_mode ? 0.0f : sqrtf(P.kernel_threshold / chi2);
+  } else {
+    acc[92 + kInlier] = 1.0f;
+  }
+  int k = 0;
+#pragma unroll
+  for (int r = 0; r < 6; ++r)
+#pragma unroll
+    for (int c = r; c < 6; ++c, ++k) acc[k] = (Jp[r] * Jp[c]) * gamma;
+#pragma unroll
+  for (int r = 0; r < 6; ++r, ++k) acc[k] = (Jp[r] * e) * gamma;
+#pragma unroll
+  for (int r = 0; r < 6; ++r)
+#pragma unroll
+    for (int c = r; c < 6; ++c, ++k) acc[k] = (Jv[r] * Jv[c]) * gamma;
+#pragma unroll
+  for (int r = 0; r < 6; ++r, ++k) acc[k] = (Jv[r] * e) * gamma;
+  // ICPWsItemCross_::pack: ja^T jb * gamma (icp_ws_item_.h:72-75)
+#pragma unroll
+  for (int r = 0; r < 6; ++r)
+#pragma unroll
+    for (int c = 0; c < 6; ++c, ++k) acc[k] = (Jp[r] * Jv[c]) * gamma;
+  acc[90] = chi2;
+  acc[91] = chi2 * gamma;
+  return leaf;
+}
+
+}  // namespace kdh

@@ -1,0 +1,968 @@
+// ORACLE -- test infrastructure only.  Nothing in the product links or calls
+// this code: only tests/, the smoke check and the CPU-baseline leg of bench.py
+// load it, as the checker and as the reference CPU timing.
+//
+// A CPU restatement of the reference aligners ICP_CPU_ / CTICP_CPU_
+// (rvp-group/kd_slam2), written without Eigen and with the reference's
+// evaluation order made explicit:
+//   ICP_::updateCache / oneRound / quadraticTerm
+//       kd_slam2/src/lib/kd_slam/icp/icp_impl_.h:16-21, 53-90, 103-152
+//   ICP_CPU_::_buildQuadraticForm (<= 8 std::async chunks, StableAdder_)
+//       kd_slam2/src/lib/kd_slam/icp/icp_cpu_impl_.h:28-98
+//   ICPWsItemDiag_ / ICPWsItemCross_ / ICPStats_
+//       icp/icp_ws_item_.h:14-91, icp/icp_stats_.h:10-102
+//   StableAdder_  utils/stable_adder_.h:21-74
+//   CTICP_::oneRound / quadraticTerm, CTICP_CPU_::_buildQuadraticForm
+//       cticp/cticp_impl_.h:48-92, 107-181; cticp/cticp_cpu_impl_.h:27-126
+//   3D traits  d3/icp_.h:41-60, d3/cticp_impl_.h:11-39, 76-85
+//   geometry   utils/geometry_.h:11-37, utils/geometry_3d_impl_.h:22-46, 143-194
+//   compute()  icp/tree_aligner_base_.h:100-115, pose prior :139-152
+// Compiled with -ffp-contract=off for parity (the decision expressions must
+// not be fused).  The single shared piece with the product is the sine/cosine
+// of the CT deskew (kdh_sincos.h): host libm and the device math library
+// differ in the last bit, so both sides call one function; it is pinned
+// against libm separately.
+//
+// Every function also has an FP64 accumulation twin: the same discrete
+// decisions, with the per-leaf error and Jacobian re-evaluated in FP64 from the
+// same FP32 inputs and summed in FP64, as the error yardstick for H and b.
+#include <math.h>
+#include <stdint.h>
+#include <string.h>
+
+#include <algorithm>
+#include <future>
+#include <vector>
+
+#include "../kd_slam2_amd/csrc/kdh_sincos.h"
+
+namespace orc {
+
+// ---- Node_<Point3fTraits> memory layout (node_.h:12-43) --------------------
+struct Node {
+  double dts_mean;
+  double dts_covariance;
+  float mean[3];
+  float direction[3];
+  uint64_t idx_start;
+  uint64_t idx_end;
+  int32_t idx_left;
+  int32_t idx_right;
+  bool isLeaf() const { return idx_left < 0 && idx_right < 0; }
+};
+static_assert(sizeof(Node) == 64, "Node_ layout");
+
+// ---- small Eigen-free types --------------------------------------------------
+template <typename T> struct Vec3 { T v[3]; T& operator[](int i) { return v[i]; } T operator[](int i) const { return v[i]; } };
+template <typename T> struct Mat3 { T m[3][3]; };
+template <typename T> struct Iso3 { Mat3<T> R; Vec3<T> t; };
+
+template <typename T> static Vec3<T> mk(T a, T b, T c) { Vec3<T> r; r.v[0] = a; r.v[1] = b; r.v[2] = c; return r; }
+template <typename T> static Vec3<T> add(Vec3<T> a, Vec3<T> b) { return mk(a[0] + b[0], a[1] + b[1], a[2] + b[2]); }
+template <typename T> static Vec3<T> sub(Vec3<T> a, Vec3<T> b) { return mk(a[0] - b[0], a[1] - b[1], a[2] - b[2]); }
+// Eigen dot for 3-vectors: (a0 b0 + a1 b1) + a2 b2
+template <typename T> static T dot(Vec3<T> a, Vec3<T> b) { return (a[0] * b[0] + a[1] * b[1]) + a[2] * b[2]; }
+template <typename T> static Vec3<T> cross(Vec3<T> a, Vec3<T> b) {
+  return mk(a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]);
+}
+template <typename T> static Vec3<T> matvec(const Mat3<T>& M, Vec3<T> p) {
+  Vec3<T> r;
+  for (int i = 0; i < 3; ++i) r[i] = (M.m[i][0] * p[0] + M.m[i][1] * p[1]) + M.m[i][2] * p[2];
+  return r;
+}
+template <typename T> static Mat3<T> matmul(const Mat3<T>& A, const Mat3<T>& B) {
+  Mat3<T> C;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) C.m[i][j] = (A.m[i][0] * B.m[0][j] + A.m[i][1] * B.m[1][j]) + A.m[i][2] * B.m[2][j];
+  return C;
+}
+template <typename T> static Mat3<T> transp(const Mat3<T>& A) {
+  Mat3<T> C;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) C.m[i][j] = A.m[j][i];
+  return C;
+}
+template <typename T> static Mat3<T> eye() {
+  Mat3<T> C;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) C.m[i][j] = (i == j) ? T(1) : T(0);
+  return C;
+}
+template <typename T> static Mat3<T> skew(Vec3<T> v) {
+  Mat3<T> m;
+  m.m[0][0] = 0;     m.m[0][1] = -v[2]; m.m[0][2] = v[1];
+  m.m[1][0] = v[2];  m.m[1][1] = 0;     m.m[1][2] = -v[0];
+  m.m[2][0] = -v[1]; m.m[2][1] = v[0];  m.m[2][2] = 0;
+  return m;
+}
+// applyIsometry_ / applyRotation_ (geometry_.h:11-37): explicit loop order.
+template <typename T> static Vec3<T> applyIsometry(const Iso3<T>& X, Vec3<T> p) {
+  Vec3<T> r = X.t;
+  for (int i = 0; i < 3; ++i)
+    for (int c = 0; c < 3; ++c) r[i] += X.R.m[i][c] * p[c];
+  return r;
+}
+template <typename T> static Vec3<T> applyRotation(const Iso3<T>& X, Vec3<T> p) {
+  Vec3<T> r = mk(T(0), T(0), T(0));
+  for (int i = 0; i < 3; ++i)
+    for (int c = 0; c < 3; ++c) r[i] += X.R.m[i][c] * p[c];
+  return r;
+}
+// Eigen Isometry product and inverse.
+template <typename T> static Iso3<T> compose(const Iso3<T>& A, const Iso3<T>& B) {
+  Iso3<T> C;
+  C.R = matmul(A.R, B.R);
+  C.t = add(matvec(A.R, B.t), A.t);
+  return C;
+}
+template <typename T> static Iso3<T> inverse(const Iso3<T>& A) {
+  Iso3<T> C;
+  C.R = transp(A.R);
+  Vec3<T> t = matvec(C.R, A.t);
+  C.t = mk(-t[0], -t[1], -t[2]);
+  return C;
+}
+template <typename T, typename S> static Iso3<T> from34(const S* a) {
+  Iso3<T> X;
+  for (int r = 0; r < 3; ++r) {
+    for (int c = 0; c < 3; ++c) X.R.m[r][c] = T(a[r * 4 + c]);
+    X.t[r] = T(a[r * 4 + 3]);
+  }
+  return X;
+}
+template <typename T, typename S> static void to34(const Iso3<T>& X, S* a) {
+  for (int r = 0; r < 3; ++r) {
+    for (int c = 0; c < 3; ++c) a[r * 4 + c] = S(X.R.m[r][c]);
+    a[r * 4 + 3] = S(X.t[r]);
+  }
+}
+template <typename To, typename From> static Iso3<To> cast(const Iso3<From>& A) {
+  Iso3<To> C;
+  for (int r = 0; r < 3; ++r) {
+    for (int c = 0; c < 3; ++c) C.R.m[r][c] = To(A.R.m[r][c]);
+    C.t[r] = To(A.t[r]);
+  }
+  return C;
+}
+
+// ---- rodrigues / expmap / dRxp (geometry_3d_impl_.h) -------------------------
+static void sincos_f(float a, float* s, float* c) { kdh::sincos_shared(a, s, c); }
+static void sincos_f(double a, double* s, double* c) { *s = sin(a); *c = cos(a); }
+
+template <typename T> static Mat3<T> rodrigues(Vec3<T> w) {
+  T angle = sqrt((w[0] * w[0] + w[1] * w[1]) + w[2] * w[2]);
+  if (angle < T(1e-9)) return eye<T>();
+  T inv = T(1.) / angle;
+  Vec3<T> n = mk(w[0] * inv, w[1] * inv, w[2] * inv);
+  Mat3<T> S = skew(n);
+  Mat3<T> S2 = matmul(S, S);
+  T s, c;
+  sincos_f(angle, &s, &c);
+  Mat3<T> R;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) R.m[i][j] = (T(i == j ? 1 : 0) + s * S.m[i][j]) + (T(1.) - c) * S2.m[i][j];
+  return R;
+}
+
+template <typename T> static Iso3<T> expmap(const T* v) {
+  Iso3<T> X;
+  X.t = mk(v[0], v[1], v[2]);
+  X.R = rodrigues(mk(v[3], v[4], v[5]));
+  return X;
+}
+
+template <typename T> static Vec3<T> logSO3(const Mat3<T>& R) {
+  T tr = (R.m[0][0] + R.m[1][1]) + R.m[2][2];
+  T cos_a = T(0.5) * (tr - T(1));
+  cos_a = std::max(T(-1), std::min(T(1), cos_a));
+  T angle = acos(cos_a);
+  Vec3<T> ax = mk(R.m[2][1] - R.m[1][2], R.m[0][2] - R.m[2][0], R.m[1][0] - R.m[0][1]);
+  if (angle < T(1e-7)) return mk(ax[0] * T(0.5), ax[1] * T(0.5), ax[2] * T(0.5));
+  T f = angle / (T(2) * sin(angle));
+  return mk(ax[0] * f, ax[1] * f, ax[2] * f);
+}
+
+// d(R(w) u)/dw (Gallego & Yezzi, geometry_3d_impl_.h:143-174)
+template <typename T> static void dRxp(Vec3<T>& ur, Mat3<T>& dR, Vec3<T> w, Vec3<T> u) {
+  T angle = sqrt((w[0] * w[0] + w[1] * w[1]) + w[2] * w[2]);
+  if (angle < T(1e-9)) {
+    ur = u;
+    Mat3<T> s = skew(u);
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) dR.m[i][j] = -s.m[i][j];
+    return;
+  }
+  T inv = T(1.) / angle;
+  Vec3<T> n = mk(w[0] * inv, w[1] * inv, w[2] * inv);
+  Mat3<T> S = skew(n);
+  Mat3<T> S2 = matmul(S, S);
+  T s, c;
+  sincos_f(angle, &s, &c);
+  Mat3<T> R;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) R.m[i][j] = (T(i == j ? 1 : 0) + s * S.m[i][j]) + (T(1.) - c) * S2.m[i][j];
+  ur = matvec(R, u);
+  Mat3<T> Rt = transp(R);
+  for (int i = 0; i < 3; ++i) Rt.m[i][i] -= T(1);
+  Mat3<T> B = matmul(Rt, S);
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) B.m[i][j] = n[i] * n[j] + B.m[i][j] * inv;
+  Mat3<T> A = matmul(R, skew(u));
+  Mat3<T> D = matmul(A, B);
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) dR.m[i][j] = -D.m[i][j];
+}
+
+// ---- StableAdder_ (stable_adder_.h:21-74) ----------------------------------
+template <typename T, int N> struct Packed {
+  T a[N];
+  void zero() { for (int i = 0; i < N; ++i) a[i] = T(0); }
+  Packed& operator+=(const Packed& o) { for (int i = 0; i < N; ++i) a[i] += o.a[i]; return *this; }
+};
+
+template <typename V, int BATCH = 1024, int LEVELS = 5> struct StableAdder {
+  V value[LEVELS];
+  size_t count[LEVELS];
+  StableAdder() { reset(); }
+  void reset() {
+    for (int i = 0; i < LEVELS; ++i) {
+      value[i].zero();
+      count[i] = 0;
+    }
+  }
+  void add(V v) {
+    int i = 0;
+    while (i < LEVELS) {
+      value[i] += v;
+      ++count[i];
+      if (count[i] < (size_t)BATCH) break;
+      v = value[i];
+      value[i].zero();
+      count[i] = 0;
+      ++i;
+    }
+  }
+  V sum() const {
+    V acc;
+    acc.zero();
+    for (int i = 0; i < LEVELS; ++i) acc += value[i];
+    return acc;
+  }
+};
+
+// ---- the aligner -------------------------------------------------------------
+struct Params {
+  float kernel_threshold, max_error, min_normal_cos, damping, velocity_damping;
+  int min_good_measurements, max_iterations, use_term_crit;
+  float d_pert, d_score;
+};
+
+// ICPStats_ counters + chi2 sums, as a packed float vector for the adder
+// (counters are summed as floats: exact up to 2^24 leaves per adder level).
+enum { S_IN = 0, S_OUT, S_BAD, S_NORM, S_FN, S_CHI2, S_CHI2K, S_N };
+
+struct TermOut {
+  int outcome;        // 0 inlier 1 outlier 2 bad 3 err_normal 4 err_fn
+  int leaf;           // fixed leaf index or -1
+  int depth;
+  float stats[S_N];
+  float acc[90];      // ICP: 27 used, CT: 90
+  double acc64[90];   // FP64 twin
+  double chi2_64, chi2k_64;
+};
+
+struct Cache {
+  Iso3<float> Xm, Xf, Xmf;
+  float v[6];
+};
+
+// Descent (icp_impl_.h:116-121; whichSide node_.h:54-56)
+static int descend(const Node* fixed, Vec3<float> q, int* depth) {
+  int aux = 0;
+  int d = 0;
+  while (aux >= 0 && !fixed[aux].isLeaf()) {
+    const Node& n = fixed[aux];
+    Vec3<float> diff = sub(q, mk(n.mean[0], n.mean[1], n.mean[2]));
+    bool left = dot(mk(n.direction[0], n.direction[1], n.direction[2]), diff) > 0;
+    aux = left ? n.idx_left : n.idx_right;
+    ++d;
+  }
+  *depth = d;
+  return aux;
+}
+
+template <bool CT>
+static void quadraticTerm(TermOut& o, const Node* fixed, const Node& leafnode, const Cache& C, const Params& P,
+                          bool stats_mode) {
+  for (int i = 0; i < S_N; ++i) o.stats[i] = 0.0f;
+  for (int i = 0; i < 90; ++i) {
+    o.acc[i] = 0.0f;
+    o.acc64[i] = 0.0;
+  }
+  o.chi2_64 = o.chi2k_64 = 0.0;
+  Vec3<float> mean = mk(leafnode.mean[0], leafnode.mean[1], leafnode.mean[2]);
+  Vec3<float> dir = mk(leafnode.direction[0], leafnode.direction[1], leafnode.direction[2]);
+  const float dt = (float)leafnode.dts_mean;
+  Vec3<float> wmean = mean, wdir = dir;
+  if (CT) {
+    // CTICP_3D_Traits::applyVelocities (d3/cticp_impl_.h:76-85)
+    Vec3<float> td = mk(C.v[0] * dt, C.v[1] * dt, C.v[2] * dt);
+    Vec3<float> rdl = mk(C.v[3] * dt, C.v[4] * dt, C.v[5] * dt);
+    Mat3<float> R = rodrigues(rdl);
+    wmean = add(matvec(R, mean), td);
+    wdir = matvec(R, dir);
+  }
+  const Vec3<float> p_mif = applyIsometry(C.Xmf, wmean);
+  int depth = 0;
+  const int aux = descend(fixed, p_mif, &depth);
+  o.depth = depth;
+  o.leaf = aux;
+  if (aux < 0) {
+    o.outcome = 2;
+    o.stats[S_BAD] = 1;
+    return;
+  }
+  const Node& fl = fixed[aux];
+  const Vec3<float> pf = mk(fl.mean[0], fl.mean[1], fl.mean[2]);
+  const Vec3<float> nf = mk(fl.direction[0], fl.direction[1], fl.direction[2]);
+  const Vec3<float> p_m = applyIsometry(C.Xm, wmean);
+  const Vec3<float> n_m = applyRotation(C.Xm, wdir);
+  const Vec3<float> p_f = applyIsometry(C.Xf, pf);
+  const Vec3<float> n_f = applyRotation(C.Xf, nf);
+  // errorAndJacobian (d3/icp_.h:41-60)
+  if (dot(n_f, n_m) < P.min_normal_cos) {
+    o.outcome = 3;
+    o.stats[S_NORM] = 1;
+    return;
+  }
+  const Vec3<float> nsum = add(n_m, n_f);
+  const float e = dot(sub(p_m, p_f), nsum);
+  if (fabsf(e) > P.max_error) {
+    o.outcome = 4;
+    o.stats[S_FN] = 1;
+    return;
+  }
+  const Vec3<float> jr = add(cross(p_m, n_f), cross(p_f, n_m));
+  float J[6] = {nsum[0], nsum[1], nsum[2], jr[0], jr[1], jr[2]};
+  float Jv[6] = {0, 0, 0, 0, 0, 0};
+  if (CT) {
+    // errorAndJacobianVelocityA (d3/cticp_impl_.h:11-39)
+    float ma[6];
+    for (int i = 0; i < 6; ++i) ma[i] = C.v[i] * dt;
+    Vec3<float> pa_m, na_m;
+    Mat3<float> dRp, dRn;
+    dRxp(pa_m, dRp, mk(ma[3], ma[4], ma[5]), mean);
+    dRxp(na_m, dRn, mk(ma[3], ma[4], ma[5]), dir);
+    pa_m = add(pa_m, mk(ma[0], ma[1], ma[2]));
+    const Mat3<float>& Ra = C.Xm.R;
+    Vec3<float> pa_g = add(matvec(Ra, pa_m), C.Xm.t);
+    Vec3<float> na_g = matvec(Ra, na_m);
+    Vec3<float> n = add(na_g, n_f);
+    Vec3<float> dp = sub(pa_g, p_f);
+    Mat3<float> RdRp = matmul(Ra, dRp), RdRn = matmul(Ra, dRn);
+    for (int c = 0; c < 3; ++c) {
+      Jv[c] = (n[0] * Ra.m[0][c] + n[1] * Ra.m[1][c]) + n[2] * Ra.m[2][c];
+      float a = (n[0] * RdRp.m[0][c] + n[1] * RdRp.m[1][c]) + n[2] * RdRp.m[2][c];
+      float b = (dp[0] * RdRn.m[0][c] + dp[1] * RdRn.m[1][c]) + dp[2] * RdRn.m[2][c];
+      Jv[3 + c] = a + b;
+    }
+    for (int i = 0; i < 6; ++i) Jv[i] *= dt;
+  }
+  const float chi2 = e * e;
+  float gamma = 1;
+  if (chi2 > P.kernel_threshold) {
+    o.outcome = 1;
+    o.stats[S_OUT] = 1;
+    gamma = stats_mode ? 0.f : sqrtf(P.kernel_threshold / chi2);
+  } else {
+    o.outcome = 0;
+    o.stats[S_IN] = 1;
+  }
+  o.stats[S_CHI2] = chi2;
+  o.stats[S_CHI2K] = chi2 * gamma;
+  int k = 0;
+  for (int r = 0; r < 6; ++r)
+    for (int c = r; c < 6; ++c, ++k) o.acc[k] = J[r] * J[c] * gamma;
+  for (int r = 0; r < 6; ++r, ++k) o.acc[k] = J[r] * e * gamma;
+  if (CT) {
+    for (int r = 0; r < 6; ++r)
+      for (int c = r; c < 6; ++c, ++k) o.acc[k] = Jv[r] * Jv[c] * gamma;
+    for (int r = 0; r < 6; ++r, ++k) o.acc[k] = Jv[r] * e * gamma;
+    for (int r = 0; r < 6; ++r)
+      for (int c = 0; c < 6; ++c, ++k) o.acc[k] = J[r] * Jv[c] * gamma;
+  }
+
+  // FP64 twin: same inputs and decisions, FP64 arithmetic for e, J, gamma.
+  {
+    Iso3<double> Xm = cast<double>(C.Xm), Xf = cast<double>(C.Xf);
+    Vec3<double> wm = mk((double)wmean[0], (double)wmean[1], (double)wmean[2]);
+    Vec3<double> wd = mk((double)wdir[0], (double)wdir[1], (double)wdir[2]);
+    if (CT) {
+      double dtd = (double)dt;
+      double vv[6];
+      for (int i = 0; i < 6; ++i) vv[i] = (double)C.v[i] * dtd;
+      Mat3<double> R = rodrigues(mk(vv[3], vv[4], vv[5]));
+      Vec3<double> md = mk((double)mean[0], (double)mean[1], (double)mean[2]);
+      Vec3<double> dd = mk((double)dir[0], (double)dir[1], (double)dir[2]);
+      wm = add(matvec(R, md), mk(vv[0], vv[1], vv[2]));
+      wd = matvec(R, dd);
+    }
+    Vec3<double> pm = applyIsometry(Xm, wm), nm = applyRotation(Xm, wd);
+    Vec3<double> pfd = applyIsometry(Xf, mk((double)pf[0], (double)pf[1], (double)pf[2]));
+    Vec3<double> nfd = applyRotation(Xf, mk((double)nf[0], (double)nf[1], (double)nf[2]));
+    Vec3<double> ns = add(nm, nfd);
+    double ed = dot(sub(pm, pfd), ns);
+    Vec3<double> jrd = add(cross(pm, nfd), cross(pfd, nm));
+    double Jd[6] = {ns[0], ns[1], ns[2], jrd[0], jrd[1], jrd[2]};
+    double Jvd[6] = {0, 0, 0, 0, 0, 0};
+    if (CT) {
+      double dtd = (double)dt;
+      double ma[6];
+      for (int i = 0; i < 6; ++i) ma[i] = (double)C.v[i] * dtd;
+      Vec3<double> pa_m, na_m;
+      Mat3<double> dRp, dRn;
+      Vec3<double> md = mk((double)mean[0], (double)mean[1], (double)mean[2]);
+      Vec3<double> dd = mk((double)dir[0], (double)dir[1], (double)dir[2]);
+      dRxp(pa_m, dRp, mk(ma[3], ma[4], ma[5]), md);
+      dRxp(na_m, dRn, mk(ma[3], ma[4], ma[5]), dd);
+      pa_m = add(pa_m, mk(ma[0], ma[1], ma[2]));
+      const Mat3<double>& Ra = Xm.R;
+      Vec3<double> pa_g = add(matvec(Ra, pa_m), Xm.t);
+      Vec3<double> na_g = matvec(Ra, na_m);
+      Vec3<double> n = add(na_g, nfd);
+      Vec3<double> dp = sub(pa_g, pfd);
+      Mat3<double> RdRp = matmul(Ra, dRp), RdRn = matmul(Ra, dRn);
+      for (int c = 0; c < 3; ++c) {
+        Jvd[c] = (n[0] * Ra.m[0][c] + n[1] * Ra.m[1][c]) + n[2] * Ra.m[2][c];
+        double a = (n[0] * RdRp.m[0][c] + n[1] * RdRp.m[1][c]) + n[2] * RdRp.m[2][c];
+        double b = (dp[0] * RdRn.m[0][c] + dp[1] * RdRn.m[1][c]) + dp[2] * RdRn.m[2][c];
+        Jvd[3 + c] = a + b;
+      }
+      for (int i = 0; i < 6; ++i) Jvd[i] *= dtd;
+    }
+    double c2 = ed * ed;
+    double g = 1.0;
+    if (o.outcome == 1) g = stats_mode ? 0.0 : sqrt((double)P.kernel_threshold / c2);
+    o.chi2_64 = c2;
+    o.chi2k_64 = c2 * g;
+    int kk = 0;
+    for (int r = 0; r < 6; ++r)
+      for (int c = r; c < 6; ++c, ++kk) o.acc64[kk] = Jd[r] * Jd[c] * g;
+    for (int r = 0; r < 6; ++r, ++kk) o.acc64[kk] = Jd[r] * ed * g;
+    if (CT) {
+      for (int r = 0; r < 6; ++r)
+        for (int c = r; c < 6; ++c, ++kk) o.acc64[kk] = Jvd[r] * Jvd[c] * g;
+      for (int r = 0; r < 6; ++r, ++kk) o.acc64[kk] = Jvd[r] * ed * g;
+      for (int r = 0; r < 6; ++r)
+        for (int c = 0; c < 6; ++c, ++kk) o.acc64[kk] = Jd[r] * Jvd[c] * g;
+    }
+  }
+}
+
+// ---- _buildQuadraticForm (icp_cpu_impl_.h:28-98, cticp_cpu_impl_.h:27-126) ---
+
+struct BuildOut {
+  float acc[90];       // summed per-leaf packs
+  float stats[S_N];
+  double acc64[90];
+  double chi2_64, chi2k_64;
+};
+
+template <bool CT>
+static BuildOut bqf_chunk(const Node* fixed, const Node* moving, const uint64_t* leaves, size_t i0, size_t i1,
+                          const Cache& C, const Params& P, bool stats_mode, int32_t* corr, int32_t* depth) {
+  constexpr int NA = CT ? 90 : 27;
+  StableAdder<Packed<float, S_N>> sa;
+  // The reference keeps one adder per workspace item (diag pose, diag vel,
+  // cross); the items are independent element-wise sums, so one adder over
+  // the concatenation gives identical results.
+  StableAdder<Packed<float, NA>> da;
+  BuildOut r;
+  for (int i = 0; i < 90; ++i) r.acc64[i] = 0.0;
+  r.chi2_64 = r.chi2k_64 = 0.0;
+  TermOut o;
+  for (size_t i = i0; i < i1; ++i) {
+    quadraticTerm<CT>(o, fixed, moving[leaves[i]], C, P, stats_mode);
+    Packed<float, S_N> s;
+    for (int k = 0; k < S_N; ++k) s.a[k] = o.stats[k];
+    Packed<float, NA> d;
+    for (int k = 0; k < NA; ++k) d.a[k] = o.acc[k];
+    sa.add(s);
+    da.add(d);
+    for (int k = 0; k < NA; ++k) r.acc64[k] += o.acc64[k];
+    r.chi2_64 += o.chi2_64;
+    r.chi2k_64 += o.chi2k_64;
+    if (corr) corr[i] = o.leaf;
+    if (depth) depth[i] = o.depth;
+  }
+  Packed<float, S_N> s = sa.sum();
+  Packed<float, NA> d = da.sum();
+  for (int k = 0; k < S_N; ++k) r.stats[k] = s.a[k];
+  for (int k = 0; k < 90; ++k) r.acc[k] = k < NA ? d.a[k] : 0.0f;
+  return r;
+}
+
+struct Form {
+  int D;
+  float H[144], b[12];
+  double H64[144], b64[12];
+  float stats[S_N];
+  double chi2_64, chi2k_64;
+};
+
+static void unpack_diag(const float* acc, float* H, int ld, int o, float* b) {
+  int k = 0;
+  for (int r = 0; r < 6; ++r)
+    for (int c = r; c < 6; ++c, ++k) H[(o + r) * ld + o + c] = H[(o + c) * ld + o + r] = acc[k];
+  for (int r = 0; r < 6; ++r, ++k) b[o + r] = acc[k];
+}
+static void unpack_diag64(const double* acc, double* H, int ld, int o, double* b) {
+  int k = 0;
+  for (int r = 0; r < 6; ++r)
+    for (int c = r; c < 6; ++c, ++k) H[(o + r) * ld + o + c] = H[(o + c) * ld + o + r] = acc[k];
+  for (int r = 0; r < 6; ++r, ++k) b[o + r] = acc[k];
+}
+
+template <bool CT>
+static void assemble(const BuildOut& r, Form& f) {
+  constexpr int D = CT ? 12 : 6;
+  f.D = D;
+  for (int i = 0; i < 144; ++i) {
+    f.H[i] = 0;
+    f.H64[i] = 0;
+  }
+  unpack_diag(r.acc, f.H, D, 0, f.b);
+  unpack_diag64(r.acc64, f.H64, D, 0, f.b64);
+  if (CT) {
+    unpack_diag(r.acc + 27, f.H, D, 6, f.b);
+    unpack_diag64(r.acc64 + 27, f.H64, D, 6, f.b64);
+    for (int i = 0; i < 6; ++i)
+      for (int j = 0; j < 6; ++j) {
+        f.H[i * D + 6 + j] = f.H[(6 + j) * D + i] = r.acc[54 + i * 6 + j];
+        f.H64[i * D + 6 + j] = f.H64[(6 + j) * D + i] = r.acc64[54 + i * 6 + j];
+      }
+  }
+  for (int k = 0; k < S_N; ++k) f.stats[k] = r.stats[k];
+  f.chi2_64 = r.chi2_64;
+  f.chi2k_64 = r.chi2k_64;
+}
+
+// nt = min(num_threads, 8) contiguous chunks of n/nt leaves (the first n % nt
+// chunks one longer), summed in chunk order; nt <= 1 runs one chunk.
+template <bool CT>
+static void build_form(const Node* fixed, int64_t n_fixed, const Node* moving, const uint64_t* leaves,
+                       int64_t n_leaves, const Cache& C, const Params& P, bool stats_mode, int num_threads,
+                       Form& f, int32_t* corr, int32_t* depth) {
+  const int MAX_THREADS = 8;
+  int nt = std::min(num_threads, MAX_THREADS);
+  BuildOut tot;
+  if (n_fixed <= 0) {
+    // An empty fixed tree contributes nothing (the reference GPU kernel returns
+    // before any term; the CPU path would read node 0 of an empty array).
+    memset(&tot, 0, sizeof(tot));
+    for (int64_t i = 0; i < n_leaves; ++i) {
+      if (corr) corr[i] = -1;
+      if (depth) depth[i] = 0;
+    }
+  } else if (nt <= 1) {
+    tot = bqf_chunk<CT>(fixed, moving, leaves, 0, (size_t)n_leaves, C, P, stats_mode, corr, depth);
+  } else {
+    std::future<BuildOut> thr[MAX_THREADS];
+    size_t n = (size_t)n_leaves, chunk = n / nt, rem = n % nt, start = 0;
+    for (int i = 0; i < nt; ++i) {
+      size_t end = start + chunk + (i < (int)rem ? 1 : 0);
+      thr[i] = std::async(std::launch::async, [=] {
+        return bqf_chunk<CT>(fixed, moving, leaves, start, end, C, P, stats_mode, corr, depth);
+      });
+      start = end;
+    }
+    memset(&tot, 0, sizeof(tot));
+    for (int i = 0; i < nt; ++i) {
+      BuildOut r = thr[i].get();
+      // H += ret.H ... in chunk order, on the unpacked blocks; element-wise
+      // equal to adding the packed accumulators.
+      for (int k = 0; k < 90; ++k) {
+        tot.acc[k] += r.acc[k];
+        tot.acc64[k] += r.acc64[k];
+      }
+      for (int k = 0; k < S_N; ++k) tot.stats[k] += r.stats[k];
+      tot.chi2_64 += r.chi2_64;
+      tot.chi2k_64 += r.chi2k_64;
+    }
+  }
+  assemble<CT>(tot, f);
+}
+
+// ---- Eigen-style LDLT with diagonal pivoting, FP32 (icp_impl_.h:82) --------
+template <typename T>
+static void ldlt_solve(int n, const T* A_in, const T* rhs, T* x) {
+  T A[144];
+  int perm[12];
+  for (int i = 0; i < n * n; ++i) A[i] = A_in[i];
+  for (int i = 0; i < n; ++i) perm[i] = i;
+  for (int k = 0; k < n; ++k) {
+    // pivot: largest remaining diagonal magnitude
+    int p = k;
+    T best = fabs(A[k * n + k]);
+    for (int i = k + 1; i < n; ++i)
+      if (fabs(A[i * n + i]) > best) {
+        best = fabs(A[i * n + i]);
+        p = i;
+      }
+    if (p != k) {
+      for (int j = 0; j < n; ++j) std::swap(A[k * n + j], A[p * n + j]);
+      for (int j = 0; j < n; ++j) std::swap(A[j * n + k], A[j * n + p]);
+      std::swap(perm[k], perm[p]);
+    }
+    // A(k,k) -= sum_{j<k} L(k,j)^2 D(j) ; L(i,k) = (A(i,k) - sum L(i,j) L(k,j) D(j)) / D(k)
+    T d = A[k * n + k];
+    for (int j = 0; j < k; ++j) d -= A[k * n + j] * A[k * n + j] * A[j * n + j];
+    A[k * n + k] = d;
+    for (int i = k + 1; i < n; ++i) {
+      T s = A[i * n + k];
+      for (int j = 0; j < k; ++j) s -= A[i * n + j] * A[k * n + j] * A[j * n + j];
+      A[i * n + k] = d != T(0) ? s / d : T(0);
+    }
+  }
+  T y[12];
+  for (int i = 0; i < n; ++i) y[i] = rhs[perm[i]];
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < i; ++j) y[i] -= A[i * n + j] * y[j];
+  for (int i = 0; i < n; ++i) y[i] = A[i * n + i] != T(0) ? y[i] / A[i * n + i] : T(0);
+  for (int i = n - 1; i >= 0; --i)
+    for (int j = i + 1; j < n; ++j) y[i] -= A[j * n + i] * y[j];
+  for (int i = 0; i < n; ++i) x[perm[i]] = y[i];
+}
+
+// ---- pose prior in FP32 (tree_aligner_base_.h:139-152, geometry_3d_impl_.h) ---
+static void dlogSO3_f(const Mat3<float>& R, float J[27]) {
+  for (int i = 0; i < 27; ++i) J[i] = 0;
+  Vec3<float> ax = mk(R.m[2][1] - R.m[1][2], R.m[0][2] - R.m[2][0], R.m[1][0] - R.m[0][1]);
+  float cos_a = 0.5f * (((R.m[0][0] + R.m[1][1]) + R.m[2][2]) - 1.0f);
+  cos_a = std::max(-1.0f, std::min(1.0f, cos_a));
+  float angle = acosf(cos_a);
+  bool small = angle < 1e-7f;
+  float f = small ? 0.5f : angle / (2.0f * sinf(angle));
+  J[0 * 9 + 5] = f; J[0 * 9 + 7] = -f;
+  J[1 * 9 + 6] = f; J[1 * 9 + 2] = -f;
+  J[2 * 9 + 1] = f; J[2 * 9 + 3] = -f;
+  if (!small) {
+    float s = sinf(angle);
+    float c = (s - angle * cosf(angle)) / (4.0f * s * s * s);
+    for (int k = 0; k < 3; ++k) {
+      J[k * 9 + 0] -= c * ax[k];
+      J[k * 9 + 4] -= c * ax[k];
+      J[k * 9 + 8] -= c * ax[k];
+    }
+  }
+}
+
+static void prior_form(const Iso3<float>& ZXi, const Iso3<float>& X, const float* info, float* H6, float* b6) {
+  Iso3<float> P = compose(ZXi, X);
+  Vec3<float> w = logSO3(P.R);
+  float e[6] = {P.t[0], P.t[1], P.t[2], w[0], w[1], w[2]};
+  float B[72];
+  for (int i = 0; i < 72; ++i) B[i] = 0;
+  Mat3<float> Rst = matmul(ZXi.R, skew(X.t));
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) {
+      B[r * 6 + c] = ZXi.R.m[r][c];
+      B[r * 6 + 3 + c] = -Rst.m[r][c];
+    }
+  for (int i = 0; i < 3; ++i) {
+    Mat3<float> Rsc = matmul(ZXi.R, skew(mk(X.R.m[0][i], X.R.m[1][i], X.R.m[2][i])));
+    for (int r = 0; r < 3; ++r)
+      for (int c = 0; c < 3; ++c) B[(3 + 3 * i + r) * 6 + 3 + c] = -Rsc.m[r][c];
+  }
+  float Dl[27];
+  dlogSO3_f(P.R, Dl);
+  float J[36];
+  for (int r = 0; r < 6; ++r)
+    for (int c = 0; c < 6; ++c) {
+      float s = 0;
+      if (r < 3) s = B[r * 6 + c];
+      else
+        for (int k = 0; k < 9; ++k) s += Dl[(r - 3) * 9 + k] * B[(3 + k) * 6 + c];
+      J[r * 6 + c] = s;
+    }
+  for (int r = 0; r < 6; ++r) {
+    float sb = 0;
+    for (int k = 0; k < 6; ++k) {
+      float oe = 0;
+      for (int m = 0; m < 6; ++m) oe += info[k * 6 + m] * e[m];
+      sb += J[k * 6 + r] * oe;
+    }
+    b6[r] = sb;
+    for (int c = 0; c < 6; ++c) {
+      float s = 0;
+      for (int k = 0; k < 6; ++k) {
+        float oj = 0;
+        for (int m = 0; m < 6; ++m) oj += info[k * 6 + m] * J[m * 6 + c];
+        s += J[k * 6 + r] * oj;
+      }
+      H6[r * 6 + c] = s;
+    }
+  }
+}
+
+}  // namespace orc
+
+// ---- C exports (ctypes) -------------------------------------------------------
+using namespace orc;
+
+extern "C" {
+
+struct orc_params {
+  float kernel_threshold, damping, max_error, min_normal_cos;
+  int32_t min_good_measurements, max_iterations;
+  float velocity_damping;
+  int32_t use_term_crit;
+  float term_d_pert, term_d_score;
+  int32_t num_threads;
+};
+
+struct orc_stats {
+  int32_t num_inliers, num_outliers, num_bad, num_err_normal, num_err_fn, pad;
+  double chi2, chi2_kernelized, pert_pose_norm, pert_vel_norm;
+};
+
+static Params to_params(const orc_params* p) {
+  Params P;
+  P.kernel_threshold = p->kernel_threshold;
+  P.max_error = p->max_error;
+  P.min_normal_cos = p->min_normal_cos;
+  P.damping = p->damping;
+  P.velocity_damping = p->velocity_damping;
+  P.min_good_measurements = p->min_good_measurements;
+  P.max_iterations = p->max_iterations;
+  P.use_term_crit = p->use_term_crit;
+  P.d_pert = p->term_d_pert;
+  P.d_score = p->term_d_score;
+  return P;
+}
+
+static void stats_out(const Form& f, orc_stats* s) {
+  s->num_inliers = (int)f.stats[S_IN];
+  s->num_outliers = (int)f.stats[S_OUT];
+  s->num_bad = (int)f.stats[S_BAD];
+  s->num_err_normal = (int)f.stats[S_NORM];
+  s->num_err_fn = (int)f.stats[S_FN];
+  s->pad = 0;
+  s->chi2 = f.stats[S_CHI2];
+  s->chi2_kernelized = f.stats[S_CHI2K];
+  s->pert_pose_norm = 0;
+  s->pert_vel_norm = 0;
+}
+
+// One quadratic form at a given FP32 cache.  H/b: FP32 oracle (D x D, D);
+// H64/b64/chi2_64: the FP64 twin; corr/depth: per moving leaf (may be NULL).
+int orc_build(int ct, const void* fixed_nodes, int64_t n_fixed, const void* moving_nodes, const uint64_t* leaves,
+              int64_t n_leaves, const float* Xm, const float* Xf, const float* Xmf, const float* v,
+              const orc_params* p, int stats_mode, float* H, float* b, double* H64, double* b64, double* chi2_64,
+              orc_stats* st, int32_t* corr, int32_t* depth) {
+  Cache C;
+  C.Xm = from34<float>(Xm);
+  C.Xf = from34<float>(Xf);
+  C.Xmf = from34<float>(Xmf);
+  for (int i = 0; i < 6; ++i) C.v[i] = (ct && v) ? v[i] : 0.0f;
+  Params P = to_params(p);
+  Form f;
+  const Node* fx = (const Node*)fixed_nodes;
+  const Node* mv = (const Node*)moving_nodes;
+  if (ct)
+    build_form<true>(fx, n_fixed, mv, leaves, n_leaves, C, P, stats_mode != 0, p->num_threads, f, corr, depth);
+  else
+    build_form<false>(fx, n_fixed, mv, leaves, n_leaves, C, P, stats_mode != 0, p->num_threads, f, corr, depth);
+  const int D = f.D;
+  if (H) memcpy(H, f.H, sizeof(float) * D * D);
+  if (b) memcpy(b, f.b, sizeof(float) * D);
+  if (H64) memcpy(H64, f.H64, sizeof(double) * D * D);
+  if (b64) memcpy(b64, f.b64, sizeof(double) * D);
+  if (chi2_64) {
+    chi2_64[0] = f.chi2_64;
+    chi2_64[1] = f.chi2k_64;
+  }
+  if (st) stats_out(f, st);
+  return 0;
+}
+
+// updateCache (icp_impl_.h:16-21) in FP32.
+void orc_update_cache(const float* Xm, const float* Xf, float* Xmf) {
+  Iso3<float> m = from34<float>(Xm), f = from34<float>(Xf);
+  to34(compose(inverse(f), m), Xmf);
+}
+
+// TreeAlignerBase_::compute with ICP_/CTICP_::oneRound, FP32 throughout
+// (tree_aligner_base_.h:100-115, icp_impl_.h:53-90, cticp_impl_.h:48-92).
+// X: moving pose in/out (3x4 float), v: velocities in/out (CT), Xf: fixed
+// pose.  prior_* may be NULL.  stats_log: max_iterations entries.
+int orc_compute(int ct, const void* fixed_nodes, int64_t n_fixed, const void* moving_nodes, const uint64_t* leaves,
+                int64_t n_leaves, float* X, float* v, const float* Xf, const orc_params* p,
+                const float* prior_ZXi, const float* prior_info, int stats_mode, orc_stats* stats_log, int* n_log,
+                int* result) {
+  Params P = to_params(p);
+  Iso3<float> Xs = from34<float>(X);
+  Iso3<float> Xfs = from34<float>(Xf);
+  float vel[6];
+  for (int i = 0; i < 6; ++i) vel[i] = (ct && v) ? v[i] : 0.0f;
+  const int D = ct ? 12 : 6;
+  const int max_iter = stats_mode ? 1 : P.max_iterations;
+  float prev_score = 0;
+  int ret = P.max_iterations;
+  int nl = 0;
+  for (int it = 0; it < max_iter; ++it) {
+    Cache C;
+    C.Xm = Xs;
+    C.Xf = Xfs;
+    C.Xmf = compose(inverse(Xfs), Xs);
+    for (int i = 0; i < 6; ++i) C.v[i] = vel[i];
+    Form f;
+    const Node* fx = (const Node*)fixed_nodes;
+    const Node* mv = (const Node*)moving_nodes;
+    if (ct)
+      build_form<true>(fx, n_fixed, mv, leaves, n_leaves, C, P, stats_mode != 0, p->num_threads, f, nullptr, nullptr);
+    else
+      build_form<false>(fx, n_fixed, mv, leaves, n_leaves, C, P, stats_mode != 0, p->num_threads, f, nullptr,
+                        nullptr);
+    orc_stats st;
+    stats_out(f, &st);
+    if (prior_ZXi && prior_info) {
+      float Hp[36], bp[6];
+      prior_form(from34<float>(prior_ZXi), Xs, prior_info, Hp, bp);
+      for (int r = 0; r < 6; ++r) {
+        f.b[r] += bp[r];
+        for (int c = 0; c < 6; ++c) f.H[r * D + c] += Hp[r * 6 + c];
+      }
+    }
+    const int good = st.num_outliers + st.num_inliers;
+    if (good < P.min_good_measurements) {
+      stats_log[nl++] = st;
+      ret = -it;
+      break;
+    }
+    float A[144], rhs[12], dx[12];
+    for (int i = 0; i < D * D; ++i) A[i] = f.H[i];
+    for (int i = 0; i < D; ++i) {
+      A[i * D + i] += (i < 6) ? P.damping : P.velocity_damping;
+      rhs[i] = -f.b[i];
+    }
+    ldlt_solve<float>(D, A, rhs, dx);
+    float pn = 0, vn = 0;
+    for (int i = 0; i < 6; ++i) pn += dx[i] * dx[i];
+    for (int i = 6; i < D; ++i) vn += dx[i] * dx[i];
+    st.pert_pose_norm = sqrtf(pn);
+    st.pert_vel_norm = ct ? sqrtf(vn) : 0.0;
+    stats_log[nl++] = st;
+    if (!stats_mode) {
+      Xs = compose(expmap<float>(dx), Xs);
+      if (ct)
+        for (int i = 0; i < 6; ++i) vel[i] += dx[6 + i];
+    }
+    if (P.use_term_crit) {
+      const int nleaves = st.num_inliers + st.num_outliers + st.num_bad + st.num_err_normal + st.num_err_fn;
+      const float ratio = nleaves ? (float)st.num_inliers / (float)nleaves : 0.0f;
+      const float score = ratio / ((float)st.chi2_kernelized / st.num_inliers + 1e-6f);
+      bool stop = false;
+      if (it > 0) {
+        bool pert_ok = (float)st.pert_pose_norm < P.d_pert;
+        float ds = score - prev_score;
+        bool score_ok = (ds > 0) && ds / (prev_score + 1e-6) < P.d_score;
+        stop = pert_ok && score_ok;
+      }
+      prev_score = score;
+      if (stop) {
+        ret = it;
+        break;
+      }
+    }
+  }
+  to34(Xs, X);
+  if (ct && v)
+    for (int i = 0; i < 6; ++i) v[i] = vel[i];
+  if (n_log) *n_log = nl;
+  if (result) *result = ret;
+  return 0;
+}
+
+// ---- pieces exposed for unit checks ---------------------------------------
+int orc_ldlt_solve_f(int n, const float* A, const float* rhs, float* x) {
+  if (n < 1 || n > 12) return -1;
+  ldlt_solve<float>(n, A, rhs, x);
+  return 0;
+}
+int orc_ldlt_solve_d(int n, const double* A, const double* rhs, double* x) {
+  if (n < 1 || n > 12) return -1;
+  ldlt_solve<double>(n, A, rhs, x);
+  return 0;
+}
+
+void orc_sincos(const float* x, int64_t n, float* s, float* c) {
+  for (int64_t i = 0; i < n; ++i) kdh::sincos_shared(x[i], &s[i], &c[i]);
+}
+
+void orc_rodrigues_f(const float* w, float* R) {
+  Mat3<float> M = rodrigues(mk(w[0], w[1], w[2]));
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) R[i * 3 + j] = M.m[i][j];
+}
+
+void orc_expmap_f(const float* v, float* X34) { to34(expmap<float>(v), X34); }
+
+// errorAndJacobian (d3/icp_.h:41-60): returns 0, -1 (normal gate), -2 (error gate).
+int orc_error_and_jacobian(const float* p_fixed, const float* n_fixed, const float* p_moving, const float* n_moving,
+                           float min_normal_cos, float max_error, float* e, float* J) {
+  Vec3<float> pf = mk(p_fixed[0], p_fixed[1], p_fixed[2]), nf = mk(n_fixed[0], n_fixed[1], n_fixed[2]);
+  Vec3<float> pm = mk(p_moving[0], p_moving[1], p_moving[2]), nm = mk(n_moving[0], n_moving[1], n_moving[2]);
+  if (dot(nf, nm) < min_normal_cos) return -1;
+  Vec3<float> n = add(nm, nf);
+  *e = dot(sub(pm, pf), n);
+  if (fabsf(*e) > max_error) return -2;
+  Vec3<float> jr = add(cross(pm, nf), cross(pf, nm));
+  J[0] = n[0]; J[1] = n[1]; J[2] = n[2]; J[3] = jr[0]; J[4] = jr[1]; J[5] = jr[2];
+  return 0;
+}
+
+// errorAndJacobianVelocityA (d3/cticp_impl_.h:11-39), FP32.
+void orc_error_and_jacobian_velocity_a(const float* Ta34, const float* pa_local, const float* na_local,
+                                       const float* pb_global, const float* nb_global, const float* va, float dt,
+                                       float* e, float* J) {
+  Iso3<float> Ta = from34<float>(Ta34);
+  float ma[6];
+  for (int i = 0; i < 6; ++i) ma[i] = va[i] * dt;
+  Vec3<float> pa_m, na_m;
+  Mat3<float> dRp, dRn;
+  dRxp(pa_m, dRp, mk(ma[3], ma[4], ma[5]), mk(pa_local[0], pa_local[1], pa_local[2]));
+  dRxp(na_m, dRn, mk(ma[3], ma[4], ma[5]), mk(na_local[0], na_local[1], na_local[2]));
+  pa_m = add(pa_m, mk(ma[0], ma[1], ma[2]));
+  const Mat3<float>& Ra = Ta.R;
+  Vec3<float> pa_g = add(matvec(Ra, pa_m), Ta.t);
+  Vec3<float> na_g = matvec(Ra, na_m);
+  Vec3<float> n = add(na_g, mk(nb_global[0], nb_global[1], nb_global[2]));
+  Vec3<float> dp = sub(pa_g, mk(pb_global[0], pb_global[1], pb_global[2]));
+  *e = dot(dp, n);
+  Mat3<float> RdRp = matmul(Ra, dRp), RdRn = matmul(Ra, dRn);
+  for (int c = 0; c < 3; ++c) {
+    J[c] = ((n[0] * Ra.m[0][c] + n[1] * Ra.m[1][c]) + n[2] * Ra.m[2][c]) * dt;
+    float a = (n[0] * RdRp.m[0][c] + n[1] * RdRp.m[1][c]) + n[2] * RdRp.m[2][c];
+    float bb = (dp[0] * RdRn.m[0][c] + dp[1] * RdRn.m[1][c]) + dp[2] * RdRn.m[2][c];
+    J[3 + c] = (a + bb) * dt;
+  }
+}
+
+// StableAdder_<float> over a stream of scalars (for unit checks).
+float orc_stable_sum_f(const float* x, int64_t n) {
+  StableAdder<Packed<float, 1>> a;
+  for (int64_t i = 0; i < n; ++i) {
+    Packed<float, 1> p;
+    p.a[0] = x[i];
+    a.add(p);
+  }
+  return a.sum().a[0];
+}
+
+// Pose prior H/b (FP32) for unit checks against finite differences.
+void orc_prior_form(const float* ZXi34, const float* X34, const float* info, float* H6, float* b6) {
+  prior_form(from34<float>(ZXi34), from34<float>(X34), info, H6, b6);
+}
+
+}  // extern "C"

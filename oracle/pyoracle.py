@@ -1,0 +1,170 @@
+"""ctypes wrapper of oracle/liboracle.so (test infrastructure only).
+
+See oracle/oracle.cpp for the reference functions each entry point restates.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_libs = {}
+
+
+class OrcParams(C.Structure):
+    _fields_ = [
+        ("kernel_threshold", C.c_float),
+        ("damping", C.c_float),
+        ("max_error", C.c_float),
+        ("min_normal_cos", C.c_float),
+        ("min_good_measurements", C.c_int32),
+        ("max_iterations", C.c_int32),
+        ("velocity_damping", C.c_float),
+        ("use_term_crit", C.c_int32),
+        ("term_d_pert", C.c_float),
+        ("term_d_score", C.c_float),
+        ("num_threads", C.c_int32),
+    ]
+
+
+class OrcStats(C.Structure):
+    _fields_ = [
+        ("num_inliers", C.c_int32),
+        ("num_outliers", C.c_int32),
+        ("num_bad", C.c_int32),
+        ("num_err_normal", C.c_int32),
+        ("num_err_fn", C.c_int32),
+        ("pad", C.c_int32),
+        ("chi2", C.c_double),
+        ("chi2_kernelized", C.c_double),
+        ("pert_pose_norm", C.c_double),
+        ("pert_vel_norm", C.c_double),
+    ]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_ if f != "pad"}
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib(fast=False):
+    name = "liboracle_fast.so" if fast else "liboracle.so"
+    if name not in _libs:
+        path = os.path.join(_HERE, name)
+        if not os.path.exists(path):
+            build()
+        L = C.CDLL(path)
+        vp, f32, f64, i32 = C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_double), C.POINTER(C.c_int32)
+        L.orc_build.restype = C.c_int
+        L.orc_build.argtypes = [C.c_int, vp, C.c_int64, vp, vp, C.c_int64, f32, f32, f32, f32, C.POINTER(OrcParams),
+                                C.c_int, f32, f32, f64, f64, f64, C.POINTER(OrcStats), i32, i32]
+        L.orc_compute.restype = C.c_int
+        L.orc_compute.argtypes = [C.c_int, vp, C.c_int64, vp, vp, C.c_int64, f32, f32, f32, C.POINTER(OrcParams), f32,
+                                  f32, C.c_int, C.POINTER(OrcStats), C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        L.orc_update_cache.argtypes = [f32, f32, f32]
+        L.orc_ldlt_solve_f.argtypes = [C.c_int, f32, f32, f32]
+        L.orc_ldlt_solve_d.argtypes = [C.c_int, f64, f64, f64]
+        L.orc_sincos.argtypes = [f32, C.c_int64, f32, f32]
+        L.orc_rodrigues_f.argtypes = [f32, f32]
+        L.orc_expmap_f.argtypes = [f32, f32]
+        L.orc_error_and_jacobian.restype = C.c_int
+        L.orc_error_and_jacobian.argtypes = [f32, f32, f32, f32, C.c_float, C.c_float, f32, f32]
+        L.orc_error_and_jacobian_velocity_a.argtypes = [f32, f32, f32, f32, f32, f32, C.c_float, f32, f32]
+        L.orc_stable_sum_f.restype = C.c_float
+        L.orc_stable_sum_f.argtypes = [f32, C.c_int64]
+        L.orc_prior_form.argtypes = [f32, f32, f32, f32, f32]
+        _libs[name] = L
+    return _libs[name]
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def p32(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def p64(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def pi32(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int32))
+
+
+def make_params(kernel_threshold=0.1, damping=1.0, max_error=1.0, min_normal_cos=None, min_good_measurements=10,
+                max_iterations=20, velocity_damping=100.0, use_term_crit=0, term_d_pert=1e-3, term_d_score=1e-3,
+                num_threads=0):
+    p = OrcParams()
+    p.kernel_threshold = kernel_threshold
+    p.damping = damping
+    p.max_error = max_error
+    p.min_normal_cos = np.float32(np.cos(25.0 / 180.0 * np.pi)) if min_normal_cos is None else min_normal_cos
+    p.min_good_measurements = min_good_measurements
+    p.max_iterations = max_iterations
+    p.velocity_damping = velocity_damping
+    p.use_term_crit = use_term_crit
+    p.term_d_pert = term_d_pert
+    p.term_d_score = term_d_score
+    p.num_threads = num_threads
+    return p
+
+
+def from_kdh_params(kp, num_threads=0):
+    return make_params(kp.kernel_threshold, kp.damping, kp.max_error, kp.min_normal_cos, kp.min_good_measurements,
+                       kp.max_iterations, kp.velocity_damping, kp.use_term_crit, kp.term_d_pert, kp.term_d_score,
+                       num_threads)
+
+
+def build_form(fixed, moving, Xm34, Xf34, Xmf34, params, ct=0, v=None, stats_mode=False, fast=False):
+    """One quadratic form at a given FP32 cache.  fixed/moving: HostTree-like
+    objects with .nodes (64-byte Node_ records) and .leaves."""
+    D = 12 if ct else 6
+    H = np.zeros(D * D, dtype=np.float32)
+    b = np.zeros(D, dtype=np.float32)
+    H64 = np.zeros(D * D)
+    b64 = np.zeros(D)
+    c64 = np.zeros(2)
+    st = OrcStats()
+    n = len(moving.leaves)
+    corr = np.zeros(max(n, 1), dtype=np.int32)
+    depth = np.zeros(max(n, 1), dtype=np.int32)
+    vv = _f32(np.zeros(6) if v is None else v)
+    Xm, Xf, Xmf = _f32(Xm34), _f32(Xf34), _f32(Xmf34)
+    lib(fast).orc_build(ct, fixed.nodes.ctypes.data, len(fixed.nodes), moving.nodes.ctypes.data,
+                        moving.leaves.ctypes.data, n, p32(Xm), p32(Xf), p32(Xmf), p32(vv), C.byref(params),
+                        int(stats_mode), p32(H), p32(b), p64(H64), p64(b64), p64(c64), C.byref(st), pi32(corr),
+                        pi32(depth))
+    return dict(H=H.reshape(D, D), b=b, H64=H64.reshape(D, D), b64=b64, chi2_64=c64[0], chi2k_64=c64[1],
+                stats=st.as_dict(), corr=corr[:n], depth=depth[:n])
+
+
+def compute(fixed, moving, X0, Xf, params, ct=0, v0=None, prior_ZXi=None, prior_info=None, stats_mode=False,
+            fast=False):
+    """TreeAlignerBase_::compute in FP32.  Returns (X 4x4, v, stats_log, result)."""
+    X = _f32(np.asarray(X0, dtype=np.float64)[:3, :4].reshape(12))
+    v = _f32(np.zeros(6) if v0 is None else v0)
+    Xf34 = _f32(np.asarray(Xf, dtype=np.float64)[:3, :4].reshape(12))
+    n = max(1, params.max_iterations)
+    log = (OrcStats * n)()
+    nlog, res = C.c_int(), C.c_int()
+    pz = None if prior_ZXi is None else _f32(prior_ZXi)
+    pinf = None if prior_info is None else _f32(prior_info)
+    lib(fast).orc_compute(ct, fixed.nodes.ctypes.data, len(fixed.nodes), moving.nodes.ctypes.data,
+                          moving.leaves.ctypes.data, len(moving.leaves), p32(X), p32(v), p32(Xf34), C.byref(params),
+                          None if pz is None else p32(pz), None if pinf is None else p32(pinf), int(stats_mode), log,
+                          C.byref(nlog), C.byref(res))
+    Xo = np.eye(4)
+    Xo[:3, :4] = X.reshape(3, 4).astype(np.float64)
+    return Xo, v.astype(np.float64), [log[i].as_dict() for i in range(nlog.value)], res.value
+
+
+def update_cache(Xm34, Xf34):
+    Xm, Xf = _f32(Xm34), _f32(Xf34)
+    out = np.zeros(12, dtype=np.float32)
+    lib().orc_update_cache(p32(Xm), p32(Xf), p32(out))
+    return out
