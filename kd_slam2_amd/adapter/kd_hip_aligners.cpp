@@ -1,0 +1,373 @@
+// Implementation of the ICPHIP3D / CTICPHIP3D adapter (see kd_hip_aligners.h).
+#include "kd_hip_aligners.h"
+
+#include <math.h>
+#include <string.h>
+
+#include <mutex>
+#include <stdexcept>
+
+#include "../csrc/kdh_solve.h"
+
+namespace kd_slam {
+namespace hip {
+
+namespace {
+std::mutex g_ctx_mu;
+kdh_ctx* g_ctx[16] = {};
+
+void check(kdh_ctx* ctx, int rc, const char* what) {
+  if (rc != KDH_OK) throw std::runtime_error(std::string(what) + ": " + kdh_last_error(ctx));
+}
+
+// updateCache (icp/icp_impl_.h:16-21): FP32 Xm, Xf, Xmf = Xf^-1 Xm, formed
+// from the FP64 state and rounded once (the device loop does the same).
+void make_cache(const double* X, const double* Xf, float* Xm32, float* Xf32, float* Xmf32) {
+  kdh::IsoD m = kdh::iso_from_34<double>(X);
+  kdh::IsoD f = kdh::iso_from_34<double>(Xf);
+  kdh::iso_to_34(m, Xm32);
+  kdh::iso_to_34(f, Xf32);
+  kdh::iso_to_34(kdh::compose(kdh::inverse(f), m), Xmf32);
+}
+}  // namespace
+
+kdh_ctx* defaultContext(int device) {
+  if (device < 0 || device >= 16) throw std::runtime_error("device out of range");
+  std::lock_guard<std::mutex> lk(g_ctx_mu);
+  if (!g_ctx[device]) {
+    int rc = kdh_ctx_create(device, &g_ctx[device]);
+    if (rc != KDH_OK) throw std::runtime_error("kdh_ctx_create failed");
+  }
+  return g_ctx[device];
+}
+
+TreeHIP3D::TreeHIP3D(kdh_ctx* ctx, const kdh_node3f* nodes, int64_t n_nodes, const uint64_t* leaves,
+                     int64_t n_leaves)
+    : ctx_(ctx), n_leaves_(n_leaves) {
+  check(ctx_, kdh_tree_upload(ctx_, nodes, n_nodes, leaves, n_leaves, &h_), "kdh_tree_upload");
+}
+
+TreeHIP3D::~TreeHIP3D() {
+  if (h_) kdh_tree_free(ctx_, h_);
+}
+
+static std::shared_ptr<TreeHIP3D> toComputeCPUOnly(const TreeCPU3D&) {
+  throw std::runtime_error("no GPU library loaded");
+}
+static std::shared_ptr<TreeHIP3D> toComputeHIP(const TreeCPU3D& t) {
+  return std::make_shared<TreeHIP3D>(defaultContext(0), t.nodes.data(), (int64_t)t.nodes.size(), t.leaves.data(),
+                                     (int64_t)t.leaves.size());
+}
+ToComputeFn g_tree_toCompute3d = toComputeCPUOnly;
+
+bool ICPTerminationCriteriaScore3D::hasToStop(ICPStats& s, int it) {
+  if (it > 0) {
+    bool pert_ok = s.pert_pose_norm < param_d_pert.value();
+    float ds = s.score() - prev_score;
+    bool score_ok = (ds > 0) && ds / (prev_score + 1e-6) < param_d_score.value();
+    if (pert_ok && score_ok) return true;
+  }
+  prev_score = s.score();
+  return false;
+}
+
+template <int CT>
+AlignerHIP3D<CT>::AlignerHIP3D() {
+  kdh_params_default(&params);
+}
+
+template <int CT>
+Isometry3f AlignerHIP3D<CT>::getX() const {
+  Isometry3f r;
+  for (int i = 0; i < 12; ++i) r.m[i] = (float)X[i];
+  return r;
+}
+template <int CT>
+void AlignerHIP3D<CT>::setX(const Isometry3f& x) {
+  for (int i = 0; i < 12; ++i) X[i] = x.m[i];
+}
+template <int CT>
+void AlignerHIP3D<CT>::setFixedX(const Isometry3f& x) {
+  for (int i = 0; i < 12; ++i) Xf[i] = x.m[i];
+  if (CT) {
+    // CTICP_::setFixedX also clears the fixed velocities (cticp_.h:222-225);
+    // the fixed cloud carries no velocity state here.
+  }
+}
+template <int CT>
+Vector6f AlignerHIP3D<CT>::velocities() const {
+  Vector6f r{};
+  for (int i = 0; i < 6; ++i) r[i] = CT ? (float)v[i] : 0.f;
+  return r;
+}
+template <int CT>
+void AlignerHIP3D<CT>::setVelocities(const Vector6f& vv) {
+  if (!CT) return;  // TreeAlignerBase_ default: no-op
+  for (int i = 0; i < 6; ++i) v[i] = vv[i];
+}
+template <int CT>
+void AlignerHIP3D<CT>::resetMovingState() {
+  for (int i = 0; i < 6; ++i) v[i] = 0;
+}
+template <int CT>
+void AlignerHIP3D<CT>::setPosePrior(const Isometry3f& X_ref, const Isometry3f& Z, const Matrix6f& info) {
+  _has_prior = false;
+  for (int i = 0; i < 36; ++i) _has_prior = _has_prior || info[i] != 0.f;
+  for (int i = 0; i < 12; ++i) {
+    _prior.X_ref[i] = X_ref.m[i];
+    _prior.Z[i] = Z.m[i];
+  }
+  for (int i = 0; i < 36; ++i) _prior.info[i] = info[i];
+}
+
+template <int CT>
+void AlignerHIP3D<CT>::syncParams() {
+  if (!_params_changed) return;
+  params.kernel_threshold = param_kernel_threshold.value();
+  params.damping = param_damping.value();
+  params.max_error = param_max_error.value();
+  params.min_normal_cos = param_min_normal_cos.value();
+  params.min_good_measurements = param_min_good_measurements.value();
+  params.max_iterations = param_max_iterations.value();
+  params.velocity_damping = param_velocity_damping.value();
+  params.recenter = param_recenter.value() ? 1 : 0;
+  auto tc = param_term_crit.value();
+  params.use_term_crit = tc ? 1 : 0;
+  if (tc) {
+    params.term_d_pert = tc->param_d_pert.value();
+    params.term_d_score = tc->param_d_score.value();
+  }
+  _params_changed = false;
+}
+
+template <int CT>
+void AlignerHIP3D<CT>::buildQuadraticForm(bool stats_mode) {
+  syncParams();
+  if (!_fixed || !_moving) throw std::runtime_error("setFixed/setMoving not called");
+  kdh_ctx* ctx = _ctx ? _ctx : defaultContext(0);
+  float Xm32[12], Xf32[12], Xmf32[12];
+  make_cache(X, Xf, Xm32, Xf32, Xmf32);
+  kdh_stats st;
+  if (CT) {
+    float v32[6];
+    for (int i = 0; i < 6; ++i) v32[i] = (float)v[i];
+    check(ctx, kdh_cticp_build(ctx, _fixed, _moving, Xm32, Xf32, Xmf32, v32, &params,
+                               stats_mode ? 1 : 0, H, b, &st),
+          "kdh_cticp_build");
+  } else {
+    check(ctx, kdh_icp_build(ctx, _fixed, _moving, Xm32, Xf32, Xmf32, &params,
+                             stats_mode ? 1 : 0, H, b, &st),
+          "kdh_icp_build");
+  }
+  stats.num_inliers = st.num_inliers;
+  stats.num_outliers = st.num_outliers;
+  stats.num_bad = st.num_bad;
+  stats.num_err_normal = st.num_err_normal;
+  stats.num_err_fn = st.num_err_fn;
+  stats.chi2 = (float)st.chi2;
+  stats.chi2_kernelized = (float)st.chi2_kernelized;
+  stats.pert_pose_norm = 0;
+  stats.pert_vel_norm = 0;
+}
+
+// ICP_::oneRound / CTICP_::oneRound (icp/icp_impl_.h:53-90, cticp/cticp_impl_.h:48-92)
+// with the damped system solved in FP64.
+template <int CT>
+bool AlignerHIP3D<CT>::oneRound(bool stats_mode) {
+  buildQuadraticForm(stats_mode);
+  double Hs[D * D], bs[D];
+  memcpy(Hs, H, sizeof(Hs));
+  memcpy(bs, b, sizeof(bs));
+  if (_has_prior) {
+    kdh::IsoD Xc = kdh::iso_from_34<double>(X);
+    kdh::IsoD ZXi = kdh::inverse(kdh::compose(kdh::iso_from_34<double>(_prior.X_ref),
+                                              kdh::iso_from_34<double>(_prior.Z)));
+    double e[6], J[36];
+    kdh::logmap_d(kdh::compose(ZXi, Xc), e);
+    kdh::dPoseToPoseError_d(ZXi, Xc, J);
+    for (int r = 0; r < 6; ++r) {
+      for (int c = 0; c < 6; ++c) {
+        double s = 0;
+        for (int k = 0; k < 6; ++k)
+          for (int m = 0; m < 6; ++m) s += J[k * 6 + r] * _prior.info[k * 6 + m] * J[m * 6 + c];
+        Hs[r * D + c] += s;
+      }
+      double sb = 0;
+      for (int k = 0; k < 6; ++k)
+        for (int m = 0; m < 6; ++m) sb += J[k * 6 + r] * _prior.info[k * 6 + m] * e[m];
+      bs[r] += sb;
+    }
+  }
+  memcpy(H, Hs, sizeof(Hs));
+  memcpy(b, bs, sizeof(bs));
+  if (stats.num_outliers + stats.num_inliers < params.min_good_measurements) return false;
+  double A[D * D], rhs[D];
+  memcpy(A, Hs, sizeof(A));
+  for (int i = 0; i < D; ++i) {
+    A[i * D + i] += (i < 6) ? params.damping : params.velocity_damping;
+    rhs[i] = -bs[i];
+  }
+  kdh::ldlt_solve_d<D>(A, rhs, dx);
+  double pn = 0, vn = 0;
+  for (int i = 0; i < 6; ++i) pn += dx[i] * dx[i];
+  for (int i = 6; i < D; ++i) vn += dx[i] * dx[i];
+  stats.pert_pose_norm = (float)sqrt(pn);
+  stats.pert_vel_norm = (float)sqrt(vn);
+  if (stats_mode) return true;
+  kdh::IsoD Xn = kdh::compose(kdh::expmap_d(dx), kdh::iso_from_34<double>(X));
+  kdh::iso_to_34(Xn, X);
+  if (CT)
+    for (int i = 0; i < 6; ++i) v[i] += dx[6 + i];
+  return true;
+}
+
+// TreeAlignerBase_::compute (tree_aligner_base_.h:100-115)
+template <int CT>
+int AlignerHIP3D<CT>::compute(bool stats_mode) {
+  syncParams();
+  stats_log.clear();
+  const int max_iter = stats_mode ? 1 : param_max_iterations.value();
+  for (int i = 0; i < max_iter; ++i) {
+    bool ok = oneRound(stats_mode);
+    stats_log.push_back(stats);
+    if (!ok) return -i;
+    auto tc = param_term_crit.value();
+    if (tc && tc->hasToStop(stats, i)) return i;
+  }
+  return param_max_iterations.value();
+}
+
+template <int CT>
+int AlignerHIP3D<CT>::computeOnDevice(bool stats_mode) {
+  syncParams();
+  if (!_fixed || !_moving) throw std::runtime_error("setFixed/setMoving not called");
+  kdh_ctx* ctx = _ctx ? _ctx : defaultContext(0);
+  std::vector<kdh_stats> log(std::max(1, param_max_iterations.value()));
+  int n = 0, res = 0;
+  double Xo[12], vo[6];
+  check(ctx, kdh_align(ctx, _fixed, _moving, CT, X, v, Xf, &params,
+                       _has_prior ? &_prior : nullptr, stats_mode ? 1 : 0, Xo, vo, log.data(), &n, &res, nullptr),
+        "kdh_align");
+  memcpy(X, Xo, sizeof(X));
+  if (CT) memcpy(v, vo, sizeof(v));
+  stats_log.clear();
+  for (int i = 0; i < n; ++i) {
+    ICPStats s;
+    s.num_inliers = log[i].num_inliers;
+    s.num_outliers = log[i].num_outliers;
+    s.num_bad = log[i].num_bad;
+    s.num_err_normal = log[i].num_err_normal;
+    s.num_err_fn = log[i].num_err_fn;
+    s.chi2 = (float)log[i].chi2;
+    s.chi2_kernelized = (float)log[i].chi2_kernelized;
+    s.pert_pose_norm = (float)log[i].pert_pose_norm;
+    s.pert_vel_norm = (float)log[i].pert_vel_norm;
+    stats_log.push_back(s);
+  }
+  if (n) stats = stats_log.back();
+  return res;
+}
+
+template struct AlignerHIP3D<0>;
+template struct AlignerHIP3D<1>;
+
+// ---- registration (d3/icp.cu:13-17, d3/cticp.cu:14-18, d3/tree.cu:22-30) ----
+void registerHIPTypes3D() {
+  BOSS_REGISTER_CLASS(ICPHIP3D);
+  BOSS_REGISTER_CLASS(CTICPHIP3D);
+  BOSS_REGISTER_CLASS(ICPTerminationCriteriaScore3D);
+  g_tree_toCompute3d = toComputeHIP;
+}
+
+void __attribute__((constructor)) initializeHIPTypes3D() { registerHIPTypes3D(); }
+
+}  // namespace hip
+}  // namespace kd_slam
+
+// ---- C entry points used by tests and bench.py ------------------------------
+using namespace kd_slam::hip;
+
+namespace {
+
+template <typename A>
+int run_aligner(A* al, kdh_ctx* ctx, const kdh_tree* fixed, const kdh_tree* moving, const double* X0,
+                const double* v0, const double* Xf, const kdh_params* p, int on_device, int stats_mode,
+                double* X_out, double* v_out, int* result, int* n_iter, kdh_stats* last) {
+  al->setContext(ctx);
+  al->param_kernel_threshold.setValue(p->kernel_threshold);
+  al->param_damping.setValue(p->damping);
+  al->param_max_error.setValue(p->max_error);
+  al->param_min_normal_cos.setValue(p->min_normal_cos);
+  al->param_min_good_measurements.setValue(p->min_good_measurements);
+  al->param_max_iterations.setValue(p->max_iterations);
+  al->param_velocity_damping.setValue(p->velocity_damping);
+  al->param_recenter.setValue(p->recenter != 0);
+  if (p->use_term_crit) {
+    auto tc = std::make_shared<ICPTerminationCriteriaScore3D>();
+    tc->param_d_pert.setValue(p->term_d_pert);
+    tc->param_d_score.setValue(p->term_d_score);
+    al->param_term_crit.setValue(tc);
+  }
+  al->setFixed(fixed);
+  al->setMoving(moving);
+  memcpy(al->X, X0, sizeof(al->X));
+  memcpy(al->Xf, Xf, sizeof(al->Xf));
+  for (int i = 0; i < 6; ++i) al->v[i] = v0 ? v0[i] : 0.0;
+  *result = on_device ? al->computeOnDevice(stats_mode != 0) : al->compute(stats_mode != 0);
+  memcpy(X_out, al->X, sizeof(al->X));
+  if (v_out) memcpy(v_out, al->v, sizeof(al->v));
+  *n_iter = (int)al->stats_log.size();
+  if (last) {
+    const ICPStats& s = al->stats;
+    last->num_inliers = s.num_inliers;
+    last->num_outliers = s.num_outliers;
+    last->num_bad = s.num_bad;
+    last->num_err_normal = s.num_err_normal;
+    last->num_err_fn = s.num_err_fn;
+    last->pad = 0;
+    last->chi2 = s.chi2;
+    last->chi2_kernelized = s.chi2_kernelized;
+    last->pert_pose_norm = s.pert_pose_norm;
+    last->pert_vel_norm = s.pert_vel_norm;
+  }
+  return KDH_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+// Comma-separated registered type names; returns the needed buffer size when
+// buf is NULL.
+int kdha_registered_types(char* buf, int len) {
+  std::string s;
+  for (auto& n : srrg2_core::TypeRegistry::instance().names()) s += (s.empty() ? "" : ",") + n;
+  if (!buf || len <= 0) return (int)s.size() + 1;
+  strncpy(buf, s.c_str(), (size_t)len - 1);
+  buf[len - 1] = 0;
+  return 0;
+}
+
+// Creates `type_name` through the registry, configures it through its PARAM
+// properties and runs compute() (on_device = 0: the plain per-call path) or
+// computeOnDevice() on trees uploaded with kdh_tree_upload.
+int kdha_align(const char* type_name, kdh_ctx* ctx, const kdh_tree* fixed, const kdh_tree* moving,
+               const double X0[12], const double v0[6], const double Xf[12], const kdh_params* p, int on_device,
+               int stats_mode, double X_out[12], double v_out[6], int* result, int* n_iter, kdh_stats* last) {
+  if (!type_name || !ctx || !fixed || !moving || !X0 || !Xf || !p || !X_out || !result || !n_iter)
+    return KDH_ERR_INVALID;
+  try {
+    auto obj = srrg2_core::TypeRegistry::instance().create(type_name);
+    if (auto* a = dynamic_cast<ICPHIP3D*>(obj.get()))
+      return run_aligner(a, ctx, fixed, moving, X0, v0, Xf, p, on_device, stats_mode, X_out, v_out, result, n_iter,
+                         last);
+    if (auto* c = dynamic_cast<CTICPHIP3D*>(obj.get()))
+      return run_aligner(c, ctx, fixed, moving, X0, v0, Xf, p, on_device, stats_mode, X_out, v_out, result, n_iter,
+                         last);
+    return KDH_ERR_INVALID;
+  } catch (const std::exception&) {
+    return KDH_ERR_INVALID;
+  }
+}
+
+}  // extern "C"

@@ -58,8 +58,12 @@ def from34(a):
 def pose_error(Xa, Xb):
     """(translation error in m, rotation error in deg) between two poses."""
     D = np.linalg.inv(Xa) @ Xb
-    c = (np.trace(D[:3, :3]) - 1) / 2
-    ang = math.degrees(math.acos(max(-1.0, min(1.0, c))))
+    R = D[:3, :3]
+    # atan2(|vee(R - R^T)| / 2, (tr R - 1) / 2) is well conditioned at small
+    # angles, where acos of the trace term loses half the digits.
+    s = 0.5 * math.sqrt((R[2, 1] - R[1, 2]) ** 2 + (R[0, 2] - R[2, 0]) ** 2 + (R[1, 0] - R[0, 1]) ** 2)
+    c = (np.trace(R) - 1) / 2
+    ang = math.degrees(math.atan2(s, c))
     return float(np.linalg.norm(Xa[:3, 3] - Xb[:3, 3])), ang
 
 
