@@ -60,7 +60,8 @@ def main():
               "with no communication, see DESIGN.md", file=sys.stderr)
         sys.exit(2)
 
-    from kd_slam2_amd import CTICPHIP3D, Context, DeviceTree, TreeGenerator3D, batch_align, default_params, synth
+    from kd_slam2_amd import (CTICPHIP3D, Context, DeviceTree, TreeGenerator3D, adapter_align, batch_align,
+                              default_params, synth)
 
     t0 = time.time()
     w = synth.config2()
@@ -93,12 +94,13 @@ def main():
     n_tiles = (moving.num_leaves + 255) // 256
     b_iter = algorithmic_bytes_per_iter(depth, 1, n_tiles)
 
-    # ---- plain per-call path (host loop) --------------------------------
+    # ---- plain per-call path: CTICPHIP3D::compute() of the C++ plugin -----
+    # (host Gauss-Newton loop, one kdh_cticp_build + host FP64 solve per
+    # iteration -- the drop-in for CTICPCUDA3D)
     plain_lat = []
     for i in range(args.plain_runs + 1):
-        al = aligner()
         t = time.perf_counter()
-        al.compute()
+        Xp, vp, res_p, _, _ = adapter_align("CTICPHIP3D", ctx, F, M, w["X0"], w["Xf"], p, v0=w["v0"])
         dt = time.perf_counter() - t
         if i > 0:
             plain_lat.append(dt)
@@ -140,11 +142,23 @@ def main():
     log(f"served: {1e3 * np.mean(step_t):.2f} ms/step of {args.batch}, {value / 1e6:.1f} Mpts/s, "
         f"launch {np.median(batch_kern):.1f} us, max err {max(e[0] for e in errs):.4f} m")
 
-    # roofline of the tile kernel in the served launch
-    kern_s = np.median(batch_kern) * 1e-6
-    achieved = args.batch * b_iter / kern_s / 1e9
+    # roofline of the tile kernel in the served launch: one extra profiled
+    # batch (no graph, HIP events around the tile kernel and the finish
+    # kernel separately, on the context stream)
+    ctx.set_profiling(True)
+    prof_tile, prof_fin = [], []
+    for _ in range(3):
+        batch_align(ctx, pairs)
+        tu, fu = ctx.kernel_breakdown()
+        prof_tile.append(tu)
+        prof_fin.append(fu)
+    ctx.set_profiling(False)
+    tile_us = float(np.median(prof_tile))
+    achieved = args.batch * b_iter / (tile_us * 1e-6) / 1e9
     roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                     frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None)
+    log(f"tile kernel {tile_us:.1f} us, finish kernel {np.median(prof_fin):.1f} us per iteration "
+        f"-> {achieved:.0f} GB/s algorithmic")
 
     cpu = None
     if not args.no_cpu:
@@ -188,7 +202,9 @@ def main():
                         served_batch_p50=round(1e3 * float(np.percentile(step_t, 50)), 3),
                         served_batch_p99=round(1e3 * float(np.percentile(step_t, 99)), 3)),
         kernel_us_per_iter=dict(device_loop=round(float(np.median(kern_us)), 2),
-                                served_launch=round(float(np.median(batch_kern)), 2)),
+                                served_launch=round(float(np.median(batch_kern)), 2),
+                                served_tile_kernel=round(tile_us, 2),
+                                served_finish_kernel=round(float(np.median(prof_fin)), 2)),
         algorithmic_bytes_per_iter=b_iter,
         roofline=roofline,
         cpu_baseline=cpu,

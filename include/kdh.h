@@ -196,6 +196,14 @@ int kdh_batch_align(kdh_ctx* ctx, int n_pairs, const kdh_pair* pairs, kdh_pair_r
  * HIP events recorded on the context stream. */
 int kdh_last_kernel_time(kdh_ctx* ctx, double* avg_us, int* n_launches);
 
+/* Profiling switch: when on, kdh_align / kdh_batch_align launch without a
+ * graph and record HIP events around the tile kernel (run 4 times back to
+ * back -- it is idempotent -- to amortise dispatch) and around the finish
+ * kernel; kdh_last_kernel_breakdown then reports both averages (us per
+ * iteration).  Off by default. */
+int kdh_set_profiling(kdh_ctx* ctx, int on);
+int kdh_last_kernel_breakdown(kdh_ctx* ctx, double* tile_us, double* finish_us);
+
 /* Tuning knobs (project autotuner): workgroups per launch (0 = one per tile)
  * and LDS-staged top-of-tree nodes (0/256/512/1024/2048). */
 int kdh_set_launch_config(kdh_ctx* ctx, int grid, int lds_nodes);

@@ -11,6 +11,7 @@ from .api import (  # noqa: F401
     HostTree,
     ICPHIP3D,
     TreeGenerator3D,
+    adapter_align,
     batch_align,
     default_params,
     update_cache_f32,

@@ -157,6 +157,8 @@ _SIGS = {
     "kdh_batch_align": (C.c_int, [P, C.c_int, C.POINTER(Pair), C.POINTER(PairResult)]),
     "kdh_last_kernel_time": (C.c_int, [P, C.POINTER(C.c_double), C.POINTER(C.c_int)]),
     "kdh_set_launch_config": (C.c_int, [P, C.c_int, C.c_int]),
+    "kdh_set_profiling": (C.c_int, [P, C.c_int]),
+    "kdh_last_kernel_breakdown": (C.c_int, [P, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "kdh_tree_gen_params_default": (None, [C.POINTER(TreeGenParams)]),
     "kdh_host_tree_build": (C.c_int, [F32P, F64P, C.c_int64, C.POINTER(TreeGenParams), C.POINTER(P)]),
     "kdh_host_tree_info": (C.c_int, [P, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_double)]),
@@ -165,9 +167,6 @@ _SIGS = {
     "kdh_host_tree_points": (P, [P]),
     "kdh_host_tree_free": (C.c_int, [P]),
 }
-
-# Entry points added after the first release of the ABI are bound when present.
-_OPTIONAL = set()
 
 
 def lib():
@@ -181,13 +180,32 @@ def lib():
             )
         L = C.CDLL(LIB_PATH)
         for name, (res, args) in _SIGS.items():
-            if name in _OPTIONAL and not hasattr(L, name):
-                continue
             f = getattr(L, name)
             f.restype = res
             f.argtypes = args
         _lib = L
     return _lib
+
+
+ADAPTER_PATH = os.path.join(_HERE, "libkd_slam_hip_3d.so")
+_adapter = None
+
+
+def adapter():
+    """Load libkd_slam_hip_3d.so (the ICPHIP3D / CTICPHIP3D configurable types)."""
+    global _adapter
+    if _adapter is None:
+        lib()  # the adapter links against libkdhip.so
+        if not os.path.exists(ADAPTER_PATH):
+            raise RuntimeError(f"{ADAPTER_PATH} is missing: build it with kd_slam2_amd.build.build()")
+        A = C.CDLL(ADAPTER_PATH)
+        A.kdha_registered_types.restype = C.c_int
+        A.kdha_registered_types.argtypes = [C.c_char_p, C.c_int]
+        A.kdha_align.restype = C.c_int
+        A.kdha_align.argtypes = [C.c_char_p, P, P, P, F64P, F64P, F64P, C.POINTER(Params), C.c_int, C.c_int, F64P,
+                                 F64P, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(Stats)]
+        _adapter = A
+    return _adapter
 
 
 def f32p(a):

@@ -114,6 +114,16 @@ class Context:
     def set_launch_config(self, grid=0, lds_nodes=1024):
         self.check(L.lib().kdh_set_launch_config(self.h, grid, lds_nodes), "kdh_set_launch_config")
 
+    def set_profiling(self, on=True):
+        self.check(L.lib().kdh_set_profiling(self.h, int(bool(on))), "kdh_set_profiling")
+
+    def kernel_breakdown(self):
+        """(tile kernel us, finish kernel us) per iteration of the last
+        profiled kdh_align / kdh_batch_align."""
+        t, f = C.c_double(), C.c_double()
+        self.check(L.lib().kdh_last_kernel_breakdown(self.h, C.byref(t), C.byref(f)), "kdh_last_kernel_breakdown")
+        return t.value, f.value
+
     def last_kernel_time(self):
         us, n = C.c_double(), C.c_int()
         self.check(L.lib().kdh_last_kernel_time(self.h, C.byref(us), C.byref(n)), "kdh_last_kernel_time")
@@ -426,6 +436,28 @@ class CTICPHIP3D(_AlignerBase):
 
     def setFixedX(self, X):
         self.Xf = np.array(X, dtype=np.float64)
+
+
+def adapter_align(type_name, ctx, fixed, moving, X0, Xf, params, v0=None, on_device=False, stats_mode=False):
+    """Create `type_name` ("ICPHIP3D" / "CTICPHIP3D") through the C++ plugin
+    registry and run compute() (plain per-call path) or computeOnDevice().
+    Returns (X 4x4, v, result, iterations, last stats)."""
+    A = L.adapter()
+    X0a = np.ascontiguousarray(np.asarray(X0, dtype=np.float64)[:3, :4].reshape(12))
+    Xfa = np.ascontiguousarray(np.asarray(Xf, dtype=np.float64)[:3, :4].reshape(12))
+    v0a = np.ascontiguousarray(np.zeros(6) if v0 is None else v0, dtype=np.float64)
+    Xo = np.zeros(12)
+    vo = np.zeros(6)
+    res, nit = C.c_int(), C.c_int()
+    st = L.Stats()
+    rc = A.kdha_align(type_name.encode(), ctx.h, fixed.h, moving.h, L.f64p(X0a), L.f64p(v0a), L.f64p(Xfa),
+                      C.byref(params), int(on_device), int(stats_mode), L.f64p(Xo), L.f64p(vo), C.byref(res),
+                      C.byref(nit), C.byref(st))
+    if rc != L.KDH_OK:
+        raise RuntimeError(f"kdha_align({type_name}) failed: {L.ERRORS.get(rc, rc)}")
+    X = np.eye(4)
+    X[:3, :4] = Xo.reshape(3, 4)
+    return X, vo, res.value, nit.value, st.as_dict()
 
 
 def batch_align(ctx, pairs):

@@ -10,4 +10,7 @@ def build(jobs=8):
     csrc = os.path.join(_HERE, "csrc")
     subprocess.run(["make", "-s", "-C", csrc, "clean"], check=True)
     subprocess.run(["make", "-s", "-C", csrc, f"-j{jobs}"], check=True)
+    adapter = os.path.join(_HERE, "adapter")
+    subprocess.run(["make", "-s", "-C", adapter, "clean"], check=True)
+    subprocess.run(["make", "-s", "-C", adapter], check=True)
     return os.path.join(_HERE, "libkdhip.so")

@@ -95,43 +95,73 @@ int kdh_batch_align(kdh_ctx* ctx, int n_pairs, const kdh_pair* pairs, kdh_pair_r
   KDH_HIP_TRY(ctx, hipMemcpyAsync(ctx->d_pairs, ctx->h_pairs, sizeof(PairState) * n_pairs, hipMemcpyHostToDevice,
                                   ctx->stream));
 
-  // Capture one iteration (one launch per non-empty group) as a graph.
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t exec = nullptr;
-  KDH_HIP_TRY(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-  int lrc = KDH_OK;
-  for (int k = 0; k < 2 && lrc == KDH_OK; ++k) {
-    if (g[k].n_tiles == 0) continue;
-    lrc = kdh_launch(ctx, ctx->d_pairs, ctx->d_tile_pair + g[k].tile_off, g[k].n_tiles, g[k].ct, nullptr, nullptr,
-                     ctx->d_partials + g[k].partial_off);
-  }
-  hipError_t ce = hipStreamEndCapture(ctx->stream, &graph);
-  if (lrc != KDH_OK) {
-    if (graph) hipGraphDestroy(graph);
-    return lrc;
-  }
-  if (ce != hipSuccess) return kdh_hip_fail(ctx, ce, "hipStreamEndCapture");
-  hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-  hipGraphDestroy(graph);
-  if (ie != hipSuccess) return kdh_hip_fail(ctx, ie, "hipGraphInstantiate");
   const int iters = std::min(max_iter, kMaxIter);
-  hipError_t le = hipEventRecord(ctx->ev[0], ctx->stream);
-  for (int it = 0; it < iters && le == hipSuccess; ++it) {
-    le = hipGraphLaunch(exec, ctx->stream);
-    if (le == hipSuccess) le = hipEventRecord(ctx->ev[it + 1], ctx->stream);
+  auto launch_phase = [&](int phases) -> int {
+    for (int k = 0; k < 2; ++k) {
+      if (g[k].n_tiles == 0) continue;
+      int lrc = kdh_launch(ctx, ctx->d_pairs, ctx->d_tile_pair + g[k].tile_off, g[k].n_tiles, g[k].ct, nullptr,
+                           nullptr, ctx->d_partials + g[k].partial_off, n_pairs, phases);
+      if (lrc != KDH_OK) return lrc;
+    }
+    return KDH_OK;
+  };
+  hipError_t le = hipSuccess;
+  if (ctx->profile) {
+    // Profiling: plain launches with events around each kernel phase.
+    le = hipEventRecord(ctx->ev[0], ctx->stream);
+    for (int it = 0; it < iters && le == hipSuccess; ++it) {
+      int lrc = KDH_OK;
+      for (int r = 0; r < kdh_ctx::kProfileReps && lrc == KDH_OK; ++r) lrc = launch_phase(1);
+      if (lrc != KDH_OK) return lrc;
+      le = hipEventRecord(ctx->ev[2 * it + 1], ctx->stream);
+      if (le != hipSuccess) break;
+      lrc = launch_phase(2);
+      if (lrc != KDH_OK) return lrc;
+      le = hipEventRecord(ctx->ev[2 * it + 2], ctx->stream);
+    }
+  } else {
+    // Capture every iteration (tile + finish launch per non-empty group) in
+    // one graph and launch it once: the pair states live in device memory,
+    // so the same launches replay each Gauss-Newton step, and pairs that have
+    // finished skip their tiles.
+    const long long key[12] = {n_pairs, g[0].n_tiles, g[1].n_tiles, iters, ctx->lds_nodes, ctx->grid_override,
+                               ctx->xcd_remap, (long long)(intptr_t)ctx->d_pairs,
+                               (long long)(intptr_t)ctx->d_partials, (long long)(intptr_t)ctx->d_tile_pair, 2, 0};
+    hipGraphExec_t exec = kdh_graph_lookup(ctx->graph_batch, key);
+    if (!exec) {
+      hipGraph_t graph = nullptr;
+      KDH_HIP_TRY(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+      int lrc = KDH_OK;
+      for (int it = 0; it < iters && lrc == KDH_OK; ++it) lrc = launch_phase(3);
+      hipError_t ce = hipStreamEndCapture(ctx->stream, &graph);
+      if (lrc != KDH_OK) {
+        if (graph) hipGraphDestroy(graph);
+        return lrc;
+      }
+      if (ce != hipSuccess) return kdh_hip_fail(ctx, ce, "hipStreamEndCapture");
+      hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+      hipGraphDestroy(graph);
+      if (ie != hipSuccess) return kdh_hip_fail(ctx, ie, "hipGraphInstantiate");
+      kdh_graph_store(ctx->graph_batch, key, exec);
+    }
+    le = hipEventRecord(ctx->ev[0], ctx->stream);
+    if (le == hipSuccess) le = hipGraphLaunch(exec, ctx->stream);
+    if (le == hipSuccess) le = hipEventRecord(ctx->ev[1], ctx->stream);
+    if (le == hipSuccess) le = hipStreamSynchronize(ctx->stream);
   }
   if (le == hipSuccess)
     le = hipMemcpyAsync(ctx->h_pairs, ctx->d_pairs, sizeof(PairState) * n_pairs, hipMemcpyDeviceToHost, ctx->stream);
   if (le == hipSuccess) le = hipStreamSynchronize(ctx->stream);
-  hipGraphExecDestroy(exec);
   if (le != hipSuccess) return kdh_hip_fail(ctx, le, "batch replay");
-  double tot = 0.0;
-  for (int it = 0; it < iters; ++it) {
+  if (ctx->profile) {
+    kdh_collect_times(ctx, iters, true);
+  } else {
     float ms = 0.0f;
-    if (hipEventElapsedTime(&ms, ctx->ev[it], ctx->ev[it + 1]) == hipSuccess) tot += ms;
+    hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]);
+    ctx->last_kernel_us = iters ? 1000.0 * ms / iters : 0.0;
+    ctx->last_tile_us = ctx->last_finish_us = 0.0;
+    ctx->last_launches = iters;
   }
-  ctx->last_kernel_us = iters ? 1000.0 * tot / iters : 0.0;
-  ctx->last_launches = iters;
   for (int i = 0; i < n_pairs; ++i) {
     const PairState& ps = ctx->h_pairs[i];
     kdh_pair_result& r = results[i];

@@ -72,7 +72,7 @@ int kdh_ctx_create(int device, kdh_ctx** out) {
     return KDH_ERR_NOMEM;
   }
   c->h_pairs_cap = 1;
-  for (int i = 0; i < kMaxIter + 1; ++i) {
+  for (int i = 0; i < kdh_ctx::kEvents; ++i) {
     if (hipEventCreate(&c->ev[i]) != hipSuccess) {
       delete c;
       return KDH_ERR_HIP;
@@ -86,14 +86,15 @@ int kdh_ctx_destroy(kdh_ctx* ctx) {
   if (!ctx) return KDH_ERR_INVALID;
   hipSetDevice(ctx->device);
   hipStreamSynchronize(ctx->stream);
-  if (ctx->graph_exec) hipGraphExecDestroy(ctx->graph_exec);
+  if (ctx->graph_align.exec) hipGraphExecDestroy(ctx->graph_align.exec);
+  if (ctx->graph_batch.exec) hipGraphExecDestroy(ctx->graph_batch.exec);
   if (ctx->d_pairs) hipFree(ctx->d_pairs);
   if (ctx->d_partials) hipFree(ctx->d_partials);
   if (ctx->d_tile_pair) hipFree(ctx->d_tile_pair);
   if (ctx->d_corr) hipFree(ctx->d_corr);
   if (ctx->d_depth) hipFree(ctx->d_depth);
   if (ctx->h_pairs) hipHostFree(ctx->h_pairs);
-  for (int i = 0; i < kMaxIter + 1; ++i)
+  for (int i = 0; i < kdh_ctx::kEvents; ++i)
     if (ctx->ev[i]) hipEventDestroy(ctx->ev[i]);
   hipStreamDestroy(ctx->stream);
   delete ctx;
@@ -307,13 +308,15 @@ int kdh_fill_pair(kdh_ctx* ctx, PairState& ps, const kdh_tree* fixed, const kdh_
   ps.moving_root_mean = moving->root_mean;
   ps.status = 0;
   ps.iter = 0;
-  ps.tiles_done = 0;
   return KDH_OK;
 }
 
 int kdh_launch(kdh_ctx* ctx, PairState* d_pairs, const int* d_tile_pair, int n_tiles, int ct, int* corr,
-               int* depth, float* partials) {
+               int* depth, float* partials, int n_pairs, int phases) {
   LaunchArgs a;
+  a.pair_first = 0;
+  a.n_pairs = n_pairs;
+  a.phases = phases;
   a.pairs = d_pairs;
   a.tile_pair = d_tile_pair;
   a.n_tiles_total = n_tiles;
@@ -363,7 +366,7 @@ static int build_common(kdh_ctx* ctx, const kdh_tree* fixed, const kdh_tree* mov
   KDH_HIP_TRY(ctx, hipMemcpyAsync(ctx->d_pairs, &ps, offsetof(PairState, stats_log), hipMemcpyHostToDevice,
                                   ctx->stream));
   rc = kdh_launch(ctx, ctx->d_pairs, nullptr, nt, ct, leaf_out ? ctx->d_corr : nullptr,
-                  leaf_out ? ctx->d_depth : nullptr, nullptr);
+                  leaf_out ? ctx->d_depth : nullptr, nullptr, 1);
   if (rc) return rc;
   if (nt > 0) {
     KDH_HIP_TRY(ctx, hipMemcpyAsync(&ps.stats_log[0], &ctx->d_pairs->stats_log[0], sizeof(DevStats),
@@ -469,6 +472,43 @@ int kdh_align(kdh_ctx* ctx, const kdh_tree* fixed, const kdh_tree* moving, int c
     ps.iter = 1;
     ps.status = params->min_good_measurements > 0 ? 2 : 1;
   } else {
+    const bool prof = ctx->profile != 0;
+    if (!prof && !cache_log) {
+      // Production path: every iteration captured in one graph, one launch;
+      // the executable graph is cached while the launch shape is unchanged.
+      const long long key[12] = {nt, ct, iters, ctx->lds_nodes, ctx->grid_override, ctx->xcd_remap,
+                                 (long long)(intptr_t)ctx->d_pairs, (long long)(intptr_t)ctx->d_partials, 1, 0, 0, 0};
+      hipGraphExec_t exec = kdh_graph_lookup(ctx->graph_align, key);
+      if (!exec) {
+        hipGraph_t graph = nullptr;
+        KDH_HIP_TRY(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+        int lrc = KDH_OK;
+        for (int it = 0; it < iters && lrc == KDH_OK; ++it)
+          lrc = kdh_launch(ctx, ctx->d_pairs, nullptr, nt, ct, nullptr, nullptr, nullptr, 1, 3);
+        hipError_t ce = hipStreamEndCapture(ctx->stream, &graph);
+        if (lrc != KDH_OK) {
+          if (graph) hipGraphDestroy(graph);
+          return lrc;
+        }
+        if (ce != hipSuccess) return kdh_hip_fail(ctx, ce, "hipStreamEndCapture");
+        hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        hipGraphDestroy(graph);
+        if (ie != hipSuccess) return kdh_hip_fail(ctx, ie, "hipGraphInstantiate");
+        kdh_graph_store(ctx->graph_align, key, exec);
+      }
+      hipError_t le = hipEventRecord(ctx->ev[0], ctx->stream);
+      if (le == hipSuccess) le = hipGraphLaunch(exec, ctx->stream);
+      if (le == hipSuccess) le = hipEventRecord(ctx->ev[1], ctx->stream);
+      if (le == hipSuccess)
+        le = hipMemcpyAsync(&ps, ctx->d_pairs, sizeof(PairState), hipMemcpyDeviceToHost, ctx->stream);
+      if (le == hipSuccess) le = hipStreamSynchronize(ctx->stream);
+      if (le != hipSuccess) return kdh_hip_fail(ctx, le, "device loop graph");
+      float ms = 0.0f;
+      hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]);
+      ctx->last_kernel_us = 1000.0 * ms / iters;
+      ctx->last_tile_us = ctx->last_finish_us = 0.0;
+      ctx->last_launches = iters;
+    } else {
     KDH_HIP_TRY(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
     for (int it = 0; it < iters; ++it) {
       if (cache_log) {
@@ -479,19 +519,27 @@ int kdh_align(kdh_ctx* ctx, const kdh_tree* fixed, const kdh_tree* moving, int c
         iso_to_34(ps.cache.Xf, cache_log + it * 36 + 12);
         iso_to_34(ps.cache.Xmf, cache_log + it * 36 + 24);
       }
-      rc = kdh_launch(ctx, ctx->d_pairs, nullptr, nt, ct, nullptr, nullptr, nullptr);
-      if (rc) return rc;
-      KDH_HIP_TRY(ctx, hipEventRecord(ctx->ev[it + 1], ctx->stream));
+      if (prof) {
+        // The tile kernel is idempotent (it only rewrites the partial rows),
+        // so it runs kProfileReps times back to back between the events.
+        for (int r = 0; r < kdh_ctx::kProfileReps; ++r) {
+          rc = kdh_launch(ctx, ctx->d_pairs, nullptr, nt, ct, nullptr, nullptr, nullptr, 1, 1);
+          if (rc) return rc;
+        }
+        KDH_HIP_TRY(ctx, hipEventRecord(ctx->ev[2 * it + 1], ctx->stream));
+        rc = kdh_launch(ctx, ctx->d_pairs, nullptr, nt, ct, nullptr, nullptr, nullptr, 1, 2);
+        if (rc) return rc;
+        KDH_HIP_TRY(ctx, hipEventRecord(ctx->ev[2 * it + 2], ctx->stream));
+      } else {
+        rc = kdh_launch(ctx, ctx->d_pairs, nullptr, nt, ct, nullptr, nullptr, nullptr, 1, 3);
+        if (rc) return rc;
+        KDH_HIP_TRY(ctx, hipEventRecord(ctx->ev[2 * it + 2], ctx->stream));
+      }
     }
     KDH_HIP_TRY(ctx, hipMemcpyAsync(&ps, ctx->d_pairs, sizeof(PairState), hipMemcpyDeviceToHost, ctx->stream));
     KDH_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-    double tot = 0.0;
-    for (int it = 0; it < iters; ++it) {
-      float ms = 0.0f;
-      if (hipEventElapsedTime(&ms, ctx->ev[it], ctx->ev[it + 1]) == hipSuccess) tot += ms;
+    kdh_collect_times(ctx, iters, prof);
     }
-    ctx->last_kernel_us = iters ? 1000.0 * tot / iters : 0.0;
-    ctx->last_launches = iters;
   }
   if (X_out) memcpy(X_out, ps.X, sizeof(ps.X));
   if (v_out) memcpy(v_out, ps.v, sizeof(ps.v));
@@ -502,9 +550,57 @@ int kdh_align(kdh_ctx* ctx, const kdh_tree* fixed, const kdh_tree* moving, int c
   return KDH_OK;
 }
 
+// Event layout per iteration it: ev[2it] .. ev[2it+2] the whole iteration;
+// in profiling mode ev[2it] .. ev[2it+1] brackets kProfileReps back-to-back
+// tile kernels and ev[2it+1] .. ev[2it+2] the finish kernel.
+void kdh_collect_times(kdh_ctx* ctx, int iters, bool prof) {
+  double tot = 0.0, tile = 0.0, fin = 0.0;
+  for (int it = 0; it < iters; ++it) {
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, ctx->ev[2 * it], ctx->ev[2 * it + 2]) == hipSuccess) tot += ms;
+    if (prof) {
+      if (hipEventElapsedTime(&ms, ctx->ev[2 * it], ctx->ev[2 * it + 1]) == hipSuccess)
+        tile += ms / kdh_ctx::kProfileReps;
+      if (hipEventElapsedTime(&ms, ctx->ev[2 * it + 1], ctx->ev[2 * it + 2]) == hipSuccess) fin += ms;
+    }
+  }
+  ctx->last_kernel_us = iters ? 1000.0 * tot / iters : 0.0;
+  ctx->last_tile_us = (prof && iters) ? 1000.0 * tile / iters : 0.0;
+  ctx->last_finish_us = (prof && iters) ? 1000.0 * fin / iters : 0.0;
+  ctx->last_launches = iters;
+}
+
+hipGraphExec_t kdh_graph_lookup(kdh_ctx::GraphSlot& slot, const long long* key) {
+  if (slot.exec && memcmp(slot.key, key, sizeof(slot.key)) == 0) return slot.exec;
+  if (slot.exec) {
+    (void)hipGraphExecDestroy(slot.exec);
+    slot.exec = nullptr;
+  }
+  return nullptr;
+}
+
+void kdh_graph_store(kdh_ctx::GraphSlot& slot, const long long* key, hipGraphExec_t exec) {
+  memcpy(slot.key, key, sizeof(slot.key));
+  slot.exec = exec;
+}
+
 int kdh_last_kernel_time(kdh_ctx* ctx, double* avg_us, int* n_launches) {
   if (!ctx) return KDH_ERR_INVALID;
   if (avg_us) *avg_us = ctx->last_kernel_us;
   if (n_launches) *n_launches = ctx->last_launches;
+  return KDH_OK;
+}
+
+int kdh_last_kernel_breakdown(kdh_ctx* ctx, double* tile_us, double* finish_us) {
+  if (!ctx) return KDH_ERR_INVALID;
+  if (tile_us) *tile_us = ctx->last_tile_us;
+  if (finish_us) *finish_us = ctx->last_finish_us;
+  return KDH_OK;
+}
+
+int kdh_set_profiling(kdh_ctx* ctx, int on) {
+  if (!ctx) return KDH_ERR_INVALID;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  ctx->profile = on ? 1 : 0;
   return KDH_OK;
 }

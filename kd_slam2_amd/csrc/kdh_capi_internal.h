@@ -35,15 +35,25 @@ struct kdh_ctx {
   size_t corr_cap = 0;
   int* d_depth = nullptr;
   size_t depth_cap = 0;
-  hipGraphExec_t graph_exec = nullptr;
+  // Instantiated graphs, reused while the launch shape and buffers match.
+  struct GraphSlot {
+    hipGraphExec_t exec = nullptr;
+    long long key[12] = {};
+  };
+  GraphSlot graph_align, graph_batch;
   // launch configuration (autotuner)
   int grid_override = 0;
   int lds_nodes = 1024;
   int xcd_remap = 1;
   // timing of the last device loop
-  hipEvent_t ev[kdh::kMaxIter + 1] = {};
-  double last_kernel_us = 0.0;
+  static constexpr int kEvents = 2 * kdh::kMaxIter + 2;
+  static constexpr int kProfileReps = 4;
+  hipEvent_t ev[kEvents] = {};
+  double last_kernel_us = 0.0;   // tile + finish per iteration
+  double last_tile_us = 0.0;     // tile kernel alone (profiling mode)
+  double last_finish_us = 0.0;   // finish kernel alone (profiling mode)
   int last_launches = 0;
+  int profile = 0;               // 1: no graph, events around each kernel
 };
 
 int kdh_fail(kdh_ctx* ctx, int code, const std::string& msg);
@@ -56,5 +66,10 @@ int kdh_fill_pair(kdh_ctx* ctx, kdh::PairState& ps, const kdh_tree* fixed, const
 void kdh_cache_from_state(kdh::PairState& ps);
 int kdh_setup_prior(kdh::PairState& ps, const kdh_prior* prior);
 int kdh_launch(kdh_ctx* ctx, kdh::PairState* d_pairs, const int* d_tile_pair, int n_tiles, int ct, int* corr,
-               int* depth, float* partials);
+               int* depth, float* partials, int n_pairs, int phases = 3);
 int kdh_result_code(const kdh::PairState& ps, int max_iterations_param);
+void kdh_collect_times(kdh_ctx* ctx, int iters, bool prof);
+// Returns the cached executable graph for `key`, or nullptr (and drops the
+// stale one) when the key differs.
+hipGraphExec_t kdh_graph_lookup(kdh_ctx::GraphSlot& slot, const long long* key);
+void kdh_graph_store(kdh_ctx::GraphSlot& slot, const long long* key, hipGraphExec_t exec);

@@ -72,9 +72,8 @@ struct PairState {
   double v[6];          // moving velocities (CT)
   TermCache cache;      // FP32 inputs of the current iteration
   int iter;             // completed iterations
-  int status;           // 0 running, 1 finished (max_iterations or criterion), -k failed at k
+  int status;           // 0 running, 1 finished (max_iterations), 2 failed min_good gate, 3 stopped by criterion
   float prev_score;
-  unsigned tiles_done;  // last-block detection counter (reset by the last block)
   // outputs
   DevStats stats_log[kMaxIter];
   double H[144];        // last H (n x n row-major, n = 6 or 12)

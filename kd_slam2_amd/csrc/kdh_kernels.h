@@ -18,6 +18,9 @@ struct LaunchArgs {
   int ct;
   int lds_nodes;          // top-of-tree nodes staged in LDS: 0, 256, 512, 1024, 2048 (ICP)
   int xcd_remap;          // 1: give each XCD a contiguous range of tiles (needs grid % 8 == 0)
+  int pair_first;         // finish stage: alignments [pair_first, pair_first + n_pairs)
+  int n_pairs;            // (0: no finish launch)
+  int phases;             // bit 0: tile kernel, bit 1: finish stage
 };
 
 hipError_t launch_tiles(const LaunchArgs& a, hipStream_t s);

@@ -8,16 +8,20 @@
 // one alignment ("pair").  Per tile the workgroup
 //   1. descends the fixed tree once per leaf (one lane per leaf) with the top
 //      of the tree staged in LDS,
-//   2. evaluates the point-to-plane term (kdh_term.h),
-//   3. reduces the 34 (ICP) or 97 (CT) accumulator slots across the 64 lanes
-//      of each wave with a butterfly reduce-scatter (63 or 127 shuffles per
-//      wave instead of slots x 6), then across the 4 waves through LDS,
+//   2. evaluates the point-to-plane term (kdh_term.h) into a compact 16-value
+//      record per lane,
+//   3. expands the 34 (ICP) or 97 (CT) accumulator slots from the record in
+//      chunks of 64 and reduces each chunk across the 64 lanes of the wave
+//      with a butterfly reduce-scatter (63 shuffles per chunk instead of
+//      64 x 6), then across the 4 waves through LDS,
 //   4. writes one FP32 partial row per tile to a fixed slot.
-// The last workgroup to finish a pair's tiles (atomic counter) sums that
-// pair's partial rows in FP64 in tile order -- the result does not depend on
-// which workgroup ran which tile, nor on what else shares the launch -- and
-// then either publishes H/b (build mode) or runs the FP64 damped LDLT solve,
-// the SE(3) update and the next iteration's FP32 cache (solve mode).
+// A second kernel, `finish_kernel`, runs one workgroup per alignment: it sums
+// the alignment's partial rows in FP64 in tile order -- the result does not
+// depend on which workgroup ran which tile, nor on what else shares the
+// launch -- and then either publishes H/b (build mode) or runs the FP64
+// damped LDLT solve, the SE(3) update and the next iteration's FP32 cache
+// (solve mode).  Keeping the FP64 solve out of the tile kernel keeps the
+// tile kernel's register budget small enough for several waves per SIMD.
 //
 // Reference counterparts: ICP_kernel / CTICP_kernel + Reduce_ + host
 // oneRound (kd_slam2/src/lib/kd_slam/icp/icp_cuda_impl_.cuh:69-132,
@@ -40,21 +44,29 @@ template <bool CT> struct Slots;
 template <> struct Slots<false> { static constexpr int N = kIcpSlots; static constexpr int P = 64; };
 template <> struct Slots<true> { static constexpr int N = kCtSlots; static constexpr int P = 128; };
 
-// Butterfly reduce-scatter over one wave: on entry every lane holds P values
-// (P a multiple of 64); on exit lane l holds the wave sums of slots
-// j + (P/64)*l, j < P/64, in v[0..P/64).  Fixed pairing => deterministic.
-template <int P>
-__device__ __forceinline__ void wave_reduce_scatter(float* v, int lane) {
+// Butterfly reduce-scatter over one wave: on entry every lane holds 64
+// values; after the step with lane offset OFF the lane keeps OFF values (the
+// upper or lower half, by its OFF bit) and adds its partner's other half.
+// On exit lane l holds the wave sum of slot l in v[0].  The pairing is fixed,
+// so the summation order -- and the result -- is deterministic.
+template <int OFF>
+__device__ __forceinline__ void reduce_scatter_step(float* v, int lane) {
+  const bool upper = (lane & OFF) != 0;
 #pragma unroll
-  for (int off = 32, h = P / 2; off >= 1; off >>= 1, h >>= 1) {
-    const bool upper = (lane & off) != 0;
-#pragma unroll
-    for (int i = 0; i < h; ++i) {
-      float send = upper ? v[i] : v[i + h];
-      float keep = upper ? v[i + h] : v[i];
-      v[i] = keep + __shfl_xor(send, off, 64);
-    }
+  for (int i = 0; i < OFF; ++i) {
+    const float send = upper ? v[i] : v[i + OFF];
+    const float keep = upper ? v[i + OFF] : v[i];
+    v[i] = keep + __shfl_xor(send, OFF, 64);
   }
+}
+
+__device__ __forceinline__ void wave_reduce_scatter64(float* v, int lane) {
+  reduce_scatter_step<32>(v, lane);
+  reduce_scatter_step<16>(v, lane);
+  reduce_scatter_step<8>(v, lane);
+  reduce_scatter_step<4>(v, lane);
+  reduce_scatter_step<2>(v, lane);
+  reduce_scatter_step<1>(v, lane);
 }
 
 struct GlobalFetch {
@@ -80,68 +92,213 @@ struct LdsFetch {
   }
 };
 
-// ---- per-pair final stage (runs on the last workgroup of a pair) ----------
+// ---- per-pair final stage (finish_kernel) ----------------------------------
+
+// Broadcast of a double from lane `src` (wave-uniform) through v_readlane:
+// a scalar read of one lane's register, much shorter latency than an LDS
+// permute, which matters on the serial chain of the factorisation.
+__device__ __forceinline__ double bcast(double x, int src) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(x), src);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(x), src);
+  return __hiloint2double(hi, lo);
+}
+
+// Damped solve (A + diag(damp)) x = -b on one wave: lane i (< D) holds row i
+// of the symmetric matrix in registers; right-looking LDL^T without pivoting
+// (the damped normal matrix is positive definite), then forward and backward
+// substitution, all with lane broadcasts.  Lanes >= D carry zeros.
+template <int D>
+__device__ void wave_ldlt_solve(const double* sH, const double* sb, double damp_pose, double damp_vel, int lane,
+                                double* sx) {
+  double a[D];
+  const bool row = lane < D;
+#pragma unroll
+  for (int j = 0; j < D; ++j) a[j] = row ? sH[lane * D + j] : 0.0;
+  if (row) {
+#pragma unroll
+    for (int j = 0; j < D; ++j)
+      if (j == lane) a[j] += (j < 6) ? damp_pose : damp_vel;
+  }
+  double r = row ? -sb[lane] : 0.0;
+  double dd = 0.0;  // this lane's pivot D_i
+#pragma unroll
+  for (int k = 0; k < D; ++k) {
+    double akk = 0.0;
+#pragma unroll
+    for (int j = 0; j < D; ++j)
+      if (j == k) akk = a[j];
+    const double dk = bcast(akk, k);
+    if (lane == k) dd = dk;
+    const double inv = (dk != 0.0) ? 1.0 / dk : 0.0;
+    const double lik = (lane > k && row) ? a[k] * inv : 0.0;
+    // A_ij -= L_ik * A_kj for j > k (row k broadcast from lane k)
+#pragma unroll
+    for (int j = k + 1; j < D; ++j) {
+      const double akj = bcast(a[j], k);
+      a[j] -= lik * akj;
+    }
+    if (lane > k && row) a[k] = lik;  // store L_ik in the strictly lower part
+  }
+  // forward: L y = r
+#pragma unroll
+  for (int k = 0; k < D; ++k) {
+    const double yk = bcast(r, k);
+    if (lane > k && row) r -= a[k] * yk;
+  }
+  r = (row && dd != 0.0) ? r / dd : 0.0;
+  // backward: L^T x = z; lane i needs L_ki = a[i] of lane k
+#pragma unroll
+  for (int k = D - 1; k >= 0; --k) {
+    const double xk = bcast(r, k);
+#pragma unroll
+    for (int i = 0; i < k; ++i) {
+      const double lki = bcast(a[i], k);
+      if (lane == i) r -= lki * xk;
+    }
+  }
+  if (row) sx[lane] = r;
+}
+
+// LDS needed by finish_pair (bytes); the caller passes an area at least this
+// large, 16-byte aligned.
+template <bool CT>
+struct FinishLds {
+  static constexpr int N = Slots<CT>::N;
+  static constexpr int D = CT ? 12 : 6;
+  static constexpr int CH = 32;  // tiles staged per round
+  static constexpr int kRed = 0;                          // double[N]
+  static constexpr int kH = kRed + 8 * ((N + 1) & ~1);    // double[D*D]
+  static constexpr int kG = kH + 8 * D * D;               // double[D*D]
+  static constexpr int kB = kG + 8 * D * D;               // double[D]
+  static constexpr int kX = kB + 8 * D;                   // double[D]
+  static constexpr int kGo = kX + 8 * D;                  // int (padded)
+  static constexpr int kStage = kGo + 16;                 // float[CH*N]
+  static constexpr int kBytes = kStage + 4 * CH * N;
+};
 
 template <bool CT>
-__device__ void finish_pair(PairState& ps, const float* __restrict__ partials, double* red) {
+__device__ void finish_pair(PairState& ps, const float* __restrict__ partials, char* lds) {
+  using FL = FinishLds<CT>;
   constexpr int N = Slots<CT>::N;
-  const int tid = threadIdx.x;
-  // FP64 sum of the pair's partial rows, in tile order.
-  for (int s = tid; s < N; s += kThreads) {
-    double acc = 0.0;
-    const float* p = partials + (size_t)ps.tile_first * N + s;
-    for (int t = 0; t < ps.n_tiles; ++t) acc += (double)p[(size_t)t * N];
-    red[s] = acc;
-  }
-  __syncthreads();
-  if (tid != 0) return;
-
   constexpr int D = CT ? 12 : 6;
   constexpr int CS = CT ? 90 : 27;  // chi2 slot
+  constexpr int CH = FL::CH;
+  float* s_stage = (float*)(lds + FL::kStage);
+  double* s_red = (double*)(lds + FL::kRed);
+  double* sH = (double*)(lds + FL::kH);
+  double* sG = (double*)(lds + FL::kG);
+  double* sb = (double*)(lds + FL::kB);
+  double* sx = (double*)(lds + FL::kX);
+  int& s_go = *(int*)(lds + FL::kGo);
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  // Read the whole pair state the stage needs up front (uniform addresses:
+  // scalar loads issued together), so its latency overlaps the partial-row
+  // loads instead of adding round trips at the end.
+  const int mode = ps.mode;
+  const int it = ps.iter;
+  const int stats_mode = ps.tp.stats_mode;
+  const SolveParams sp = ps.sp;
+  const bool has_prior = ps.has_prior != 0.0;
+  const V3f root_mean = ps.moving_root_mean;
+  const float prev_score = ps.prev_score;
+  double Xs[12], Xfs[12], vs[6];
+#pragma unroll
+  for (int i = 0; i < 12; ++i) {
+    Xs[i] = ps.X[i];
+    Xfs[i] = ps.Xf[i];
+  }
+#pragma unroll
+  for (int i = 0; i < 6; ++i) vs[i] = ps.v[i];
+
+  // 1. FP64 sum of the pair's partial rows in tile order.  Rows are staged
+  //    through LDS CH at a time with coalesced loads by all threads; thread s
+  //    (< N) adds them in order, so the result is deterministic.
+  const float* base = partials + (size_t)ps.tile_first * N;
+  double acc = 0.0;
+  for (int t0 = 0; t0 < ps.n_tiles; t0 += CH) {
+    const int nt = min(CH, ps.n_tiles - t0);
+    __syncthreads();
+    for (int i = tid; i < nt * N; i += kThreads) s_stage[i] = base[(size_t)t0 * N + i];
+    __syncthreads();
+    if (tid < N)
+      for (int t = 0; t < nt; ++t) acc += (double)s_stage[t * N + tid];
+  }
+  if (tid < N) s_red[tid] = acc;
+  __syncthreads();
+
+  // 2. Assemble the re-centred H_c, b_c, then map back: G = H_c M, H = M^T G,
+  //    b = M^T b_c with M = [[I, -[c]x], [0, I]] (see uncenter_d).
+  const V3d cc = v3((double)ps.cache.c.x, (double)ps.cache.c.y, (double)ps.cache.c.z);
+  const M3d Cx = skew(cc);
+  for (int e = tid; e < D * D; e += kThreads) {
+    const int r = e / D, c = e % D;
+    double h;
+    if (r < 6 && c < 6) {
+      const int rr = r < c ? r : c, c2 = r < c ? c : r;
+      const int k = rr * 6 - (rr * (rr - 1)) / 2 + (c2 - rr);
+      h = s_red[k];
+    } else if (r >= 6 && c >= 6) {
+      const int rr = (r < c ? r : c) - 6, c2 = (r < c ? c : r) - 6;
+      const int k = rr * 6 - (rr * (rr - 1)) / 2 + (c2 - rr);
+      h = s_red[27 + k];
+    } else if (r < 6) {
+      h = s_red[54 + r * 6 + (c - 6)];
+    } else {
+      h = s_red[54 + c * 6 + (r - 6)];
+    }
+    sH[e] = h;
+  }
+  if (tid < D) sb[tid] = s_red[tid < 6 ? 21 + tid : 48 + (tid - 6)];
+  __syncthreads();
+  for (int e = tid; e < D * D; e += kThreads) {
+    const int r = e / D, c = e % D;
+    double g = sH[e];
+    if (c >= 3 && c < 6)
+      g -= (sH[r * D + 0] * Cx(0, c - 3) + sH[r * D + 1] * Cx(1, c - 3)) + sH[r * D + 2] * Cx(2, c - 3);
+    sG[e] = g;
+  }
+  __syncthreads();
+  for (int e = tid; e < D * D; e += kThreads) {
+    const int r = e / D, c = e % D;
+    double h = sG[e];
+    if (r >= 3 && r < 6)
+      h -= (Cx(0, r - 3) * sG[0 * D + c] + Cx(1, r - 3) * sG[1 * D + c]) + Cx(2, r - 3) * sG[2 * D + c];
+    sH[e] = h;
+  }
+  double bj = 0.0;
+  if (tid >= 3 && tid < 6)
+    bj = sb[tid] - ((Cx(0, tid - 3) * sb[0] + Cx(1, tid - 3) * sb[1]) + Cx(2, tid - 3) * sb[2]);
+  __syncthreads();
+  if (tid >= 3 && tid < 6) sb[tid] = bj;
+  __syncthreads();
+
   DevStats st;
-  st.num_inliers = (int)red[CS + 2 + kInlier];
-  st.num_outliers = (int)red[CS + 2 + kOutlier];
-  st.num_bad = (int)red[CS + 2 + kBad];
-  st.num_err_normal = (int)red[CS + 2 + kErrNormal];
-  st.num_err_fn = (int)red[CS + 2 + kErrFn];
+  st.num_inliers = (int)s_red[CS + 2 + kInlier];
+  st.num_outliers = (int)s_red[CS + 2 + kOutlier];
+  st.num_bad = (int)s_red[CS + 2 + kBad];
+  st.num_err_normal = (int)s_red[CS + 2 + kErrNormal];
+  st.num_err_fn = (int)s_red[CS + 2 + kErrFn];
   st.pad = 0;
-  st.chi2 = red[CS];
-  st.chi2_kernelized = red[CS + 1];
+  st.chi2 = s_red[CS];
+  st.chi2_kernelized = s_red[CS + 1];
   st.pert_pose_norm = 0.0;
   st.pert_vel_norm = 0.0;
 
-  double H[D * D];
-  double b[D];
-  for (int i = 0; i < D * D; ++i) H[i] = 0.0;
-  unpack_diag_d(red, H, D, 0, b);
-  if (CT) {
-    unpack_diag_d(red + 27, H, D, 6, b);
-    for (int r = 0; r < 6; ++r)
-      for (int c = 0; c < 6; ++c) {
-        H[r * D + 6 + c] = red[54 + r * 6 + c];
-        H[(6 + c) * D + r] = red[54 + r * 6 + c];
-      }
-  }
-  const V3d cc = v3((double)ps.cache.c.x, (double)ps.cache.c.y, (double)ps.cache.c.z);
-  uncenter_d<D>(H, b, cc);
-
-  const int it = ps.iter;
-  if (ps.mode == 1) {
-    for (int i = 0; i < D * D; ++i) ps.H[i] = H[i];
-    for (int i = 0; i < D; ++i) ps.b[i] = b[i];
-    ps.stats_log[0] = st;
-    ps.tiles_done = 0;
+  if (mode == 1) {  // build only: publish H, b, stats
+    for (int e = tid; e < D * D; e += kThreads) ps.H[e] = sH[e];
+    if (tid < D) ps.b[tid] = sb[tid];
+    if (tid == 0) ps.stats_log[0] = st;
     return;
   }
 
-  IsoD X = iso_from_34<double>(ps.X);
-  // pose prior (tree_aligner_base_.h:139-152)
-  if (ps.has_prior != 0.0) {
+  // 3. pose prior (tree_aligner_base_.h:139-152), serial on thread 0
+  if (has_prior && tid == 0) {
+    IsoD X = iso_from_34<double>(Xs);
     IsoD ZXi = iso_from_34<double>(ps.prior_ZXi);
-    double e[6], J[36];
+    double e[6], J[36], OJ[36], Oe[6];
     logmap_d(compose(ZXi, X), e);
     dPoseToPoseError_d(ZXi, X, J);
-    double OJ[36], Oe[6];
     for (int r = 0; r < 6; ++r) {
       double se = 0.0;
       for (int k = 0; k < 6; ++k) se += ps.prior_info[r * 6 + k] * e[k];
@@ -153,34 +310,40 @@ __device__ void finish_pair(PairState& ps, const float* __restrict__ partials, d
       }
     }
     for (int r = 0; r < 6; ++r) {
-      double sb = 0.0;
-      for (int k = 0; k < 6; ++k) sb += J[k * 6 + r] * Oe[k];
-      b[r] += sb;
+      double sbv = 0.0;
+      for (int k = 0; k < 6; ++k) sbv += J[k * 6 + r] * Oe[k];
+      sb[r] += sbv;
       for (int c = 0; c < 6; ++c) {
         double s = 0.0;
         for (int k = 0; k < 6; ++k) s += J[k * 6 + r] * OJ[k * 6 + c];
-        H[r * D + c] += s;
+        sH[r * D + c] += s;
       }
     }
   }
-  for (int i = 0; i < D * D; ++i) ps.H[i] = H[i];
-  for (int i = 0; i < D; ++i) ps.b[i] = b[i];
+  __syncthreads();
+  for (int e = tid; e < D * D; e += kThreads) ps.H[e] = sH[e];
+  if (tid < D) ps.b[tid] = sb[tid];
 
-  const int good = st.num_inliers + st.num_outliers;
-  if (good < ps.sp.min_good_measurements) {
-    ps.stats_log[it] = st;
-    ps.iter = it + 1;
-    ps.status = 2;  // failed: compute() returns -it
-    ps.tiles_done = 0;
+  // 4. min_good gate (icp_impl_.h:76-80)
+  if (tid == 0) s_go = (st.num_inliers + st.num_outliers >= sp.min_good_measurements) ? 1 : 0;
+  __syncthreads();
+  if (!s_go) {
+    if (tid == 0) {
+      ps.stats_log[it] = st;
+      ps.iter = it + 1;
+      ps.status = 2;  // failed: compute() returns -it
+    }
     return;
   }
-  double A[D * D], rhs[D], dx[D];
-  for (int i = 0; i < D * D; ++i) A[i] = H[i];
-  for (int i = 0; i < D; ++i) {
-    A[i * D + i] += (i < 6) ? ps.sp.damping : ps.sp.velocity_damping;
-    rhs[i] = -b[i];
-  }
-  ldlt_solve_d<D>(A, rhs, dx);
+
+  // 5. damped solve on wave 0
+  if (tid < 64) wave_ldlt_solve<D>(sH, sb, sp.damping, sp.velocity_damping, lane, sx);
+  __syncthreads();
+  if (tid != 0) return;
+
+  // 6. update, stats, termination, next cache (thread 0)
+  double dx[D];
+  for (int i = 0; i < D; ++i) dx[i] = sx[i];
   double pn = 0.0, vn = 0.0;
   for (int i = 0; i < 6; ++i) pn += dx[i] * dx[i];
   if (CT)
@@ -189,46 +352,52 @@ __device__ void finish_pair(PairState& ps, const float* __restrict__ partials, d
   st.pert_vel_norm = sqrt(vn);
   ps.stats_log[it] = st;
   ps.iter = it + 1;
-  if (!ps.tp.stats_mode) {
-    IsoD Xn = compose(expmap_d(dx), X);
-    iso_to_34(Xn, ps.X);
+  IsoD Xc = iso_from_34<double>(Xs);
+  if (!stats_mode) {
+    Xc = compose(expmap_d(dx), Xc);
+    iso_to_34(Xc, ps.X);
     if (CT)
-      for (int i = 0; i < 6; ++i) ps.v[i] += dx[6 + i];
+      for (int i = 0; i < 6; ++i) {
+        vs[i] += dx[6 + i];
+        ps.v[i] = vs[i];
+      }
   }
   // termination: stats mode runs one round; the score criterion
   // (icp_stats_.h:85-102) may stop early; otherwise max_iterations.
-  bool stop = ps.tp.stats_mode != 0 || ps.iter >= ps.sp.max_iterations;
-  if (!stop && ps.sp.use_term_crit) {
+  bool stop = stats_mode != 0 || it + 1 >= sp.max_iterations;
+  int status = 0;
+  if (!stop && sp.use_term_crit) {
     const float ni = (float)st.num_inliers;
     const int nl = st.num_inliers + st.num_outliers + st.num_bad + st.num_err_normal + st.num_err_fn;
     const float ratio = nl ? ni / (float)nl : 0.0f;
     const float score = ratio / ((float)st.chi2_kernelized / ni + 1e-6f);
     if (it > 0) {
-      const bool pert_ok = (float)st.pert_pose_norm < ps.sp.d_pert;
-      const float ds = score - ps.prev_score;
-      const bool score_ok = (ds > 0.0f) && ds / (ps.prev_score + 1e-6f) < ps.sp.d_score;
+      const bool pert_ok = (float)st.pert_pose_norm < sp.d_pert;
+      const float ds = score - prev_score;
+      const bool score_ok = (ds > 0.0f) && ds / (prev_score + 1e-6f) < sp.d_score;
       if (pert_ok && score_ok) stop = true;
     }
     ps.prev_score = score;
-    if (stop) ps.status = 3;
+    if (stop) status = 3;
   }
-  if (stop && ps.status == 0) ps.status = 1;
+  if (stop && status == 0) status = 1;
+  ps.status = status;
   // next iteration's FP32 cache (updateCache, icp_impl_.h:16-21), formed in
   // FP64 and rounded once.
-  IsoD Xc = iso_from_34<double>(ps.X);
-  IsoD Xf = iso_from_34<double>(ps.Xf);
-  ps.cache.Xm = iso_cast<float>(Xc);
-  ps.cache.Xf = iso_cast<float>(Xf);
-  ps.cache.Xmf = iso_cast<float>(compose(inverse(Xf), Xc));
-  for (int i = 0; i < 6; ++i) ps.cache.v[i] = (float)ps.v[i];
-  ps.cache.c = ps.sp.recenter ? apply_iso(ps.cache.Xm, ps.moving_root_mean) : v3(0.0f, 0.0f, 0.0f);
-  ps.tiles_done = 0;
+  const IsoD Xf = iso_from_34<double>(Xfs);
+  TermCache nc;
+  nc.Xm = iso_cast<float>(Xc);
+  nc.Xf = iso_cast<float>(Xf);
+  nc.Xmf = iso_cast<float>(compose(inverse(Xf), Xc));
+  for (int i = 0; i < 6; ++i) nc.v[i] = (float)vs[i];
+  nc.c = sp.recenter ? apply_iso(nc.Xm, root_mean) : v3(0.0f, 0.0f, 0.0f);
+  ps.cache = nc;
 }
 
 // ---- the tile kernel --------------------------------------------------------
 
 template <bool CT, int LDS_NODES>
-__global__ __launch_bounds__(kThreads) void tiles_kernel(PairState* __restrict__ pairs,
+__global__ __launch_bounds__(kThreads) void tiles_kernel(const PairState* __restrict__ pairs,
                                                          const int* __restrict__ tile_pair,
                                                          int n_tiles_total,
                                                          float* __restrict__ partials,
@@ -236,12 +405,9 @@ __global__ __launch_bounds__(kThreads) void tiles_kernel(PairState* __restrict__
                                                          int* __restrict__ depth_out,
                                                          int xcd_remap) {
   constexpr int N = Slots<CT>::N;
-  constexpr int P = Slots<CT>::P;
-  constexpr int PL = P / 64;  // slots held per lane after the wave reduce
+  constexpr int NCH = (N + 63) / 64;  // 64-slot chunks
   __shared__ float4 s_nodes[LDS_NODES > 0 ? 2 * LDS_NODES : 2];
-  __shared__ float s_wave[kWaves][P];
-  __shared__ double s_red[N];
-  __shared__ int s_last;
+  __shared__ float s_wave[kWaves][NCH * 64];
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
@@ -270,7 +436,7 @@ __global__ __launch_bounds__(kThreads) void tiles_kernel(PairState* __restrict__
       t = k;
     }
     const int p = tile_pair ? tile_pair[t] : 0;
-    PairState& ps = pairs[p];
+    const PairState& ps = pairs[p];
     if (ps.status != 0) continue;
     const DevTree fixed = ps.fixed;
     if (LDS_NODES > 0 && p != cached_pair) {
@@ -283,10 +449,8 @@ __global__ __launch_bounds__(kThreads) void tiles_kernel(PairState* __restrict__
     const int lt = t - ps.tile_first;
     const int q = lt * kTile + tid;
 
-    float v[P];
-#pragma unroll
-    for (int i = N; i < P; ++i) v[i] = 0.0f;
-    int leaf = -1, depth = 0;
+    TermRec R;
+    rec_reject(R, -1);  // padding lanes contribute nothing
     if (q < ps.moving.n_leaves && fixed.n_nodes > 0) {
       const float4 la = ps.moving.leaves[2 * q];
       const float4 lb = ps.moving.leaves[2 * q + 1];
@@ -296,50 +460,65 @@ __global__ __launch_bounds__(kThreads) void tiles_kernel(PairState* __restrict__
       L.dts = lb.z;
       const TermCache C = ps.cache;
       const TermParams TP = ps.tp;
+      int leaf, depth = 0;
       if (LDS_NODES > 0) {
         LdsFetch f{fixed.nodes, s_nodes, n_lds};
-        leaf = CT ? ct_term(f, C, TP, L, v, &depth) : icp_term(f, C, TP, L, v, &depth);
+        leaf = CT ? ct_term(f, C, TP, L, R, &depth) : icp_term(f, C, TP, L, R, &depth);
       } else {
         GlobalFetch f{fixed.nodes};
-        leaf = CT ? ct_term(f, C, TP, L, v, &depth) : icp_term(f, C, TP, L, v, &depth);
+        leaf = CT ? ct_term(f, C, TP, L, R, &depth) : icp_term(f, C, TP, L, R, &depth);
       }
       if (corr_out) corr_out[q] = leaf;
       if (depth_out) depth_out[q] = depth;
-    } else {
-#pragma unroll
-      for (int i = 0; i < N; ++i) v[i] = 0.0f;
     }
 
-    wave_reduce_scatter<P>(v, lane);
+    // Expand slots chunk by chunk and reduce each over the wave.
 #pragma unroll
-    for (int j = 0; j < PL; ++j) s_wave[wave][j + PL * lane] = v[j];
+    for (int ch = 0; ch < NCH; ++ch) {
+      float v[64];
+#pragma unroll
+      for (int i = 0; i < 64; ++i) v[i] = rec_slot<CT>(R, ch * 64 + i);
+      wave_reduce_scatter64(v, lane);
+      s_wave[wave][ch * 64 + lane] = v[0];
+    }
     __syncthreads();
     if (tid < N) {
       const float s = (s_wave[0][tid] + s_wave[1][tid]) + (s_wave[2][tid] + s_wave[3][tid]);
       partials[(size_t)t * N + tid] = s;
     }
-    __threadfence();
     __syncthreads();
-    if (tid == 0) {
-      const unsigned prev = atomicAdd(&ps.tiles_done, 1u);
-      s_last = (prev == (unsigned)(ps.n_tiles - 1)) ? 1 : 0;
-    }
-    __syncthreads();
-    if (s_last) {
-      __threadfence();
-      finish_pair<CT>(ps, partials, s_red);
-      __syncthreads();
-    }
   }
+}
+
+// One workgroup per alignment: FP64 reduction of its partial rows, then
+// build-mode publish or solve-mode Gauss-Newton step.
+template <bool CT>
+__global__ __launch_bounds__(kThreads) void finish_kernel(PairState* __restrict__ pairs, int pair_first,
+                                                          int n_pairs, const float* __restrict__ partials) {
+  __shared__ __align__(16) char s_area[FinishLds<CT>::kBytes];
+  const int p = pair_first + blockIdx.x;
+  if (blockIdx.x >= n_pairs) return;
+  PairState& ps = pairs[p];
+  if (ps.status != 0 || ps.ct != (CT ? 1 : 0) || ps.n_tiles == 0) return;
+  finish_pair<CT>(ps, partials, s_area);
 }
 
 // ---- launch wrappers --------------------------------------------------------
 
 template <bool CT, int LDS>
 static hipError_t launch_t(const LaunchArgs& a, hipStream_t s) {
-  hipLaunchKernelGGL((tiles_kernel<CT, LDS>), dim3(a.grid), dim3(kThreads), 0, s, a.pairs, a.tile_pair,
-                     a.n_tiles_total, a.partials, a.corr_out, a.depth_out, a.xcd_remap);
-  return hipGetLastError();
+  if (a.phases & 1) {
+    hipLaunchKernelGGL((tiles_kernel<CT, LDS>), dim3(a.grid), dim3(kThreads), 0, s, a.pairs, a.tile_pair,
+                       a.n_tiles_total, a.partials, a.corr_out, a.depth_out, a.xcd_remap);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  if ((a.phases & 2) && a.n_pairs > 0) {
+    hipLaunchKernelGGL((finish_kernel<CT>), dim3(a.n_pairs), dim3(kThreads), 0, s, a.pairs, a.pair_first,
+                       a.n_pairs, a.partials);
+    return hipGetLastError();
+  }
+  return hipSuccess;
 }
 
 hipError_t launch_tiles(const LaunchArgs& a, hipStream_t s) {

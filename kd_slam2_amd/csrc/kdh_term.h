@@ -156,18 +156,67 @@ struct LeafIn {
   float dts;
 };
 
-// Rigid ICP term.  Writes kIcpSlots accumulator values into acc (zeros for a
-// rejected leaf, with the one-hot outcome counter set) and returns the fixed
-// leaf index reached by the descent (-1 for num_bad).
+// Compact result of one leaf: everything the accumulator slots are formed
+// from.  A rejected leaf has gamma = 0, e = 0, J = 0 and its outcome set, so
+// every product slot expands to 0 and only its one-hot counter is 1.
+struct TermRec {
+  float J[6];   // pose Jacobian (re-centred rotational part)
+  float Jv[6];  // velocity Jacobian (CT)
+  float e;
+  float gamma;
+  float chi2;
+  int outcome;  // kInlier .. kErrFn
+};
+
+KDH_HD void rec_reject(TermRec& r, int outcome) {
+  for (int i = 0; i < 6; ++i) {
+    r.J[i] = 0.0f;
+    r.Jv[i] = 0.0f;
+  }
+  r.e = 0.0f;
+  r.gamma = 0.0f;
+  r.chi2 = 0.0f;
+  r.outcome = outcome;
+}
+
+// Accumulator slot s of a leaf record, with the packing of
+// ICPWsItemDiag_::pack (upper-triangular J_r J_c g, then J_r e g;
+// icp_ws_item_.h:26-34) and ICPWsItemCross_::pack (J_pose_r J_vel_c g,
+// icp_ws_item_.h:72-75).  s must be a compile-time constant after unrolling.
+KDH_HD int tri_r(int k) {  // row of the k-th upper-triangular entry of a 6x6
+  return k < 6 ? 0 : k < 11 ? 1 : k < 15 ? 2 : k < 18 ? 3 : k < 20 ? 4 : 5;
+}
+KDH_HD int tri_c(int k) {
+  const int r = tri_r(k);
+  const int start = r == 0 ? 0 : r == 1 ? 6 : r == 2 ? 11 : r == 3 ? 15 : r == 4 ? 18 : 20;
+  return r + (k - start);
+}
+
+template <bool CT>
+KDH_HD float rec_slot(const TermRec& r, int s) {
+  constexpr int CS = CT ? 90 : 27;
+  if (s < 21) return (r.J[tri_r(s)] * r.J[tri_c(s)]) * r.gamma;
+  if (s < 27) return (r.J[s - 21] * r.e) * r.gamma;
+  if (CT) {
+    if (s < 48) return (r.Jv[tri_r(s - 27)] * r.Jv[tri_c(s - 27)]) * r.gamma;
+    if (s < 54) return (r.Jv[s - 48] * r.e) * r.gamma;
+    if (s < 90) return (r.J[(s - 54) / 6] * r.Jv[(s - 54) % 6]) * r.gamma;
+  }
+  if (s == CS) return r.chi2;
+  if (s == CS + 1) return r.chi2 * r.gamma;
+  if (s < CS + 7) return r.outcome == s - CS - 2 ? 1.0f : 0.0f;
+  return 0.0f;
+}
+
+// Rigid ICP term (icp_impl_.h:103-152).  Returns the fixed leaf index reached
+// by the descent (-1 for num_bad).
 template <typename NodeFetch>
-KDH_HD int icp_term(const NodeFetch& fetch, const TermCache& C, const TermParams& P,
-                    const LeafIn& L, float* acc, int* depth_out) {
-#pragma unroll
-  for (int i = 0; i < kIcpSlots; ++i) acc[i] = 0.0f;
+KDH_HD int icp_term(const NodeFetch& fetch, const TermCache& C, const TermParams& P, const LeafIn& L, TermRec& R,
+                    int* depth_out) {
   const V3f p_mif = apply_iso(C.Xmf, L.mean);
   const int leaf = descend(fetch, p_mif, depth_out);
   if (leaf < 0) {
-    acc[29 + kBad] = 1.0f;
+    rec_reject(R, kBad);
     return -1;
   }
   float4 a, b;
@@ -180,44 +229,35 @@ KDH_HD int icp_term(const NodeFetch& fetch, const TermCache& C, const TermParams
   const V3f n_f = apply_rot(C.Xf, nf);
   // errorAndJacobian (d3/icp_.h:41-60)
   if (dot(n_f, n_m) < P.min_normal_cos) {
-    acc[29 + kErrNormal] = 1.0f;
+    rec_reject(R, kErrNormal);
     return leaf;
   }
   const V3f n = n_m + n_f;
   const float e = dot(p_m - p_f, n);
   if (fabsf(e) > P.max_error) {
-    acc[29 + kErrFn] = 1.0f;
+    rec_reject(R, kErrFn);
     return leaf;
   }
   const V3f jr = cross(p_m - C.c, n_f) + cross(p_f - C.c, n_m);
-  const float J[6] = {n.x, n.y, n.z, jr.x, jr.y, jr.z};
-  const float chi2 = e * e;
-  float gamma = 1.0f;
-  if (chi2 > P.kernel_threshold) {
-    acc[29 + kOutlier] = 1.0f;
-    gamma = P.stats_mode ? 0.0f : sqrtf(P.kernel_threshold / chi2);
+  R.J[0] = n.x; R.J[1] = n.y; R.J[2] = n.z;
+  R.J[3] = jr.x; R.J[4] = jr.y; R.J[5] = jr.z;
+  for (int i = 0; i < 6; ++i) R.Jv[i] = 0.0f;
+  R.e = e;
+  R.chi2 = e * e;
+  if (R.chi2 > P.kernel_threshold) {
+    R.outcome = kOutlier;
+    R.gamma = P.stats_mode ? 0.0f : sqrtf(P.kernel_threshold / R.chi2);
   } else {
-    acc[29 + kInlier] = 1.0f;
+    R.outcome = kInlier;
+    R.gamma = 1.0f;
   }
-  // ICPWsItemDiag_::pack (icp_ws_item_.h:26-34)
-  int k = 0;
-#pragma unroll
-  for (int r = 0; r < 6; ++r)
-#pragma unroll
-    for (int c = r; c < 6; ++c, ++k) acc[k] = (J[r] * J[c]) * gamma;
-#pragma unroll
-  for (int r = 0; r < 6; ++r, ++k) acc[k] = (J[r] * e) * gamma;
-  acc[27] = chi2;
-  acc[28] = chi2 * gamma;
   return leaf;
 }
 
-// Continuous-time term (pose + velocity, 12 DoF).
+// Continuous-time term (cticp_impl_.h:107-181).
 template <typename NodeFetch>
-KDH_HD int ct_term(const NodeFetch& fetch, const TermCache& C, const TermParams& P,
-                   const LeafIn& L, float* acc, int* depth_out) {
-#pragma unroll
-  for (int i = 0; i < kCtSlots; ++i) acc[i] = 0.0f;
+KDH_HD int ct_term(const NodeFetch& fetch, const TermCache& C, const TermParams& P, const LeafIn& L, TermRec& R,
+                   int* depth_out) {
   // applyVelocities on the leaf (d3/cticp_impl_.h:76-85)
   const float dt = L.dts;
   const V3f t_delta = v3(C.v[0] * dt, C.v[1] * dt, C.v[2] * dt);
@@ -228,7 +268,7 @@ KDH_HD int ct_term(const NodeFetch& fetch, const TermCache& C, const TermParams&
   const V3f p_mif = apply_iso(C.Xmf, wmean);
   const int leaf = descend(fetch, p_mif, depth_out);
   if (leaf < 0) {
-    acc[92 + kBad] = 1.0f;
+    rec_reject(R, kBad);
     return -1;
   }
   float4 a, b;
@@ -240,55 +280,38 @@ KDH_HD int ct_term(const NodeFetch& fetch, const TermCache& C, const TermParams&
   const V3f p_f = apply_iso(C.Xf, pf);
   const V3f n_f = apply_rot(C.Xf, nf);
   if (dot(n_f, n_m) < P.min_normal_cos) {
-    acc[92 + kErrNormal] = 1.0f;
+    rec_reject(R, kErrNormal);
     return leaf;
   }
   const V3f n = n_m + n_f;
   const V3f dp = p_m - p_f;
   const float e = dot(dp, n);
   if (fabsf(e) > P.max_error) {
-    acc[92 + kErrFn] = 1.0f;
+    rec_reject(R, kErrFn);
     return leaf;
   }
   const V3f jr = cross(p_m - C.c, n_f) + cross(p_f - C.c, n_m);
-  const float Jp[6] = {n.x, n.y, n.z, jr.x, jr.y, jr.z};
+  R.J[0] = n.x; R.J[1] = n.y; R.J[2] = n.z;
+  R.J[3] = jr.x; R.J[4] = jr.y; R.J[5] = jr.z;
   // errorAndJacobianVelocityA (d3/cticp_impl_.h:11-39) at the unwarped leaf:
-  //   J_t = dt * n^T Ra ; J_w = dt * (n^T Ra dR_p + dp^T Ra dR_n)
+  //   J_t = dt n^T Ra ; J_w = dt (n^T Ra dR_p + dp^T Ra dR_n)
   const M3f& Ra = C.Xm.R;
-  const V3f nR = mul_t(Ra, n);    // (n^T Ra)^T
-  const V3f dpR = mul_t(Ra, dp);  // (dp^T Ra)^T
+  const V3f nR = mul_t(Ra, n);
+  const V3f dpR = mul_t(Ra, dp);
   const M3f dRp = dRxp_f(rd, L.mean);
   const M3f dRn = dRxp_f(rd, L.dir);
   const V3f jw = mul_t(dRp, nR) + mul_t(dRn, dpR);
-  const float Jv[6] = {nR.x * dt, nR.y * dt, nR.z * dt, jw.x * dt, jw.y * dt, jw.z * dt};
-  const float chi2 = e * e;
-  float gamma = 1.0f;
-  if (chi2 > P.kernel_threshold) {
-    acc[92 + kOutlier] = 1.0f;
-    gamma = P.stats_mode ? 0.0f : sqrtf(P.kernel_threshold / chi2);
+  R.Jv[0] = nR.x * dt; R.Jv[1] = nR.y * dt; R.Jv[2] = nR.z * dt;
+  R.Jv[3] = jw.x * dt; R.Jv[4] = jw.y * dt; R.Jv[5] = jw.z * dt;
+  R.e = e;
+  R.chi2 = e * e;
+  if (R.chi2 > P.kernel_threshold) {
+    R.outcome = kOutlier;
+    R.gamma = P.stats_mode ? 0.0f : sqrtf(P.kernel_threshold / R.chi2);
   } else {
-    acc[92 + kInlier] = 1.0f;
+    R.outcome = kInlier;
+    R.gamma = 1.0f;
   }
-  int k = 0;
-#pragma unroll
-  for (int r = 0; r < 6; ++r)
-#pragma unroll
-    for (int c = r; c < 6; ++c, ++k) acc[k] = (Jp[r] * Jp[c]) * gamma;
-#pragma unroll
-  for (int r = 0; r < 6; ++r, ++k) acc[k] = (Jp[r] * e) * gamma;
-#pragma unroll
-  for (int r = 0; r < 6; ++r)
-#pragma unroll
-    for (int c = r; c < 6; ++c, ++k) acc[k] = (Jv[r] * Jv[c]) * gamma;
-#pragma unroll
-  for (int r = 0; r < 6; ++r, ++k) acc[k] = (Jv[r] * e) * gamma;
-  // ICPWsItemCross_::pack: ja^T jb * gamma (icp_ws_item_.h:72-75)
-#pragma unroll
-  for (int r = 0; r < 6; ++r)
-#pragma unroll
-    for (int c = 0; c < 6; ++c, ++k) acc[k] = (Jp[r] * Jv[c]) * gamma;
-  acc[90] = chi2;
-  acc[91] = chi2 * gamma;
   return leaf;
 }
 
