@@ -191,6 +191,62 @@ typedef struct kdh_pair_result {
  * with n_pairs = 1. */
 int kdh_batch_align(kdh_ctx* ctx, int n_pairs, const kdh_pair* pairs, kdh_pair_result* results);
 
+/* ---- serving: continuous batching ----------------------------------------
+ * An engine thread runs one Gauss-Newton iteration of every in-flight
+ * alignment per step (one tile + one finish launch per aligner kind), retires
+ * finished alignments and admits queued requests between steps while slots
+ * and the HBM budget allow.  Results are bit-identical to running the request
+ * alone.  Maps are uploaded by the caller (kdh_tree_upload on a context of the
+ * same device) and declared resident; moving trees travel with the request. */
+typedef struct kdh_request {
+  const kdh_tree* fixed;            /* resident map */
+  const kdh_node3f* moving_nodes;   /* host moving tree (Node_ array), copied at submit */
+  int64_t n_moving_nodes;
+  const uint64_t* moving_leaves;
+  int64_t n_moving_leaves;
+  int32_t ct;                       /* 0 rigid ICP, 1 CT-ICP */
+  int32_t pad;
+  double X0[12];
+  double v0[6];
+  double Xf[12];
+  kdh_params params;
+  const kdh_prior* prior;           /* may be NULL; copied at submit */
+} kdh_request;
+
+typedef struct kdh_request_result {
+  kdh_pair_result r;
+  double latency_us;                /* submit -> result visible on the host */
+  double queue_us;                  /* submit -> admission */
+  int32_t status;                   /* KDH_OK, KDH_ERR_CANCELLED or an error */
+  int32_t pad;
+} kdh_request_result;
+
+typedef struct kdh_server_info {
+  int64_t bytes_budget;
+  int64_t bytes_resident;
+  int64_t bytes_inflight;
+  int32_t inflight;
+  int32_t queued;
+  int64_t steps;
+  int64_t completed;
+} kdh_server_info;
+
+int kdh_server_create(int device, int64_t budget_bytes, int32_t max_inflight, kdh_server** out);
+int kdh_server_destroy(kdh_server* server);
+const char* kdh_server_last_error(kdh_server* server);
+/* Counts a resident map against the budget (KDH_ERR_BUDGET if it cannot fit). */
+int kdh_server_add_resident(kdh_server* server, const kdh_tree* map);
+int kdh_server_remove_resident(kdh_server* server, const kdh_tree* map);
+/* Queues a request; KDH_ERR_BUDGET if it can never fit beside the resident maps. */
+int kdh_server_submit(kdh_server* server, const kdh_request* request, int64_t* id);
+/* Non-blocking: *done = 1 and the result (consumed) once finished. */
+int kdh_server_poll(kdh_server* server, int64_t id, int* done, kdh_request_result* out);
+/* Blocks until finished (timeout_s <= 0: no timeout); consumes the result. */
+int kdh_server_wait(kdh_server* server, int64_t id, double timeout_s, kdh_request_result* out);
+/* Cancels a queued or in-flight request (its result status is KDH_ERR_CANCELLED). */
+int kdh_server_cancel(kdh_server* server, int64_t id);
+int kdh_server_info_get(kdh_server* server, kdh_server_info* info);
+
 /* Timing hook for benchmarks: average duration in microseconds of the tile
  * kernel launches of the last kdh_align / kdh_batch_align call on ctx, from
  * HIP events recorded on the context stream. */

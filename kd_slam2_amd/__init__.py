@@ -10,6 +10,7 @@ from .api import (  # noqa: F401
     DeviceTree,
     HostTree,
     ICPHIP3D,
+    Server,
     TreeGenerator3D,
     adapter_align,
     batch_align,

@@ -114,6 +114,45 @@ class PairResult(C.Structure):
     ]
 
 
+class Request(C.Structure):
+    _fields_ = [
+        ("fixed", C.c_void_p),
+        ("moving_nodes", C.c_void_p),
+        ("n_moving_nodes", C.c_int64),
+        ("moving_leaves", C.c_void_p),
+        ("n_moving_leaves", C.c_int64),
+        ("ct", C.c_int32),
+        ("pad", C.c_int32),
+        ("X0", C.c_double * 12),
+        ("v0", C.c_double * 6),
+        ("Xf", C.c_double * 12),
+        ("params", Params),
+        ("prior", C.c_void_p),
+    ]
+
+
+class RequestResult(C.Structure):
+    _fields_ = [
+        ("r", PairResult),
+        ("latency_us", C.c_double),
+        ("queue_us", C.c_double),
+        ("status", C.c_int32),
+        ("pad", C.c_int32),
+    ]
+
+
+class ServerInfo(C.Structure):
+    _fields_ = [
+        ("bytes_budget", C.c_int64),
+        ("bytes_resident", C.c_int64),
+        ("bytes_inflight", C.c_int64),
+        ("inflight", C.c_int32),
+        ("queued", C.c_int32),
+        ("steps", C.c_int64),
+        ("completed", C.c_int64),
+    ]
+
+
 class TreeGenParams(C.Structure):
     _fields_ = [
         ("propagate_normal", C.c_int32),
@@ -159,6 +198,16 @@ _SIGS = {
     "kdh_set_launch_config": (C.c_int, [P, C.c_int, C.c_int]),
     "kdh_set_profiling": (C.c_int, [P, C.c_int]),
     "kdh_last_kernel_breakdown": (C.c_int, [P, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "kdh_server_create": (C.c_int, [C.c_int, C.c_int64, C.c_int32, C.POINTER(P)]),
+    "kdh_server_destroy": (C.c_int, [P]),
+    "kdh_server_last_error": (C.c_char_p, [P]),
+    "kdh_server_add_resident": (C.c_int, [P, P]),
+    "kdh_server_remove_resident": (C.c_int, [P, P]),
+    "kdh_server_submit": (C.c_int, [P, C.POINTER(Request), C.POINTER(C.c_int64)]),
+    "kdh_server_poll": (C.c_int, [P, C.c_int64, C.POINTER(C.c_int), C.POINTER(RequestResult)]),
+    "kdh_server_wait": (C.c_int, [P, C.c_int64, C.c_double, C.POINTER(RequestResult)]),
+    "kdh_server_cancel": (C.c_int, [P, C.c_int64]),
+    "kdh_server_info_get": (C.c_int, [P, C.POINTER(ServerInfo)]),
     "kdh_tree_gen_params_default": (None, [C.POINTER(TreeGenParams)]),
     "kdh_host_tree_build": (C.c_int, [F32P, F64P, C.c_int64, C.POINTER(TreeGenParams), C.POINTER(P)]),
     "kdh_host_tree_info": (C.c_int, [P, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_double)]),

@@ -492,3 +492,89 @@ def batch_align(ctx, pairs):
         X[:3, :4] = np.array(r.X[:]).reshape(3, 4)
         out.append(dict(X=X, v=np.array(r.v[:]), iterations=r.iterations, result=r.result, last=r.last.as_dict()))
     return out
+
+
+# ---------------------------------------------------------------------------
+# Serving
+
+
+class Server:
+    """Continuous-batching alignment server (kdh_server_*): an engine thread
+    runs one Gauss-Newton iteration of every in-flight request per step and
+    admits queued requests between steps while slots and the HBM budget allow."""
+
+    def __init__(self, device=0, budget_bytes=64 << 30, max_inflight=64):
+        h = C.c_void_p()
+        rc = L.lib().kdh_server_create(device, int(budget_bytes), int(max_inflight), C.byref(h))
+        if rc != L.KDH_OK:
+            raise RuntimeError(f"kdh_server_create failed: {L.ERRORS.get(rc, rc)}")
+        self.h = h
+        self._keep = {}
+
+    def close(self):
+        if self.h:
+            L.lib().kdh_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc, what):
+        if rc != L.KDH_OK:
+            m = L.lib().kdh_server_last_error(self.h)
+            raise RuntimeError(f"{what} failed: {L.ERRORS.get(rc, rc)}: {m.decode() if m else ''}")
+
+    def add_resident(self, tree):
+        self._check(L.lib().kdh_server_add_resident(self.h, tree.h), "kdh_server_add_resident")
+
+    def submit(self, fixed, moving_host, X0, Xf, params, ct=0, v0=None, prior=None, raise_on_error=True):
+        """Queue an alignment of host tree `moving_host` against resident `fixed`.
+        Returns the request id (or the negative error code if not raise_on_error)."""
+        rq = L.Request()
+        rq.fixed = fixed.h.value
+        rq.moving_nodes = moving_host.nodes.ctypes.data
+        rq.n_moving_nodes = len(moving_host.nodes)
+        rq.moving_leaves = moving_host.leaves.ctypes.data
+        rq.n_moving_leaves = len(moving_host.leaves)
+        rq.ct = int(ct)
+        rq.X0[:] = list(np.asarray(X0)[:3, :4].reshape(12))
+        rq.v0[:] = list(np.asarray(np.zeros(6) if v0 is None else v0, dtype=np.float64))
+        rq.Xf[:] = list(np.asarray(Xf)[:3, :4].reshape(12))
+        rq.params = params
+        rq.prior = C.cast(C.pointer(prior), C.c_void_p).value if prior is not None else None
+        rid = C.c_int64()
+        rc = L.lib().kdh_server_submit(self.h, C.byref(rq), C.byref(rid))
+        if rc != L.KDH_OK:
+            if raise_on_error:
+                self._check(rc, "kdh_server_submit")
+            return rc
+        return rid.value
+
+    @staticmethod
+    def _result(rr):
+        X = np.eye(4)
+        X[:3, :4] = np.array(rr.r.X[:]).reshape(3, 4)
+        return dict(X=X, v=np.array(rr.r.v[:]), iterations=rr.r.iterations, result=rr.r.result,
+                    last=rr.r.last.as_dict(), latency_us=rr.latency_us, queue_us=rr.queue_us, status=rr.status)
+
+    def wait(self, rid, timeout_s=0.0):
+        rr = L.RequestResult()
+        self._check(L.lib().kdh_server_wait(self.h, int(rid), float(timeout_s), C.byref(rr)), "kdh_server_wait")
+        return self._result(rr)
+
+    def poll(self, rid):
+        rr = L.RequestResult()
+        done = C.c_int()
+        self._check(L.lib().kdh_server_poll(self.h, int(rid), C.byref(done), C.byref(rr)), "kdh_server_poll")
+        return self._result(rr) if done.value else None
+
+    def cancel(self, rid):
+        self._check(L.lib().kdh_server_cancel(self.h, int(rid)), "kdh_server_cancel")
+
+    def info(self):
+        i = L.ServerInfo()
+        self._check(L.lib().kdh_server_info_get(self.h, C.byref(i)), "kdh_server_info_get")
+        return {f: getattr(i, f) for f, _ in i._fields_}

@@ -317,6 +317,8 @@ int kdh_launch(kdh_ctx* ctx, PairState* d_pairs, const int* d_tile_pair, int n_t
   a.pair_first = 0;
   a.n_pairs = n_pairs;
   a.phases = phases;
+  a.pair_tile_first = nullptr;
+  a.status_out = nullptr;
   a.pairs = d_pairs;
   a.tile_pair = d_tile_pair;
   a.n_tiles_total = n_tiles;

@@ -21,6 +21,8 @@ struct LaunchArgs {
   int pair_first;         // finish stage: alignments [pair_first, pair_first + n_pairs)
   int n_pairs;            // (0: no finish launch)
   int phases;             // bit 0: tile kernel, bit 1: finish stage
+  const int* pair_tile_first;  // optional: first tile of each pair in this launch (else PairState::tile_first)
+  int* status_out;        // optional: per-pair status mirror written by the finish stage
 };
 
 hipError_t launch_tiles(const LaunchArgs& a, hipStream_t s);

@@ -177,7 +177,8 @@ struct FinishLds {
 };
 
 template <bool CT>
-__device__ void finish_pair(PairState& ps, const float* __restrict__ partials, char* lds) {
+__device__ void finish_pair(PairState& ps, const float* __restrict__ partials, char* lds, int tile_first,
+                            int* status_out) {
   using FL = FinishLds<CT>;
   constexpr int N = Slots<CT>::N;
   constexpr int D = CT ? 12 : 6;
@@ -214,7 +215,7 @@ __device__ void finish_pair(PairState& ps, const float* __restrict__ partials, c
   // 1. FP64 sum of the pair's partial rows in tile order.  Rows are staged
   //    through LDS CH at a time with coalesced loads by all threads; thread s
   //    (< N) adds them in order, so the result is deterministic.
-  const float* base = partials + (size_t)ps.tile_first * N;
+  const float* base = partials + (size_t)tile_first * N;
   double acc = 0.0;
   for (int t0 = 0; t0 < ps.n_tiles; t0 += CH) {
     const int nt = min(CH, ps.n_tiles - t0);
@@ -332,6 +333,7 @@ __device__ void finish_pair(PairState& ps, const float* __restrict__ partials, c
       ps.stats_log[it] = st;
       ps.iter = it + 1;
       ps.status = 2;  // failed: compute() returns -it
+      if (status_out) status_out[0] = 2;
     }
     return;
   }
@@ -382,6 +384,7 @@ __device__ void finish_pair(PairState& ps, const float* __restrict__ partials, c
   }
   if (stop && status == 0) status = 1;
   ps.status = status;
+  if (status_out) status_out[0] = status;
   // next iteration's FP32 cache (updateCache, icp_impl_.h:16-21), formed in
   // FP64 and rounded once.
   const IsoD Xf = iso_from_34<double>(Xfs);
@@ -403,7 +406,8 @@ __global__ __launch_bounds__(kThreads) void tiles_kernel(const PairState* __rest
                                                          float* __restrict__ partials,
                                                          int* __restrict__ corr_out,
                                                          int* __restrict__ depth_out,
-                                                         int xcd_remap) {
+                                                         int xcd_remap,
+                                                         const int* __restrict__ pair_tile_first) {
   constexpr int N = Slots<CT>::N;
   constexpr int NCH = (N + 63) / 64;  // 64-slot chunks
   __shared__ float4 s_nodes[LDS_NODES > 0 ? 2 * LDS_NODES : 2];
@@ -446,7 +450,7 @@ __global__ __launch_bounds__(kThreads) void tiles_kernel(const PairState* __rest
       cached_pair = p;
       __syncthreads();
     }
-    const int lt = t - ps.tile_first;
+    const int lt = t - (pair_tile_first ? pair_tile_first[p] : ps.tile_first);
     const int q = lt * kTile + tid;
 
     TermRec R;
@@ -494,13 +498,16 @@ __global__ __launch_bounds__(kThreads) void tiles_kernel(const PairState* __rest
 // build-mode publish or solve-mode Gauss-Newton step.
 template <bool CT>
 __global__ __launch_bounds__(kThreads) void finish_kernel(PairState* __restrict__ pairs, int pair_first,
-                                                          int n_pairs, const float* __restrict__ partials) {
+                                                          int n_pairs, const float* __restrict__ partials,
+                                                          const int* __restrict__ pair_tile_first,
+                                                          int* __restrict__ status_out) {
   __shared__ __align__(16) char s_area[FinishLds<CT>::kBytes];
   const int p = pair_first + blockIdx.x;
   if (blockIdx.x >= n_pairs) return;
   PairState& ps = pairs[p];
   if (ps.status != 0 || ps.ct != (CT ? 1 : 0) || ps.n_tiles == 0) return;
-  finish_pair<CT>(ps, partials, s_area);
+  finish_pair<CT>(ps, partials, s_area, pair_tile_first ? pair_tile_first[p] : ps.tile_first,
+                  status_out ? status_out + p : nullptr);
 }
 
 // ---- launch wrappers --------------------------------------------------------
@@ -509,13 +516,13 @@ template <bool CT, int LDS>
 static hipError_t launch_t(const LaunchArgs& a, hipStream_t s) {
   if (a.phases & 1) {
     hipLaunchKernelGGL((tiles_kernel<CT, LDS>), dim3(a.grid), dim3(kThreads), 0, s, a.pairs, a.tile_pair,
-                       a.n_tiles_total, a.partials, a.corr_out, a.depth_out, a.xcd_remap);
+                       a.n_tiles_total, a.partials, a.corr_out, a.depth_out, a.xcd_remap, a.pair_tile_first);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
   }
   if ((a.phases & 2) && a.n_pairs > 0) {
     hipLaunchKernelGGL((finish_kernel<CT>), dim3(a.n_pairs), dim3(kThreads), 0, s, a.pairs, a.pair_first,
-                       a.n_pairs, a.partials);
+                       a.n_pairs, a.partials, a.pair_tile_first, a.status_out);
     return hipGetLastError();
   }
   return hipSuccess;

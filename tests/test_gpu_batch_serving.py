@@ -1,0 +1,125 @@
+"""Batched and served forms: results bit-identical alone and in a mix,
+against the single-alignment device loop, admission by budget, cancellation.
+GPU."""
+import numpy as np
+import pytest
+
+from kd_slam2_amd import DeviceTree, Server, TreeGenerator3D, batch_align, synth
+from oracle import pyoracle as O
+
+from .workloads import cfg1, cfg2, icp_params
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def small_pairs():
+    """Six rigid loop-closure-like pairs of uneven size and two CT pairs."""
+    g = TreeGenerator3D(leaf_max_points=16)
+    out = []
+    for i, n in enumerate([3000, 9000, 20000, 5000, 40000, 12000]):
+        w = synth.config1(seed=3000 + i, n_map=3 * n, n_scan=n, rot_deg=5.0, trans_m=1.0, uniform_init=True)
+        w["fixed"] = g.makeTree(w["map"])
+        w["moving"] = g.makeTree(w["scan"])
+        w["ct"] = 0
+        out.append(w)
+    w2 = cfg2()
+    for k in range(2):
+        rng = np.random.default_rng(10 + k)
+        w = dict(w2)
+        w["X0"] = synth.perturb(w2["X_gt"], rng, 1.0, 0.3)
+        w["ct"] = 1
+        out.append(w)
+    return out
+
+
+def _pairs(ctx, ws, trees):
+    pairs = []
+    for w, (F, M) in zip(ws, trees):
+        p = icp_params(max_iterations=15 if w["ct"] else 10, velocity_damping=10.0)
+        pairs.append(dict(fixed=F, moving=M, ct=w["ct"], X0=w["X0"], Xf=w["Xf"], v0=w["v0"], params=p))
+    return pairs
+
+
+def test_batch_identical_alone_and_in_mix(ctx, small_pairs):
+    trees = [(DeviceTree(ctx, w["fixed"]), DeviceTree(ctx, w["moving"])) for w in small_pairs]
+    pairs = _pairs(ctx, small_pairs, trees)
+    mix = batch_align(ctx, pairs)
+    for i, pr in enumerate(pairs):
+        alone = batch_align(ctx, [pr])[0]
+        assert np.array_equal(alone["X"], mix[i]["X"]), i
+        assert np.array_equal(alone["v"], mix[i]["v"]), i
+        assert alone["last"] == mix[i]["last"], i
+        assert alone["result"] == mix[i]["result"]
+    # reversed order gives the same per-pair results
+    rev = batch_align(ctx, pairs[::-1])[::-1]
+    for i in range(len(pairs)):
+        assert np.array_equal(rev[i]["X"], mix[i]["X"])
+    # and the batch agrees with the FP32 oracle within the pose tolerance
+    for i, w in enumerate(small_pairs):
+        op = O.from_kdh_params(pairs[i]["params"], 8)
+        Xo, vo, _, res_o = O.compute(w["fixed"], w["moving"], w["X0"], w["Xf"], op, ct=w["ct"], v0=w["v0"])
+        dt, dr = synth.pose_error(mix[i]["X"], Xo)
+        print(i, "vs oracle", dt, dr, "vs gt", synth.pose_error(mix[i]["X"], w["X_gt"]))
+        assert dt < 1e-3 and dr < 1e-3, i
+
+
+def test_server_identical_alone_and_in_mix(ctx, small_pairs):
+    srv = Server(0, budget_bytes=8 << 30, max_inflight=16)
+    try:
+        fixed = {}
+        for i, w in enumerate(small_pairs):
+            key = id(w["fixed"])
+            if key not in fixed:
+                fixed[key] = DeviceTree(ctx, w["fixed"])
+                srv.add_resident(fixed[key])
+        prm = [icp_params(max_iterations=15 if w["ct"] else 10, velocity_damping=10.0) for w in small_pairs]
+        alone = []
+        for i, w in enumerate(small_pairs):
+            rid = srv.submit(fixed[id(w["fixed"])], w["moving"], w["X0"], w["Xf"], prm[i], ct=w["ct"], v0=w["v0"])
+            alone.append(srv.wait(rid, 60.0))
+        # all at once: arrivals overlap in-flight work
+        ids = [srv.submit(fixed[id(w["fixed"])], w["moving"], w["X0"], w["Xf"], prm[i], ct=w["ct"], v0=w["v0"])
+               for i, w in enumerate(small_pairs)]
+        mixed = [srv.wait(r, 60.0) for r in ids]
+        for a, m in zip(alone, mixed):
+            assert a["status"] == 0 and m["status"] == 0
+            assert np.array_equal(a["X"], m["X"]) and np.array_equal(a["v"], m["v"])
+            assert a["last"] == m["last"] and a["result"] == m["result"]
+        # same answer as the batched launch
+        trees = [(fixed[id(w["fixed"])], DeviceTree(ctx, w["moving"])) for w in small_pairs]
+        b = batch_align(ctx, _pairs(ctx, small_pairs, trees))
+        for i in range(len(small_pairs)):
+            assert np.array_equal(b[i]["X"], mixed[i]["X"]), i
+        info = srv.info()
+        assert info["inflight"] == 0 and info["queued"] == 0 and info["bytes_inflight"] == 0
+    finally:
+        srv.close()
+
+
+def test_server_budget_and_cancel(ctx, small_pairs):
+    w = small_pairs[4]
+    F = DeviceTree(ctx, w["fixed"])
+    srv = Server(0, budget_bytes=F.device_bytes() + 4096, max_inflight=4)
+    try:
+        srv.add_resident(F)
+        # a moving tree that can never fit beside the resident map is refused
+        rc = srv.submit(F, w["moving"], w["X0"], w["Xf"], icp_params(), raise_on_error=False)
+        assert rc == -4  # KDH_ERR_BUDGET
+    finally:
+        srv.close()
+    srv = Server(0, budget_bytes=1 << 30, max_inflight=1)
+    try:
+        srv.add_resident(F)
+        p = icp_params(max_iterations=64)
+        ids = [srv.submit(F, w["moving"], w["X0"], w["Xf"], p) for _ in range(3)]
+        srv.cancel(ids[2])  # queued behind a single slot: cancelled before it runs
+        r2 = srv.wait(ids[2], 60.0)
+        assert r2["status"] == -6  # KDH_ERR_CANCELLED
+        r0 = srv.wait(ids[0], 60.0)
+        r1 = srv.wait(ids[1], 60.0)
+        assert r0["status"] == 0 and r1["status"] == 0
+        assert np.array_equal(r0["X"], r1["X"])
+        assert r1["queue_us"] >= 0.0
+    finally:
+        srv.close()
