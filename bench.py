@@ -1,27 +1,43 @@
 """Benchmark of the MI355X scan-alignment path (one JSON line on stdout).
 
-Workload: CT-ICP with deskew, scan to map -- a 128-beam, 0.1 s sweep of
-131,072 points (per-point stamps, 10 m/s forward, 30 deg/s yaw) against a
-2,000,000-point outdoor map, 12-DoF pose + velocity, 15 Gauss-Newton
-iterations, leaf policy defaults.
+Headline workload (`value`): CT-ICP with deskew, scan to map -- a 128-beam,
+0.1 s sweep of 131,072 points (per-point stamps, 10 m/s forward, 30 deg/s
+yaw) against a 2,000,000-point outdoor map, 12-DoF pose + velocity, 15
+Gauss-Newton iterations, leaf policy defaults.
 
 A step is one served batch: `--batch` concurrent alignments of that workload
-(the same map resident in HBM, each request with its own scan tree and its
-own initial pose / velocity) run to completion by kdh_batch_align -- one
-persistent launch per Gauss-Newton iteration, replayed from a HIP graph.
+(the map resident in HBM, each alignment with its own initial pose and
+velocity) run to completion by kdh_batch_align -- one tile launch and one
+finish launch per Gauss-Newton iteration, all iterations in one HIP graph.
 value = scan points of completed alignments per second over the timed steps
 (all iterations included; inputs resident in HBM before timing starts).
 
-Beside it the line reports the plain one-alignment-per-call path (host
-Gauss-Newton loop, one kernel build + host FP64 solve per iteration), the
-device-resident single-alignment loop, per-alignment latency percentiles,
-the roofline of the tile kernel (HIP events around each launch) and the CPU
-oracle timed on the host cores.
+Beside it the line reports
+  * the plain one-alignment-per-call path (CTICPHIP3D::compute from the C++
+    plugin: host Gauss-Newton loop, one kernel build + host FP64 solve per
+    iteration) and the device-resident single-alignment loop;
+  * the serving mix: 4 odometry sequences at 10 Hz (64k-point CT sweeps
+    against their own resident 4M-point maps) and loop-closure requests at
+    20/s (rigid, 8k-200k points, map 3x), Poisson arrivals into the
+    continuous-batching server with a 64 GB budget, p50/p99 latency from
+    arrival to host-visible result;
+  * the tile kernel's roofline (HIP events on the context stream) and its
+    HBM traffic from two rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE)
+    scaled by the same counters on a device copy of known size;
+  * the CPU oracle (restated ICP_CPU_ / CTICP_CPU_) on 8 host threads.
 """
 import argparse
+import csv
+import ctypes
+import glob
 import json
+import multiprocessing as mp
 import os
+import shutil
+import subprocess
 import sys
+import tempfile
+import threading
 import time
 
 import numpy as np
@@ -31,6 +47,7 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+COPY_BYTES = 256 << 20
 
 
 def log(*a):
@@ -45,6 +62,132 @@ def algorithmic_bytes_per_iter(depth, ct, n_tiles):
     return float(per_q * len(depth) + 32 * int(np.sum(depth)) + n_tiles * 4 * (90 if ct else 27))
 
 
+# ---------------------------------------------------------------------------
+# workload generation (host only; runs in worker processes before any GPU use)
+
+
+def _gen_cfg2(_):
+    from kd_slam2_amd import TreeGenerator3D, synth
+
+    w = synth.config2()
+    g = TreeGenerator3D(parallel_level=2)
+    w["fixed"] = g.makeTree(w["map"])
+    w["moving"] = g.makeTree(w["scan"], w["stamps"])
+    del w["map"]
+    return w
+
+
+def _gen_seq(s):
+    from kd_slam2_amd import TreeGenerator3D, synth
+
+    q = synth.config5_sequence(s)
+    g = TreeGenerator3D(parallel_level=2)
+    q["fixed"] = g.makeTree(q["map"])
+    del q["map"]
+    for f in q["frames"]:
+        f["moving"] = g.makeTree(f["scan"], f["stamps"])
+        f["n_pts"] = len(f["scan"])
+        del f["scan"], f["stamps"]
+    return q
+
+
+def _gen_loop(j):
+    from kd_slam2_amd import TreeGenerator3D, synth
+
+    w = synth.config5_loop_pair(j)
+    g = TreeGenerator3D(leaf_max_points=16)
+    w["fixed"] = g.makeTree(w["map"])
+    w["moving"] = g.makeTree(w["scan"])
+    w["n_pts"] = len(w["scan"])
+    del w["map"], w["scan"]
+    return w
+
+
+# ---------------------------------------------------------------------------
+# PMC traffic: rocprofv3 --pmc passes over a child run of this script
+
+
+def _hip():
+    h = ctypes.CDLL("libamdhip64.so")
+    h.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+    h.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+    h.hipFree.argtypes = [ctypes.c_void_p]
+    return h
+
+
+def pmc_child(args):
+    """One served batch of the headline workload plus a device copy of
+    COPY_BYTES bytes, for the counter passes."""
+    from kd_slam2_amd import Context, DeviceTree, batch_align, default_params, synth
+
+    w = _gen_cfg2(0)
+    ctx = Context(0)
+    F, M = DeviceTree(ctx, w["fixed"]), DeviceTree(ctx, w["moving"])
+    p = _cfg2_params(default_params)
+    rng = np.random.default_rng(np.random.PCG64(99))
+    pairs = [dict(fixed=F, moving=M, ct=1, X0=synth.perturb(w["X_gt"], rng, 1.0, 0.3), Xf=w["Xf"], v0=w["v0"],
+                  params=p) for _ in range(args.batch)]
+    batch_align(ctx, pairs)
+    hip = _hip()
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    hip.hipMalloc(ctypes.byref(a), COPY_BYTES)
+    hip.hipMalloc(ctypes.byref(b), COPY_BYTES)
+    hip.hipMemcpy(b, a, COPY_BYTES, 3)  # device to device
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print("pmc child done", flush=True)
+
+
+def pmc_pass(counter, batch, tiles_per_alignment):
+    """Per-dispatch counter (KiB) for the served tile kernel and for the copy
+    kernel, from one rocprofv3 --pmc run.  Returns (tile, copy) or None."""
+    exe = shutil.which("rocprofv3")
+    if not exe:
+        return None
+    out = tempfile.mkdtemp(prefix="kdh_pmc_")
+    cmd = ["timeout", "-s", "KILL", "150", exe, "--pmc", counter, "-d", out, "-o", "pmc", "--output-format", "csv",
+           "--", sys.executable, os.path.abspath(__file__), "--pmc-child", "--batch", str(batch)]
+    try:
+        r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, timeout=170, cwd=ROOT,
+                           env=dict(os.environ, TMPDIR="/tmp"))
+        files = glob.glob(os.path.join(out, "**", "*counter_collection.csv"), recursive=True)
+        if not files:
+            log(f"pmc {counter}: no counter file (rc {r.returncode})")
+            return None
+        tile, copy = [], []
+        with open(files[0]) as fh:
+            for row in csv.DictReader(fh):
+                if row.get("Counter_Name") != counter:
+                    continue
+                name = row.get("Kernel_Name", "")
+                grid = int(row.get("Grid_Size", row.get("Grid_Size_X", "0")) or 0)
+                v = float(row["Counter_Value"])
+                if "tiles_kernel" in name and grid == 256 * batch * tiles_per_alignment:
+                    tile.append(v)
+                elif "copyBuffer" in name and grid > 1 << 16:
+                    copy.append(v)
+        if not tile or not copy:
+            log(f"pmc {counter}: tile {len(tile)} copy {len(copy)} dispatches")
+            return None
+        return float(np.median(tile)), float(max(copy))
+    except Exception as e:  # counters are reported as null when unavailable
+        log(f"pmc {counter}: {e}")
+        return None
+    finally:
+        shutil.rmtree(out, ignore_errors=True)
+
+
+def _cfg2_params(default_params):
+    p = default_params()
+    p.max_error = 0.5
+    p.max_iterations = 15
+    p.velocity_damping = 10.0
+    return p
+
+
+# ---------------------------------------------------------------------------
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -54,30 +197,40 @@ def main():
     ap.add_argument("--plain-runs", type=int, default=5)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--serve-seconds", type=float, default=3.0, help="length of the serving-mix arrival window")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-pmc", action="store_true")
+    ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
+    if args.pmc_child:
+        pmc_child(args)
+        return
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the alignment path shards by pair "
-              "with no communication, see DESIGN.md", file=sys.stderr)
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); alignments shard by pair with no "
+              "communication across GPUs, see DESIGN.md", file=sys.stderr)
         sys.exit(2)
 
-    from kd_slam2_amd import (CTICPHIP3D, Context, DeviceTree, TreeGenerator3D, adapter_align, batch_align,
-                              default_params, synth)
-
     t0 = time.time()
-    w = synth.config2()
-    g = TreeGenerator3D(parallel_level=3)
-    fixed = g.makeTree(w["map"])
-    moving = g.makeTree(w["scan"], w["stamps"])
-    n_pts = len(w["scan"])
-    log(f"workload ready in {time.time() - t0:.1f}s: fixed {fixed.num_nodes} nodes, moving {moving.num_leaves} leaves")
+    n_loop_pool = 16
+    with mp.get_context("fork").Pool(min(16, os.cpu_count() or 4)) as pool:
+        j_cfg2 = pool.apply_async(_gen_cfg2, (0,))
+        j_seq = None if args.no_serve else pool.map_async(_gen_seq, range(4))
+        j_loop = None if args.no_serve else pool.map_async(_gen_loop, range(n_loop_pool))
+        w = j_cfg2.get()
+        seqs = j_seq.get() if j_seq else []
+        loops = j_loop.get() if j_loop else []
+    fixed, moving = w["fixed"], w["moving"]
+    n_pts = 131_072
+    log(f"workloads ready in {time.time() - t0:.1f}s: fixed {fixed.num_nodes} nodes, moving {moving.num_leaves} "
+        f"leaves")
+
+    from kd_slam2_amd import (CTICPHIP3D, Context, DeviceTree, Server, adapter_align, batch_align,
+                              default_params, synth)
 
     ctx = Context(0)
     F = DeviceTree(ctx, fixed)
     M = DeviceTree(ctx, moving)
-    p = default_params()
-    p.max_error = 0.5
-    p.max_iterations = 15
-    p.velocity_damping = 10.0
+    p = _cfg2_params(default_params)
 
     def aligner():
         al = CTICPHIP3D(ctx)
@@ -95,12 +248,10 @@ def main():
     b_iter = algorithmic_bytes_per_iter(depth, 1, n_tiles)
 
     # ---- plain per-call path: CTICPHIP3D::compute() of the C++ plugin -----
-    # (host Gauss-Newton loop, one kdh_cticp_build + host FP64 solve per
-    # iteration -- the drop-in for CTICPCUDA3D)
     plain_lat = []
     for i in range(args.plain_runs + 1):
         t = time.perf_counter()
-        Xp, vp, res_p, _, _ = adapter_align("CTICPHIP3D", ctx, F, M, w["X0"], w["Xf"], p, v0=w["v0"])
+        adapter_align("CTICPHIP3D", ctx, F, M, w["X0"], w["Xf"], p, v0=w["v0"])
         dt = time.perf_counter() - t
         if i > 0:
             plain_lat.append(dt)
@@ -108,8 +259,7 @@ def main():
     log(f"plain per-call: median {1e3 * np.median(plain_lat):.2f} ms/alignment, {plain_pps / 1e6:.1f} Mpts/s")
 
     # ---- device-resident loop, one alignment ------------------------------
-    dev_lat = []
-    kern_us = []
+    dev_lat, kern_us = [], []
     for i in range(args.plain_runs + 1):
         al = aligner()
         t = time.perf_counter()
@@ -119,18 +269,15 @@ def main():
             dev_lat.append(dt)
             kern_us.append(ctx.last_kernel_time()[0])
     dev_pps = n_pts / np.median(dev_lat)
-    log(f"device loop: median {1e3 * np.median(dev_lat):.2f} ms/alignment, kernel {np.median(kern_us):.1f} us/iter")
+    log(f"device loop: median {1e3 * np.median(dev_lat):.2f} ms/alignment, {np.median(kern_us):.1f} us/iter")
 
-    # ---- served batches ---------------------------------------------------
+    # ---- served batches (headline) ----------------------------------------
     rng = np.random.default_rng(np.random.PCG64(99))
-    pairs = []
-    for k in range(args.batch):
-        X0 = synth.perturb(w["X_gt"], rng, 1.0, 0.3)
-        pairs.append(dict(fixed=F, moving=M, ct=1, X0=X0, Xf=w["Xf"], v0=w["v0"], params=p))
+    pairs = [dict(fixed=F, moving=M, ct=1, X0=synth.perturb(w["X_gt"], rng, 1.0, 0.3), Xf=w["Xf"], v0=w["v0"],
+                  params=p) for _ in range(args.batch)]
     for _ in range(args.warmup):
         batch_align(ctx, pairs)
-    step_t = []
-    batch_kern = []
+    step_t, batch_kern = [], []
     for _ in range(args.steps):
         t = time.perf_counter()
         out = batch_align(ctx, pairs)
@@ -140,11 +287,10 @@ def main():
     value = args.batch * n_pts * args.steps / total
     errs = [synth.pose_error(o["X"], w["X_gt"]) for o in out]
     log(f"served: {1e3 * np.mean(step_t):.2f} ms/step of {args.batch}, {value / 1e6:.1f} Mpts/s, "
-        f"launch {np.median(batch_kern):.1f} us, max err {max(e[0] for e in errs):.4f} m")
+        f"{np.median(batch_kern):.1f} us/iter, max err {max(e[0] for e in errs):.4f} m")
 
-    # roofline of the tile kernel in the served launch: one extra profiled
-    # batch (no graph, HIP events around the tile kernel and the finish
-    # kernel separately, on the context stream)
+    # roofline of the tile kernel in the served launch (profiled batches: no
+    # graph, HIP events around 4 back-to-back tile launches and the finish)
     ctx.set_profiling(True)
     prof_tile, prof_fin = [], []
     for _ in range(3):
@@ -155,11 +301,15 @@ def main():
     ctx.set_profiling(False)
     tile_us = float(np.median(prof_tile))
     achieved = args.batch * b_iter / (tile_us * 1e-6) / 1e9
-    roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None)
     log(f"tile kernel {tile_us:.1f} us, finish kernel {np.median(prof_fin):.1f} us per iteration "
         f"-> {achieved:.0f} GB/s algorithmic")
 
+    # ---- serving mix --------------------------------------------------------
+    serving = None
+    if not args.no_serve:
+        serving = run_serving(ctx, seqs, loops, args.serve_seconds, DeviceTree, Server, default_params)
+
+    # ---- CPU baseline -------------------------------------------------------
     cpu = None
     if not args.no_cpu:
         from oracle import pyoracle as O
@@ -172,10 +322,28 @@ def main():
             n_done += 1
         el = time.perf_counter() - t
         cpu = dict(value=round(n_done * n_pts / el, 1), unit="scan-pts/s", cores=8, kind="port",
-                   sample=f"{n_done} full CT-ICP alignments (15 iterations) of the same workload on 8 threads "
+                   sample=f"{n_done} full CT-ICP alignments (15 iterations) of the headline workload on 8 threads "
                           f"(oracle/liboracle_fast.so, std::async chunks as ICP_CPU_), {el:.1f} s")
         log(f"cpu oracle: {cpu['value'] / 1e6:.2f} Mpts/s")
 
+    # ---- HBM traffic of the tile kernel -------------------------------------
+    traffic = None
+    pmc = {}
+    if not args.no_pmc:
+        f = pmc_pass("FETCH_SIZE", args.batch, n_tiles)
+        wr = pmc_pass("WRITE_SIZE", args.batch, n_tiles)
+        if f and wr:
+            kib = COPY_BYTES / 1024.0
+            f_scale = kib / f[1]
+            w_scale = kib / wr[1]
+            traffic = round((f[0] * f_scale + wr[0] * w_scale) * 1024.0, 1)
+            pmc = dict(fetch_kib=round(f[0], 1), write_kib=round(wr[0], 1), copy_fetch_kib=round(f[1], 1),
+                       copy_write_kib=round(wr[1], 1), fetch_scale=round(f_scale, 4), write_scale=round(w_scale, 4))
+            log(f"pmc: tile fetch {f[0]:.0f} KiB, write {wr[0]:.0f} KiB per launch (copy {f[1]:.0f}/{wr[1]:.0f} "
+                f"KiB for {kib:.0f} KiB)")
+
+    roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic)
     line = dict(
         metric="scan-pts aligned/s + p99 align latency, served vs plain HIP, <=1mm/1e-3deg",
         value=round(value, 1),
@@ -202,14 +370,115 @@ def main():
                         served_batch_p50=round(1e3 * float(np.percentile(step_t, 50)), 3),
                         served_batch_p99=round(1e3 * float(np.percentile(step_t, 99)), 3)),
         kernel_us_per_iter=dict(device_loop=round(float(np.median(kern_us)), 2),
-                                served_launch=round(float(np.median(batch_kern)), 2),
+                                served_iteration=round(float(np.median(batch_kern)), 2),
                                 served_tile_kernel=round(tile_us, 2),
                                 served_finish_kernel=round(float(np.median(prof_fin)), 2)),
         algorithmic_bytes_per_iter=b_iter,
+        pmc=pmc or None,
+        serving_mix=serving,
         roofline=roofline,
         cpu_baseline=cpu,
     )
     print(json.dumps(line), flush=True)
+
+
+def run_serving(ctx, seqs, loops, seconds, DeviceTree, Server, default_params):
+    """Config 5: Poisson arrivals into the continuous-batching server."""
+    srv = Server(0, budget_bytes=64 << 30, max_inflight=64)
+    keep = []
+    seq_maps = []
+    for q in seqs:
+        T = DeviceTree(ctx, q["fixed"])
+        srv.add_resident(T)
+        seq_maps.append(T)
+    loop_maps = []
+    for lp in loops:
+        T = DeviceTree(ctx, lp["fixed"])
+        srv.add_resident(T)
+        loop_maps.append(T)
+    keep += seq_maps + loop_maps
+    p_ct = default_params()
+    p_ct.max_error = 0.5
+    p_ct.max_iterations = 15
+    p_ct.velocity_damping = 10.0
+    p_icp = default_params()
+    p_icp.max_error = 0.5
+    p_icp.max_iterations = 10
+
+    def submit(kind, idx, frame):
+        if kind == "odom":
+            q, f = seqs[idx], seqs[idx]["frames"][frame]
+            return srv.submit(seq_maps[idx], f["moving"], f["X0"], q["Xf"], p_ct, ct=1, v0=f["v0"]), f["n_pts"]
+        lp = loops[idx]
+        return srv.submit(loop_maps[idx], lp["moving"], lp["X0"], lp["Xf"], p_icp, ct=0), lp["n_pts"]
+
+    # warm the engine with one request of each kind (not timed)
+    for kind in ("odom", "loop"):
+        rid, _ = submit(kind, 0, 0)
+        srv.wait(rid, 60.0)
+    rng = np.random.default_rng(np.random.PCG64(5999))
+    from kd_slam2_amd import synth
+
+    events = []
+    for s in range(len(seqs)):
+        for k, t in enumerate(synth.poisson_arrivals(10.0, seconds, rng)):
+            events.append((t, "odom", s, k % len(seqs[s]["frames"])))
+    for t in synth.poisson_arrivals(20.0, seconds, rng):
+        events.append((t, "loop", int(rng.integers(len(loops))), 0))
+    events.sort()
+    submitted = []
+    lock = threading.Lock()
+
+    def submitter():
+        t0 = time.perf_counter()
+        for t, kind, idx, frame in events:
+            now = time.perf_counter() - t0
+            if t > now:
+                time.sleep(t - now)
+            rid, npts = submit(kind, idx, frame)
+            with lock:
+                submitted.append((rid, kind, npts))
+
+    th = threading.Thread(target=submitter)
+    t_start = time.perf_counter()
+    th.start()
+    results = []
+    k = 0
+    while True:
+        with lock:
+            pending = submitted[k:] if k < len(submitted) else []
+        if not pending:
+            if not th.is_alive() and k >= len(submitted):
+                break
+            time.sleep(0.0005)
+            continue
+        for rid, kind, npts in pending:
+            r = srv.wait(rid, 60.0)
+            results.append((kind, npts, r))
+            k += 1
+    th.join()
+    wall = time.perf_counter() - t_start
+    srv_info = srv.info()
+    srv.close()
+    for T in keep:
+        T.close()
+
+    def pct(kind=None):
+        lat = [r["latency_us"] / 1e3 for kk, _, r in results if kind is None or kk == kind]
+        if not lat:
+            return None
+        return dict(n=len(lat), p50_ms=round(float(np.percentile(lat, 50)), 3),
+                    p99_ms=round(float(np.percentile(lat, 99)), 3), max_ms=round(float(np.max(lat)), 3))
+
+    ok = all(r["status"] == 0 for _, _, r in results)
+    pts = sum(n for _, n, _ in results)
+    out = dict(window_s=seconds, requests=len(results), all_ok=ok, odometry=pct("odom"), loop_closure=pct("loop"),
+               all=pct(), pts_per_s=round(pts / wall, 1), engine_steps=srv_info["steps"],
+               budget_gb=64, resident_maps=len(keep))
+    log(f"serving: {len(results)} requests in {wall:.2f}s, odometry p50/p99 {out['odometry']['p50_ms']}/"
+        f"{out['odometry']['p99_ms']} ms, loop p50/p99 {out['loop_closure']['p50_ms']}/"
+        f"{out['loop_closure']['p99_ms']} ms")
+    return out
 
 
 if __name__ == "__main__":

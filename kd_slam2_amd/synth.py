@@ -384,3 +384,44 @@ def config3_pair(i, n_scan):
     scan, initial error U(0, 5 deg) / U(0, 1 m)."""
     return config1(seed=3000 + i, n_map=3 * int(n_scan), n_scan=int(n_scan), rot_deg=5.0, trans_m=1.0,
                    uniform_init=True)
+
+
+def config5_sequence(s, n_map=4_000_000, n_scan=65_536, n_frames=4, extent=120.0):
+    """Odometry sequence s of the serving mix: a resident n_map-point outdoor
+    map (seed 5000+s) and n_frames CT sweeps of n_scan points taken 0.1 s
+    apart while driving at 10 m/s with a 30 deg/s yaw rate."""
+    rng = np.random.default_rng(np.random.PCG64(5000 + s))
+    S = outdoor_scene(rng, extent=extent)
+    Mw = S.sample(n_map, rng, 0.02)
+    p0 = free_space_position(S, rng, (-30.0, -3.0), (30.0, 3.0), 1.8)
+    X = pose(rodrigues([0, 0, rng.uniform(-0.2, 0.2)]), p0)
+    v_gt = np.array([10.0, 0.0, 0.0, 0.0, 0.0, math.radians(30.0)])
+    Xf = anchor_frame(rng, p0, offset=2.0)
+    frames = []
+    for f in range(n_frames):
+        P, T = lidar_scan(S, rng, X, 128, (-22.5, 22.5), n_columns=512, range_sigma=0.02, max_range=200.0,
+                          velocity=v_gt, scan_time=0.1, n_points=n_scan)
+        X0 = perturb(X, rng, 1.0, 0.3)
+        v0 = v_gt * np.array([0.9, 1, 1, 1, 1, 0.9]) + np.array([0.0, 0.3, 0.0, 0.0, 0.0, 0.0])
+        frames.append(dict(scan=P, stamps=T, X_gt=X.copy(), X0=X0, v0=v0, v_gt=v_gt))
+        # the next sweep starts where this one ended
+        X = X @ pose(rodrigues(v_gt[3:] * 0.1), v_gt[:3] * 0.1)
+    return dict(map=to_frame(Mw, Xf), Xf=Xf, frames=frames)
+
+
+def config5_loop_pair(j):
+    """Loop-closure request j of the serving mix: config-1 generator with seed
+    5100+j, scan size log-uniform in [8k, 200k], map 3x the scan."""
+    rng = np.random.default_rng(np.random.PCG64(5100 + j))
+    n = int(math.exp(rng.uniform(math.log(8000), math.log(200000))))
+    return config1(seed=5100 + j, n_map=3 * n, n_scan=n, rot_deg=5.0, trans_m=1.0, uniform_init=True)
+
+
+def poisson_arrivals(rate_hz, duration_s, rng):
+    """Arrival times of a Poisson process on [0, duration_s)."""
+    t, out = 0.0, []
+    while True:
+        t += rng.exponential(1.0 / rate_hz)
+        if t >= duration_s:
+            return out
+        out.append(t)
