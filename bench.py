@@ -91,6 +91,20 @@ def _gen_seq(s):
     return q
 
 
+def _gen_pair3(i_n):
+    """Config 3 pair i: config-1 generator, seed 3000+i, scan n, map 3n."""
+    from kd_slam2_amd import TreeGenerator3D, synth
+
+    i, n = i_n
+    w = synth.config3_pair(i, n)
+    g = TreeGenerator3D(leaf_max_points=16)
+    w["fixed"] = g.makeTree(w["map"])
+    w["moving"] = g.makeTree(w["scan"])
+    w["n_pts"] = len(w["scan"])
+    del w["map"], w["scan"], w["stamps"]
+    return w
+
+
 def _gen_loop(j):
     from kd_slam2_amd import TreeGenerator3D, synth
 
@@ -199,6 +213,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--serve-seconds", type=float, default=3.0, help="length of the serving-mix arrival window")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--pairs3", type=int, default=256, help="loop-closure pairs of the batched config (0: skip)")
     ap.add_argument("--no-pmc", action="store_true")
     ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
@@ -216,9 +231,16 @@ def main():
         j_cfg2 = pool.apply_async(_gen_cfg2, (0,))
         j_seq = None if args.no_serve else pool.map_async(_gen_seq, range(4))
         j_loop = None if args.no_serve else pool.map_async(_gen_loop, range(n_loop_pool))
+        j_p3 = None
+        if args.pairs3:
+            from kd_slam2_amd import synth as _s
+
+            sizes = _s.config3_sizes(args.pairs3)
+            j_p3 = pool.map_async(_gen_pair3, list(enumerate(sizes)))
         w = j_cfg2.get()
         seqs = j_seq.get() if j_seq else []
         loops = j_loop.get() if j_loop else []
+        pairs3 = j_p3.get() if j_p3 else []
     fixed, moving = w["fixed"], w["moving"]
     n_pts = 131_072
     log(f"workloads ready in {time.time() - t0:.1f}s: fixed {fixed.num_nodes} nodes, moving {moving.num_leaves} "
@@ -309,6 +331,11 @@ def main():
     if not args.no_serve:
         serving = run_serving(ctx, seqs, loops, args.serve_seconds, DeviceTree, Server, default_params)
 
+    # ---- batched loop-closure verification (config 3) ----------------------
+    batch3 = None
+    if pairs3:
+        batch3 = run_batch3(ctx, pairs3, DeviceTree, batch_align, adapter_align, default_params, synth)
+
     # ---- CPU baseline -------------------------------------------------------
     cpu = None
     if not args.no_cpu:
@@ -376,10 +403,49 @@ def main():
         algorithmic_bytes_per_iter=b_iter,
         pmc=pmc or None,
         serving_mix=serving,
+        loop_closure_batch=batch3,
         roofline=roofline,
         cpu_baseline=cpu,
     )
     print(json.dumps(line), flush=True)
+
+
+def run_batch3(ctx, pairs3, DeviceTree, batch_align, adapter_align, default_params, synth):
+    """Config 3: 256 independent rigid pairs (scan 4k-120k points, map 3x) in
+    one batched call vs the plain per-call path; the launch shape is picked
+    by the autotuner on this batch first."""
+    from kd_slam2_amd import autotune
+
+    p = default_params()
+    p.max_error = 0.5
+    p.max_iterations = 10
+    trees = [(DeviceTree(ctx, q["fixed"]), DeviceTree(ctx, q["moving"])) for q in pairs3]
+    pairs = [dict(fixed=F, moving=M, ct=0, X0=q["X0"], Xf=q["Xf"], params=p) for q, (F, M) in zip(pairs3, trees)]
+    n_pts = sum(q["n_pts"] for q in pairs3)
+    best, _ = autotune.tune(ctx, pairs, batch_align, ct=0)
+    batch_align(ctx, pairs)  # warm (graph capture)
+    tb = []
+    for _ in range(3):
+        t = time.perf_counter()
+        out = batch_align(ctx, pairs)
+        tb.append(time.perf_counter() - t)
+    t_batch = float(np.median(tb))
+    t = time.perf_counter()
+    for q, (F, M) in zip(pairs3, trees):
+        adapter_align("ICPHIP3D", ctx, F, M, q["X0"], q["Xf"], p)
+    t_plain = time.perf_counter() - t
+    ok = sum(1 for o, q in zip(out, pairs3) if synth.pose_error(o["X"], q["X_gt"])[0] < 0.05)
+    ctx.set_launch_config()  # back to the defaults
+    for F, M in trees:
+        F.close()
+        M.close()
+    res = dict(pairs=len(pairs3), scan_points=int(n_pts), batch_ms=round(1e3 * t_batch, 3),
+               batch_pts_per_s=round(n_pts / t_batch, 1), plain_ms=round(1e3 * t_plain, 3),
+               plain_pts_per_s=round(n_pts / t_plain, 1), speedup=round(t_plain / t_batch, 2),
+               autotuned=best, converged_within_5cm=ok)
+    log(f"config 3: {len(pairs3)} pairs, batch {res['batch_ms']} ms vs plain {res['plain_ms']} ms "
+        f"({res['speedup']}x), tuned {best}")
+    return res
 
 
 def run_serving(ctx, seqs, loops, seconds, DeviceTree, Server, default_params):

@@ -120,6 +120,7 @@ KDH_HD Rodrigues rodrigues_f(V3f w) {
   sincos_shared(o.angle, &s, &c);
   float omc = 1.0f - c;
   M3f I = m3_identity<float>();
+#pragma unroll
   for (int i = 0; i < 9; ++i) o.R.m[i] = (I.m[i] + s * o.S.m[i]) + omc * S2.m[i];
   return o;
 }
@@ -129,16 +130,21 @@ KDH_HD Rodrigues rodrigues_f(V3f w) {
 KDH_HD M3f dRxp_f(const Rodrigues& rd, V3f u) {
   if (rd.small) return skew(-u);  // -skew(u)
   // -R * skew(u) * (n n^T + (R^T - I) S / angle)
+  const float nv[3] = {rd.n.x, rd.n.y, rd.n.z};
   M3f nn;
+#pragma unroll
   for (int r = 0; r < 3; ++r)
-    for (int c = 0; c < 3; ++c) nn.m[r * 3 + c] = rd.n[r] * rd.n[c];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) nn.m[r * 3 + c] = nv[r] * nv[c];
   M3f RtmI = transpose(rd.R);
   RtmI.m[0] -= 1.0f; RtmI.m[4] -= 1.0f; RtmI.m[8] -= 1.0f;
   M3f B = mul(RtmI, rd.S);
   float ia = 1.0f / rd.angle;
+#pragma unroll
   for (int i = 0; i < 9; ++i) B.m[i] = nn.m[i] + B.m[i] * ia;
   M3f A = mul(rd.R, skew(u));
   M3f out = mul(A, B);
+#pragma unroll
   for (int i = 0; i < 9; ++i) out.m[i] = -out.m[i];
   return out;
 }
