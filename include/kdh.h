@@ -71,8 +71,13 @@ typedef struct kdh_params {
   int32_t use_term_crit;       /* 0: run max_iterations */
   float term_d_pert;           /* 1e-3 */
   float term_d_score;          /* 1e-3 */
-  int32_t recenter;            /* 1: re-centred Jacobian (FP32 terms about the
-                                  scan centroid), 0: reference formula */
+  int32_t recenter;            /* 1: residual and Jacobian evaluated in frame-
+                                  local FP32 coordinates about an anchor at the
+                                  scan centroid, mapped back in FP64 (exact in
+                                  real arithmetic; no cancellation far from the
+                                  origin); 0: the reference's world-coordinate
+                                  formulas.  Correspondences, gates and inlier
+                                  decisions are the reference's in both. */
 } kdh_params;
 
 /* ICPStats_ (icp/icp_stats_.h:10-20). */
@@ -137,6 +142,18 @@ int kdh_cticp_build(kdh_ctx* ctx, const kdh_tree* fixed, const kdh_tree* moving,
                     const float Xm[12], const float Xf[12], const float Xmf[12],
                     const float v[6], const kdh_params* params, int stats_mode,
                     double H[144], double b[12], kdh_stats* stats);
+
+/* The form of kdh_icp_build (ct = 0) or kdh_cticp_build (ct = 1) at an FP64
+ * state instead of an FP32 cache: the cache is rounded once from (X, Xf, v),
+ * as kdh_align does every iteration, and re-centring (params->recenter)
+ * anchors at the FP64 state.  X, Xf: 3x4 row-major; v: 6 doubles (CT only,
+ * may be NULL for ct = 0).  No reference counterpart: it is the form one
+ * iteration of the device loop builds, exposed for callers that keep their
+ * pose in FP64 and for precision checks far from the origin. */
+int kdh_build_at(kdh_ctx* ctx, const kdh_tree* fixed, const kdh_tree* moving, int ct,
+                 const double X[12], const double Xf[12], const double v[6],
+                 const kdh_params* params, int stats_mode,
+                 double* H, double* b, kdh_stats* stats);
 
 /* Parity hook: the fixed-tree leaf index each moving leaf descends to
  * (-1 = num_bad) and the number of internal nodes visited, at the given

@@ -285,6 +285,23 @@ class _AlignerBase:
         self.stats["pert_pose_norm"] = 0.0
         self.stats["pert_vel_norm"] = 0.0
 
+    def buildQuadraticFormAt(self, stats_mode=False):
+        """buildQuadraticForm at the FP64 state (kdh_build_at): the cache is
+        rounded once from (X, Xf, v), as the device loop forms it."""
+        X = np.ascontiguousarray(self.X[:3, :4].reshape(12))
+        Xf = np.ascontiguousarray(self.Xf[:3, :4].reshape(12))
+        v = np.ascontiguousarray(self.v if self.CT else np.zeros(6), dtype=np.float64)
+        H = np.zeros(self.DIM * self.DIM)
+        b = np.zeros(self.DIM)
+        st = L.Stats()
+        rc = L.lib().kdh_build_at(self.ctx.h, self._fixed.h, self._moving.h, self.CT, L.f64p(X), L.f64p(Xf),
+                                  L.f64p(v), C.byref(self.params), int(stats_mode), L.f64p(H), L.f64p(b),
+                                  C.byref(st))
+        self.ctx.check(rc, "kdh_build_at")
+        self.H = H.reshape(self.DIM, self.DIM)
+        self.b = b
+        self.stats = st.as_dict()
+
     def oneRound(self, stats_mode=False):
         """ICP_::oneRound / CTICP_::oneRound with an FP64 solve on the host."""
         from .synth import expmap

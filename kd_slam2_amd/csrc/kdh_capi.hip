@@ -259,6 +259,7 @@ static TermParams term_params(const kdh_params& p, int stats_mode) {
   t.max_error = p.max_error;
   t.min_normal_cos = p.min_normal_cos;
   t.stats_mode = stats_mode ? 1 : 0;
+  t.recenter = p.recenter ? 1 : 0;
   return t;
 }
 
@@ -286,7 +287,7 @@ void kdh_cache_from_state(PairState& ps) {
   ps.cache.Xf = iso_cast<float>(Xf);
   ps.cache.Xmf = iso_cast<float>(compose(inverse(Xf), Xc));
   for (int i = 0; i < 6; ++i) ps.cache.v[i] = (float)ps.v[i];
-  ps.cache.c = ps.sp.recenter ? apply_iso(ps.cache.Xm, ps.moving_root_mean) : v3(0.0f, 0.0f, 0.0f);
+  set_anchor(ps.cache, ps.c64, make_anchor(Xc, Xf, ps.moving_root_mean, ps.sp.recenter));
 }
 
 int kdh_fill_pair(kdh_ctx* ctx, PairState& ps, const kdh_tree* fixed, const kdh_tree* moving, int ct,
@@ -339,12 +340,20 @@ int kdh_launch(kdh_ctx* ctx, PairState* d_pairs, const int* d_tile_pair, int n_t
 
 // ---- per-call build ---------------------------------------------------------------
 
-static int build_common(kdh_ctx* ctx, const kdh_tree* fixed, const kdh_tree* moving, int ct, const float Xm[12],
-                        const float Xf[12], const float Xmf[12], const float v[6], const kdh_params* params,
-                        int stats_mode, double* H, double* b, kdh_stats* stats, int32_t* leaf_out,
-                        int32_t* depth_out) {
+// The inputs of one build: the reference's FP32 cache (Xm, Xf, Xmf, v), or an
+// FP64 state (X, Xf, v) the cache is rounded from.
+struct BuildInput {
+  const float *Xm = nullptr, *Xf = nullptr, *Xmf = nullptr, *v = nullptr;
+  const double *X64 = nullptr, *Xf64 = nullptr, *v64 = nullptr;
+};
+
+static int build_common(kdh_ctx* ctx, const kdh_tree* fixed, const kdh_tree* moving, int ct, const BuildInput& in,
+                        const kdh_params* params, int stats_mode, double* H, double* b, kdh_stats* stats,
+                        int32_t* leaf_out, int32_t* depth_out) {
   if (!ctx) return KDH_ERR_INVALID;
-  if (!Xm || !Xf || !Xmf || !params || (ct && !v)) return kdh_fail(ctx, KDH_ERR_INVALID, "null argument");
+  const bool at_state = in.X64 != nullptr;
+  if (!params || (at_state ? (!in.Xf64 || (ct && !in.v64)) : (!in.Xm || !in.Xf || !in.Xmf || (ct && !in.v))))
+    return kdh_fail(ctx, KDH_ERR_INVALID, "null argument");
   std::lock_guard<std::mutex> lk(ctx->mu);
   KDH_HIP_TRY(ctx, hipSetDevice(ctx->device));
   int rc = kdh_ensure_pairs(ctx, 1);
@@ -352,11 +361,23 @@ static int build_common(kdh_ctx* ctx, const kdh_tree* fixed, const kdh_tree* mov
   PairState& ps = ctx->h_pairs[0];
   rc = kdh_fill_pair(ctx, ps, fixed, moving, ct, *params, stats_mode, 1);
   if (rc) return rc;
-  ps.cache.Xm = iso_from_34<float>(Xm);
-  ps.cache.Xf = iso_from_34<float>(Xf);
-  ps.cache.Xmf = iso_from_34<float>(Xmf);
-  for (int i = 0; i < 6; ++i) ps.cache.v[i] = ct ? v[i] : 0.0f;
-  ps.cache.c = params->recenter ? apply_iso(ps.cache.Xm, moving->root_mean) : v3(0.0f, 0.0f, 0.0f);
+  if (at_state) {
+    for (int i = 0; i < 12; ++i) {
+      ps.X[i] = in.X64[i];
+      ps.Xf[i] = in.Xf64[i];
+    }
+    for (int i = 0; i < 6; ++i) ps.v[i] = ct ? in.v64[i] : 0.0;
+    kdh_cache_from_state(ps);
+  } else {
+    ps.cache.Xm = iso_from_34<float>(in.Xm);
+    ps.cache.Xf = iso_from_34<float>(in.Xf);
+    ps.cache.Xmf = iso_from_34<float>(in.Xmf);
+    for (int i = 0; i < 6; ++i) ps.cache.v[i] = ct ? in.v[i] : 0.0f;
+    // The caller's FP32 cache is the state here.
+    set_anchor(ps.cache, ps.c64,
+               make_anchor(iso_cast<double>(ps.cache.Xm), iso_cast<double>(ps.cache.Xf), moving->root_mean,
+                           params->recenter));
+  }
   const int nt = ps.n_tiles;
   rc = kdh_ensure_partials(ctx, (size_t)std::max(nt, 1) * slots(ct));
   if (rc) return rc;
@@ -399,14 +420,32 @@ static int build_common(kdh_ctx* ctx, const kdh_tree* fixed, const kdh_tree* mov
 int kdh_icp_build(kdh_ctx* ctx, const kdh_tree* fixed, const kdh_tree* moving, const float Xm[12], const float Xf[12],
                   const float Xmf[12], const kdh_params* params, int stats_mode, double H[36], double b[6],
                   kdh_stats* stats) {
-  return build_common(ctx, fixed, moving, 0, Xm, Xf, Xmf, nullptr, params, stats_mode, H, b, stats, nullptr,
-                      nullptr);
+  BuildInput in;
+  in.Xm = Xm;
+  in.Xf = Xf;
+  in.Xmf = Xmf;
+  return build_common(ctx, fixed, moving, 0, in, params, stats_mode, H, b, stats, nullptr, nullptr);
 }
 
 int kdh_cticp_build(kdh_ctx* ctx, const kdh_tree* fixed, const kdh_tree* moving, const float Xm[12],
                     const float Xf[12], const float Xmf[12], const float v[6], const kdh_params* params,
                     int stats_mode, double H[144], double b[12], kdh_stats* stats) {
-  return build_common(ctx, fixed, moving, 1, Xm, Xf, Xmf, v, params, stats_mode, H, b, stats, nullptr, nullptr);
+  BuildInput in;
+  in.Xm = Xm;
+  in.Xf = Xf;
+  in.Xmf = Xmf;
+  in.v = v;
+  return build_common(ctx, fixed, moving, 1, in, params, stats_mode, H, b, stats, nullptr, nullptr);
+}
+
+int kdh_build_at(kdh_ctx* ctx, const kdh_tree* fixed, const kdh_tree* moving, int ct, const double X[12],
+                 const double Xf[12], const double v[6], const kdh_params* params, int stats_mode, double* H,
+                 double* b, kdh_stats* stats) {
+  BuildInput in;
+  in.X64 = X;
+  in.Xf64 = Xf;
+  in.v64 = v;
+  return build_common(ctx, fixed, moving, ct ? 1 : 0, in, params, stats_mode, H, b, stats, nullptr, nullptr);
 }
 
 int kdh_correspondences(kdh_ctx* ctx, const kdh_tree* fixed, const kdh_tree* moving, int ct, const float Xm[12],
@@ -414,8 +453,12 @@ int kdh_correspondences(kdh_ctx* ctx, const kdh_tree* fixed, const kdh_tree* mov
                         int32_t* leaf_out, int32_t* depth_out) {
   if (!leaf_out) return kdh_fail(ctx, KDH_ERR_INVALID, "leaf_out is null");
   float zero[6] = {0, 0, 0, 0, 0, 0};
-  return build_common(ctx, fixed, moving, ct, Xm, Xf, Xmf, v ? v : zero, params, 0, nullptr, nullptr, nullptr,
-                      leaf_out, depth_out);
+  BuildInput in;
+  in.Xm = Xm;
+  in.Xf = Xf;
+  in.Xmf = Xmf;
+  in.v = v ? v : zero;
+  return build_common(ctx, fixed, moving, ct, in, params, 0, nullptr, nullptr, nullptr, leaf_out, depth_out);
 }
 
 // ---- whole alignment on the device ---------------------------------------------------

@@ -67,6 +67,7 @@ struct PairState {
   double prior_ZXi[12]; // (X_ref * Z)^-1
   double prior_info[36];
   V3f moving_root_mean; // re-centring anchor in the moving frame
+  double c64[3];        // the anchor in world coordinates, FP64 (0 when not re-centred)
   // state
   double X[12];         // moving pose
   double v[6];          // moving velocities (CT)
@@ -79,5 +80,34 @@ struct PairState {
   double H[144];        // last H (n x n row-major, n = 6 or 12)
   double b[12];
 };
+
+// Re-centring anchor of a state (the TermCache fields mroot and cf, and the
+// FP64 anchor the solver maps the form back with): c = Xm * mroot in FP64,
+// cf = Xf^-1 * c rounded once.  All zero when re-centring is off.
+struct Anchor {
+  V3f mroot, cf;
+  V3d c;
+};
+KDH_HD Anchor make_anchor(const IsoD& Xm, const IsoD& Xf, V3f mroot, int on) {
+  Anchor a;
+  if (!on) {
+    a.mroot = a.cf = v3(0.0f, 0.0f, 0.0f);
+    a.c = v3(0.0, 0.0, 0.0);
+    return a;
+  }
+  a.c = apply_iso(Xm, v3((double)mroot.x, (double)mroot.y, (double)mroot.z));
+  const V3d cf = apply_iso(inverse(Xf), a.c);
+  a.mroot = mroot;
+  a.cf = v3((float)cf.x, (float)cf.y, (float)cf.z);
+  return a;
+}
+// Writes the anchor into a cache and the state's FP64 copy.
+KDH_HD void set_anchor(TermCache& C, double* c64, const Anchor& a) {
+  C.mroot = a.mroot;
+  C.cf = a.cf;
+  c64[0] = a.c.x;
+  c64[1] = a.c.y;
+  c64[2] = a.c.z;
+}
 
 }  // namespace kdh

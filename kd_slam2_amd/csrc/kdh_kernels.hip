@@ -77,9 +77,17 @@ struct GlobalFetch {
   }
 };
 
+// The LDS copy is addressed through an explicit LDS (address space 3)
+// pointer: with a generic one the backend must select between the two
+// apertures per load.
+#if defined(__HIP_DEVICE_COMPILE__)
+typedef __attribute__((address_space(3))) const float4 lds_f4;
+#else
+typedef const float4 lds_f4;
+#endif
 struct LdsFetch {
   const float4* __restrict__ nodes;
-  const float4* lds;
+  lds_f4* lds;
   int n_lds;
   __device__ __forceinline__ void operator()(int idx, float4& a, float4& b) const {
     if (idx < n_lds) {
@@ -230,7 +238,7 @@ __device__ void finish_pair(PairState& ps, const float* __restrict__ partials, c
 
   // 2. Assemble the re-centred H_c, b_c, then map back: G = H_c M, H = M^T G,
   //    b = M^T b_c with M = [[I, -[c]x], [0, I]] (see uncenter_d).
-  const V3d cc = v3((double)ps.cache.c.x, (double)ps.cache.c.y, (double)ps.cache.c.z);
+  const V3d cc = v3(ps.c64[0], ps.c64[1], ps.c64[2]);
   const M3d Cx = skew(cc);
   for (int e = tid; e < D * D; e += kThreads) {
     const int r = e / D, c = e % D;
@@ -393,8 +401,13 @@ __device__ void finish_pair(PairState& ps, const float* __restrict__ partials, c
   nc.Xf = iso_cast<float>(Xf);
   nc.Xmf = iso_cast<float>(compose(inverse(Xf), Xc));
   for (int i = 0; i < 6; ++i) nc.v[i] = (float)vs[i];
-  nc.c = sp.recenter ? apply_iso(nc.Xm, root_mean) : v3(0.0f, 0.0f, 0.0f);
+  const Anchor an = make_anchor(Xc, Xf, root_mean, sp.recenter);
+  nc.mroot = an.mroot;
+  nc.cf = an.cf;
   ps.cache = nc;
+  ps.c64[0] = an.c.x;
+  ps.c64[1] = an.c.y;
+  ps.c64[2] = an.c.z;
 }
 
 // ---- the tile kernel --------------------------------------------------------
@@ -466,7 +479,7 @@ __global__ __launch_bounds__(kThreads) void tiles_kernel(const PairState* __rest
       const TermParams TP = ps.tp;
       int leaf, depth = 0;
       if (LDS_NODES > 0) {
-        LdsFetch f{fixed.nodes, s_nodes, n_lds};
+        LdsFetch f{fixed.nodes, (lds_f4*)s_nodes, n_lds};
         leaf = CT ? ct_term(f, C, TP, L, R, &depth) : icp_term(f, C, TP, L, R, &depth);
       } else {
         GlobalFetch f{fixed.nodes};

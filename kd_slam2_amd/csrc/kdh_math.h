@@ -135,6 +135,13 @@ template <typename T> KDH_HD V3<T> apply_rot(const Iso<T>& X, V3<T> p) {
             ((T(0) + X.R.m[3] * p.x) + X.R.m[4] * p.y) + X.R.m[5] * p.z,
             ((T(0) + X.R.m[6] * p.x) + X.R.m[7] * p.y) + X.R.m[8] * p.z);
 }
+// The same loop starting from a given vector: apply_from(X, p, X.t) is
+// apply_iso(X, p) and apply_from(X, p, 0) is apply_rot(X, p), bit for bit.
+template <typename T> KDH_HD V3<T> apply_from(const Iso<T>& X, V3<T> p, V3<T> t0) {
+  return v3(((t0.x + X.R.m[0] * p.x) + X.R.m[1] * p.y) + X.R.m[2] * p.z,
+            ((t0.y + X.R.m[3] * p.x) + X.R.m[4] * p.y) + X.R.m[5] * p.z,
+            ((t0.z + X.R.m[6] * p.x) + X.R.m[7] * p.y) + X.R.m[8] * p.z);
+}
 template <typename T> KDH_HD Iso<T> compose(const Iso<T>& A, const Iso<T>& B) {
   Iso<T> C;
   C.R = mul(A.R, B.R);

@@ -47,16 +47,29 @@ struct TermParams {
   float max_error;
   float min_normal_cos;
   int stats_mode;
+  int recenter;  // 0: the reference's e and J; 1: e and J about the anchor (below)
 };
 
 // Per-iteration inputs of the term (QuadraticTermCache, icp_.h:97-102, plus
-// the CT velocity and the re-centring point).
+// the CT velocity and the re-centring anchor).
+//
+// Re-centring.  The reference evaluates e = (p_m - p_f).n and
+// J_rot = p_m x n_f + p_f x n_m on world points; far from the origin both
+// cancel catastrophically in FP32 (a 10 km coordinate carries 1 mm of
+// rounding).  With the anchor c = Xm * mroot (mroot = moving tree root mean)
+// the same quantities come from frame-local coordinates only:
+//   e     = (p_mif - pf) . (Rmf d_m + n_f_local)        (Xf^-1 applied to both)
+//   J_rot = Rm (m - mroot) x n_f + Rf (pf - cf) x n_m   (= p_m x n_f + p_f x n_m - c x n)
+// with cf = Xf^-1 c.  The FP64 solver maps the form back with the FP64 c.
+// The discrete decisions (descent, gates, inlier test) always use the
+// reference's own FP32 e.
 struct TermCache {
   IsoF Xm;
   IsoF Xf;
   IsoF Xmf;
   float v[6];   // moving velocities [t; w] (CT only)
-  V3f c;        // re-centring point for the Jacobian
+  V3f mroot;    // anchor in the moving tree frame
+  V3f cf;       // anchor in the fixed tree frame
 };
 
 // Greedy descent (icp_impl_.h:116-121, node_.h:54-63).  Node record layout:
@@ -214,6 +227,20 @@ KDH_HD float rec_slot(const TermRec& r, int s) {
   return 0.0f;
 }
 
+// Robust weight and outcome (icp_impl_.h:138-151): the inlier test on the
+// reference's FP32 chi2 = e_ref^2; the weight and the chi2 statistics from the
+// residual R.e that enters b (equal to e_ref unless re-centred).
+KDH_HD void classify(TermRec& R, float e_ref, const TermParams& P) {
+  R.chi2 = R.e * R.e;
+  if (e_ref * e_ref > P.kernel_threshold) {
+    R.outcome = kOutlier;
+    R.gamma = P.stats_mode ? 0.0f : sqrtf(P.kernel_threshold / R.chi2);
+  } else {
+    R.outcome = kInlier;
+    R.gamma = 1.0f;
+  }
+}
+
 // Rigid ICP term (icp_impl_.h:103-152).  Returns the fixed leaf index reached
 // by the descent (-1 for num_bad).
 template <typename NodeFetch>
@@ -244,19 +271,17 @@ KDH_HD int icp_term(const NodeFetch& fetch, const TermCache& C, const TermParams
     rec_reject(R, kErrFn);
     return leaf;
   }
-  const V3f jr = cross(p_m - C.c, n_f) + cross(p_f - C.c, n_m);
+  // Reference mode has mroot = cf = 0 and starts from the full translations,
+  // which makes jr exactly p_m x n_f + p_f x n_m.
+  const V3f z = v3(0.0f, 0.0f, 0.0f);
+  const V3f jr = cross(apply_from(C.Xm, L.mean - C.mroot, P.recenter ? z : C.Xm.t), n_f) +
+                 cross(apply_from(C.Xf, pf - C.cf, P.recenter ? z : C.Xf.t), n_m);
+  const float e_loc = dot(p_mif - pf, apply_rot(C.Xmf, L.dir) + nf);
+  R.e = P.recenter ? e_loc : e;
   R.J[0] = n.x; R.J[1] = n.y; R.J[2] = n.z;
   R.J[3] = jr.x; R.J[4] = jr.y; R.J[5] = jr.z;
   for (int i = 0; i < 6; ++i) R.Jv[i] = 0.0f;
-  R.e = e;
-  R.chi2 = e * e;
-  if (R.chi2 > P.kernel_threshold) {
-    R.outcome = kOutlier;
-    R.gamma = P.stats_mode ? 0.0f : sqrtf(P.kernel_threshold / R.chi2);
-  } else {
-    R.outcome = kInlier;
-    R.gamma = 1.0f;
-  }
+  classify(R, e, P);
   return leaf;
 }
 
@@ -296,28 +321,26 @@ KDH_HD int ct_term(const NodeFetch& fetch, const TermCache& C, const TermParams&
     rec_reject(R, kErrFn);
     return leaf;
   }
-  const V3f jr = cross(p_m - C.c, n_f) + cross(p_f - C.c, n_m);
+  // As in icp_term; re-centred, Ra^T dp = Rmf^T (p_mif - pf) as well.
+  const M3f& Ra = C.Xm.R;
+  const V3f z = v3(0.0f, 0.0f, 0.0f);
+  const V3f jr = cross(apply_from(C.Xm, wmean - C.mroot, P.recenter ? z : C.Xm.t), n_f) +
+                 cross(apply_from(C.Xf, pf - C.cf, P.recenter ? z : C.Xf.t), n_m);
+  const V3f dl = p_mif - pf;
+  const float e_loc = dot(dl, apply_rot(C.Xmf, wdir) + nf);
+  R.e = P.recenter ? e_loc : e;
+  const V3f dpR = P.recenter ? mul_t(C.Xmf.R, dl) : mul_t(Ra, dp);
   R.J[0] = n.x; R.J[1] = n.y; R.J[2] = n.z;
   R.J[3] = jr.x; R.J[4] = jr.y; R.J[5] = jr.z;
   // errorAndJacobianVelocityA (d3/cticp_impl_.h:11-39) at the unwarped leaf:
   //   J_t = dt n^T Ra ; J_w = dt (n^T Ra dR_p + dp^T Ra dR_n)
-  const M3f& Ra = C.Xm.R;
   const V3f nR = mul_t(Ra, n);
-  const V3f dpR = mul_t(Ra, dp);
   const M3f dRp = dRxp_f(rd, L.mean);
   const M3f dRn = dRxp_f(rd, L.dir);
   const V3f jw = mul_t(dRp, nR) + mul_t(dRn, dpR);
   R.Jv[0] = nR.x * dt; R.Jv[1] = nR.y * dt; R.Jv[2] = nR.z * dt;
   R.Jv[3] = jw.x * dt; R.Jv[4] = jw.y * dt; R.Jv[5] = jw.z * dt;
-  R.e = e;
-  R.chi2 = e * e;
-  if (R.chi2 > P.kernel_threshold) {
-    R.outcome = kOutlier;
-    R.gamma = P.stats_mode ? 0.0f : sqrtf(P.kernel_threshold / R.chi2);
-  } else {
-    R.outcome = kInlier;
-    R.gamma = 1.0f;
-  }
+  classify(R, e, P);
   return leaf;
 }
 

@@ -425,3 +425,51 @@ def poisson_arrivals(rate_hz, duration_s, rng):
         if t >= duration_s:
             return out
         out.append(t)
+
+
+# --------------------------------------------------------------------------
+# Config 4: precision stress (large scene far from the origin)
+
+CONFIG4_OFFSET = np.array([10_000.0, 0.0, 0.0])
+
+
+def config4_scene(seed=4, extent=220.0):
+    rng = np.random.default_rng(np.random.PCG64(seed))
+    S = outdoor_scene(rng, extent=extent, n_buildings=120, n_poles=300)
+    p = free_space_position(S, rng, (-20.0, -3.0), (20.0, 3.0), 1.8)
+    X_local = pose(rodrigues([0, 0, rng.uniform(-0.2, 0.2)]), p)
+    Xf_local = anchor_frame(rng, p, offset=2.0)
+    X0_local = perturb(X_local, rng, 0.5, 0.2)
+    return S, X_local, Xf_local, X0_local
+
+
+def config4_map_block(block, n_blocks=8, n_map=8_000_000, seed=4):
+    """Block of the 8M-point map (sigma 0.02 m), in the anchor frame."""
+    S, _, Xf_local, _ = config4_scene(seed)
+    rng = np.random.default_rng(np.random.PCG64(40_000 + 100 * seed + block))
+    return to_frame(S.sample(n_map // n_blocks, rng, 0.02), Xf_local)
+
+
+def config4_scan_block(block, n_blocks=8, n_columns=22000, seed=4):
+    """Columns [block/n_blocks of a 128-beam +-22.5 deg sweep) of the static
+    sensor, ranges up to 200 m, range sigma 0.02 m; sensor-frame points."""
+    S, X_local, _, _ = config4_scene(seed)
+    rng = np.random.default_rng(np.random.PCG64(41_000 + 100 * seed + block))
+    c0 = block * n_columns // n_blocks
+    c1 = (block + 1) * n_columns // n_blocks
+    elev = np.radians(np.linspace(-22.5, 22.5, 128))
+    az = -math.pi + 2 * math.pi * (np.arange(c0, c1) + 0.5) / n_columns
+    ce, se = np.cos(elev), np.sin(elev)
+    dirs = np.stack([ce[None, :] * np.cos(az)[:, None], ce[None, :] * np.sin(az)[:, None],
+                     np.broadcast_to(se[None, :], (len(az), len(elev)))], axis=2).reshape(-1, 3)
+    r = S.cast(X_local[:3, 3], dirs @ X_local[:3, :3].T, 200.0)
+    ok = np.isfinite(r) & (r > 0.3)
+    rr = r[ok] + rng.normal(scale=0.02, size=int(ok.sum()))
+    return (dirs[ok] * rr[:, None]).astype(np.float32)
+
+
+def config4_poses(seed=4):
+    """World poses with the 10 km offset: ground truth, initial guess, map anchor."""
+    _, X_local, Xf_local, X0_local = config4_scene(seed)
+    O = pose(None, CONFIG4_OFFSET)
+    return O @ X_local, O @ X0_local, O @ Xf_local

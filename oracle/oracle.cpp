@@ -274,6 +274,11 @@ struct TermOut {
 struct Cache {
   Iso3<float> Xm, Xf, Xmf;
   float v[6];
+  // State the FP64 twin evaluates at: the FP32 cache widened (s64 false) or
+  // the FP64 state the cache was rounded from (s64 true).
+  bool s64 = false;
+  Iso3<double> Xm64, Xf64;
+  double v64[6];
 };
 
 // Descent (icp_impl_.h:116-121; whichSide node_.h:54-56)
@@ -394,13 +399,15 @@ static void quadraticTerm(TermOut& o, const Node* fixed, const Node& leafnode, c
 
   // FP64 twin: same inputs and decisions, FP64 arithmetic for e, J, gamma.
   {
-    Iso3<double> Xm = cast<double>(C.Xm), Xf = cast<double>(C.Xf);
+    Iso3<double> Xm = C.s64 ? C.Xm64 : cast<double>(C.Xm), Xf = C.s64 ? C.Xf64 : cast<double>(C.Xf);
+    double v64[6];
+    for (int i = 0; i < 6; ++i) v64[i] = C.s64 ? C.v64[i] : (double)C.v[i];
     Vec3<double> wm = mk((double)wmean[0], (double)wmean[1], (double)wmean[2]);
     Vec3<double> wd = mk((double)wdir[0], (double)wdir[1], (double)wdir[2]);
     if (CT) {
       double dtd = (double)dt;
       double vv[6];
-      for (int i = 0; i < 6; ++i) vv[i] = (double)C.v[i] * dtd;
+      for (int i = 0; i < 6; ++i) vv[i] = v64[i] * dtd;
       Mat3<double> R = rodrigues(mk(vv[3], vv[4], vv[5]));
       Vec3<double> md = mk((double)mean[0], (double)mean[1], (double)mean[2]);
       Vec3<double> dd = mk((double)dir[0], (double)dir[1], (double)dir[2]);
@@ -418,7 +425,7 @@ static void quadraticTerm(TermOut& o, const Node* fixed, const Node& leafnode, c
     if (CT) {
       double dtd = (double)dt;
       double ma[6];
-      for (int i = 0; i < 6; ++i) ma[i] = (double)C.v[i] * dtd;
+      for (int i = 0; i < 6; ++i) ma[i] = v64[i] * dtd;
       Vec3<double> pa_m, na_m;
       Mat3<double> dRp, dRn;
       Vec3<double> md = mk((double)mean[0], (double)mean[1], (double)mean[2]);
@@ -786,6 +793,45 @@ int orc_build(int ct, const void* fixed_nodes, int64_t n_fixed, const void* movi
   return 0;
 }
 
+// FP32 cache rounded once from an FP64 state (updateCache, icp_impl_.h:16-21,
+// formed in FP64), remembering the state for the FP64 twin.
+static Cache cache_from_state(const Iso3<double>& X, const Iso3<double>& Xf, const double* v, bool ct) {
+  Cache C;
+  C.Xm = cast<float>(X);
+  C.Xf = cast<float>(Xf);
+  C.Xmf = cast<float>(compose(inverse(Xf), X));
+  C.s64 = true;
+  C.Xm64 = X;
+  C.Xf64 = Xf;
+  for (int i = 0; i < 6; ++i) {
+    C.v64[i] = (ct && v) ? v[i] : 0.0;
+    C.v[i] = (float)C.v64[i];
+  }
+  return C;
+}
+
+// orc_build at an FP64 state: FP32 form at the rounded cache, FP64 twin at
+// the state itself (X, Xf 3x4 row-major, v 6 or NULL).
+int orc_build_at(int ct, const void* fixed_nodes, int64_t n_fixed, const void* moving_nodes, const uint64_t* leaves,
+                 int64_t n_leaves, const double* X, const double* Xf, const double* v, const orc_params* p,
+                 float* H, float* b, double* H64, double* b64, int32_t* corr) {
+  Cache C = cache_from_state(from34<double>(X), from34<double>(Xf), v, ct != 0);
+  Params P = to_params(p);
+  Form f;
+  const Node* fx = (const Node*)fixed_nodes;
+  const Node* mv = (const Node*)moving_nodes;
+  if (ct)
+    build_form<true>(fx, n_fixed, mv, leaves, n_leaves, C, P, false, p->num_threads, f, corr, nullptr);
+  else
+    build_form<false>(fx, n_fixed, mv, leaves, n_leaves, C, P, false, p->num_threads, f, corr, nullptr);
+  const int D = f.D;
+  if (H) memcpy(H, f.H, sizeof(float) * D * D);
+  if (b) memcpy(b, f.b, sizeof(float) * D);
+  if (H64) memcpy(H64, f.H64, sizeof(double) * D * D);
+  if (b64) memcpy(b64, f.b64, sizeof(double) * D);
+  return 0;
+}
+
 // updateCache (icp_impl_.h:16-21) in FP32.
 void orc_update_cache(const float* Xm, const float* Xf, float* Xmf) {
   Iso3<float> m = from34<float>(Xm), f = from34<float>(Xf);
@@ -881,6 +927,48 @@ int orc_compute(int ct, const void* fixed_nodes, int64_t n_fixed, const void* mo
     for (int i = 0; i < 6; ++i) v[i] = vel[i];
   if (n_log) *n_log = nl;
   if (result) *result = ret;
+  return 0;
+}
+
+// FP64 yardstick for whole alignments: the reference's FP32 discrete
+// decisions (descent, gates, inlier test) at an FP32 cache rounded once from
+// an FP64 state, with every accepted term re-evaluated in FP64 at that state,
+// summed in FP64, solved in FP64 and the pose updated in FP64.  Rigid or CT,
+// no prior.
+int orc_compute64(int ct, const void* fixed_nodes, int64_t n_fixed, const void* moving_nodes, const uint64_t* leaves,
+                  int64_t n_leaves, double* X, double* v, const double* Xf, const orc_params* p, int* n_iter) {
+  Params P = to_params(p);
+  Iso3<double> Xs = from34<double>(X);
+  const Iso3<double> Xfs = from34<double>(Xf);
+  double vel[6];
+  for (int i = 0; i < 6; ++i) vel[i] = (ct && v) ? v[i] : 0.0;
+  const int D = ct ? 12 : 6;
+  int it = 0;
+  for (; it < P.max_iterations; ++it) {
+    Cache C = cache_from_state(Xs, Xfs, vel, ct != 0);
+    Form f;
+    const Node* fx = (const Node*)fixed_nodes;
+    const Node* mv = (const Node*)moving_nodes;
+    if (ct)
+      build_form<true>(fx, n_fixed, mv, leaves, n_leaves, C, P, false, p->num_threads, f, nullptr, nullptr);
+    else
+      build_form<false>(fx, n_fixed, mv, leaves, n_leaves, C, P, false, p->num_threads, f, nullptr, nullptr);
+    if ((int)(f.stats[S_IN] + f.stats[S_OUT]) < P.min_good_measurements) break;
+    double A[144], rhs[12], dx[12];
+    for (int i = 0; i < D * D; ++i) A[i] = f.H64[i];
+    for (int i = 0; i < D; ++i) {
+      A[i * D + i] += (i < 6) ? (double)P.damping : (double)P.velocity_damping;
+      rhs[i] = -f.b64[i];
+    }
+    ldlt_solve<double>(D, A, rhs, dx);
+    Xs = compose(expmap<double>(dx), Xs);
+    if (ct)
+      for (int i = 0; i < 6; ++i) vel[i] += dx[6 + i];
+  }
+  to34(Xs, X);
+  if (ct && v)
+    for (int i = 0; i < 6; ++i) v[i] = vel[i];
+  if (n_iter) *n_iter = it;
   return 0;
 }
 

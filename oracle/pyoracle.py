@@ -64,6 +64,12 @@ def lib(fast=False):
         L.orc_compute.restype = C.c_int
         L.orc_compute.argtypes = [C.c_int, vp, C.c_int64, vp, vp, C.c_int64, f32, f32, f32, C.POINTER(OrcParams), f32,
                                   f32, C.c_int, C.POINTER(OrcStats), C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        L.orc_build_at.restype = C.c_int
+        L.orc_build_at.argtypes = [C.c_int, vp, C.c_int64, vp, vp, C.c_int64, f64, f64, f64, C.POINTER(OrcParams),
+                                   f32, f32, f64, f64, i32]
+        L.orc_compute64.restype = C.c_int
+        L.orc_compute64.argtypes = [C.c_int, vp, C.c_int64, vp, vp, C.c_int64, f64, f64, f64, C.POINTER(OrcParams),
+                                    C.POINTER(C.c_int)]
         L.orc_update_cache.argtypes = [f32, f32, f32]
         L.orc_ldlt_solve_f.argtypes = [C.c_int, f32, f32, f32]
         L.orc_ldlt_solve_d.argtypes = [C.c_int, f64, f64, f64]
@@ -161,6 +167,40 @@ def compute(fixed, moving, X0, Xf, params, ct=0, v0=None, prior_ZXi=None, prior_
     Xo = np.eye(4)
     Xo[:3, :4] = X.reshape(3, 4).astype(np.float64)
     return Xo, v.astype(np.float64), [log[i].as_dict() for i in range(nlog.value)], res.value
+
+
+def build_form_at(fixed, moving, X, Xf, params, ct=0, v=None, fast=False):
+    """build_form at an FP64 state: the FP32 form at the cache rounded from
+    (X, Xf, v), and the FP64 twin evaluated at the state itself."""
+    D = 12 if ct else 6
+    H = np.zeros(D * D, dtype=np.float32)
+    b = np.zeros(D, dtype=np.float32)
+    H64 = np.zeros(D * D)
+    b64 = np.zeros(D)
+    n = len(moving.leaves)
+    corr = np.zeros(max(n, 1), dtype=np.int32)
+    X34 = np.array(np.asarray(X, dtype=np.float64)[:3, :4].reshape(12))
+    Xf34 = np.array(np.asarray(Xf, dtype=np.float64)[:3, :4].reshape(12))
+    vv = np.array(np.zeros(6) if v is None else v, dtype=np.float64)
+    lib(fast).orc_build_at(ct, fixed.nodes.ctypes.data, len(fixed.nodes), moving.nodes.ctypes.data,
+                           moving.leaves.ctypes.data, n, p64(X34), p64(Xf34), p64(vv), C.byref(params), p32(H),
+                           p32(b), p64(H64), p64(b64), pi32(corr))
+    return dict(H=H.reshape(D, D), b=b, H64=H64.reshape(D, D), b64=b64, corr=corr[:n])
+
+
+def compute64(fixed, moving, X0, Xf, params, ct=0, v0=None, fast=False):
+    """FP64 yardstick of a whole alignment (FP32 decisions, FP64 terms, sums,
+    solve and state).  Returns (X 4x4, v, iterations)."""
+    X = np.array(np.asarray(X0, dtype=np.float64)[:3, :4].reshape(12))  # own copy: written in place
+    v = np.array(np.zeros(6) if v0 is None else v0, dtype=np.float64)
+    Xf34 = np.array(np.asarray(Xf, dtype=np.float64)[:3, :4].reshape(12))
+    nit = C.c_int()
+    lib(fast).orc_compute64(ct, fixed.nodes.ctypes.data, len(fixed.nodes), moving.nodes.ctypes.data,
+                            moving.leaves.ctypes.data, len(moving.leaves), p64(X), p64(v), p64(Xf34),
+                            C.byref(params), C.byref(nit))
+    Xo = np.eye(4)
+    Xo[:3, :4] = X.reshape(3, 4)
+    return Xo, v, nit.value
 
 
 def update_cache(Xm34, Xf34):

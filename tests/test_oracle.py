@@ -263,3 +263,49 @@ def test_velocity_jacobian_fp64_finite_differences():
         hh = 1e-6
         Jn = np.array([(err(va + np.eye(6)[k] * hh) - err(va - np.eye(6)[k] * hh)) / (2 * hh) for k in range(6)])
         assert np.abs(J - Jn).max() < 1e-3 * max(1.0, np.abs(Jn).max())
+
+
+def _small_pair():
+    from kd_slam2_amd import default_params
+
+    from .workloads import cfg1
+
+    w = cfg1(n_map=50_000, n_scan=4096)
+    p = default_params()
+    p.max_error = 0.5
+    p.max_iterations = 8
+    return w, p
+
+
+def test_fp64_state_twin_equals_cast_twin_at_fp32_exact_state():
+    """build_form_at evaluates the FP64 twin at the state; for a state that
+    FP32 holds exactly that is the twin of the rounded cache, bit for bit."""
+    from kd_slam2_amd import update_cache_f32
+
+    w, p = _small_pair()
+    X = w["X0"].astype(np.float32).astype(np.float64)
+    Xf = w["Xf"].astype(np.float32).astype(np.float64)
+    a = O.build_form_at(w["fixed"], w["moving"], X, Xf, O.from_kdh_params(p, 4))
+    Xm, Xfc, _ = update_cache_f32(X, Xf)
+    Xmf = np.linalg.inv(Xf) @ X
+    b = O.build_form(w["fixed"], w["moving"], Xm, Xfc, Xmf[:3, :4].astype(np.float32).reshape(12),
+                     O.from_kdh_params(p, 4))
+    assert np.array_equal(a["corr"], b["corr"])
+    assert np.array_equal(a["H64"], b["H64"]) and np.array_equal(a["b64"], b["b64"])
+
+
+def test_fp64_loop_agrees_with_fp32_loop_near_origin():
+    """Near the origin the FP64 yardstick loop and the reference's FP32 loop
+    take the same decisions and land within FP32 noise of each other, and
+    both move towards the generating pose."""
+    from kd_slam2_amd import synth
+
+    w, p = _small_pair()
+    X64, _, n = O.compute64(w["fixed"], w["moving"], w["X0"], w["Xf"], O.from_kdh_params(p, 4))
+    X32, _, _, res = O.compute(w["fixed"], w["moving"], w["X0"], w["Xf"], O.from_kdh_params(p, 4))
+    assert n == p.max_iterations and res == p.max_iterations
+    t, r = synth.pose_error(X64, X32)
+    assert t < 1e-4 and r < 1e-3
+    t, r = synth.pose_error(X64, w["X_gt"])
+    t0, r0 = synth.pose_error(w["X0"], w["X_gt"])
+    assert t < t0 / 5 and r < r0 / 5
