@@ -24,6 +24,9 @@ Beside it the line reports
   * the tile kernel's roofline (HIP events on the context stream) and its
     HBM traffic from two rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE)
     scaled by the same counters on a device copy of known size;
+  * the precision stress: a 2M-point scan against an 8M-point map with the
+    world frame 10 km away -- naive FP32, re-centred FP32 and the reference's
+    FP32 form against the oracle's FP64 twin and FP64 loop;
   * the CPU oracle (restated ICP_CPU_ / CTICP_CPU_) on 8 host threads.
 """
 import argparse
@@ -115,6 +118,15 @@ def _gen_loop(j):
     w["n_pts"] = len(w["scan"])
     del w["map"], w["scan"]
     return w
+
+
+def _gen_cfg4_block(kb):
+    """Precision stress: block b of the 2M-point scan (kind 0) or of the
+    8M-point map (kind 1)."""
+    from kd_slam2_amd import synth
+
+    kind, b = kb
+    return synth.config4_scan_block(b) if kind == 0 else synth.config4_map_block(b)
 
 
 # ---------------------------------------------------------------------------
@@ -215,6 +227,7 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--pairs3", type=int, default=256, help="loop-closure pairs of the batched config (0: skip)")
     ap.add_argument("--no-pmc", action="store_true")
+    ap.add_argument("--no-precision", action="store_true", help="skip the 10 km precision-stress workload")
     ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.pmc_child:
@@ -237,10 +250,23 @@ def main():
 
             sizes = _s.config3_sizes(args.pairs3)
             j_p3 = pool.map_async(_gen_pair3, list(enumerate(sizes)))
+        j_c4 = None if args.no_precision else pool.map_async(_gen_cfg4_block, [(k, b) for k in (0, 1)
+                                                                                for b in range(8)])
         w = j_cfg2.get()
+        c4_blocks = j_c4.get() if j_c4 else None
         seqs = j_seq.get() if j_seq else []
         loops = j_loop.get() if j_loop else []
         pairs3 = j_p3.get() if j_p3 else []
+    w4 = None
+    if c4_blocks is not None:
+        from kd_slam2_amd import TreeGenerator3D, synth as _s
+
+        scan4 = np.concatenate(c4_blocks[:8])
+        g4 = TreeGenerator3D(parallel_level=3)
+        X_gt4, X04, Xf4 = _s.config4_poses()
+        w4 = dict(n_pts=len(scan4), moving=g4.makeTree(scan4), fixed=g4.makeTree(np.concatenate(c4_blocks[8:])),
+                  X_gt=X_gt4, X0=X04, Xf=Xf4)
+        del scan4, c4_blocks
     fixed, moving = w["fixed"], w["moving"]
     n_pts = 131_072
     log(f"workloads ready in {time.time() - t0:.1f}s: fixed {fixed.num_nodes} nodes, moving {moving.num_leaves} "
@@ -336,6 +362,11 @@ def main():
     if pairs3:
         batch3 = run_batch3(ctx, pairs3, DeviceTree, batch_align, adapter_align, default_params, synth)
 
+    # ---- precision stress (config 4) ---------------------------------------
+    precision = None
+    if w4 is not None:
+        precision = run_precision(ctx, w4, DeviceTree, default_params, synth)
+
     # ---- CPU baseline -------------------------------------------------------
     cpu = None
     if not args.no_cpu:
@@ -404,10 +435,82 @@ def main():
         pmc=pmc or None,
         serving_mix=serving,
         loop_closure_batch=batch3,
+        precision_stress=precision,
         roofline=roofline,
         cpu_baseline=cpu,
     )
     print(json.dumps(line), flush=True)
+
+
+def run_precision(ctx, w4, DeviceTree, default_params, synth):
+    """Precision stress: rigid ICP of a 2M-point, 128-beam scan (ranges up to
+    200 m) against an 8M-point map, the world frame 10 km away.  At the
+    initial FP64 state: H/b relative error and Gauss-Newton step error of
+    naive FP32 (the reference's world-coordinate formulas), re-centred FP32
+    and the reference's FP32 CPU form, each against the oracle's FP64 twin.
+    After 10 iterations: pose error of each loop against the oracle's FP64
+    loop, and against the generating pose."""
+    from kd_slam2_amd import ICPHIP3D
+    from oracle import pyoracle as O
+
+    F, M = DeviceTree(ctx, w4["fixed"]), DeviceTree(ctx, w4["moving"])
+    p = default_params()
+    p.max_error = 0.5
+    p.max_iterations = 10
+    op = O.from_kdh_params(p, num_threads=8)
+
+    def step(H, b):
+        return np.linalg.solve(np.asarray(H, dtype=np.float64) + p.damping * np.eye(6), -np.asarray(b, np.float64))
+
+    def sdiff(dx, ref):
+        return [float(np.linalg.norm(dx[:3] - ref[:3])), float(np.degrees(np.linalg.norm(dx[3:] - ref[3:])))]
+
+    def rel(A, B):
+        return float(np.abs(np.asarray(A, np.float64) - B).max() / np.abs(B).max())
+
+    ref = O.build_form_at(w4["fixed"], w4["moving"], w4["X0"], w4["Xf"], op, fast=True)
+    dx64 = step(ref["H64"], ref["b64"])
+    forms, poses, lat = {}, {}, {}
+    for rc, name in ((0, "naive_fp32"), (1, "recentred_fp32")):
+        pr = default_params()
+        pr.max_error, pr.max_iterations, pr.recenter = 0.5, 10, rc
+
+        def mk():
+            al = ICPHIP3D(ctx)
+            al.params = pr
+            al.setFixed(F)
+            al.setMoving(M)
+            al.setX(w4["X0"])
+            al.setFixedX(w4["Xf"])
+            return al
+
+        al = mk()
+        al.buildQuadraticFormAt()
+        forms[name] = dict(H_rel=rel(al.H, ref["H64"]), b_rel=rel(al.b, ref["b64"]),
+                           step_err_m_deg=sdiff(step(al.H, al.b), dx64))
+        ts = []
+        for _ in range(4):
+            al = mk()
+            t = time.perf_counter()
+            al.computeOnDevice()
+            ts.append(time.perf_counter() - t)
+        lat[name] = 1e3 * float(np.median(ts[1:]))
+        poses[name] = al.getX()
+    forms["reference_fp32_cpu"] = dict(H_rel=rel(ref["H"], ref["H64"]), b_rel=rel(ref["b"], ref["b64"]),
+                                       step_err_m_deg=sdiff(step(ref["H"], ref["b"]), dx64))
+    X64, _, _ = O.compute64(w4["fixed"], w4["moving"], w4["X0"], w4["Xf"], op, fast=True)
+    X32, _, _, _ = O.compute(w4["fixed"], w4["moving"], w4["X0"], w4["Xf"], op, fast=True)
+    poses["reference_fp32_cpu"] = X32
+    out = dict(workload=f"rigid ICP, {w4['n_pts']}-pt 128-beam scan (ranges <= 200 m) vs 8M-pt map, world frame "
+                        f"offset (10 km, 0, 0), 10 iterations",
+               moving_leaves=int(w4["moving"].num_leaves), fixed_nodes=int(w4["fixed"].num_nodes),
+               form_at_initial_state=forms,
+               pose_vs_fp64_loop_m_deg={k: list(synth.pose_error(X, X64)) for k, X in poses.items()},
+               fp64_loop_vs_ground_truth_m_deg=list(synth.pose_error(X64, w4["X_gt"])),
+               device_loop_ms={k: round(v, 3) for k, v in lat.items()})
+    out["recentred_pts_per_s"] = round(w4["n_pts"] / (lat["recentred_fp32"] * 1e-3), 1)
+    log(f"precision: forms {forms}; poses vs FP64 {out['pose_vs_fp64_loop_m_deg']}; device loop {lat}")
+    return out
 
 
 def run_batch3(ctx, pairs3, DeviceTree, batch_align, adapter_align, default_params, synth):
