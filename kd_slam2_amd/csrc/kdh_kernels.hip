@@ -111,23 +111,32 @@ __device__ __forceinline__ double bcast(double x, int src) {
   return __hiloint2double(hi, lo);
 }
 
+// 1/d to full double precision: the hardware reciprocal estimate refined by
+// two Newton steps -- half the dependent operations of an IEEE division on
+// the factorisation's serial pivot chain (not correctly rounded).
+__device__ __forceinline__ double rcp_nr(double d) {
+  double y = __builtin_amdgcn_rcp(d);
+  double e = fma(-d, y, 1.0);
+  y = fma(y, e, y);
+  e = fma(-d, y, 1.0);
+  return fma(y, e, y);
+}
+
 // Damped solve (A + diag(damp)) x = -b on one wave: lane i (< D) holds row i
-// of the symmetric matrix in registers; right-looking LDL^T without pivoting
-// (the damped normal matrix is positive definite), then forward and backward
-// substitution, all with lane broadcasts.  Lanes >= D carry zeros.
+// of the symmetric matrix in a[] and b_i in bi; right-looking LDL^T without
+// pivoting (the damped normal matrix is positive definite), then forward and
+// backward substitution with lane broadcasts.  Lanes >= D carry zeros.  The
+// caller's workgroup is this one wave.
 template <int D>
-__device__ void wave_ldlt_solve(const double* sH, const double* sb, double damp_pose, double damp_vel, int lane,
-                                double* sx) {
-  double a[D];
+__device__ void wave_ldlt_solve(double (&a)[D], double bi, double damp_pose, double damp_vel, int lane,
+                                double* sx, double* sL) {
   const bool row = lane < D;
-#pragma unroll
-  for (int j = 0; j < D; ++j) a[j] = row ? sH[lane * D + j] : 0.0;
   if (row) {
 #pragma unroll
     for (int j = 0; j < D; ++j)
       if (j == lane) a[j] += (j < 6) ? damp_pose : damp_vel;
   }
-  double r = row ? -sb[lane] : 0.0;
+  double r = row ? -bi : 0.0;
   double dd = 0.0;  // this lane's pivot D_i
 #pragma unroll
   for (int k = 0; k < D; ++k) {
@@ -137,7 +146,7 @@ __device__ void wave_ldlt_solve(const double* sH, const double* sb, double damp_
       if (j == k) akk = a[j];
     const double dk = bcast(akk, k);
     if (lane == k) dd = dk;
-    const double inv = (dk != 0.0) ? 1.0 / dk : 0.0;
+    const double inv = (dk != 0.0) ? rcp_nr(dk) : 0.0;
     const double lik = (lane > k && row) ? a[k] * inv : 0.0;
     // A_ij -= L_ik * A_kj for j > k (row k broadcast from lane k)
 #pragma unroll
@@ -154,15 +163,20 @@ __device__ void wave_ldlt_solve(const double* sH, const double* sb, double damp_
     if (lane > k && row) r -= a[k] * yk;
   }
   r = (row && dd != 0.0) ? r / dd : 0.0;
-  // backward: L^T x = z; lane i needs L_ki = a[i] of lane k
+  // backward: L^T x = z.  Lane i needs column i of L (L_ki = a[i] of lane k,
+  // k > i): one transpose through LDS, then one broadcast per step.
+  if (row) {
+#pragma unroll
+    for (int j = 0; j < D; ++j) sL[lane * D + j] = a[j];
+  }
+  __syncthreads();
+  double lt[D];
+#pragma unroll
+  for (int k = 0; k < D; ++k) lt[k] = (row && k > lane) ? sL[k * D + lane] : 0.0;
 #pragma unroll
   for (int k = D - 1; k >= 0; --k) {
     const double xk = bcast(r, k);
-#pragma unroll
-    for (int i = 0; i < k; ++i) {
-      const double lki = bcast(a[i], k);
-      if (lane == i) r -= lki * xk;
-    }
+    if (lane < k) r -= lt[k] * xk;
   }
   if (row) sx[lane] = r;
 }
@@ -173,114 +187,131 @@ template <bool CT>
 struct FinishLds {
   static constexpr int N = Slots<CT>::N;
   static constexpr int D = CT ? 12 : 6;
-  static constexpr int CH = 32;  // tiles staged per round
   static constexpr int kRed = 0;                          // double[N]
-  static constexpr int kH = kRed + 8 * ((N + 1) & ~1);    // double[D*D]
-  static constexpr int kG = kH + 8 * D * D;               // double[D*D]
-  static constexpr int kB = kG + 8 * D * D;               // double[D]
-  static constexpr int kX = kB + 8 * D;                   // double[D]
-  static constexpr int kGo = kX + 8 * D;                  // int (padded)
-  static constexpr int kStage = kGo + 16;                 // float[CH*N]
-  static constexpr int kBytes = kStage + 4 * CH * N;
+  static constexpr int kG = kRed + 8 * ((N + 1) & ~1);    // double[3*D]: rows 0..2 of G
+  static constexpr int kP = kG + 8 * 3 * D;               // double[36 + 6]: prior H, b
+  static constexpr int kX = kP + 8 * 42;                  // double[D]
+  static constexpr int kL = kX + 8 * D;                   // double[D*D]
+  static constexpr int kBytes = kL + 8 * D * D;
 };
 
+// Entry (r, c) of the re-centred H_c from the reduced slots (upper-triangular
+// ICPWsItemDiag_ blocks and the ICPWsItemCross_ block).
 template <bool CT>
-__device__ void finish_pair(PairState& ps, const float* __restrict__ partials, char* lds, int tile_first,
-                            int* status_out) {
+__device__ __forceinline__ double hc_entry(const double* s_red, int r, int c) {
+  if (r < 6 && c < 6) {
+    const int rr = r < c ? r : c, c2 = r < c ? c : r;
+    return s_red[rr * 6 - (rr * (rr - 1)) / 2 + (c2 - rr)];
+  }
+  if (r >= 6 && c >= 6) {
+    const int rr = (r < c ? r : c) - 6, c2 = (r < c ? c : r) - 6;
+    return s_red[27 + rr * 6 - (rr * (rr - 1)) / 2 + (c2 - rr)];
+  }
+  return r < 6 ? s_red[54 + r * 6 + (c - 6)] : s_red[54 + c * 6 + (r - 6)];
+}
+
+// Column j of [c]x (skew), chosen by selects: indexing a private matrix with
+// a lane-dependent j would put it in scratch memory.
+__device__ __forceinline__ V3d skew_col(const V3d& c, int j) {
+  return j == 0 ? v3(0.0, c.z, -c.y) : j == 1 ? v3(-c.z, 0.0, c.x) : v3(c.y, -c.x, 0.0);
+}
+
+// The part of PairState the finish stage reads (inputs and state, up to the
+// outputs), in 16-byte units.
+constexpr int kStateVec4 = (int)((offsetof(PairState, stats_log) + 15) / 16);
+static_assert(sizeof(PairState) % 16 == 0, "PairState rows must stay 16-byte aligned");
+
+// One wave (64 lanes) per alignment.  Every step is lane-parallel or a
+// one-wave chain; the workgroup barrier of a one-wave group costs nothing.
+template <bool CT>
+__device__ void finish_pair(PairState& ps, const PairState& S, const float* __restrict__ partials, char* lds,
+                            int tile_first, int* status_out) {
   using FL = FinishLds<CT>;
   constexpr int N = Slots<CT>::N;
   constexpr int D = CT ? 12 : 6;
   constexpr int CS = CT ? 90 : 27;  // chi2 slot
-  constexpr int CH = FL::CH;
-  float* s_stage = (float*)(lds + FL::kStage);
   double* s_red = (double*)(lds + FL::kRed);
-  double* sH = (double*)(lds + FL::kH);
   double* sG = (double*)(lds + FL::kG);
-  double* sb = (double*)(lds + FL::kB);
+  double* sP = (double*)(lds + FL::kP);
   double* sx = (double*)(lds + FL::kX);
-  int& s_go = *(int*)(lds + FL::kGo);
-  const int tid = threadIdx.x;
-  const int lane = tid & 63;
-  // Read the whole pair state the stage needs up front (uniform addresses:
-  // scalar loads issued together), so its latency overlaps the partial-row
-  // loads instead of adding round trips at the end.
-  const int mode = ps.mode;
-  const int it = ps.iter;
-  const int stats_mode = ps.tp.stats_mode;
-  const SolveParams sp = ps.sp;
-  const bool has_prior = ps.has_prior != 0.0;
-  const V3f root_mean = ps.moving_root_mean;
-  const float prev_score = ps.prev_score;
-  double Xs[12], Xfs[12], vs[6];
-#pragma unroll
-  for (int i = 0; i < 12; ++i) {
-    Xs[i] = ps.X[i];
-    Xfs[i] = ps.Xf[i];
-  }
-#pragma unroll
-  for (int i = 0; i < 6; ++i) vs[i] = ps.v[i];
+  double* sL = (double*)(lds + FL::kL);
+  const int lane = threadIdx.x;
+  const int tid = lane;
+  // Inputs come from S, the LDS copy of the pair state taken at kernel start;
+  // results go to ps in global memory.
+  const int mode = S.mode;
+  const int it = S.iter;
+  const int nt = S.n_tiles;
+  const int stats_mode = S.tp.stats_mode;
+  const SolveParams sp = S.sp;
+  const bool has_prior = S.has_prior != 0.0;
+  const V3f root_mean = S.moving_root_mean;
+  const float prev_score = S.prev_score;
+  const V3d cc = v3(S.c64[0], S.c64[1], S.c64[2]);
 
-  // 1. FP64 sum of the pair's partial rows in tile order.  Rows are staged
-  //    through LDS CH at a time with coalesced loads by all threads; thread s
-  //    (< N) adds them in order, so the result is deterministic.
-  const float* base = partials + (size_t)tile_first * N;
-  double acc = 0.0;
-  for (int t0 = 0; t0 < ps.n_tiles; t0 += CH) {
-    const int nt = min(CH, ps.n_tiles - t0);
-    __syncthreads();
-    for (int i = tid; i < nt * N; i += kThreads) s_stage[i] = base[(size_t)t0 * N + i];
-    __syncthreads();
-    if (tid < N)
-      for (int t = 0; t < nt; ++t) acc += (double)s_stage[t * N + tid];
-  }
-  if (tid < N) s_red[tid] = acc;
-  __syncthreads();
-
-  // 2. Assemble the re-centred H_c, b_c, then map back: G = H_c M, H = M^T G,
-  //    b = M^T b_c with M = [[I, -[c]x], [0, I]] (see uncenter_d).
-  const V3d cc = v3(ps.c64[0], ps.c64[1], ps.c64[2]);
-  const M3d Cx = skew(cc);
-  for (int e = tid; e < D * D; e += kThreads) {
-    const int r = e / D, c = e % D;
-    double h;
-    if (r < 6 && c < 6) {
-      const int rr = r < c ? r : c, c2 = r < c ? c : r;
-      const int k = rr * 6 - (rr * (rr - 1)) / 2 + (c2 - rr);
-      h = s_red[k];
-    } else if (r >= 6 && c >= 6) {
-      const int rr = (r < c ? r : c) - 6, c2 = (r < c ? c : r) - 6;
-      const int k = rr * 6 - (rr * (rr - 1)) / 2 + (c2 - rr);
-      h = s_red[27 + k];
-    } else if (r < 6) {
-      h = s_red[54 + r * 6 + (c - 6)];
-    } else {
-      h = s_red[54 + c * 6 + (r - 6)];
+  // 1. FP64 sum of the pair's partial rows in tile order: lane l owns slots
+  //    l and l + 64 and adds their column straight from global memory.  All
+  //    loads of a 32-row block are issued before the first add, so a block
+  //    costs one memory round trip.  The order is fixed, so the result does
+  //    not depend on the batch.
+  {
+    constexpr int TB = 32;
+    const float* base = partials + (size_t)tile_first * N;
+    const bool has0 = lane < N, has1 = lane + 64 < N;
+    double acc0 = 0.0, acc1 = 0.0;
+    for (int t0 = 0; t0 < nt; t0 += TB) {
+      float v0[TB], v1[TB];
+      // Unconditional loads from clamped, in-range addresses (nt >= 1): a
+      // guarded load would become a branch with a wait after every load.
+      const int s0 = has0 ? lane : 0, s1 = has1 ? lane + 64 : 0;
+#pragma unroll
+      for (int u = 0; u < TB; ++u) {
+        const size_t row = (size_t)min(t0 + u, nt - 1) * N;
+        v0[u] = base[row + s0];
+        v1[u] = base[row + s1];
+      }
+#pragma unroll
+      for (int u = 0; u < TB; ++u)
+        if (t0 + u < nt) {
+          acc0 += (double)v0[u];
+          acc1 += (double)v1[u];
+        }
     }
-    sH[e] = h;
-  }
-  if (tid < D) sb[tid] = s_red[tid < 6 ? 21 + tid : 48 + (tid - 6)];
-  __syncthreads();
-  for (int e = tid; e < D * D; e += kThreads) {
-    const int r = e / D, c = e % D;
-    double g = sH[e];
-    if (c >= 3 && c < 6)
-      g -= (sH[r * D + 0] * Cx(0, c - 3) + sH[r * D + 1] * Cx(1, c - 3)) + sH[r * D + 2] * Cx(2, c - 3);
-    sG[e] = g;
+    if (has0) s_red[lane] = acc0;
+    if (has1) s_red[lane + 64] = acc1;
   }
   __syncthreads();
-  for (int e = tid; e < D * D; e += kThreads) {
-    const int r = e / D, c = e % D;
-    double h = sG[e];
-    if (r >= 3 && r < 6)
-      h -= (Cx(0, r - 3) * sG[0 * D + c] + Cx(1, r - 3) * sG[1 * D + c]) + Cx(2, r - 3) * sG[2 * D + c];
-    sH[e] = h;
+
+  // 2. Lane r (< D) assembles row r of H and b_r: the re-centred row from the
+  //    reduced slots, mapped back with M = [[I, -[c]x], [0, I]] (uncenter_d):
+  //    G = H_c M changes columns 3..5, H = M^T G changes rows 3..5 using rows
+  //    0..2 of G, which go through LDS.
+  const bool row = lane < D;
+  const int r = row ? lane : 0;
+  double h[D];
+#pragma unroll
+  for (int c = 0; c < D; ++c) h[c] = hc_entry<CT>(s_red, r, c);
+  {
+    const double h0 = h[0], h1 = h[1], h2 = h[2];
+#pragma unroll
+    for (int c = 3; c < 6; ++c) {
+      const V3d k = skew_col(cc, c - 3);
+      h[c] -= (h0 * k.x + h1 * k.y) + h2 * k.z;
+    }
   }
-  double bj = 0.0;
-  if (tid >= 3 && tid < 6)
-    bj = sb[tid] - ((Cx(0, tid - 3) * sb[0] + Cx(1, tid - 3) * sb[1]) + Cx(2, tid - 3) * sb[2]);
+  if (lane < 3) {
+#pragma unroll
+    for (int c = 0; c < D; ++c) sG[lane * D + c] = h[c];
+  }
+  auto bc = [&](int j) { return s_red[j < 6 ? 21 + j : 48 + (j - 6)]; };
+  double bv = bc(r);
   __syncthreads();
-  if (tid >= 3 && tid < 6) sb[tid] = bj;
-  __syncthreads();
+  if (lane >= 3 && lane < 6) {
+    const V3d k = skew_col(cc, lane - 3);
+#pragma unroll
+    for (int c = 0; c < D; ++c) h[c] -= (k.x * sG[0 * D + c] + k.y * sG[1 * D + c]) + k.z * sG[2 * D + c];
+    bv = bv - ((k.x * bc(0) + k.y * bc(1)) + k.z * bc(2));
+  }
 
   DevStats st;
   st.num_inliers = (int)s_red[CS + 2 + kInlier];
@@ -295,48 +326,60 @@ __device__ void finish_pair(PairState& ps, const float* __restrict__ partials, c
   st.pert_vel_norm = 0.0;
 
   if (mode == 1) {  // build only: publish H, b, stats
-    for (int e = tid; e < D * D; e += kThreads) ps.H[e] = sH[e];
-    if (tid < D) ps.b[tid] = sb[tid];
+    if (row) {
+#pragma unroll
+      for (int c = 0; c < D; ++c) ps.H[r * D + c] = h[c];
+      ps.b[r] = bv;
+    }
     if (tid == 0) ps.stats_log[0] = st;
     return;
   }
 
-  // 3. pose prior (tree_aligner_base_.h:139-152), serial on thread 0
-  if (has_prior && tid == 0) {
-    IsoD X = iso_from_34<double>(Xs);
-    IsoD ZXi = iso_from_34<double>(ps.prior_ZXi);
-    double e[6], J[36], OJ[36], Oe[6];
-    logmap_d(compose(ZXi, X), e);
-    dPoseToPoseError_d(ZXi, X, J);
-    for (int r = 0; r < 6; ++r) {
-      double se = 0.0;
-      for (int k = 0; k < 6; ++k) se += ps.prior_info[r * 6 + k] * e[k];
-      Oe[r] = se;
-      for (int c = 0; c < 6; ++c) {
-        double s = 0.0;
-        for (int k = 0; k < 6; ++k) s += ps.prior_info[r * 6 + k] * J[k * 6 + c];
-        OJ[r * 6 + c] = s;
+  // 3. pose prior (tree_aligner_base_.h:139-152): lane 0 forms the 6x6 block
+  //    and its b, the pose rows add them
+  if (has_prior) {
+    if (tid == 0) {
+      IsoD X = iso_from_34<double>(S.X);
+      IsoD ZXi = iso_from_34<double>(S.prior_ZXi);
+      double e[6], J[36], OJ[36], Oe[6];
+      logmap_d(compose(ZXi, X), e);
+      dPoseToPoseError_d(ZXi, X, J);
+      for (int rr = 0; rr < 6; ++rr) {
+        double se = 0.0;
+        for (int k = 0; k < 6; ++k) se += S.prior_info[rr * 6 + k] * e[k];
+        Oe[rr] = se;
+        for (int c = 0; c < 6; ++c) {
+          double sm = 0.0;
+          for (int k = 0; k < 6; ++k) sm += S.prior_info[rr * 6 + k] * J[k * 6 + c];
+          OJ[rr * 6 + c] = sm;
+        }
+      }
+      for (int rr = 0; rr < 6; ++rr) {
+        double sbv = 0.0;
+        for (int k = 0; k < 6; ++k) sbv += J[k * 6 + rr] * Oe[k];
+        sP[36 + rr] = sbv;
+        for (int c = 0; c < 6; ++c) {
+          double sm = 0.0;
+          for (int k = 0; k < 6; ++k) sm += J[k * 6 + rr] * OJ[k * 6 + c];
+          sP[rr * 6 + c] = sm;
+        }
       }
     }
-    for (int r = 0; r < 6; ++r) {
-      double sbv = 0.0;
-      for (int k = 0; k < 6; ++k) sbv += J[k * 6 + r] * Oe[k];
-      sb[r] += sbv;
-      for (int c = 0; c < 6; ++c) {
-        double s = 0.0;
-        for (int k = 0; k < 6; ++k) s += J[k * 6 + r] * OJ[k * 6 + c];
-        sH[r * D + c] += s;
-      }
+    __syncthreads();
+    if (lane < 6) {
+#pragma unroll
+      for (int c = 0; c < 6; ++c) h[c] += sP[lane * 6 + c];
+      bv += sP[36 + lane];
     }
   }
-  __syncthreads();
-  for (int e = tid; e < D * D; e += kThreads) ps.H[e] = sH[e];
-  if (tid < D) ps.b[tid] = sb[tid];
+  if (row) {
+#pragma unroll
+    for (int c = 0; c < D; ++c) ps.H[r * D + c] = h[c];
+    ps.b[r] = bv;
+  }
 
-  // 4. min_good gate (icp_impl_.h:76-80)
-  if (tid == 0) s_go = (st.num_inliers + st.num_outliers >= sp.min_good_measurements) ? 1 : 0;
-  __syncthreads();
-  if (!s_go) {
+  // 4. min_good gate (icp_impl_.h:76-80); st is wave-uniform
+  if (st.num_inliers + st.num_outliers < sp.min_good_measurements) {
     if (tid == 0) {
       ps.stats_log[it] = st;
       ps.iter = it + 1;
@@ -346,32 +389,90 @@ __device__ void finish_pair(PairState& ps, const float* __restrict__ partials, c
     return;
   }
 
-  // 5. damped solve on wave 0
-  if (tid < 64) wave_ldlt_solve<D>(sH, sb, sp.damping, sp.velocity_damping, lane, sx);
+  // 5. damped solve from the rows in registers
+  if (!row) {
+#pragma unroll
+    for (int c = 0; c < D; ++c) h[c] = 0.0;
+    bv = 0.0;
+  }
+  wave_ldlt_solve<D>(h, bv, sp.damping, sp.velocity_damping, lane, sx, sL);
   __syncthreads();
-  if (tid != 0) return;
 
-  // 6. update, stats, termination, next cache (thread 0)
+  // 6. Update, termination and the next FP32 cache (updateCache,
+  //    icp_impl_.h:16-21), formed in FP64 and rounded once.  Lane 0 forms
+  //    exp(dx); lane e < 12 then forms element e (3x4 row-major) of the new
+  //    pose and of Xmf = Xf^-1 X with the operation order of compose() and
+  //    inverse(); lanes 0..2 form the re-centring anchor.  Xf, and mroot in
+  //    the cache, do not change over the alignment and are not rewritten.
   double dx[D];
+#pragma unroll
   for (int i = 0; i < D; ++i) dx[i] = sx[i];
   double pn = 0.0, vn = 0.0;
+#pragma unroll
   for (int i = 0; i < 6; ++i) pn += dx[i] * dx[i];
-  if (CT)
+  if (CT) {
+#pragma unroll
     for (int i = 6; i < D; ++i) vn += dx[i] * dx[i];
+  }
+  double* sE = sL;        // exp(dx), 3x4 (sL is free after the solve)
+  double* sXc = sL + 12;  // new pose, 3x4
+  double* sC = sL + 24;   // anchor, world frame
+  if (tid == 0 && !stats_mode) iso_to_34(expmap_d(dx), sE);
+  __syncthreads();
+  const int ei = lane >> 2, ej = lane & 3;
+  float* cache_Xm = reinterpret_cast<float*>(&ps.cache.Xm);
+  float* cache_Xmf = reinterpret_cast<float*>(&ps.cache.Xmf);
+  const int cache_slot = ej < 3 ? ei * 3 + ej : 9 + ei;  // Iso layout: R row-major, then t
+  if (lane < 12) {
+    double xc = S.X[lane];
+    if (!stats_mode) {
+      xc = (sE[ei * 4 + 0] * S.X[0 * 4 + ej] + sE[ei * 4 + 1] * S.X[1 * 4 + ej]) + sE[ei * 4 + 2] * S.X[2 * 4 + ej];
+      if (ej == 3) xc = xc + sE[ei * 4 + 3];
+      ps.X[lane] = xc;
+    }
+    sXc[lane] = xc;
+    cache_Xm[cache_slot] = (float)xc;
+  }
+  if (CT && lane < 6) {
+    double v = S.v[lane];
+    if (!stats_mode) {
+      v += sx[6 + lane];
+      ps.v[lane] = v;
+    }
+    ps.cache.v[lane] = (float)v;
+  }
+  __syncthreads();
+  // Xf^-1 = [R^T | -(R^T t)]: entry (i, k) of its rotation is Xf(k, i)
+  auto xf_inv = [&](int i, int k) { return S.Xf[k * 4 + i]; };
+  auto xf_inv_t = [&](int i) {
+    return -((S.Xf[0 * 4 + i] * S.Xf[3] + S.Xf[1 * 4 + i] * S.Xf[7]) + S.Xf[2 * 4 + i] * S.Xf[11]);
+  };
+  if (lane < 12) {
+    double m = (xf_inv(ei, 0) * sXc[0 * 4 + ej] + xf_inv(ei, 1) * sXc[1 * 4 + ej]) + xf_inv(ei, 2) * sXc[2 * 4 + ej];
+    if (ej == 3) m = m + xf_inv_t(ei);
+    cache_Xmf[cache_slot] = (float)m;
+  }
+  if (lane < 3) {
+    double c = 0.0;
+    if (sp.recenter)
+      c = ((sXc[lane * 4 + 3] + sXc[lane * 4 + 0] * (double)root_mean.x) + sXc[lane * 4 + 1] * (double)root_mean.y) +
+          sXc[lane * 4 + 2] * (double)root_mean.z;
+    sC[lane] = c;
+    ps.c64[lane] = c;
+  }
+  __syncthreads();
+  if (lane < 3) {
+    double cf = 0.0;
+    if (sp.recenter)
+      cf = ((xf_inv_t(lane) + xf_inv(lane, 0) * sC[0]) + xf_inv(lane, 1) * sC[1]) + xf_inv(lane, 2) * sC[2];
+    (&ps.cache.cf.x)[lane] = (float)cf;
+  }
+  if (tid != 0) return;
+
   st.pert_pose_norm = sqrt(pn);
   st.pert_vel_norm = sqrt(vn);
   ps.stats_log[it] = st;
   ps.iter = it + 1;
-  IsoD Xc = iso_from_34<double>(Xs);
-  if (!stats_mode) {
-    Xc = compose(expmap_d(dx), Xc);
-    iso_to_34(Xc, ps.X);
-    if (CT)
-      for (int i = 0; i < 6; ++i) {
-        vs[i] += dx[6 + i];
-        ps.v[i] = vs[i];
-      }
-  }
   // termination: stats mode runs one round; the score criterion
   // (icp_stats_.h:85-102) may stop early; otherwise max_iterations.
   bool stop = stats_mode != 0 || it + 1 >= sp.max_iterations;
@@ -393,21 +494,6 @@ __device__ void finish_pair(PairState& ps, const float* __restrict__ partials, c
   if (stop && status == 0) status = 1;
   ps.status = status;
   if (status_out) status_out[0] = status;
-  // next iteration's FP32 cache (updateCache, icp_impl_.h:16-21), formed in
-  // FP64 and rounded once.
-  const IsoD Xf = iso_from_34<double>(Xfs);
-  TermCache nc;
-  nc.Xm = iso_cast<float>(Xc);
-  nc.Xf = iso_cast<float>(Xf);
-  nc.Xmf = iso_cast<float>(compose(inverse(Xf), Xc));
-  for (int i = 0; i < 6; ++i) nc.v[i] = (float)vs[i];
-  const Anchor an = make_anchor(Xc, Xf, root_mean, sp.recenter);
-  nc.mroot = an.mroot;
-  nc.cf = an.cf;
-  ps.cache = nc;
-  ps.c64[0] = an.c.x;
-  ps.c64[1] = an.c.y;
-  ps.c64[2] = an.c.z;
 }
 
 // ---- the tile kernel --------------------------------------------------------
@@ -510,17 +596,24 @@ __global__ __launch_bounds__(kThreads) void tiles_kernel(const PairState* __rest
 // One workgroup per alignment: FP64 reduction of its partial rows, then
 // build-mode publish or solve-mode Gauss-Newton step.
 template <bool CT>
-__global__ __launch_bounds__(kThreads) void finish_kernel(PairState* __restrict__ pairs, int pair_first,
+__global__ __launch_bounds__(64) void finish_kernel(PairState* __restrict__ pairs, int pair_first,
                                                           int n_pairs, const float* __restrict__ partials,
                                                           const int* __restrict__ pair_tile_first,
                                                           int* __restrict__ status_out) {
   __shared__ __align__(16) char s_area[FinishLds<CT>::kBytes];
+  __shared__ __align__(16) float4 s_state[kStateVec4];
   const int p = pair_first + blockIdx.x;
   if (blockIdx.x >= n_pairs) return;
   PairState& ps = pairs[p];
-  if (ps.status != 0 || ps.ct != (CT ? 1 : 0) || ps.n_tiles == 0) return;
-  finish_pair<CT>(ps, partials, s_area, pair_tile_first ? pair_tile_first[p] : ps.tile_first,
-                  status_out ? status_out + p : nullptr);
+  // One memory round trip for the whole state: 16 bytes per lane into LDS,
+  // issued together with the tile-range load.
+  const int tf = pair_tile_first ? pair_tile_first[p] : -1;
+  const float4* src = reinterpret_cast<const float4*>(&ps);
+  for (int i = threadIdx.x; i < kStateVec4; i += 64) s_state[i] = src[i];
+  __syncthreads();
+  const PairState& S = *reinterpret_cast<const PairState*>(s_state);
+  if ((S.status != 0) | (S.ct != (CT ? 1 : 0)) | (S.n_tiles == 0)) return;
+  finish_pair<CT>(ps, S, partials, s_area, tf >= 0 ? tf : S.tile_first, status_out ? status_out + p : nullptr);
 }
 
 // ---- launch wrappers --------------------------------------------------------
@@ -534,7 +627,7 @@ static hipError_t launch_t(const LaunchArgs& a, hipStream_t s) {
     if (e != hipSuccess) return e;
   }
   if ((a.phases & 2) && a.n_pairs > 0) {
-    hipLaunchKernelGGL((finish_kernel<CT>), dim3(a.n_pairs), dim3(kThreads), 0, s, a.pairs, a.pair_first,
+    hipLaunchKernelGGL((finish_kernel<CT>), dim3(a.n_pairs), dim3(64), 0, s, a.pairs, a.pair_first,
                        a.n_pairs, a.partials, a.pair_tile_first, a.status_out);
     return hipGetLastError();
   }
