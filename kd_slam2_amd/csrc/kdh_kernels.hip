@@ -44,29 +44,47 @@ template <bool CT> struct Slots;
 template <> struct Slots<false> { static constexpr int N = kIcpSlots; static constexpr int P = 64; };
 template <> struct Slots<true> { static constexpr int N = kCtSlots; static constexpr int P = 128; };
 
-// Butterfly reduce-scatter over one wave: on entry every lane holds 64
-// values; after the step with lane offset OFF the lane keeps OFF values (the
-// upper or lower half, by its OFF bit) and adds its partner's other half.
-// On exit lane l holds the wave sum of slot l in v[0].  The pairing is fixed,
-// so the summation order -- and the result -- is deterministic.
-template <int OFF>
-__device__ __forceinline__ void reduce_scatter_step(float* v, int lane) {
-  const bool upper = (lane & OFF) != 0;
-#pragma unroll
-  for (int i = 0; i < OFF; ++i) {
-    const float send = upper ? v[i] : v[i + OFF];
-    const float keep = upper ? v[i + OFF] : v[i];
-    v[i] = keep + __shfl_xor(send, OFF, 64);
-  }
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+// Orders this wave's LDS writes before its later LDS reads (one wave: LDS
+// executes its instructions in order; the fences keep the compiler from
+// moving them).
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-__device__ __forceinline__ void wave_reduce_scatter64(float* v, int lane) {
-  reduce_scatter_step<32>(v, lane);
-  reduce_scatter_step<16>(v, lane);
-  reduce_scatter_step<8>(v, lane);
-  reduce_scatter_step<4>(v, lane);
-  reduce_scatter_step<2>(v, lane);
-  reduce_scatter_step<1>(v, lane);
+// Cell (j * 16 + i) of C = sum w w^T holding accumulator slot s < CS + 2, in
+// the packing of ICPWsItemDiag_::pack (upper-triangular J J^T g, then J e g;
+// icp_ws_item_.h:26-34) and ICPWsItemCross_::pack (J_pose J_vel^T g,
+// icp_ws_item_.h:72-75), then chi2 and chi2 kernelised.
+template <bool CT>
+__device__ __forceinline__ int slot_cell(int s) {
+  constexpr int E = CT ? 12 : 6;
+  constexpr int CS = CT ? 90 : 27;
+  int i, j;
+  if (s < 21) {
+    i = tri_r(s);
+    j = tri_c(s);
+  } else if (s < 27) {
+    i = s - 21;
+    j = E;
+  } else if (CT && s < 48) {
+    i = 6 + tri_r(s - 27);
+    j = 6 + tri_c(s - 27);
+  } else if (CT && s < 54) {
+    i = 6 + (s - 48);
+    j = E;
+  } else if (CT && s < 90) {
+    i = (s - 54) / 6;
+    j = 6 + (s - 54) % 6;
+  } else if (s == CS) {
+    i = j = E + 1;  // chi2 = sum e^2
+  } else {
+    i = j = E;      // chi2 kernelised = sum g e^2
+  }
+  return j * 16 + i;
 }
 
 struct GlobalFetch {
@@ -499,7 +517,7 @@ __device__ void finish_pair(PairState& ps, const PairState& S, const float* __re
 // ---- the tile kernel --------------------------------------------------------
 
 template <bool CT, int LDS_NODES>
-__global__ __launch_bounds__(kThreads) void tiles_kernel(const PairState* __restrict__ pairs,
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) void tiles_kernel(const PairState* __restrict__ pairs,
                                                          const int* __restrict__ tile_pair,
                                                          int n_tiles_total,
                                                          float* __restrict__ partials,
@@ -508,9 +526,15 @@ __global__ __launch_bounds__(kThreads) void tiles_kernel(const PairState* __rest
                                                          int xcd_remap,
                                                          const int* __restrict__ pair_tile_first) {
   constexpr int N = Slots<CT>::N;
-  constexpr int NCH = (N + 63) / 64;  // 64-slot chunks
+  constexpr int E = CT ? 12 : 6;       // component of sqrt(g) e in w
+  constexpr int NW = E + 2;            // components of w
+  constexpr int CS = CT ? 90 : 27;     // chi2 slot
+  constexpr int HW = (NW + 1) / 2;     // components per transpose half
+  constexpr int WS = 68;               // transpose row stride (floats): 16-B aligned, spreads banks
+  static_assert(HW * WS >= 256, "the per-wave C tile reuses the transpose buffer");
   __shared__ float4 s_nodes[LDS_NODES > 0 ? 2 * LDS_NODES : 2];
-  __shared__ float s_wave[kWaves][NCH * 64];
+  __shared__ __align__(16) float s_w[kWaves][HW * WS];
+  __shared__ int s_cnt[kWaves][5];
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
@@ -575,18 +599,84 @@ __global__ __launch_bounds__(kThreads) void tiles_kernel(const PairState* __rest
       if (depth_out) depth_out[q] = depth;
     }
 
-    // Expand slots chunk by chunk and reduce each over the wave.
+    // Normal equations of the tile on the matrix cores.  With
+    // w = [sqrt(g) J, sqrt(g) Jv (CT), sqrt(g) e, e] per leaf, every
+    // accumulator slot is an entry of C = sum_leaves w w^T: the 6x6 (and CT
+    // 12x12) blocks of H, b, sum g e^2 (chi2 kernelised) and sum e^2 (chi2).
+    // Per wave C is 16 v_mfma_f32_16x16x4f32 with the 64 leaves as the
+    // K dimension; the leaf vectors are transposed through LDS into the
+    // operand layout (lane i + 16k holds component i of leaf 16k + m for
+    // step m).  The outcome counts come from ballots.
+    {
+      float w[NW];
+      const float sg = sqrtf(R.gamma);
 #pragma unroll
-    for (int ch = 0; ch < NCH; ++ch) {
-      float v[64];
+      for (int i = 0; i < 6; ++i) w[i] = sg * R.J[i];
+      if (CT) {
 #pragma unroll
-      for (int i = 0; i < 64; ++i) v[i] = rec_slot<CT>(R, ch * 64 + i);
-      wave_reduce_scatter64(v, lane);
-      s_wave[wave][ch * 64 + lane] = v[0];
+        for (int i = 0; i < 6; ++i) w[6 + i] = sg * R.Jv[i];
+      }
+      w[E] = sg * R.e;
+      w[E + 1] = R.e;
+      // The transpose goes through LDS in two halves of HW components, so
+      // the buffer stays small enough for four workgroups per CU.
+      float* sw = s_w[wave];
+      const int ci = lane & 15, cg = lane >> 4;
+      float a[16];
+#pragma unroll
+      for (int m = 0; m < 16; ++m) a[m] = 0.0f;
+#pragma unroll
+      for (int half = 0; half < 2; ++half) {
+#pragma unroll
+        for (int i = 0; i < HW; ++i)
+          if (half * HW + i < NW) sw[i * WS + lane] = w[half * HW + i];
+        wave_lds_sync();
+        if (ci >= half * HW && ci < half * HW + HW && ci < NW) {
+          const float4* src = reinterpret_cast<const float4*>(sw + (ci - half * HW) * WS + cg * 16);
+#pragma unroll
+          for (int m4 = 0; m4 < 4; ++m4) {
+            const float4 v = src[m4];
+            a[4 * m4 + 0] = v.x;
+            a[4 * m4 + 1] = v.y;
+            a[4 * m4 + 2] = v.z;
+            a[4 * m4 + 3] = v.w;
+          }
+        }
+        wave_lds_sync();
+      }
+      floatx4 c0 = {0.0f, 0.0f, 0.0f, 0.0f}, c1 = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+      for (int m = 0; m < 16; m += 2) {
+        c0 = __builtin_amdgcn_mfma_f32_16x16x4f32(a[m], a[m], c0, 0, 0, 0);
+        c1 = __builtin_amdgcn_mfma_f32_16x16x4f32(a[m + 1], a[m + 1], c1, 0, 0, 0);
+      }
+      const floatx4 c = c0 + c1;
+      // lane (ci, cg) holds C[4 cg + r][ci], stored at ci * 16 + 4 cg + r,
+      // in this wave's part of the transpose buffer
+      *reinterpret_cast<float4*>(&sw[ci * 16 + 4 * cg]) = make_float4(c[0], c[1], c[2], c[3]);
+      const unsigned long long m_in = __ballot(R.outcome == kInlier);
+      const unsigned long long m_out = __ballot(R.outcome == kOutlier);
+      const unsigned long long m_bad = __ballot(R.outcome == kBad);
+      const unsigned long long m_nrm = __ballot(R.outcome == kErrNormal);
+      const unsigned long long m_fn = __ballot(R.outcome == kErrFn);
+      if (lane == 0) {
+        s_cnt[wave][kInlier] = __popcll(m_in);
+        s_cnt[wave][kOutlier] = __popcll(m_out);
+        s_cnt[wave][kBad] = __popcll(m_bad);
+        s_cnt[wave][kErrNormal] = __popcll(m_nrm);
+        s_cnt[wave][kErrFn] = __popcll(m_fn);
+      }
     }
     __syncthreads();
     if (tid < N) {
-      const float s = (s_wave[0][tid] + s_wave[1][tid]) + (s_wave[2][tid] + s_wave[3][tid]);
+      float s;
+      if (tid < CS + 2) {
+        const int cell = slot_cell<CT>(tid);
+        s = (s_w[0][cell] + s_w[1][cell]) + (s_w[2][cell] + s_w[3][cell]);
+      } else {
+        const int k = tid - CS - 2;
+        s = (float)((s_cnt[0][k] + s_cnt[1][k]) + (s_cnt[2][k] + s_cnt[3][k]));
+      }
       partials[(size_t)t * N + tid] = s;
     }
     __syncthreads();
