@@ -562,10 +562,20 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
       if (k >= n_tiles_total) break;
       t = k;
     }
+    // Three memory round trips before the descent: the tile's pair, the pair
+    // fields, then the leaf records together with the top-of-tree staging.
     const int p = tile_pair ? tile_pair[t] : 0;
     const PairState& ps = pairs[p];
-    if (ps.status != 0) continue;
+    const int status = ps.status;
     const DevTree fixed = ps.fixed;
+    const DevTree moving = ps.moving;
+    const int tfirst = pair_tile_first ? pair_tile_first[p] : ps.tile_first;
+    if (status != 0) continue;
+    const int q = (t - tfirst) * kTile + tid;
+    const bool live = q < moving.n_leaves && fixed.n_nodes > 0;
+    const int qc = live ? q : 0;  // a tile exists, so the pair has >= 1 leaf
+    const float4 la = moving.leaves[2 * qc];
+    const float4 lb = moving.leaves[2 * qc + 1];
     if (LDS_NODES > 0 && p != cached_pair) {
       __syncthreads();
       n_lds = fixed.n_nodes < LDS_NODES ? fixed.n_nodes : LDS_NODES;
@@ -573,14 +583,10 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
       cached_pair = p;
       __syncthreads();
     }
-    const int lt = t - (pair_tile_first ? pair_tile_first[p] : ps.tile_first);
-    const int q = lt * kTile + tid;
 
     TermRec R;
     rec_reject(R, -1);  // padding lanes contribute nothing
-    if (q < ps.moving.n_leaves && fixed.n_nodes > 0) {
-      const float4 la = ps.moving.leaves[2 * q];
-      const float4 lb = ps.moving.leaves[2 * q + 1];
+    if (live) {
       LeafIn L;
       L.mean = v3(la.x, la.y, la.z);
       L.dir = v3(la.w, lb.x, lb.y);
