@@ -186,7 +186,14 @@ typedef struct kdh_pair {
   const kdh_tree* fixed;
   const kdh_tree* moving;
   int32_t ct;
-  int32_t pad;
+  int32_t forest;              /* 0: a plain pair.  k > 0: this pair and the
+                                  next k pairs are one alignment against a
+                                  fixed forest (ICP_::_fixed_forest +
+                                  makeFixedEntry, icp/icp_impl_.h:53-71,
+                                  icp/icp_.h:90-95): the next k pairs give only
+                                  their fixed tree and Xf; they must name the
+                                  same moving tree and ct.  H, b and the stats
+                                  are summed over the entries each round. */
   double X0[12];
   double v0[6];
   double Xf[12];
@@ -197,15 +204,18 @@ typedef struct kdh_pair {
 typedef struct kdh_pair_result {
   double X[12];
   double v[6];
-  kdh_stats last;
+  kdh_stats last;              /* the aligner's stats of the last round */
+  kdh_stats entry;             /* this pair's own fixed tree's share of the
+                                  last round (FixedEntry::stats); equal to
+                                  last for a plain pair.  No pose norms. */
   int32_t iterations;
   int32_t result;
 } kdh_pair_result;
 
-/* Runs every pair to completion: one persistent launch per Gauss-Newton
- * iteration over the (pair, tile) list, captured as a HIP graph.  Each
- * pair's result is bit-identical to running it alone through kdh_batch_align
- * with n_pairs = 1. */
+/* Runs every pair (or forest) to completion: one tile launch and one finish
+ * launch per Gauss-Newton iteration over the (pair, tile) list, captured as
+ * a HIP graph.  Each result is bit-identical to running it alone through
+ * kdh_batch_align; forest entries' results repeat their leader's pose. */
 int kdh_batch_align(kdh_ctx* ctx, int n_pairs, const kdh_pair* pairs, kdh_pair_result* results);
 
 /* ---- serving: continuous batching ----------------------------------------

@@ -8,6 +8,7 @@ from .api import (  # noqa: F401
     CTICPHIP3D,
     Context,
     DeviceTree,
+    FixedEntry,
     HostTree,
     ICPHIP3D,
     Server,

@@ -95,7 +95,7 @@ class Pair(C.Structure):
         ("fixed", C.c_void_p),
         ("moving", C.c_void_p),
         ("ct", C.c_int32),
-        ("pad", C.c_int32),
+        ("forest", C.c_int32),
         ("X0", C.c_double * 12),
         ("v0", C.c_double * 6),
         ("Xf", C.c_double * 12),
@@ -109,6 +109,7 @@ class PairResult(C.Structure):
         ("X", C.c_double * 12),
         ("v", C.c_double * 6),
         ("last", Stats),
+        ("entry", Stats),
         ("iterations", C.c_int32),
         ("result", C.c_int32),
     ]

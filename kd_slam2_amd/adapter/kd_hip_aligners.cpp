@@ -170,11 +170,71 @@ void AlignerHIP3D<CT>::buildQuadraticForm(bool stats_mode) {
   stats.pert_vel_norm = 0;
 }
 
+template <int CT>
+std::shared_ptr<typename AlignerHIP3D<CT>::FixedEntry> AlignerHIP3D<CT>::makeFixedEntry(const TreeHIP3D& tree,
+                                                                                        const Isometry3f& X) {
+  double X34[12];
+  for (int i = 0; i < 12; ++i) X34[i] = X.m[i];  // 3x4 row-major
+  return makeFixedEntry(tree.handle(), X34);
+}
+
+template <int CT>
+std::shared_ptr<typename AlignerHIP3D<CT>::FixedEntry> AlignerHIP3D<CT>::makeFixedEntry(const kdh_tree* tree,
+                                                                                        const double X34[12]) {
+  auto e = std::make_shared<FixedEntry>();
+  e->fixed_tree = tree;
+  memcpy(e->fixed_X, X34, sizeof(e->fixed_X));
+  return e;
+}
+
+// ICP_::buildQuadraticForm(FixedEntry&) (icp/icp_impl_.h:30-40): the form
+// against one entry of the fixed forest, kept in the entry.
+template <int CT>
+void AlignerHIP3D<CT>::buildQuadraticForm(FixedEntry& entry, bool stats_mode) {
+  const kdh_tree* fixed = _fixed;
+  double Xf_keep[12];
+  memcpy(Xf_keep, Xf, sizeof(Xf));
+  _fixed = entry.fixed_tree;
+  memcpy(Xf, entry.fixed_X, sizeof(Xf));
+  try {
+    buildQuadraticForm(stats_mode);
+  } catch (...) {
+    _fixed = fixed;
+    memcpy(Xf, Xf_keep, sizeof(Xf));
+    throw;
+  }
+  _fixed = fixed;
+  memcpy(Xf, Xf_keep, sizeof(Xf));
+  entry.stats = stats;
+  memcpy(entry.H, H, sizeof(H));
+  memcpy(entry.b, b, sizeof(b));
+}
+
 // ICP_::oneRound / CTICP_::oneRound (icp/icp_impl_.h:53-90, cticp/cticp_impl_.h:48-92)
 // with the damped system solved in FP64.
 template <int CT>
 bool AlignerHIP3D<CT>::oneRound(bool stats_mode) {
-  buildQuadraticForm(stats_mode);
+  if (_fixed_forest.empty()) {
+    buildQuadraticForm(stats_mode);
+  } else {
+    double tH[D * D] = {}, tb[D] = {};
+    ICPStats ts;
+    for (auto& [ref, e] : _fixed_forest) {
+      buildQuadraticForm(*e, stats_mode);
+      for (int i = 0; i < D * D; ++i) tH[i] += e->H[i];
+      for (int i = 0; i < D; ++i) tb[i] += e->b[i];
+      ts.num_inliers += e->stats.num_inliers;
+      ts.num_outliers += e->stats.num_outliers;
+      ts.num_bad += e->stats.num_bad;
+      ts.num_err_normal += e->stats.num_err_normal;
+      ts.num_err_fn += e->stats.num_err_fn;
+      ts.chi2 += e->stats.chi2;
+      ts.chi2_kernelized += e->stats.chi2_kernelized;
+    }
+    memcpy(H, tH, sizeof(H));
+    memcpy(b, tb, sizeof(b));
+    stats = ts;
+  }
   double Hs[D * D], bs[D];
   memcpy(Hs, H, sizeof(Hs));
   memcpy(bs, b, sizeof(bs));
@@ -240,6 +300,51 @@ int AlignerHIP3D<CT>::compute(bool stats_mode) {
 template <int CT>
 int AlignerHIP3D<CT>::computeOnDevice(bool stats_mode) {
   syncParams();
+  if (!_fixed_forest.empty()) {
+    // one round without update per entry: the per-call path
+    if (stats_mode) return compute(true);
+    if (!_moving) throw std::runtime_error("setMoving not called");
+    kdh_ctx* ctx = _ctx ? _ctx : defaultContext(0);
+    const int n = (int)_fixed_forest.size();
+    std::vector<kdh_pair> pairs(n);
+    std::vector<kdh_pair_result> res(n);
+    int j = 0;
+    for (auto& [ref, e] : _fixed_forest) {
+      kdh_pair& pr = pairs[j];
+      memset(&pr, 0, sizeof(pr));
+      pr.fixed = e->fixed_tree;
+      pr.moving = _moving;
+      pr.ct = CT;
+      pr.forest = j == 0 ? n - 1 : 0;
+      memcpy(pr.X0, X, sizeof(pr.X0));
+      memcpy(pr.v0, v, sizeof(pr.v0));
+      memcpy(pr.Xf, e->fixed_X, sizeof(pr.Xf));
+      pr.params = params;
+      pr.prior = (j == 0 && _has_prior) ? &_prior : nullptr;
+      ++j;
+    }
+    check(ctx, kdh_batch_align(ctx, n, pairs.data(), res.data()), "kdh_batch_align");
+    auto to_stats = [](const kdh_stats& k) {
+      ICPStats s;
+      s.num_inliers = k.num_inliers;
+      s.num_outliers = k.num_outliers;
+      s.num_bad = k.num_bad;
+      s.num_err_normal = k.num_err_normal;
+      s.num_err_fn = k.num_err_fn;
+      s.chi2 = (float)k.chi2;
+      s.chi2_kernelized = (float)k.chi2_kernelized;
+      s.pert_pose_norm = (float)k.pert_pose_norm;
+      s.pert_vel_norm = (float)k.pert_vel_norm;
+      return s;
+    };
+    j = 0;
+    for (auto& [ref, e] : _fixed_forest) e->stats = to_stats(res[j++].entry);
+    memcpy(X, res[0].X, sizeof(X));
+    if (CT) memcpy(v, res[0].v, sizeof(v));
+    stats = to_stats(res[0].last);
+    stats_log.assign(1, stats);
+    return res[0].result;
+  }
   if (!_fixed || !_moving) throw std::runtime_error("setFixed/setMoving not called");
   kdh_ctx* ctx = _ctx ? _ctx : defaultContext(0);
   std::vector<kdh_stats> log(std::max(1, param_max_iterations.value()));

@@ -19,6 +19,7 @@
 
 #include <array>
 #include <memory>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -89,6 +90,17 @@ struct ICPTerminationCriteriaScore3D : public srrg2_core::Configurable {
 
 enum AlignerType { AlignerICP, AlignerCTICP };
 
+// TreeAlignerBase_::FixedEntry_ (icp/tree_aligner_base_.h:57-74): one tree of
+// the fixed forest at its own pose, and its share of the last round.
+template <int D>
+struct FixedEntryHIP {
+  const kdh_tree* fixed_tree = nullptr;
+  double fixed_X[12] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0};
+  ICPStats stats;
+  double H[D * D] = {};
+  double b[D] = {};
+};
+
 // ---- the aligners ---------------------------------------------------------
 template <int CT>
 struct AlignerHIP3D : public srrg2_core::Configurable {
@@ -129,8 +141,16 @@ struct AlignerHIP3D : public srrg2_core::Configurable {
   void resetMovingState();
   void setPosePrior(const Isometry3f& X_ref, const Isometry3f& Z, const Matrix6f& info);
 
+  // ---- fixed forest (icp/icp_.h:90-95, icp/icp_impl_.h:30-71) ----
+  using FixedEntry = FixedEntryHIP<D>;
+  std::shared_ptr<FixedEntry> makeFixedEntry(const TreeHIP3D& tree, const Isometry3f& X);
+  std::shared_ptr<FixedEntry> makeFixedEntry(const kdh_tree* tree, const double X[12]);
+  // std::map: entries are summed in key order, as the reference iterates it
+  std::map<int, std::shared_ptr<FixedEntry>> _fixed_forest;
+
   void syncParams();
   void buildQuadraticForm(bool stats_mode = false);
+  void buildQuadraticForm(FixedEntry& entry, bool stats_mode = false);
   bool oneRound(bool stats_mode = false);
   int compute(bool stats_mode = false);
   int computeOnDevice(bool stats_mode = false);

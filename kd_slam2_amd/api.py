@@ -204,6 +204,18 @@ def update_cache_f32(Xm, Xf):
     return m.reshape(12).copy(), f.reshape(12).copy(), out.reshape(12)
 
 
+class FixedEntry:
+    """TreeAlignerBase_::FixedEntry_ (tree_aligner_base_.h:57-74): a fixed
+    tree, its pose, and its share (H, b, stats) of the last round."""
+
+    def __init__(self, tree, X):
+        self.fixed_tree = tree
+        self.fixed_X = np.array(X, dtype=np.float64)
+        self.stats = None
+        self.H = None
+        self.b = None
+
+
 class _AlignerBase:
     """TreeAlignerBase_ state and parameters shared by ICPHIP3D / CTICPHIP3D."""
 
@@ -226,6 +238,8 @@ class _AlignerBase:
         self.H = np.zeros((self.DIM, self.DIM))
         self.b = np.zeros(self.DIM)
         self.dx = np.zeros(self.DIM)
+        # fixed forest (ICP_::_fixed_forest, key order = iteration order)
+        self._fixed_forest = {}
 
     # ---- parameters (PARAM, tree_aligner_base_.h:39-47) ----
     def set(self, name, value):
@@ -249,6 +263,11 @@ class _AlignerBase:
 
     def setFixedX(self, X):
         self.Xf = np.array(X, dtype=np.float64)
+
+    def makeFixedEntry(self, tree, Xf):
+        """FixedEntry of the fixed forest (icp/icp_.h:90-95): a fixed tree at
+        its own pose.  Store it as aligner._fixed_forest[ref] = entry."""
+        return FixedEntry(tree, Xf)
 
     def setPosePrior(self, X_ref=None, Z=None, info=None):
         if info is None or not np.any(info):
@@ -302,11 +321,39 @@ class _AlignerBase:
         self.b = b
         self.stats = st.as_dict()
 
+    def _buildForest(self, stats_mode):
+        """buildQuadraticForm(FixedEntry&) per entry, summed in key order
+        (icp_impl_.h:55-70)."""
+        fixed, Xf = self._fixed, self.Xf
+        H = np.zeros((self.DIM, self.DIM))
+        b = np.zeros(self.DIM)
+        tot = None
+        try:
+            for ref in sorted(self._fixed_forest):
+                e = self._fixed_forest[ref]
+                self._fixed, self.Xf = e.fixed_tree, e.fixed_X
+                self.buildQuadraticForm(stats_mode)
+                e.stats, e.H, e.b = dict(self.stats), self.H.copy(), self.b.copy()
+                H += self.H
+                b += self.b
+                if tot is None:
+                    tot = dict(self.stats)
+                else:
+                    for k in ("num_inliers", "num_outliers", "num_bad", "num_err_normal", "num_err_fn", "chi2",
+                              "chi2_kernelized"):
+                        tot[k] += self.stats[k]
+        finally:
+            self._fixed, self.Xf = fixed, Xf
+        self.H, self.b, self.stats = H, b, tot
+
     def oneRound(self, stats_mode=False):
         """ICP_::oneRound / CTICP_::oneRound with an FP64 solve on the host."""
         from .synth import expmap
 
-        self.buildQuadraticForm(stats_mode)
+        if self._fixed_forest:
+            self._buildForest(stats_mode)
+        else:
+            self.buildQuadraticForm(stats_mode)
         H = self.H.copy()
         b = self.b.copy()
         if self.prior is not None:
@@ -354,8 +401,34 @@ class _AlignerBase:
         return self.params.max_iterations
 
     # ---- the device-resident loop ----
+    def _computeForestOnDevice(self, stats_mode):
+        if stats_mode:
+            # one round without update: the per-call form of every entry
+            return self.compute(stats_mode=True)
+        refs = sorted(self._fixed_forest)
+        p = L.Params.from_buffer_copy(self.params)
+        pairs = []
+        for j, ref in enumerate(refs):
+            e = self._fixed_forest[ref]
+            pairs.append(dict(fixed=e.fixed_tree, moving=self._moving, ct=self.CT, X0=self.X, v0=self.v,
+                              Xf=e.fixed_X, params=p, prior=self.prior if j == 0 else None,
+                              forest=len(refs) - 1 if j == 0 else 0))
+        out = batch_align(self.ctx, pairs)
+        for ref, r in zip(refs, out):
+            self._fixed_forest[ref].stats = r["entry"]
+        r = out[0]
+        self.X = r["X"]
+        if self.CT:
+            self.v = r["v"].copy()
+        self.stats = r["last"]
+        self.stats_log = [self.stats]
+        return r["result"]
+
     def computeOnDevice(self, stats_mode=False, cache_log=False):
-        """compute() with the whole Gauss-Newton loop on the GPU (kdh_align)."""
+        """compute() with the whole Gauss-Newton loop on the GPU (kdh_align;
+        with a fixed forest, one kdh_batch_align forest)."""
+        if self._fixed_forest:
+            return self._computeForestOnDevice(stats_mode)
         X0 = np.ascontiguousarray(self.X[:3, :4].reshape(12))
         Xf = np.ascontiguousarray(self.Xf[:3, :4].reshape(12))
         v0 = np.ascontiguousarray(self.v, dtype=np.float64)
@@ -481,7 +554,9 @@ def batch_align(ctx, pairs):
     """Run many alignments in one batched launch (kdh_batch_align).
 
     pairs: list of dicts with keys fixed, moving (DeviceTree), ct, X0, Xf, v0,
-    params (L.Params).  Returns a list of dicts (X, v, iterations, result, last)."""
+    params (L.Params), prior, forest (k > 0: this pair leads a fixed forest
+    with the next k pairs).  Returns a list of dicts (X, v, iterations,
+    result, last, entry)."""
     n = len(pairs)
     arr = (L.Pair * n)()
     keep = []
@@ -490,6 +565,7 @@ def batch_align(ctx, pairs):
         a.fixed = p["fixed"].h.value
         a.moving = p["moving"].h.value
         a.ct = int(p.get("ct", 0))
+        a.forest = int(p.get("forest", 0))
         a.X0[:] = list(np.asarray(p["X0"])[:3, :4].reshape(12))
         a.v0[:] = list(np.asarray(p.get("v0", np.zeros(6)), dtype=np.float64))
         a.Xf[:] = list(np.asarray(p.get("Xf", np.eye(4)))[:3, :4].reshape(12))
@@ -507,7 +583,8 @@ def batch_align(ctx, pairs):
         r = res[i]
         X = np.eye(4)
         X[:3, :4] = np.array(r.X[:]).reshape(3, 4)
-        out.append(dict(X=X, v=np.array(r.v[:]), iterations=r.iterations, result=r.result, last=r.last.as_dict()))
+        out.append(dict(X=X, v=np.array(r.v[:]), iterations=r.iterations, result=r.result, last=r.last.as_dict(),
+                        entry=r.entry.as_dict()))
     return out
 
 

@@ -54,25 +54,52 @@ int kdh_batch_align(kdh_ctx* ctx, int n_pairs, const kdh_pair* pairs, kdh_pair_r
   g[0].ct = 0;
   g[1].ct = 1;
   int max_iter = 0;
+  // Forests: pair i with forest = k leads pairs i+1 .. i+k.
+  std::vector<int> leader(n_pairs);
+  for (int i = 0; i < n_pairs;) {
+    const int k = pairs[i].forest;
+    if (k < 0 || k >= n_pairs - i)
+      return kdh_fail(ctx, KDH_ERR_INVALID, "pair " + std::to_string(i) + ": forest runs past the last pair");
+    for (int j = 1; j <= k; ++j) {
+      const kdh_pair& m = pairs[i + j];
+      if (m.moving != pairs[i].moving || m.ct != pairs[i].ct || m.forest != 0)
+        return kdh_fail(ctx, KDH_ERR_INVALID,
+                        "pair " + std::to_string(i + j) + ": forest entry must share the moving tree and ct and "
+                        "lead no forest");
+    }
+    for (int j = 0; j <= k; ++j) leader[i + j] = i;
+    i += k + 1;
+  }
   for (int i = 0; i < n_pairs; ++i) {
-    const kdh_pair& pr = pairs[i];
+    const kdh_pair& lp = pairs[leader[i]];  // pose, velocities, params, prior
+    const kdh_pair& pr = pairs[i];           // fixed tree and pose
     PairState& ps = ctx->h_pairs[i];
-    rc = kdh_fill_pair(ctx, ps, pr.fixed, pr.moving, pr.ct, pr.params, 0, 0);
+    rc = kdh_fill_pair(ctx, ps, pr.fixed, lp.moving, lp.ct, lp.params, 0, 0);
     if (rc) return kdh_fail(ctx, rc, "pair " + std::to_string(i) + ": " + ctx->err);
-    memcpy(ps.X, pr.X0, sizeof(ps.X));
+    memcpy(ps.X, lp.X0, sizeof(ps.X));
     memcpy(ps.Xf, pr.Xf, sizeof(ps.Xf));
-    for (int k = 0; k < 6; ++k) ps.v[k] = pr.ct ? pr.v0[k] : 0.0;
-    kdh_setup_prior(ps, pr.prior);
+    for (int k = 0; k < 6; ++k) ps.v[k] = lp.ct ? lp.v0[k] : 0.0;
+    kdh_setup_prior(ps, leader[i] == i ? lp.prior : nullptr);
     kdh_cache_from_state(ps);
     Group& G = g[ps.ct];
     ps.tile_first = G.n_tiles;
     G.n_tiles += ps.n_tiles;
-    if (ps.n_tiles == 0) {
-      // no moving leaves: the first round fails the min_good gate
-      ps.iter = 1;
-      ps.status = ps.sp.min_good_measurements > 0 ? 2 : 1;
+    if (leader[i] != i) {
+      ps.group_n = 0;
+      PairState& L = ctx->h_pairs[leader[i]];
+      L.group_n += 1;
+      L.group_tiles += ps.n_tiles;
     }
     max_iter = std::max(max_iter, ps.sp.max_iterations);
+  }
+  for (int i = 0; i < n_pairs; ++i) {
+    const PairState& L = ctx->h_pairs[leader[i]];
+    if (L.group_tiles == 0) {
+      // no moving leaves: the first round fails the min_good gate
+      PairState& ps = ctx->h_pairs[i];
+      ps.iter = 1;
+      ps.status = L.sp.min_good_measurements > 0 ? 2 : 1;
+    }
   }
   g[0].partial_off = 0;
   g[1].partial_off = (size_t)g[0].n_tiles * slots(0);
@@ -162,8 +189,43 @@ int kdh_batch_align(kdh_ctx* ctx, int n_pairs, const kdh_pair* pairs, kdh_pair_r
     ctx->last_tile_us = ctx->last_finish_us = 0.0;
     ctx->last_launches = iters;
   }
+  // Per-entry stats of forests: each entry's share of the last round, summed
+  // on the host from the partial rows of that round (rows of finished pairs
+  // are not rewritten).
+  std::vector<kdh_stats> entry(n_pairs);
+  std::vector<float> rows;
   for (int i = 0; i < n_pairs; ++i) {
-    const PairState& ps = ctx->h_pairs[i];
+    const PairState& L = ctx->h_pairs[leader[i]];
+    memset(&entry[i], 0, sizeof(kdh_stats));
+    if (L.iter > 0) memcpy(&entry[i], &L.stats_log[L.iter - 1], sizeof(kdh_stats));
+    entry[i].pert_pose_norm = entry[i].pert_vel_norm = 0.0;
+  }
+  for (int i = 0; i < n_pairs; ++i) {
+    const PairState& L = ctx->h_pairs[i];
+    if (leader[i] != i || L.group_n < 2 || L.iter == 0 || L.group_tiles == 0) continue;
+    const int N = slots(L.ct), CS = L.ct ? 90 : 27;
+    rows.resize((size_t)L.group_tiles * N);
+    const float* src = ctx->d_partials + g[L.ct].partial_off + (size_t)L.tile_first * N;
+    KDH_HIP_TRY(ctx, hipMemcpy(rows.data(), src, rows.size() * sizeof(float), hipMemcpyDeviceToHost));
+    for (int e = 0; e < L.group_n; ++e) {
+      const PairState& ps = ctx->h_pairs[i + e];
+      double acc[7] = {0, 0, 0, 0, 0, 0, 0};
+      for (int t = 0; t < ps.n_tiles; ++t) {
+        const float* row = rows.data() + (size_t)(ps.tile_first - L.tile_first + t) * N;
+        for (int k = 0; k < 7; ++k) acc[k] += (double)row[CS + k];
+      }
+      kdh_stats& st = entry[i + e];
+      st.chi2 = acc[0];
+      st.chi2_kernelized = acc[1];
+      st.num_inliers = (int)acc[2 + kInlier];
+      st.num_outliers = (int)acc[2 + kOutlier];
+      st.num_bad = (int)acc[2 + kBad];
+      st.num_err_normal = (int)acc[2 + kErrNormal];
+      st.num_err_fn = (int)acc[2 + kErrFn];
+    }
+  }
+  for (int i = 0; i < n_pairs; ++i) {
+    const PairState& ps = ctx->h_pairs[leader[i]];
     kdh_pair_result& r = results[i];
     memcpy(r.X, ps.X, sizeof(r.X));
     memcpy(r.v, ps.v, sizeof(r.v));
@@ -172,7 +234,9 @@ int kdh_batch_align(kdh_ctx* ctx, int n_pairs, const kdh_pair* pairs, kdh_pair_r
       memcpy(&r.last, &ps.stats_log[ps.iter - 1], sizeof(kdh_stats));
     else
       memset(&r.last, 0, sizeof(kdh_stats));
-    r.result = kdh_result_code(ps, pairs[i].params.max_iterations);
+    r.entry = entry[i];
+    if (ps.group_n < 2) r.entry = r.last;
+    r.result = kdh_result_code(ps, pairs[leader[i]].params.max_iterations);
   }
   return KDH_OK;
 }

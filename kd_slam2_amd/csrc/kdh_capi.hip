@@ -306,6 +306,8 @@ int kdh_fill_pair(kdh_ctx* ctx, PairState& ps, const kdh_tree* fixed, const kdh_
   ps.ct = ct ? 1 : 0;
   ps.mode = mode;
   ps.n_tiles = n_tiles_of(moving->dt.n_leaves);
+  ps.group_n = 1;
+  ps.group_tiles = ps.n_tiles;
   ps.moving_root_mean = moving->root_mean;
   ps.status = 0;
   ps.iter = 0;

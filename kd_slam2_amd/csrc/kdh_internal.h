@@ -62,6 +62,14 @@ struct PairState {
   int tile_first;       // first tile of this pair in the launch's tile list
   int n_tiles;
   int mode;             // 0: solve + update (align), 1: build only (write H/b)
+  // Fixed forest (_fixed_forest, icp_impl_.h:53-71): a group of consecutive
+  // pair states with one moving tree and pose, one fixed tree and pose each.
+  // The first (leader) holds group_n = entries and group_tiles = their tiles
+  // (contiguous in the launch); the others hold group_n = 0.  A plain pair
+  // is a group of one.
+  int group_n;
+  int group_tiles;
+  int group_pad[2];     // keeps sizeof(PairState) a multiple of 16
   double Xf[12];        // fixed pose, 3x4 row-major
   double has_prior;
   double prior_ZXi[12]; // (X_ref * Z)^-1

@@ -209,8 +209,8 @@ struct FinishLds {
   static constexpr int kG = kRed + 8 * ((N + 1) & ~1);    // double[3*D]: rows 0..2 of G
   static constexpr int kP = kG + 8 * 3 * D;               // double[36 + 6]: prior H, b
   static constexpr int kX = kP + 8 * 42;                  // double[D]
-  static constexpr int kL = kX + 8 * D;                   // double[D*D]
-  static constexpr int kBytes = kL + 8 * D * D;
+  static constexpr int kL = kX + 8 * D;                   // double[max(D*D, 40)]: L, then update scratch
+  static constexpr int kBytes = kL + 8 * (D * D > 40 ? D * D : 40);
 };
 
 // Entry (r, c) of the re-centred H_c from the reduced slots (upper-triangular
@@ -259,7 +259,9 @@ __device__ void finish_pair(PairState& ps, const PairState& S, const float* __re
   // results go to ps in global memory.
   const int mode = S.mode;
   const int it = S.iter;
-  const int nt = S.n_tiles;
+  const int nt = S.group_tiles;  // the tiles of every forest entry, contiguous
+  const int n_entries = S.group_n;
+  PairState* members = &ps;     // entries 1 .. n_entries - 1 follow the leader
   const int stats_mode = S.tp.stats_mode;
   const SolveParams sp = S.sp;
   const bool has_prior = S.has_prior != 0.0;
@@ -404,6 +406,10 @@ __device__ void finish_pair(PairState& ps, const PairState& S, const float* __re
       ps.status = 2;  // failed: compute() returns -it
       if (status_out) status_out[0] = 2;
     }
+    for (int m = 1 + tid; m < n_entries; m += 64) {
+      members[m].iter = it + 1;
+      members[m].status = 2;
+    }
     return;
   }
 
@@ -485,6 +491,33 @@ __device__ void finish_pair(PairState& ps, const PairState& S, const float* __re
       cf = ((xf_inv_t(lane) + xf_inv(lane, 0) * sC[0]) + xf_inv(lane, 1) * sC[1]) + xf_inv(lane, 2) * sC[2];
     (&ps.cache.cf.x)[lane] = (float)cf;
   }
+  // Forest entries: the same moving pose and velocities, their own Xmf and
+  // anchor in their fixed frame.
+  for (int m = 1; m < n_entries; ++m) {
+    PairState& pm = members[m];
+    double* sXfm = sL + 28;  // entry m's Xf, 3x4
+    if (lane < 12) sXfm[lane] = pm.Xf[lane];
+    __syncthreads();
+    auto xfm_inv = [&](int i, int k) { return sXfm[k * 4 + i]; };
+    auto xfm_inv_t = [&](int i) {
+      return -((sXfm[0 * 4 + i] * sXfm[3] + sXfm[1 * 4 + i] * sXfm[7]) + sXfm[2 * 4 + i] * sXfm[11]);
+    };
+    float* m_Xm = reinterpret_cast<float*>(&pm.cache.Xm);
+    float* m_Xmf = reinterpret_cast<float*>(&pm.cache.Xmf);
+    if (lane < 12) {
+      double mf = (xfm_inv(ei, 0) * sXc[0 * 4 + ej] + xfm_inv(ei, 1) * sXc[1 * 4 + ej]) + xfm_inv(ei, 2) * sXc[2 * 4 + ej];
+      if (ej == 3) mf = mf + xfm_inv_t(ei);
+      m_Xmf[cache_slot] = (float)mf;
+      m_Xm[cache_slot] = (float)sXc[lane];
+    }
+    if (CT && lane < 6) pm.cache.v[lane] = (float)(stats_mode ? S.v[lane] : S.v[lane] + sx[6 + lane]);
+    if (lane < 3) {
+      double cf = 0.0;
+      if (sp.recenter) cf = ((xfm_inv_t(lane) + xfm_inv(lane, 0) * sC[0]) + xfm_inv(lane, 1) * sC[1]) + xfm_inv(lane, 2) * sC[2];
+      (&pm.cache.cf.x)[lane] = (float)cf;
+    }
+    __syncthreads();
+  }
   if (tid != 0) return;
 
   st.pert_pose_norm = sqrt(pn);
@@ -512,6 +545,10 @@ __device__ void finish_pair(PairState& ps, const PairState& S, const float* __re
   if (stop && status == 0) status = 1;
   ps.status = status;
   if (status_out) status_out[0] = status;
+  for (int m = 1; m < n_entries; ++m) {
+    members[m].iter = it + 1;
+    members[m].status = status;
+  }
 }
 
 // ---- the tile kernel --------------------------------------------------------
@@ -708,7 +745,8 @@ __global__ __launch_bounds__(64) void finish_kernel(PairState* __restrict__ pair
   for (int i = threadIdx.x; i < kStateVec4; i += 64) s_state[i] = src[i];
   __syncthreads();
   const PairState& S = *reinterpret_cast<const PairState*>(s_state);
-  if ((S.status != 0) | (S.ct != (CT ? 1 : 0)) | (S.n_tiles == 0)) return;
+  // forest entries other than the leader are finished by their leader
+  if ((S.status != 0) | (S.ct != (CT ? 1 : 0)) | (S.group_n == 0) | (S.group_tiles == 0)) return;
   finish_pair<CT>(ps, S, partials, s_area, tf >= 0 ? tf : S.tile_first, status_out ? status_out + p : nullptr);
 }
 

@@ -298,6 +298,7 @@ static void engine(kdh_server* S) {
         memcpy(&out.last, &ps.stats_log[ps.iter - 1], sizeof(kdh_stats));
       else
         memset(&out.last, 0, sizeof(kdh_stats));
+      out.entry = out.last;
       out.result = kdh_result_code(ps, r->rq.params.max_iterations);
       if (r->moving) kdh_tree_free(ctx, r->moving);
       r->moving = nullptr;

@@ -842,13 +842,19 @@ void orc_update_cache(const float* Xm, const float* Xf, float* Xmf) {
 // (tree_aligner_base_.h:100-115, icp_impl_.h:53-90, cticp_impl_.h:48-92).
 // X: moving pose in/out (3x4 float), v: velocities in/out (CT), Xf: fixed
 // pose.  prior_* may be NULL.  stats_log: max_iterations entries.
-int orc_compute(int ct, const void* fixed_nodes, int64_t n_fixed, const void* moving_nodes, const uint64_t* leaves,
-                int64_t n_leaves, float* X, float* v, const float* Xf, const orc_params* p,
-                const float* prior_ZXi, const float* prior_info, int stats_mode, orc_stats* stats_log, int* n_log,
-                int* result) {
+// TreeAlignerBase_::compute with ICP_/CTICP_::oneRound over n_entries fixed
+// trees (tree_aligner_base_.h:100-115, icp_impl_.h:53-90, cticp_impl_.h:48-92):
+// with one entry the plain buildQuadraticForm; with a fixed forest every
+// entry's form is built at its own fixed pose (buildQuadraticForm(FixedEntry&),
+// icp_impl_.h:30-40) and H, b and the stats are summed in entry order in FP32.
+// entry_stats (may be NULL) receives each entry's stats of the last round.
+static int compute_impl(int ct, int n_entries, const void* const* fixed_nodes, const int64_t* n_fixed,
+                        const float* Xf, bool forest, const void* moving_nodes, const uint64_t* leaves,
+                        int64_t n_leaves, float* X, float* v, const orc_params* p, const float* prior_ZXi,
+                        const float* prior_info, int stats_mode, orc_stats* stats_log, int* n_log, int* result,
+                        orc_stats* entry_stats) {
   Params P = to_params(p);
   Iso3<float> Xs = from34<float>(X);
-  Iso3<float> Xfs = from34<float>(Xf);
   float vel[6];
   for (int i = 0; i < 6; ++i) vel[i] = (ct && v) ? v[i] : 0.0f;
   const int D = ct ? 12 : 6;
@@ -856,28 +862,52 @@ int orc_compute(int ct, const void* fixed_nodes, int64_t n_fixed, const void* mo
   float prev_score = 0;
   int ret = P.max_iterations;
   int nl = 0;
+  const Node* mv = (const Node*)moving_nodes;
   for (int it = 0; it < max_iter; ++it) {
-    Cache C;
-    C.Xm = Xs;
-    C.Xf = Xfs;
-    C.Xmf = compose(inverse(Xfs), Xs);
-    for (int i = 0; i < 6; ++i) C.v[i] = vel[i];
-    Form f;
-    const Node* fx = (const Node*)fixed_nodes;
-    const Node* mv = (const Node*)moving_nodes;
-    if (ct)
-      build_form<true>(fx, n_fixed, mv, leaves, n_leaves, C, P, stats_mode != 0, p->num_threads, f, nullptr, nullptr);
-    else
-      build_form<false>(fx, n_fixed, mv, leaves, n_leaves, C, P, stats_mode != 0, p->num_threads, f, nullptr,
-                        nullptr);
+    float H[144] = {0}, bv[12] = {0};
     orc_stats st;
-    stats_out(f, &st);
+    memset(&st, 0, sizeof(st));
+    for (int e = 0; e < n_entries; ++e) {
+      const Iso3<float> Xfs = from34<float>(Xf + 12 * e);
+      Cache C;
+      C.Xm = Xs;
+      C.Xf = Xfs;
+      C.Xmf = compose(inverse(Xfs), Xs);
+      for (int i = 0; i < 6; ++i) C.v[i] = vel[i];
+      Form f;
+      const Node* fx = (const Node*)fixed_nodes[e];
+      if (ct)
+        build_form<true>(fx, n_fixed[e], mv, leaves, n_leaves, C, P, stats_mode != 0, p->num_threads, f, nullptr,
+                         nullptr);
+      else
+        build_form<false>(fx, n_fixed[e], mv, leaves, n_leaves, C, P, stats_mode != 0, p->num_threads, f, nullptr,
+                          nullptr);
+      orc_stats se;
+      stats_out(f, &se);
+      if (entry_stats) entry_stats[e] = se;
+      if (!forest) {
+        memcpy(H, f.H, sizeof(float) * D * D);
+        memcpy(bv, f.b, sizeof(float) * D);
+        st = se;
+      } else {
+        // temp_H += entry.H; temp_b += entry.b; temp_stats += entry.stats
+        for (int i = 0; i < D * D; ++i) H[i] += f.H[i];
+        for (int i = 0; i < D; ++i) bv[i] += f.b[i];
+        st.num_inliers += se.num_inliers;
+        st.num_outliers += se.num_outliers;
+        st.num_bad += se.num_bad;
+        st.num_err_normal += se.num_err_normal;
+        st.num_err_fn += se.num_err_fn;
+        st.chi2 = (float)st.chi2 + (float)se.chi2;
+        st.chi2_kernelized = (float)st.chi2_kernelized + (float)se.chi2_kernelized;
+      }
+    }
     if (prior_ZXi && prior_info) {
       float Hp[36], bp[6];
       prior_form(from34<float>(prior_ZXi), Xs, prior_info, Hp, bp);
       for (int r = 0; r < 6; ++r) {
-        f.b[r] += bp[r];
-        for (int c = 0; c < 6; ++c) f.H[r * D + c] += Hp[r * 6 + c];
+        bv[r] += bp[r];
+        for (int c = 0; c < 6; ++c) H[r * D + c] += Hp[r * 6 + c];
       }
     }
     const int good = st.num_outliers + st.num_inliers;
@@ -887,10 +917,10 @@ int orc_compute(int ct, const void* fixed_nodes, int64_t n_fixed, const void* mo
       break;
     }
     float A[144], rhs[12], dx[12];
-    for (int i = 0; i < D * D; ++i) A[i] = f.H[i];
+    for (int i = 0; i < D * D; ++i) A[i] = H[i];
     for (int i = 0; i < D; ++i) {
       A[i * D + i] += (i < 6) ? P.damping : P.velocity_damping;
-      rhs[i] = -f.b[i];
+      rhs[i] = -bv[i];
     }
     ldlt_solve<float>(D, A, rhs, dx);
     float pn = 0, vn = 0;
@@ -928,6 +958,26 @@ int orc_compute(int ct, const void* fixed_nodes, int64_t n_fixed, const void* mo
   if (n_log) *n_log = nl;
   if (result) *result = ret;
   return 0;
+}
+
+int orc_compute(int ct, const void* fixed_nodes, int64_t n_fixed, const void* moving_nodes, const uint64_t* leaves,
+                int64_t n_leaves, float* X, float* v, const float* Xf, const orc_params* p,
+                const float* prior_ZXi, const float* prior_info, int stats_mode, orc_stats* stats_log, int* n_log,
+                int* result) {
+  const void* fx[1] = {fixed_nodes};
+  return compute_impl(ct, 1, fx, &n_fixed, Xf, false, moving_nodes, leaves, n_leaves, X, v, p, prior_ZXi,
+                      prior_info, stats_mode, stats_log, n_log, result, nullptr);
+}
+
+// compute() with a fixed forest (_fixed_forest, makeFixedEntry; icp_.h:90-95):
+// n_entries fixed trees with their poses Xf (n_entries x 12).
+int orc_compute_forest(int ct, int n_entries, const void* const* fixed_nodes, const int64_t* n_fixed,
+                       const float* Xf, const void* moving_nodes, const uint64_t* leaves, int64_t n_leaves,
+                       float* X, float* v, const orc_params* p, const float* prior_ZXi, const float* prior_info,
+                       int stats_mode, orc_stats* stats_log, int* n_log, int* result, orc_stats* entry_stats) {
+  if (n_entries < 1) return -1;
+  return compute_impl(ct, n_entries, fixed_nodes, n_fixed, Xf, true, moving_nodes, leaves, n_leaves, X, v, p,
+                      prior_ZXi, prior_info, stats_mode, stats_log, n_log, result, entry_stats);
 }
 
 // FP64 yardstick for whole alignments: the reference's FP32 discrete

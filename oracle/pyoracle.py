@@ -64,6 +64,11 @@ def lib(fast=False):
         L.orc_compute.restype = C.c_int
         L.orc_compute.argtypes = [C.c_int, vp, C.c_int64, vp, vp, C.c_int64, f32, f32, f32, C.POINTER(OrcParams), f32,
                                   f32, C.c_int, C.POINTER(OrcStats), C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        L.orc_compute_forest.restype = C.c_int
+        L.orc_compute_forest.argtypes = [C.c_int, C.c_int, C.POINTER(vp), C.POINTER(C.c_int64), f32, vp, vp,
+                                         C.c_int64, f32, f32, C.POINTER(OrcParams), f32, f32, C.c_int,
+                                         C.POINTER(OrcStats), C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                         C.POINTER(OrcStats)]
         L.orc_build_at.restype = C.c_int
         L.orc_build_at.argtypes = [C.c_int, vp, C.c_int64, vp, vp, C.c_int64, f64, f64, f64, C.POINTER(OrcParams),
                                    f32, f32, f64, f64, i32]
@@ -167,6 +172,32 @@ def compute(fixed, moving, X0, Xf, params, ct=0, v0=None, prior_ZXi=None, prior_
     Xo = np.eye(4)
     Xo[:3, :4] = X.reshape(3, 4).astype(np.float64)
     return Xo, v.astype(np.float64), [log[i].as_dict() for i in range(nlog.value)], res.value
+
+
+def compute_forest(fixed_list, Xf_list, moving, X0, params, ct=0, v0=None, prior_ZXi=None, prior_info=None,
+                   stats_mode=False, fast=False):
+    """compute() against a fixed forest (one tree and pose per entry), FP32.
+    Returns (X 4x4, v, stats_log, result, per-entry stats of the last round)."""
+    n = len(fixed_list)
+    X = _f32(np.asarray(X0, dtype=np.float64)[:3, :4].reshape(12))
+    v = _f32(np.zeros(6) if v0 is None else v0)
+    Xf = _f32(np.concatenate([np.asarray(x, dtype=np.float64)[:3, :4].reshape(12) for x in Xf_list]))
+    nodes = (C.c_void_p * n)(*[t.nodes.ctypes.data for t in fixed_list])
+    counts = (C.c_int64 * n)(*[len(t.nodes) for t in fixed_list])
+    m = max(1, params.max_iterations)
+    log = (OrcStats * m)()
+    ent = (OrcStats * n)()
+    nlog, res = C.c_int(), C.c_int()
+    pz = None if prior_ZXi is None else _f32(prior_ZXi)
+    pinf = None if prior_info is None else _f32(prior_info)
+    lib(fast).orc_compute_forest(ct, n, nodes, counts, p32(Xf), moving.nodes.ctypes.data, moving.leaves.ctypes.data,
+                                 len(moving.leaves), p32(X), p32(v), C.byref(params),
+                                 None if pz is None else p32(pz), None if pinf is None else p32(pinf),
+                                 int(stats_mode), log, C.byref(nlog), C.byref(res), ent)
+    Xo = np.eye(4)
+    Xo[:3, :4] = X.reshape(3, 4).astype(np.float64)
+    return (Xo, v.astype(np.float64), [log[i].as_dict() for i in range(nlog.value)], res.value,
+            [ent[i].as_dict() for i in range(n)])
 
 
 def build_form_at(fixed, moving, X, Xf, params, ct=0, v=None, fast=False):

@@ -309,3 +309,29 @@ def test_fp64_loop_agrees_with_fp32_loop_near_origin():
     t, r = synth.pose_error(X64, w["X_gt"])
     t0, r0 = synth.pose_error(w["X0"], w["X_gt"])
     assert t < t0 / 5 and r < r0 / 5
+
+
+def test_forest_of_one_is_the_plain_compute_and_blocks_recover_the_pose():
+    """A fixed forest of one entry is the plain compute bit for bit; the
+    config-1 map cut into three re-framed slabs moves the pose most of the way
+    to the generating one (every moving leaf is matched in every slab, so
+    slabs add false matches near their cuts: the forest lands a few cm off,
+    where the whole map lands within 2 mm; cutting in the anchor frame instead
+    gives the same result to 1 mm, so the framing is not the cause)."""
+    from kd_slam2_amd import synth
+
+    from .workloads import forest_cfg1, icp_params
+
+    f = forest_cfg1()
+    w = f["w"]
+    p = O.from_kdh_params(icp_params(), 8)
+    a = O.compute(w["fixed"], w["moving"], w["X0"], w["Xf"], p, fast=True)
+    b = O.compute_forest([w["fixed"]], [w["Xf"]], w["moving"], w["X0"], p, fast=True)
+    assert np.array_equal(a[0], b[0]) and a[2] == b[2] and a[3] == b[3]
+    X, _, log, res, ent = O.compute_forest(f["trees"], f["poses"], w["moving"], w["X0"], p, fast=True)
+    assert res == 10 and len(ent) == 3
+    for k in ("num_inliers", "num_outliers", "num_bad", "num_err_normal", "num_err_fn"):
+        assert sum(e[k] for e in ent) == log[-1][k]
+    t, r = synth.pose_error(X, w["X_gt"])
+    t0, r0 = synth.pose_error(w["X0"], w["X_gt"])
+    assert t < t0 / 5 and r < r0 / 5

@@ -40,3 +40,25 @@ def rel_err(A, B):
     A = np.asarray(A, dtype=np.float64)
     B = np.asarray(B, dtype=np.float64)
     return float(np.abs(A - B).max() / max(np.abs(B).max(), 1e-30))
+
+
+@functools.lru_cache(maxsize=None)
+def forest_cfg1(n_blocks=3):
+    """The config-1 map cut into n_blocks slabs along x, each re-expressed in
+    its own frame F_e (a shift to the slab centre and a yaw of 10 deg x e) and
+    given the fixed pose Xf @ F_e: a fixed forest covering the same scene."""
+    w = cfg1()
+    M = w["map"]
+    edges = np.quantile(M[:, 0], np.linspace(0.0, 1.0, n_blocks + 1))
+    g = TreeGenerator3D(leaf_max_points=16)
+    trees, poses = [], []
+    for e in range(n_blocks):
+        sel = (M[:, 0] >= edges[e]) & ((M[:, 0] < edges[e + 1]) if e < n_blocks - 1 else True)
+        P = M[sel].astype(np.float64)
+        a = np.radians(10.0 * e)
+        R = np.array([[np.cos(a), -np.sin(a), 0.0], [np.sin(a), np.cos(a), 0.0], [0.0, 0.0, 1.0]])
+        F = synth.pose(R, P.mean(axis=0))
+        local = (P - F[:3, 3]) @ F[:3, :3]  # F^-1 p
+        trees.append(g.makeTree(local.astype(np.float32)))
+        poses.append(w["Xf"] @ F)
+    return dict(w=w, trees=trees, poses=poses)
