@@ -4,7 +4,9 @@
 #include <math.h>
 #include <string.h>
 
+#include <fstream>
 #include <mutex>
+#include <sstream>
 #include <stdexcept>
 
 #include "../csrc/kdh_solve.h"
@@ -137,7 +139,91 @@ void AlignerHIP3D<CT>::syncParams() {
     params.term_d_pert = tc->param_d_pert.value();
     params.term_d_score = tc->param_d_score.value();
   }
+  // log_file (TreeAlignerBase_::syncParams, tree_aligner_base_.h:77-84)
+  const std::string& lf = param_log_file.value();
+  if (lf.empty()) {
+    _log_stream.reset();
+    _log_path.clear();
+  } else if (!_log_stream || lf != _log_path) {
+    _log_stream.reset(new std::ofstream(lf));
+    _log_path = lf;
+  }
   _params_changed = false;
+}
+
+// getPoseHessian (icp/icp_.h:47, cticp/cticp_.h:169-177, 226): the pose block
+// of H; for CT-ICP the velocities are marginalised out by the Schur
+// complement H_pp - H_pv (H_vv + 1e-9 I)^-1 H_pv^T (FP64 here).
+template <int CT>
+Matrix6f AlignerHIP3D<CT>::getPoseHessian() const {
+  Matrix6f out{};
+  double P[36];
+  for (int r = 0; r < 6; ++r)
+    for (int c = 0; c < 6; ++c) P[r * 6 + c] = H[r * D + c];
+  if (CT) {
+    // columns of (H_vv + 1e-9 I)^-1 H_pv^T, one LDL^T solve per pose row
+    double Hvv[36];
+    for (int r = 0; r < 6; ++r)
+      for (int c = 0; c < 6; ++c) Hvv[r * 6 + c] = H[(6 + r) * D + 6 + c] + (r == c ? 1e-9 : 0.0);
+    for (int j = 0; j < 6; ++j) {
+      double rhs[6], y[6];
+      for (int k = 0; k < 6; ++k) rhs[k] = H[j * D + 6 + k];  // row j of H_pv
+      kdh::ldlt_solve_d<6>(Hvv, rhs, y);
+      for (int i = 0; i < 6; ++i) {
+        double s = 0;
+        for (int k = 0; k < 6; ++k) s += H[i * D + 6 + k] * y[k];
+        P[i * 6 + j] -= s;
+      }
+    }
+  }
+  for (int i = 0; i < 36; ++i) out[i] = (float)P[i];
+  return out;
+}
+
+// saveStartState (icp/icp_.h:142-145, cticp/cticp_.h:248-251)
+template <int CT>
+void AlignerHIP3D<CT>::saveStartState() {
+  memcpy(_start_X, X, sizeof(X));
+  memcpy(_start_Xf, Xf, sizeof(Xf));
+  memcpy(_start_v, v, sizeof(v));
+}
+
+// iterationLogHeader (icp/icp_impl_.h:155-169, cticp/cticp_impl_.h:296-310):
+// fixed/moving poses as logmap and velocities, at the start and now.
+template <int CT>
+std::string AlignerHIP3D<CT>::iterationLogHeader() const {
+  auto vec = [](const double* x, int n) {
+    std::ostringstream os;
+    for (int i = 0; i < n; ++i) os << (i ? " " : "") << (float)x[i];
+    return os.str();
+  };
+  auto lm = [&](const double* X34) {
+    double e[6];
+    kdh::logmap_d(kdh::iso_from_34<double>(X34), e);
+    return vec(e, 6);
+  };
+  const double zero[6] = {0, 0, 0, 0, 0, 0};
+  std::ostringstream os;
+  os << (CT ? "CTICP" : "ICP") << 3 << "D| START "
+     << " fp: " << lm(_start_Xf) << " fv: " << (CT ? vec(zero, 6) : "none")
+     << " mp: " << lm(_start_X) << " mv: " << (CT ? vec(_start_v, 6) : "none") << " | END "
+     << " fp: " << lm(Xf) << " fv: " << (CT ? vec(zero, 6) : "none")
+     << " mp: " << lm(X) << " mv: " << (CT ? vec(v, 6) : "none");
+  return os.str();
+}
+
+// dumpIterationLog (tree_aligner_base_.h:118-124)
+template <int CT>
+void AlignerHIP3D<CT>::dumpIterationLog() {
+  if (!_log_stream) return;
+  *_log_stream << iterationLogHeader() << std::endl;
+  for (size_t i = 0; i < stats_log.size(); ++i) {
+    const ICPStats& s = stats_log[i];
+    *_log_stream << "  iter: " << i << " stats: k2: " << s.chi2 << " kk: " << s.chi2_kernelized
+                 << " ni: " << s.num_inliers << " no: " << s.num_outliers << " nb: " << s.num_bad
+                 << " nn: " << s.num_err_normal << " nf: " << s.num_err_fn << " pp: " << s.pert_pose_norm
+                 << " pv: " << s.pert_vel_norm << std::endl;
+  }
 }
 
 template <int CT>
@@ -286,6 +372,7 @@ template <int CT>
 int AlignerHIP3D<CT>::compute(bool stats_mode) {
   syncParams();
   stats_log.clear();
+  saveStartState();
   const int max_iter = stats_mode ? 1 : param_max_iterations.value();
   for (int i = 0; i < max_iter; ++i) {
     bool ok = oneRound(stats_mode);
@@ -300,6 +387,7 @@ int AlignerHIP3D<CT>::compute(bool stats_mode) {
 template <int CT>
 int AlignerHIP3D<CT>::computeOnDevice(bool stats_mode) {
   syncParams();
+  saveStartState();
   if (!_fixed_forest.empty()) {
     // one round without update per entry: the per-call path
     if (stats_mode) return compute(true);

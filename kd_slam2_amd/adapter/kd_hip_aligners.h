@@ -148,6 +148,11 @@ struct AlignerHIP3D : public srrg2_core::Configurable {
   // std::map: entries are summed in key order, as the reference iterates it
   std::map<int, std::shared_ptr<FixedEntry>> _fixed_forest;
 
+  Matrix6f getPoseHessian() const;
+  void saveStartState();
+  std::string iterationLogHeader() const;
+  void dumpIterationLog();
+
   void syncParams();
   void buildQuadraticForm(bool stats_mode = false);
   void buildQuadraticForm(FixedEntry& entry, bool stats_mode = false);
@@ -173,6 +178,11 @@ struct AlignerHIP3D : public srrg2_core::Configurable {
   const kdh_tree* _moving = nullptr;
   bool _has_prior = false;
   kdh_prior _prior{};
+  double _start_X[12] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0};
+  double _start_Xf[12] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0};
+  double _start_v[6] = {0, 0, 0, 0, 0, 0};
+  std::unique_ptr<std::ostream> _log_stream;
+  std::string _log_path;
 };
 
 struct ICPHIP3D : public AlignerHIP3D<0> {};

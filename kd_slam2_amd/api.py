@@ -264,6 +264,30 @@ class _AlignerBase:
     def setFixedX(self, X):
         self.Xf = np.array(X, dtype=np.float64)
 
+    def getPoseHessian(self):
+        """Pose information of the last round (icp/icp_.h:47; for CT-ICP the
+        Schur complement marginalising the velocities, cticp/cticp_.h:169-177)."""
+        H = np.asarray(self.H, dtype=np.float64)
+        if not self.CT:
+            return H.copy()
+        Hpv = H[:6, 6:]
+        return H[:6, :6] - Hpv @ np.linalg.solve(H[6:, 6:] + 1e-9 * np.eye(6), Hpv.T)
+
+    def saveStartState(self):
+        """saveStartState (icp/icp_.h:142-145): the poses at compute() start."""
+        self._start = (self.X.copy(), self.Xf.copy(), self.v.copy())
+
+    def iterationLogHeader(self):
+        """iterationLogHeader (icp/icp_impl_.h:155-169, cticp/cticp_impl_.h:296-310)."""
+        from .synth import logmap
+
+        X0, Xf0, v0 = getattr(self, "_start", (self.X, self.Xf, self.v))
+        fmt = lambda a: " ".join(f"{float(x):g}" for x in a)  # noqa: E731
+        fv = fmt(np.zeros(6)) if self.CT else "none"
+        return (f"{'CTICP' if self.CT else 'ICP'}3D| START  fp: {fmt(logmap(Xf0))} fv: {fv} mp: {fmt(logmap(X0))}"
+                f" mv: {fmt(v0) if self.CT else 'none'} | END  fp: {fmt(logmap(self.Xf))} fv: {fv}"
+                f" mp: {fmt(logmap(self.X))} mv: {fmt(self.v) if self.CT else 'none'}")
+
     def makeFixedEntry(self, tree, Xf):
         """FixedEntry of the fixed forest (icp/icp_.h:90-95): a fixed tree at
         its own pose.  Store it as aligner._fixed_forest[ref] = entry."""
@@ -380,6 +404,7 @@ class _AlignerBase:
     def compute(self, stats_mode=False):
         """TreeAlignerBase_::compute (tree_aligner_base_.h:100-115), host loop."""
         self.stats_log = []
+        self.saveStartState()
         max_iter = 1 if stats_mode else self.params.max_iterations
         prev_score = 0.0
         for i in range(max_iter):

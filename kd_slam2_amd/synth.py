@@ -45,6 +45,16 @@ def expmap(v):
     return pose(rodrigues(v[3:]), v[:3])
 
 
+def logmap(X):
+    """logmap of the reference (geometry_3d_impl_.h:48-115): [t; axis*angle]."""
+    R = np.asarray(X)[:3, :3]
+    c = min(1.0, max(-1.0, 0.5 * (np.trace(R) - 1.0)))
+    a = math.acos(c)
+    ax = np.array([R[2, 1] - R[1, 2], R[0, 2] - R[2, 0], R[1, 0] - R[0, 1]])
+    w = 0.5 * ax if a < 1e-7 else ax * (a / (2.0 * math.sin(a)))
+    return np.concatenate([np.asarray(X)[:3, 3], w])
+
+
 def to34(X):
     return np.ascontiguousarray(np.asarray(X)[:3, :4], dtype=np.float64).reshape(12)
 
