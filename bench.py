@@ -32,6 +32,7 @@ Beside it the line reports
 import argparse
 import csv
 import ctypes
+import gc
 import glob
 import json
 import multiprocessing as mp
@@ -217,8 +218,8 @@ def _cfg2_params(default_params):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--batch", type=int, default=32, help="concurrent alignments per served step")
     ap.add_argument("--plain-runs", type=int, default=5)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
@@ -297,17 +298,21 @@ def main():
 
     # ---- plain per-call path: CTICPHIP3D::compute() of the C++ plugin -----
     plain_lat = []
+    gc.collect()
+    gc.disable()
     for i in range(args.plain_runs + 1):
         t = time.perf_counter()
         adapter_align("CTICPHIP3D", ctx, F, M, w["X0"], w["Xf"], p, v0=w["v0"])
         dt = time.perf_counter() - t
         if i > 0:
             plain_lat.append(dt)
+    gc.enable()
     plain_pps = n_pts / np.median(plain_lat)
     log(f"plain per-call: median {1e3 * np.median(plain_lat):.2f} ms/alignment, {plain_pps / 1e6:.1f} Mpts/s")
 
     # ---- device-resident loop, one alignment ------------------------------
     dev_lat, kern_us = [], []
+    gc.disable()
     for i in range(args.plain_runs + 1):
         al = aligner()
         t = time.perf_counter()
@@ -316,6 +321,7 @@ def main():
         if i > 0:
             dev_lat.append(dt)
             kern_us.append(ctx.last_kernel_time()[0])
+    gc.enable()
     dev_pps = n_pts / np.median(dev_lat)
     log(f"device loop: median {1e3 * np.median(dev_lat):.2f} ms/alignment, {np.median(kern_us):.1f} us/iter")
 
@@ -325,17 +331,42 @@ def main():
                   params=p) for _ in range(args.batch)]
     for _ in range(args.warmup):
         batch_align(ctx, pairs)
+    # The Python collector stays off inside timed regions: a full collection
+    # of the workload's large object graph would be timed as kernel work.
+    gc.collect()
+    gc.disable()
     step_t, batch_kern = [], []
     for _ in range(args.steps):
         t = time.perf_counter()
         out = batch_align(ctx, pairs)
         step_t.append(time.perf_counter() - t)
         batch_kern.append(ctx.last_kernel_time()[0])
+    gc.enable()
     total = float(np.sum(step_t))
     value = args.batch * n_pts * args.steps / total
     errs = [synth.pose_error(o["X"], w["X_gt"]) for o in out]
+    slow = [f"{i}:{1e3 * x:.2f}" for i, x in enumerate(step_t) if x > 3 * np.median(step_t)]
+    log(f"served steps: median {1e3 * np.median(step_t):.3f} ms, slow (>3x median) {slow}")
     log(f"served: {1e3 * np.mean(step_t):.2f} ms/step of {args.batch}, {value / 1e6:.1f} Mpts/s, "
         f"{np.median(batch_kern):.1f} us/iter, max err {max(e[0] for e in errs):.4f} m")
+
+    # the same served step at other batch sizes (not the headline)
+    sweep = {}
+    for nb in (1, 8, 16, 64, 128):
+        pb = [dict(pairs[i % len(pairs)]) for i in range(nb)]
+        batch_align(ctx, pb)
+        gc.disable()
+        ts = []
+        for _ in range(5):
+            t = time.perf_counter()
+            batch_align(ctx, pb)
+            ts.append(time.perf_counter() - t)
+        gc.enable()
+        sweep[str(nb)] = dict(ms_per_step=round(1e3 * float(np.median(ts)), 3),
+                              pts_per_s=round(nb * n_pts / float(np.median(ts)), 1))
+    sweep[str(args.batch)] = dict(ms_per_step=round(1e3 * float(np.median(step_t)), 3),
+                                  pts_per_s=round(args.batch * n_pts / float(np.median(step_t)), 1))
+    log("batch sweep: " + ", ".join(f"{k}: {v['pts_per_s'] / 1e6:.0f} Mpts/s" for k, v in sweep.items()))
 
     # roofline of the tile kernel in the served launch (profiled batches: no
     # graph, HIP events around 4 back-to-back tile launches and the finish)
@@ -419,6 +450,7 @@ def main():
                     served_batch=args.batch, moving_leaves=int(moving.num_leaves),
                     fixed_nodes=int(fixed.num_nodes), mean_depth=round(float(np.mean(depth)), 2)),
         served_speedup_vs_plain=round(value / plain_pps, 2),
+        served_batch_sweep=sweep,
         plain_pts_per_s=round(plain_pps, 1),
         device_loop_pts_per_s=round(dev_pps, 1),
         latency_ms=dict(plain_p50=round(1e3 * float(np.percentile(plain_lat, 50)), 3),
@@ -609,6 +641,8 @@ def run_serving(ctx, seqs, loops, seconds, DeviceTree, Server, default_params):
                 submitted.append((rid, kind, npts))
 
     th = threading.Thread(target=submitter)
+    gc.collect()
+    gc.disable()
     t_start = time.perf_counter()
     th.start()
     results = []
@@ -627,6 +661,7 @@ def run_serving(ctx, seqs, loops, seconds, DeviceTree, Server, default_params):
             k += 1
     th.join()
     wall = time.perf_counter() - t_start
+    gc.enable()
     srv_info = srv.info()
     srv.close()
     for T in keep:

@@ -223,6 +223,26 @@ _SIGS = {
 }
 
 
+_NP_SCALARS = {C.c_void_p: np.uint64, C.c_int32: np.int32, C.c_int64: np.int64, C.c_float: np.float32,
+               C.c_double: np.float64}
+
+
+def np_dtype(struct):
+    """numpy dtype with the byte layout of a ctypes Structure (pointers as
+    uint64), so arrays of the C structs can be filled and read vectorised."""
+    names, formats, offsets = [], [], []
+    for name, ctype in struct._fields_:
+        names.append(name)
+        offsets.append(getattr(struct, name).offset)
+        if isinstance(ctype, type) and issubclass(ctype, C.Array):
+            formats.append((_NP_SCALARS[ctype._type_], (ctype._length_,)))
+        elif isinstance(ctype, type) and issubclass(ctype, C.Structure):
+            formats.append(np_dtype(ctype))
+        else:
+            formats.append(_NP_SCALARS[ctype])
+    return np.dtype(dict(names=names, formats=formats, offsets=offsets, itemsize=C.sizeof(struct)))
+
+
 def lib():
     """Load libkdhip.so (raises if it has not been built)."""
     global _lib

@@ -575,42 +575,59 @@ def adapter_align(type_name, ctx, fixed, moving, X0, Xf, params, v0=None, on_dev
     return X, vo, res.value, nit.value, st.as_dict()
 
 
+_PAIR_DT = None
+_RES_DT = None
+_I4 = np.eye(4)
+_Z6 = np.zeros(6)
+
+
+def _stats_dicts(rec):
+    names = [f for f, _ in L.Stats._fields_ if f != "pad"]
+    cols = {f: rec[f].tolist() for f in names}
+    return [{f: cols[f][i] for f in names} for i in range(len(rec))]
+
+
 def batch_align(ctx, pairs):
     """Run many alignments in one batched launch (kdh_batch_align).
 
     pairs: list of dicts with keys fixed, moving (DeviceTree), ct, X0, Xf, v0,
     params (L.Params), prior, forest (k > 0: this pair leads a fixed forest
     with the next k pairs).  Returns a list of dicts (X, v, iterations,
-    result, last, entry)."""
+    result, last, entry).  The C structs are filled and read through numpy
+    views, column by column."""
+    global _PAIR_DT, _RES_DT
+    if _PAIR_DT is None:
+        _PAIR_DT, _RES_DT = L.np_dtype(L.Pair), L.np_dtype(L.PairResult)
     n = len(pairs)
     arr = (L.Pair * n)()
-    keep = []
+    A = np.frombuffer(arr, dtype=_PAIR_DT)
+    A["fixed"] = [p["fixed"].h.value for p in pairs]
+    A["moving"] = [p["moving"].h.value for p in pairs]
+    A["ct"] = [int(p.get("ct", 0)) for p in pairs]
+    A["forest"] = [int(p.get("forest", 0)) for p in pairs]
+    A["X0"] = np.asarray([p["X0"] for p in pairs], dtype=np.float64)[:, :3, :4].reshape(n, 12)
+    A["Xf"] = np.asarray([p["Xf"] if "Xf" in p else _I4 for p in pairs], dtype=np.float64)[:, :3, :4].reshape(n, 12)
+    A["v0"] = np.asarray([p["v0"] if "v0" in p else _Z6 for p in pairs], dtype=np.float64).reshape(n, 6)
+    pd = _PAIR_DT["params"]
+    cache = {}
     for i, p in enumerate(pairs):
-        a = arr[i]
-        a.fixed = p["fixed"].h.value
-        a.moving = p["moving"].h.value
-        a.ct = int(p.get("ct", 0))
-        a.forest = int(p.get("forest", 0))
-        a.X0[:] = list(np.asarray(p["X0"])[:3, :4].reshape(12))
-        a.v0[:] = list(np.asarray(p.get("v0", np.zeros(6)), dtype=np.float64))
-        a.Xf[:] = list(np.asarray(p.get("Xf", np.eye(4)))[:3, :4].reshape(12))
-        a.params = p["params"]
-        pr = p.get("prior")
-        if pr is not None:
-            keep.append(pr)
-            a.prior = C.cast(C.pointer(pr), C.c_void_p).value
-        else:
-            a.prior = None
+        prm = p["params"]
+        rec = cache.get(id(prm))
+        if rec is None:
+            rec = cache[id(prm)] = np.frombuffer(bytes(prm), dtype=pd)[0]
+        A["params"][i] = rec
+    keep = [p["prior"] for p in pairs if p.get("prior") is not None]
+    if keep:
+        A["prior"] = [C.addressof(p["prior"]) if p.get("prior") is not None else 0 for p in pairs]
     res = (L.PairResult * n)()
     ctx.check(L.lib().kdh_batch_align(ctx.h, n, arr, res), "kdh_batch_align")
-    out = []
-    for i in range(n):
-        r = res[i]
-        X = np.eye(4)
-        X[:3, :4] = np.array(r.X[:]).reshape(3, 4)
-        out.append(dict(X=X, v=np.array(r.v[:]), iterations=r.iterations, result=r.result, last=r.last.as_dict(),
-                        entry=r.entry.as_dict()))
-    return out
+    R = np.frombuffer(res, dtype=_RES_DT)
+    Xs = np.tile(np.eye(4), (n, 1, 1))
+    Xs[:, :3, :4] = R["X"].reshape(n, 3, 4)
+    vs = R["v"].copy()
+    its, rs = R["iterations"].tolist(), R["result"].tolist()
+    last, entry = _stats_dicts(R["last"]), _stats_dicts(R["entry"])
+    return [dict(X=Xs[i], v=vs[i], iterations=its[i], result=rs[i], last=last[i], entry=entry[i]) for i in range(n)]
 
 
 # ---------------------------------------------------------------------------

@@ -117,8 +117,8 @@ int kdh_batch_align(kdh_ctx* ctx, int n_pairs, const kdh_pair* pairs, kdh_pair_r
     const Group& G = g[ps.ct];
     for (int t = 0; t < ps.n_tiles; ++t) tp[G.tile_off + ps.tile_first + t] = i;
   }
-  KDH_HIP_TRY(ctx, hipMemcpyAsync(ctx->d_tile_pair, tp.data(), tp.size() * sizeof(int), hipMemcpyHostToDevice,
-                                  ctx->stream));
+  rc = kdh_upload_tile_pair(ctx, tp);
+  if (rc) return rc;
   KDH_HIP_TRY(ctx, hipMemcpyAsync(ctx->d_pairs, ctx->h_pairs, sizeof(PairState) * n_pairs, hipMemcpyHostToDevice,
                                   ctx->stream));
 

@@ -94,6 +94,7 @@ int kdh_ctx_destroy(kdh_ctx* ctx) {
   if (ctx->d_corr) hipFree(ctx->d_corr);
   if (ctx->d_depth) hipFree(ctx->d_depth);
   if (ctx->h_pairs) hipHostFree(ctx->h_pairs);
+  if (ctx->h_tile_pair) hipHostFree(ctx->h_tile_pair);
   for (int i = 0; i < kdh_ctx::kEvents; ++i)
     if (ctx->ev[i]) hipEventDestroy(ctx->ev[i]);
   hipStreamDestroy(ctx->stream);
@@ -247,7 +248,28 @@ int kdh_ensure_partials(kdh_ctx* ctx, size_t n_floats) {
 }
 
 int kdh_ensure_tile_pair(kdh_ctx* ctx, size_t n) {
+  int* before = ctx->d_tile_pair;
   KDH_HIP_TRY(ctx, grow(&ctx->d_tile_pair, &ctx->tile_cap, n));
+  if (ctx->d_tile_pair != before) ctx->tile_pair_on_device.clear();
+  return KDH_OK;
+}
+
+int kdh_upload_tile_pair(kdh_ctx* ctx, const std::vector<int>& tp) {
+  if (tp == ctx->tile_pair_on_device) return KDH_OK;
+  if (tp.size() > ctx->h_tile_cap) {
+    if (ctx->h_tile_pair) hipHostFree(ctx->h_tile_pair);
+    ctx->h_tile_pair = nullptr;
+    ctx->h_tile_cap = 0;
+    const size_t nc = std::max(tp.size(), (size_t)4096);
+    KDH_HIP_TRY(ctx, hipHostMalloc((void**)&ctx->h_tile_pair, nc * sizeof(int), hipHostMallocDefault));
+    ctx->h_tile_cap = nc;
+  }
+  // the staging buffer may still feed an earlier copy on the stream
+  KDH_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  memcpy(ctx->h_tile_pair, tp.data(), tp.size() * sizeof(int));
+  KDH_HIP_TRY(ctx, hipMemcpyAsync(ctx->d_tile_pair, ctx->h_tile_pair, tp.size() * sizeof(int), hipMemcpyHostToDevice,
+                                  ctx->stream));
+  ctx->tile_pair_on_device = tp;
   return KDH_OK;
 }
 

@@ -5,6 +5,7 @@
 
 #include <mutex>
 #include <string>
+#include <vector>
 
 #include "../../include/kdh.h"
 #include "kdh_internal.h"
@@ -31,6 +32,11 @@ struct kdh_ctx {
   size_t partials_cap = 0;
   int* d_tile_pair = nullptr;
   size_t tile_cap = 0;
+  // pinned staging of the tile list, and the list the device holds now (an
+  // unchanged batch shape skips the upload)
+  int* h_tile_pair = nullptr;
+  size_t h_tile_cap = 0;
+  std::vector<int> tile_pair_on_device;
   int* d_corr = nullptr;
   size_t corr_cap = 0;
   int* d_depth = nullptr;
@@ -61,6 +67,9 @@ int kdh_hip_fail(kdh_ctx* ctx, hipError_t e, const char* where);
 int kdh_ensure_pairs(kdh_ctx* ctx, size_t n_pairs);
 int kdh_ensure_partials(kdh_ctx* ctx, size_t n_floats);
 int kdh_ensure_tile_pair(kdh_ctx* ctx, size_t n);
+// Uploads the tile list (through pinned staging) unless the device already
+// holds exactly this list.
+int kdh_upload_tile_pair(kdh_ctx* ctx, const std::vector<int>& tp);
 int kdh_fill_pair(kdh_ctx* ctx, kdh::PairState& ps, const kdh_tree* fixed, const kdh_tree* moving, int ct,
                   const kdh_params& p, int stats_mode, int mode);
 void kdh_cache_from_state(kdh::PairState& ps);
