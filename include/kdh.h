@@ -122,6 +122,28 @@ int kdh_tree_upload(kdh_ctx* ctx, const kdh_node3f* nodes, int64_t n_nodes,
 int kdh_tree_free(kdh_ctx* ctx, kdh_tree* tree);
 int kdh_tree_info(const kdh_tree* tree, int64_t* n_nodes, int64_t* n_leaves, int64_t* device_bytes);
 
+/* ---- tree maintenance on the device ---------------------------------------
+ * Replaces TreeCUDA_::applyVelocitiesInPlace / treeApplyVelocities_kernel
+ * (tree/tree_cuda_impl_.h:138-160): every node's mean and direction warped by
+ * R(w dts) and v dts with the node's own (float) _dts_mean, leaves skipped
+ * when skip_leaves (Node_::applyVelocities, tree/node_impl_.h:86-92).
+ * v = [t; w], 6 floats.  The re-centring anchor follows the root. */
+int kdh_tree_apply_velocities(kdh_ctx* ctx, kdh_tree* tree, const float v[6], int skip_leaves);
+
+/* Replaces TreeCUDA_::applyIsometryInPlace / treeApplyIsometry_kernel
+ * (tree/tree_cuda_impl_.h:162-173; applyIsometry_ on a node, tree/node_.h:127-137):
+ * mean <- iso * mean, direction <- R * direction.  iso: 3x4 row-major. */
+int kdh_tree_apply_isometry(kdh_ctx* ctx, kdh_tree* tree, const float iso[12]);
+
+/* Replaces TreeCUDA_::extractCompressedLeaves() and (v) (tree/tree_cuda_impl_.h:43-136):
+ * per leaf, in leaf-index order, mean then direction (out: 6 * n_leaves
+ * floats), warped by v when v is not NULL. */
+int kdh_tree_extract_leaves(kdh_ctx* ctx, const kdh_tree* tree, const float* v, float* out);
+
+/* Parity hook: every node's mean and direction as the device holds them
+ * (out: 6 * n_nodes floats).  No reference counterpart. */
+int kdh_tree_read_nodes(kdh_ctx* ctx, const kdh_tree* tree, float* out);
+
 /* ---- one quadratic form (the plain per-call path) -------------------------
  * Replaces ICP_CUDA_::_buildQuadraticForm (icp/icp_cuda_impl_.cuh:95-132):
  * for every leaf of `moving`, descend `fixed`, apply the gates, and reduce

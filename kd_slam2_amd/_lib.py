@@ -183,6 +183,10 @@ _SIGS = {
     "kdh_tree_upload": (C.c_int, [P, P, C.c_int64, P, C.c_int64, C.POINTER(P)]),
     "kdh_tree_free": (C.c_int, [P, P]),
     "kdh_tree_info": (C.c_int, [P, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+    "kdh_tree_apply_velocities": (C.c_int, [P, P, F32P, C.c_int]),
+    "kdh_tree_apply_isometry": (C.c_int, [P, P, F32P]),
+    "kdh_tree_extract_leaves": (C.c_int, [P, P, F32P, F32P]),
+    "kdh_tree_read_nodes": (C.c_int, [P, P, F32P]),
     "kdh_icp_build": (C.c_int, [P, P, P, F32P, F32P, F32P, C.POINTER(Params), C.c_int, F64P, F64P, C.POINTER(Stats)]),
     "kdh_build_at": (
         C.c_int,

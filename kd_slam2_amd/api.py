@@ -155,6 +155,35 @@ class DeviceTree:
         L.lib().kdh_tree_info(self.h, None, None, C.byref(nb))
         return nb.value
 
+    # ---- maintenance on the device (TreeCUDA_, tree/tree_cuda_impl_.h:43-173) ----
+    def applyVelocitiesInPlace(self, v, skip_leaves=False):
+        """Deskew every node (leaves too unless skip_leaves) by v = [t; w]."""
+        v32 = np.ascontiguousarray(v, dtype=np.float32)
+        self.ctx.check(L.lib().kdh_tree_apply_velocities(self.ctx.h, self.h, L.f32p(v32), int(skip_leaves)),
+                       "kdh_tree_apply_velocities")
+
+    def applyIsometryInPlace(self, X):
+        """Move the tree by the isometry X (4x4 or 3x4)."""
+        m = np.ascontiguousarray(np.asarray(X, dtype=np.float64)[:3, :4], dtype=np.float32).reshape(12)
+        self.ctx.check(L.lib().kdh_tree_apply_isometry(self.ctx.h, self.h, L.f32p(m)), "kdh_tree_apply_isometry")
+
+    def extractCompressedLeaves(self, v=None):
+        """(n_leaves, 2, 3) float32: per leaf its mean and direction, warped by
+        v when given."""
+        out = np.zeros((max(self.host.num_leaves, 0), 2, 3), dtype=np.float32)
+        vp = None if v is None else L.f32p(np.ascontiguousarray(v, dtype=np.float32))
+        if len(out):
+            self.ctx.check(L.lib().kdh_tree_extract_leaves(self.ctx.h, self.h, vp, L.f32p(out)),
+                           "kdh_tree_extract_leaves")
+        return out
+
+    def readNodes(self):
+        """(n_nodes, 2, 3) float32: every node's mean and direction on the device."""
+        out = np.zeros((len(self.host.nodes), 2, 3), dtype=np.float32)
+        if len(out):
+            self.ctx.check(L.lib().kdh_tree_read_nodes(self.ctx.h, self.h, L.f32p(out)), "kdh_tree_read_nodes")
+        return out
+
     def close(self):
         if self.h and self.ctx.h:
             L.lib().kdh_tree_free(self.ctx.h, self.h)

@@ -93,6 +93,7 @@ int kdh_ctx_destroy(kdh_ctx* ctx) {
   if (ctx->d_tile_pair) hipFree(ctx->d_tile_pair);
   if (ctx->d_corr) hipFree(ctx->d_corr);
   if (ctx->d_depth) hipFree(ctx->d_depth);
+  if (ctx->d_extract) hipFree(ctx->d_extract);
   if (ctx->h_pairs) hipHostFree(ctx->h_pairs);
   if (ctx->h_tile_pair) hipHostFree(ctx->h_tile_pair);
   for (int i = 0; i < kdh_ctx::kEvents; ++i)
@@ -146,7 +147,8 @@ int kdh_tree_upload(kdh_ctx* ctx, const kdh_node3f* nodes, int64_t n_nodes, cons
   }
   const size_t node_bytes = align256((size_t)std::max<int64_t>(n_nodes, 1) * 32);
   const size_t leaf_bytes = align256((size_t)std::max<int64_t>(n_leaves, 1) * 32);
-  std::vector<float> staging((node_bytes + leaf_bytes) / 4, 0.0f);
+  const size_t dts_bytes = align256((size_t)std::max<int64_t>(n_nodes, 1) * 4);
+  std::vector<float> staging((node_bytes + leaf_bytes + dts_bytes) / 4, 0.0f);
   float* sn = staging.data();
   for (int64_t i = 0; i < n_nodes; ++i) {
     const kdh_node3f& n = nodes[i];
@@ -166,12 +168,15 @@ int kdh_tree_upload(kdh_ctx* ctx, const kdh_node3f* nodes, int64_t n_nodes, cons
     int32_t idx = (int32_t)leaf_indices[i];
     memcpy(&o[7], &idx, 4);
   }
+  float* sd = staging.data() + (node_bytes + leaf_bytes) / 4;
+  for (int64_t i = 0; i < n_nodes; ++i) sd[i] = (float)nodes[i].dts_mean;  // Scalar_(dts_mean)
   std::lock_guard<std::mutex> lk(ctx->mu);
   KDH_HIP_TRY(ctx, hipSetDevice(ctx->device));
   void* d = nullptr;
-  hipError_t e = hipMalloc(&d, node_bytes + leaf_bytes);
+  const size_t total = node_bytes + leaf_bytes + dts_bytes;
+  hipError_t e = hipMalloc(&d, total);
   if (e != hipSuccess) return kdh_fail(ctx, KDH_ERR_NOMEM, std::string("kdh_tree_upload: hipMalloc: ") + hipGetErrorString(e));
-  e = hipMemcpyAsync(d, staging.data(), node_bytes + leaf_bytes, hipMemcpyHostToDevice, ctx->stream);
+  e = hipMemcpyAsync(d, staging.data(), total, hipMemcpyHostToDevice, ctx->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
   if (e != hipSuccess) {
     hipFree(d);
@@ -180,9 +185,10 @@ int kdh_tree_upload(kdh_ctx* ctx, const kdh_node3f* nodes, int64_t n_nodes, cons
   kdh_tree* t = new kdh_tree();
   t->device = ctx->device;
   t->dmem = d;
-  t->bytes = node_bytes + leaf_bytes;
+  t->bytes = total;
   t->dt.nodes = (const float4*)d;
   t->dt.leaves = (const float4*)((char*)d + node_bytes);
+  t->node_dts = (float*)((char*)d + node_bytes + leaf_bytes);
   t->dt.n_nodes = (int)n_nodes;
   t->dt.n_leaves = (int)n_leaves;
   t->root_mean = n_nodes > 0 ? v3(nodes[0].mean[0], nodes[0].mean[1], nodes[0].mean[2]) : v3(0.0f, 0.0f, 0.0f);

@@ -16,6 +16,7 @@ struct kdh_tree {
   size_t bytes = 0;
   kdh::DevTree dt{};
   kdh::V3f root_mean{};
+  float* node_dts = nullptr;  // per node (float)_dts_mean, for the in-place deskew
 };
 
 struct kdh_ctx {
@@ -41,6 +42,8 @@ struct kdh_ctx {
   size_t corr_cap = 0;
   int* d_depth = nullptr;
   size_t depth_cap = 0;
+  float* d_extract = nullptr;  // staging of tree read-backs
+  size_t extract_cap = 0;
   // Instantiated graphs, reused while the launch shape and buffers match.
   struct GraphSlot {
     hipGraphExec_t exec = nullptr;

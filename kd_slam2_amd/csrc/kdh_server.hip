@@ -97,8 +97,10 @@ struct kdh_server {
 
 static int64_t request_bytes(const kdh_request& r) {
   const int64_t tiles = (r.n_moving_leaves + kTile - 1) / kTile;
+  // the moving tree as kdh_tree_upload lays it out: node and leaf records, node stamps
   return (int64_t)(align256((size_t)std::max<int64_t>(r.n_moving_nodes, 1) * 32) +
-                   align256((size_t)std::max<int64_t>(r.n_moving_leaves, 1) * 32)) +
+                   align256((size_t)std::max<int64_t>(r.n_moving_leaves, 1) * 32) +
+                   align256((size_t)std::max<int64_t>(r.n_moving_nodes, 1) * 4)) +
          tiles * 4 * kCtSlots + (int64_t)sizeof(PairState);
 }
 

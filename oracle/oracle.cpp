@@ -1022,6 +1022,58 @@ int orc_compute64(int ct, const void* fixed_nodes, int64_t n_fixed, const void* 
   return 0;
 }
 
+// ---- tree maintenance (tree/tree_cuda_impl_.h:43-173 restated on Node_) ----
+
+// GeometryTraits_::applyVelocities (utils/geometry_3d_impl_.h:63-72) on one
+// node with its own Scalar_(_dts_mean) (Node_::applyVelocities, node_impl_.h:86-92).
+static void node_apply_velocities(Node& n, const float* v) {
+  const float dts = (float)n.dts_mean;
+  const Vec3<float> td = mk(v[0] * dts, v[1] * dts, v[2] * dts);
+  const Vec3<float> rd = mk(v[3] * dts, v[4] * dts, v[5] * dts);
+  const Mat3<float> R = rodrigues(rd);
+  const Vec3<float> m = add(matvec(R, mk(n.mean[0], n.mean[1], n.mean[2])), td);
+  const Vec3<float> d = matvec(R, mk(n.direction[0], n.direction[1], n.direction[2]));
+  for (int i = 0; i < 3; ++i) {
+    n.mean[i] = m[i];
+    n.direction[i] = d[i];
+  }
+}
+
+// treeApplyVelocities_kernel: every node, leaves skipped when skip_leaves.
+void orc_tree_apply_velocities(void* nodes, int64_t n_nodes, const float* v, int skip_leaves) {
+  Node* N = (Node*)nodes;
+  for (int64_t i = 0; i < n_nodes; ++i)
+    if (!(N[i].isLeaf() && skip_leaves)) node_apply_velocities(N[i], v);
+}
+
+// treeApplyIsometry_kernel: applyIsometry_ on every node (tree/node_.h:127-137).
+void orc_tree_apply_isometry(void* nodes, int64_t n_nodes, const float* iso) {
+  Node* N = (Node*)nodes;
+  const Iso3<float> X = from34<float>(iso);
+  for (int64_t i = 0; i < n_nodes; ++i) {
+    const Vec3<float> m = applyIsometry(X, mk(N[i].mean[0], N[i].mean[1], N[i].mean[2]));
+    const Vec3<float> d = applyRotation(X, mk(N[i].direction[0], N[i].direction[1], N[i].direction[2]));
+    for (int k = 0; k < 3; ++k) {
+      N[i].mean[k] = m[k];
+      N[i].direction[k] = d[k];
+    }
+  }
+}
+
+// extractCompressedLeaves(_kernel / Vel_kernel): per leaf index, mean then
+// direction, warped by v when v is not NULL.
+void orc_extract_leaves(const void* nodes, const uint64_t* leaves, int64_t n_leaves, const float* v, float* out) {
+  const Node* N = (const Node*)nodes;
+  for (int64_t i = 0; i < n_leaves; ++i) {
+    Node n = N[leaves[i]];
+    if (v) node_apply_velocities(n, v);
+    for (int k = 0; k < 3; ++k) {
+      out[6 * i + k] = n.mean[k];
+      out[6 * i + 3 + k] = n.direction[k];
+    }
+  }
+}
+
 // ---- pieces exposed for unit checks ---------------------------------------
 int orc_ldlt_solve_f(int n, const float* A, const float* rhs, float* x) {
   if (n < 1 || n > 12) return -1;

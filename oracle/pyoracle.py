@@ -69,6 +69,9 @@ def lib(fast=False):
                                          C.c_int64, f32, f32, C.POINTER(OrcParams), f32, f32, C.c_int,
                                          C.POINTER(OrcStats), C.POINTER(C.c_int), C.POINTER(C.c_int),
                                          C.POINTER(OrcStats)]
+        L.orc_tree_apply_velocities.argtypes = [vp, C.c_int64, f32, C.c_int]
+        L.orc_tree_apply_isometry.argtypes = [vp, C.c_int64, f32]
+        L.orc_extract_leaves.argtypes = [vp, vp, C.c_int64, f32, f32]
         L.orc_build_at.restype = C.c_int
         L.orc_build_at.argtypes = [C.c_int, vp, C.c_int64, vp, vp, C.c_int64, f64, f64, f64, C.POINTER(OrcParams),
                                    f32, f32, f64, f64, i32]
@@ -198,6 +201,30 @@ def compute_forest(fixed_list, Xf_list, moving, X0, params, ct=0, v0=None, prior
     Xo[:3, :4] = X.reshape(3, 4).astype(np.float64)
     return (Xo, v.astype(np.float64), [log[i].as_dict() for i in range(nlog.value)], res.value,
             [ent[i].as_dict() for i in range(n)])
+
+
+def tree_apply_velocities(nodes, v, skip_leaves=False):
+    """treeApplyVelocities_kernel on a copy of a Node_ array; returns it."""
+    out = np.array(nodes, copy=True)
+    lib().orc_tree_apply_velocities(out.ctypes.data, len(out), p32(_f32(v)), int(skip_leaves))
+    return out
+
+
+def tree_apply_isometry(nodes, X):
+    """treeApplyIsometry_kernel on a copy of a Node_ array; returns it."""
+    out = np.array(nodes, copy=True)
+    m = _f32(np.asarray(X, dtype=np.float64)[:3, :4].reshape(12))
+    lib().orc_tree_apply_isometry(out.ctypes.data, len(out), p32(m))
+    return out
+
+
+def extract_leaves(nodes, leaves, v=None):
+    """extractCompressedLeaves(v): (n_leaves, 2, 3) float32."""
+    n = len(leaves)
+    out = np.zeros((n, 2, 3), dtype=np.float32)
+    vv = None if v is None else _f32(v)
+    lib().orc_extract_leaves(nodes.ctypes.data, leaves.ctypes.data, n, None if vv is None else p32(vv), p32(out))
+    return out
 
 
 def build_form_at(fixed, moving, X, Xf, params, ct=0, v=None, fast=False):

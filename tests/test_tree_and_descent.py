@@ -161,3 +161,58 @@ def test_make_tree_returns_none_like_reference():
     assert g.makeTree(np.zeros((3, 3), dtype=np.float32)) is None  # < leaf_min_points
     rng = np.random.default_rng(3)
     assert g.makeTree(rng.normal(size=(12, 3)).astype(np.float32)) is None  # < tree_min_good_leaves
+
+
+# ---- tree maintenance restated in the oracle (tree/tree_cuda_impl_.h:43-173) ----
+
+def _stamped_tree():
+    from kd_slam2_amd import TreeGenerator3D
+
+    rng = np.random.default_rng(11)
+    P = np.concatenate([rng.uniform([-10, -10, -0.05], [10, 10, 0.05], size=(4000, 3)),
+                        rng.uniform([4, -10, 0], [4.1, 10, 3], size=(2000, 3))]).astype(np.float32)
+    stamps = rng.uniform(0.0, 0.1, size=len(P))
+    return TreeGenerator3D(leaf_max_points=16).makeTree(P, stamps)
+
+
+def test_tree_ops_identity_and_translation_are_exact():
+    t = _stamped_tree()
+    same = O.tree_apply_velocities(t.nodes, np.zeros(6))
+    assert np.array_equal(same["mean"], t.nodes["mean"]) and np.array_equal(same["direction"], t.nodes["direction"])
+    v = np.array([1.0, -2.0, 0.5, 0.0, 0.0, 0.0], dtype=np.float32)
+    moved = O.tree_apply_velocities(t.nodes, v)
+    dts = t.nodes["dts_mean"].astype(np.float32)[:, None]
+    assert np.array_equal(moved["mean"], t.nodes["mean"] + v[:3][None, :] * dts)
+    assert np.array_equal(moved["direction"], t.nodes["direction"])
+    skip = O.tree_apply_velocities(t.nodes, v, skip_leaves=True)
+    leaf = (t.nodes["idx_left"] < 0) & (t.nodes["idx_right"] < 0)
+    assert np.array_equal(skip["mean"][leaf], t.nodes["mean"][leaf])
+    assert np.array_equal(skip["mean"][~leaf], moved["mean"][~leaf])
+
+
+def test_tree_ops_rotation_and_isometry_match_fp64():
+    from kd_slam2_amd import synth
+
+    t = _stamped_tree()
+    w = np.array([0.0, 0.0, 0.0, 0.3, -0.2, 0.5])
+    moved = O.tree_apply_velocities(t.nodes, w)
+    for i in range(0, len(t.nodes), 97):
+        R = synth.rodrigues(w[3:] * float(np.float32(t.nodes["dts_mean"][i])))
+        assert np.allclose(moved["mean"][i], R @ t.nodes["mean"][i].astype(np.float64), atol=1e-5)
+    X = synth.expmap(np.array([3.0, -1.0, 0.5, 0.1, 0.2, -0.3]))
+    Y = O.tree_apply_isometry(t.nodes, X)
+    ref = t.nodes["mean"].astype(np.float64) @ X[:3, :3].T + X[:3, 3]
+    assert np.allclose(Y["mean"], ref, atol=1e-5)
+    back = O.tree_apply_isometry(Y, np.linalg.inv(X))
+    assert np.allclose(back["mean"], t.nodes["mean"], atol=2e-5)
+    assert np.allclose(back["direction"], t.nodes["direction"], atol=2e-6)
+
+
+def test_extract_leaves_is_the_leaf_gather_with_optional_warp():
+    t = _stamped_tree()
+    plain = O.extract_leaves(t.nodes, t.leaves)
+    assert np.array_equal(plain[:, 0], t.nodes["mean"][t.leaves]) and np.array_equal(plain[:, 1], t.nodes["direction"][t.leaves])
+    v = np.array([0.4, 0.1, 0.0, 0.0, 0.0, 0.7])
+    warped = O.extract_leaves(t.nodes, t.leaves, v)
+    moved = O.tree_apply_velocities(t.nodes, v)
+    assert np.array_equal(warped[:, 0], moved["mean"][t.leaves])
