@@ -240,6 +240,18 @@ typedef struct kdh_pair_result {
  * kdh_batch_align; forest entries' results repeat their leader's pose. */
 int kdh_batch_align(kdh_ctx* ctx, int n_pairs, const kdh_pair* pairs, kdh_pair_result* results);
 
+/* One quadratic form per pair (or forest) at its X0 / v0 / Xf, all in one
+ * tile launch and one finish launch: the per-factor builds of a bundle
+ * adjustment batched (MultiViewICPFactor_::compute calls buildQuadraticForm
+ * per factor, map/multiview_icp_factor_impl_.h:10-62).  For a rigid factor
+ * the two-pose blocks follow from J_B = -J_A (icp/icp_impl_.h:44-50):
+ * H_AA = H_BB = H, H_AB = -H, b_B = -b.  The cache is rounded from the FP64
+ * state as kdh_build_at does; stats_mode as in quadraticTerm.
+ * Outputs per pair i: H + 144 i (D x D row-major, D = 6 or 12), b + 12 i,
+ * stats[i].  Forest entries repeat their leader's summed form. */
+int kdh_batch_build(kdh_ctx* ctx, int n_pairs, const kdh_pair* pairs, int stats_mode,
+                    double* H, double* b, kdh_stats* stats);
+
 /* ---- serving: continuous batching ----------------------------------------
  * An engine thread runs one Gauss-Newton iteration of every in-flight
  * alignment per step (one tile + one finish launch per aligner kind), retires

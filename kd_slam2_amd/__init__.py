@@ -15,6 +15,7 @@ from .api import (  # noqa: F401
     TreeGenerator3D,
     adapter_align,
     batch_align,
+    batch_build,
     default_params,
     update_cache_f32,
 )

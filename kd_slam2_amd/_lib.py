@@ -203,6 +203,7 @@ _SIGS = {
          C.POINTER(Stats), C.POINTER(C.c_int), C.POINTER(C.c_int), F32P],
     ),
     "kdh_batch_align": (C.c_int, [P, C.c_int, C.POINTER(Pair), C.POINTER(PairResult)]),
+    "kdh_batch_build": (C.c_int, [P, C.c_int, C.POINTER(Pair), C.c_int, F64P, F64P, C.POINTER(Stats)]),
     "kdh_last_kernel_time": (C.c_int, [P, C.POINTER(C.c_double), C.POINTER(C.c_int)]),
     "kdh_set_launch_config": (C.c_int, [P, C.c_int, C.c_int]),
     "kdh_set_profiling": (C.c_int, [P, C.c_int]),

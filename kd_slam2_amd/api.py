@@ -616,14 +616,7 @@ def _stats_dicts(rec):
     return [{f: cols[f][i] for f in names} for i in range(len(rec))]
 
 
-def batch_align(ctx, pairs):
-    """Run many alignments in one batched launch (kdh_batch_align).
-
-    pairs: list of dicts with keys fixed, moving (DeviceTree), ct, X0, Xf, v0,
-    params (L.Params), prior, forest (k > 0: this pair leads a fixed forest
-    with the next k pairs).  Returns a list of dicts (X, v, iterations,
-    result, last, entry).  The C structs are filled and read through numpy
-    views, column by column."""
+def _pair_array(pairs):
     global _PAIR_DT, _RES_DT
     if _PAIR_DT is None:
         _PAIR_DT, _RES_DT = L.np_dtype(L.Pair), L.np_dtype(L.PairResult)
@@ -645,9 +638,37 @@ def batch_align(ctx, pairs):
         if rec is None:
             rec = cache[id(prm)] = np.frombuffer(bytes(prm), dtype=pd)[0]
         A["params"][i] = rec
-    keep = [p["prior"] for p in pairs if p.get("prior") is not None]
-    if keep:
+    if any(p.get("prior") is not None for p in pairs):
         A["prior"] = [C.addressof(p["prior"]) if p.get("prior") is not None else 0 for p in pairs]
+    return arr
+
+
+def batch_build(ctx, pairs, stats_mode=False):
+    """One quadratic form per pair at its X0 / v0 / Xf in one launch
+    (kdh_batch_build).  Returns a list of (H, b, stats)."""
+    n = len(pairs)
+    arr = _pair_array(pairs)
+    H = np.zeros((n, 144))
+    b = np.zeros((n, 12))
+    st = (L.Stats * n)()
+    ctx.check(L.lib().kdh_batch_build(ctx.h, n, arr, int(stats_mode), L.f64p(H), L.f64p(b), st), "kdh_batch_build")
+    out = []
+    for i, p in enumerate(pairs):
+        D = 12 if int(p.get("ct", 0)) else 6
+        out.append((H[i, :D * D].reshape(D, D).copy(), b[i, :D].copy(), st[i].as_dict()))
+    return out
+
+
+def batch_align(ctx, pairs):
+    """Run many alignments in one batched launch (kdh_batch_align).
+
+    pairs: list of dicts with keys fixed, moving (DeviceTree), ct, X0, Xf, v0,
+    params (L.Params), prior, forest (k > 0: this pair leads a fixed forest
+    with the next k pairs).  Returns a list of dicts (X, v, iterations,
+    result, last, entry).  The C structs are filled and read through numpy
+    views, column by column."""
+    n = len(pairs)
+    arr = _pair_array(pairs)
     res = (L.PairResult * n)()
     ctx.check(L.lib().kdh_batch_align(ctx.h, n, arr, res), "kdh_batch_align")
     R = np.frombuffer(res, dtype=_RES_DT)

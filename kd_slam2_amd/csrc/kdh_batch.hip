@@ -42,20 +42,20 @@ struct Group {
 
 }  // namespace
 
-int kdh_batch_align(kdh_ctx* ctx, int n_pairs, const kdh_pair* pairs, kdh_pair_result* results) {
-  if (!ctx) return KDH_ERR_INVALID;
-  if (n_pairs < 0 || (n_pairs > 0 && (!pairs || !results))) return kdh_fail(ctx, KDH_ERR_INVALID, "bad arguments");
-  if (n_pairs == 0) return KDH_OK;
-  std::lock_guard<std::mutex> lk(ctx->mu);
-  KDH_HIP_TRY(ctx, hipSetDevice(ctx->device));
+// Pair states, forests and the per-kind tile lists of a batch, uploaded.
+// mode 0: align (solve + update each iteration); mode 1: build the form once.
+// Called with ctx->mu held and the device set.
+static int batch_setup(kdh_ctx* ctx, int n_pairs, const kdh_pair* pairs, int mode, int stats_mode, Group* g,
+                       std::vector<int>& leader, int& max_iter) {
   int rc = kdh_ensure_pairs(ctx, (size_t)n_pairs);
   if (rc) return rc;
-  Group g[2];
+  g[0] = Group();
+  g[1] = Group();
   g[0].ct = 0;
   g[1].ct = 1;
-  int max_iter = 0;
+  max_iter = 0;
   // Forests: pair i with forest = k leads pairs i+1 .. i+k.
-  std::vector<int> leader(n_pairs);
+  leader.assign(n_pairs, 0);
   for (int i = 0; i < n_pairs;) {
     const int k = pairs[i].forest;
     if (k < 0 || k >= n_pairs - i)
@@ -74,7 +74,7 @@ int kdh_batch_align(kdh_ctx* ctx, int n_pairs, const kdh_pair* pairs, kdh_pair_r
     const kdh_pair& lp = pairs[leader[i]];  // pose, velocities, params, prior
     const kdh_pair& pr = pairs[i];           // fixed tree and pose
     PairState& ps = ctx->h_pairs[i];
-    rc = kdh_fill_pair(ctx, ps, pr.fixed, lp.moving, lp.ct, lp.params, 0, 0);
+    rc = kdh_fill_pair(ctx, ps, pr.fixed, lp.moving, lp.ct, lp.params, stats_mode, mode);
     if (rc) return kdh_fail(ctx, rc, "pair " + std::to_string(i) + ": " + ctx->err);
     memcpy(ps.X, lp.X0, sizeof(ps.X));
     memcpy(ps.Xf, pr.Xf, sizeof(ps.Xf));
@@ -121,6 +121,20 @@ int kdh_batch_align(kdh_ctx* ctx, int n_pairs, const kdh_pair* pairs, kdh_pair_r
   if (rc) return rc;
   KDH_HIP_TRY(ctx, hipMemcpyAsync(ctx->d_pairs, ctx->h_pairs, sizeof(PairState) * n_pairs, hipMemcpyHostToDevice,
                                   ctx->stream));
+  return KDH_OK;
+}
+
+int kdh_batch_align(kdh_ctx* ctx, int n_pairs, const kdh_pair* pairs, kdh_pair_result* results) {
+  if (!ctx) return KDH_ERR_INVALID;
+  if (n_pairs < 0 || (n_pairs > 0 && (!pairs || !results))) return kdh_fail(ctx, KDH_ERR_INVALID, "bad arguments");
+  if (n_pairs == 0) return KDH_OK;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  KDH_HIP_TRY(ctx, hipSetDevice(ctx->device));
+  Group g[2];
+  std::vector<int> leader;
+  int max_iter = 0;
+  int rc = batch_setup(ctx, n_pairs, pairs, 0, 0, g, leader, max_iter);
+  if (rc) return rc;
 
   const int iters = std::min(max_iter, kMaxIter);
   auto launch_phase = [&](int phases) -> int {
@@ -237,6 +251,44 @@ int kdh_batch_align(kdh_ctx* ctx, int n_pairs, const kdh_pair* pairs, kdh_pair_r
     r.entry = entry[i];
     if (ps.group_n < 2) r.entry = r.last;
     r.result = kdh_result_code(ps, pairs[leader[i]].params.max_iterations);
+  }
+  return KDH_OK;
+}
+
+int kdh_batch_build(kdh_ctx* ctx, int n_pairs, const kdh_pair* pairs, int stats_mode, double* H, double* b,
+                    kdh_stats* stats) {
+  if (!ctx) return KDH_ERR_INVALID;
+  if (n_pairs < 0 || (n_pairs > 0 && (!pairs || !H || !b || !stats)))
+    return kdh_fail(ctx, KDH_ERR_INVALID, "bad arguments");
+  if (n_pairs == 0) return KDH_OK;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  KDH_HIP_TRY(ctx, hipSetDevice(ctx->device));
+  Group g[2];
+  std::vector<int> leader;
+  int max_iter = 0;
+  int rc = batch_setup(ctx, n_pairs, pairs, 1, stats_mode, g, leader, max_iter);
+  if (rc) return rc;
+  for (int k = 0; k < 2; ++k) {
+    if (g[k].n_tiles == 0) continue;
+    rc = kdh_launch(ctx, ctx->d_pairs, ctx->d_tile_pair + g[k].tile_off, g[k].n_tiles, g[k].ct, nullptr, nullptr,
+                    ctx->d_partials + g[k].partial_off, n_pairs, 3);
+    if (rc) return rc;
+  }
+  KDH_HIP_TRY(ctx, hipMemcpyAsync(ctx->h_pairs, ctx->d_pairs, sizeof(PairState) * n_pairs, hipMemcpyDeviceToHost,
+                                  ctx->stream));
+  KDH_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  for (int i = 0; i < n_pairs; ++i) {
+    const PairState& L = ctx->h_pairs[leader[i]];
+    const int D = L.ct ? 12 : 6;
+    double* Hi = H + (size_t)i * 144;
+    double* bi = b + (size_t)i * 12;
+    memset(Hi, 0, sizeof(double) * 144);
+    memset(bi, 0, sizeof(double) * 12);
+    memset(&stats[i], 0, sizeof(kdh_stats));
+    if (L.group_tiles == 0) continue;  // no moving leaves: an empty form
+    memcpy(Hi, L.H, sizeof(double) * D * D);
+    memcpy(bi, L.b, sizeof(double) * D);
+    memcpy(&stats[i], &L.stats_log[0], sizeof(kdh_stats));
   }
   return KDH_OK;
 }
