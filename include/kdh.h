@@ -252,6 +252,46 @@ int kdh_batch_align(kdh_ctx* ctx, int n_pairs, const kdh_pair* pairs, kdh_pair_r
 int kdh_batch_build(kdh_ctx* ctx, int n_pairs, const kdh_pair* pairs, int stats_mode,
                     double* H, double* b, kdh_stats* stats);
 
+/* ---- CT dual form: bundle-adjustment factors over two CT states -----------
+ *
+ * One CT factor links a moving cloud A (pose X_moving, velocities v_moving)
+ * and a fixed cloud B (pose X_fixed, velocities v_fixed).  Besides the CT
+ * form of A (H 12x12, b 12, as kdh_build_ct) the dual build gives the terms
+ * of B's velocities, taken at each fixed leaf's own stamp (CTICP_::
+ * quadraticTermDual, cticp/cticp_impl_.h:185-293; CTICP_CPU_::
+ * _buildQuadraticFormDual, cticp/cticp_cpu_impl_.h:129-205), which
+ * MultiViewCTICPFactor_::compute scatters into the four-variable factor
+ * (map/multiview_cticp_factor_impl_.h:8-80). */
+typedef struct kdh_dual_pair {
+  const kdh_tree* fixed;       /* cloud B: its leaves' dts_mean are the stamps of J_vel_B */
+  const kdh_tree* moving;      /* cloud A */
+  double X_moving[12];         /* moving_state.X */
+  double v_moving[6];          /* moving_state.velocities [t; w] */
+  double X_fixed[12];          /* fixed_state.X */
+  double v_fixed[6];           /* fixed_state.velocities [t; w] */
+  kdh_params params;
+} kdh_dual_pair;
+
+typedef struct kdh_dual_form {
+  double H[144];                        /* 12x12 row-major: CTICP_::H */
+  double b[12];                         /* CTICP_::b */
+  double H_dual_vel_B[36];              /* 6x6: sum g J_vB^T J_vB */
+  double b_dual_vel_B[6];               /* sum g J_vB^T e */
+  double H_dual_cross_pose_vel_B[36];   /* 6x6: sum g J_pose^T J_vB (row: pose) */
+  double H_dual_cross_vel_AB[36];       /* 6x6: sum g J_vA^T J_vB (row: vel A) */
+  kdh_stats stats;
+} kdh_dual_form;
+
+/* The dual forms of n_factors factors in one tile launch and one finish
+ * launch (the per-factor buildQuadraticFormDual calls of a bundle adjustment,
+ * batched).  The cache is rounded from the FP64 states as kdh_build_at does;
+ * params.recenter as there.  stats_mode: inlier terms weigh zero and outlier
+ * terms keep their kernel weight, in every block (the dual weight rule,
+ * cticp/cticp_impl_.h:240-248).  A factor whose moving tree has no leaves
+ * gets an all-zero form. */
+int kdh_batch_build_dual(kdh_ctx* ctx, int n_factors, const kdh_dual_pair* factors, int stats_mode,
+                         kdh_dual_form* out);
+
 /* ---- serving: continuous batching ----------------------------------------
  * An engine thread runs one Gauss-Newton iteration of every in-flight
  * alignment per step (one tile + one finish launch per aligner kind), retires

@@ -16,6 +16,7 @@ from .api import (  # noqa: F401
     adapter_align,
     batch_align,
     batch_build,
+    batch_build_dual,
     default_params,
     update_cache_f32,
 )

@@ -104,6 +104,30 @@ class Pair(C.Structure):
     ]
 
 
+class DualPair(C.Structure):
+    _fields_ = [
+        ("fixed", C.c_void_p),
+        ("moving", C.c_void_p),
+        ("X_moving", C.c_double * 12),
+        ("v_moving", C.c_double * 6),
+        ("X_fixed", C.c_double * 12),
+        ("v_fixed", C.c_double * 6),
+        ("params", Params),
+    ]
+
+
+class DualForm(C.Structure):
+    _fields_ = [
+        ("H", C.c_double * 144),
+        ("b", C.c_double * 12),
+        ("H_dual_vel_B", C.c_double * 36),
+        ("b_dual_vel_B", C.c_double * 6),
+        ("H_dual_cross_pose_vel_B", C.c_double * 36),
+        ("H_dual_cross_vel_AB", C.c_double * 36),
+        ("stats", Stats),
+    ]
+
+
 class PairResult(C.Structure):
     _fields_ = [
         ("X", C.c_double * 12),
@@ -204,6 +228,7 @@ _SIGS = {
     ),
     "kdh_batch_align": (C.c_int, [P, C.c_int, C.POINTER(Pair), C.POINTER(PairResult)]),
     "kdh_batch_build": (C.c_int, [P, C.c_int, C.POINTER(Pair), C.c_int, F64P, F64P, C.POINTER(Stats)]),
+    "kdh_batch_build_dual": (C.c_int, [P, C.c_int, C.POINTER(DualPair), C.c_int, C.POINTER(DualForm)]),
     "kdh_last_kernel_time": (C.c_int, [P, C.POINTER(C.c_double), C.POINTER(C.c_int)]),
     "kdh_set_launch_config": (C.c_int, [P, C.c_int, C.c_int]),
     "kdh_set_profiling": (C.c_int, [P, C.c_int]),

@@ -581,6 +581,23 @@ class CTICPHIP3D(_AlignerBase):
     def setFixedX(self, X):
         self.Xf = np.array(X, dtype=np.float64)
 
+    def setFixedVelocities(self, v):
+        """fixed_state.velocities: the fixed cloud's own motion, used only by
+        the dual form."""
+        self.vf = np.array(v, dtype=np.float64)
+
+    def buildQuadraticFormDual(self, stats_mode=False):
+        """CTICP_::buildQuadraticFormDual (cticp/cticp_impl_.h:42-45): H, b
+        and the fixed side's velocity blocks H_dual_vel_B, b_dual_vel_B,
+        H_dual_cross_pose_vel_B, H_dual_cross_vel_AB (cticp/cticp_.h:179-186)
+        at the FP64 state (X, v) / (Xf, vf)."""
+        f = batch_build_dual(self.ctx, [dict(fixed=self._fixed, moving=self._moving, X_moving=self.X, v_moving=self.v,
+                                             X_fixed=self.Xf, v_fixed=getattr(self, "vf", np.zeros(6)),
+                                             params=self.params)], stats_mode)[0]
+        for k in ("H", "b", "H_dual_vel_B", "b_dual_vel_B", "H_dual_cross_pose_vel_B", "H_dual_cross_vel_AB",
+                  "stats"):
+            setattr(self, k, f[k])
+
 
 def adapter_align(type_name, ctx, fixed, moving, X0, Xf, params, v0=None, on_device=False, stats_mode=False):
     """Create `type_name` ("ICPHIP3D" / "CTICPHIP3D") through the C++ plugin
@@ -657,6 +674,36 @@ def batch_build(ctx, pairs, stats_mode=False):
         D = 12 if int(p.get("ct", 0)) else 6
         out.append((H[i, :D * D].reshape(D, D).copy(), b[i, :D].copy(), st[i].as_dict()))
     return out
+
+
+def batch_build_dual(ctx, factors, stats_mode=False):
+    """The CT dual forms of many bundle-adjustment factors in one launch
+    (kdh_batch_build_dual).  factors: dicts with fixed, moving (DeviceTree),
+    X_moving, v_moving, X_fixed, v_fixed, params.  Returns dicts with H
+    (12x12), b, H_dual_vel_B (6x6), b_dual_vel_B, H_dual_cross_pose_vel_B
+    (6x6, rows: pose), H_dual_cross_vel_AB (6x6, rows: velocities A), stats."""
+    n = len(factors)
+    arr = (L.DualPair * n)()
+    for i, f in enumerate(factors):
+        d = arr[i]
+        d.fixed = f["fixed"].h.value
+        d.moving = f["moving"].h.value
+        d.X_moving[:] = np.asarray(f["X_moving"], dtype=np.float64)[:3, :4].reshape(12).tolist()
+        d.v_moving[:] = np.asarray(f.get("v_moving", _Z6), dtype=np.float64).reshape(6).tolist()
+        d.X_fixed[:] = np.asarray(f.get("X_fixed", _I4), dtype=np.float64)[:3, :4].reshape(12).tolist()
+        d.v_fixed[:] = np.asarray(f.get("v_fixed", _Z6), dtype=np.float64).reshape(6).tolist()
+        d.params = f["params"]
+    out = (L.DualForm * n)()
+    ctx.check(L.lib().kdh_batch_build_dual(ctx.h, n, arr, int(stats_mode), out), "kdh_batch_build_dual")
+    res = []
+    for i in range(n):
+        o = out[i]
+        g = lambda name, shape: np.array(getattr(o, name), dtype=np.float64).reshape(shape)  # noqa: E731
+        res.append(dict(H=g("H", (12, 12)), b=g("b", (12,)), H_dual_vel_B=g("H_dual_vel_B", (6, 6)),
+                        b_dual_vel_B=g("b_dual_vel_B", (6,)),
+                        H_dual_cross_pose_vel_B=g("H_dual_cross_pose_vel_B", (6, 6)),
+                        H_dual_cross_vel_AB=g("H_dual_cross_vel_AB", (6, 6)), stats=o.stats.as_dict()))
+    return res
 
 
 def batch_align(ctx, pairs):

@@ -44,9 +44,10 @@ struct Group {
 
 // Pair states, forests and the per-kind tile lists of a batch, uploaded.
 // mode 0: align (solve + update each iteration); mode 1: build the form once.
-// Called with ctx->mu held and the device set.
+// dual: the CT pairs build the dual form, with the fixed velocities vf
+// (n_pairs x 6).  Called with ctx->mu held and the device set.
 static int batch_setup(kdh_ctx* ctx, int n_pairs, const kdh_pair* pairs, int mode, int stats_mode, Group* g,
-                       std::vector<int>& leader, int& max_iter) {
+                       std::vector<int>& leader, int& max_iter, int dual = 0, const double* vf = nullptr) {
   int rc = kdh_ensure_pairs(ctx, (size_t)n_pairs);
   if (rc) return rc;
   g[0] = Group();
@@ -81,6 +82,8 @@ static int batch_setup(kdh_ctx* ctx, int n_pairs, const kdh_pair* pairs, int mod
     for (int k = 0; k < 6; ++k) ps.v[k] = lp.ct ? lp.v0[k] : 0.0;
     kdh_setup_prior(ps, leader[i] == i ? lp.prior : nullptr);
     kdh_cache_from_state(ps);
+    if (vf)
+      for (int k = 0; k < 6; ++k) ps.cache.vf[k] = (float)vf[6 * i + k];
     Group& G = g[ps.ct];
     ps.tile_first = G.n_tiles;
     G.n_tiles += ps.n_tiles;
@@ -103,9 +106,10 @@ static int batch_setup(kdh_ctx* ctx, int n_pairs, const kdh_pair* pairs, int mod
   }
   g[0].partial_off = 0;
   g[1].partial_off = (size_t)g[0].n_tiles * slots(0);
+  const int ct_slots = slots(1, dual);
   g[0].tile_off = 0;
   g[1].tile_off = (size_t)g[0].n_tiles;
-  const size_t n_partials = g[1].partial_off + (size_t)g[1].n_tiles * slots(1);
+  const size_t n_partials = g[1].partial_off + (size_t)g[1].n_tiles * ct_slots;
   const size_t n_tiles_all = (size_t)g[0].n_tiles + (size_t)g[1].n_tiles;
   rc = kdh_ensure_partials(ctx, std::max<size_t>(n_partials, 1));
   if (rc) return rc;
@@ -289,6 +293,62 @@ int kdh_batch_build(kdh_ctx* ctx, int n_pairs, const kdh_pair* pairs, int stats_
     memcpy(Hi, L.H, sizeof(double) * D * D);
     memcpy(bi, L.b, sizeof(double) * D);
     memcpy(&stats[i], &L.stats_log[0], sizeof(kdh_stats));
+  }
+  return KDH_OK;
+}
+
+int kdh_batch_build_dual(kdh_ctx* ctx, int n_factors, const kdh_dual_pair* factors, int stats_mode,
+                         kdh_dual_form* out) {
+  if (!ctx) return KDH_ERR_INVALID;
+  if (n_factors < 0 || (n_factors > 0 && (!factors || !out))) return kdh_fail(ctx, KDH_ERR_INVALID, "bad arguments");
+  if (n_factors == 0) return KDH_OK;
+  // The dual form is a CT build: each factor becomes a CT pair with its
+  // moving state as (X0, v0) and the fixed pose as Xf; the fixed velocities
+  // enter the cache beside it.
+  std::vector<kdh_pair> pairs(n_factors);
+  std::vector<double> vf((size_t)n_factors * 6);
+  for (int i = 0; i < n_factors; ++i) {
+    const kdh_dual_pair& f = factors[i];
+    kdh_pair& p = pairs[i];
+    memset(&p, 0, sizeof(p));
+    p.fixed = f.fixed;
+    p.moving = f.moving;
+    p.ct = 1;
+    p.forest = 0;
+    memcpy(p.X0, f.X_moving, sizeof(p.X0));
+    memcpy(p.v0, f.v_moving, sizeof(p.v0));
+    memcpy(p.Xf, f.X_fixed, sizeof(p.Xf));
+    p.params = f.params;
+    p.prior = nullptr;
+    memcpy(&vf[(size_t)6 * i], f.v_fixed, sizeof(double) * 6);
+  }
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  KDH_HIP_TRY(ctx, hipSetDevice(ctx->device));
+  Group g[2];
+  std::vector<int> leader;
+  int max_iter = 0;
+  int rc = batch_setup(ctx, n_factors, pairs.data(), 1, stats_mode, g, leader, max_iter, 1, vf.data());
+  if (rc) return rc;
+  if (g[1].n_tiles > 0) {
+    rc = kdh_launch(ctx, ctx->d_pairs, ctx->d_tile_pair + g[1].tile_off, g[1].n_tiles, 1, nullptr, nullptr,
+                    ctx->d_partials + g[1].partial_off, n_factors, 3, 1);
+    if (rc) return rc;
+  }
+  KDH_HIP_TRY(ctx, hipMemcpyAsync(ctx->h_pairs, ctx->d_pairs, sizeof(PairState) * n_factors, hipMemcpyDeviceToHost,
+                                  ctx->stream));
+  KDH_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  for (int i = 0; i < n_factors; ++i) {
+    const PairState& ps = ctx->h_pairs[i];
+    kdh_dual_form& o = out[i];
+    memset(&o, 0, sizeof(o));
+    if (ps.group_tiles == 0) continue;  // no moving leaves: an empty form
+    memcpy(o.H, ps.H, sizeof(o.H));
+    memcpy(o.b, ps.b, sizeof(o.b));
+    memcpy(o.H_dual_vel_B, ps.Hd, sizeof(o.H_dual_vel_B));
+    memcpy(o.b_dual_vel_B, ps.Hd + 36, sizeof(o.b_dual_vel_B));
+    memcpy(o.H_dual_cross_pose_vel_B, ps.Hd + 42, sizeof(o.H_dual_cross_pose_vel_B));
+    memcpy(o.H_dual_cross_vel_AB, ps.Hd + 78, sizeof(o.H_dual_cross_vel_AB));
+    memcpy(&o.stats, &ps.stats_log[0], sizeof(kdh_stats));
   }
   return KDH_OK;
 }

@@ -189,6 +189,7 @@ int kdh_tree_upload(kdh_ctx* ctx, const kdh_node3f* nodes, int64_t n_nodes, cons
   t->dt.nodes = (const float4*)d;
   t->dt.leaves = (const float4*)((char*)d + node_bytes);
   t->node_dts = (float*)((char*)d + node_bytes + leaf_bytes);
+  t->dt.node_dts = t->node_dts;
   t->dt.n_nodes = (int)n_nodes;
   t->dt.n_leaves = (int)n_leaves;
   t->root_mean = n_nodes > 0 ? v3(nodes[0].mean[0], nodes[0].mean[1], nodes[0].mean[2]) : v3(0.0f, 0.0f, 0.0f);
@@ -343,8 +344,9 @@ int kdh_fill_pair(kdh_ctx* ctx, PairState& ps, const kdh_tree* fixed, const kdh_
 }
 
 int kdh_launch(kdh_ctx* ctx, PairState* d_pairs, const int* d_tile_pair, int n_tiles, int ct, int* corr,
-               int* depth, float* partials, int n_pairs, int phases) {
+               int* depth, float* partials, int n_pairs, int phases, int dual) {
   LaunchArgs a;
+  a.dual = dual;
   a.pair_first = 0;
   a.n_pairs = n_pairs;
   a.phases = phases;
@@ -361,6 +363,7 @@ int kdh_launch(kdh_ctx* ctx, PairState* d_pairs, const int* d_tile_pair, int n_t
   a.ct = ct;
   int lds = ctx->lds_nodes;
   if (ct && lds > 1024) lds = 1024;
+  if (dual && lds > 512) lds = 512;
   a.lds_nodes = lds;
   if (n_tiles <= 0) return KDH_OK;
   hipError_t e = launch_tiles(a, ctx->stream);

@@ -78,7 +78,7 @@ int kdh_fill_pair(kdh_ctx* ctx, kdh::PairState& ps, const kdh_tree* fixed, const
 void kdh_cache_from_state(kdh::PairState& ps);
 int kdh_setup_prior(kdh::PairState& ps, const kdh_prior* prior);
 int kdh_launch(kdh_ctx* ctx, kdh::PairState* d_pairs, const int* d_tile_pair, int n_tiles, int ct, int* corr,
-               int* depth, float* partials, int n_pairs, int phases = 3);
+               int* depth, float* partials, int n_pairs, int phases = 3, int dual = 0);
 int kdh_result_code(const kdh::PairState& ps, int max_iterations_param);
 void kdh_collect_times(kdh_ctx* ctx, int iters, bool prof);
 // Returns the cached executable graph for `key`, or nullptr (and drops the

@@ -28,6 +28,7 @@ struct DevTree {
   const float4* leaves;
   int n_nodes;
   int n_leaves;
+  const float* node_dts;  // per node (float)_dts_mean
 };
 
 // Per-iteration statistics, ICPStats_ (icp_stats_.h:10-20) in FP64 sums.
@@ -69,7 +70,7 @@ struct PairState {
   // is a group of one.
   int group_n;
   int group_tiles;
-  int group_pad[2];     // keeps sizeof(PairState) a multiple of 16
+  int group_pad[4];     // keeps sizeof(PairState) a multiple of 16
   double Xf[12];        // fixed pose, 3x4 row-major
   double has_prior;
   double prior_ZXi[12]; // (X_ref * Z)^-1
@@ -87,6 +88,9 @@ struct PairState {
   DevStats stats_log[kMaxIter];
   double H[144];        // last H (n x n row-major, n = 6 or 12)
   double b[12];
+  // dual form (build mode, kdh_batch_build_dual): H_vel_B 36, b_vel_B 6,
+  // H_cross_pose_vel_B 36, H_cross_vel_AB 36
+  double Hd[114];
 };
 
 // Re-centring anchor of a state (the TermCache fields mroot and cf, and the

@@ -11,7 +11,7 @@ struct LaunchArgs {
   PairState* pairs;       // device array of alignments
   const int* tile_pair;   // device array: tile -> pair (nullptr: all tiles belong to pair 0)
   int n_tiles_total;
-  float* partials;        // n_tiles_total x slots(ct) floats
+  float* partials;        // n_tiles_total x slots(ct, dual) floats
   int* corr_out;          // optional, single-pair launches: fixed leaf per moving leaf
   int* depth_out;         // optional, single-pair launches: descent depth per moving leaf
   int grid;               // workgroups (tiles are strided over them)
@@ -23,9 +23,10 @@ struct LaunchArgs {
   int phases;             // bit 0: tile kernel, bit 1: finish stage
   const int* pair_tile_first;  // optional: first tile of each pair in this launch (else PairState::tile_first)
   int* status_out;        // optional: per-pair status mirror written by the finish stage
+  int dual = 0;           // 1: CT dual form of a bundle-adjustment factor (build mode, ct = 1)
 };
 
 hipError_t launch_tiles(const LaunchArgs& a, hipStream_t s);
-int slots(int ct);
+int slots(int ct, int dual = 0);
 
 }  // namespace kdh

@@ -43,6 +43,10 @@ constexpr int kWaves = kThreads / 64;
 template <bool CT> struct Slots;
 template <> struct Slots<false> { static constexpr int N = kIcpSlots; static constexpr int P = 64; };
 template <> struct Slots<true> { static constexpr int N = kCtSlots; static constexpr int P = 128; };
+// The dual form appends Diag(vel_B) 27, Cross(pose, vel_B) 36 and
+// Cross(vel_A, vel_B) 36 (icp_ws_item_.h) to the 97 CT slots.
+constexpr int kDualSlots = kCtSlots + 99;
+template <bool CT, bool DUAL> constexpr int n_slots() { return DUAL ? kDualSlots : Slots<CT>::N; }
 
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 
@@ -53,6 +57,16 @@ __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Dual extras (slot s >= 97): cell of C_zz = sum z z^T (tile 2) or of
+// C_uz = sum w z^T (tile 1), z = sqrt(g) J_vel_B; returned as tile * 256 + cell.
+__device__ __forceinline__ int dual_slot_cell(int s) {
+  const int t = s - kCtSlots;
+  if (t < 21) return 2 * 256 + tri_c(t) * 16 + tri_r(t);  // Diag(vel_B) J_b J_b^T g
+  if (t < 27) return 256 + (t - 21) * 16 + 12;             // J_b e g: row sqrt(g) e of w
+  if (t < 63) return 256 + ((t - 27) % 6) * 16 + (t - 27) / 6;        // J_pose J_b^T g
+  return 256 + ((t - 63) % 6) * 16 + 6 + (t - 63) / 6;                 // J_vel J_b^T g
 }
 
 // Cell (j * 16 + i) of C = sum w w^T holding accumulator slot s < CS + 2, in
@@ -203,7 +217,7 @@ __device__ void wave_ldlt_solve(double (&a)[D], double bi, double damp_pose, dou
 // large, 16-byte aligned.
 template <bool CT>
 struct FinishLds {
-  static constexpr int N = Slots<CT>::N;
+  static constexpr int N = kDualSlots;  // room for the largest (dual) row
   static constexpr int D = CT ? 12 : 6;
   static constexpr int kRed = 0;                          // double[N]
   static constexpr int kG = kRed + 8 * ((N + 1) & ~1);    // double[3*D]: rows 0..2 of G
@@ -241,11 +255,11 @@ static_assert(sizeof(PairState) % 16 == 0, "PairState rows must stay 16-byte ali
 
 // One wave (64 lanes) per alignment.  Every step is lane-parallel or a
 // one-wave chain; the workgroup barrier of a one-wave group costs nothing.
-template <bool CT>
+template <bool CT, bool DUAL>
 __device__ void finish_pair(PairState& ps, const PairState& S, const float* __restrict__ partials, char* lds,
                             int tile_first, int* status_out) {
   using FL = FinishLds<CT>;
-  constexpr int N = Slots<CT>::N;
+  constexpr int N = n_slots<CT, DUAL>();
   constexpr int D = CT ? 12 : 6;
   constexpr int CS = CT ? 90 : 27;  // chi2 slot
   double* s_red = (double*)(lds + FL::kRed);
@@ -274,16 +288,17 @@ __device__ void finish_pair(PairState& ps, const PairState& S, const float* __re
   //    loads of a 32-row block are issued before the first add, so a block
   //    costs one memory round trip.  The order is fixed, so the result does
   //    not depend on the batch.
-  {
+  // (The dual form's 196 slots take two such passes of 128.)
+  for (int sb = 0; sb < N; sb += 128) {
     constexpr int TB = 32;
     const float* base = partials + (size_t)tile_first * N;
-    const bool has0 = lane < N, has1 = lane + 64 < N;
+    const bool has0 = sb + lane < N, has1 = sb + lane + 64 < N;
     double acc0 = 0.0, acc1 = 0.0;
     for (int t0 = 0; t0 < nt; t0 += TB) {
       float v0[TB], v1[TB];
       // Unconditional loads from clamped, in-range addresses (nt >= 1): a
       // guarded load would become a branch with a wait after every load.
-      const int s0 = has0 ? lane : 0, s1 = has1 ? lane + 64 : 0;
+      const int s0 = has0 ? sb + lane : 0, s1 = has1 ? sb + lane + 64 : 0;
 #pragma unroll
       for (int u = 0; u < TB; ++u) {
         const size_t row = (size_t)min(t0 + u, nt - 1) * N;
@@ -297,8 +312,8 @@ __device__ void finish_pair(PairState& ps, const PairState& S, const float* __re
           acc1 += (double)v1[u];
         }
     }
-    if (has0) s_red[lane] = acc0;
-    if (has1) s_red[lane + 64] = acc1;
+    if (has0) s_red[sb + lane] = acc0;
+    if (has1) s_red[sb + lane + 64] = acc1;
   }
   __syncthreads();
 
@@ -352,6 +367,26 @@ __device__ void finish_pair(PairState& ps, const PairState& S, const float* __re
       ps.b[r] = bv;
     }
     if (tid == 0) ps.stats_log[0] = st;
+    if constexpr (DUAL) {
+      // The dual blocks (CTICP_ H_dual_*, cticp_cpu_impl_.h:202-204) into
+      // ps.Hd: H_vel_B 36 | b_vel_B 6 | H_cross_pose_vel_B 36 | H_cross_vel_AB 36.
+      // Only the pose rows of H_cross_pose_vel_B carry the re-centring; they
+      // map back like b: rows 3..5 -= [c]x^T rows 0..2.
+      const double* x = s_red + kCtSlots;
+      for (int e = tid; e < 36; e += 64) {
+        const int rr = e / 6, cc2 = e % 6;
+        const int a0 = rr < cc2 ? rr : cc2, a1 = rr < cc2 ? cc2 : rr;
+        ps.Hd[e] = x[a0 * 6 - (a0 * (a0 - 1)) / 2 + (a1 - a0)];
+        double cp = x[27 + e];
+        if (rr >= 3) {
+          const V3d k = skew_col(cc, rr - 3);
+          cp -= (k.x * x[27 + cc2] + k.y * x[27 + 6 + cc2]) + k.z * x[27 + 12 + cc2];
+        }
+        ps.Hd[42 + e] = cp;
+        ps.Hd[78 + e] = x[63 + e];
+      }
+      if (tid < 6) ps.Hd[36 + tid] = x[21 + tid];
+    }
     return;
   }
 
@@ -553,8 +588,10 @@ __device__ void finish_pair(PairState& ps, const PairState& S, const float* __re
 
 // ---- the tile kernel --------------------------------------------------------
 
-template <bool CT, int LDS_NODES>
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) void tiles_kernel(const PairState* __restrict__ pairs,
+// DUAL (CT only): the dual form of quadraticTermDual, built for bundle
+// adjustment; three MFMA products per wave instead of one.
+template <bool CT, int LDS_NODES, bool DUAL>
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(DUAL ? 3 : 4))) void tiles_kernel(const PairState* __restrict__ pairs,
                                                          const int* __restrict__ tile_pair,
                                                          int n_tiles_total,
                                                          float* __restrict__ partials,
@@ -562,15 +599,17 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
                                                          int* __restrict__ depth_out,
                                                          int xcd_remap,
                                                          const int* __restrict__ pair_tile_first) {
-  constexpr int N = Slots<CT>::N;
+  static_assert(CT || !DUAL, "the dual form is a CT form");
+  constexpr int N = n_slots<CT, DUAL>();
   constexpr int E = CT ? 12 : 6;       // component of sqrt(g) e in w
   constexpr int NW = E + 2;            // components of w
   constexpr int CS = CT ? 90 : 27;     // chi2 slot
   constexpr int HW = (NW + 1) / 2;     // components per transpose half
   constexpr int WS = 68;               // transpose row stride (floats): 16-B aligned, spreads banks
-  static_assert(HW * WS >= 256, "the per-wave C tile reuses the transpose buffer");
+  constexpr int WBUF = DUAL ? 3 * 256 : HW * WS;  // per wave: transpose, then the C tile(s)
+  static_assert(HW * WS >= 256 && WBUF >= HW * WS, "the per-wave C tiles reuse the transpose buffer");
   __shared__ float4 s_nodes[LDS_NODES > 0 ? 2 * LDS_NODES : 2];
-  __shared__ __align__(16) float s_w[kWaves][HW * WS];
+  __shared__ __align__(16) float s_w[kWaves][WBUF];
   __shared__ int s_cnt[kWaves][5];
 
   const int tid = threadIdx.x;
@@ -623,6 +662,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
 
     TermRec R;
     rec_reject(R, -1);  // padding lanes contribute nothing
+    float Jb[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
     if (live) {
       LeafIn L;
       L.mean = v3(la.x, la.y, la.z);
@@ -631,13 +671,17 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
       const TermCache C = ps.cache;
       const TermParams TP = ps.tp;
       int leaf, depth = 0;
+      const float* fdts = fixed.node_dts;
+      auto dts_of = [fdts](int i) { return fdts[i]; };
       if (LDS_NODES > 0) {
         LdsFetch f{fixed.nodes, (lds_f4*)s_nodes, n_lds};
-        leaf = CT ? ct_term(f, C, TP, L, R, &depth) : icp_term(f, C, TP, L, R, &depth);
+        leaf = CT ? ct_term<DUAL>(f, C, TP, L, R, &depth, Jb, dts_of) : icp_term(f, C, TP, L, R, &depth);
       } else {
         GlobalFetch f{fixed.nodes};
-        leaf = CT ? ct_term(f, C, TP, L, R, &depth) : icp_term(f, C, TP, L, R, &depth);
+        leaf = CT ? ct_term<DUAL>(f, C, TP, L, R, &depth, Jb, dts_of) : icp_term(f, C, TP, L, R, &depth);
       }
+      if (DUAL && R.outcome != kInlier && R.outcome != kOutlier)
+        for (int i = 0; i < 6; ++i) Jb[i] = 0.0f;
       if (corr_out) corr_out[q] = leaf;
       if (depth_out) depth_out[q] = depth;
     }
@@ -694,6 +738,36 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
         c1 = __builtin_amdgcn_mfma_f32_16x16x4f32(a[m + 1], a[m + 1], c1, 0, 0, 0);
       }
       const floatx4 c = c0 + c1;
+      if constexpr (DUAL) {
+        // z = sqrt(g) J_vel_B: a third transpose, then C_uz = sum w z^T and
+        // C_zz = sum z z^T
+        float az[16];
+#pragma unroll
+        for (int m = 0; m < 16; ++m) az[m] = 0.0f;
+#pragma unroll
+        for (int i = 0; i < 6; ++i) sw[i * WS + lane] = sg * Jb[i];
+        wave_lds_sync();
+        if (ci < 6) {
+          const float4* src = reinterpret_cast<const float4*>(sw + ci * WS + cg * 16);
+#pragma unroll
+          for (int m4 = 0; m4 < 4; ++m4) {
+            const float4 v = src[m4];
+            az[4 * m4 + 0] = v.x;
+            az[4 * m4 + 1] = v.y;
+            az[4 * m4 + 2] = v.z;
+            az[4 * m4 + 3] = v.w;
+          }
+        }
+        wave_lds_sync();
+        floatx4 cuz = {0.0f, 0.0f, 0.0f, 0.0f}, czz = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int m = 0; m < 16; ++m) {
+          cuz = __builtin_amdgcn_mfma_f32_16x16x4f32(a[m], az[m], cuz, 0, 0, 0);
+          czz = __builtin_amdgcn_mfma_f32_16x16x4f32(az[m], az[m], czz, 0, 0, 0);
+        }
+        *reinterpret_cast<float4*>(&sw[256 + ci * 16 + 4 * cg]) = make_float4(cuz[0], cuz[1], cuz[2], cuz[3]);
+        *reinterpret_cast<float4*>(&sw[512 + ci * 16 + 4 * cg]) = make_float4(czz[0], czz[1], czz[2], czz[3]);
+      }
       // lane (ci, cg) holds C[4 cg + r][ci], stored at ci * 16 + 4 cg + r,
       // in this wave's part of the transpose buffer
       *reinterpret_cast<float4*>(&sw[ci * 16 + 4 * cg]) = make_float4(c[0], c[1], c[2], c[3]);
@@ -713,8 +787,8 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
     __syncthreads();
     if (tid < N) {
       float s;
-      if (tid < CS + 2) {
-        const int cell = slot_cell<CT>(tid);
+      if (tid < CS + 2 || (DUAL && tid >= kCtSlots)) {
+        const int cell = (DUAL && tid >= kCtSlots) ? dual_slot_cell(tid) : slot_cell<CT>(tid);
         s = (s_w[0][cell] + s_w[1][cell]) + (s_w[2][cell] + s_w[3][cell]);
       } else {
         const int k = tid - CS - 2;
@@ -728,7 +802,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
 
 // One workgroup per alignment: FP64 reduction of its partial rows, then
 // build-mode publish or solve-mode Gauss-Newton step.
-template <bool CT>
+template <bool CT, bool DUAL>
 __global__ __launch_bounds__(64) void finish_kernel(PairState* __restrict__ pairs, int pair_first,
                                                           int n_pairs, const float* __restrict__ partials,
                                                           const int* __restrict__ pair_tile_first,
@@ -747,21 +821,21 @@ __global__ __launch_bounds__(64) void finish_kernel(PairState* __restrict__ pair
   const PairState& S = *reinterpret_cast<const PairState*>(s_state);
   // forest entries other than the leader are finished by their leader
   if ((S.status != 0) | (S.ct != (CT ? 1 : 0)) | (S.group_n == 0) | (S.group_tiles == 0)) return;
-  finish_pair<CT>(ps, S, partials, s_area, tf >= 0 ? tf : S.tile_first, status_out ? status_out + p : nullptr);
+  finish_pair<CT, DUAL>(ps, S, partials, s_area, tf >= 0 ? tf : S.tile_first, status_out ? status_out + p : nullptr);
 }
 
 // ---- launch wrappers --------------------------------------------------------
 
-template <bool CT, int LDS>
+template <bool CT, int LDS, bool DUAL>
 static hipError_t launch_t(const LaunchArgs& a, hipStream_t s) {
   if (a.phases & 1) {
-    hipLaunchKernelGGL((tiles_kernel<CT, LDS>), dim3(a.grid), dim3(kThreads), 0, s, a.pairs, a.tile_pair,
+    hipLaunchKernelGGL((tiles_kernel<CT, LDS, DUAL>), dim3(a.grid), dim3(kThreads), 0, s, a.pairs, a.tile_pair,
                        a.n_tiles_total, a.partials, a.corr_out, a.depth_out, a.xcd_remap, a.pair_tile_first);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
   }
   if ((a.phases & 2) && a.n_pairs > 0) {
-    hipLaunchKernelGGL((finish_kernel<CT>), dim3(a.n_pairs), dim3(64), 0, s, a.pairs, a.pair_first,
+    hipLaunchKernelGGL((finish_kernel<CT, DUAL>), dim3(a.n_pairs), dim3(64), 0, s, a.pairs, a.pair_first,
                        a.n_pairs, a.partials, a.pair_tile_first, a.status_out);
     return hipGetLastError();
   }
@@ -769,23 +843,32 @@ static hipError_t launch_t(const LaunchArgs& a, hipStream_t s) {
 }
 
 hipError_t launch_tiles(const LaunchArgs& a, hipStream_t s) {
+  if (a.dual) {
+    // the dual form caches at most 512 tree nodes in LDS: its third C tile
+    // takes the room of the larger caches
+    switch (a.lds_nodes) {
+      case 0: return launch_t<true, 0, true>(a, s);
+      case 256: return launch_t<true, 256, true>(a, s);
+      default: return launch_t<true, 512, true>(a, s);
+    }
+  }
   if (a.ct) {
     switch (a.lds_nodes) {
-      case 0: return launch_t<true, 0>(a, s);
-      case 256: return launch_t<true, 256>(a, s);
-      case 512: return launch_t<true, 512>(a, s);
-      default: return launch_t<true, 1024>(a, s);
+      case 0: return launch_t<true, 0, false>(a, s);
+      case 256: return launch_t<true, 256, false>(a, s);
+      case 512: return launch_t<true, 512, false>(a, s);
+      default: return launch_t<true, 1024, false>(a, s);
     }
   }
   switch (a.lds_nodes) {
-    case 0: return launch_t<false, 0>(a, s);
-    case 256: return launch_t<false, 256>(a, s);
-    case 512: return launch_t<false, 512>(a, s);
-    case 1024: return launch_t<false, 1024>(a, s);
-    default: return launch_t<false, 2048>(a, s);
+    case 0: return launch_t<false, 0, false>(a, s);
+    case 256: return launch_t<false, 256, false>(a, s);
+    case 512: return launch_t<false, 512, false>(a, s);
+    case 1024: return launch_t<false, 1024, false>(a, s);
+    default: return launch_t<false, 2048, false>(a, s);
   }
 }
 
-int slots(int ct) { return ct ? kCtSlots : kIcpSlots; }
+int slots(int ct, int dual) { return dual ? kDualSlots : ct ? kCtSlots : kIcpSlots; }
 
 }  // namespace kdh

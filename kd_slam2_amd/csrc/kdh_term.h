@@ -70,6 +70,7 @@ struct TermCache {
   float v[6];   // moving velocities [t; w] (CT only)
   V3f mroot;    // anchor in the moving tree frame
   V3f cf;       // anchor in the fixed tree frame
+  float vf[6];  // fixed velocities [t; w] (dual form only)
 };
 
 // Greedy descent (icp_impl_.h:116-121, node_.h:54-63).  Node record layout:
@@ -230,14 +231,17 @@ KDH_HD float rec_slot(const TermRec& r, int s) {
 // Robust weight and outcome (icp_impl_.h:138-151): the inlier test on the
 // reference's FP32 chi2 = e_ref^2; the weight and the chi2 statistics from the
 // residual R.e that enters b (equal to e_ref unless re-centred).
+// The dual form weighs differently (quadraticTermDual, cticp_impl_.h:240-248):
+// outliers are always weighted, inliers drop to 0 in stats mode.
+template <bool DUAL = false>
 KDH_HD void classify(TermRec& R, float e_ref, const TermParams& P) {
   R.chi2 = R.e * R.e;
   if (e_ref * e_ref > P.kernel_threshold) {
     R.outcome = kOutlier;
-    R.gamma = P.stats_mode ? 0.0f : sqrtf(P.kernel_threshold / R.chi2);
+    R.gamma = (P.stats_mode && !DUAL) ? 0.0f : sqrtf(P.kernel_threshold / R.chi2);
   } else {
     R.outcome = kInlier;
-    R.gamma = 1.0f;
+    R.gamma = (P.stats_mode && DUAL) ? 0.0f : 1.0f;
   }
 }
 
@@ -281,14 +285,20 @@ KDH_HD int icp_term(const NodeFetch& fetch, const TermCache& C, const TermParams
   R.J[0] = n.x; R.J[1] = n.y; R.J[2] = n.z;
   R.J[3] = jr.x; R.J[4] = jr.y; R.J[5] = jr.z;
   for (int i = 0; i < 6; ++i) R.Jv[i] = 0.0f;
-  classify(R, e, P);
+  classify<false>(R, e, P);
   return leaf;
 }
 
-// Continuous-time term (cticp_impl_.h:107-181).
-template <typename NodeFetch>
+// Continuous-time term (cticp_impl_.h:107-181).  With DUAL also the
+// quadraticTermDual extra (cticp_impl_.h:185-293): Jb, the Jacobian with
+// respect to the fixed side's velocities C.vf at the fixed leaf's own stamp
+// (dts_of(leaf)), errorAndJacobianVelocityB (d3/cticp_impl_.h:43-73):
+//   J_t = -dt n^T Rb ; J_w = dt (-n^T Rb dR_pb + dp^T Rb dR_nb)
+// with the fixed leaf warped by vf; re-centred, Rb^T n = Rmf wdir + nb_moved
+// and Rb^T dp = p_mif - pb_moved, in the fixed tree's frame.
+template <bool DUAL = false, typename NodeFetch, typename DtsFetch = int>
 KDH_HD int ct_term(const NodeFetch& fetch, const TermCache& C, const TermParams& P, const LeafIn& L, TermRec& R,
-                   int* depth_out) {
+                   int* depth_out, float* Jb = nullptr, const DtsFetch& dts_of = 0) {
   // applyVelocities on the leaf (d3/cticp_impl_.h:76-85)
   const float dt = L.dts;
   const V3f t_delta = v3(C.v[0] * dt, C.v[1] * dt, C.v[2] * dt);
@@ -340,7 +350,30 @@ KDH_HD int ct_term(const NodeFetch& fetch, const TermCache& C, const TermParams&
   const V3f jw = mul_t(dRp, nR) + mul_t(dRn, dpR);
   R.Jv[0] = nR.x * dt; R.Jv[1] = nR.y * dt; R.Jv[2] = nR.z * dt;
   R.Jv[3] = jw.x * dt; R.Jv[4] = jw.y * dt; R.Jv[5] = jw.z * dt;
-  classify(R, e, P);
+  if constexpr (DUAL) {
+    const float dtb = dts_of(leaf);
+    const V3f tb = v3(C.vf[0] * dtb, C.vf[1] * dtb, C.vf[2] * dtb);
+    const Rodrigues rb = rodrigues_f(v3(C.vf[3] * dtb, C.vf[4] * dtb, C.vf[5] * dtb));
+    const V3f pb_moved = mul(rb.R, pf) + tb;
+    const V3f nb_moved = mul(rb.R, nf);
+    V3f nbR, dpbR;  // Rb^T n and Rb^T dp
+    if (P.recenter) {
+      nbR = apply_rot(C.Xmf, wdir) + nb_moved;
+      dpbR = p_mif - pb_moved;
+    } else {
+      const M3f& Rb = C.Xf.R;
+      const V3f nb = n_m + apply_rot(C.Xf, nb_moved);
+      const V3f dpb = p_m - apply_iso(C.Xf, pb_moved);
+      nbR = mul_t(Rb, nb);
+      dpbR = mul_t(Rb, dpb);
+    }
+    const M3f dRpb = dRxp_f(rb, pf);
+    const M3f dRnb = dRxp_f(rb, nf);
+    const V3f jwb = mul_t(dRnb, dpbR) - mul_t(dRpb, nbR);
+    Jb[0] = -nbR.x * dtb; Jb[1] = -nbR.y * dtb; Jb[2] = -nbR.z * dtb;
+    Jb[3] = jwb.x * dtb; Jb[4] = jwb.y * dtb; Jb[5] = jwb.z * dtb;
+  }
+  classify<DUAL>(R, e, P);
   return leaf;
 }
 

@@ -69,6 +69,10 @@ def lib(fast=False):
                                          C.c_int64, f32, f32, C.POINTER(OrcParams), f32, f32, C.c_int,
                                          C.POINTER(OrcStats), C.POINTER(C.c_int), C.POINTER(C.c_int),
                                          C.POINTER(OrcStats)]
+        L.orc_build_dual.restype = C.c_int
+        L.orc_build_dual.argtypes = [vp, C.c_int64, vp, vp, C.c_int64, f64, f64, f64, f64, C.POINTER(OrcParams),
+                                     C.c_int, f32, f64, C.POINTER(OrcStats), i32]
+        L.orc_jacobian_velocity_b.argtypes = [f32, f32, f32, f32, f32, f32, C.c_float, f32]
         L.orc_tree_apply_velocities.argtypes = [vp, C.c_int64, f32, C.c_int]
         L.orc_tree_apply_isometry.argtypes = [vp, C.c_int64, f32]
         L.orc_extract_leaves.argtypes = [vp, vp, C.c_int64, f32, f32]
@@ -201,6 +205,33 @@ def compute_forest(fixed_list, Xf_list, moving, X0, params, ct=0, v0=None, prior
     Xo[:3, :4] = X.reshape(3, 4).astype(np.float64)
     return (Xo, v.astype(np.float64), [log[i].as_dict() for i in range(nlog.value)], res.value,
             [ent[i].as_dict() for i in range(n)])
+
+
+DUAL_BLOCKS = (("H", 0, (12, 12)), ("b", 144, (12,)), ("H_vel_B", 156, (6, 6)), ("b_vel_B", 192, (6,)),
+               ("H_cross_pose_vel_B", 198, (6, 6)), ("H_cross_vel_AB", 234, (6, 6)))
+
+
+def build_form_dual(fixed, moving, Xm, vm, Xf, vf, params, stats_mode=False):
+    """CTICP_::buildQuadraticFormDual at an FP64 state (moving pose and
+    velocities, fixed pose and velocities): the FP32 reference blocks, the
+    FP64 twin's (suffix 64), the stats and the correspondences."""
+    out32 = np.zeros(270, dtype=np.float32)
+    out64 = np.zeros(270)
+    st = OrcStats()
+    n = len(moving.leaves)
+    corr = np.zeros(max(n, 1), dtype=np.int32)
+    a = lambda X: np.array(np.asarray(X, dtype=np.float64)[:3, :4].reshape(12))  # noqa: E731
+    Xm34, Xf34 = a(Xm), a(Xf)
+    vm6, vf6 = np.array(vm, dtype=np.float64), np.array(vf, dtype=np.float64)
+    lib().orc_build_dual(fixed.nodes.ctypes.data, len(fixed.nodes), moving.nodes.ctypes.data,
+                         moving.leaves.ctypes.data, n, p64(Xm34), p64(vm6), p64(Xf34), p64(vf6), C.byref(params),
+                         int(stats_mode), p32(out32), p64(out64), C.byref(st), pi32(corr))
+    d = dict(stats=st.as_dict(), corr=corr[:n])
+    for name, off, shape in DUAL_BLOCKS:
+        size = int(np.prod(shape))
+        d[name] = out32[off:off + size].reshape(shape).copy()
+        d[name + "64"] = out64[off:off + size].reshape(shape).copy()
+    return d
 
 
 def tree_apply_velocities(nodes, v, skip_leaves=False):

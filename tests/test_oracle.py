@@ -335,3 +335,116 @@ def test_forest_of_one_is_the_plain_compute_and_blocks_recover_the_pose():
     t, r = synth.pose_error(X, w["X_gt"])
     t0, r0 = synth.pose_error(w["X0"], w["X_gt"])
     assert t < t0 / 5 and r < r0 / 5
+
+
+def test_velocity_b_jacobian_fp64_finite_differences():
+    """errorAndJacobianVelocityB: the FP32 analytic Jacobian of the error with
+    respect to the fixed side's velocities against FP64 central differences."""
+    rng = np.random.default_rng(16)
+    for _ in range(50):
+        Tb = np.eye(4); Tb[:3, :3] = _rot(rng.normal(size=3)); Tb[:3, 3] = rng.uniform(-2, 2, 3)
+        pa = rng.uniform(-3, 3, 3); na = rng.normal(size=3); na /= np.linalg.norm(na)
+        pb = rng.uniform(-3, 3, 3); nb = rng.normal(size=3); nb /= np.linalg.norm(nb)
+        vb = np.concatenate([rng.uniform(-.5, .5, 3), rng.uniform(-.3, .3, 3)])
+        dt = 0.07
+
+        def err(v):
+            m = v * dt
+            R = _rot(m[3:])
+            pg = Tb[:3, :3] @ (R @ pb + m[:3]) + Tb[:3, 3]
+            ng = Tb[:3, :3] @ (R @ nb)
+            return (pa - pg) @ (na + ng)
+
+        J = np.zeros(6, dtype=np.float32)
+        L.orc_jacobian_velocity_b(O.p32(f32(Tb[:3, :4].reshape(12))), O.p32(f32(pa)), O.p32(f32(na)),
+                                  O.p32(f32(pb)), O.p32(f32(nb)), O.p32(f32(vb)), np.float32(dt), O.p32(J))
+        hh = 1e-6
+        Jn = np.array([(err(vb + np.eye(6)[k] * hh) - err(vb - np.eye(6)[k] * hh)) / (2 * hh) for k in range(6)])
+        assert np.abs(J - Jn).max() < 1e-3 * max(1.0, np.abs(Jn).max())
+
+
+def test_dual_form_a_side_is_the_ct_form():
+    """Outside stats mode the dual's weight rule equals quadraticTerm's, so its
+    moving-side blocks are the CT form (FP64 twins to rounding, counts exact);
+    its dual blocks are symmetric where they must be."""
+    from .workloads import cfg2, icp_params
+
+    w = cfg2(n_map=200_000, n_scan=16_384)
+    p = O.from_kdh_params(icp_params(max_error=1.0, velocity_damping=10.0), 8)
+    vf = np.array([1.0, 0.0, 0.0, 0.0, 0.0, 0.2])
+    d = O.build_form_dual(w["fixed"], w["moving"], w["X0"], w["v0"], w["Xf"], vf, p)
+    ref = O.build_form_at(w["fixed"], w["moving"], w["X0"], w["Xf"], p, ct=1, v=w["v0"])
+    assert np.array_equal(d["corr"], ref["corr"])
+    assert np.abs(d["H64"] - ref["H64"]).max() <= 1e-12 * np.abs(ref["H64"]).max()
+    assert np.abs(d["b64"] - ref["b64"]).max() <= 1e-12 * np.abs(ref["b64"]).max()
+    assert np.allclose(d["H_vel_B64"], d["H_vel_B64"].T)
+    assert d["stats"]["num_inliers"] > 100
+    # stats mode: inliers drop out of H, outliers stay weighted
+    s = O.build_form_dual(w["fixed"], w["moving"], w["X0"], w["v0"], w["Xf"], vf, p, stats_mode=True)
+    assert s["stats"]["num_inliers"] == d["stats"]["num_inliers"]
+    assert np.abs(s["H64"]).max() < np.abs(d["H64"]).max()
+
+
+def _dual_full(d, suffix=""):
+    """The 18x18 normal matrix over [pose A, vel A, vel B] and its b."""
+    G = np.zeros((18, 18))
+    G[:12, :12] = d["H" + suffix]
+    G[:6, 12:] = d["H_cross_pose_vel_B" + suffix]
+    G[6:12, 12:] = d["H_cross_vel_AB" + suffix]
+    G[12:, :12] = G[:12, 12:].T
+    G[12:, 12:] = d["H_vel_B" + suffix]
+    return G, np.concatenate([d["b" + suffix], d["b_vel_B" + suffix]])
+
+
+def test_dual_single_term_is_rank_one_and_b_vel_b_is_the_chi2_gradient():
+    """One moving leaf gives one term: the 18x18 dual normal matrix is
+    g u u^T and b = g e u for u = [J_pose, J_vel_A, J_vel_B], so
+    b b^T = g e^2 G entry for entry -- a transposed or misplaced block
+    breaks it.  The residual takes the fixed leaf as stored; J_vel_B is
+    the error's derivative when the fixed cloud moves with velocities v_B over
+    its stamps.  At v_B = 0 that motion is the tree warp treeApplyVelocities
+    (an independent routine), so for an inlier (g = 1) the central difference
+    of chi2 over fixed trees warped by +-h e_k is 2 b_vel_B: this pins the
+    leaf stamp, the sides and the sign J_vel_B enters the form with."""
+    import types
+
+    from .workloads import cfg2, icp_params
+
+    w = cfg2(n_map=200_000, n_scan=16_384)
+    p = O.from_kdh_params(icp_params(max_error=1.0, velocity_damping=10.0), 8)
+    fixed = w["moving"]  # a stamped cloud as the fixed side, so J_vel_B is live
+    moving = w["moving"]
+    X_fixed = w["X0"]
+    zero = np.zeros(6)
+    checked = 0
+    for leaf in moving.leaves[::11]:
+        one = types.SimpleNamespace(nodes=moving.nodes, leaves=np.array([leaf], dtype=np.uint64))
+        d = O.build_form_dual(fixed, one, w["X_gt"], w["v0"], X_fixed, zero, p)
+        if d["stats"]["num_inliers"] != 1:
+            continue
+        G, b = _dual_full(d, "64")
+        assert np.abs(G - G.T).max() == 0.0
+        # b b^T = (g e^2) G: the ratio from the largest entry fits every entry
+        # (FP64 twin), and it is chi2_kernelised (FP32 e, so 1e-4)
+        bb = np.outer(b, b)
+        m = np.unravel_index(np.abs(G).argmax(), G.shape)
+        c = bb[m] / G[m]
+        assert np.abs(c * G - bb).max() <= 1e-9 * np.abs(bb).max()
+        assert abs(c - d["stats"]["chi2_kernelized"]) <= 1e-4 * c
+        assert np.abs(d["b_vel_B64"]).max() > 0.0
+        h = 1e-2
+        fd = np.zeros(6)
+        for k in range(6):
+            chi = []
+            for s in (1.0, -1.0):
+                warped = types.SimpleNamespace(nodes=O.tree_apply_velocities(fixed.nodes, s * h * np.eye(6)[k]),
+                                               leaves=fixed.leaves)
+                ds = O.build_form_dual(warped, one, w["X_gt"], w["v0"], X_fixed, zero, p)
+                assert ds["stats"]["num_inliers"] == 1 and np.array_equal(ds["corr"], d["corr"])
+                chi.append(ds["stats"]["chi2"])
+            fd[k] = (chi[0] - chi[1]) / (2 * h)
+        assert np.abs(fd - 2 * d["b_vel_B64"]).max() <= 2e-2 * np.abs(fd).max(), (fd, d["b_vel_B64"])
+        checked += 1
+        if checked == 8:
+            break
+    assert checked >= 4

@@ -62,3 +62,17 @@ def forest_cfg1(n_blocks=3):
         trees.append(g.makeTree(local.astype(np.float32)))
         poses.append(w["Xf"] @ F)
     return dict(w=w, trees=trees, poses=poses)
+
+
+@functools.lru_cache(maxsize=None)
+def dual_factors(n_scan=65_536, n_frames=4):
+    """Bundle-adjustment factors between consecutive CT sweeps of one drive
+    (the serving-mix odometry generator, sequence 0): factor k has sweep k as
+    the fixed cloud (its stamps give J_vel_B) at its true pose and velocities,
+    and sweep k+1 as the moving cloud at its perturbed initial state."""
+    seq = synth.config5_sequence(0, n_map=1000, n_scan=n_scan, n_frames=n_frames)
+    g = TreeGenerator3D()
+    trees = [g.makeTree(f["scan"], f["stamps"]) for f in seq["frames"]]
+    fr = seq["frames"]
+    return [dict(fixed=trees[k], moving=trees[k + 1], X_moving=fr[k + 1]["X0"], v_moving=fr[k + 1]["v0"],
+                 X_fixed=fr[k]["X_gt"], v_fixed=fr[k]["v_gt"]) for k in range(n_frames - 1)]
