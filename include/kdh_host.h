@@ -52,6 +52,26 @@ const uint64_t* kdh_host_tree_leaves(const kdh_host_tree* t);
 const float* kdh_host_tree_points(const kdh_host_tree* t);
 int kdh_host_tree_free(kdh_host_tree* t);
 
+/* ---- tree generation on the device ----------------------------------------
+ * Replaces TreeGenerator_::makeTree (tree/tree_generator_impl_.h:9-19) plus
+ * the device upload (Tree_CUDAfromCPU_, tree/tree_cuda_impl_.h:198-222): the
+ * tree is built, pruned and breadth-first ordered on the GPU and stays there,
+ * bit-identical to kdh_host_tree_build's (same moments order, eigen-solver,
+ * labelling and split rule).  xyz: 3 floats per point; stamps: one double per
+ * point or NULL; both host pointers, or device pointers of this context's
+ * device when inputs_on_device.  *out is NULL (and KDH_OK returned) where
+ * makeTree returns nullptr.  params.parallel_level is ignored.  Free with
+ * kdh_tree_free. */
+int kdh_tree_build(kdh_ctx* ctx, const float* xyz, const double* stamps, int64_t n_points,
+                   const kdh_tree_gen_params* params, int inputs_on_device, kdh_tree** out);
+
+/* A device-built tree's full Node_ records (n_nodes), leaf list (n_leaves),
+ * permuted points (3 floats each) and stamps (kdh_tree_info gives the
+ * counts); any output may be NULL.  TreeCPU_'s nodes / leaves / points
+ * (tree/tree_cpu_.h).  KDH_ERR_INVALID for a tree made by kdh_tree_upload. */
+int kdh_tree_download(kdh_ctx* ctx, const kdh_tree* tree, kdh_node3f* nodes, uint64_t* leaves, float* xyz,
+                      double* stamps);
+
 #ifdef __cplusplus
 }
 #endif

@@ -245,6 +245,8 @@ _SIGS = {
     "kdh_server_info_get": (C.c_int, [P, C.POINTER(ServerInfo)]),
     "kdh_tree_gen_params_default": (None, [C.POINTER(TreeGenParams)]),
     "kdh_host_tree_build": (C.c_int, [F32P, F64P, C.c_int64, C.POINTER(TreeGenParams), C.POINTER(P)]),
+    "kdh_tree_build": (C.c_int, [P, F32P, F64P, C.c_int64, C.POINTER(TreeGenParams), C.c_int, C.POINTER(P)]),
+    "kdh_tree_download": (C.c_int, [P, P, P, P, F32P, F64P]),
     "kdh_host_tree_info": (C.c_int, [P, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_double)]),
     "kdh_host_tree_nodes": (P, [P]),
     "kdh_host_tree_leaves": (P, [P]),
@@ -308,6 +310,9 @@ def adapter():
         A.kdha_align.restype = C.c_int
         A.kdha_align.argtypes = [C.c_char_p, P, P, P, F64P, F64P, F64P, C.POINTER(Params), C.c_int, C.c_int, F64P,
                                  F64P, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(Stats)]
+        A.kdha_build_dual.restype = C.c_int
+        A.kdha_build_dual.argtypes = [P, P, P, F64P, F64P, F64P, F64P, C.POINTER(Params), C.c_int,
+                                      C.POINTER(DualForm)]
         _adapter = A
     return _adapter
 

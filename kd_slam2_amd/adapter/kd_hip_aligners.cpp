@@ -464,6 +464,43 @@ int AlignerHIP3D<CT>::computeOnDevice(bool stats_mode) {
 template struct AlignerHIP3D<0>;
 template struct AlignerHIP3D<1>;
 
+void CTICPHIP3D::setFixedVelocities(const Vector6f& vv) {
+  for (int i = 0; i < 6; ++i) vf[i] = vv[i];
+}
+
+void CTICPHIP3D::buildQuadraticFormDual(bool stats_mode) {
+  syncParams();
+  if (!_fixed || !_moving) throw std::runtime_error("setFixed/setMoving not called");
+  kdh_ctx* ctx = _ctx ? _ctx : defaultContext(0);
+  kdh_dual_pair in;
+  memset(&in, 0, sizeof(in));
+  in.fixed = _fixed;
+  in.moving = _moving;
+  memcpy(in.X_moving, X, sizeof(in.X_moving));
+  memcpy(in.v_moving, v, sizeof(in.v_moving));
+  memcpy(in.X_fixed, Xf, sizeof(in.X_fixed));
+  memcpy(in.v_fixed, vf, sizeof(in.v_fixed));
+  in.params = params;
+  kdh_dual_form out;
+  check(ctx, kdh_batch_build_dual(ctx, 1, &in, stats_mode ? 1 : 0, &out), "kdh_batch_build_dual");
+  memcpy(H, out.H, sizeof(H));
+  memcpy(b, out.b, sizeof(b));
+  memcpy(H_dual_vel_B, out.H_dual_vel_B, sizeof(H_dual_vel_B));
+  memcpy(b_dual_vel_B, out.b_dual_vel_B, sizeof(b_dual_vel_B));
+  memcpy(H_dual_cross_pose_vel_B, out.H_dual_cross_pose_vel_B, sizeof(H_dual_cross_pose_vel_B));
+  memcpy(H_dual_cross_vel_AB, out.H_dual_cross_vel_AB, sizeof(H_dual_cross_vel_AB));
+  const kdh_stats& st = out.stats;
+  stats.num_inliers = st.num_inliers;
+  stats.num_outliers = st.num_outliers;
+  stats.num_bad = st.num_bad;
+  stats.num_err_normal = st.num_err_normal;
+  stats.num_err_fn = st.num_err_fn;
+  stats.chi2 = (float)st.chi2;
+  stats.chi2_kernelized = (float)st.chi2_kernelized;
+  stats.pert_pose_norm = 0;
+  stats.pert_vel_norm = 0;
+}
+
 // ---- registration (d3/icp.cu:13-17, d3/cticp.cu:14-18, d3/tree.cu:22-30) ----
 void registerHIPTypes3D() {
   BOSS_REGISTER_CLASS(ICPHIP3D);
@@ -558,6 +595,54 @@ int kdha_align(const char* type_name, kdh_ctx* ctx, const kdh_tree* fixed, const
       return run_aligner(c, ctx, fixed, moving, X0, v0, Xf, p, on_device, stats_mode, X_out, v_out, result, n_iter,
                          last);
     return KDH_ERR_INVALID;
+  } catch (const std::exception&) {
+    return KDH_ERR_INVALID;
+  }
+}
+
+// Creates CTICPHIP3D through the registry and runs buildQuadraticFormDual
+// at the given states; the blocks come back in a kdh_dual_form.
+int kdha_build_dual(kdh_ctx* ctx, const kdh_tree* fixed, const kdh_tree* moving, const double X_moving[12],
+                    const double v_moving[6], const double X_fixed[12], const double v_fixed[6], const kdh_params* p,
+                    int stats_mode, kdh_dual_form* out) {
+  if (!ctx || !fixed || !moving || !X_moving || !v_moving || !X_fixed || !v_fixed || !p || !out)
+    return KDH_ERR_INVALID;
+  try {
+    auto obj = srrg2_core::TypeRegistry::instance().create("CTICPHIP3D");
+    auto* c = dynamic_cast<CTICPHIP3D*>(obj.get());
+    if (!c) return KDH_ERR_INVALID;
+    c->setContext(ctx);
+    c->param_kernel_threshold.setValue(p->kernel_threshold);
+    c->param_damping.setValue(p->damping);
+    c->param_max_error.setValue(p->max_error);
+    c->param_min_normal_cos.setValue(p->min_normal_cos);
+    c->param_min_good_measurements.setValue(p->min_good_measurements);
+    c->param_max_iterations.setValue(p->max_iterations);
+    c->param_velocity_damping.setValue(p->velocity_damping);
+    c->param_recenter.setValue(p->recenter != 0);
+    c->setFixed(fixed);
+    c->setMoving(moving);
+    memcpy(c->X, X_moving, sizeof(c->X));
+    memcpy(c->v, v_moving, sizeof(c->v));
+    memcpy(c->Xf, X_fixed, sizeof(c->Xf));
+    memcpy(c->vf, v_fixed, sizeof(c->vf));
+    c->buildQuadraticFormDual(stats_mode != 0);
+    memcpy(out->H, c->H, sizeof(out->H));
+    memcpy(out->b, c->b, sizeof(out->b));
+    memcpy(out->H_dual_vel_B, c->H_dual_vel_B, sizeof(out->H_dual_vel_B));
+    memcpy(out->b_dual_vel_B, c->b_dual_vel_B, sizeof(out->b_dual_vel_B));
+    memcpy(out->H_dual_cross_pose_vel_B, c->H_dual_cross_pose_vel_B, sizeof(out->H_dual_cross_pose_vel_B));
+    memcpy(out->H_dual_cross_vel_AB, c->H_dual_cross_vel_AB, sizeof(out->H_dual_cross_vel_AB));
+    const ICPStats& s = c->stats;
+    memset(&out->stats, 0, sizeof(out->stats));
+    out->stats.num_inliers = s.num_inliers;
+    out->stats.num_outliers = s.num_outliers;
+    out->stats.num_bad = s.num_bad;
+    out->stats.num_err_normal = s.num_err_normal;
+    out->stats.num_err_fn = s.num_err_fn;
+    out->stats.chi2 = s.chi2;
+    out->stats.chi2_kernelized = s.chi2_kernelized;
+    return KDH_OK;
   } catch (const std::exception&) {
     return KDH_ERR_INVALID;
   }

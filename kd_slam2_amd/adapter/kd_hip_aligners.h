@@ -186,7 +186,19 @@ struct AlignerHIP3D : public srrg2_core::Configurable {
 };
 
 struct ICPHIP3D : public AlignerHIP3D<0> {};
-struct CTICPHIP3D : public AlignerHIP3D<1> {};
+struct CTICPHIP3D : public AlignerHIP3D<1> {
+  // fixed_state.velocities (cticp/cticp_.h): the fixed cloud's motion, used
+  // only by the dual form
+  void setFixedVelocities(const Vector6f& v);
+  // CTICP_::buildQuadraticFormDual (cticp/cticp_impl_.h:42-45): H, b and the
+  // fixed side's velocity blocks (cticp/cticp_.h:179-186), row-major
+  void buildQuadraticFormDual(bool stats_mode = false);
+  double vf[6] = {0, 0, 0, 0, 0, 0};
+  double H_dual_vel_B[36] = {};
+  double b_dual_vel_B[6] = {};
+  double H_dual_cross_pose_vel_B[36] = {};  // rows: pose
+  double H_dual_cross_vel_AB[36] = {};      // rows: moving velocities
+};
 
 void registerHIPTypes3D();
 

@@ -81,6 +81,12 @@ class TreeGenerator3D:
             L.lib().kdh_host_tree_free(h)
         return HostTree(nodes, leaves, pp, ts.value)
 
+    def makeTreeOnDevice(self, ctx, points, stamps=None, download=True):
+        """makeTree on the GPU (kdh_tree_build): a DeviceTree, or None where
+        the reference returns nullptr.  The tree is bit-identical to
+        makeTree's; with download its full records come back as .host."""
+        return DeviceTree.build(ctx, points, stamps, self._p, download)
+
 
 # ---------------------------------------------------------------------------
 # Device context and trees
@@ -146,9 +152,42 @@ class DeviceTree:
         ctx.check(rc, "kdh_tree_upload")
         self.h = h
 
+    @classmethod
+    def build(cls, ctx, points, stamps, gen_params, download=True):
+        """Build the tree on the device (kdh_tree_build); None where makeTree
+        returns nullptr."""
+        pts = np.ascontiguousarray(points, dtype=np.float32).reshape(-1, 3)
+        st = None if stamps is None else np.ascontiguousarray(stamps, dtype=np.float64)
+        h = C.c_void_p()
+        ctx.check(L.lib().kdh_tree_build(ctx.h, L.f32p(pts), None if st is None else L.f64p(st), len(pts),
+                                         C.byref(gen_params), 0, C.byref(h)), "kdh_tree_build")
+        if not h.value:
+            return None
+        self = cls.__new__(cls)
+        self.ctx = ctx
+        self.h = h
+        self.host = None
+        if download:
+            nn, nl = self._counts()
+            nodes = np.zeros(nn, dtype=L.NODE_DTYPE)
+            leaves = np.zeros(nl, dtype=np.uint64)
+            xyz = np.zeros((len(pts), 3), dtype=np.float32)
+            ts = np.zeros(len(pts))
+            ctx.check(L.lib().kdh_tree_download(ctx.h, h, nodes.ctypes.data_as(C.c_void_p),
+                                                leaves.ctypes.data_as(C.c_void_p), L.f32p(xyz), L.f64p(ts)),
+                      "kdh_tree_download")
+            self.host = HostTree(nodes, leaves, xyz, float(ts.min()) if len(ts) else 0.0)
+            self.stamps = ts
+        return self
+
+    def _counts(self):
+        nn, nl = C.c_int64(), C.c_int64()
+        L.lib().kdh_tree_info(self.h, C.byref(nn), C.byref(nl), None)
+        return nn.value, nl.value
+
     @property
     def num_leaves(self):
-        return self.host.num_leaves
+        return self._counts()[1]
 
     def device_bytes(self):
         nb = C.c_int64()
@@ -170,7 +209,7 @@ class DeviceTree:
     def extractCompressedLeaves(self, v=None):
         """(n_leaves, 2, 3) float32: per leaf its mean and direction, warped by
         v when given."""
-        out = np.zeros((max(self.host.num_leaves, 0), 2, 3), dtype=np.float32)
+        out = np.zeros((max(self.num_leaves, 0), 2, 3), dtype=np.float32)
         vp = None if v is None else L.f32p(np.ascontiguousarray(v, dtype=np.float32))
         if len(out):
             self.ctx.check(L.lib().kdh_tree_extract_leaves(self.ctx.h, self.h, vp, L.f32p(out)),
@@ -179,7 +218,7 @@ class DeviceTree:
 
     def readNodes(self):
         """(n_nodes, 2, 3) float32: every node's mean and direction on the device."""
-        out = np.zeros((len(self.host.nodes), 2, 3), dtype=np.float32)
+        out = np.zeros((self._counts()[0], 2, 3), dtype=np.float32)
         if len(out):
             self.ctx.check(L.lib().kdh_tree_read_nodes(self.ctx.h, self.h, L.f32p(out)), "kdh_tree_read_nodes")
         return out
@@ -619,6 +658,22 @@ def adapter_align(type_name, ctx, fixed, moving, X0, Xf, params, v0=None, on_dev
     X = np.eye(4)
     X[:3, :4] = Xo.reshape(3, 4)
     return X, vo, res.value, nit.value, st.as_dict()
+
+
+def adapter_build_dual(ctx, fixed, moving, X_moving, v_moving, X_fixed, v_fixed, params, stats_mode=False):
+    """CTICPHIP3D::buildQuadraticFormDual through the C++ plugin registry;
+    returns the kdh_dual_form fields as a dict."""
+    A = L.adapter()
+    a34 = lambda X: np.ascontiguousarray(np.asarray(X, dtype=np.float64)[:3, :4].reshape(12))  # noqa: E731
+    a6 = lambda v: np.ascontiguousarray(np.asarray(v, dtype=np.float64).reshape(6))  # noqa: E731
+    Xm, vm, Xf, vf = a34(X_moving), a6(v_moving), a34(X_fixed), a6(v_fixed)
+    out = L.DualForm()
+    rc = A.kdha_build_dual(ctx.h, fixed.h, moving.h, L.f64p(Xm), L.f64p(vm), L.f64p(Xf), L.f64p(vf),
+                           C.byref(params), int(stats_mode), C.byref(out))
+    if rc != L.KDH_OK:
+        raise RuntimeError(f"kdha_build_dual failed: {L.ERRORS.get(rc, rc)}")
+    return {name: np.array(getattr(out, name)) for name, _ in L.DualForm._fields_ if name != "stats"} | dict(
+        stats=out.stats.as_dict())
 
 
 _PAIR_DT = None

@@ -17,6 +17,13 @@ struct kdh_tree {
   kdh::DevTree dt{};
   kdh::V3f root_mean{};
   float* node_dts = nullptr;  // per node (float)_dts_mean, for the in-place deskew
+  // trees built on the device (kdh_tree_build) also keep, in dmem, the full
+  // Node_ records, the leaf list and the permuted points
+  kdh_node3f* full_nodes = nullptr;
+  uint64_t* full_leaves = nullptr;
+  float* full_xyz = nullptr;
+  double* full_stamps = nullptr;
+  int64_t n_points = 0;
 };
 
 struct kdh_ctx {
@@ -44,6 +51,7 @@ struct kdh_ctx {
   size_t depth_cap = 0;
   float* d_extract = nullptr;  // staging of tree read-backs
   size_t extract_cap = 0;
+  void* tree_build_ws = nullptr;  // device tree builder workspace (kdh_tree_build.hip)
   // Instantiated graphs, reused while the launch shape and buffers match.
   struct GraphSlot {
     hipGraphExec_t exec = nullptr;
@@ -66,6 +74,7 @@ struct kdh_ctx {
 };
 
 int kdh_fail(kdh_ctx* ctx, int code, const std::string& msg);
+void kdh_tree_build_ws_free(kdh_ctx* ctx);
 int kdh_hip_fail(kdh_ctx* ctx, hipError_t e, const char* where);
 int kdh_ensure_pairs(kdh_ctx* ctx, size_t n_pairs);
 int kdh_ensure_partials(kdh_ctx* ctx, size_t n_floats);

@@ -8,16 +8,9 @@
 //     label_policy_base_impl_.h:13-97),
 //   prune with IsGoodLeafPredicate_ (tree_predicates_.h:8-29, tree_cpu_.h:173-207),
 //   breadth-first renumbering (tree_cpu_impl_.h:48-74, 137-158) and the leaf list.
-// Differences from the reference, none of which change what the aligner reads
-// from a given tree:
-//   * node mean, covariance and stamp moments are computed in FP64 with a
-//     centred two-pass formula (the reference sums E[xx^T] in FP32 and
-//     subtracts mu mu^T, which cancels badly far from the origin), then
-//     rounded to FP32;
-//   * the 3x3 symmetric eigenproblem is solved by cyclic Jacobi in FP64
-//     (the reference calls Eigen's SelfAdjointEigenSolver::computeDirect);
-//   * a split that leaves one side empty turns the node into a leaf (the
-//     reference would recurse on the same range without end).
+// The moments, the eigen-solver, the labelling and the split rule are shared
+// with the device builder (kdh_treegen.h, which lists where this build fixes
+// what the reference leaves open); both give the same tree bit for bit.
 #include <math.h>
 #include <string.h>
 
@@ -30,282 +23,129 @@
 
 #include "../../include/kdh.h"
 #include "../../include/kdh_host.h"
+#include "kdh_treegen.h"
 
 namespace {
+
+using kdh::tg::kChunk;
+using kdh::tg::Label;
 
 struct Pt {
   float x, y, z;
   double stamp;
 };
 
-struct Node {
-  kdh_node3f n;
-};
-
-struct Meta {
-  double mean[3] = {0, 0, 0};
-  float meanf[3] = {0, 0, 0};
-  double evals[3] = {0, 0, 0};  // ascending
-  float evec[3][3] = {};        // evec[k] = k-th eigenvector (ascending eigenvalue)
-  size_t n_points = 0;
-  bool is_node = false, is_leaf = false, is_bad = true;
-  int level = 0;
-  double ts_start = 0;
-  bool is_flat = false, has_propagated_normal = false;
-  float propagated_normal[3] = {0, 0, 0};
-};
-
-static inline float dotf(const float a[3], const float b[3]) { return (a[0] * b[0] + a[1] * b[1]) + a[2] * b[2]; }
-
-// Cyclic Jacobi eigen-decomposition of a symmetric 3x3 matrix (FP64).
-// Returns eigenvalues ascending and unit eigenvectors as columns of V.
-static void eig3(const double A_in[3][3], double w[3], double V[3][3]) {
-  double A[3][3];
-  memcpy(A, A_in, sizeof(A));
-  for (int i = 0; i < 3; ++i)
-    for (int j = 0; j < 3; ++j) V[i][j] = (i == j) ? 1.0 : 0.0;
-  for (int sweep = 0; sweep < 50; ++sweep) {
-    double off = fabs(A[0][1]) + fabs(A[0][2]) + fabs(A[1][2]);
-    double scale = fabs(A[0][0]) + fabs(A[1][1]) + fabs(A[2][2]);
-    if (off <= 1e-300 || off <= 1e-18 * scale) break;
-    for (int p = 0; p < 2; ++p)
-      for (int q = p + 1; q < 3; ++q) {
-        if (fabs(A[p][q]) <= 1e-300) continue;
-        double theta = (A[q][q] - A[p][p]) / (2.0 * A[p][q]);
-        double t = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
-        double c = 1.0 / sqrt(t * t + 1.0), s = t * c;
-        for (int k = 0; k < 3; ++k) {  // A <- J^T A J
-          double akp = A[k][p], akq = A[k][q];
-          A[k][p] = c * akp - s * akq;
-          A[k][q] = s * akp + c * akq;
-        }
-        for (int k = 0; k < 3; ++k) {
-          double apk = A[p][k], aqk = A[q][k];
-          A[p][k] = c * apk - s * aqk;
-          A[q][k] = s * apk + c * aqk;
-        }
-        for (int k = 0; k < 3; ++k) {
-          double vkp = V[k][p], vkq = V[k][q];
-          V[k][p] = c * vkp - s * vkq;
-          V[k][q] = s * vkp + c * vkq;
-        }
-      }
-  }
-  int idx[3] = {0, 1, 2};
-  double d[3] = {A[0][0], A[1][1], A[2][2]};
-  std::sort(idx, idx + 3, [&](int a, int b) { return d[a] < d[b]; });
-  double Vs[3][3];
-  for (int k = 0; k < 3; ++k) {
-    w[k] = d[idx[k]];
-    for (int i = 0; i < 3; ++i) Vs[i][k] = V[i][idx[k]];
-  }
-  memcpy(V, Vs, sizeof(Vs));
-}
-
 struct Builder {
   const kdh_tree_gen_params& P;
   Pt* pts;
-  std::vector<Node> nodes;
-  explicit Builder(const kdh_tree_gen_params& p, Pt* points) : P(p), pts(points) {}
+  double ts_start;
+  std::vector<kdh_node3f> nodes;
+  std::vector<Pt> tmp;
+  std::vector<double> acc;
+  Builder(const kdh_tree_gen_params& p, Pt* points, double ts0) : P(p), pts(points), ts_start(ts0) {}
 
-  // LabelPolicyBase_::fillMetadata (label_policy_base_impl_.h:13-97)
-  void fill_base(Meta& m, kdh_node3f& node, const Meta* parent) const {
-    const size_t s = node.idx_start, e = node.idx_end;
-    m.n_points = e - s;
-    if (!m.n_points) {
-      m.is_bad = true;
-      return;
-    }
-    if (!parent) {
-      double ts = std::numeric_limits<double>::max();
-      for (size_t i = s; i < e; ++i) ts = std::min(ts, pts[i].stamp);
-      m.ts_start = ts;
-    } else {
-      m.ts_start = parent->ts_start;
-    }
-    m.level = parent ? parent->level + 1 : 0;
-    const double inv = 1.0 / (double)m.n_points;
-    double t1 = 0, t2 = 0;
-    for (size_t i = s; i < e; ++i) {
-      double dts = pts[i].stamp - m.ts_start;
-      t1 += dts;
-      t2 += dts * dts;
-    }
-    t1 *= inv;
-    t2 *= inv;
-    node.dts_mean = t1;
-    node.dts_covariance = t2 - t1 * t1;
-    double mu[3] = {0, 0, 0};
-    for (size_t i = s; i < e; ++i) {
-      mu[0] += pts[i].x;
-      mu[1] += pts[i].y;
-      mu[2] += pts[i].z;
-    }
-    for (int k = 0; k < 3; ++k) {
-      mu[k] *= inv;
-      m.mean[k] = mu[k];
-      m.meanf[k] = (float)mu[k];
-      node.mean[k] = m.meanf[k];
-    }
-    if (m.n_points < 3) {
-      node.direction[0] = node.direction[1] = node.direction[2] = 0.0f;
-      m.is_bad = true;
-      return;
-    }
-    double C[3][3] = {};
-    for (size_t i = s; i < e; ++i) {
-      double d[3] = {pts[i].x - mu[0], pts[i].y - mu[1], pts[i].z - mu[2]};
-      for (int r = 0; r < 3; ++r)
-        for (int c = r; c < 3; ++c) C[r][c] += d[r] * d[c];
-    }
-    for (int r = 0; r < 3; ++r)
-      for (int c = r; c < 3; ++c) {
-        C[r][c] *= inv;
-        C[c][r] = C[r][c];
+  // The node's moments in the canonical order: s1 = sums of (x, y, z, dts,
+  // dts^2); c2 = sums of the centred products (xx, xy, xz, yy, yz, zz).
+  void moments(size_t s, size_t e, double s1[5], double c2[6]) {
+    const size_t n = e - s, m = (n + kChunk - 1) / kChunk;
+    acc.assign(m * 6, 0.0);
+    double(*v5)[5] = reinterpret_cast<double(*)[5]>(acc.data());
+    for (size_t c = 0; c < m; ++c) {
+      const size_t b = s + c * kChunk, t = std::min(b + kChunk, e);
+      double* a = v5[c];
+      for (size_t i = b; i < t; ++i) {
+        const double dts = pts[i].stamp - ts_start;
+        const double it[5] = {(double)pts[i].x, (double)pts[i].y, (double)pts[i].z, dts, dts * dts};
+        for (int k = 0; k < 5; ++k) a[k] = (i == b) ? it[k] : a[k] + it[k];
       }
-    double V[3][3];
-    eig3(C, m.evals, V);
-    for (int k = 0; k < 3; ++k)
-      for (int i = 0; i < 3; ++i) m.evec[k][i] = (float)V[i][k];
-    // reorient the normal (smallest) and the direction (largest) against the mean
-    if (dotf(m.meanf, m.evec[0]) > 0)
-      for (int i = 0; i < 3; ++i) m.evec[0][i] = -m.evec[0][i];
-    if (dotf(m.meanf, m.evec[2]) > 0)
-      for (int i = 0; i < 3; ++i) m.evec[2][i] = -m.evec[2][i];
-    m.is_bad = false;
+    }
+    kdh::tg::reduce_chunks<5>(v5, (long)m);
+    for (int k = 0; k < 5; ++k) s1[k] = v5[0][k];
+    const double inv = 1.0 / (double)n;
+    const double mu[3] = {s1[0] * inv, s1[1] * inv, s1[2] * inv};
+    double(*v6)[6] = reinterpret_cast<double(*)[6]>(acc.data());
+    for (size_t c = 0; c < m; ++c) {
+      const size_t b = s + c * kChunk, t = std::min(b + kChunk, e);
+      double* a = v6[c];
+      for (size_t i = b; i < t; ++i) {
+        const double d0 = (double)pts[i].x - mu[0], d1 = (double)pts[i].y - mu[1], d2 = (double)pts[i].z - mu[2];
+        const double it[6] = {d0 * d0, d0 * d1, d0 * d2, d1 * d1, d1 * d2, d2 * d2};
+        for (int k = 0; k < 6; ++k) a[k] = (i == b) ? it[k] : a[k] + it[k];
+      }
+    }
+    kdh::tg::reduce_chunks<6>(v6, (long)m);
+    for (int k = 0; k < 6; ++k) c2[k] = v6[0][k];
   }
 
-  static void set_dir(kdh_node3f& node, const float d[3]) {
-    node.direction[0] = d[0];
-    node.direction[1] = d[1];
-    node.direction[2] = d[2];
-  }
-
-  // FlatLeafPolicy_::propagateNormal (flat_leaf_policy_.h:38-50)
-  bool propagate(kdh_node3f& node, Meta& m) const {
-    m.is_leaf = true;
-    if (m.has_propagated_normal) {
-      set_dir(node, m.propagated_normal);
-      if (dotf(node.mean, node.direction) > 0)
-        for (int i = 0; i < 3; ++i) node.direction[i] = -node.direction[i];
-      return true;
-    }
-    return false;
-  }
-
-  // CountLeafPolicy_ + FlatLeafPolicy_::fillMetadata
-  void fill(Meta& m, kdh_node3f& node, const Meta* parent) const {
-    fill_base(m, node, parent);
-    // CountLeafPolicy_ (count_leaf_policy_.h:27-43)
-    if (m.is_bad) {
-      m.is_leaf = true;
-    } else if ((double)m.n_points > (double)P.leaf_max_points) {
-      m.is_node = true;
-      set_dir(node, m.evec[2]);
-    } else {
-      m.is_leaf = true;
-      set_dir(node, m.evec[0]);
-    }
-    // FlatLeafPolicy_ (flat_leaf_policy_.h:52-108)
-    if (parent && P.propagate_normal) {
-      memcpy(m.propagated_normal, parent->propagated_normal, sizeof(m.propagated_normal));
-      m.has_propagated_normal = parent->has_propagated_normal;
-    } else {
-      m.propagated_normal[0] = m.propagated_normal[1] = m.propagated_normal[2] = 0.0f;
-      m.has_propagated_normal = false;
-    }
-    if (m.is_bad || (double)m.n_points < (double)P.leaf_min_points) {
-      m.is_bad = !propagate(node, m);
-      return;
-    }
-    const float lmax = (float)m.evals[2], lmin = (float)m.evals[0];
-    const float ratio = lmin / lmax;
-    m.is_flat = ratio < P.leaf_normal_eigenratio && fabsf(dotf(node.mean, m.evec[0])) > P.leaf_min_normal_incidence;
-    if (m.is_flat) {
-      m.has_propagated_normal = true;
-      memcpy(m.propagated_normal, m.evec[0], sizeof(m.propagated_normal));
-    }
-    if (m.is_node) return;
-    float volume = 1.0f;
-    for (int i = 1; i < 3; ++i) volume *= (float)m.evals[i];
-    const float density = (float)m.n_points / volume;
-    if (density < P.leaf_density) {
-      m.is_leaf = false;
-      m.is_node = true;
-      m.is_bad = false;
-      set_dir(node, m.evec[2]);
-      return;
-    }
-    if (m.is_leaf && m.is_flat) {
-      set_dir(node, m.evec[0]);
-      return;
-    }
-    m.is_bad = !propagate(node, m);
-  }
-
-  // splitRange (node_impl_.h:13-26): left = direction . (p - mean) > 0
+  // splitRange with the order of each side kept
   size_t split(const kdh_node3f& node) {
-    long s = (long)node.idx_start, e = (long)node.idx_end - 1;
-    while (s <= e) {
-      const Pt& p = pts[s];
-      float d[3] = {p.x - node.mean[0], p.y - node.mean[1], p.z - node.mean[2]};
-      bool left = dotf(node.direction, d) > 0;
-      if (left) {
-        ++s;
-      } else {
-        std::swap(pts[s], pts[e]);
-        --e;
-      }
+    const size_t s = node.idx_start, e = node.idx_end;
+    tmp.clear();
+    size_t w = s;
+    for (size_t i = s; i < e; ++i) {
+      if (kdh::tg::goes_left(node.direction, node.mean, pts[i].x, pts[i].y, pts[i].z))
+        pts[w++] = pts[i];
+      else
+        tmp.push_back(pts[i]);
     }
-    return (size_t)s;
+    std::copy(tmp.begin(), tmp.end(), pts + w);
+    return w;
   }
 
-  int build(size_t start, size_t end, const Meta* parent) {
+  int build(size_t start, size_t end, const Label* parent, int level) {
     const int me = (int)nodes.size();
-    nodes.push_back(Node{});
+    nodes.push_back(kdh_node3f{});
     kdh_node3f node;
     memset(&node, 0, sizeof(node));
     node.idx_start = start;
     node.idx_end = end;
     node.idx_left = -1;
     node.idx_right = -1;
-    Meta m;
-    fill(m, node, parent);
-    if (!m.is_node) {
-      nodes[me].n = node;
+    double s1[5], c2[6];
+    moments(start, end, s1, c2);
+    static const float zero3[3] = {0.0f, 0.0f, 0.0f};
+    Label L;
+    kdh::tg::label_node(P, (long)(end - start), s1, c2, parent == nullptr, parent ? parent->prop : zero3,
+                        parent ? parent->has_prop : 0, L);
+    node.dts_mean = L.dts_mean;
+    node.dts_covariance = L.dts_cov;
+    memcpy(node.mean, L.mean, sizeof(node.mean));
+    memcpy(node.direction, L.dir, sizeof(node.direction));
+    if (!L.is_node) {
+      nodes[me] = node;
       return me;
     }
     const size_t mid = split(node);
     if (mid == start || mid == end) {
-      // degenerate split: keep the node as a leaf with its normal
-      set_dir(node, m.evec[0]);
-      nodes[me].n = node;
+      // one-sided split: keep the node as a leaf with its normal
+      memcpy(node.direction, L.evec0, sizeof(node.direction));
+      nodes[me] = node;
       return me;
     }
-    nodes[me].n = node;
+    nodes[me] = node;
     int l, r;
-    if (P.parallel_level > 0 && m.level < P.parallel_level) {
-      Builder bl(P, pts), br(P, pts);
-      auto fl = std::async(std::launch::async, [&] { return bl.build(start, mid, &m); });
-      br.build(mid, end, &m);
+    if (P.parallel_level > 0 && level < P.parallel_level) {
+      Builder bl(P, pts, ts_start), br(P, pts, ts_start);
+      auto fl = std::async(std::launch::async, [&] { return bl.build(start, mid, &L, level + 1); });
+      br.build(mid, end, &L, level + 1);
       fl.get();
       l = append(bl);
       r = append(br);
     } else {
-      l = build(start, mid, &m);
-      r = build(mid, end, &m);
+      l = build(start, mid, &L, level + 1);
+      r = build(mid, end, &L, level + 1);
     }
-    nodes[me].n.idx_left = l;
-    nodes[me].n.idx_right = r;
+    nodes[me].idx_left = l;
+    nodes[me].idx_right = r;
     return me;
   }
 
   int append(const Builder& sub) {
     const int off = (int)nodes.size();
-    for (const Node& s : sub.nodes) {
-      Node n = s;
-      if (n.n.idx_left >= 0) n.n.idx_left += off;
-      if (n.n.idx_right >= 0) n.n.idx_right += off;
+    for (const kdh_node3f& s : sub.nodes) {
+      kdh_node3f n = s;
+      if (n.idx_left >= 0) n.idx_left += off;
+      if (n.idx_right >= 0) n.idx_right += off;
       nodes.push_back(n);
     }
     return off;
@@ -316,18 +156,11 @@ static bool is_leaf(const kdh_node3f& n) { return n.idx_left < 0 && n.idx_right 
 
 // IsGoodLeafPredicate_ (tree_predicates_.h:8-29) with its defaults
 // (min_normal_incidence 0.1; its min_points is unused by the reference).
-static bool good_leaf(const kdh_node3f& n) {
-  if (!is_leaf(n)) return false;
-  const float sq = (n.direction[0] * n.direction[0] + n.direction[1] * n.direction[1]) + n.direction[2] * n.direction[2];
-  if (!(sq > 0)) return false;
-  const float inc = dotf(n.mean, n.direction);
-  if (fabsf(inc) <= 0.1f) return false;
-  return true;
-}
+static bool good_leaf(const kdh_node3f& n) { return is_leaf(n) && kdh::tg::good_leaf(n.mean, n.direction); }
 
 // TreeCPU_::_prune (tree_cpu_.h:209-232)
-static int prune_rec(std::vector<Node>& nodes, int idx) {
-  kdh_node3f& n = nodes[idx].n;
+static int prune_rec(std::vector<kdh_node3f>& nodes, int idx) {
+  kdh_node3f& n = nodes[idx];
   if (is_leaf(n)) return good_leaf(n) ? idx : -1;
   if (n.idx_left >= 0) n.idx_left = prune_rec(nodes, n.idx_left);
   if (n.idx_right >= 0) n.idx_right = prune_rec(nodes, n.idx_right);
@@ -368,21 +201,22 @@ int kdh_host_tree_build(const float* xyz, const double* stamps, int64_t n_points
   if (!out || !params || n_points < 0 || (n_points > 0 && !xyz)) return KDH_ERR_INVALID;
   *out = nullptr;
   const kdh_tree_gen_params& P = *params;
-  if (n_points < (int64_t)P.leaf_min_points) return KDH_OK;  // makeTree returns nullptr
+  // makeTree returns nullptr below leaf_min_points; an empty cloud gives no tree
+  if (n_points < (int64_t)P.leaf_min_points || n_points == 0) return KDH_OK;
   std::vector<Pt> pts((size_t)n_points);
+  double ts_start = std::numeric_limits<double>::max();
   for (int64_t i = 0; i < n_points; ++i) {
     pts[i].x = xyz[3 * i];
     pts[i].y = xyz[3 * i + 1];
     pts[i].z = xyz[3 * i + 2];
     pts[i].stamp = stamps ? stamps[i] : 0.0;
+    ts_start = std::min(ts_start, pts[i].stamp);
   }
-  Builder b(P, pts.data());
+  Builder b(P, pts.data(), ts_start);
   b.nodes.reserve((size_t)n_points / 4 + 16);
-  b.build(0, (size_t)n_points, nullptr);
-  double ts_start = std::numeric_limits<double>::max();
-  for (const Pt& p : pts) ts_start = std::min(ts_start, p.stamp);
+  b.build(0, (size_t)n_points, nullptr, 0);
 
-  std::vector<Node>& nodes = b.nodes;
+  std::vector<kdh_node3f>& nodes = b.nodes;
   int root = prune_rec(nodes, 0);
   auto* t = new kdh_host_tree();
   t->ts_start = ts_start;
@@ -396,7 +230,7 @@ int kdh_host_tree_build(const float* xyz, const double* stamps, int64_t n_points
       int cur = frontier.front();
       frontier.pop_front();
       perm.push_back(cur);
-      const kdh_node3f& n = nodes[cur].n;
+      const kdh_node3f& n = nodes[cur];
       if (n.idx_left >= 0) frontier.push_back(n.idx_left);
       if (n.idx_right >= 0) frontier.push_back(n.idx_right);
     }
@@ -404,7 +238,7 @@ int kdh_host_tree_build(const float* xyz, const double* stamps, int64_t n_points
     for (size_t i = 0; i < perm.size(); ++i) inv[perm[i]] = (int)i;
     t->nodes.resize(perm.size());
     for (size_t i = 0; i < perm.size(); ++i) {
-      kdh_node3f n = nodes[perm[i]].n;
+      kdh_node3f n = nodes[perm[i]];
       if (n.idx_left >= 0) n.idx_left = inv[n.idx_left];
       if (n.idx_right >= 0) n.idx_right = inv[n.idx_right];
       t->nodes[i] = n;

@@ -31,7 +31,10 @@
 #include <string.h>
 
 #include <algorithm>
+#include <array>
+#include <deque>
 #include <future>
+#include <limits>
 #include <type_traits>
 #include <vector>
 
@@ -1421,6 +1424,355 @@ float orc_stable_sum_f(const float* x, int64_t n) {
 // Pose prior H/b (FP32) for unit checks against finite differences.
 void orc_prior_form(const float* ZXi34, const float* X34, const float* info, float* H6, float* b6) {
   prior_form(from34<float>(ZXi34), from34<float>(X34), info, H6, b6);
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Tree generation: TreeGenerator_::makeTree (tree/tree_generator_impl_.h:9-19)
+// restated -- the recursive Node_ constructor with pre-order indices
+// (tree/node_impl_.h:29-70), LabelPolicyBase_::fillMetadata
+// (tree/label_policy_base_impl_.h:13-97), CountLeafPolicy_
+// (tree/count_leaf_policy_.h:27-43), FlatLeafPolicy_
+// (tree/flat_leaf_policy_.h:38-108), TreeCPU_::prune / _prune,
+// breadthFirstPermutation, applyPermutation and recomputeLeaves
+// (tree/tree_cpu_.h:173-232, tree/tree_cpu_impl_.h:48-74, 137-158) with
+// IsGoodLeafPredicate_ (tree/tree_predicates_.h:8-29).
+//
+// What the reference leaves to Eigen or to scheduling is fixed as the product
+// builders fix it, so that their trees can be compared bit for bit:
+//   * node moments: FP64 sums in the canonical order -- chunks of 64 items
+//     left to right, then groups of 64 values combined by the pairwise tree
+//     T(i, 2w) = T(i, w) + T(i + w, w) (the second term only where it
+//     exists), repeated until one value is left -- and the covariance centred
+//     on the FP64 mean;
+//   * eigenproblem: cyclic Jacobi sweeps in FP64 (pivots (0,1), (0,2), (1,2);
+//     stop when the off-diagonal sum is below 1e-18 of the diagonal sum),
+//     eigenvalues sorted ascending with ties in diagonal order;
+//   * splitRange keeps the order of each side; a one-sided split makes the
+//     node a leaf with its normal.
+// ---------------------------------------------------------------------------
+namespace orc {
+
+struct TreeGenParams {  // kdh_tree_gen_params layout
+  int32_t propagate_normal, tree_min_good_leaves, leaf_min_points, leaf_max_points;
+  float leaf_density, leaf_min_normal_incidence, leaf_eigenvalue_threshold, leaf_normal_eigenratio;
+  int32_t parallel_level;
+};
+
+struct TPoint {
+  float c[3];
+  double t;
+};
+
+template <int K>
+using Acc = std::array<double, K>;
+
+// T(i, w) over v[lo, lo + m): the pairwise tree of one group
+template <int K>
+static Acc<K> pair_tree(const std::vector<Acc<K>>& v, size_t lo, size_t m, size_t i, size_t w) {
+  if (w == 1) return v[lo + i];
+  Acc<K> a = pair_tree<K>(v, lo, m, i, w / 2);
+  if (i + w / 2 < m) {
+    const Acc<K> b = pair_tree<K>(v, lo, m, i + w / 2, w / 2);
+    for (int k = 0; k < K; ++k) a[k] += b[k];
+  }
+  return a;
+}
+
+template <int K, typename F>
+static Acc<K> canonical_sum(size_t s, size_t e, F item) {
+  std::vector<Acc<K>> v;
+  for (size_t b = s; b < e; b += 64) {
+    Acc<K> a = item(b);
+    for (size_t i = b + 1; i < std::min(b + 64, e); ++i) {
+      const Acc<K> x = item(i);
+      for (int k = 0; k < K; ++k) a[k] += x[k];
+    }
+    v.push_back(a);
+  }
+  while (v.size() > 1) {
+    std::vector<Acc<K>> w;
+    for (size_t lo = 0; lo < v.size(); lo += 64) w.push_back(pair_tree<K>(v, lo, std::min<size_t>(64, v.size() - lo), 0, 64));
+    v.swap(w);
+  }
+  return v[0];
+}
+
+static void jacobi3(double A[3][3], double lam[3], double V[3][3]) {
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) V[i][j] = i == j;
+  static const int PQ[3][2] = {{0, 1}, {0, 2}, {1, 2}};
+  for (int sweep = 0; sweep < 50; ++sweep) {
+    const double off = (fabs(A[0][1]) + fabs(A[0][2])) + fabs(A[1][2]);
+    const double dia = (fabs(A[0][0]) + fabs(A[1][1])) + fabs(A[2][2]);
+    if (off <= 1e-300 || off <= 1e-18 * dia) break;
+    for (const auto& pq : PQ) {
+      const int p = pq[0], q = pq[1];
+      if (fabs(A[p][q]) <= 1e-300) continue;
+      const double th = (A[q][q] - A[p][p]) / (2.0 * A[p][q]);
+      const double t = (th >= 0 ? 1.0 : -1.0) / (fabs(th) + sqrt(th * th + 1.0));
+      const double c = 1.0 / sqrt(t * t + 1.0), s = t * c;
+      for (int k = 0; k < 3; ++k) {
+        const double x = A[k][p], y = A[k][q];
+        A[k][p] = c * x - s * y;
+        A[k][q] = s * x + c * y;
+      }
+      for (int k = 0; k < 3; ++k) {
+        const double x = A[p][k], y = A[q][k];
+        A[p][k] = c * x - s * y;
+        A[q][k] = s * x + c * y;
+      }
+      for (int k = 0; k < 3; ++k) {
+        const double x = V[k][p], y = V[k][q];
+        V[k][p] = c * x - s * y;
+        V[k][q] = s * x + c * y;
+      }
+    }
+  }
+  int order[3] = {0, 1, 2};
+  std::stable_sort(order, order + 3, [&](int a, int b) { return A[a][a] < A[b][b]; });
+  double W[3][3];
+  for (int k = 0; k < 3; ++k) {
+    lam[k] = A[order[k]][order[k]];
+    for (int i = 0; i < 3; ++i) W[i][k] = V[i][order[k]];
+  }
+  memcpy(V, W, sizeof(W));
+}
+
+// LabelPolicyMetadata of the three policies
+struct TMeta {
+  size_t n_points = 0;
+  int level = 0;
+  double ts_start = 0;
+  Vec3<float> mean{};
+  double lam[3] = {0, 0, 0};
+  Vec3<float> evec[3]{};  // column k: k-th eigenvector
+  bool is_bad = true, is_node = false, is_leaf = false, is_flat = false, has_propagated_normal = false;
+  Vec3<float> propagated_normal{};
+};
+
+struct TreeGen {
+  TreeGenParams P;
+  TPoint* pts;
+  std::vector<Node> nodes;
+  int idx = 0;
+
+  static Vec3<float> dir_of(const Node& n) { return mk(n.direction[0], n.direction[1], n.direction[2]); }
+  static void set_dir(Node& n, Vec3<float> d) {
+    for (int i = 0; i < 3; ++i) n.direction[i] = d[i];
+  }
+
+  void fill_base(TMeta& m, Node& node, const TMeta* parent) {
+    m.n_points = node.idx_end - node.idx_start;
+    if (!m.n_points) {
+      m.is_bad = true;
+      return;
+    }
+    if (!parent) {
+      double ts = std::numeric_limits<double>::max();
+      for (size_t i = node.idx_start; i < node.idx_end; ++i) ts = std::min(ts, pts[i].t);
+      m.ts_start = ts;
+    } else {
+      m.ts_start = parent->ts_start;
+    }
+    m.level = parent ? parent->level + 1 : 0;
+    const double isize = 1.0 / (double)m.n_points;
+    const Acc<5> s1 = canonical_sum<5>(node.idx_start, node.idx_end, [&](size_t i) {
+      const double d = pts[i].t - m.ts_start;
+      return Acc<5>{(double)pts[i].c[0], (double)pts[i].c[1], (double)pts[i].c[2], d, d * d};
+    });
+    const double ts1 = s1[3] * isize, ts2 = s1[4] * isize;
+    node.dts_mean = ts1;
+    node.dts_covariance = ts2 - ts1 * ts1;
+    const double mu[3] = {s1[0] * isize, s1[1] * isize, s1[2] * isize};
+    for (int k = 0; k < 3; ++k) node.mean[k] = m.mean[k] = (float)mu[k];
+    if (m.n_points < 3) {
+      set_dir(node, mk(0.0f, 0.0f, 0.0f));
+      m.is_bad = true;
+      return;
+    }
+    const Acc<6> c2 = canonical_sum<6>(node.idx_start, node.idx_end, [&](size_t i) {
+      const double a = (double)pts[i].c[0] - mu[0], b = (double)pts[i].c[1] - mu[1], c = (double)pts[i].c[2] - mu[2];
+      return Acc<6>{a * a, a * b, a * c, b * b, b * c, c * c};
+    });
+    double C[3][3] = {{c2[0] * isize, c2[1] * isize, c2[2] * isize},
+                      {c2[1] * isize, c2[3] * isize, c2[4] * isize},
+                      {c2[2] * isize, c2[4] * isize, c2[5] * isize}};
+    double V[3][3];
+    jacobi3(C, m.lam, V);
+    for (int k = 0; k < 3; ++k) m.evec[k] = mk((float)V[0][k], (float)V[1][k], (float)V[2][k]);
+    if (dot(m.mean, m.evec[0]) > 0) m.evec[0] = mk(-m.evec[0][0], -m.evec[0][1], -m.evec[0][2]);
+    if (dot(m.mean, m.evec[2]) > 0) m.evec[2] = mk(-m.evec[2][0], -m.evec[2][1], -m.evec[2][2]);
+    m.is_bad = false;
+  }
+
+  bool propagate_normal(Node& node, TMeta& m) {
+    m.is_leaf = true;
+    if (m.has_propagated_normal) {
+      set_dir(node, m.propagated_normal);
+      if (dot(m.mean, dir_of(node)) > 0) set_dir(node, mk(-node.direction[0], -node.direction[1], -node.direction[2]));
+      return true;
+    }
+    return false;
+  }
+
+  void fill(TMeta& m, Node& node, const TMeta* parent) {
+    fill_base(m, node, parent);
+    // CountLeafPolicy_
+    if (m.is_bad) {
+      m.is_leaf = true;
+    } else if ((double)m.n_points > (double)P.leaf_max_points) {
+      m.is_node = true;
+      set_dir(node, m.evec[2]);
+    } else {
+      m.is_leaf = true;
+      set_dir(node, m.evec[0]);
+    }
+    // FlatLeafPolicy_
+    if (parent && P.propagate_normal) {
+      m.propagated_normal = parent->propagated_normal;
+      m.has_propagated_normal = parent->has_propagated_normal;
+    } else {
+      m.propagated_normal = mk(0.0f, 0.0f, 0.0f);
+      m.has_propagated_normal = false;
+    }
+    if (m.is_bad || (double)m.n_points < (double)P.leaf_min_points) {
+      m.is_bad = !propagate_normal(node, m);
+      return;
+    }
+    const float lmax = (float)m.lam[2], lmin = (float)m.lam[0];
+    m.is_flat = lmin / lmax < P.leaf_normal_eigenratio && fabsf(dot(m.mean, m.evec[0])) > P.leaf_min_normal_incidence;
+    if (m.is_flat) {
+      m.has_propagated_normal = true;
+      m.propagated_normal = m.evec[0];
+    }
+    if (m.is_node) return;
+    float volume = 1.0f;
+    for (int i = 1; i < 3; ++i) volume *= (float)m.lam[i];
+    if ((float)m.n_points / volume < P.leaf_density) {
+      m.is_leaf = false;
+      m.is_node = true;
+      m.is_bad = false;
+      set_dir(node, m.evec[2]);
+      return;
+    }
+    if (m.is_leaf && m.is_flat) {
+      set_dir(node, m.evec[0]);
+      return;
+    }
+    m.is_bad = !propagate_normal(node, m);
+  }
+
+  // Node_ constructor: label, split, children at pre-order indices
+  void make(int me, size_t s, size_t e, const TMeta* parent) {
+    Node node;
+    memset(&node, 0, sizeof(node));
+    node.idx_start = s;
+    node.idx_end = e;
+    node.idx_left = node.idx_right = -1;
+    TMeta m;
+    fill(m, node, parent);
+    if (m.is_node) {
+      // splitRange, each side kept in order
+      std::vector<TPoint> left, right;
+      for (size_t i = s; i < e; ++i) {
+        const Vec3<float> d = mk(pts[i].c[0] - node.mean[0], pts[i].c[1] - node.mean[1], pts[i].c[2] - node.mean[2]);
+        (dot(dir_of(node), d) > 0 ? left : right).push_back(pts[i]);
+      }
+      const size_t mid = s + left.size();
+      std::copy(left.begin(), left.end(), pts + s);
+      std::copy(right.begin(), right.end(), pts + mid);
+      if (mid == s || mid == e) {
+        set_dir(node, m.evec[0]);
+      } else {
+        nodes[me] = node;
+        node.idx_left = ++idx;
+        make(node.idx_left, s, mid, &m);
+        node.idx_right = ++idx;
+        make(node.idx_right, mid, e, &m);
+      }
+    }
+    nodes[me] = node;
+  }
+
+  static bool is_leaf(const Node& n) { return n.idx_left < 0 && n.idx_right < 0; }
+
+  int prune(int i) {
+    Node& n = nodes[i];
+    if (is_leaf(n)) {
+      const Vec3<float> d = dir_of(n), mean = mk(n.mean[0], n.mean[1], n.mean[2]);
+      const bool good = dot(d, d) > 0 && fabsf(dot(mean, d)) > 0.1f;
+      return good ? i : -1;
+    }
+    if (n.idx_left >= 0) n.idx_left = prune(n.idx_left);
+    if (n.idx_right >= 0) n.idx_right = prune(n.idx_right);
+    if (is_leaf(n)) {
+      set_dir(n, mk(0.0f, 0.0f, 0.0f));
+      return -1;
+    }
+    return i;
+  }
+};
+
+}  // namespace orc
+
+extern "C" {
+
+// makeTree on n points (xyz 3 floats each, stamps or NULL).  Returns 1 and
+// fills the BFS-ordered nodes (capacity 2n), the leaf list (capacity n) and
+// the permuted points / stamps; returns 0 where makeTree returns nullptr
+// (fewer than leaf_min_points points, fewer than tree_min_good_leaves leaves,
+// or no points).
+int orc_tree_build(const float* xyz, const double* stamps, int64_t n, const void* params, void* nodes_out,
+                   int64_t* n_nodes, uint64_t* leaves_out, int64_t* n_leaves, float* xyz_out, double* stamps_out) {
+  using namespace orc;
+  *n_nodes = *n_leaves = 0;
+  TreeGen g;
+  memcpy(&g.P, params, sizeof(TreeGenParams));
+  if (n < (int64_t)g.P.leaf_min_points || n == 0) return 0;
+  std::vector<TPoint> pts((size_t)n);
+  for (int64_t i = 0; i < n; ++i) {
+    for (int k = 0; k < 3; ++k) pts[i].c[k] = xyz[3 * i + k];
+    pts[i].t = stamps ? stamps[i] : 0.0;
+  }
+  g.pts = pts.data();
+  g.nodes.assign((size_t)(2 * n), Node{});
+  g.make(0, 0, (size_t)n, nullptr);
+  g.nodes.resize((size_t)g.idx + 1);
+  std::vector<Node> out;
+  if (g.prune(0) >= 0) {
+    // breadthFirstPermutation + applyPermutation
+    std::vector<int> perm, inv(g.nodes.size(), -1);
+    std::deque<int> q{0};
+    while (!q.empty()) {
+      const int c = q.front();
+      q.pop_front();
+      perm.push_back(c);
+      if (g.nodes[c].idx_left >= 0) q.push_back(g.nodes[c].idx_left);
+      if (g.nodes[c].idx_right >= 0) q.push_back(g.nodes[c].idx_right);
+    }
+    for (size_t i = 0; i < perm.size(); ++i) inv[perm[i]] = (int)i;
+    for (int p : perm) {
+      Node d = g.nodes[p];
+      if (d.idx_left >= 0) d.idx_left = inv[d.idx_left];
+      if (d.idx_right >= 0) d.idx_right = inv[d.idx_right];
+      out.push_back(d);
+    }
+  }
+  std::vector<uint64_t> leaves;  // recomputeLeaves
+  for (size_t i = 0; i < out.size(); ++i)
+    if (TreeGen::is_leaf(out[i])) leaves.push_back(i);
+  if ((int64_t)leaves.size() < (int64_t)g.P.tree_min_good_leaves) return 0;
+  memcpy(nodes_out, out.data(), out.size() * sizeof(Node));
+  memcpy(leaves_out, leaves.data(), leaves.size() * sizeof(uint64_t));
+  *n_nodes = (int64_t)out.size();
+  *n_leaves = (int64_t)leaves.size();
+  for (int64_t i = 0; i < n; ++i) {
+    for (int k = 0; k < 3; ++k) xyz_out[3 * i + k] = pts[i].c[k];
+    if (stamps_out) stamps_out[i] = pts[i].t;
+  }
+  return 1;
 }
 
 }  // extern "C"

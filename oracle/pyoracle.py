@@ -46,6 +46,32 @@ class OrcStats(C.Structure):
         return {f: getattr(self, f) for f, _ in self._fields_ if f != "pad"}
 
 
+class OrcTreeGenParams(C.Structure):
+    """kdh_tree_gen_params / TreeGenerator_ parameters (tree_generator_.h:18-25)."""
+
+    _fields_ = [
+        ("propagate_normal", C.c_int32),
+        ("tree_min_good_leaves", C.c_int32),
+        ("leaf_min_points", C.c_int32),
+        ("leaf_max_points", C.c_int32),
+        ("leaf_density", C.c_float),
+        ("leaf_min_normal_incidence", C.c_float),
+        ("leaf_eigenvalue_threshold", C.c_float),
+        ("leaf_normal_eigenratio", C.c_float),
+        ("parallel_level", C.c_int32),
+    ]
+
+
+def tree_gen_params(**kw):
+    """The reference defaults (tree_generator_.h:18-25), overridden by kw."""
+    p = OrcTreeGenParams(propagate_normal=0, tree_min_good_leaves=10, leaf_min_points=5, leaf_max_points=30,
+                         leaf_density=100.0, leaf_min_normal_incidence=0.1, leaf_eigenvalue_threshold=1e-3,
+                         leaf_normal_eigenratio=0.1, parallel_level=0)
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
 def build():
     subprocess.run(["make", "-s", "-C", _HERE], check=True)
 
@@ -72,6 +98,9 @@ def lib(fast=False):
         L.orc_build_dual.restype = C.c_int
         L.orc_build_dual.argtypes = [vp, C.c_int64, vp, vp, C.c_int64, f64, f64, f64, f64, C.POINTER(OrcParams),
                                      C.c_int, f32, f64, C.POINTER(OrcStats), i32]
+        L.orc_tree_build.restype = C.c_int
+        L.orc_tree_build.argtypes = [f32, f64, C.c_int64, vp, vp, C.POINTER(C.c_int64), vp, C.POINTER(C.c_int64),
+                                     f32, f64]
         L.orc_jacobian_velocity_b.argtypes = [f32, f32, f32, f32, f32, f32, C.c_float, f32]
         L.orc_tree_apply_velocities.argtypes = [vp, C.c_int64, f32, C.c_int]
         L.orc_tree_apply_isometry.argtypes = [vp, C.c_int64, f32]
@@ -297,3 +326,26 @@ def update_cache(Xm34, Xf34):
     out = np.zeros(12, dtype=np.float32)
     lib().orc_update_cache(p32(Xm), p32(Xf), p32(out))
     return out
+
+
+def tree_build(points, stamps=None, **params):
+    """TreeGenerator_::makeTree: a dict with the BFS-ordered Node_ array
+    (NODE_DTYPE), the leaf list, the permuted points and stamps; None where
+    makeTree returns nullptr."""
+    from kd_slam2_amd._lib import NODE_DTYPE
+
+    pts = np.ascontiguousarray(points, dtype=np.float32).reshape(-1, 3)
+    n = len(pts)
+    st = None if stamps is None else np.ascontiguousarray(stamps, dtype=np.float64)
+    nodes = np.zeros(max(2 * n, 1), dtype=NODE_DTYPE)
+    leaves = np.zeros(max(n, 1), dtype=np.uint64)
+    xyz = np.zeros((max(n, 1), 3), dtype=np.float32)
+    sto = np.zeros(max(n, 1))
+    nn, nl = C.c_int64(), C.c_int64()
+    p = tree_gen_params(**params)
+    ok = lib().orc_tree_build(p32(pts), None if st is None else p64(st), n, C.byref(p), nodes.ctypes.data,
+                              C.byref(nn), leaves.ctypes.data, C.byref(nl), p32(xyz), p64(sto))
+    if not ok:
+        return None
+    return dict(nodes=nodes[:nn.value].copy(), leaves=leaves[:nl.value].copy(), points=xyz[:n].copy(),
+                stamps=sto[:n].copy())

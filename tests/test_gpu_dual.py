@@ -9,6 +9,7 @@ import numpy as np
 import pytest
 
 from kd_slam2_amd import CTICPHIP3D, DeviceTree, batch_build_dual
+from kd_slam2_amd.api import adapter_build_dual
 from oracle import pyoracle as O
 
 from .workloads import dual_factors, icp_params, rel_err
@@ -109,3 +110,17 @@ def test_dual_moving_side_is_the_ct_build(ctx):
     for c in COUNTS:
         assert al.stats[c] == st[c]
     assert al.H_dual_cross_vel_AB.shape == (6, 6) and al.b_dual_vel_B.shape == (6,)
+
+
+def test_plugin_build_dual_is_the_c_abi_build(ctx):
+    """CTICPHIP3D::buildQuadraticFormDual of the C++ plugin gives the
+    kdh_batch_build_dual blocks bit for bit."""
+    p = _params()
+    fac, _keep = _device_factors(ctx, p)
+    f = fac[1]
+    ref = batch_build_dual(ctx, [f])[0]
+    got = adapter_build_dual(ctx, f["fixed"], f["moving"], f["X_moving"], f["v_moving"], f["X_fixed"], f["v_fixed"], p)
+    for mine, _ in BLOCKS:
+        assert np.array_equal(got[mine].reshape(ref[mine].shape), ref[mine]), mine
+    for c in COUNTS:
+        assert got["stats"][c] == ref["stats"][c]

@@ -216,3 +216,34 @@ def test_extract_leaves_is_the_leaf_gather_with_optional_warp():
     warped = O.extract_leaves(t.nodes, t.leaves, v)
     moved = O.tree_apply_velocities(t.nodes, v)
     assert np.array_equal(warped[:, 0], moved["mean"][t.leaves])
+
+
+def _same_tree(a, b):
+    """Two trees equal bit for bit: every Node_ field, the leaf list, the
+    permuted points."""
+    assert (a is None) == (b is None)
+    if a is None:
+        return
+    na, nb = a["nodes"], b["nodes"]
+    assert len(na) == len(nb)
+    assert na.tobytes() == nb.tobytes()
+    assert np.array_equal(a["leaves"], b["leaves"])
+    assert np.array_equal(a["points"], b["points"])
+
+
+def test_oracle_tree_build_equals_host_builder():
+    """The oracle's restatement of makeTree and the product's host builder
+    (one recursive, one shared with the device build) give the same tree bit
+    for bit, threaded or not, on scans, maps and degenerate clouds."""
+    from .workloads import tree_clouds
+
+    for name, pts, st, kw in tree_clouds():
+        ref = O.tree_build(pts, st, **kw)
+        for pl in (0, 2):
+            t = TreeGenerator3D(parallel_level=pl, **kw).makeTree(pts, st)
+            got = None if t is None else dict(nodes=t.nodes, leaves=t.leaves, points=t.points)
+            _same_tree(ref, got)
+        if name == "too few points":
+            assert ref is None
+        else:
+            assert ref is not None, name

@@ -76,3 +76,26 @@ def dual_factors(n_scan=65_536, n_frames=4):
     fr = seq["frames"]
     return [dict(fixed=trees[k], moving=trees[k + 1], X_moving=fr[k + 1]["X0"], v_moving=fr[k + 1]["v0"],
                  X_fixed=fr[k]["X_gt"], v_fixed=fr[k]["v_gt"]) for k in range(n_frames - 1)]
+
+
+def tree_clouds():
+    """Point clouds for tree-generation parity: (name, points, stamps,
+    generator parameters).  Seeded; small enough for the oracle."""
+    rng = np.random.default_rng(np.random.PCG64(77))
+    w2 = synth.config2(n_map=60_000, n_scan=32_768)
+    w1 = synth.config1(n_map=50_000, n_scan=8_192)
+    plane = np.c_[rng.uniform(-5, 5, (3000, 2)), np.full(3000, 2.0)].astype(np.float32)
+    line = np.c_[np.linspace(-3, 3, 500), np.full(500, 1.0), np.full(500, 0.5)].astype(np.float32)
+    same = np.tile(np.array([[1.0, 2.0, 3.0]], dtype=np.float32), (300, 1))
+    blob = rng.normal(size=(5000, 3)).astype(np.float32) * np.float32(0.3) + np.float32(4.0)
+    return [
+        ("cfg2 scan, stamped", w2["scan"], w2["stamps"], dict(leaf_max_points=16)),
+        ("cfg2 map", w2["map"], None, dict(leaf_max_points=16)),
+        ("cfg1 map, propagate", w1["map"], None, dict(leaf_max_points=16, propagate_normal=1)),
+        ("cfg1 scan, defaults", w1["scan"], w1["stamps"], dict()),
+        ("plane", plane, None, dict(leaf_max_points=16)),
+        ("line", line, None, dict(leaf_max_points=16, tree_min_good_leaves=0)),
+        ("identical points", same, None, dict(tree_min_good_leaves=0)),
+        ("gaussian blob, propagate", blob, rng.uniform(0, 0.1, 5000), dict(propagate_normal=1, leaf_max_points=8)),
+        ("too few points", blob[:4], None, dict()),
+    ]
