@@ -77,8 +77,16 @@ def _gen_cfg2(_):
     g = TreeGenerator3D(parallel_level=2)
     w["fixed"] = g.makeTree(w["map"])
     w["moving"] = g.makeTree(w["scan"], w["stamps"])
-    del w["map"]
     return w
+
+
+def _gen_sweep5(_):
+    """One 64k-point odometry sweep of the serving mix (sequence 0's
+    generator; its map is not needed here)."""
+    from kd_slam2_amd import synth
+
+    f = synth.config5_sequence(0, n_map=1000, n_frames=1)["frames"][0]
+    return dict(scan=f["scan"], stamps=f["stamps"])
 
 
 def _gen_seq(s):
@@ -229,6 +237,7 @@ def main():
     ap.add_argument("--pairs3", type=int, default=256, help="loop-closure pairs of the batched config (0: skip)")
     ap.add_argument("--no-pmc", action="store_true")
     ap.add_argument("--no-precision", action="store_true", help="skip the 10 km precision-stress workload")
+    ap.add_argument("--no-tree-build", action="store_true", help="skip the tree-generation timings")
     ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.pmc_child:
@@ -243,6 +252,7 @@ def main():
     n_loop_pool = 16
     with mp.get_context("fork").Pool(min(16, os.cpu_count() or 4)) as pool:
         j_cfg2 = pool.apply_async(_gen_cfg2, (0,))
+        j_sw5 = None if args.no_tree_build else pool.apply_async(_gen_sweep5, (0,))
         j_seq = None if args.no_serve else pool.map_async(_gen_seq, range(4))
         j_loop = None if args.no_serve else pool.map_async(_gen_loop, range(n_loop_pool))
         j_p3 = None
@@ -254,6 +264,7 @@ def main():
         j_c4 = None if args.no_precision else pool.map_async(_gen_cfg4_block, [(k, b) for k in (0, 1)
                                                                                 for b in range(8)])
         w = j_cfg2.get()
+        sweep5 = j_sw5.get() if j_sw5 else None
         c4_blocks = j_c4.get() if j_c4 else None
         seqs = j_seq.get() if j_seq else []
         loops = j_loop.get() if j_loop else []
@@ -398,6 +409,12 @@ def main():
     if w4 is not None:
         precision = run_precision(ctx, w4, DeviceTree, default_params, synth)
 
+    # ---- tree generation on the device (makeTree + upload) ------------------
+    tree_build = None
+    if not args.no_tree_build:
+        tree_build = run_tree_build(ctx, w, sweep5, DeviceTree)
+    w.pop("map", None)
+
     # ---- CPU baseline -------------------------------------------------------
     cpu = None
     if not args.no_cpu:
@@ -468,10 +485,54 @@ def main():
         serving_mix=serving,
         loop_closure_batch=batch3,
         precision_stress=precision,
+        tree_build=tree_build,
         roofline=roofline,
         cpu_baseline=cpu,
     )
     print(json.dumps(line), flush=True)
+
+
+def run_tree_build(ctx, w, sweep5, DeviceTree):
+    """makeTree + upload on the device (kdh_tree_build, bit-identical to the
+    host builder) against the host builder single-threaded and with 16
+    threads plus kdh_tree_upload: wall time from host points to a resident
+    tree.  Generator parameters as the workloads (leaf_max_points 30)."""
+    from kd_slam2_amd import TreeGenerator3D
+
+    g = TreeGenerator3D()
+    g16 = TreeGenerator3D(parallel_level=4)
+    cases = []
+    if sweep5 is not None:
+        cases.append(("sweep_64k", sweep5["scan"], sweep5["stamps"], 30, 3))
+    cases.append(("sweep_131k", w["scan"], w["stamps"], 30, 3))
+    cases.append(("map_2M", w["map"], None, 10, 1))
+    out = {}
+    for name, pts, st, reps, host_reps in cases:
+        for _ in range(3):
+            g.makeTreeOnDevice(ctx, pts, st, download=False).close()
+        gc.disable()
+        dev = []
+        for _ in range(reps):
+            t = time.perf_counter()
+            tr = g.makeTreeOnDevice(ctx, pts, st, download=False)
+            dev.append(time.perf_counter() - t)
+            tr.close()
+        host1, host16 = [], []
+        for _ in range(host_reps):
+            t = time.perf_counter()
+            DeviceTree(ctx, g.makeTree(pts, st)).close()
+            host1.append(time.perf_counter() - t)
+            t = time.perf_counter()
+            DeviceTree(ctx, g16.makeTree(pts, st)).close()
+            host16.append(time.perf_counter() - t)
+        gc.enable()
+        out[name] = dict(points=len(pts), device_ms_p50=round(1e3 * float(np.median(dev)), 3),
+                         device_ms_p99=round(1e3 * float(np.percentile(dev, 99)), 3),
+                         host_1thread_ms=round(1e3 * float(np.median(host1)), 2),
+                         host_16thread_ms=round(1e3 * float(np.median(host16)), 2))
+        log(f"tree build {name}: device {out[name]['device_ms_p50']:.2f} ms, host 1 thread "
+            f"{out[name]['host_1thread_ms']:.1f} ms, 16 threads {out[name]['host_16thread_ms']:.1f} ms")
+    return out
 
 
 def run_precision(ctx, w4, DeviceTree, default_params, synth):

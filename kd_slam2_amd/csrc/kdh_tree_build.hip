@@ -8,14 +8,16 @@
 //
 // The build runs level by level.  Level L holds a contiguous range of nodes;
 // their points are cut into chunks of 64 (a chunk never straddles two
-// nodes), and every pass over the points is one lane per chunk:
-//   mom1    lane per chunk: sums of (x, y, z, dts, dts^2), left to right
+// nodes), and every pass over the points is one wave per chunk, one lane per
+// point, loads coalesced:
+//   mom1    wave per chunk: pairwise sums of (x, y, z, dts, dts^2)
 //   red1    wave per node:  the chunk sums in the canonical pairwise order
-//   mom2    lane per chunk: sums of the products centred on the FP64 mean
+//   mom2    wave per chunk: the products centred on the FP64 mean
 //   red2    wave per node:  their reduction, the eigenproblem, the labels
-//   count   lane per chunk: points going left
+//   count   wave per chunk: points going left (ballot)
 //   level   one workgroup:  prefix sums -> split sizes, children, next chunks
-//   scatter lane per chunk: stable partition into the other point buffer
+//   scatter wave per chunk: stable partition into the other point buffer
+//                           (ballot ranks)
 // The node and chunk counts stay on the device; the host queues levels
 // ahead and reads a status word every few levels.  Then one workgroup
 // prunes bottom-up (IsGoodLeafPredicate_) and numbers the kept nodes in
@@ -202,67 +204,71 @@ struct LevelArgs {
   const Status* st;
 };
 
+// One group of the canonical order in a wave: lane i holds value i (i < cnt);
+// lane 0 ends with the group sum.
+template <int K>
+__device__ __forceinline__ void wave_pairwise(double (&v)[K], int cnt) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int s = 1; s < tg::kGroup; s <<= 1) {
+    const bool take = ((lane & (2 * s - 1)) == 0) && (lane + s < cnt);
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const double o = __shfl_down(v[k], s, 64);
+      if (take) v[k] += o;
+    }
+  }
+}
+
+// The chunk of this wave: its node and point range.  Wave-uniform.
+struct ChunkRef {
+  int j;       // node within the level
+  int p0;      // first point
+  int cnt;     // points in the chunk (1..64)
+};
+
+__device__ __forceinline__ ChunkRef chunk_ref(const LevelArgs& a, const LevelCtl& L, int c) {
+  ChunkRef r;
+  r.j = find_node(a.cbeg, L.le - L.lb, c);
+  const BNode& nd = a.nodes[L.lb + r.j];
+  r.p0 = nd.start + (c - a.cbeg[r.j]) * kC;
+  r.cnt = min(kC, nd.end - r.p0);
+  return r;
+}
+
 __global__ void k_mom1(LevelArgs a) {
   const LevelCtl L = *a.ctl;
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  const int c = (blockIdx.x * blockDim.x + threadIdx.x) >> 6, lane = threadIdx.x & 63;
   if (!L.active || c >= L.n_chunks) return;
-  const int j = find_node(a.cbeg, L.le - L.lb, c);
-  const BNode& nd = a.nodes[L.lb + j];
-  const int p0 = nd.start + (c - a.cbeg[j]) * kC, p1 = min(p0 + kC, nd.end);
-  const double ts0 = a.st->ts_start;
-  double s[5];
-  {
-    const double d = a.pt[p0] - ts0;
-    s[0] = (double)a.px[p0];
-    s[1] = (double)a.py[p0];
-    s[2] = (double)a.pz[p0];
-    s[3] = d;
-    s[4] = d * d;
-  }
-  for (int i = p0 + 1; i < p1; ++i) {
-    const double d = a.pt[i] - ts0;
-    s[0] += (double)a.px[i];
-    s[1] += (double)a.py[i];
-    s[2] += (double)a.pz[i];
-    s[3] += d;
-    s[4] += d * d;
-  }
-  double* o = a.csum + (size_t)c * 6;
+  const ChunkRef r = chunk_ref(a, L, c);
+  const int i = r.p0 + min(lane, r.cnt - 1);
+  const double d = a.pt[i] - a.st->ts_start;
+  double v[5] = {(double)a.px[i], (double)a.py[i], (double)a.pz[i], d, d * d};
+  wave_pairwise<5>(v, r.cnt);
+  if (lane == 0) {
+    double* o = a.csum + (size_t)c * 6;
 #pragma unroll
-  for (int k = 0; k < 5; ++k) o[k] = s[k];
+    for (int k = 0; k < 5; ++k) o[k] = v[k];
+  }
 }
 
 __global__ void k_mom2(LevelArgs a) {
   const LevelCtl L = *a.ctl;
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  const int c = (blockIdx.x * blockDim.x + threadIdx.x) >> 6, lane = threadIdx.x & 63;
   if (!L.active || c >= L.n_chunks) return;
-  const int j = find_node(a.cbeg, L.le - L.lb, c);
-  const BNode& nd = a.nodes[L.lb + j];
-  const int p0 = nd.start + (c - a.cbeg[j]) * kC, p1 = min(p0 + kC, nd.end);
+  const ChunkRef r = chunk_ref(a, L, c);
+  const BNode& nd = a.nodes[L.lb + r.j];
   const double inv = 1.0 / (double)(nd.end - nd.start);
   const double mu0 = nd.s1[0] * inv, mu1 = nd.s1[1] * inv, mu2 = nd.s1[2] * inv;
-  double s[6];
-  {
-    const double d0 = (double)a.px[p0] - mu0, d1 = (double)a.py[p0] - mu1, d2 = (double)a.pz[p0] - mu2;
-    s[0] = d0 * d0;
-    s[1] = d0 * d1;
-    s[2] = d0 * d2;
-    s[3] = d1 * d1;
-    s[4] = d1 * d2;
-    s[5] = d2 * d2;
-  }
-  for (int i = p0 + 1; i < p1; ++i) {
-    const double d0 = (double)a.px[i] - mu0, d1 = (double)a.py[i] - mu1, d2 = (double)a.pz[i] - mu2;
-    s[0] += d0 * d0;
-    s[1] += d0 * d1;
-    s[2] += d0 * d2;
-    s[3] += d1 * d1;
-    s[4] += d1 * d2;
-    s[5] += d2 * d2;
-  }
-  double* o = a.csum + (size_t)c * 6;
+  const int i = r.p0 + min(lane, r.cnt - 1);
+  const double d0 = (double)a.px[i] - mu0, d1 = (double)a.py[i] - mu1, d2 = (double)a.pz[i] - mu2;
+  double v[6] = {d0 * d0, d0 * d1, d0 * d2, d1 * d1, d1 * d2, d2 * d2};
+  wave_pairwise<6>(v, r.cnt);
+  if (lane == 0) {
+    double* o = a.csum + (size_t)c * 6;
 #pragma unroll
-  for (int k = 0; k < 6; ++k) o[k] = s[k];
+    for (int k = 0; k < 6; ++k) o[k] = v[k];
+  }
 }
 
 // The canonical reduction of the m chunk sums at csum[base..] by one wave;
@@ -279,15 +285,7 @@ __device__ void wave_reduce_chunks(double* csum, int base, int m, double out[K])
       const double* src = csum + (size_t)(base + b + min(lane, cnt - 1)) * 6;
 #pragma unroll
       for (int k = 0; k < K; ++k) v[k] = src[k];
-#pragma unroll
-      for (int s = 1; s < tg::kGroup; s <<= 1) {
-        const bool take = ((lane & (2 * s - 1)) == 0) && (lane + s < cnt);
-#pragma unroll
-        for (int k = 0; k < K; ++k) {
-          const double o = __shfl_down(v[k], s, 64);
-          if (take) v[k] += o;
-        }
-      }
+      wave_pairwise<K>(v, cnt);
       // every lane has its value (the shuffles consumed them) before lane 0
       // overwrites slot g; the fences make the store visible to the next
       // round's loads by the other lanes
@@ -355,16 +353,17 @@ __global__ void k_red2(LevelArgs a, kdh_tree_gen_params P) {
 
 __global__ void k_count(LevelArgs a, int* cleft) {
   const LevelCtl L = *a.ctl;
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  const int c = (blockIdx.x * blockDim.x + threadIdx.x) >> 6, lane = threadIdx.x & 63;
   if (!L.active || c >= L.n_chunks) return;
-  const int j = find_node(a.cbeg, L.le - L.lb, c);
-  const BNode& nd = a.nodes[L.lb + j];
-  int cnt = 0;
-  if (nd.is_node) {
-    const int p0 = nd.start + (c - a.cbeg[j]) * kC, p1 = min(p0 + kC, nd.end);
-    for (int i = p0; i < p1; ++i) cnt += tg::goes_left(nd.dir, nd.mean, a.px[i], a.py[i], a.pz[i]) ? 1 : 0;
+  const ChunkRef r = chunk_ref(a, L, c);
+  const BNode& nd = a.nodes[L.lb + r.j];
+  bool left = false;
+  if (nd.is_node && lane < r.cnt) {
+    const int i = r.p0 + lane;
+    left = tg::goes_left(nd.dir, nd.mean, a.px[i], a.py[i], a.pz[i]);
   }
-  cleft[c] = cnt;
+  const unsigned long long m = __ballot(left);
+  if (lane == 0) cleft[c] = __popcll(m);
 }
 
 // One workgroup: split sizes from the chunk counts, the children of the
@@ -455,31 +454,32 @@ __global__ __launch_bounds__(kLevelThreads) void k_level(BNode* nodes, const Lev
 
 __global__ void k_scatter(LevelArgs a, const int* coff, float* qx, float* qy, float* qz, double* qt) {
   const LevelCtl L = *a.ctl;
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  const int c = (blockIdx.x * blockDim.x + threadIdx.x) >> 6, lane = threadIdx.x & 63;
   if (!L.active || c >= L.n_chunks) return;
-  const int j = find_node(a.cbeg, L.le - L.lb, c);
-  const BNode& nd = a.nodes[L.lb + j];
-  const int cb = a.cbeg[j];
-  const int p0 = nd.start + (c - cb) * kC, p1 = min(p0 + kC, nd.end);
-  if (!nd.split) {
-    for (int i = p0; i < p1; ++i) {
-      qx[i] = a.px[i];
-      qy[i] = a.py[i];
-      qz[i] = a.pz[i];
-      qt[i] = a.pt[i];
-    }
-    return;
+  const ChunkRef r = chunk_ref(a, L, c);
+  const BNode& nd = a.nodes[L.lb + r.j];
+  const bool valid = lane < r.cnt;
+  const int i = r.p0 + min(lane, r.cnt - 1);
+  const float x = a.px[i], y = a.py[i], z = a.pz[i];
+  const double t = a.pt[i];
+  int o = i;
+  if (nd.split) {
+    // stable partition: a left point's rank is the number of left points
+    // before it in the chunk, a right point's the number of right points
+    const bool left = valid && tg::goes_left(nd.dir, nd.mean, x, y, z);
+    const unsigned long long m = __ballot(left);
+    const unsigned long long below = lane ? (~0ull >> (64 - lane)) : 0ull;
+    const int rank_l = __popcll(m & below);
+    const int cb = a.cbeg[r.j];
+    const int before_left = coff[c] - coff[cb];
+    const int before_right = (c - cb) * kC - before_left;
+    o = left ? nd.start + before_left + rank_l : nd.start + nd.n_left + before_right + (lane - rank_l);
   }
-  const int before_left = coff[c] - coff[cb];
-  int wl = nd.start + before_left;
-  int wr = nd.start + nd.n_left + ((c - cb) * kC - before_left);
-  for (int i = p0; i < p1; ++i) {
-    const float x = a.px[i], y = a.py[i], z = a.pz[i];
-    const int o = tg::goes_left(nd.dir, nd.mean, x, y, z) ? wl++ : wr++;
+  if (valid) {
     qx[o] = x;
     qy[o] = y;
     qz[o] = z;
-    qt[o] = a.pt[i];
+    qt[o] = t;
   }
 }
 
@@ -700,7 +700,7 @@ int kdh_tree_build(kdh_ctx* ctx, const float* xyz, const double* stamps, int64_t
     const int cur = L & 1, nxt = cur ^ 1;
     const int64_t width_bound = std::min<int64_t>((int64_t)1 << std::min(L, 30), n);
     const int64_t chunk_bound = std::min<int64_t>(n, n_chunks_max + width_bound);
-    const int gc = (int)((chunk_bound + 255) / 256);
+    const int gc = (int)((chunk_bound + 3) / 4);  // one wave per chunk
     const int gw = (int)std::min<int64_t>((width_bound + 3) / 4, 4096);
     LevelArgs a;
     a.nodes = ws->nodes;

@@ -22,12 +22,12 @@
 //     normal (the reference would recurse on the same range without end).
 //
 // Canonical summation order of a node's n items (points in node order):
-//   chunk sums: items [64c, 64c + 64) summed left to right, starting from the
-//   first item; then, while more than one value is left, groups of 64
-//   consecutive values are combined by a stride-doubling pairwise tree
-//   (for s = 1, 2, .., 32: v[i] += v[i + s] for i = 0 mod 2s, i + s < group
-//   size) into one value per group.  A parallel machine evaluates exactly
-//   this: one lane per chunk, one wave per group.
+//   while more than one value is left, groups of 64 consecutive values are
+//   combined by a stride-doubling pairwise tree (for s = 1, 2, .., 32:
+//   v[i] += v[i + s] for i = 0 mod 2s, i + s < group size) into one value per
+//   group.  The first round turns the points of each 64-point chunk into one
+//   chunk sum.  A wave evaluates one group exactly: lane i holds value i and
+//   the tree is six rounds of shuffles.
 #pragma once
 
 #include "../../include/kdh_host.h"
@@ -49,10 +49,10 @@ KDH_HD void group_pairwise(double (*v)[K], int m) {
       for (int k = 0; k < K; ++k) v[i][k] += v[i + s][k];
 }
 
-// The canonical reduction of m chunk sums (host form); v is overwritten and
-// the result is left in v[0].
+// The canonical reduction of m values (host form); v is overwritten and the
+// result is left in v[0].
 template <int K>
-KDH_HD void reduce_chunks(double (*v)[K], long m) {
+KDH_HD void reduce_canonical(double (*v)[K], long m) {
   while (m > 1) {
     const long groups = (m + kGroup - 1) / kGroup;
     for (long g = 0; g < groups; ++g) {

@@ -27,7 +27,6 @@
 
 namespace {
 
-using kdh::tg::kChunk;
 using kdh::tg::Label;
 
 struct Pt {
@@ -47,33 +46,34 @@ struct Builder {
   // The node's moments in the canonical order: s1 = sums of (x, y, z, dts,
   // dts^2); c2 = sums of the centred products (xx, xy, xz, yy, yz, zz).
   void moments(size_t s, size_t e, double s1[5], double c2[6]) {
-    const size_t n = e - s, m = (n + kChunk - 1) / kChunk;
-    acc.assign(m * 6, 0.0);
+    const size_t n = e - s;
+    acc.resize(n * 6);
     double(*v5)[5] = reinterpret_cast<double(*)[5]>(acc.data());
-    for (size_t c = 0; c < m; ++c) {
-      const size_t b = s + c * kChunk, t = std::min(b + kChunk, e);
-      double* a = v5[c];
-      for (size_t i = b; i < t; ++i) {
-        const double dts = pts[i].stamp - ts_start;
-        const double it[5] = {(double)pts[i].x, (double)pts[i].y, (double)pts[i].z, dts, dts * dts};
-        for (int k = 0; k < 5; ++k) a[k] = (i == b) ? it[k] : a[k] + it[k];
-      }
+    for (size_t i = s; i < e; ++i) {
+      const double dts = pts[i].stamp - ts_start;
+      double* a = v5[i - s];
+      a[0] = (double)pts[i].x;
+      a[1] = (double)pts[i].y;
+      a[2] = (double)pts[i].z;
+      a[3] = dts;
+      a[4] = dts * dts;
     }
-    kdh::tg::reduce_chunks<5>(v5, (long)m);
+    kdh::tg::reduce_canonical<5>(v5, (long)n);
     for (int k = 0; k < 5; ++k) s1[k] = v5[0][k];
     const double inv = 1.0 / (double)n;
     const double mu[3] = {s1[0] * inv, s1[1] * inv, s1[2] * inv};
     double(*v6)[6] = reinterpret_cast<double(*)[6]>(acc.data());
-    for (size_t c = 0; c < m; ++c) {
-      const size_t b = s + c * kChunk, t = std::min(b + kChunk, e);
-      double* a = v6[c];
-      for (size_t i = b; i < t; ++i) {
-        const double d0 = (double)pts[i].x - mu[0], d1 = (double)pts[i].y - mu[1], d2 = (double)pts[i].z - mu[2];
-        const double it[6] = {d0 * d0, d0 * d1, d0 * d2, d1 * d1, d1 * d2, d2 * d2};
-        for (int k = 0; k < 6; ++k) a[k] = (i == b) ? it[k] : a[k] + it[k];
-      }
+    for (size_t i = s; i < e; ++i) {
+      const double d0 = (double)pts[i].x - mu[0], d1 = (double)pts[i].y - mu[1], d2 = (double)pts[i].z - mu[2];
+      double* a = v6[i - s];
+      a[0] = d0 * d0;
+      a[1] = d0 * d1;
+      a[2] = d0 * d2;
+      a[3] = d1 * d1;
+      a[4] = d1 * d2;
+      a[5] = d2 * d2;
     }
-    kdh::tg::reduce_chunks<6>(v6, (long)m);
+    kdh::tg::reduce_canonical<6>(v6, (long)n);
     for (int k = 0; k < 6; ++k) c2[k] = v6[0][k];
   }
 

@@ -1441,11 +1441,11 @@ void orc_prior_form(const float* ZXi34, const float* X34, const float* info, flo
 //
 // What the reference leaves to Eigen or to scheduling is fixed as the product
 // builders fix it, so that their trees can be compared bit for bit:
-//   * node moments: FP64 sums in the canonical order -- chunks of 64 items
-//     left to right, then groups of 64 values combined by the pairwise tree
+//   * node moments: FP64 sums in the canonical order -- groups of 64
+//     consecutive values combined by the pairwise tree
 //     T(i, 2w) = T(i, w) + T(i + w, w) (the second term only where it
-//     exists), repeated until one value is left -- and the covariance centred
-//     on the FP64 mean;
+//     exists), repeated on the group sums until one value is left -- and the
+//     covariance centred on the FP64 mean;
 //   * eigenproblem: cyclic Jacobi sweeps in FP64 (pivots (0,1), (0,2), (1,2);
 //     stop when the off-diagonal sum is below 1e-18 of the diagonal sum),
 //     eigenvalues sorted ascending with ties in diagonal order;
@@ -1483,14 +1483,7 @@ static Acc<K> pair_tree(const std::vector<Acc<K>>& v, size_t lo, size_t m, size_
 template <int K, typename F>
 static Acc<K> canonical_sum(size_t s, size_t e, F item) {
   std::vector<Acc<K>> v;
-  for (size_t b = s; b < e; b += 64) {
-    Acc<K> a = item(b);
-    for (size_t i = b + 1; i < std::min(b + 64, e); ++i) {
-      const Acc<K> x = item(i);
-      for (int k = 0; k < K; ++k) a[k] += x[k];
-    }
-    v.push_back(a);
-  }
+  for (size_t i = s; i < e; ++i) v.push_back(item(i));
   while (v.size() > 1) {
     std::vector<Acc<K>> w;
     for (size_t lo = 0; lo < v.size(); lo += 64) w.push_back(pair_tree<K>(v, lo, std::min<size_t>(64, v.size() - lo), 0, 64));
