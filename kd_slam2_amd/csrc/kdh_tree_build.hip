@@ -7,23 +7,31 @@
 // order of each side of a split.
 //
 // The build runs level by level.  Level L holds a contiguous range of nodes;
-// their points are cut into chunks of 64 (a chunk never straddles two
-// nodes), and every pass over the points is one wave per chunk, one lane per
-// point, loads coalesced:
-//   mom1    wave per chunk: pairwise sums of (x, y, z, dts, dts^2)
-//   red1    wave per node:  the chunk sums in the canonical pairwise order
-//   mom2    wave per chunk: the products centred on the FP64 mean
-//   red2    wave per node:  their reduction, the eigenproblem, the labels
-//   count   wave per chunk: points going left (ballot)
-//   level   one workgroup:  prefix sums -> split sizes, children, next chunks
-//   scatter wave per chunk: stable partition into the other point buffer
-//                           (ballot ranks)
-// The node and chunk counts stay on the device; the host queues levels
-// ahead and reads a status word every few levels.  Then one workgroup
-// prunes bottom-up (IsGoodLeafPredicate_) and numbers the kept nodes in
-// breadth-first order, and a last pass writes the device tree layout of
-// kdh_tree_upload plus the full Node_ records, the leaf list and the
-// permuted points.
+// their points are cut into chunks of 64 (a chunk never straddles two nodes;
+// an explicit chunk-to-node map replaces any search).  Per level:
+//   mom1, mom2  wave per chunk: pairwise sums of (x, y, z, dts, dts^2), then
+//               of the products centred on the FP64 mean
+//   red1, red2  wave per node: the chunk sums in the canonical pairwise order;
+//               red2 also solves the eigenproblem and labels the node
+//   count       wave per chunk: points going left (ballot); integer totals
+//               per node through 64-bit atomics whose return value tells
+//               the last chunk, which decides the split
+//   alloc       prefix sums over 1024-node blocks of the split flags and the
+//               children's chunks; alloc2 (one workgroup) the block offsets
+//   split       wave per chunk: stable partition into the other point buffer
+//               and the next level's chunk-to-node map; thread per node: the
+//               children
+// Floating-point partial sums cross workgroups only at kernel boundaries
+// (the XCDs' L2 caches are not coherent with each other; a fence that
+// publishes data across them writes the L2 back).  Integer totals and the
+// prune's keep flags travel in atomic return values, which need no fence.
+// The node and chunk counts stay on the device; the host queues levels ahead
+// and reads a status word every few levels.  The prune climbs from the leaves
+// (the second child to arrive at a parent decides it), the kept nodes are
+// numbered by prefix sums over the level order, which is the breadth-first
+// order of the pruned tree, and a last pass writes the device tree layout of
+// kdh_tree_upload plus the full Node_ records, the leaf list and the permuted
+// points.
 #include <hip/hip_runtime.h>
 
 #include <string.h>
@@ -50,22 +58,26 @@ using namespace kdh;
 namespace {
 
 constexpr int kC = tg::kChunk;
-constexpr int kLevelThreads = 1024;
+constexpr int kScanThreads = 1024;  // prefix-sum blocks
 constexpr int kMaxLevels = 4096;
-constexpr int kBatchLevels = 4;  // levels queued between status checks
+constexpr int kBatchLevels = 4;     // levels queued between status checks
 
 // One node of the level-ordered build (before pruning).
 struct BNode {
   int start, end;     // point range
   int left, right;    // children (-1)
+  int parent;         // -1 for the root
   int level;
   int is_node;        // the labelling asks for a split
   int split;          // split into two non-empty sides
   int has_prop;       // FlatLeafPolicy_ propagated normal (inherited, then own)
   int n_left;
+  int ex_split;       // within its prefix-sum block: splits before it
+  int ex_chunks;      //   children's chunks before it
   int keep;           // survives the prune
-  int new_index;      // breadth-first index after the prune
-  int leaf_index;     // position in the leaf list
+  int ex_keep;        // within its prefix-sum block: kept nodes before it
+  int ex_leaf;        //   kept leaves before it
+  int pad;
   float mean[3], dir[3], evec0[3], prop[3];
   double dts_mean, dts_cov;
   double s1[5];       // (x, y, z, dts, dts^2) sums
@@ -82,7 +94,7 @@ struct Status {
   int overflow;
   int n_out, n_leaves_out;
   int root_kept;
-  int n_built;        // nodes of the unpruned tree
+  int pad;
   double ts_start;
 };
 
@@ -92,37 +104,33 @@ struct Ws {
   float* py[2] = {nullptr, nullptr};
   float* pz[2] = {nullptr, nullptr};
   double* pt[2] = {nullptr, nullptr};
-  BNode* nodes = nullptr;   // 2 * cap
-  double* csum = nullptr;   // cap x 6
-  int* cleft = nullptr;     // cap
-  int* coff = nullptr;      // cap
+  BNode* nodes = nullptr;   // 2 cap
+  double* csum = nullptr;   // chunk sums, 6 per chunk (cap)
+  int* cleft = nullptr;     // left points per chunk (cap)
+  int* gleft = nullptr;     // left points per 64-chunk group slot (cap)
+  unsigned long long* cnt64 = nullptr;  // per node: arrivals << 32 | count (2 cap); zero between uses
+  int* keep_word = nullptr; // per node: 2 x arrivals + keep bits (2 cap); zero between uses
+  int* chunk_node[2] = {nullptr, nullptr};  // node of each chunk (cap each)
+  int* blk_a = nullptr;     // prefix-sum block totals, then offsets (n_blk)
+  int* blk_b = nullptr;
+  int n_blk = 0;
   int* cbeg[2] = {nullptr, nullptr};  // cap + 1 each
   int* level_begin = nullptr;         // kMaxLevels + 2
   LevelCtl* ctl = nullptr;            // [2]
   Status* status = nullptr;
-  float* stage_xyz = nullptr;         // 3 * cap (host inputs)
+  float* stage_xyz = nullptr;         // 3 cap (host inputs)
   double* stage_t = nullptr;          // cap
   Status* h_status = nullptr;         // pinned
+  int* h_count = nullptr;             // pinned
   void* mem = nullptr;
 };
 
-__device__ __forceinline__ int find_node(const int* cbeg, int w, int c) {
-  // last j in [0, w) with cbeg[j] <= c
-  int lo = 0, hi = w - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (cbeg[mid] <= c)
-      lo = mid;
-    else
-      hi = mid - 1;
-  }
-  return lo;
-}
+__device__ __forceinline__ int lane_id() { return threadIdx.x & 63; }
 
-// Exclusive prefix sum over the 1024 threads of the block; *total gets the
-// block sum.  s: 16 ints of LDS.
+// Exclusive prefix sum over the threads of the block (a multiple of 64, at
+// most 1024); *total gets the block sum.  s: 16 ints of LDS.
 __device__ int block_exscan(int v, int* s, int* total) {
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const int lane = lane_id(), wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
   int x = v;
 #pragma unroll
   for (int o = 1; o < 64; o <<= 1) {
@@ -132,19 +140,86 @@ __device__ int block_exscan(int v, int* s, int* total) {
   if (lane == 63) s[wid] = x;
   __syncthreads();
   if (wid == 0) {
-    int t = lane < 16 ? s[lane] : 0;
+    int t = lane < nw ? s[lane] : 0;
 #pragma unroll
     for (int o = 1; o < 16; o <<= 1) {
       const int y = __shfl_up(t, o, 64);
       if (lane >= o) t += y;
     }
-    if (lane < 16) s[lane] = t;
+    if (lane < nw) s[lane] = t;
   }
   __syncthreads();
   const int off = wid ? s[wid - 1] : 0;
-  *total = s[15];
+  *total = s[nw - 1];
   __syncthreads();
   return off + x - v;
+}
+
+__device__ __forceinline__ int wave_sum_int(int v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// One group of the canonical order in a wave: lane i holds value i (i < cnt);
+// lane 0 ends with the group sum.
+template <int K>
+__device__ __forceinline__ void wave_pairwise(double (&v)[K], int cnt) {
+  const int lane = lane_id();
+#pragma unroll
+  for (int s = 1; s < tg::kGroup; s <<= 1) {
+    const bool take = ((lane & (2 * s - 1)) == 0) && (lane + s < cnt);
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const double o = __shfl_down(v[k], s, 64);
+      if (take) v[k] += o;
+    }
+  }
+}
+
+// The canonical reduction of the m values at vals[base..] (6 doubles apart)
+// by one wave, in place; every lane ends with the result.  Four groups are
+// loaded per round trip; group g's sum goes to slot g, which lies in a group
+// already consumed.
+template <int K>
+__device__ void wave_reduce_values(double* vals, int base, int m, double (&out)[K]) {
+  const int lane = lane_id();
+  while (m > 1) {
+    const int groups = (m + tg::kGroup - 1) / tg::kGroup;
+    for (int g = 0; g < groups; g += 4) {
+      double v[4][K];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int b = (g + u) * tg::kGroup;
+        const int cnt = g + u < groups ? min(m - b, tg::kGroup) : 1;
+        const double* src = vals + (size_t)(base + (g + u < groups ? b + min(lane, cnt - 1) : 0)) * 6;
+#pragma unroll
+        for (int k = 0; k < K; ++k) v[u][k] = src[k];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        if (g + u >= groups) break;
+        const int b = (g + u) * tg::kGroup;
+        wave_pairwise<K>(v[u], min(m - b, tg::kGroup));
+      }
+      if (lane == 0) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          if (g + u >= groups) break;
+          double* dst = vals + (size_t)(base + g + u) * 6;
+#pragma unroll
+          for (int k = 0; k < K; ++k) dst[k] = v[u][k];
+        }
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    }
+    m = groups;
+  }
+  const double* r = vals + (size_t)base * 6;
+#pragma unroll
+  for (int k = 0; k < K; ++k) out[k] = r[k];
 }
 
 // ---- set-up ------------------------------------------------------------------
@@ -159,14 +234,17 @@ __global__ void k_stage(const float* __restrict__ xyz, const double* __restrict_
   pt[i] = stamps ? stamps[i] : 0.0;
 }
 
-__global__ __launch_bounds__(kLevelThreads) void k_init(const double* __restrict__ pt, int n, BNode* nodes,
-                                                        LevelCtl* ctl, int* cbeg0, int* level_begin, Status* st) {
-  __shared__ double s_min[kLevelThreads];
+__global__ __launch_bounds__(kScanThreads) void k_init(const double* __restrict__ pt, int n, BNode* nodes,
+                                                       LevelCtl* ctl, int* cbeg0, int* chunk_node0, int* level_begin,
+                                                       Status* st) {
+  const int nc0 = (n + kC - 1) / kC;
+  for (int c = threadIdx.x; c < nc0; c += kScanThreads) chunk_node0[c] = 0;
+  __shared__ double s_min[kScanThreads];
   double m = 1.7976931348623157e308;
-  for (int i = threadIdx.x; i < n; i += kLevelThreads) m = fmin(m, pt[i]);
+  for (int i = threadIdx.x; i < n; i += kScanThreads) m = fmin(m, pt[i]);
   s_min[threadIdx.x] = m;
   __syncthreads();
-  for (int s = kLevelThreads / 2; s > 0; s >>= 1) {
+  for (int s = kScanThreads / 2; s > 0; s >>= 1) {
     if (threadIdx.x < s) s_min[threadIdx.x] = fmin(s_min[threadIdx.x], s_min[threadIdx.x + s]);
     __syncthreads();
   }
@@ -175,7 +253,7 @@ __global__ __launch_bounds__(kLevelThreads) void k_init(const double* __restrict
     memset(&r, 0, sizeof(r));
     r.start = 0;
     r.end = n;
-    r.left = r.right = -1;
+    r.left = r.right = r.parent = -1;
     nodes[0] = r;
     const int nc = (n + kC - 1) / kC;
     ctl[0] = LevelCtl{0, 1, nc, 1};
@@ -197,64 +275,68 @@ __global__ __launch_bounds__(kLevelThreads) void k_init(const double* __restrict
 struct LevelArgs {
   BNode* nodes;
   const LevelCtl* ctl;  // this level's
+  LevelCtl* ctl_next;
   const int* cbeg;      // this level's chunk starts (w + 1)
+  int* cbeg_next;
   const float *px, *py, *pz;
   const double* pt;
+  float *qx, *qy, *qz;  // the other point buffer
+  double* qt;
+  const int* chunk_node; // this level's
+  int* chunk_node_next;
   double* csum;
-  const Status* st;
+  int* cleft;
+  int* gleft;
+  unsigned long long* cnt64;
+  int* blk_a;
+  int* blk_b;
+  int* level_begin;
+  Status* st;
+  int level;
+  int cap_nodes;
 };
 
-// One group of the canonical order in a wave: lane i holds value i (i < cnt);
-// lane 0 ends with the group sum.
-template <int K>
-__device__ __forceinline__ void wave_pairwise(double (&v)[K], int cnt) {
-  const int lane = threadIdx.x & 63;
-#pragma unroll
-  for (int s = 1; s < tg::kGroup; s <<= 1) {
-    const bool take = ((lane & (2 * s - 1)) == 0) && (lane + s < cnt);
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
-      const double o = __shfl_down(v[k], s, 64);
-      if (take) v[k] += o;
-    }
-  }
-}
-
-// The chunk of this wave: its node and point range.  Wave-uniform.
+// The chunk of this wave.  Wave-uniform.
 struct ChunkRef {
   int j;       // node within the level
+  int cb, m;   // the node's first chunk and chunk count
   int p0;      // first point
   int cnt;     // points in the chunk (1..64)
 };
 
 __device__ __forceinline__ ChunkRef chunk_ref(const LevelArgs& a, const LevelCtl& L, int c) {
   ChunkRef r;
-  r.j = find_node(a.cbeg, L.le - L.lb, c);
+  r.j = a.chunk_node[c] - L.lb;
+  r.cb = a.cbeg[r.j];
+  r.m = a.cbeg[r.j + 1] - r.cb;
   const BNode& nd = a.nodes[L.lb + r.j];
-  r.p0 = nd.start + (c - a.cbeg[r.j]) * kC;
+  r.p0 = nd.start + (c - r.cb) * kC;
   r.cnt = min(kC, nd.end - r.p0);
   return r;
 }
 
+__device__ __forceinline__ void store_chunk(double* csum, int c, const double* v, int K) {
+  if (lane_id() == 0)
+    for (int k = 0; k < K; ++k) csum[(size_t)c * 6 + k] = v[k];
+}
+
 __global__ void k_mom1(LevelArgs a) {
   const LevelCtl L = *a.ctl;
-  const int c = (blockIdx.x * blockDim.x + threadIdx.x) >> 6, lane = threadIdx.x & 63;
+  const int c = (blockIdx.x * blockDim.x + threadIdx.x) >> 6, lane = lane_id();
   if (!L.active || c >= L.n_chunks) return;
   const ChunkRef r = chunk_ref(a, L, c);
   const int i = r.p0 + min(lane, r.cnt - 1);
   const double d = a.pt[i] - a.st->ts_start;
   double v[5] = {(double)a.px[i], (double)a.py[i], (double)a.pz[i], d, d * d};
   wave_pairwise<5>(v, r.cnt);
-  if (lane == 0) {
-    double* o = a.csum + (size_t)c * 6;
-#pragma unroll
-    for (int k = 0; k < 5; ++k) o[k] = v[k];
-  }
+  store_chunk(a.csum, c, v, 5);
+  // the left-point totals of this level's 64-chunk groups start at zero
+  if (lane == 0 && ((c - r.cb) & (kC - 1)) == 0) a.gleft[r.cb + ((c - r.cb) >> 6)] = 0;
 }
 
 __global__ void k_mom2(LevelArgs a) {
   const LevelCtl L = *a.ctl;
-  const int c = (blockIdx.x * blockDim.x + threadIdx.x) >> 6, lane = threadIdx.x & 63;
+  const int c = (blockIdx.x * blockDim.x + threadIdx.x) >> 6, lane = lane_id();
   if (!L.active || c >= L.n_chunks) return;
   const ChunkRef r = chunk_ref(a, L, c);
   const BNode& nd = a.nodes[L.lb + r.j];
@@ -264,45 +346,7 @@ __global__ void k_mom2(LevelArgs a) {
   const double d0 = (double)a.px[i] - mu0, d1 = (double)a.py[i] - mu1, d2 = (double)a.pz[i] - mu2;
   double v[6] = {d0 * d0, d0 * d1, d0 * d2, d1 * d1, d1 * d2, d2 * d2};
   wave_pairwise<6>(v, r.cnt);
-  if (lane == 0) {
-    double* o = a.csum + (size_t)c * 6;
-#pragma unroll
-    for (int k = 0; k < 6; ++k) o[k] = v[k];
-  }
-}
-
-// The canonical reduction of the m chunk sums at csum[base..] by one wave;
-// the result is returned in every lane.
-template <int K>
-__device__ void wave_reduce_chunks(double* csum, int base, int m, double out[K]) {
-  const int lane = threadIdx.x & 63;
-  while (m > 1) {
-    const int groups = (m + tg::kGroup - 1) / tg::kGroup;
-    for (int g = 0; g < groups; ++g) {
-      const int b = g * tg::kGroup;
-      const int cnt = min(m - b, tg::kGroup);
-      double v[K];
-      const double* src = csum + (size_t)(base + b + min(lane, cnt - 1)) * 6;
-#pragma unroll
-      for (int k = 0; k < K; ++k) v[k] = src[k];
-      wave_pairwise<K>(v, cnt);
-      // every lane has its value (the shuffles consumed them) before lane 0
-      // overwrites slot g; the fences make the store visible to the next
-      // round's loads by the other lanes
-      if (lane == 0) {
-        double* dst = csum + (size_t)(base + g) * 6;
-#pragma unroll
-        for (int k = 0; k < K; ++k) dst[k] = v[k];
-      }
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-    }
-    m = groups;
-  }
-  const double* r = csum + (size_t)base * 6;
-#pragma unroll
-  for (int k = 0; k < K; ++k) out[k] = r[k];
+  store_chunk(a.csum, c, v, 6);
 }
 
 __global__ void k_red1(LevelArgs a) {
@@ -313,12 +357,12 @@ __global__ void k_red1(LevelArgs a) {
   const int n_waves = (gridDim.x * blockDim.x) >> 6;
   for (int j = wave; j < w; j += n_waves) {
     const int base = a.cbeg[j], m = a.cbeg[j + 1] - base;
-    double s[5];
-    wave_reduce_chunks<5>(a.csum, base, m, s);
-    if ((threadIdx.x & 63) == 0) {
+    double s1[5];
+    wave_reduce_values<5>(a.csum, base, m, s1);
+    if (lane_id() == 0) {
       BNode& nd = a.nodes[L.lb + j];
 #pragma unroll
-      for (int k = 0; k < 5; ++k) nd.s1[k] = s[k];
+      for (int k = 0; k < 5; ++k) nd.s1[k] = s1[k];
     }
   }
 }
@@ -332,8 +376,8 @@ __global__ void k_red2(LevelArgs a, kdh_tree_gen_params P) {
   for (int j = wave; j < w; j += n_waves) {
     const int base = a.cbeg[j], m = a.cbeg[j + 1] - base;
     double c2[6];
-    wave_reduce_chunks<6>(a.csum, base, m, c2);
-    if ((threadIdx.x & 63) == 0) {
+    wave_reduce_values<6>(a.csum, base, m, c2);
+    if (lane_id() == 0) {
       BNode& nd = a.nodes[L.lb + j];
       tg::Label lab;
       tg::label_node(P, (long)(nd.end - nd.start), nd.s1, c2, nd.level == 0, nd.prop, nd.has_prop, lab);
@@ -351,111 +395,150 @@ __global__ void k_red2(LevelArgs a, kdh_tree_gen_params P) {
   }
 }
 
-__global__ void k_count(LevelArgs a, int* cleft) {
+// Left points per chunk, per 64-chunk group and per node (integer sums: any
+// order).  The node total and the arrival count share one 64-bit atomic, so
+// the chunk that arrives last reads the total from its return value and
+// decides the split.  Every other chunk of the node has evaluated its points
+// before its atomic, so the one-sided case may then replace the direction.
+__global__ void k_count(LevelArgs a) {
   const LevelCtl L = *a.ctl;
-  const int c = (blockIdx.x * blockDim.x + threadIdx.x) >> 6, lane = threadIdx.x & 63;
+  const int c = (blockIdx.x * blockDim.x + threadIdx.x) >> 6, lane = lane_id();
   if (!L.active || c >= L.n_chunks) return;
   const ChunkRef r = chunk_ref(a, L, c);
-  const BNode& nd = a.nodes[L.lb + r.j];
+  BNode& nd = a.nodes[L.lb + r.j];
+  if (!nd.is_node) {
+    if (c == r.cb && lane == 0) {
+      nd.split = 0;
+      nd.n_left = 0;
+    }
+    return;
+  }
   bool left = false;
-  if (nd.is_node && lane < r.cnt) {
+  if (lane < r.cnt) {
     const int i = r.p0 + lane;
     left = tg::goes_left(nd.dir, nd.mean, a.px[i], a.py[i], a.pz[i]);
   }
-  const unsigned long long m = __ballot(left);
-  if (lane == 0) cleft[c] = __popcll(m);
+  const int cnt = __popcll(__ballot(left));
+  if (lane != 0) return;
+  int total = cnt;
+  if (r.m > 1) {
+    a.cleft[c] = cnt;
+    if (r.m > kC) atomicAdd(&a.gleft[r.cb + ((c - r.cb) >> 6)], cnt);
+    const unsigned long long old = atomicAdd(&a.cnt64[L.lb + r.j], (1ull << 32) | (unsigned long long)cnt);
+    if ((int)(old >> 32) != r.m - 1) return;
+    total = (int)(old & 0xffffffffull) + cnt;
+    a.cnt64[L.lb + r.j] = 0ull;
+  }
+  const int n = nd.end - nd.start;
+  const int split = (total > 0 && total < n) ? 1 : 0;
+  if (!split)  // one-sided split: a leaf with its normal
+    for (int k = 0; k < 3; ++k) nd.dir[k] = nd.evec0[k];
+  nd.split = split;
+  nd.n_left = total;
 }
 
-// One workgroup: split sizes from the chunk counts, the children of the
-// split nodes in breadth-first order, and the next level's chunk starts.
-__global__ __launch_bounds__(kLevelThreads) void k_level(BNode* nodes, const LevelCtl* ctl, LevelCtl* ctl_next,
-                                                         const int* cbeg, int* cbeg_next, const int* cleft,
-                                                         int* coff, int* level_begin, int level, int cap_nodes,
-                                                         Status* st) {
+// Children's slots in breadth-first order: prefix sums of the split flags
+// and of the children's chunks within each 1024-node block.
+__global__ __launch_bounds__(kScanThreads) void k_alloc(LevelArgs a) {
   __shared__ int s_scan[16];
-  const LevelCtl L = *ctl;
-  const int tid = threadIdx.x;
+  const LevelCtl L = *a.ctl;
+  if (!L.active) return;
+  const int w = L.le - L.lb;
+  const int j = blockIdx.x * kScanThreads + threadIdx.x;
+  int split = 0, chunks = 0;
+  if (j < w) {
+    const BNode& nd = a.nodes[L.lb + j];
+    if (nd.split) {
+      split = 1;
+      const int n = nd.end - nd.start;
+      chunks = (nd.n_left + kC - 1) / kC + (n - nd.n_left + kC - 1) / kC;
+    }
+  }
+  int tot_s, tot_c;
+  const int es = block_exscan(split, s_scan, &tot_s);
+  const int ec = block_exscan(chunks, s_scan, &tot_c);
+  if (j < w) {
+    BNode& nd = a.nodes[L.lb + j];
+    nd.ex_split = es;
+    nd.ex_chunks = ec;
+  }
+  if (threadIdx.x == 0) {
+    a.blk_a[blockIdx.x] = tot_s;
+    a.blk_b[blockIdx.x] = tot_c;
+  }
+}
+
+// One workgroup: block totals -> exclusive block offsets (in place), and the
+// next level's range and chunk count.
+__global__ __launch_bounds__(kScanThreads) void k_alloc2(LevelArgs a, int nb) {
+  __shared__ int s_scan[16];
+  const LevelCtl L = *a.ctl;
   if (!L.active) {
-    if (tid == 0) *ctl_next = LevelCtl{L.le, L.le, 0, 0};
+    if (threadIdx.x == 0) *a.ctl_next = LevelCtl{L.le, L.le, 0, 0};
     return;
   }
-  const int C = L.n_chunks, w = L.le - L.lb;
-  int carry = 0;
-  for (int b = 0; b < C; b += kLevelThreads) {
-    const int c = b + tid;
-    int tot;
-    const int ex = block_exscan(c < C ? cleft[c] : 0, s_scan, &tot);
-    if (c < C) coff[c] = carry + ex;
-    carry += tot;
-  }
-  const int all_left = carry;
-  __syncthreads();
-  int carry_split = 0, carry_chunks = 0;
-  for (int b = 0; b < w; b += kLevelThreads) {
-    const int j = b + tid;
-    int split = 0, nl = 0, cl = 0, cr = 0;
-    if (j < w) {
-      BNode& nd = nodes[L.lb + j];
-      const int cb = cbeg[j], ce = cbeg[j + 1];
-      nl = (ce < C ? coff[ce] : all_left) - coff[cb];
-      const int n = nd.end - nd.start;
-      split = (nd.is_node && nl > 0 && nl < n) ? 1 : 0;
-      if (nd.is_node && !split)  // one-sided split: a leaf with its normal
-        for (int k = 0; k < 3; ++k) nd.dir[k] = nd.evec0[k];
-      nd.split = split;
-      nd.n_left = nl;
-      if (split) {
-        cl = (nl + kC - 1) / kC;
-        cr = (n - nl + kC - 1) / kC;
-      }
+  int carry_s = 0, carry_c = 0;
+  for (int b = 0; b < nb; b += kScanThreads) {
+    const int i = b + threadIdx.x;
+    const int vs = i < nb ? a.blk_a[i] : 0, vc = i < nb ? a.blk_b[i] : 0;
+    int ts, tc;
+    const int xs = block_exscan(vs, s_scan, &ts);
+    const int xc = block_exscan(vc, s_scan, &tc);
+    if (i < nb) {
+      a.blk_a[i] = carry_s + xs;
+      a.blk_b[i] = carry_c + xc;
     }
-    int tot_split, tot_chunks;
-    const int ks = block_exscan(split, s_scan, &tot_split);
-    const int kc = block_exscan(cl + cr, s_scan, &tot_chunks);
-    if (split) {
-      BNode& nd = nodes[L.lb + j];
-      const int k = carry_split + ks;
-      const int li = L.le + 2 * k, ri = li + 1;
-      if (ri < cap_nodes) {
-        nd.left = li;
-        nd.right = ri;
-        BNode c;
-        memset(&c, 0, sizeof(c));
-        c.left = c.right = -1;
-        c.level = nd.level + 1;
-        c.has_prop = nd.has_prop;
-        for (int q = 0; q < 3; ++q) c.prop[q] = nd.prop[q];
-        c.start = nd.start;
-        c.end = nd.start + nl;
-        nodes[li] = c;
-        c.start = nd.start + nl;
-        c.end = nd.end;
-        nodes[ri] = c;
-        cbeg_next[2 * k] = carry_chunks + kc;
-        cbeg_next[2 * k + 1] = carry_chunks + kc + cl;
-      } else {  // cannot happen (a tree of n points has at most 2 n - 1 nodes)
-        nd.split = 0;
-        st->overflow = 1;
-      }
-    }
-    carry_split += tot_split;
-    carry_chunks += tot_chunks;
+    carry_s += ts;
+    carry_c += tc;
   }
-  __syncthreads();
-  if (tid == 0) {
-    const int S = carry_split;
-    const bool ok = L.le + 2 * S <= cap_nodes && !st->overflow;
-    cbeg_next[2 * S] = carry_chunks;
-    *ctl_next = LevelCtl{L.le, L.le + 2 * S, carry_chunks, (S > 0 && ok) ? 1 : 0};
-    if (level + 2 < kMaxLevels + 2) level_begin[level + 2] = L.le + 2 * S;
-    if (S == 0 || !ok) st->last_level = level + 1;
+  if (threadIdx.x == 0) {
+    const int S = carry_s;
+    const bool ok = L.le + 2 * S <= a.cap_nodes;
+    if (!ok) a.st->overflow = 1;
+    a.cbeg_next[2 * S] = carry_c;
+    *a.ctl_next = LevelCtl{L.le, L.le + 2 * S, carry_c, (S > 0 && ok) ? 1 : 0};
+    if (a.level + 2 < kMaxLevels + 2) a.level_begin[a.level + 2] = L.le + 2 * S;
+    if (S == 0 || !ok) a.st->last_level = a.level + 1;
   }
 }
 
-__global__ void k_scatter(LevelArgs a, const int* coff, float* qx, float* qy, float* qz, double* qt) {
+// Blocks [0, chunk_blocks): one wave per chunk, the stable partition into
+// the other point buffer.  The rest: one thread per node, the children.
+__global__ void k_split(LevelArgs a, int chunk_blocks) {
   const LevelCtl L = *a.ctl;
-  const int c = (blockIdx.x * blockDim.x + threadIdx.x) >> 6, lane = threadIdx.x & 63;
-  if (!L.active || c >= L.n_chunks) return;
+  if (!L.active) return;
+  if ((int)blockIdx.x >= chunk_blocks) {
+    const int j = ((int)blockIdx.x - chunk_blocks) * blockDim.x + threadIdx.x;
+    if (j >= L.le - L.lb) return;
+    BNode& nd = a.nodes[L.lb + j];
+    if (!nd.split) return;
+    const int blk = j / kScanThreads;
+    const int k = a.blk_a[blk] + nd.ex_split;
+    const int li = L.le + 2 * k, ri = li + 1;
+    if (ri >= a.cap_nodes) return;  // flagged by k_alloc
+    nd.left = li;
+    nd.right = ri;
+    const int nl = nd.n_left, n = nd.end - nd.start;
+    const int c0 = a.blk_b[blk] + nd.ex_chunks;
+    a.cbeg_next[2 * k] = c0;
+    a.cbeg_next[2 * k + 1] = c0 + (nl + kC - 1) / kC;
+    BNode ch;
+    memset(&ch, 0, sizeof(ch));
+    ch.left = ch.right = -1;
+    ch.parent = L.lb + j;
+    ch.level = nd.level + 1;
+    ch.has_prop = nd.has_prop;
+    for (int q = 0; q < 3; ++q) ch.prop[q] = nd.prop[q];
+    ch.start = nd.start;
+    ch.end = nd.start + nl;
+    a.nodes[li] = ch;
+    ch.start = nd.start + nl;
+    ch.end = nd.start + n;
+    a.nodes[ri] = ch;
+    return;
+  }
+  const int c = (blockIdx.x * blockDim.x + threadIdx.x) >> 6, lane = lane_id();
+  if (c >= L.n_chunks) return;
   const ChunkRef r = chunk_ref(a, L, c);
   const BNode& nd = a.nodes[L.lb + r.j];
   const bool valid = lane < r.cnt;
@@ -464,75 +547,123 @@ __global__ void k_scatter(LevelArgs a, const int* coff, float* qx, float* qy, fl
   const double t = a.pt[i];
   int o = i;
   if (nd.split) {
-    // stable partition: a left point's rank is the number of left points
-    // before it in the chunk, a right point's the number of right points
+    // left points of the node's earlier chunks: whole 64-chunk groups, then
+    // the earlier chunks of this group (integer sums)
+    int before_left = 0;
+    if (r.m > 1) {
+      const int g = (c - r.cb) >> 6;
+      for (int b = 0; b < g; b += 64) before_left += wave_sum_int(b + lane < g ? a.gleft[r.cb + b + lane] : 0);
+      const int in_group = c - r.cb - kC * g;
+      before_left += wave_sum_int(lane < in_group ? a.cleft[r.cb + kC * g + lane] : 0);
+    }
+    // the next level's chunk-to-node map: the children's cl + cr chunks
+    // (m or m + 1 of them) are written by the parent's chunks in order
+    if (lane == 0) {
+      const int blk = r.j / kScanThreads;
+      const int k = a.blk_a[blk] + nd.ex_split;
+      const int li = L.le + 2 * k;
+      const int c0 = a.blk_b[blk] + nd.ex_chunks;
+      const int cl = (nd.n_left + kC - 1) / kC;
+      const int cr = (nd.end - nd.start - nd.n_left + kC - 1) / kC;
+      const int t = c - r.cb;
+      if (t < cl + cr) a.chunk_node_next[c0 + t] = t < cl ? li : li + 1;
+      if (t == r.m - 1 && cl + cr == r.m + 1) a.chunk_node_next[c0 + r.m] = li + 1;
+    }
+    const int before_right = (c - r.cb) * kC - before_left;
+    // stable partition: a point's rank among the chunk's points of its side
     const bool left = valid && tg::goes_left(nd.dir, nd.mean, x, y, z);
-    const unsigned long long m = __ballot(left);
+    const unsigned long long mk = __ballot(left);
     const unsigned long long below = lane ? (~0ull >> (64 - lane)) : 0ull;
-    const int rank_l = __popcll(m & below);
-    const int cb = a.cbeg[r.j];
-    const int before_left = coff[c] - coff[cb];
-    const int before_right = (c - cb) * kC - before_left;
+    const int rank_l = __popcll(mk & below);
     o = left ? nd.start + before_left + rank_l : nd.start + nd.n_left + before_right + (lane - rank_l);
   }
   if (valid) {
-    qx[o] = x;
-    qy[o] = y;
-    qz[o] = z;
-    qt[o] = t;
+    a.qx[o] = x;
+    a.qy[o] = y;
+    a.qz[o] = z;
+    a.qt[o] = t;
   }
 }
 
 // ---- prune and breadth-first numbering -------------------------------------------
 
-__global__ __launch_bounds__(kLevelThreads) void k_prune(BNode* nodes, const int* level_begin, int n_levels,
-                                                         Status* st) {
+// TreeCPU_::_prune bottom-up: every leaf decides its keep flag and climbs.
+// A parent's word collects 2 + keep per child; the second child to arrive
+// reads its sibling's flag from the atomic's return value and decides the
+// parent.
+__global__ void k_keep(BNode* nodes, int* keep_word, int n_built, const Status* st) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (st->overflow || i >= n_built) return;
+  BNode* nd = &nodes[i];
+  if (nd->split) return;
+  int keep = tg::good_leaf(nd->mean, nd->dir) ? 1 : 0;
+  nd->keep = keep;
+  int p = nd->parent;
+  while (p >= 0) {
+    const int old = atomicAdd(&keep_word[p], 2 + keep);
+    if (old == 0) return;  // the sibling decides
+    keep_word[p] = 0;
+    keep = ((old - 2) | keep) & 1;
+    nodes[p].keep = keep;
+    p = nodes[p].parent;
+  }
+}
+
+// Prefix sums of the keep and kept-leaf flags over the level order within
+// each 1024-node block.
+__global__ __launch_bounds__(kScanThreads) void k_number(BNode* nodes, int* blk_a, int* blk_b, int n_built,
+                                                         const Status* st) {
   __shared__ int s_scan[16];
-  const int tid = threadIdx.x;
   if (st->overflow) return;
-  // keep flags, deepest level first (TreeCPU_::_prune)
-  for (int l = n_levels - 1; l >= 0; --l) {
-    const int b = level_begin[l], e = level_begin[l + 1];
-    for (int i = b + tid; i < e; i += kLevelThreads) {
-      BNode& nd = nodes[i];
-      if (!nd.split)
-        nd.keep = tg::good_leaf(nd.mean, nd.dir) ? 1 : 0;
-      else
-        nd.keep = (nodes[nd.left].keep | nodes[nd.right].keep) ? 1 : 0;
-    }
-    __syncthreads();
+  const int i = blockIdx.x * kScanThreads + threadIdx.x;
+  int keep = 0, leaf = 0;
+  if (i < n_built) {
+    keep = nodes[i].keep;
+    leaf = keep && !nodes[i].split;
   }
-  // the level order of the unpruned tree, restricted to the kept nodes, is
-  // the breadth-first order of the pruned tree
-  const int n = level_begin[n_levels];
-  int carry = 0, carry_leaf = 0;
-  for (int b = 0; b < n; b += kLevelThreads) {
-    const int i = b + tid;
-    int keep = 0, leaf = 0;
-    if (i < n) {
-      keep = nodes[i].keep;
-      leaf = keep && !nodes[i].split;
-    }
-    int tot, tot_leaf;
-    const int ex = block_exscan(keep, s_scan, &tot);
-    const int exl = block_exscan(leaf, s_scan, &tot_leaf);
-    if (i < n) {
-      nodes[i].new_index = keep ? carry + ex : -1;
-      nodes[i].leaf_index = leaf ? carry_leaf + exl : -1;
-    }
-    carry += tot;
-    carry_leaf += tot_leaf;
+  int tk, tl;
+  const int ek = block_exscan(keep, s_scan, &tk);
+  const int el = block_exscan(leaf, s_scan, &tl);
+  if (i < n_built) {
+    nodes[i].ex_keep = ek;
+    nodes[i].ex_leaf = el;
   }
-  if (tid == 0) {
-    st->n_built = n;
-    st->n_out = carry;
-    st->n_leaves_out = carry_leaf;
-    st->root_kept = n > 0 ? nodes[0].keep : 0;
+  if (threadIdx.x == 0) {
+    blk_a[blockIdx.x] = tk;
+    blk_b[blockIdx.x] = tl;
+  }
+}
+
+// One workgroup: the block offsets and the totals.
+__global__ __launch_bounds__(kScanThreads) void k_number2(const BNode* nodes, int* blk_a, int* blk_b, int nb,
+                                                          Status* st) {
+  __shared__ int s_scan[16];
+  if (st->overflow) return;
+  int ck = 0, cl = 0;
+  for (int b = 0; b < nb; b += kScanThreads) {
+    const int j = b + threadIdx.x;
+    const int vk = j < nb ? blk_a[j] : 0, vl = j < nb ? blk_b[j] : 0;
+    int sk, sl;
+    const int xk = block_exscan(vk, s_scan, &sk);
+    const int xl = block_exscan(vl, s_scan, &sl);
+    if (j < nb) {
+      blk_a[j] = ck + xk;
+      blk_b[j] = cl + xl;
+    }
+    ck += sk;
+    cl += sl;
+  }
+  if (threadIdx.x == 0) {
+    st->n_out = ck;
+    st->n_leaves_out = cl;
+    st->root_kept = nodes[0].keep;
   }
 }
 
 struct EmitArgs {
   const BNode* nodes;
+  const int* blk_a;    // kept-node offsets per prefix-sum block
+  const int* blk_b;    // kept-leaf offsets per prefix-sum block
   int n_nodes;
   float* dnodes;       // 8 floats per node
   float* dleaves;      // 8 floats per leaf
@@ -541,16 +672,20 @@ struct EmitArgs {
   uint64_t* leaf_ids;  // leaf list
 };
 
+__device__ __forceinline__ int new_index(const EmitArgs& a, int i) {
+  return a.blk_a[i / kScanThreads] + a.nodes[i].ex_keep;
+}
+
 __global__ void k_emit(EmitArgs a) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= a.n_nodes) return;
   const BNode& nd = a.nodes[i];
   if (!nd.keep) return;
-  const int o = nd.new_index;
+  const int o = new_index(a, i);
   int l = -1, r = -1;
   if (nd.split) {
-    if (a.nodes[nd.left].keep) l = a.nodes[nd.left].new_index;
-    if (a.nodes[nd.right].keep) r = a.nodes[nd.right].new_index;
+    if (a.nodes[nd.left].keep) l = new_index(a, nd.left);
+    if (a.nodes[nd.right].keep) r = new_index(a, nd.right);
   }
   float* rec = a.dnodes + (size_t)o * 8;
   rec[0] = nd.mean[0];
@@ -574,8 +709,9 @@ __global__ void k_emit(EmitArgs a) {
   f.idx_left = l;
   f.idx_right = r;
   a.full[o] = f;
-  if (nd.leaf_index >= 0) {
-    float* lr = a.dleaves + (size_t)nd.leaf_index * 8;
+  if (!nd.split) {
+    const int li = a.blk_b[i / kScanThreads] + nd.ex_leaf;
+    float* lr = a.dleaves + (size_t)li * 8;
     lr[0] = nd.mean[0];
     lr[1] = nd.mean[1];
     lr[2] = nd.mean[2];
@@ -584,7 +720,7 @@ __global__ void k_emit(EmitArgs a) {
     lr[5] = nd.dir[2];
     lr[6] = (float)nd.dts_mean;
     lr[7] = __int_as_float(o);
-    a.leaf_ids[nd.leaf_index] = (uint64_t)o;
+    a.leaf_ids[li] = (uint64_t)o;
   }
 }
 
@@ -607,21 +743,34 @@ int ensure_ws(kdh_ctx* ctx, int64_t n) {
   if (ws) cap = std::max<int64_t>(cap, 2 * ws->cap);
   kdh_tree_build_ws_free(ctx);
   const size_t N = (size_t)cap;
-  const size_t bytes = 3 * 2 * a256(N * 4) + 2 * a256(N * 8) + a256(2 * N * sizeof(BNode)) + a256(N * 6 * 8) +
-                       2 * a256(N * 4) + 2 * a256((N + 1) * 4) + a256((kMaxLevels + 2) * 4) +
-                       a256(2 * sizeof(LevelCtl)) + a256(sizeof(Status)) + a256(3 * N * 4) + a256(N * 8);
+  const int n_blk = (int)((2 * N + kScanThreads - 1) / kScanThreads) + 1;
+  const size_t zero_bytes = a256(2 * N * 8) + a256(2 * N * 4);  // arrival words
+  const size_t bytes = zero_bytes + 3 * 2 * a256(N * 4) + 2 * a256(N * 8) + a256(2 * N * sizeof(BNode)) +
+                       a256(N * 6 * 8) + 2 * a256(N * 4) + 2 * a256(N * 4) + 2 * a256((size_t)n_blk * 4) +
+                       2 * a256((N + 1) * 4) + a256((kMaxLevels + 2) * 4) + a256(2 * sizeof(LevelCtl)) +
+                       a256(sizeof(Status)) + a256(3 * N * 4) + a256(N * 8);
   void* mem = nullptr;
   hipError_t e = hipMalloc(&mem, bytes);
-  if (e != hipSuccess) return kdh_fail(ctx, KDH_ERR_NOMEM, std::string("kdh_tree_build: workspace: ") + hipGetErrorString(e));
+  if (e != hipSuccess)
+    return kdh_fail(ctx, KDH_ERR_NOMEM, std::string("kdh_tree_build: workspace: ") + hipGetErrorString(e));
+  // arrival counters start at zero and every use leaves them at zero
+  e = hipMemset(mem, 0, zero_bytes);
+  if (e != hipSuccess) {
+    hipFree(mem);
+    return kdh_hip_fail(ctx, e, "kdh_tree_build: workspace clear");
+  }
   ws = new Ws();
   ws->cap = cap;
   ws->mem = mem;
+  ws->n_blk = n_blk;
   char* p = (char*)mem;
   auto take = [&](size_t b) {
     char* q = p;
     p += a256(b);
     return q;
   };
+  ws->cnt64 = (unsigned long long*)take(2 * N * 8);
+  ws->keep_word = (int*)take(2 * N * 4);
   for (int k = 0; k < 2; ++k) {
     ws->px[k] = (float*)take(N * 4);
     ws->py[k] = (float*)take(N * 4);
@@ -631,7 +780,11 @@ int ensure_ws(kdh_ctx* ctx, int64_t n) {
   ws->nodes = (BNode*)take(2 * N * sizeof(BNode));
   ws->csum = (double*)take(N * 6 * 8);
   ws->cleft = (int*)take(N * 4);
-  ws->coff = (int*)take(N * 4);
+  ws->gleft = (int*)take(N * 4);
+  ws->chunk_node[0] = (int*)take(N * 4);
+  ws->chunk_node[1] = (int*)take(N * 4);
+  ws->blk_a = (int*)take((size_t)n_blk * 4);
+  ws->blk_b = (int*)take((size_t)n_blk * 4);
   ws->cbeg[0] = (int*)take((N + 1) * 4);
   ws->cbeg[1] = (int*)take((N + 1) * 4);
   ws->level_begin = (int*)take((kMaxLevels + 2) * 4);
@@ -639,11 +792,13 @@ int ensure_ws(kdh_ctx* ctx, int64_t n) {
   ws->status = (Status*)take(sizeof(Status));
   ws->stage_xyz = (float*)take(3 * N * 4);
   ws->stage_t = (double*)take(N * 8);
-  e = hipHostMalloc((void**)&ws->h_status, sizeof(Status) * (kMaxLevels / kBatchLevels + 2), hipHostMallocDefault);
+  e = hipHostMalloc((void**)&ws->h_status, sizeof(Status) * (kMaxLevels / kBatchLevels + 2) + 64,
+                    hipHostMallocDefault);
   if (e != hipSuccess) {
     kdh_tree_build_ws_free(ctx);
     return kdh_fail(ctx, KDH_ERR_NOMEM, "kdh_tree_build: pinned status");
   }
+  ws->h_count = reinterpret_cast<int*>(ws->h_status + (kMaxLevels / kBatchLevels + 2));
   return KDH_OK;
 }
 
@@ -691,8 +846,8 @@ int kdh_tree_build(kdh_ctx* ctx, const float* xyz, const double* stamps, int64_t
   }
   hipLaunchKernelGGL(k_stage, dim3((n + 255) / 256), dim3(256), 0, s, d_xyz, d_t, n, ws->px[0], ws->py[0], ws->pz[0],
                      ws->pt[0]);
-  hipLaunchKernelGGL(k_init, dim3(1), dim3(kLevelThreads), 0, s, ws->pt[0], n, ws->nodes, ws->ctl, ws->cbeg[0],
-                     ws->level_begin, ws->status);
+  hipLaunchKernelGGL(k_init, dim3(1), dim3(kScanThreads), 0, s, ws->pt[0], n, ws->nodes, ws->ctl, ws->cbeg[0],
+                     ws->chunk_node[0], ws->level_begin, ws->status);
   KDH_HIP_TRY(ctx, hipGetLastError());
 
   const int n_chunks_max = (n + kC - 1) / kC;
@@ -700,27 +855,44 @@ int kdh_tree_build(kdh_ctx* ctx, const float* xyz, const double* stamps, int64_t
     const int cur = L & 1, nxt = cur ^ 1;
     const int64_t width_bound = std::min<int64_t>((int64_t)1 << std::min(L, 30), n);
     const int64_t chunk_bound = std::min<int64_t>(n, n_chunks_max + width_bound);
-    const int gc = (int)((chunk_bound + 3) / 4);  // one wave per chunk
-    const int gw = (int)std::min<int64_t>((width_bound + 3) / 4, 4096);
+    const int gc = (int)((chunk_bound + 3) / 4);  // one wave per chunk, four per block
+    const int gn = (int)((width_bound + 255) / 256);
+    const int ga = (int)((width_bound + kScanThreads - 1) / kScanThreads);
+    const int gw = (int)std::min<int64_t>((width_bound + 3) / 4, 4096);  // one wave per node
     LevelArgs a;
     a.nodes = ws->nodes;
     a.ctl = ws->ctl + cur;
+    a.ctl_next = ws->ctl + nxt;
     a.cbeg = ws->cbeg[cur];
+    a.cbeg_next = ws->cbeg[nxt];
     a.px = ws->px[cur];
     a.py = ws->py[cur];
     a.pz = ws->pz[cur];
     a.pt = ws->pt[cur];
+    a.qx = ws->px[nxt];
+    a.qy = ws->py[nxt];
+    a.qz = ws->pz[nxt];
+    a.qt = ws->pt[nxt];
+    a.chunk_node = ws->chunk_node[cur];
+    a.chunk_node_next = ws->chunk_node[nxt];
     a.csum = ws->csum;
+    a.cleft = ws->cleft;
+    a.gleft = ws->gleft;
+    a.cnt64 = ws->cnt64;
+    a.blk_a = ws->blk_a;
+    a.blk_b = ws->blk_b;
+    a.level_begin = ws->level_begin;
     a.st = ws->status;
+    a.level = L;
+    a.cap_nodes = cap_nodes;
     hipLaunchKernelGGL(k_mom1, dim3(gc), dim3(256), 0, s, a);
     hipLaunchKernelGGL(k_red1, dim3(gw), dim3(256), 0, s, a);
     hipLaunchKernelGGL(k_mom2, dim3(gc), dim3(256), 0, s, a);
     hipLaunchKernelGGL(k_red2, dim3(gw), dim3(256), 0, s, a, P);
-    hipLaunchKernelGGL(k_count, dim3(gc), dim3(256), 0, s, a, ws->cleft);
-    hipLaunchKernelGGL(k_level, dim3(1), dim3(kLevelThreads), 0, s, ws->nodes, ws->ctl + cur, ws->ctl + nxt,
-                       ws->cbeg[cur], ws->cbeg[nxt], ws->cleft, ws->coff, ws->level_begin, L, cap_nodes, ws->status);
-    hipLaunchKernelGGL(k_scatter, dim3(gc), dim3(256), 0, s, a, ws->coff, ws->px[nxt], ws->py[nxt], ws->pz[nxt],
-                       ws->pt[nxt]);
+    hipLaunchKernelGGL(k_count, dim3(gc), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(k_alloc, dim3(ga), dim3(kScanThreads), 0, s, a);
+    hipLaunchKernelGGL(k_alloc2, dim3(1), dim3(kScanThreads), 0, s, a, ga);
+    hipLaunchKernelGGL(k_split, dim3(gc + gn), dim3(256), 0, s, a, gc);
     return hipGetLastError();
   };
 
@@ -756,12 +928,22 @@ int kdh_tree_build(kdh_ctx* ctx, const float* xyz, const double* stamps, int64_t
   }
   release();
 
-  hipLaunchKernelGGL(k_prune, dim3(1), dim3(kLevelThreads), 0, s, ws->nodes, ws->level_begin, n_levels, ws->status);
+  // nodes of the unpruned tree: level_begin[n_levels]
+  KDH_HIP_TRY(ctx, hipMemcpyAsync(ws->h_count, ws->level_begin + n_levels, sizeof(int), hipMemcpyDeviceToHost, s));
+  KDH_HIP_TRY(ctx, hipStreamSynchronize(s));
+  const int n_built = *ws->h_count;
+  if (n_built < 1 || n_built > cap_nodes) return kdh_fail(ctx, KDH_ERR_INVALID, "kdh_tree_build: node capacity exceeded");
+  const int gp = (n_built + kScanThreads - 1) / kScanThreads;
+  if (gp > ws->n_blk) return kdh_fail(ctx, KDH_ERR_INVALID, "kdh_tree_build: prefix-sum blocks exceeded");
+  hipLaunchKernelGGL(k_keep, dim3((n_built + 255) / 256), dim3(256), 0, s, ws->nodes, ws->keep_word, n_built,
+                     ws->status);
+  hipLaunchKernelGGL(k_number, dim3(gp), dim3(kScanThreads), 0, s, ws->nodes, ws->blk_a, ws->blk_b, n_built,
+                     ws->status);
+  hipLaunchKernelGGL(k_number2, dim3(1), dim3(kScanThreads), 0, s, ws->nodes, ws->blk_a, ws->blk_b, gp, ws->status);
   KDH_HIP_TRY(ctx, hipGetLastError());
-  Status st;
   KDH_HIP_TRY(ctx, hipMemcpyAsync(ws->h_status, ws->status, sizeof(Status), hipMemcpyDeviceToHost, s));
   KDH_HIP_TRY(ctx, hipStreamSynchronize(s));
-  st = ws->h_status[0];
+  const Status st = ws->h_status[0];
   if (st.overflow) return kdh_fail(ctx, KDH_ERR_INVALID, "kdh_tree_build: node capacity exceeded");
   const int n_out = st.root_kept ? st.n_out : 0, n_leaves = st.root_kept ? st.n_leaves_out : 0;
   if (n_leaves < P.tree_min_good_leaves) return KDH_OK;  // makeTree returns nullptr
@@ -778,11 +960,14 @@ int kdh_tree_build(kdh_ctx* ctx, const float* xyz, const double* stamps, int64_t
   const size_t total = node_bytes + leaf_bytes + dts_bytes + full_bytes + ids_bytes + xyz_bytes + t_bytes;
   void* d = nullptr;
   hipError_t e = hipMalloc(&d, total);
-  if (e != hipSuccess) return kdh_fail(ctx, KDH_ERR_NOMEM, std::string("kdh_tree_build: hipMalloc: ") + hipGetErrorString(e));
+  if (e != hipSuccess)
+    return kdh_fail(ctx, KDH_ERR_NOMEM, std::string("kdh_tree_build: hipMalloc: ") + hipGetErrorString(e));
   char* p = (char*)d;
   EmitArgs ea;
   ea.nodes = ws->nodes;
-  ea.n_nodes = n_out > 0 ? st.n_built : 0;
+  ea.blk_a = ws->blk_a;
+  ea.blk_b = ws->blk_b;
+  ea.n_nodes = n_out > 0 ? n_built : 0;
   ea.dnodes = (float*)p;
   ea.dleaves = (float*)(p + node_bytes);
   ea.ddts = (float*)(p + node_bytes + leaf_bytes);
