@@ -12,6 +12,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libkdhip.so")
 
 KDH_OK = 0
+KDH_ERR_NOT_FOUND = -5
 ERRORS = {
     -1: "KDH_ERR_INVALID",
     -2: "KDH_ERR_HIP",
@@ -125,6 +126,33 @@ class DualForm(C.Structure):
         ("H_dual_cross_pose_vel_B", C.c_double * 36),
         ("H_dual_cross_vel_AB", C.c_double * 36),
         ("stats", Stats),
+    ]
+
+
+class KDTree3fInfo(C.Structure):
+    _fields_ = [
+        ("stamp_sec", C.c_int32),
+        ("stamp_nanosec", C.c_uint32),
+        ("frame_id", C.c_char_p),
+        ("src_topic", C.c_char_p),
+        ("ts_start", C.c_double),
+        ("root_mean", C.c_float * 3),
+        ("root_eigenvalues", C.c_float * 3),
+        ("root_eigenvectors", C.c_float * 9),
+    ]
+
+
+class McapMessage(C.Structure):
+    _fields_ = [
+        ("topic", C.c_char_p),
+        ("schema_name", C.c_char_p),
+        ("message_encoding", C.c_char_p),
+        ("log_time", C.c_uint64),
+        ("publish_time", C.c_uint64),
+        ("sequence", C.c_uint32),
+        ("channel_id", C.c_uint16),
+        ("data", C.c_void_p),
+        ("size", C.c_uint64),
     ]
 
 
@@ -247,6 +275,21 @@ _SIGS = {
     "kdh_host_tree_build": (C.c_int, [F32P, F64P, C.c_int64, C.POINTER(TreeGenParams), C.POINTER(P)]),
     "kdh_tree_build": (C.c_int, [P, F32P, F64P, C.c_int64, C.POINTER(TreeGenParams), C.c_int, C.POINTER(P)]),
     "kdh_tree_download": (C.c_int, [P, P, P, P, F32P, F64P]),
+    "kdh_kdtree3f_encoded_size": (C.c_int, [C.POINTER(KDTree3fInfo), C.c_int64, C.POINTER(C.c_size_t)]),
+    "kdh_kdtree3f_encode": (C.c_int, [C.POINTER(KDTree3fInfo), P, C.c_int64, P, C.c_size_t,
+                                      C.POINTER(C.c_size_t)]),
+    "kdh_kdtree3f_decode": (C.c_int, [P, C.c_size_t, C.POINTER(KDTree3fInfo), C.c_char_p, C.c_size_t, C.c_char_p,
+                                      C.c_size_t, P, C.c_int64, C.POINTER(C.c_int64)]),
+    "kdh_tree_leaves": (C.c_int, [P, C.c_int64, P, C.c_int64, C.POINTER(C.c_int64)]),
+    "kdh_kdtree3f_schema_text": (C.c_char_p, []),
+    "kdh_mcap_reader_open": (C.c_int, [C.c_char_p, C.POINTER(P)]),
+    "kdh_mcap_reader_next": (C.c_int, [P, C.POINTER(McapMessage)]),
+    "kdh_mcap_reader_error": (C.c_char_p, [P]),
+    "kdh_mcap_reader_close": (C.c_int, [P]),
+    "kdh_mcap_writer_open": (C.c_int, [C.c_char_p, C.POINTER(P)]),
+    "kdh_mcap_writer_add_channel": (C.c_int, [P, C.c_char_p, C.c_char_p, C.c_char_p, C.POINTER(C.c_uint16)]),
+    "kdh_mcap_writer_write": (C.c_int, [P, C.c_uint16, C.c_uint64, C.c_uint64, P, C.c_uint64]),
+    "kdh_mcap_writer_close": (C.c_int, [P]),
     "kdh_host_tree_info": (C.c_int, [P, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_double)]),
     "kdh_host_tree_nodes": (P, [P]),
     "kdh_host_tree_leaves": (P, [P]),

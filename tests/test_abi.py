@@ -18,7 +18,7 @@ def _declared(header):
 
 def test_every_declared_symbol_is_exported():
     L = _lib.lib()
-    names = _declared("kdh.h") + _declared("kdh_host.h")
+    names = _declared("kdh.h") + _declared("kdh_host.h") + _declared("kdh_io.h")
     assert len(names) > 30
     missing = [n for n in names if not hasattr(L, n)]
     assert not missing, missing

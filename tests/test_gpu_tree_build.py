@@ -121,3 +121,28 @@ def test_no_tree_cases(ctx):
     rng = np.random.default_rng(5)
     few = rng.normal(size=(40, 3)).astype(np.float32)
     assert (g.makeTreeOnDevice(ctx, few) is None) == (O.tree_build(few) is None)
+
+
+def test_tree_from_a_kdtree3f_message_aligns_identically(ctx):
+    """A device-built sweep tree read back, sent through the KDTree3f wire
+    format and uploaded again gives the same alignment bit for bit."""
+    from kd_slam2_amd import kdtree_msg as KM
+
+    w = synth.config2(n_map=200_000, n_scan=32_768)
+    g = TreeGenerator3D(leaf_max_points=16)
+    F = g.makeTreeOnDevice(ctx, w["map"])
+    M = g.makeTreeOnDevice(ctx, w["scan"], w["stamps"])
+    M2 = DeviceTree(ctx, KM.tree_from_message(KM.decode(KM.encode(KM.message_from_tree(M.host, frame_id="base")))))
+    p = icp_params(max_error=1.0, velocity_damping=10.0, max_iterations=6)
+    out = []
+    for moving in (M, M2):
+        al = CTICPHIP3D(ctx)
+        al.params = p
+        al.setFixed(F)
+        al.setMoving(moving)
+        al.setX(w["X0"])
+        al.setVelocities(w["v0"])
+        al.setFixedX(w["Xf"])
+        al.computeOnDevice()
+        out.append((al.getX(), al.velocities()))
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
