@@ -61,6 +61,8 @@ constexpr int kC = tg::kChunk;
 constexpr int kScanThreads = 1024;  // prefix-sum blocks
 constexpr int kMaxLevels = 4096;
 constexpr int kBatchLevels = 4;     // levels queued between status checks
+constexpr int kSubLimit = 2048;     // subtrees of at most this many points finish in one workgroup
+constexpr int kSubThreads = 256;
 
 // One node of the level-ordered build (before pruning).
 struct BNode {
@@ -77,10 +79,13 @@ struct BNode {
   int keep;           // survives the prune
   int ex_keep;        // within its prefix-sum block: kept nodes before it
   int ex_leaf;        //   kept leaves before it
-  int pad;
+  int gidx;           // subtree phase: the global index of a staged node
+  int n_depths;       // subtree phase, staged root: depths of its subtree
+  int n_staged;       //   and nodes
   float mean[3], dir[3], evec0[3], prop[3];
   double dts_mean, dts_cov;
   double s1[5];       // (x, y, z, dts, dts^2) sums
+  double c2[6];       // subtree phase: centred sums waiting for the labelling
 };
 
 struct LevelCtl {
@@ -94,6 +99,11 @@ struct Status {
   int overflow;
   int n_out, n_leaves_out;
   int root_kept;
+  int handoff;        // 1: the levels from handoff_level on are built by subtrees
+  int handoff_level;
+  int handoff_lb, handoff_n;  // the subtree roots: nodes [lb, lb + n)
+  int max_depth;      // deepest subtree (levels below its root, root included)
+  int n_built;
   int pad;
   double ts_start;
 };
@@ -113,7 +123,10 @@ struct Ws {
   int* chunk_node[2] = {nullptr, nullptr};  // node of each chunk (cap each)
   int* blk_a = nullptr;     // prefix-sum block totals, then offsets (n_blk)
   int* blk_b = nullptr;
+  int* blk_c = nullptr;     // largest child per block
   int n_blk = 0;
+  BNode* stage = nullptr;   // subtree phase: subtree s stages its nodes at [2 start_s, 2 end_s) (2 cap)
+  int* depth_cnt = nullptr; // subtree phase: subtree s's nodes per depth at [start_s + s, ...) (2 cap)
   int* cbeg[2] = {nullptr, nullptr};  // cap + 1 each
   int* level_begin = nullptr;         // kMaxLevels + 2
   LevelCtl* ctl = nullptr;            // [2]
@@ -290,6 +303,7 @@ struct LevelArgs {
   unsigned long long* cnt64;
   int* blk_a;
   int* blk_b;
+  int* blk_c;
   int* level_begin;
   Status* st;
   int level;
@@ -445,18 +459,23 @@ __global__ __launch_bounds__(kScanThreads) void k_alloc(LevelArgs a) {
   if (!L.active) return;
   const int w = L.le - L.lb;
   const int j = blockIdx.x * kScanThreads + threadIdx.x;
-  int split = 0, chunks = 0;
+  __shared__ int s_max;
+  if (threadIdx.x == 0) s_max = 0;
+  int split = 0, chunks = 0, largest = 0;
   if (j < w) {
     const BNode& nd = a.nodes[L.lb + j];
     if (nd.split) {
       split = 1;
       const int n = nd.end - nd.start;
       chunks = (nd.n_left + kC - 1) / kC + (n - nd.n_left + kC - 1) / kC;
+      largest = max(nd.n_left, n - nd.n_left);
     }
   }
   int tot_s, tot_c;
   const int es = block_exscan(split, s_scan, &tot_s);
   const int ec = block_exscan(chunks, s_scan, &tot_c);
+  if (largest) atomicMax(&s_max, largest);
+  __syncthreads();
   if (j < w) {
     BNode& nd = a.nodes[L.lb + j];
     nd.ex_split = es;
@@ -465,6 +484,7 @@ __global__ __launch_bounds__(kScanThreads) void k_alloc(LevelArgs a) {
   if (threadIdx.x == 0) {
     a.blk_a[blockIdx.x] = tot_s;
     a.blk_b[blockIdx.x] = tot_c;
+    a.blk_c[blockIdx.x] = s_max;
   }
 }
 
@@ -477,9 +497,13 @@ __global__ __launch_bounds__(kScanThreads) void k_alloc2(LevelArgs a, int nb) {
     if (threadIdx.x == 0) *a.ctl_next = LevelCtl{L.le, L.le, 0, 0};
     return;
   }
+  __shared__ int s_max;
+  if (threadIdx.x == 0) s_max = 0;
+  __syncthreads();
   int carry_s = 0, carry_c = 0;
   for (int b = 0; b < nb; b += kScanThreads) {
     const int i = b + threadIdx.x;
+    if (i < nb) atomicMax(&s_max, a.blk_c[i]);
     const int vs = i < nb ? a.blk_a[i] : 0, vc = i < nb ? a.blk_b[i] : 0;
     int ts, tc;
     const int xs = block_exscan(vs, s_scan, &ts);
@@ -495,10 +519,18 @@ __global__ __launch_bounds__(kScanThreads) void k_alloc2(LevelArgs a, int nb) {
     const int S = carry_s;
     const bool ok = L.le + 2 * S <= a.cap_nodes;
     if (!ok) a.st->overflow = 1;
+    // every child small enough: the subtree phase builds the rest
+    const bool handoff = S > 0 && ok && s_max <= kSubLimit;
     a.cbeg_next[2 * S] = carry_c;
-    *a.ctl_next = LevelCtl{L.le, L.le + 2 * S, carry_c, (S > 0 && ok) ? 1 : 0};
+    *a.ctl_next = LevelCtl{L.le, L.le + 2 * S, carry_c, (S > 0 && ok && !handoff) ? 1 : 0};
     if (a.level + 2 < kMaxLevels + 2) a.level_begin[a.level + 2] = L.le + 2 * S;
-    if (S == 0 || !ok) a.st->last_level = a.level + 1;
+    if (handoff) {
+      a.st->handoff = 1;
+      a.st->handoff_level = a.level + 1;
+      a.st->handoff_lb = L.le;
+      a.st->handoff_n = 2 * S;
+    }
+    if (S == 0 || !ok || handoff) a.st->last_level = a.level + 1;
   }
 }
 
@@ -582,6 +614,293 @@ __global__ void k_split(LevelArgs a, int chunk_blocks) {
     a.qy[o] = y;
     a.qz[o] = z;
     a.qt[o] = t;
+  }
+}
+
+// ---- subtree phase ---------------------------------------------------------------
+//
+// Once every node of a level holds at most kSubLimit points, one workgroup
+// per node finishes its subtree: the points live in LDS, and every depth runs
+// the level pipeline's steps with barriers instead of launches -- 64-point
+// chunk sums by a wave each (the same pairwise shuffles), the node sums in
+// the same pairwise order (at most 32 chunks: one group), the labelling, the
+// split counts, children in breadth-first order within the subtree, and the
+// stable partition by a prefix sum over the points.  The subtree's nodes are
+// staged with their depth and their index within the depth; the merge puts
+// depth d of subtree s after depth d of subtrees 0..s-1, which is the
+// breadth-first order of the whole tree.
+
+struct SubArgs {
+  BNode* nodes;          // roots at [lb, lb + n_roots)
+  int lb;
+  int level0;            // level of the roots
+  BNode* stage;
+  int* depth_cnt;
+  const float *px, *py, *pz;  // the roots' points
+  const double* pt;
+  float *ox, *oy, *oz;        // final point order
+  double* ot;
+  Status* st;
+};
+
+constexpr int kSubWaves = kSubThreads / 64;
+
+// One node's sums in the canonical order by one wave, from the points in LDS:
+// each 64-point chunk by the pairwise shuffles, the chunk sums (at most 32)
+// through the wave's LDS scratch and one more pairwise round.  item(i, v)
+// fills the K values of point i.  Every lane ends with the node sums.
+template <int K, typename Item>
+__device__ __forceinline__ void node_sums(int ns, int nn, double (*scratch)[6], const Item& item, double (&out)[K]) {
+  const int lane = lane_id();
+  const int m = (nn + kC - 1) / kC;
+  for (int t = 0; t < m; ++t) {
+    const int cnt = min(kC, nn - t * kC);
+    double v[K];
+    item(ns + t * kC + min(lane, cnt - 1), v);
+    wave_pairwise<K>(v, cnt);
+    if (lane == 0)
+      for (int k = 0; k < K; ++k) scratch[t][k] = v[k];
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+  double v[K];
+  for (int k = 0; k < K; ++k) v[k] = scratch[min(lane, m - 1)][k];
+  wave_pairwise<K>(v, m);
+  for (int k = 0; k < K; ++k) out[k] = __shfl(v[k], 0, 64);
+  __builtin_amdgcn_wave_barrier();  // the scratch is reused by the next node
+}
+
+__global__ __launch_bounds__(kSubThreads) void k_subtree(SubArgs a, kdh_tree_gen_params P) {
+  __shared__ float sx[2][kSubLimit], sy[2][kSubLimit], sz[2][kSubLimit];
+  __shared__ double stt[2][kSubLimit];
+  __shared__ double s_chunk[kSubWaves][kSubLimit / kC][6];
+  __shared__ int s_scan[16];
+  const int tid = threadIdx.x, lane = lane_id(), wave = tid >> 6;
+  const int sidx = blockIdx.x;
+  BNode* root = &a.nodes[a.lb + sidx];
+  const int p_begin = root->start, n = root->end - root->start;
+  BNode* S = a.stage + 2 * (size_t)p_begin;
+  int* DC = a.depth_cnt + p_begin + sidx;
+  const double ts0 = a.st->ts_start;
+  for (int i = tid; i < n; i += kSubThreads) {
+    sx[0][i] = a.px[p_begin + i];
+    sy[0][i] = a.py[p_begin + i];
+    sz[0][i] = a.pz[p_begin + i];
+    stt[0][i] = a.pt[p_begin + i];
+  }
+  if (tid == 0) {
+    BNode r = *root;
+    r.left = r.right = -1;
+    r.ex_split = 0;  // index within its depth
+    S[0] = r;
+  }
+  int q0 = 0, q1 = 1, d = 0;
+  __syncthreads();
+  for (;;) {
+    const int m_d = q1 - q0;
+    if (tid == 0) DC[d] = m_d;
+    // moments, a wave per node
+    for (int k = wave; k < m_d; k += kSubWaves) {
+      BNode& nd = S[q0 + k];
+      const int ns = nd.start - p_begin, nn = nd.end - nd.start;
+      double s1[5];
+      node_sums<5>(ns, nn, s_chunk[wave], [&](int i, double (&v)[5]) {
+        const double dd = stt[0][i] - ts0;
+        v[0] = (double)sx[0][i];
+        v[1] = (double)sy[0][i];
+        v[2] = (double)sz[0][i];
+        v[3] = dd;
+        v[4] = dd * dd;
+      }, s1);
+      const double inv = 1.0 / (double)nn;
+      const double mu0 = s1[0] * inv, mu1 = s1[1] * inv, mu2 = s1[2] * inv;
+      double c2[6];
+      node_sums<6>(ns, nn, s_chunk[wave], [&](int i, double (&v)[6]) {
+        const double d0 = (double)sx[0][i] - mu0, d1 = (double)sy[0][i] - mu1, d2 = (double)sz[0][i] - mu2;
+        v[0] = d0 * d0;
+        v[1] = d0 * d1;
+        v[2] = d0 * d2;
+        v[3] = d1 * d1;
+        v[4] = d1 * d2;
+        v[5] = d2 * d2;
+      }, c2);
+      if (lane == 0) {
+        for (int q = 0; q < 5; ++q) nd.s1[q] = s1[q];
+        for (int q = 0; q < 6; ++q) nd.c2[q] = c2[q];
+      }
+    }
+    __syncthreads();
+    // labels, a thread per node
+    for (int k = tid; k < m_d; k += kSubThreads) {
+      BNode& nd = S[q0 + k];
+      tg::Label lab;
+      tg::label_node(P, (long)(nd.end - nd.start), nd.s1, nd.c2, nd.level == 0, nd.prop, nd.has_prop, lab);
+      for (int q = 0; q < 3; ++q) {
+        nd.mean[q] = lab.mean[q];
+        nd.dir[q] = lab.dir[q];
+        nd.evec0[q] = lab.evec0[q];
+        nd.prop[q] = lab.prop[q];
+      }
+      nd.has_prop = lab.has_prop;
+      nd.is_node = lab.is_node;
+      nd.dts_mean = lab.dts_mean;
+      nd.dts_cov = lab.dts_cov;
+    }
+    __syncthreads();
+    // a wave per node: left count, split decision, stable partition into the
+    // second buffer and back
+    for (int k = wave; k < m_d; k += kSubWaves) {
+      BNode& nd = S[q0 + k];
+      const int ns = nd.start - p_begin, nn = nd.end - nd.start;
+      const int m = (nn + kC - 1) / kC;
+      int nl = 0;
+      if (nd.is_node) {
+        for (int t = 0; t < m; ++t) {
+          const int i = ns + t * kC + lane;
+          const bool left = (t * kC + lane < nn) && tg::goes_left(nd.dir, nd.mean, sx[0][i], sy[0][i], sz[0][i]);
+          nl += __popcll(__ballot(left));
+        }
+      }
+      const int split = (nd.is_node && nl > 0 && nl < nn) ? 1 : 0;
+      if (lane == 0) {
+        if (nd.is_node && !split)  // one-sided split: a leaf with its normal
+          for (int q = 0; q < 3; ++q) nd.dir[q] = nd.evec0[q];
+        nd.split = split;
+        nd.n_left = nd.is_node ? nl : 0;
+      }
+      if (!split) continue;
+      int lbefore = 0;
+      for (int t = 0; t < m; ++t) {
+        const int j = t * kC + lane, i = ns + j;
+        const bool valid = j < nn;
+        const float x = valid ? sx[0][i] : 0.0f, y = valid ? sy[0][i] : 0.0f, z = valid ? sz[0][i] : 0.0f;
+        const bool left = valid && tg::goes_left(nd.dir, nd.mean, x, y, z);
+        const unsigned long long mk = __ballot(left);
+        const unsigned long long below = lane ? (~0ull >> (64 - lane)) : 0ull;
+        const int rl = lbefore + __popcll(mk & below);
+        if (valid) {
+          const int o = ns + (left ? rl : nl + (j - rl));
+          sx[1][o] = x;
+          sy[1][o] = y;
+          sz[1][o] = z;
+          stt[1][o] = stt[0][i];
+        }
+        lbefore += __popcll(mk);
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+      for (int j = lane; j < nn; j += 64) {
+        sx[0][ns + j] = sx[1][ns + j];
+        sy[0][ns + j] = sy[1][ns + j];
+        sz[0][ns + j] = sz[1][ns + j];
+        stt[0][ns + j] = stt[1][ns + j];
+      }
+    }
+    __syncthreads();
+    // children in order, a thread per node
+    int carry = 0;
+    for (int b = 0; b < m_d; b += kSubThreads) {
+      const int k = b + tid;
+      const int split = k < m_d ? S[q0 + k].split : 0;
+      int tot;
+      const int ex = block_exscan(split, s_scan, &tot);
+      if (split) {
+        BNode& nd = S[q0 + k];
+        const int kk = carry + ex;
+        const int li = q1 + 2 * kk;
+        nd.left = li;
+        nd.right = li + 1;
+        for (int side = 0; side < 2; ++side) {
+          BNode& ch = S[li + side];
+          memset(&ch, 0, sizeof(BNode));
+          ch.left = ch.right = -1;
+          ch.parent = q0 + k;
+          ch.level = nd.level + 1;
+          ch.has_prop = nd.has_prop;
+          for (int q = 0; q < 3; ++q) ch.prop[q] = nd.prop[q];
+          ch.start = side ? nd.start + nd.n_left : nd.start;
+          ch.end = side ? nd.end : nd.start + nd.n_left;
+          ch.ex_split = 2 * kk + side;
+        }
+      }
+      carry += tot;
+    }
+    const int n_split = carry;
+    __syncthreads();
+    if (n_split == 0) break;
+    q0 = q1;
+    q1 += 2 * n_split;
+    ++d;
+  }
+  for (int i = tid; i < n; i += kSubThreads) {
+    a.ox[p_begin + i] = sx[0][i];
+    a.oy[p_begin + i] = sy[0][i];
+    a.oz[p_begin + i] = sz[0][i];
+    a.ot[p_begin + i] = stt[0][i];
+  }
+  if (tid == 0) {
+    S[0].n_depths = d + 1;
+    S[0].n_staged = q1;
+    atomicMax(&a.st->max_depth, d + 1);
+  }
+}
+
+// One workgroup: per depth, the prefix sum over the subtrees of their node
+// counts (in place: each count becomes the subtree's offset in the level),
+// and the level starts.
+__global__ __launch_bounds__(kScanThreads) void k_sub_levels(SubArgs a, int n_roots, int* level_begin) {
+  __shared__ int s_scan[16];
+  const int D = a.st->max_depth;
+  for (int d = 0; d < D; ++d) {
+    int carry = 0;
+    for (int b = 0; b < n_roots; b += kScanThreads) {
+      const int s = b + threadIdx.x;
+      int v = 0;
+      int* slot = nullptr;
+      if (s < n_roots) {
+        const int p_begin = a.nodes[a.lb + s].start;
+        const BNode& r = a.stage[2 * (size_t)p_begin];
+        if (d < r.n_depths) {
+          slot = a.depth_cnt + p_begin + s + d;
+          v = *slot;
+        }
+      }
+      int tot;
+      const int ex = block_exscan(v, s_scan, &tot);
+      if (slot) *slot = carry + ex;
+      carry += tot;
+    }
+    if (threadIdx.x == 0) level_begin[a.level0 + d + 1] = level_begin[a.level0 + d] + carry;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) a.st->n_built = level_begin[a.level0 + D];
+}
+
+// One workgroup per subtree: global indices of its staged nodes, then the
+// nodes with global links into the level-ordered array.
+__global__ void k_sub_place(SubArgs a, const int* level_begin) {
+  const int sidx = blockIdx.x;
+  const BNode* root = &a.nodes[a.lb + sidx];
+  const int p_begin = root->start;
+  const int parent_of_root = root->parent;
+  BNode* S = a.stage + 2 * (size_t)p_begin;
+  const int* DC = a.depth_cnt + p_begin + sidx;
+  const int n_staged = S[0].n_staged;
+  __syncthreads();
+  for (int i = threadIdx.x; i < n_staged; i += blockDim.x) {
+    const int dd = S[i].level - a.level0;
+    S[i].gidx = level_begin[a.level0 + dd] + DC[dd] + S[i].ex_split;
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < n_staged; i += blockDim.x) {
+    const BNode& r = S[i];
+    BNode& g = a.nodes[r.gidx];
+    g = r;
+    g.left = r.left >= 0 ? S[r.left].gidx : -1;
+    g.right = r.right >= 0 ? S[r.right].gidx : -1;
+    g.parent = i == 0 ? parent_of_root : S[r.parent].gidx;
   }
 }
 
@@ -746,7 +1065,8 @@ int ensure_ws(kdh_ctx* ctx, int64_t n) {
   const int n_blk = (int)((2 * N + kScanThreads - 1) / kScanThreads) + 1;
   const size_t zero_bytes = a256(2 * N * 8) + a256(2 * N * 4);  // arrival words
   const size_t bytes = zero_bytes + 3 * 2 * a256(N * 4) + 2 * a256(N * 8) + a256(2 * N * sizeof(BNode)) +
-                       a256(N * 6 * 8) + 2 * a256(N * 4) + 2 * a256(N * 4) + 2 * a256((size_t)n_blk * 4) +
+                       a256(N * 6 * 8) + 2 * a256(N * 4) + 2 * a256(N * 4) + 3 * a256((size_t)n_blk * 4) +
+                       a256(2 * N * sizeof(BNode)) + a256(2 * N * 4) +
                        2 * a256((N + 1) * 4) + a256((kMaxLevels + 2) * 4) + a256(2 * sizeof(LevelCtl)) +
                        a256(sizeof(Status)) + a256(3 * N * 4) + a256(N * 8);
   void* mem = nullptr;
@@ -785,6 +1105,9 @@ int ensure_ws(kdh_ctx* ctx, int64_t n) {
   ws->chunk_node[1] = (int*)take(N * 4);
   ws->blk_a = (int*)take((size_t)n_blk * 4);
   ws->blk_b = (int*)take((size_t)n_blk * 4);
+  ws->blk_c = (int*)take((size_t)n_blk * 4);
+  ws->stage = (BNode*)take(2 * N * sizeof(BNode));
+  ws->depth_cnt = (int*)take(2 * N * 4);
   ws->cbeg[0] = (int*)take((N + 1) * 4);
   ws->cbeg[1] = (int*)take((N + 1) * 4);
   ws->level_begin = (int*)take((kMaxLevels + 2) * 4);
@@ -792,13 +1115,12 @@ int ensure_ws(kdh_ctx* ctx, int64_t n) {
   ws->status = (Status*)take(sizeof(Status));
   ws->stage_xyz = (float*)take(3 * N * 4);
   ws->stage_t = (double*)take(N * 8);
-  e = hipHostMalloc((void**)&ws->h_status, sizeof(Status) * (kMaxLevels / kBatchLevels + 2) + 64,
-                    hipHostMallocDefault);
+  e = hipHostMalloc((void**)&ws->h_status, sizeof(Status) * (kMaxLevels / 2 + 2) + 64, hipHostMallocDefault);
   if (e != hipSuccess) {
     kdh_tree_build_ws_free(ctx);
     return kdh_fail(ctx, KDH_ERR_NOMEM, "kdh_tree_build: pinned status");
   }
-  ws->h_count = reinterpret_cast<int*>(ws->h_status + (kMaxLevels / kBatchLevels + 2));
+  ws->h_count = reinterpret_cast<int*>(ws->h_status + (kMaxLevels / 2 + 2));
   return KDH_OK;
 }
 
@@ -881,6 +1203,7 @@ int kdh_tree_build(kdh_ctx* ctx, const float* xyz, const double* stamps, int64_t
     a.cnt64 = ws->cnt64;
     a.blk_a = ws->blk_a;
     a.blk_b = ws->blk_b;
+    a.blk_c = ws->blk_c;
     a.level_begin = ws->level_begin;
     a.st = ws->status;
     a.level = L;
@@ -897,8 +1220,21 @@ int kdh_tree_build(kdh_ctx* ctx, const float* xyz, const double* stamps, int64_t
   };
 
   // Levels are queued one batch ahead of the status the host has seen; the
-  // levels queued past the last one find no nodes and return at once.
+  // levels queued past the last one find no nodes and return at once.  The
+  // first batch reaches the level where balanced splits hand over to the
+  // subtree phase; later batches are short.
+  int first_batch = kBatchLevels;
+  while (first_batch < 64 && ((int64_t)kSubLimit << (first_batch - 2)) < (int64_t)n) ++first_batch;
   int queued = 0, n_levels = -1;
+  Status seen;
+  memset(&seen, 0, sizeof(seen));
+  if (n <= kSubLimit) {  // the root is a subtree already
+    n_levels = 0;
+    seen.handoff = 1;
+    seen.handoff_level = 0;
+    seen.handoff_lb = 0;
+    seen.handoff_n = 1;
+  }
   std::vector<hipEvent_t> evs;
   auto release = [&]() {
     for (hipEvent_t ev : evs) hipEventDestroy(ev);
@@ -909,7 +1245,8 @@ int kdh_tree_build(kdh_ctx* ctx, const float* xyz, const double* stamps, int64_t
       return kdh_fail(ctx, KDH_ERR_INVALID, "kdh_tree_build: deeper than 4096 levels");
     }
     hipError_t e = hipSuccess;
-    for (int k = 0; k < kBatchLevels && e == hipSuccess; ++k) e = enqueue_level(queued++);
+    const int batch = evs.empty() ? first_batch : 2;
+    for (int k = 0; k < batch && e == hipSuccess; ++k) e = enqueue_level(queued++);
     hipEvent_t ev = nullptr;
     if (e == hipSuccess) e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
     if (e == hipSuccess) evs.push_back(ev);
@@ -919,7 +1256,10 @@ int kdh_tree_build(kdh_ctx* ctx, const float* xyz, const double* stamps, int64_t
     if (e == hipSuccess) e = hipEventRecord(ev, s);
     if (e == hipSuccess && b >= 1) {
       e = hipEventSynchronize(evs[b - 1]);
-      if (e == hipSuccess && ws->h_status[b - 1].last_level >= 0) n_levels = ws->h_status[b - 1].last_level;
+      if (e == hipSuccess && ws->h_status[b - 1].last_level >= 0) {
+        n_levels = ws->h_status[b - 1].last_level;
+        seen = ws->h_status[b - 1];
+      }
     }
     if (e != hipSuccess) {
       release();
@@ -928,10 +1268,40 @@ int kdh_tree_build(kdh_ctx* ctx, const float* xyz, const double* stamps, int64_t
   }
   release();
 
-  // nodes of the unpruned tree: level_begin[n_levels]
-  KDH_HIP_TRY(ctx, hipMemcpyAsync(ws->h_count, ws->level_begin + n_levels, sizeof(int), hipMemcpyDeviceToHost, s));
-  KDH_HIP_TRY(ctx, hipStreamSynchronize(s));
-  const int n_built = *ws->h_count;
+  int n_built = 0;
+  if (seen.handoff) {
+    // the subtree phase: the roots' levels on, then the merge into level order
+    SubArgs sa;
+    sa.nodes = ws->nodes;
+    sa.lb = seen.handoff_lb;
+    sa.level0 = seen.handoff_level;
+    sa.stage = ws->stage;
+    sa.depth_cnt = ws->depth_cnt;
+    const int cur = seen.handoff_level & 1;
+    sa.px = ws->px[cur];
+    sa.py = ws->py[cur];
+    sa.pz = ws->pz[cur];
+    sa.pt = ws->pt[cur];
+    sa.ox = ws->px[0];
+    sa.oy = ws->py[0];
+    sa.oz = ws->pz[0];
+    sa.ot = ws->pt[0];
+    sa.st = ws->status;
+    if (seen.handoff_level + 2 + kSubLimit > kMaxLevels + 2)
+      return kdh_fail(ctx, KDH_ERR_INVALID, "kdh_tree_build: deeper than 4096 levels");
+    hipLaunchKernelGGL(k_subtree, dim3(seen.handoff_n), dim3(kSubThreads), 0, s, sa, P);
+    hipLaunchKernelGGL(k_sub_levels, dim3(1), dim3(kScanThreads), 0, s, sa, seen.handoff_n, ws->level_begin);
+    hipLaunchKernelGGL(k_sub_place, dim3(seen.handoff_n), dim3(kSubThreads), 0, s, sa, ws->level_begin);
+    KDH_HIP_TRY(ctx, hipGetLastError());
+    KDH_HIP_TRY(ctx, hipMemcpyAsync(ws->h_status, ws->status, sizeof(Status), hipMemcpyDeviceToHost, s));
+    KDH_HIP_TRY(ctx, hipStreamSynchronize(s));
+    n_built = ws->h_status[0].n_built;
+  } else {
+    // nodes of the unpruned tree: level_begin[n_levels]
+    KDH_HIP_TRY(ctx, hipMemcpyAsync(ws->h_count, ws->level_begin + n_levels, sizeof(int), hipMemcpyDeviceToHost, s));
+    KDH_HIP_TRY(ctx, hipStreamSynchronize(s));
+    n_built = *ws->h_count;
+  }
   if (n_built < 1 || n_built > cap_nodes) return kdh_fail(ctx, KDH_ERR_INVALID, "kdh_tree_build: node capacity exceeded");
   const int gp = (n_built + kScanThreads - 1) / kScanThreads;
   if (gp > ws->n_blk) return kdh_fail(ctx, KDH_ERR_INVALID, "kdh_tree_build: prefix-sum blocks exceeded");
