@@ -119,6 +119,7 @@ struct Ws {
   int* cleft = nullptr;     // left points per chunk (cap)
   int* gleft = nullptr;     // left points per 64-chunk group slot (cap)
   unsigned long long* cnt64 = nullptr;  // per node: arrivals << 32 | count (2 cap); zero between uses
+  unsigned long long* cnt64_grp = nullptr;  // per 64-chunk group slot, the same (cap); zero between uses
   int* keep_word = nullptr; // per node: 2 x arrivals + keep bits (2 cap); zero between uses
   int* chunk_node[2] = {nullptr, nullptr};  // node of each chunk (cap each)
   int* blk_a = nullptr;     // prefix-sum block totals, then offsets (n_blk)
@@ -136,6 +137,11 @@ struct Ws {
   Status* h_status = nullptr;         // pinned
   int* h_count = nullptr;             // pinned
   void* mem = nullptr;
+  // the first batch of levels as a graph; its launches depend only on the
+  // workspace, the parameters and the batch length
+  hipGraphExec_t first_exec = nullptr;
+  int first_levels = 0;
+  kdh_tree_gen_params first_params{};
 };
 
 __device__ __forceinline__ int lane_id() { return threadIdx.x & 63; }
@@ -301,6 +307,7 @@ struct LevelArgs {
   int* cleft;
   int* gleft;
   unsigned long long* cnt64;
+  unsigned long long* cnt64_grp;
   int* blk_a;
   int* blk_b;
   int* blk_c;
@@ -344,8 +351,6 @@ __global__ void k_mom1(LevelArgs a) {
   double v[5] = {(double)a.px[i], (double)a.py[i], (double)a.pz[i], d, d * d};
   wave_pairwise<5>(v, r.cnt);
   store_chunk(a.csum, c, v, 5);
-  // the left-point totals of this level's 64-chunk groups start at zero
-  if (lane == 0 && ((c - r.cb) & (kC - 1)) == 0) a.gleft[r.cb + ((c - r.cb) >> 6)] = 0;
 }
 
 __global__ void k_mom2(LevelArgs a) {
@@ -437,11 +442,23 @@ __global__ void k_count(LevelArgs a) {
   int total = cnt;
   if (r.m > 1) {
     a.cleft[c] = cnt;
-    if (r.m > kC) atomicAdd(&a.gleft[r.cb + ((c - r.cb) >> 6)], cnt);
-    const unsigned long long old = atomicAdd(&a.cnt64[L.lb + r.j], (1ull << 32) | (unsigned long long)cnt);
-    if ((int)(old >> 32) != r.m - 1) return;
+    // at most 64 chunks meet on one word: first per 64-chunk group, then the
+    // last chunk of each group carries the group's total to the node's word
+    const int g = (c - r.cb) >> 6;
+    const int gcnt = min(kC, r.m - kC * g);
+    const int m2 = (r.m + kC - 1) / kC;
+    unsigned long long* word = m2 > 1 ? &a.cnt64_grp[r.cb + g] : &a.cnt64[L.lb + r.j];
+    unsigned long long old = atomicAdd(word, (1ull << 32) | (unsigned long long)cnt);
+    if ((int)(old >> 32) != (m2 > 1 ? gcnt : r.m) - 1) return;
     total = (int)(old & 0xffffffffull) + cnt;
-    a.cnt64[L.lb + r.j] = 0ull;
+    *word = 0ull;
+    if (m2 > 1) {
+      a.gleft[r.cb + g] = total;
+      old = atomicAdd(&a.cnt64[L.lb + r.j], (1ull << 32) | (unsigned long long)total);
+      if ((int)(old >> 32) != m2 - 1) return;
+      total = (int)(old & 0xffffffffull) + total;
+      a.cnt64[L.lb + r.j] = 0ull;
+    }
   }
   const int n = nd.end - nd.start;
   const int split = (total > 0 && total < n) ? 1 : 0;
@@ -1063,7 +1080,7 @@ int ensure_ws(kdh_ctx* ctx, int64_t n) {
   kdh_tree_build_ws_free(ctx);
   const size_t N = (size_t)cap;
   const int n_blk = (int)((2 * N + kScanThreads - 1) / kScanThreads) + 1;
-  const size_t zero_bytes = a256(2 * N * 8) + a256(2 * N * 4);  // arrival words
+  const size_t zero_bytes = a256(2 * N * 8) + a256(N * 8) + a256(2 * N * 4);  // arrival words
   const size_t bytes = zero_bytes + 3 * 2 * a256(N * 4) + 2 * a256(N * 8) + a256(2 * N * sizeof(BNode)) +
                        a256(N * 6 * 8) + 2 * a256(N * 4) + 2 * a256(N * 4) + 3 * a256((size_t)n_blk * 4) +
                        a256(2 * N * sizeof(BNode)) + a256(2 * N * 4) +
@@ -1090,6 +1107,7 @@ int ensure_ws(kdh_ctx* ctx, int64_t n) {
     return q;
   };
   ws->cnt64 = (unsigned long long*)take(2 * N * 8);
+  ws->cnt64_grp = (unsigned long long*)take(N * 8);
   ws->keep_word = (int*)take(2 * N * 4);
   for (int k = 0; k < 2; ++k) {
     ws->px[k] = (float*)take(N * 4);
@@ -1129,6 +1147,7 @@ int ensure_ws(kdh_ctx* ctx, int64_t n) {
 void kdh_tree_build_ws_free(kdh_ctx* ctx) {
   Ws*& ws = reinterpret_cast<Ws*&>(ctx->tree_build_ws);
   if (!ws) return;
+  if (ws->first_exec) hipGraphExecDestroy(ws->first_exec);
   if (ws->mem) hipFree(ws->mem);
   if (ws->h_status) hipHostFree(ws->h_status);
   delete ws;
@@ -1172,11 +1191,14 @@ int kdh_tree_build(kdh_ctx* ctx, const float* xyz, const double* stamps, int64_t
                      ws->chunk_node[0], ws->level_begin, ws->status);
   KDH_HIP_TRY(ctx, hipGetLastError());
 
-  const int n_chunks_max = (n + kC - 1) / kC;
+  // Launch shapes come from the workspace capacity, not from n, so that a
+  // batch of levels can be replayed as a graph for any cloud it holds.
+  const int64_t cap = ws->cap;
+  const int64_t cap_chunks = (cap + kC - 1) / kC;
   auto enqueue_level = [&](int L) -> hipError_t {
     const int cur = L & 1, nxt = cur ^ 1;
-    const int64_t width_bound = std::min<int64_t>((int64_t)1 << std::min(L, 30), n);
-    const int64_t chunk_bound = std::min<int64_t>(n, n_chunks_max + width_bound);
+    const int64_t width_bound = std::min<int64_t>((int64_t)1 << std::min(L, 30), cap);
+    const int64_t chunk_bound = std::min<int64_t>(cap, cap_chunks + width_bound);
     const int gc = (int)((chunk_bound + 3) / 4);  // one wave per chunk, four per block
     const int gn = (int)((width_bound + 255) / 256);
     const int ga = (int)((width_bound + kScanThreads - 1) / kScanThreads);
@@ -1201,6 +1223,7 @@ int kdh_tree_build(kdh_ctx* ctx, const float* xyz, const double* stamps, int64_t
     a.cleft = ws->cleft;
     a.gleft = ws->gleft;
     a.cnt64 = ws->cnt64;
+    a.cnt64_grp = ws->cnt64_grp;
     a.blk_a = ws->blk_a;
     a.blk_b = ws->blk_b;
     a.blk_c = ws->blk_c;
@@ -1245,8 +1268,30 @@ int kdh_tree_build(kdh_ctx* ctx, const float* xyz, const double* stamps, int64_t
       return kdh_fail(ctx, KDH_ERR_INVALID, "kdh_tree_build: deeper than 4096 levels");
     }
     hipError_t e = hipSuccess;
-    const int batch = evs.empty() ? first_batch : 2;
-    for (int k = 0; k < batch && e == hipSuccess; ++k) e = enqueue_level(queued++);
+    if (evs.empty()) {
+      // the first batch: a cached graph of its launches
+      if (!ws->first_exec || ws->first_levels != first_batch || memcmp(&ws->first_params, &P, sizeof(P)) != 0) {
+        if (ws->first_exec) hipGraphExecDestroy(ws->first_exec);
+        ws->first_exec = nullptr;
+        hipGraph_t graph = nullptr;
+        e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+        for (int k = 0; k < first_batch && e == hipSuccess; ++k) e = enqueue_level(k);
+        const hipError_t ce = hipStreamEndCapture(s, &graph);
+        if (e == hipSuccess) e = ce;
+        if (e == hipSuccess) e = hipGraphInstantiate(&ws->first_exec, graph, nullptr, nullptr, 0);
+        if (graph) hipGraphDestroy(graph);
+        if (e == hipSuccess) {
+          ws->first_levels = first_batch;
+          ws->first_params = P;
+        } else {
+          ws->first_exec = nullptr;
+        }
+      }
+      if (e == hipSuccess) e = hipGraphLaunch(ws->first_exec, s);
+      queued = first_batch;
+    } else {
+      for (int k = 0; k < 2 && e == hipSuccess; ++k) e = enqueue_level(queued++);
+    }
     hipEvent_t ev = nullptr;
     if (e == hipSuccess) e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
     if (e == hipSuccess) evs.push_back(ev);
