@@ -238,6 +238,8 @@ def main():
     ap.add_argument("--no-pmc", action="store_true")
     ap.add_argument("--no-precision", action="store_true", help="skip the 10 km precision-stress workload")
     ap.add_argument("--no-tree-build", action="store_true", help="skip the tree-generation timings")
+    ap.add_argument("--tree-build-only", action="store_true",
+                    help="only the tree-generation timings (for kernel traces of the builder)")
     ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.pmc_child:
@@ -247,6 +249,14 @@ def main():
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); alignments shard by pair with no "
               "communication across GPUs, see DESIGN.md", file=sys.stderr)
         sys.exit(2)
+
+    if args.tree_build_only:
+        from kd_slam2_amd import Context, DeviceTree, synth
+
+        w = synth.config2()
+        sw5 = _gen_sweep5(0)
+        print(json.dumps(run_tree_build(Context(0), w, sw5, DeviceTree)), flush=True)
+        return
 
     t0 = time.time()
     n_loop_pool = 16
