@@ -247,15 +247,17 @@ def main():
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); alignments shard by pair with no "
-              "communication across GPUs, see DESIGN.md", file=sys.stderr)
+              "communication across GPUs: run one process per GPU, each on its own share of the pairs", file=sys.stderr)
         sys.exit(2)
 
     if args.tree_build_only:
-        from kd_slam2_amd import Context, DeviceTree, synth
+        from kd_slam2_amd import Context, DeviceTree, TreeGenerator3D, default_params, synth
 
         w = synth.config2()
         sw5 = _gen_sweep5(0)
-        print(json.dumps(run_tree_build(Context(0), w, sw5, DeviceTree)), flush=True)
+        ctx = Context(0)
+        F = TreeGenerator3D().makeTreeOnDevice(ctx, w["map"], download=False)
+        print(json.dumps(run_tree_build(ctx, w, sw5, DeviceTree, F, _cfg2_params(default_params))), flush=True)
         return
 
     t0 = time.time()
@@ -422,7 +424,7 @@ def main():
     # ---- tree generation on the device (makeTree + upload) ------------------
     tree_build = None
     if not args.no_tree_build:
-        tree_build = run_tree_build(ctx, w, sweep5, DeviceTree)
+        tree_build = run_tree_build(ctx, w, sweep5, DeviceTree, F, p)
     w.pop("map", None)
 
     # ---- CPU baseline -------------------------------------------------------
@@ -502,7 +504,7 @@ def main():
     print(json.dumps(line), flush=True)
 
 
-def run_tree_build(ctx, w, sweep5, DeviceTree):
+def run_tree_build(ctx, w, sweep5, DeviceTree, fixed_tree=None, params=None):
     """makeTree + upload on the device (kdh_tree_build, bit-identical to the
     host builder) against the host builder single-threaded and with 16
     threads plus kdh_tree_upload: wall time from host points to a resident
@@ -542,7 +544,51 @@ def run_tree_build(ctx, w, sweep5, DeviceTree):
                          host_16thread_ms=round(1e3 * float(np.median(host16)), 2))
         log(f"tree build {name}: device {out[name]['device_ms_p50']:.2f} ms, host 1 thread "
             f"{out[name]['host_1thread_ms']:.1f} ms, 16 threads {out[name]['host_16thread_ms']:.1f} ms")
+    if fixed_tree is not None:
+        out["odometry_from_points"] = run_points_to_pose(ctx, w, fixed_tree, params, g, g16, DeviceTree)
     return out
+
+
+def run_points_to_pose(ctx, w, F, params, g, g16, DeviceTree):
+    """One CT odometry step from the raw 131k-point sweep to the aligned pose
+    against the resident 2M-point map: moving tree built on the device and
+    the device-resident Gauss-Newton loop, against the host builder (1 and 16
+    threads) + upload + the same loop."""
+    from kd_slam2_amd import CTICPHIP3D
+
+    def align(M):
+        al = CTICPHIP3D(ctx)
+        al.params = params
+        al.setFixed(F)
+        al.setMoving(M)
+        al.setFixedX(w["Xf"])
+        al.setX(w["X0"])
+        al.setVelocities(w["v0"])
+        al.computeOnDevice()
+        X = np.array(al.getX(), dtype=np.float64)
+        return X
+
+    res, poses = {}, {}
+    for name, make in (("device_build", lambda: g.makeTreeOnDevice(ctx, w["scan"], w["stamps"], download=False)),
+                       ("host_1thread_build", lambda: DeviceTree(ctx, g.makeTree(w["scan"], w["stamps"]))),
+                       ("host_16thread_build", lambda: DeviceTree(ctx, g16.makeTree(w["scan"], w["stamps"])))):
+        reps = 20 if name == "device_build" else 5
+        ts, X = [], None
+        for i in range(reps + 2):
+            t = time.perf_counter()
+            M = make()
+            X = align(M)
+            if i >= 2:
+                ts.append(time.perf_counter() - t)
+            M.close()
+        res[name + "_ms_p50"] = round(1e3 * float(np.median(ts)), 3)
+        poses[name] = X
+    # the trees are bit-identical, so the three routes end on the same pose
+    res["pose_max_abs_diff"] = float(max(np.abs(poses[k] - poses["device_build"]).max() for k in poses))
+    log(f"odometry from points: device build {res['device_build_ms_p50']:.2f} ms, host build "
+        f"{res['host_1thread_build_ms_p50']:.2f} ms (1 thread) / {res['host_16thread_build_ms_p50']:.2f} ms (16), "
+        f"pose max diff {res['pose_max_abs_diff']:.3g}")
+    return res
 
 
 def run_precision(ctx, w4, DeviceTree, default_params, synth):
