@@ -180,19 +180,72 @@ __device__ __forceinline__ int wave_sum_int(int v) {
   return v;
 }
 
+// Lane i gets x of lane i + S within its row of 16 lanes (DPP row_shl:S on
+// both halves; a VALU move, no LDS round trip).  Lanes whose source lies past
+// the row end get 0.
+template <int S>
+__device__ __forceinline__ double row_down(double x) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(x), 0x100 + S, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(x), 0x100 + S, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+
+// x of lane src (wave-uniform): a scalar read of that lane's register.
+__device__ __forceinline__ double lane_value(double x, int src) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(x), src);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(x), src);
+  return __hiloint2double(hi, lo);
+}
+
+// One pairwise round of the canonical order: lane i (a multiple of 2S) adds
+// lane i + S when i + S < cnt.
+template <int S, int K>
+__device__ __forceinline__ void pairwise_round(double (&v)[K], int cnt, int lane) {
+  if (S >= cnt) return;  // no lane takes (cnt is wave-uniform)
+  const bool take = ((lane & (2 * S - 1)) == 0) && (lane + S < cnt);
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const double o = row_down<S>(v[k]);
+    if (take) v[k] += o;
+  }
+}
+
+// Rounds of value stride 4S (S = 1, 2, 4, 8) over chunks held four values
+// per lane in rows of 16 (chunk_sums_rows): lane j of a row (a multiple of
+// 2S) adds lane j + S when value 4(j + S) exists.
+template <int S, int K>
+__device__ __forceinline__ void row_round(double (&v)[K], int cnt, int j) {
+  const bool take = ((j & (2 * S - 1)) == 0) && (4 * (j + S) < cnt);
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const double o = row_down<S>(v[k]);
+    if (take) v[k] += o;
+  }
+}
+
 // One group of the canonical order in a wave: lane i holds value i (i < cnt);
-// lane 0 ends with the group sum.
+// every lane ends with the group sum.  The rounds of stride 1..8 stay inside
+// rows of 16 lanes (DPP moves); the last two, (r0 + r16) + (r32 + r48) over
+// the row sums, are formed from scalar reads -- the same additions in the
+// same order as the stride-doubling tree.
 template <int K>
 __device__ __forceinline__ void wave_pairwise(double (&v)[K], int cnt) {
   const int lane = lane_id();
+  pairwise_round<1>(v, cnt, lane);
+  pairwise_round<2>(v, cnt, lane);
+  pairwise_round<4>(v, cnt, lane);
+  pairwise_round<8>(v, cnt, lane);
 #pragma unroll
-  for (int s = 1; s < tg::kGroup; s <<= 1) {
-    const bool take = ((lane & (2 * s - 1)) == 0) && (lane + s < cnt);
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
-      const double o = __shfl_down(v[k], s, 64);
-      if (take) v[k] += o;
+  for (int k = 0; k < K; ++k) {
+    const double r0 = lane_value(v[k], 0);
+    if (cnt <= 16) {
+      v[k] = r0;
+      continue;
     }
+    const double r16 = lane_value(v[k], 16), r32 = lane_value(v[k], 32), r48 = lane_value(v[k], 48);
+    const double a = r0 + r16;                 // 16 < cnt
+    const double b = 48 < cnt ? r32 + r48 : r32;
+    v[k] = 32 < cnt ? a + b : a;
   }
 }
 
@@ -666,25 +719,57 @@ constexpr int kSubWaves = kSubThreads / 64;
 // each 64-point chunk by the pairwise shuffles, the chunk sums (at most 32)
 // through the wave's LDS scratch and one more pairwise round.  item(i, v)
 // fills the K values of point i.  Every lane ends with the node sums.
+// The canonical sum of one chunk (cnt <= 64 values from index base) by one
+// row of 16 lanes: lane j of the row takes values 4j .. 4j+3 and forms the
+// first two rounds of the pairwise tree in registers ((v0 + v1) + (v2 + v3),
+// each addition only when its right operand exists); value strides 4, 8, 16
+// and 32 are lane strides 1, 2, 4 and 8 inside the row (DPP).  The row's
+// lane 0 ends with the sum.
+template <int K, typename Item>
+__device__ __forceinline__ void row_chunk_sum(int base, int cnt, int j, const Item& item, double (&a0)[K]) {
+  double a1[K], a2[K], a3[K];
+  item(base + min(4 * j + 0, cnt - 1), a0);
+  item(base + min(4 * j + 1, cnt - 1), a1);
+  item(base + min(4 * j + 2, cnt - 1), a2);
+  item(base + min(4 * j + 3, cnt - 1), a3);
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    if (4 * j + 1 < cnt) a0[k] += a1[k];
+    if (4 * j + 3 < cnt) a2[k] += a3[k];
+    if (4 * j + 2 < cnt) a0[k] += a2[k];
+  }
+  row_round<1>(a0, cnt, j);
+  row_round<2>(a0, cnt, j);
+  row_round<4>(a0, cnt, j);
+  row_round<8>(a0, cnt, j);
+}
+
+// Chunks t0 .. t0+3 of a node, one per row; each row's lane 0 stores its
+// chunk sum in scratch[t].
+template <int K, typename Item>
+__device__ __forceinline__ void chunk_sums_rows(int ns, int nn, int t0, int m, double (*scratch)[6],
+                                                const Item& item) {
+  const int lane = lane_id(), j = lane & 15;
+  const int t = t0 + (lane >> 4);
+  const int tc = min(t, m - 1);  // rows past the last chunk load a real one
+  double a0[K];
+  row_chunk_sum<K>(ns + tc * kC, min(kC, nn - tc * kC), j, item, a0);
+  if (j == 0 && t < m)
+    for (int k = 0; k < K; ++k) scratch[t][k] = a0[k];
+}
+
 template <int K, typename Item>
 __device__ __forceinline__ void node_sums(int ns, int nn, double (*scratch)[6], const Item& item, double (&out)[K]) {
   const int lane = lane_id();
   const int m = (nn + kC - 1) / kC;
-  for (int t = 0; t < m; ++t) {
-    const int cnt = min(kC, nn - t * kC);
-    double v[K];
-    item(ns + t * kC + min(lane, cnt - 1), v);
-    wave_pairwise<K>(v, cnt);
-    if (lane == 0)
-      for (int k = 0; k < K; ++k) scratch[t][k] = v[k];
-  }
+  for (int t0 = 0; t0 < m; t0 += 4) chunk_sums_rows<K>(ns, nn, t0, m, scratch, item);
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
   double v[K];
   for (int k = 0; k < K; ++k) v[k] = scratch[min(lane, m - 1)][k];
   wave_pairwise<K>(v, m);
-  for (int k = 0; k < K; ++k) out[k] = __shfl(v[k], 0, 64);
+  for (int k = 0; k < K; ++k) out[k] = v[k];
   __builtin_amdgcn_wave_barrier();  // the scratch is reused by the next node
 }
 
@@ -717,10 +802,11 @@ __global__ __launch_bounds__(kSubThreads) void k_subtree(SubArgs a, kdh_tree_gen
   for (;;) {
     const int m_d = q1 - q0;
     if (tid == 0) DC[d] = m_d;
-    // moments, a wave per node
+    // moments: a node of more than one chunk by a wave, ...
     for (int k = wave; k < m_d; k += kSubWaves) {
       BNode& nd = S[q0 + k];
       const int ns = nd.start - p_begin, nn = nd.end - nd.start;
+      if (nn <= kC) continue;
       double s1[5];
       node_sums<5>(ns, nn, s_chunk[wave], [&](int i, double (&v)[5]) {
         const double dd = stt[0][i] - ts0;
@@ -745,6 +831,46 @@ __global__ __launch_bounds__(kSubThreads) void k_subtree(SubArgs a, kdh_tree_gen
       if (lane == 0) {
         for (int q = 0; q < 5; ++q) nd.s1[q] = s1[q];
         for (int q = 0; q < 6; ++q) nd.c2[q] = c2[q];
+      }
+    }
+    // ... a single-chunk node (its sum is its chunk's sum) by a row of 16
+    // lanes, four nodes to a wave
+    {
+      const int row = lane >> 4, j = lane & 15;
+      for (int k0 = 4 * wave; k0 < m_d; k0 += 4 * kSubWaves) {
+        const int k = min(k0 + row, m_d - 1);
+        BNode& nd = S[q0 + k];
+        const int ns = nd.start - p_begin, nn = nd.end - nd.start;
+        const bool mine = k0 + row < m_d && nn <= kC;
+        if (__ballot(mine) == 0) continue;
+        const int cnt = min(nn, kC);  // a larger node's first chunk: loaded, not stored
+        double s1[5];
+        row_chunk_sum<5>(ns, cnt, j, [&](int i, double (&v)[5]) {
+          const double dd = stt[0][i] - ts0;
+          v[0] = (double)sx[0][i];
+          v[1] = (double)sy[0][i];
+          v[2] = (double)sz[0][i];
+          v[3] = dd;
+          v[4] = dd * dd;
+        }, s1);
+        const int src = lane & ~15;
+        const double inv = 1.0 / (double)nn;
+        const double mu0 = __shfl(s1[0], src, 64) * inv, mu1 = __shfl(s1[1], src, 64) * inv,
+                     mu2 = __shfl(s1[2], src, 64) * inv;
+        double c2[6];
+        row_chunk_sum<6>(ns, cnt, j, [&](int i, double (&v)[6]) {
+          const double d0 = (double)sx[0][i] - mu0, d1 = (double)sy[0][i] - mu1, d2 = (double)sz[0][i] - mu2;
+          v[0] = d0 * d0;
+          v[1] = d0 * d1;
+          v[2] = d0 * d2;
+          v[3] = d1 * d1;
+          v[4] = d1 * d2;
+          v[5] = d2 * d2;
+        }, c2);
+        if (mine && j == 0) {
+          for (int q = 0; q < 5; ++q) nd.s1[q] = s1[q];
+          for (int q = 0; q < 6; ++q) nd.c2[q] = c2[q];
+        }
       }
     }
     __syncthreads();
