@@ -116,6 +116,7 @@ struct Ws {
   double* pt[2] = {nullptr, nullptr};
   BNode* nodes = nullptr;   // 2 cap
   double* csum = nullptr;   // chunk sums, 6 per chunk (cap)
+  double* gsum = nullptr;   // group sums of the top levels, 6 per slot (cap)
   int* cleft = nullptr;     // left points per chunk (cap)
   int* gleft = nullptr;     // left points per 64-chunk group slot (cap)
   unsigned long long* cnt64 = nullptr;  // per node: arrivals << 32 | count (2 cap); zero between uses
@@ -132,6 +133,7 @@ struct Ws {
   int* level_begin = nullptr;         // kMaxLevels + 2
   LevelCtl* ctl = nullptr;            // [2]
   Status* status = nullptr;
+  unsigned long long* ts_key = nullptr;  // smallest stamp, order key; ~0 between builds
   float* stage_xyz = nullptr;         // 3 cap (host inputs)
   double* stage_t = nullptr;          // cap
   Status* h_status = nullptr;         // pinned
@@ -250,18 +252,19 @@ __device__ __forceinline__ void wave_pairwise(double (&v)[K], int cnt) {
 }
 
 // The canonical reduction of the m values at vals[base..] (6 doubles apart)
-// by one wave, in place; every lane ends with the result.  Four groups are
-// loaded per round trip; group g's sum goes to slot g, which lies in a group
-// already consumed.
+// by one wave, in place; every lane ends with the result.  kRedBatch groups
+// are loaded per round trip; group g's sum goes to slot g, which lies in a
+// group already consumed.
+constexpr int kRedBatch = 8;
 template <int K>
 __device__ void wave_reduce_values(double* vals, int base, int m, double (&out)[K]) {
   const int lane = lane_id();
   while (m > 1) {
     const int groups = (m + tg::kGroup - 1) / tg::kGroup;
-    for (int g = 0; g < groups; g += 4) {
-      double v[4][K];
+    for (int g = 0; g < groups; g += kRedBatch) {
+      double v[kRedBatch][K];
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
+      for (int u = 0; u < kRedBatch; ++u) {
         const int b = (g + u) * tg::kGroup;
         const int cnt = g + u < groups ? min(m - b, tg::kGroup) : 1;
         const double* src = vals + (size_t)(base + (g + u < groups ? b + min(lane, cnt - 1) : 0)) * 6;
@@ -269,14 +272,14 @@ __device__ void wave_reduce_values(double* vals, int base, int m, double (&out)[
         for (int k = 0; k < K; ++k) v[u][k] = src[k];
       }
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
+      for (int u = 0; u < kRedBatch; ++u) {
         if (g + u >= groups) break;
         const int b = (g + u) * tg::kGroup;
         wave_pairwise<K>(v[u], min(m - b, tg::kGroup));
       }
       if (lane == 0) {
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+        for (int u = 0; u < kRedBatch; ++u) {
           if (g + u >= groups) break;
           double* dst = vals + (size_t)(base + g + u) * 6;
 #pragma unroll
@@ -296,30 +299,52 @@ __device__ void wave_reduce_values(double* vals, int base, int m, double (&out)[
 
 // ---- set-up ------------------------------------------------------------------
 
-__global__ void k_stage(const float* __restrict__ xyz, const double* __restrict__ stamps, int n, float* px,
-                        float* py, float* pz, double* pt) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  px[i] = xyz[3 * i];
-  py[i] = xyz[3 * i + 1];
-  pz[i] = xyz[3 * i + 2];
-  pt[i] = stamps ? stamps[i] : 0.0;
+// Order-preserving integer image of a double (not NaN) and back.
+__device__ __forceinline__ unsigned long long order_key(double d) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(d);
+  return (b >> 63) ? ~b : (b | (1ull << 63));
+}
+__device__ __forceinline__ double from_order_key(unsigned long long k) {
+  return __longlong_as_double((long long)((k >> 63) ? (k & ~(1ull << 63)) : ~k));
 }
 
-__global__ __launch_bounds__(kScanThreads) void k_init(const double* __restrict__ pt, int n, BNode* nodes,
+// Points into structure-of-arrays form.  The smallest stamp (fmin over the
+// cloud from DBL_MAX, absent stamps counting as 0) goes into *ts_key as an
+// order-preserving integer, one atomicMin per block: the minimum does not
+// depend on the order of arrival.  256 threads per block.
+__global__ __launch_bounds__(256) void k_stage(const float* __restrict__ xyz, const double* __restrict__ stamps,
+                                               int n, float* px, float* py, float* pz, double* pt,
+                                               unsigned long long* ts_key) {
+  __shared__ double s_min[4];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double m = 1.7976931348623157e308;
+  if (i < n) {
+    px[i] = xyz[3 * i];
+    py[i] = xyz[3 * i + 1];
+    pz[i] = xyz[3 * i + 2];
+    const double t = stamps ? stamps[i] : 0.0;
+    pt[i] = t;
+    m = fmin(m, t);
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) m = fmin(m, __shfl_xor(m, o, 64));
+  if (lane_id() == 0) s_min[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    // most blocks find the minimum already below theirs (a stale read only
+    // costs an atomic that changes nothing)
+    const unsigned long long key = order_key(fmin(fmin(s_min[0], s_min[1]), fmin(s_min[2], s_min[3])));
+    if (key < __atomic_load_n(ts_key, __ATOMIC_RELAXED)) atomicMin(ts_key, key);
+  }
+}
+
+// The root, the first level's control and chunk map, and the status; the
+// stamp minimum is read from (and ts_key reset for the next build).
+__global__ __launch_bounds__(kScanThreads) void k_init(unsigned long long* ts_key, int n, BNode* nodes,
                                                        LevelCtl* ctl, int* cbeg0, int* chunk_node0, int* level_begin,
                                                        Status* st) {
   const int nc0 = (n + kC - 1) / kC;
   for (int c = threadIdx.x; c < nc0; c += kScanThreads) chunk_node0[c] = 0;
-  __shared__ double s_min[kScanThreads];
-  double m = 1.7976931348623157e308;
-  for (int i = threadIdx.x; i < n; i += kScanThreads) m = fmin(m, pt[i]);
-  s_min[threadIdx.x] = m;
-  __syncthreads();
-  for (int s = kScanThreads / 2; s > 0; s >>= 1) {
-    if (threadIdx.x < s) s_min[threadIdx.x] = fmin(s_min[threadIdx.x], s_min[threadIdx.x + s]);
-    __syncthreads();
-  }
   if (threadIdx.x == 0) {
     BNode r;
     memset(&r, 0, sizeof(r));
@@ -337,8 +362,9 @@ __global__ __launch_bounds__(kScanThreads) void k_init(const double* __restrict_
     Status s;
     memset(&s, 0, sizeof(s));
     s.last_level = -1;
-    s.ts_start = s_min[0];
+    s.ts_start = from_order_key(*ts_key);
     *st = s;
+    *ts_key = ~0ull;
   }
 }
 
@@ -357,6 +383,7 @@ struct LevelArgs {
   const int* chunk_node; // this level's
   int* chunk_node_next;
   double* csum;
+  double* gsum;         // first-round group sums of the top levels (k_gsum)
   int* cleft;
   int* gleft;
   unsigned long long* cnt64;
@@ -421,7 +448,33 @@ __global__ void k_mom2(LevelArgs a) {
   store_chunk(a.csum, c, v, 6);
 }
 
-__global__ void k_red1(LevelArgs a) {
+// The first round of the nodes' canonical reductions spread over many waves:
+// group g of node j (its chunks 64g .. 64g + 63) is summed by one wave into
+// gsum[cbeg[j] + g].  Used on the top levels, where few nodes hold many
+// chunks each; k_red1 / k_red2 then reduce the group sums.  Grid: one
+// workgroup column per node, gridDim.y columns of waves share its groups.
+template <int K>
+__global__ __launch_bounds__(1024) void k_gsum(LevelArgs a) {
+  const LevelCtl L = *a.ctl;
+  const int j = blockIdx.x;
+  if (!L.active || j >= L.le - L.lb) return;
+  const int base = a.cbeg[j], m = a.cbeg[j + 1] - base;
+  const int groups = (m + tg::kGroup - 1) / tg::kGroup;
+  const int per = blockDim.x >> 6, lane = lane_id();
+  for (int g = blockIdx.y * per + (threadIdx.x >> 6); g < groups; g += gridDim.y * per) {
+    const int b = g * tg::kGroup, cnt = min(m - b, tg::kGroup);
+    const double* src = a.csum + (size_t)(base + b + min(lane, cnt - 1)) * 6;
+    double v[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) v[k] = src[k];
+    wave_pairwise<K>(v, cnt);
+    if (lane == 0)
+      for (int k = 0; k < K; ++k) a.gsum[(size_t)(base + g) * 6 + k] = v[k];
+  }
+}
+
+// grouped: the first round was done by k_gsum.
+__global__ void k_red1(LevelArgs a, int grouped) {
   const LevelCtl L = *a.ctl;
   if (!L.active) return;
   const int w = L.le - L.lb;
@@ -430,7 +483,10 @@ __global__ void k_red1(LevelArgs a) {
   for (int j = wave; j < w; j += n_waves) {
     const int base = a.cbeg[j], m = a.cbeg[j + 1] - base;
     double s1[5];
-    wave_reduce_values<5>(a.csum, base, m, s1);
+    if (grouped)
+      wave_reduce_values<5>(a.gsum, base, (m + tg::kGroup - 1) / tg::kGroup, s1);
+    else
+      wave_reduce_values<5>(a.csum, base, m, s1);
     if (lane_id() == 0) {
       BNode& nd = a.nodes[L.lb + j];
 #pragma unroll
@@ -439,7 +495,7 @@ __global__ void k_red1(LevelArgs a) {
   }
 }
 
-__global__ void k_red2(LevelArgs a, kdh_tree_gen_params P) {
+__global__ void k_red2(LevelArgs a, kdh_tree_gen_params P, int grouped) {
   const LevelCtl L = *a.ctl;
   if (!L.active) return;
   const int w = L.le - L.lb;
@@ -448,7 +504,10 @@ __global__ void k_red2(LevelArgs a, kdh_tree_gen_params P) {
   for (int j = wave; j < w; j += n_waves) {
     const int base = a.cbeg[j], m = a.cbeg[j + 1] - base;
     double c2[6];
-    wave_reduce_values<6>(a.csum, base, m, c2);
+    if (grouped)
+      wave_reduce_values<6>(a.gsum, base, (m + tg::kGroup - 1) / tg::kGroup, c2);
+    else
+      wave_reduce_values<6>(a.csum, base, m, c2);
     if (lane_id() == 0) {
       BNode& nd = a.nodes[L.lb + j];
       tg::Label lab;
@@ -773,15 +832,20 @@ __device__ __forceinline__ void node_sums(int ns, int nn, double (*scratch)[6], 
   __builtin_amdgcn_wave_barrier();  // the scratch is reused by the next node
 }
 
-__global__ __launch_bounds__(kSubThreads) void k_subtree(SubArgs a, kdh_tree_gen_params P) {
-  __shared__ float sx[2][kSubLimit], sy[2][kSubLimit], sz[2][kSubLimit];
-  __shared__ double stt[2][kSubLimit];
-  __shared__ double s_chunk[kSubWaves][kSubLimit / kC][6];
+// LIMIT: the largest subtree this launch takes; it sizes the LDS, so a
+// launch for small subtrees keeps several workgroups on a CU.  Subtrees of
+// at most `lo` points belong to another launch.
+template <int LIMIT>
+__global__ __launch_bounds__(kSubThreads) void k_subtree(SubArgs a, kdh_tree_gen_params P, int lo) {
+  __shared__ float sx[2][LIMIT], sy[2][LIMIT], sz[2][LIMIT];
+  __shared__ double stt[2][LIMIT];
+  __shared__ double s_chunk[kSubWaves][LIMIT / kC][6];
   __shared__ int s_scan[16];
   const int tid = threadIdx.x, lane = lane_id(), wave = tid >> 6;
   const int sidx = blockIdx.x;
   BNode* root = &a.nodes[a.lb + sidx];
   const int p_begin = root->start, n = root->end - root->start;
+  if (n <= lo || n > LIMIT) return;
   BNode* S = a.stage + 2 * (size_t)p_begin;
   int* DC = a.depth_cnt + p_begin + sidx;
   const double ts0 = a.st->ts_start;
@@ -1208,10 +1272,10 @@ int ensure_ws(kdh_ctx* ctx, int64_t n) {
   const int n_blk = (int)((2 * N + kScanThreads - 1) / kScanThreads) + 1;
   const size_t zero_bytes = a256(2 * N * 8) + a256(N * 8) + a256(2 * N * 4);  // arrival words
   const size_t bytes = zero_bytes + 3 * 2 * a256(N * 4) + 2 * a256(N * 8) + a256(2 * N * sizeof(BNode)) +
-                       a256(N * 6 * 8) + 2 * a256(N * 4) + 2 * a256(N * 4) + 3 * a256((size_t)n_blk * 4) +
+                       2 * a256(N * 6 * 8) + 2 * a256(N * 4) + 2 * a256(N * 4) + 3 * a256((size_t)n_blk * 4) +
                        a256(2 * N * sizeof(BNode)) + a256(2 * N * 4) +
                        2 * a256((N + 1) * 4) + a256((kMaxLevels + 2) * 4) + a256(2 * sizeof(LevelCtl)) +
-                       a256(sizeof(Status)) + a256(3 * N * 4) + a256(N * 8);
+                       a256(sizeof(Status)) + a256(3 * N * 4) + a256(N * 8) + a256(8);
   void* mem = nullptr;
   hipError_t e = hipMalloc(&mem, bytes);
   if (e != hipSuccess)
@@ -1243,6 +1307,7 @@ int ensure_ws(kdh_ctx* ctx, int64_t n) {
   }
   ws->nodes = (BNode*)take(2 * N * sizeof(BNode));
   ws->csum = (double*)take(N * 6 * 8);
+  ws->gsum = (double*)take(N * 6 * 8);
   ws->cleft = (int*)take(N * 4);
   ws->gleft = (int*)take(N * 4);
   ws->chunk_node[0] = (int*)take(N * 4);
@@ -1259,6 +1324,12 @@ int ensure_ws(kdh_ctx* ctx, int64_t n) {
   ws->status = (Status*)take(sizeof(Status));
   ws->stage_xyz = (float*)take(3 * N * 4);
   ws->stage_t = (double*)take(N * 8);
+  ws->ts_key = (unsigned long long*)take(8);
+  e = hipMemset(ws->ts_key, 0xFF, 8);
+  if (e != hipSuccess) {
+    kdh_tree_build_ws_free(ctx);
+    return kdh_hip_fail(ctx, e, "kdh_tree_build: workspace clear");
+  }
   e = hipHostMalloc((void**)&ws->h_status, sizeof(Status) * (kMaxLevels / 2 + 2) + 64, hipHostMallocDefault);
   if (e != hipSuccess) {
     kdh_tree_build_ws_free(ctx);
@@ -1312,8 +1383,8 @@ int kdh_tree_build(kdh_ctx* ctx, const float* xyz, const double* stamps, int64_t
     }
   }
   hipLaunchKernelGGL(k_stage, dim3((n + 255) / 256), dim3(256), 0, s, d_xyz, d_t, n, ws->px[0], ws->py[0], ws->pz[0],
-                     ws->pt[0]);
-  hipLaunchKernelGGL(k_init, dim3(1), dim3(kScanThreads), 0, s, ws->pt[0], n, ws->nodes, ws->ctl, ws->cbeg[0],
+                     ws->pt[0], ws->ts_key);
+  hipLaunchKernelGGL(k_init, dim3(1), dim3(kScanThreads), 0, s, ws->ts_key, n, ws->nodes, ws->ctl, ws->cbeg[0],
                      ws->chunk_node[0], ws->level_begin, ws->status);
   KDH_HIP_TRY(ctx, hipGetLastError());
 
@@ -1321,7 +1392,14 @@ int kdh_tree_build(kdh_ctx* ctx, const float* xyz, const double* stamps, int64_t
   // batch of levels can be replayed as a graph for any cloud it holds.
   const int64_t cap = ws->cap;
   const int64_t cap_chunks = (cap + kC - 1) / kC;
+  // Levels whose nodes average four or more 64-chunk groups (about 16k
+  // points; the first batch reaches nodes of 2048) run the first round of
+  // their reductions across many waves (k_gsum).
+  int first_batch = kBatchLevels;
+  while (first_batch < 64 && ((int64_t)kSubLimit << (first_batch - 2)) < (int64_t)n) ++first_batch;
+  const int grouped_levels = first_batch - 4;  // levels [0, grouped_levels)
   auto enqueue_level = [&](int L) -> hipError_t {
+    const int grouped = L < grouped_levels ? 1 : 0;
     const int cur = L & 1, nxt = cur ^ 1;
     const int64_t width_bound = std::min<int64_t>((int64_t)1 << std::min(L, 30), cap);
     const int64_t chunk_bound = std::min<int64_t>(cap, cap_chunks + width_bound);
@@ -1346,6 +1424,7 @@ int kdh_tree_build(kdh_ctx* ctx, const float* xyz, const double* stamps, int64_t
     a.chunk_node = ws->chunk_node[cur];
     a.chunk_node_next = ws->chunk_node[nxt];
     a.csum = ws->csum;
+    a.gsum = ws->gsum;
     a.cleft = ws->cleft;
     a.gleft = ws->gleft;
     a.cnt64 = ws->cnt64;
@@ -1357,10 +1436,13 @@ int kdh_tree_build(kdh_ctx* ctx, const float* xyz, const double* stamps, int64_t
     a.st = ws->status;
     a.level = L;
     a.cap_nodes = cap_nodes;
+    const dim3 g_grp((unsigned)width_bound, 8);  // a node's groups over 8 x 16 waves
     hipLaunchKernelGGL(k_mom1, dim3(gc), dim3(256), 0, s, a);
-    hipLaunchKernelGGL(k_red1, dim3(gw), dim3(256), 0, s, a);
+    if (grouped) hipLaunchKernelGGL(k_gsum<5>, g_grp, dim3(1024), 0, s, a);
+    hipLaunchKernelGGL(k_red1, dim3(gw), dim3(256), 0, s, a, grouped);
     hipLaunchKernelGGL(k_mom2, dim3(gc), dim3(256), 0, s, a);
-    hipLaunchKernelGGL(k_red2, dim3(gw), dim3(256), 0, s, a, P);
+    if (grouped) hipLaunchKernelGGL(k_gsum<6>, g_grp, dim3(1024), 0, s, a);
+    hipLaunchKernelGGL(k_red2, dim3(gw), dim3(256), 0, s, a, P, grouped);
     hipLaunchKernelGGL(k_count, dim3(gc), dim3(256), 0, s, a);
     hipLaunchKernelGGL(k_alloc, dim3(ga), dim3(kScanThreads), 0, s, a);
     hipLaunchKernelGGL(k_alloc2, dim3(1), dim3(kScanThreads), 0, s, a, ga);
@@ -1372,8 +1454,6 @@ int kdh_tree_build(kdh_ctx* ctx, const float* xyz, const double* stamps, int64_t
   // levels queued past the last one find no nodes and return at once.  The
   // first batch reaches the level where balanced splits hand over to the
   // subtree phase; later batches are short.
-  int first_batch = kBatchLevels;
-  while (first_batch < 64 && ((int64_t)kSubLimit << (first_batch - 2)) < (int64_t)n) ++first_batch;
   int queued = 0, n_levels = -1;
   Status seen;
   memset(&seen, 0, sizeof(seen));
@@ -1460,7 +1540,16 @@ int kdh_tree_build(kdh_ctx* ctx, const float* xyz, const double* stamps, int64_t
     sa.st = ws->status;
     if (seen.handoff_level + 2 + kSubLimit > kMaxLevels + 2)
       return kdh_fail(ctx, KDH_ERR_INVALID, "kdh_tree_build: deeper than 4096 levels");
-    hipLaunchKernelGGL(k_subtree, dim3(seen.handoff_n), dim3(kSubThreads), 0, s, sa, P);
+    // Subtrees by size class, each launch with LDS for its largest: one
+    // round of workgroups holds every subtree when there are few; else the
+    // small ones (the most) run several to a CU.
+    if (seen.handoff_n <= 256) {
+      hipLaunchKernelGGL(k_subtree<kSubLimit>, dim3(seen.handoff_n), dim3(kSubThreads), 0, s, sa, P, 0);
+    } else {
+      hipLaunchKernelGGL(k_subtree<512>, dim3(seen.handoff_n), dim3(kSubThreads), 0, s, sa, P, 0);
+      hipLaunchKernelGGL(k_subtree<1024>, dim3(seen.handoff_n), dim3(kSubThreads), 0, s, sa, P, 512);
+      hipLaunchKernelGGL(k_subtree<kSubLimit>, dim3(seen.handoff_n), dim3(kSubThreads), 0, s, sa, P, 1024);
+    }
     hipLaunchKernelGGL(k_sub_levels, dim3(1), dim3(kScanThreads), 0, s, sa, seen.handoff_n, ws->level_begin);
     hipLaunchKernelGGL(k_sub_place, dim3(seen.handoff_n), dim3(kSubThreads), 0, s, sa, ws->level_begin);
     KDH_HIP_TRY(ctx, hipGetLastError());
