@@ -143,6 +143,10 @@ struct Ws {
   // workspace, the parameters and the batch length
   hipGraphExec_t first_exec = nullptr;
   int first_levels = 0;
+  // two more streams (default priority) for the subtree size classes, with
+  // the fork and join events; made at first use
+  hipStream_t aux[2] = {nullptr, nullptr};
+  hipEvent_t fork = nullptr, join[2] = {nullptr, nullptr};
   kdh_tree_gen_params first_params{};
 };
 
@@ -1345,6 +1349,11 @@ void kdh_tree_build_ws_free(kdh_ctx* ctx) {
   Ws*& ws = reinterpret_cast<Ws*&>(ctx->tree_build_ws);
   if (!ws) return;
   if (ws->first_exec) hipGraphExecDestroy(ws->first_exec);
+  for (int k = 0; k < 2; ++k) {
+    if (ws->aux[k]) hipStreamDestroy(ws->aux[k]);
+    if (ws->join[k]) hipEventDestroy(ws->join[k]);
+  }
+  if (ws->fork) hipEventDestroy(ws->fork);
   if (ws->mem) hipFree(ws->mem);
   if (ws->h_status) hipHostFree(ws->h_status);
   delete ws;
@@ -1542,13 +1551,25 @@ int kdh_tree_build(kdh_ctx* ctx, const float* xyz, const double* stamps, int64_t
       return kdh_fail(ctx, KDH_ERR_INVALID, "kdh_tree_build: deeper than 4096 levels");
     // Subtrees by size class, each launch with LDS for its largest: one
     // round of workgroups holds every subtree when there are few; else the
-    // small ones (the most) run several to a CU.
+    // small ones (the most) run several to a CU, and the three classes run
+    // side by side on three streams.
     if (seen.handoff_n <= 256) {
       hipLaunchKernelGGL(k_subtree<kSubLimit>, dim3(seen.handoff_n), dim3(kSubThreads), 0, s, sa, P, 0);
     } else {
-      hipLaunchKernelGGL(k_subtree<512>, dim3(seen.handoff_n), dim3(kSubThreads), 0, s, sa, P, 0);
-      hipLaunchKernelGGL(k_subtree<1024>, dim3(seen.handoff_n), dim3(kSubThreads), 0, s, sa, P, 512);
+      for (int k = 0; k < 2; ++k) {
+        if (!ws->aux[k]) KDH_HIP_TRY(ctx, hipStreamCreateWithFlags(&ws->aux[k], hipStreamNonBlocking));
+        if (!ws->join[k]) KDH_HIP_TRY(ctx, hipEventCreateWithFlags(&ws->join[k], hipEventDisableTiming));
+      }
+      if (!ws->fork) KDH_HIP_TRY(ctx, hipEventCreateWithFlags(&ws->fork, hipEventDisableTiming));
+      KDH_HIP_TRY(ctx, hipEventRecord(ws->fork, s));
+      for (int k = 0; k < 2; ++k) KDH_HIP_TRY(ctx, hipStreamWaitEvent(ws->aux[k], ws->fork, 0));
       hipLaunchKernelGGL(k_subtree<kSubLimit>, dim3(seen.handoff_n), dim3(kSubThreads), 0, s, sa, P, 1024);
+      hipLaunchKernelGGL(k_subtree<1024>, dim3(seen.handoff_n), dim3(kSubThreads), 0, ws->aux[0], sa, P, 512);
+      hipLaunchKernelGGL(k_subtree<512>, dim3(seen.handoff_n), dim3(kSubThreads), 0, ws->aux[1], sa, P, 0);
+      for (int k = 0; k < 2; ++k) {
+        KDH_HIP_TRY(ctx, hipEventRecord(ws->join[k], ws->aux[k]));
+        KDH_HIP_TRY(ctx, hipStreamWaitEvent(s, ws->join[k], 0));
+      }
     }
     hipLaunchKernelGGL(k_sub_levels, dim3(1), dim3(kScanThreads), 0, s, sa, seen.handoff_n, ws->level_begin);
     hipLaunchKernelGGL(k_sub_place, dim3(seen.handoff_n), dim3(kSubThreads), 0, s, sa, ws->level_begin);
