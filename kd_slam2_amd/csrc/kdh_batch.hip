@@ -194,9 +194,11 @@ int kdh_batch_align(kdh_ctx* ctx, int n_pairs, const kdh_pair* pairs, kdh_pair_r
     if (le == hipSuccess) le = hipEventRecord(ctx->ev[1], ctx->stream);
     if (le == hipSuccess) le = hipStreamSynchronize(ctx->stream);
   }
-  if (le == hipSuccess)
-    le = hipMemcpyAsync(ctx->h_pairs, ctx->d_pairs, sizeof(PairState) * n_pairs, hipMemcpyDeviceToHost, ctx->stream);
-  if (le == hipSuccess) le = hipStreamSynchronize(ctx->stream);
+  if (le == hipSuccess) {
+    rc = kdh_pull_results(ctx, n_pairs);
+    if (rc) return rc;
+    le = hipStreamSynchronize(ctx->stream);
+  }
   if (le != hipSuccess) return kdh_hip_fail(ctx, le, "batch replay");
   if (ctx->profile) {
     kdh_collect_times(ctx, iters, true);

@@ -71,6 +71,12 @@ int kdh_ctx_create(int device, kdh_ctx** out) {
     delete c;
     return KDH_ERR_NOMEM;
   }
+  if (hipHostGetDevicePointer((void**)&c->h_pairs_dev, c->h_pairs, 0) != hipSuccess || !c->h_pairs_dev) {
+    hipHostFree(c->h_pairs);
+    hipStreamDestroy(c->stream);
+    delete c;
+    return KDH_ERR_HIP;
+  }
   c->h_pairs_cap = 1;
   for (int i = 0; i < kdh_ctx::kEvents; ++i) {
     if (hipEventCreate(&c->ev[i]) != hipSuccess) {
@@ -241,12 +247,47 @@ int kdh_ensure_pairs(kdh_ctx* ctx, size_t n_pairs) {
     ctx->h_pairs = nullptr;
     size_t nc = std::max(n_pairs, 2 * ctx->h_pairs_cap);
     hipError_t e = hipHostMalloc((void**)&ctx->h_pairs, nc * sizeof(PairState), hipHostMallocDefault);
+    if (e == hipSuccess) e = hipHostGetDevicePointer((void**)&ctx->h_pairs_dev, ctx->h_pairs, 0);
     if (e != hipSuccess) {
+      if (ctx->h_pairs) hipHostFree(ctx->h_pairs);
+      ctx->h_pairs = ctx->h_pairs_dev = nullptr;
       ctx->h_pairs_cap = 0;
       return kdh_fail(ctx, KDH_ERR_NOMEM, "pinned pair buffer");
     }
     ctx->h_pairs_cap = nc;
   }
+  return KDH_OK;
+}
+
+// One workgroup per pair writes its results straight into the pinned
+// mirror.  A kernel rather than a DMA copy: the DMA path of a 200 KB
+// device-to-host copy stalled for about 5 ms about once in 200 served steps.
+__global__ void k_pull_results(const PairState* __restrict__ d, PairState* h, int n) {
+  const int p = blockIdx.x, t = threadIdx.x;
+  if (p >= n) return;
+  const PairState& s = d[p];
+  PairState& o = h[p];
+  if (t < 12) o.X[t] = s.X[t];
+  if (t < 6) o.v[t] = s.v[t];
+  if (t == 0) {
+    o.iter = s.iter;
+    o.status = s.status;
+    o.prev_score = s.prev_score;
+  }
+  static_assert(sizeof(DevStats) % 8 == 0, "stats copied as doubles");
+  const int words = (int)(sizeof(DevStats) / 8) * min(max(s.iter, 0), kMaxIter);
+  const double* src = reinterpret_cast<const double*>(s.stats_log);
+  double* dst = reinterpret_cast<double*>(o.stats_log);
+  for (int i = t; i < words; i += 64) dst[i] = src[i];
+  __threadfence_system();
+}
+
+int kdh_pull_results(kdh_ctx* ctx, int n_pairs) {
+  if (n_pairs <= 0) return KDH_OK;
+  if (!ctx->h_pairs_dev || (size_t)n_pairs > ctx->h_pairs_cap || (size_t)n_pairs > ctx->pairs_cap)
+    return kdh_fail(ctx, KDH_ERR_INVALID, "kdh_pull_results: pinned mirror not mapped or too small");
+  hipLaunchKernelGGL(k_pull_results, dim3(n_pairs), dim3(64), 0, ctx->stream, ctx->d_pairs, ctx->h_pairs_dev, n_pairs);
+  KDH_HIP_TRY(ctx, hipGetLastError());
   return KDH_OK;
 }
 
@@ -578,9 +619,11 @@ int kdh_align(kdh_ctx* ctx, const kdh_tree* fixed, const kdh_tree* moving, int c
       hipError_t le = hipEventRecord(ctx->ev[0], ctx->stream);
       if (le == hipSuccess) le = hipGraphLaunch(exec, ctx->stream);
       if (le == hipSuccess) le = hipEventRecord(ctx->ev[1], ctx->stream);
-      if (le == hipSuccess)
-        le = hipMemcpyAsync(&ps, ctx->d_pairs, sizeof(PairState), hipMemcpyDeviceToHost, ctx->stream);
-      if (le == hipSuccess) le = hipStreamSynchronize(ctx->stream);
+      if (le == hipSuccess) {
+        rc = kdh_pull_results(ctx, 1);  // into ps, the pinned mirror's first state
+        if (rc) return rc;
+        le = hipStreamSynchronize(ctx->stream);
+      }
       if (le != hipSuccess) return kdh_hip_fail(ctx, le, "device loop graph");
       float ms = 0.0f;
       hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]);

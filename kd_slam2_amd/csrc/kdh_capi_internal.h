@@ -35,6 +35,7 @@ struct kdh_ctx {
   kdh::PairState* d_pairs = nullptr;
   size_t pairs_cap = 0;
   kdh::PairState* h_pairs = nullptr;  // pinned mirror
+  kdh::PairState* h_pairs_dev = nullptr;  // its device address (kernels write results into it)
   size_t h_pairs_cap = 0;
   float* d_partials = nullptr;
   size_t partials_cap = 0;
@@ -78,6 +79,10 @@ void kdh_tree_build_ws_free(kdh_ctx* ctx);
 int kdh_hip_fail(kdh_ctx* ctx, hipError_t e, const char* where);
 int kdh_ensure_pairs(kdh_ctx* ctx, size_t n_pairs);
 int kdh_ensure_partials(kdh_ctx* ctx, size_t n_floats);
+// Enqueues the copy of the evolving fields (X, v, iter, status, prev_score)
+// and the statistics logs of pairs [0, n) from the device states into the
+// pinned mirror; the caller synchronizes ctx->stream.
+int kdh_pull_results(kdh_ctx* ctx, int n_pairs);
 int kdh_ensure_tile_pair(kdh_ctx* ctx, size_t n);
 // Uploads the tile list (through pinned staging) unless the device already
 // holds exactly this list.

@@ -652,6 +652,10 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(DUAL ?
     const int qc = live ? q : 0;  // a tile exists, so the pair has >= 1 leaf
     const float4 la = moving.leaves[2 * qc];
     const float4 lb = moving.leaves[2 * qc + 1];
+    // the cache and term parameters in the same round trip as the leaves
+    // and the staging, ahead of the barrier
+    const TermCache C = ps.cache;
+    const TermParams TP = ps.tp;
     if (LDS_NODES > 0 && p != cached_pair) {
       __syncthreads();
       n_lds = fixed.n_nodes < LDS_NODES ? fixed.n_nodes : LDS_NODES;
@@ -668,8 +672,6 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(DUAL ?
       L.mean = v3(la.x, la.y, la.z);
       L.dir = v3(la.w, lb.x, lb.y);
       L.dts = lb.z;
-      const TermCache C = ps.cache;
-      const TermParams TP = ps.tp;
       int leaf, depth = 0;
       const float* fdts = fixed.node_dts;
       auto dts_of = [fdts](int i) { return fdts[i]; };

@@ -161,3 +161,30 @@ def test_invalid_inputs_are_errors(ctx):
     al = _aligner(ctx, ICPHIP3D, F, M, w, icp_params(max_iterations=65))
     with pytest.raises(RuntimeError, match="max_iterations"):
         al.computeOnDevice()
+
+
+def test_fresh_context_first_call_is_a_device_loop(ctx):
+    """A new context whose first call is the device-resident loop of one
+    alignment (its pinned result mirror still at the initial size), then a
+    batch: the results equal those of the shared context, and the batch's
+    pair equals the loop."""
+    from kd_slam2_amd import Context, batch_align
+
+    w = cfg1()
+    p = icp_params()
+    fresh = Context(0)
+    try:
+        F1, M1 = DeviceTree(fresh, w["fixed"]), DeviceTree(fresh, w["moving"])
+        al = _aligner(fresh, ICPHIP3D, F1, M1, w, p)
+        res = al.computeOnDevice()
+        X_fresh, log_fresh = al.getX(), al.stats_log
+        b = batch_align(fresh, [dict(fixed=F1, moving=M1, ct=0, X0=w["X0"], Xf=w["Xf"], params=p)])[0]
+        F1.close()
+        M1.close()
+    finally:
+        fresh.close()
+    F, M = DeviceTree(ctx, w["fixed"]), DeviceTree(ctx, w["moving"])
+    ref = _aligner(ctx, ICPHIP3D, F, M, w, p)
+    assert ref.computeOnDevice() == res
+    assert np.array_equal(ref.getX(), X_fresh) and ref.stats_log == log_fresh
+    assert np.array_equal(b["X"], X_fresh) and b["iterations"] == len(log_fresh)
