@@ -682,10 +682,15 @@ _I4 = np.eye(4)
 _Z6 = np.zeros(6)
 
 
+_STAT_NAMES = None
+
+
 def _stats_dicts(rec):
-    names = [f for f, _ in L.Stats._fields_ if f != "pad"]
-    cols = {f: rec[f].tolist() for f in names}
-    return [{f: cols[f][i] for f in names} for i in range(len(rec))]
+    global _STAT_NAMES
+    if _STAT_NAMES is None:
+        _STAT_NAMES = [f for f, _ in L.Stats._fields_ if f != "pad"]
+    cols = [rec[f].tolist() for f in _STAT_NAMES]
+    return [dict(zip(_STAT_NAMES, row)) for row in zip(*cols)]
 
 
 def _pair_array(pairs):
@@ -702,14 +707,15 @@ def _pair_array(pairs):
     A["X0"] = np.asarray([p["X0"] for p in pairs], dtype=np.float64)[:, :3, :4].reshape(n, 12)
     A["Xf"] = np.asarray([p["Xf"] if "Xf" in p else _I4 for p in pairs], dtype=np.float64)[:, :3, :4].reshape(n, 12)
     A["v0"] = np.asarray([p["v0"] if "v0" in p else _Z6 for p in pairs], dtype=np.float64).reshape(n, 6)
-    pd = _PAIR_DT["params"]
-    cache = {}
-    for i, p in enumerate(pairs):
-        prm = p["params"]
-        rec = cache.get(id(prm))
-        if rec is None:
-            rec = cache[id(prm)] = np.frombuffer(bytes(prm), dtype=pd)[0]
-        A["params"][i] = rec
+    # each distinct params struct converted once, then one gathered assignment
+    slot, blobs, idx = {}, [], []
+    for p in pairs:
+        k = id(p["params"])
+        if k not in slot:
+            slot[k] = len(blobs)
+            blobs.append(bytes(p["params"]))
+        idx.append(slot[k])
+    A["params"] = np.frombuffer(b"".join(blobs), dtype=_PAIR_DT["params"])[idx]
     if any(p.get("prior") is not None for p in pairs):
         A["prior"] = [C.addressof(p["prior"]) if p.get("prior") is not None else 0 for p in pairs]
     return arr
@@ -778,7 +784,12 @@ def batch_align(ctx, pairs):
     Xs[:, :3, :4] = R["X"].reshape(n, 3, 4)
     vs = R["v"].copy()
     its, rs = R["iterations"].tolist(), R["result"].tolist()
-    last, entry = _stats_dicts(R["last"]), _stats_dicts(R["entry"])
+    last = _stats_dicts(R["last"])
+    # a pair outside a forest has entry == last (kdh_pair_result.entry)
+    if np.array_equal(R["entry"], R["last"]):
+        entry = [dict(d) for d in last]
+    else:
+        entry = _stats_dicts(R["entry"])
     return [dict(X=Xs[i], v=vs[i], iterations=its[i], result=rs[i], last=last[i], entry=entry[i]) for i in range(n)]
 
 
