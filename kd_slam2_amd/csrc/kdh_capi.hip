@@ -66,7 +66,8 @@ int kdh_ctx_create(int device, kdh_ctx** out) {
     delete c;
     return KDH_ERR_HIP;
   }
-  if (hipHostMalloc((void**)&c->h_pairs, sizeof(PairState) * 1, hipHostMallocDefault) != hipSuccess) {
+  // coherent: kernels write results into it (k_pull_results) for the host
+  if (hipHostMalloc((void**)&c->h_pairs, sizeof(PairState) * 1, hipHostMallocCoherent) != hipSuccess) {
     hipStreamDestroy(c->stream);
     delete c;
     return KDH_ERR_NOMEM;
@@ -246,7 +247,8 @@ int kdh_ensure_pairs(kdh_ctx* ctx, size_t n_pairs) {
     if (ctx->h_pairs) hipHostFree(ctx->h_pairs);
     ctx->h_pairs = nullptr;
     size_t nc = std::max(n_pairs, 2 * ctx->h_pairs_cap);
-    hipError_t e = hipHostMalloc((void**)&ctx->h_pairs, nc * sizeof(PairState), hipHostMallocDefault);
+    // coherent, as at context creation: kernels write results into it
+    hipError_t e = hipHostMalloc((void**)&ctx->h_pairs, nc * sizeof(PairState), hipHostMallocCoherent);
     if (e == hipSuccess) e = hipHostGetDevicePointer((void**)&ctx->h_pairs_dev, ctx->h_pairs, 0);
     if (e != hipSuccess) {
       if (ctx->h_pairs) hipHostFree(ctx->h_pairs);
