@@ -586,10 +586,38 @@ __global__ void k_count(LevelArgs a) {
 
 // Children's slots in breadth-first order: prefix sums of the split flags
 // and of the children's chunks within each 1024-node block.
+// The next level's range and chunk count from the level's totals (S split
+// nodes, `chunks` child chunks, the largest child), and the stop and
+// hand-over decisions.  One thread.
+__device__ void alloc_tail(const LevelArgs& a, const LevelCtl& L, int S, int chunks, int largest) {
+  const bool ok = L.le + 2 * S <= a.cap_nodes;
+  if (!ok) a.st->overflow = 1;
+  // every child small enough: the subtree phase builds the rest
+  const bool handoff = S > 0 && ok && largest <= kSubLimit;
+  a.cbeg_next[2 * S] = chunks;
+  *a.ctl_next = LevelCtl{L.le, L.le + 2 * S, chunks, (S > 0 && ok && !handoff) ? 1 : 0};
+  if (a.level + 2 < kMaxLevels + 2) a.level_begin[a.level + 2] = L.le + 2 * S;
+  if (handoff) {
+    a.st->handoff = 1;
+    a.st->handoff_level = a.level + 1;
+    a.st->handoff_lb = L.le;
+    a.st->handoff_n = 2 * S;
+  }
+  if (S == 0 || !ok || handoff) a.st->last_level = a.level + 1;
+}
+
+// Block-local prefix sums of the split flags and child chunk counts.  SINGLE
+// (a level of at most kScanThreads nodes, one block): the block offsets are
+// zero and this block also finishes the level (alloc_tail), which otherwise
+// k_alloc2 does over the block totals.
+template <bool SINGLE>
 __global__ __launch_bounds__(kScanThreads) void k_alloc(LevelArgs a) {
   __shared__ int s_scan[16];
   const LevelCtl L = *a.ctl;
-  if (!L.active) return;
+  if (!L.active) {
+    if (SINGLE && threadIdx.x == 0) *a.ctl_next = LevelCtl{L.le, L.le, 0, 0};
+    return;
+  }
   const int w = L.le - L.lb;
   const int j = blockIdx.x * kScanThreads + threadIdx.x;
   __shared__ int s_max;
@@ -615,14 +643,20 @@ __global__ __launch_bounds__(kScanThreads) void k_alloc(LevelArgs a) {
     nd.ex_chunks = ec;
   }
   if (threadIdx.x == 0) {
-    a.blk_a[blockIdx.x] = tot_s;
-    a.blk_b[blockIdx.x] = tot_c;
-    a.blk_c[blockIdx.x] = s_max;
+    if (SINGLE) {
+      a.blk_a[0] = 0;
+      a.blk_b[0] = 0;
+      alloc_tail(a, L, tot_s, tot_c, s_max);
+    } else {
+      a.blk_a[blockIdx.x] = tot_s;
+      a.blk_b[blockIdx.x] = tot_c;
+      a.blk_c[blockIdx.x] = s_max;
+    }
   }
 }
 
-// One workgroup: block totals -> exclusive block offsets (in place), and the
-// next level's range and chunk count.
+// One workgroup: block totals -> exclusive block offsets (in place), then
+// alloc_tail.
 __global__ __launch_bounds__(kScanThreads) void k_alloc2(LevelArgs a, int nb) {
   __shared__ int s_scan[16];
   const LevelCtl L = *a.ctl;
@@ -648,23 +682,7 @@ __global__ __launch_bounds__(kScanThreads) void k_alloc2(LevelArgs a, int nb) {
     carry_s += ts;
     carry_c += tc;
   }
-  if (threadIdx.x == 0) {
-    const int S = carry_s;
-    const bool ok = L.le + 2 * S <= a.cap_nodes;
-    if (!ok) a.st->overflow = 1;
-    // every child small enough: the subtree phase builds the rest
-    const bool handoff = S > 0 && ok && s_max <= kSubLimit;
-    a.cbeg_next[2 * S] = carry_c;
-    *a.ctl_next = LevelCtl{L.le, L.le + 2 * S, carry_c, (S > 0 && ok && !handoff) ? 1 : 0};
-    if (a.level + 2 < kMaxLevels + 2) a.level_begin[a.level + 2] = L.le + 2 * S;
-    if (handoff) {
-      a.st->handoff = 1;
-      a.st->handoff_level = a.level + 1;
-      a.st->handoff_lb = L.le;
-      a.st->handoff_n = 2 * S;
-    }
-    if (S == 0 || !ok || handoff) a.st->last_level = a.level + 1;
-  }
+  if (threadIdx.x == 0) alloc_tail(a, L, carry_s, carry_c, s_max);
 }
 
 // Blocks [0, chunk_blocks): one wave per chunk, the stable partition into
@@ -1453,8 +1471,12 @@ int kdh_tree_build(kdh_ctx* ctx, const float* xyz, const double* stamps, int64_t
     if (grouped) hipLaunchKernelGGL(k_gsum<6>, g_grp, dim3(1024), 0, s, a);
     hipLaunchKernelGGL(k_red2, dim3(gw), dim3(256), 0, s, a, P, grouped);
     hipLaunchKernelGGL(k_count, dim3(gc), dim3(256), 0, s, a);
-    hipLaunchKernelGGL(k_alloc, dim3(ga), dim3(kScanThreads), 0, s, a);
-    hipLaunchKernelGGL(k_alloc2, dim3(1), dim3(kScanThreads), 0, s, a, ga);
+    if (ga == 1) {
+      hipLaunchKernelGGL(k_alloc<true>, dim3(1), dim3(kScanThreads), 0, s, a);
+    } else {
+      hipLaunchKernelGGL(k_alloc<false>, dim3(ga), dim3(kScanThreads), 0, s, a);
+      hipLaunchKernelGGL(k_alloc2, dim3(1), dim3(kScanThreads), 0, s, a, ga);
+    }
     hipLaunchKernelGGL(k_split, dim3(gc + gn), dim3(256), 0, s, a, gc);
     return hipGetLastError();
   };
