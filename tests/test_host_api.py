@@ -42,3 +42,37 @@ def test_fixed_entries_keep_tree_and_pose():
     e = al.makeFixedEntry("tree", X)
     al._fixed_forest[3] = e
     assert e.fixed_tree == "tree" and np.array_equal(e.fixed_X, X) and e.stats is None
+
+
+def test_pair_array_marshals_every_pair_field():
+    """kdh_pair structs filled from the Python pair dicts (numpy views of the
+    C array): handles, kinds, forests, poses, velocities and each pair's own
+    parameter struct, with some parameter objects shared between pairs."""
+    import ctypes as C
+
+    from kd_slam2_amd import api, default_params
+
+    class _Tree:
+        def __init__(self, h):
+            self.h = C.c_void_p(h)
+
+    rng = np.random.default_rng(3)
+    ps = [default_params() for _ in range(3)]
+    for k, p in enumerate(ps):
+        p.max_iterations = 5 + k
+        p.damping = 0.5 * (k + 1)
+    pairs = []
+    for i in range(7):
+        X0 = synth.expmap(rng.normal(size=6))
+        Xf = synth.expmap(rng.normal(size=6))
+        pairs.append(dict(fixed=_Tree(1000 + i), moving=_Tree(2000 + i), ct=i % 2, forest=1 if i == 3 else 0,
+                          X0=X0, Xf=Xf, v0=rng.normal(size=6), params=ps[i % 3]))
+    arr = api._pair_array(pairs)
+    for i, (src, dst) in enumerate(zip(pairs, arr)):
+        assert dst.fixed == src["fixed"].h.value and dst.moving == src["moving"].h.value
+        assert dst.ct == src["ct"] and dst.forest == src["forest"]
+        assert np.array_equal(np.array(dst.X0), src["X0"][:3, :4].reshape(12))
+        assert np.array_equal(np.array(dst.Xf), src["Xf"][:3, :4].reshape(12))
+        assert np.array_equal(np.array(dst.v0), src["v0"])
+        assert bytes(dst.params) == bytes(src["params"]), i
+        assert not dst.prior
