@@ -689,8 +689,8 @@ def _stats_dicts(rec):
     global _STAT_NAMES
     if _STAT_NAMES is None:
         _STAT_NAMES = [f for f, _ in L.Stats._fields_ if f != "pad"]
-    cols = [rec[f].tolist() for f in _STAT_NAMES]
-    return [dict(zip(_STAT_NAMES, row)) for row in zip(*cols)]
+    names = _STAT_NAMES
+    return [dict(zip(names, row)) for row in zip(*[rec[f].tolist() for f in names])]
 
 
 def _pair_array(pairs):
@@ -786,7 +786,7 @@ def batch_align(ctx, pairs):
     its, rs = R["iterations"].tolist(), R["result"].tolist()
     last = _stats_dicts(R["last"])
     # a pair outside a forest has entry == last (kdh_pair_result.entry)
-    if np.array_equal(R["entry"], R["last"]):
+    if R["entry"].tobytes() == R["last"].tobytes():
         entry = [dict(d) for d in last]
     else:
         entry = _stats_dicts(R["entry"])
