@@ -293,10 +293,11 @@ int kdh_batch_build_dual(kdh_ctx* ctx, int n_factors, const kdh_dual_pair* facto
                          kdh_dual_form* out);
 
 /* ---- serving: continuous batching ----------------------------------------
- * An engine thread runs one Gauss-Newton iteration of every in-flight
- * alignment per step (one tile + one finish launch per aligner kind), retires
- * finished alignments and admits queued requests between steps while slots
- * and the HBM budget allow.  Results are bit-identical to running the request
+ * An engine thread runs steps of Gauss-Newton iterations over every in-flight
+ * alignment (per iteration one tile + one finish launch per aligner kind;
+ * four iterations per step while no request waits, one while one does),
+ * retires finished alignments and admits queued requests between steps while
+ * slots and the HBM budget allow.  Results are bit-identical to running the request
  * alone.  Maps are uploaded by the caller (kdh_tree_upload on a context of the
  * same device) and declared resident; moving trees travel with the request. */
 typedef struct kdh_request {
@@ -328,7 +329,7 @@ typedef struct kdh_server_info {
   int64_t bytes_inflight;
   int32_t inflight;
   int32_t queued;
-  int64_t steps;
+  int64_t steps;                    /* engine steps (one to four iterations each) */
   int64_t completed;
 } kdh_server_info;
 

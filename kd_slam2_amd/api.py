@@ -799,8 +799,9 @@ def batch_align(ctx, pairs):
 
 class Server:
     """Continuous-batching alignment server (kdh_server_*): an engine thread
-    runs one Gauss-Newton iteration of every in-flight request per step and
-    admits queued requests between steps while slots and the HBM budget allow."""
+    runs steps of Gauss-Newton iterations (four, or one while a request
+    waits) over every in-flight request and admits queued requests between
+    steps while slots and the HBM budget allow."""
 
     def __init__(self, device=0, budget_bytes=64 << 30, max_inflight=64):
         h = C.c_void_p()

@@ -1,12 +1,13 @@
 // Serving engine: continuous batching of alignment requests on one GPU.
 //
 // Requests (odometry scan-to-map, loop-closure verification, ...) arrive at
-// any time from any thread.  An engine thread runs "steps": each step is one
-// Gauss-Newton iteration for every in-flight alignment, as one tile launch
-// plus one finish launch per aligner kind over the concatenated tile list.
-// Between steps the engine retires finished alignments (their results become
-// visible to waiters) and admits queued requests while slots and the HBM
-// budget allow.  A request admitted while others are mid-way joins at the
+// any time from any thread.  An engine thread runs "steps": each step is
+// kStepIters Gauss-Newton iterations (one while a request waits) for every
+// in-flight alignment, each iteration one tile launch plus one finish launch
+// per aligner kind over the concatenated tile list, and one host
+// synchronization.  Between steps the engine retires finished alignments
+// (their results become visible to waiters) and admits queued requests while
+// slots and the HBM budget allow.  A request admitted while others are mid-way joins at the
 // next step, so arrivals overlap in-flight work instead of waiting for a
 // batch boundary.
 //
@@ -47,6 +48,11 @@ namespace {
 using Clock = std::chrono::steady_clock;
 
 constexpr int kEmpty = 9;  // status of an unused slot (finish and tile kernels skip it)
+// Gauss-Newton iterations per engine step while no request waits for
+// admission (a waiting request gets a one-iteration step, so it joins at
+// the next iteration boundary); alignments that finish inside a step skip
+// the step's remaining launches.
+constexpr int kStepIters = 4;
 
 size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
@@ -93,6 +99,10 @@ struct kdh_server {
   size_t partials_cap = 0;
   int* d_tile_pair = nullptr;
   size_t tile_cap = 0;
+  // the tile list and per-slot first tiles on the device, to skip the upload
+  // while the active set is unchanged
+  std::vector<int> dev_tp, dev_tile_first;
+  const int* dev_tp_ptr = nullptr;
 };
 
 static int64_t request_bytes(const kdh_request& r) {
@@ -222,12 +232,15 @@ static void engine(kdh_server* S) {
         S->cv_done.notify_all();
       }
     }
-    // one Gauss-Newton step for every in-flight alignment
+    // one engine step: kStepIters Gauss-Newton iterations (one while a
+    // request waits) for every in-flight alignment
     std::vector<int> active;
+    int iters = 1;
     {
       std::lock_guard<std::mutex> lk(S->mu);
       for (int i = 0; i < M; ++i)
         if (S->slots[i]) active.push_back(i);
+      if (S->queue.empty()) iters = kStepIters;
     }
     if (active.empty()) continue;
     int n_t[2] = {0, 0};
@@ -250,10 +263,17 @@ static void engine(kdh_server* S) {
     tp.insert(tp.end(), lists[0].begin(), lists[0].end());
     tp.insert(tp.end(), lists[1].begin(), lists[1].end());
     if (ok && tiles_all) {
-      ok = hipMemcpyAsync(S->d_tile_pair, tp.data(), tiles_all * sizeof(int), hipMemcpyHostToDevice,
-                          ctx->stream) == hipSuccess;
-      ok = ok && hipMemcpyAsync(S->d_tile_first, S->h_tile_first.data(), M * sizeof(int), hipMemcpyHostToDevice,
-                                ctx->stream) == hipSuccess;
+      if (tp != S->dev_tp || S->h_tile_first != S->dev_tile_first || S->dev_tp_ptr != S->d_tile_pair) {
+        ok = hipMemcpyAsync(S->d_tile_pair, tp.data(), tiles_all * sizeof(int), hipMemcpyHostToDevice,
+                            ctx->stream) == hipSuccess;
+        ok = ok && hipMemcpyAsync(S->d_tile_first, S->h_tile_first.data(), M * sizeof(int), hipMemcpyHostToDevice,
+                                  ctx->stream) == hipSuccess;
+        // what the device now holds
+        S->dev_tp = tp;
+        S->dev_tile_first = S->h_tile_first;
+        S->dev_tp_ptr = ok ? S->d_tile_pair : nullptr;
+      }
+      for (int it = 0; it < iters && ok; ++it) {
       size_t poff = 0, toff = 0;
       for (int k = 0; k < 2 && ok; ++k) {
         if (!n_t[k]) continue;
@@ -274,6 +294,7 @@ static void engine(kdh_server* S) {
         ok = launch_tiles(a, ctx->stream) == hipSuccess;
         toff += n_t[k];
         poff += (size_t)n_t[k] * slots(k);
+      }
       }
     }
     if (ok)
