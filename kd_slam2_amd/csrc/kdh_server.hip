@@ -305,11 +305,14 @@ static void engine(kdh_server* S) {
     std::vector<int> done;
     for (int slot : active)
       if (!ok || S->h_status[slot] != 0 || h_pairs[slot].status != 0) done.push_back(slot);
+    // one pull of every slot's results into the pinned mirror, one wait
+    hipError_t e = hipSuccess;
+    if (!done.empty()) {
+      e = kdh_pull_results(ctx, M) == KDH_OK ? hipSuccess : hipErrorUnknown;
+      if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    }
     for (int slot : done) {
       PairState& ps = h_pairs[slot];
-      hipError_t e = hipMemcpyAsync(&ps, &ctx->d_pairs[slot], sizeof(PairState), hipMemcpyDeviceToHost,
-                                    ctx->stream);
-      if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
       std::lock_guard<std::mutex> lk(S->mu);
       auto r = S->slots[slot];
       if (!r) continue;
