@@ -212,13 +212,14 @@ static void engine(kdh_server* S) {
           ps.status = ps.sp.min_good_measurements > 0 ? 2 : 1;
           memset(&ps.stats_log[0], 0, sizeof(DevStats));
         }
-        int zero = 0;
+        // from pinned memory, ordered before the step's launches on the same
+        // stream: no wait here (a failed copy fails the step)
+        S->h_status[r->slot] = ps.status;
         hipError_t e = hipMemcpyAsync(&ctx->d_pairs[r->slot], &ps, sizeof(PairState), hipMemcpyHostToDevice,
                                       ctx->stream);
         if (e == hipSuccess)
-          e = hipMemcpyAsync(&S->d_status[r->slot], ps.status ? &ps.status : &zero, sizeof(int),
-                             hipMemcpyHostToDevice, ctx->stream);
-        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+          e = hipMemcpyAsync(&S->d_status[r->slot], &S->h_status[r->slot], sizeof(int), hipMemcpyHostToDevice,
+                             ctx->stream);
         if (e != hipSuccess) rc = KDH_ERR_HIP;
       }
       if (rc != KDH_OK) {
