@@ -238,6 +238,7 @@ def main():
     ap.add_argument("--no-pmc", action="store_true")
     ap.add_argument("--no-precision", action="store_true", help="skip the 10 km precision-stress workload")
     ap.add_argument("--no-tree-build", action="store_true", help="skip the tree-generation timings")
+    ap.add_argument("--no-next-rows", action="store_true", help="skip tree maintenance, factor forms, KDTree3f")
     ap.add_argument("--tree-build-only", action="store_true",
                     help="only the tree-generation timings (for kernel traces of the builder)")
     ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
@@ -427,6 +428,12 @@ def main():
         tree_build = run_tree_build(ctx, w, sweep5, DeviceTree, F, p)
     w.pop("map", None)
 
+    # ---- the other rows around the path: tree maintenance, batched factor
+    # forms, the KDTree3f wire and bag format -----------------------------
+    next_rows = None
+    if not args.no_next_rows:
+        next_rows = run_next_rows(ctx, w, DeviceTree, F, M, p)
+
     # ---- CPU baseline -------------------------------------------------------
     cpu = None
     if not args.no_cpu:
@@ -498,6 +505,7 @@ def main():
         loop_closure_batch=batch3,
         precision_stress=precision,
         tree_build=tree_build,
+        next_rows=next_rows,
         roofline=roofline,
         cpu_baseline=cpu,
     )
@@ -546,6 +554,100 @@ def run_tree_build(ctx, w, sweep5, DeviceTree, fixed_tree=None, params=None):
             f"{out[name]['host_1thread_ms']:.1f} ms, 16 threads {out[name]['host_16thread_ms']:.1f} ms")
     if fixed_tree is not None:
         out["odometry_from_points"] = run_points_to_pose(ctx, w, fixed_tree, params, g, g16, DeviceTree)
+    return out
+
+
+def _median_us(fn, reps):
+    fn()
+    ts = []
+    for _ in range(reps):
+        t = time.perf_counter()
+        fn()
+        ts.append(time.perf_counter() - t)
+    return 1e6 * float(np.median(ts))
+
+
+def run_next_rows(ctx, w, DeviceTree, F, M, params):
+    """Each row beside the alignment path, measured where it runs:
+    - tree maintenance (CT deskew of a whole tree, isometry, compressed-leaf
+      extraction) on copies of the 2M-point map and the 131k sweep trees:
+      wall time per call (launch, kernel, wait; extraction includes the
+      download) and the node-record bytes it moves against HBM peak;
+    - quadratic forms of bundle-adjustment factors: 32 CT factors (sweep vs
+      map at perturbed poses) and 32 CT dual factors (sweep vs sweep) in one
+      batched launch against one plain per-call build each;
+    - the KDTree3f codec and MCAP file I/O on the map tree's message."""
+    from kd_slam2_amd import batch_build, batch_build_dual, kdtree_msg, synth
+    from kd_slam2_amd.api import adapter_build_dual
+
+    out = {}
+    v = np.array([2.0, 0.1, 0.0, 0.01, 0.02, 0.5])
+    X = synth.expmap(np.array([0.5, -0.2, 0.1, 0.01, 0.02, 0.3]))
+    maint = {}
+    for name, host in (("map_2M", w["fixed"]), ("sweep_131k", w["moving"])):
+        T = DeviceTree(ctx, host)
+        nn, nl = len(host.nodes), len(host.leaves)
+        rec = 32 * (nn + nl)  # node and leaf records, read and written
+        r = dict(nodes=nn, leaves=nl)
+        for op, fn, bytes_moved in (
+                ("apply_velocities", lambda: T.applyVelocitiesInPlace(v), 2 * rec + 4 * nn),
+                ("apply_isometry", lambda: T.applyIsometryInPlace(X), 2 * rec),
+                ("extract_leaves", lambda: T.extractCompressedLeaves(v), 32 * nl + 24 * nl)):
+            us = _median_us(fn, 30)
+            r[op + "_us"] = round(us, 1)
+            r[op + "_GBps"] = round(bytes_moved / (us * 1e-6) / 1e9, 1)
+        T.close()
+        maint[name] = r
+        log(f"tree maintenance {name}: " + ", ".join(f"{k} {r[k]}" for k in r if k.endswith("_us")))
+    out["tree_maintenance"] = maint
+
+    rng = np.random.default_rng(np.random.PCG64(5))
+    n_f = 32
+    pairs = [dict(fixed=F, moving=M, ct=1, X0=synth.perturb(w["X_gt"], rng, 0.5, 0.1), Xf=w["Xf"], v0=w["v0"],
+                  params=params) for _ in range(n_f)]
+    batch_us = _median_us(lambda: batch_build(ctx, pairs), 10)
+
+    def one_by_one():
+        for p_ in pairs:
+            batch_build(ctx, [p_])
+    plain_us = _median_us(one_by_one, 3)
+    # factors between the stamped sweep and itself at a perturbed pose: the
+    # shape of a bundle-adjustment factor between consecutive sweeps
+    facs = [dict(fixed=M, moving=M, X_moving=synth.perturb(w["X_gt"], rng, 0.3, 0.05), v_moving=w["v0"],
+                 X_fixed=w["X_gt"], v_fixed=w["v0"], params=params) for _ in range(n_f)]
+    dual_us = _median_us(lambda: batch_build_dual(ctx, facs), 10)
+
+    def dual_one_by_one():
+        for f_ in facs:
+            adapter_build_dual(ctx, f_["fixed"], f_["moving"], f_["X_moving"], f_["v_moving"], f_["X_fixed"],
+                               f_["v_fixed"], params)
+    dual_plain_us = _median_us(dual_one_by_one, 3)
+    out["factor_forms"] = dict(
+        factors=n_f, ct_batched_ms=round(batch_us / 1e3, 3), ct_one_by_one_ms=round(plain_us / 1e3, 3),
+        ct_speedup=round(plain_us / batch_us, 2), dual_batched_ms=round(dual_us / 1e3, 3),
+        dual_one_by_one_ms=round(dual_plain_us / 1e3, 3), dual_speedup=round(dual_plain_us / dual_us, 2))
+    log(f"factor forms: {n_f} CT {batch_us / 1e3:.2f} ms batched vs {plain_us / 1e3:.2f} ms one by one; dual "
+        f"{dual_us / 1e3:.2f} ms vs {dual_plain_us / 1e3:.2f} ms")
+
+    msg = kdtree_msg.message_from_tree(w["fixed"], frame_id="map", src_topic="/tree")
+    data = kdtree_msg.encode(msg)
+    enc_us = _median_us(lambda: kdtree_msg.encode(msg), 10)
+    dec_us = _median_us(lambda: kdtree_msg.decode(data), 10)
+    tmp = tempfile.mkdtemp()
+    try:
+        path = os.path.join(tmp, "trees.mcap")
+        recs = [("/tree", 1_000_000_000 + k, data) for k in range(8)]
+        wr_us = _median_us(lambda: kdtree_msg.write_mcap(path, recs), 3)
+        rd_us = _median_us(lambda: kdtree_msg.read_mcap(path), 3)
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+    mb = len(data) / 1e6
+    out["kdtree3f"] = dict(message_MB=round(mb, 2), nodes=len(msg.nodes), encode_ms=round(enc_us / 1e3, 2),
+                           decode_ms=round(dec_us / 1e3, 2), encode_MBps=round(mb / (enc_us * 1e-6), 1),
+                           decode_MBps=round(mb / (dec_us * 1e-6), 1), mcap_write_8_ms=round(wr_us / 1e3, 2),
+                           mcap_read_8_ms=round(rd_us / 1e3, 2))
+    log(f"kdtree3f: {mb:.1f} MB message, encode {enc_us / 1e3:.2f} ms, decode {dec_us / 1e3:.2f} ms, MCAP 8 "
+        f"messages write {wr_us / 1e3:.1f} ms, read {rd_us / 1e3:.1f} ms")
     return out
 
 

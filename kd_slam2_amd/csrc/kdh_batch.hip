@@ -280,8 +280,8 @@ int kdh_batch_build(kdh_ctx* ctx, int n_pairs, const kdh_pair* pairs, int stats_
                     ctx->d_partials + g[k].partial_off, n_pairs, 3);
     if (rc) return rc;
   }
-  KDH_HIP_TRY(ctx, hipMemcpyAsync(ctx->h_pairs, ctx->d_pairs, sizeof(PairState) * n_pairs, hipMemcpyDeviceToHost,
-                                  ctx->stream));
+  rc = kdh_pull_forms(ctx, n_pairs, 0);
+  if (rc) return rc;
   KDH_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
   for (int i = 0; i < n_pairs; ++i) {
     const PairState& L = ctx->h_pairs[leader[i]];
@@ -336,8 +336,8 @@ int kdh_batch_build_dual(kdh_ctx* ctx, int n_factors, const kdh_dual_pair* facto
                     ctx->d_partials + g[1].partial_off, n_factors, 3, 1);
     if (rc) return rc;
   }
-  KDH_HIP_TRY(ctx, hipMemcpyAsync(ctx->h_pairs, ctx->d_pairs, sizeof(PairState) * n_factors, hipMemcpyDeviceToHost,
-                                  ctx->stream));
+  rc = kdh_pull_forms(ctx, n_factors, 1);
+  if (rc) return rc;
   KDH_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
   for (int i = 0; i < n_factors; ++i) {
     const PairState& ps = ctx->h_pairs[i];

@@ -293,6 +293,33 @@ int kdh_pull_results(kdh_ctx* ctx, int n_pairs) {
   return KDH_OK;
 }
 
+// Build mode's outputs (H, b, the dual blocks when asked, the statistics
+// record) written into the pinned mirror the same way.
+__global__ void k_pull_forms(const PairState* __restrict__ d, PairState* h, int n, int dual) {
+  const int p = blockIdx.x, t = threadIdx.x;
+  if (p >= n) return;
+  const PairState& s = d[p];
+  PairState& o = h[p];
+  for (int i = t; i < 144; i += 64) o.H[i] = s.H[i];
+  if (t < 12) o.b[t] = s.b[t];
+  if (dual)
+    for (int i = t; i < 114; i += 64) o.Hd[i] = s.Hd[i];
+  const double* src = reinterpret_cast<const double*>(&s.stats_log[0]);
+  double* dst = reinterpret_cast<double*>(&o.stats_log[0]);
+  for (int i = t; i < (int)(sizeof(DevStats) / 8); i += 64) dst[i] = src[i];
+  __threadfence_system();
+}
+
+int kdh_pull_forms(kdh_ctx* ctx, int n_pairs, int dual) {
+  if (n_pairs <= 0) return KDH_OK;
+  if (!ctx->h_pairs_dev || (size_t)n_pairs > ctx->h_pairs_cap || (size_t)n_pairs > ctx->pairs_cap)
+    return kdh_fail(ctx, KDH_ERR_INVALID, "kdh_pull_forms: pinned mirror not mapped or too small");
+  hipLaunchKernelGGL(k_pull_forms, dim3(n_pairs), dim3(64), 0, ctx->stream, ctx->d_pairs, ctx->h_pairs_dev, n_pairs,
+                     dual);
+  KDH_HIP_TRY(ctx, hipGetLastError());
+  return KDH_OK;
+}
+
 int kdh_ensure_partials(kdh_ctx* ctx, size_t n_floats) {
   KDH_HIP_TRY(ctx, grow(&ctx->d_partials, &ctx->partials_cap, n_floats));
   return KDH_OK;

@@ -83,6 +83,8 @@ int kdh_ensure_partials(kdh_ctx* ctx, size_t n_floats);
 // and the statistics logs of pairs [0, n) from the device states into the
 // pinned mirror; the caller synchronizes ctx->stream.
 int kdh_pull_results(kdh_ctx* ctx, int n_pairs);
+// H, b (and the dual blocks when dual), stats_log[0] of build mode, the same way.
+int kdh_pull_forms(kdh_ctx* ctx, int n_pairs, int dual);
 int kdh_ensure_tile_pair(kdh_ctx* ctx, size_t n);
 // Uploads the tile list (through pinned staging) unless the device already
 // holds exactly this list.

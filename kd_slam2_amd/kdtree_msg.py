@@ -57,7 +57,7 @@ def encode(msg):
     n = C.c_size_t()
     L.check(L.lib().kdh_kdtree3f_encode(C.byref(info), msg.nodes.ctypes.data_as(C.c_void_p), len(msg.nodes), buf,
                                         size.value, C.byref(n)), what="kdh_kdtree3f_encode")
-    return bytes(buf[:n.value])
+    return C.string_at(buf, n.value)
 
 
 def decode(data):
