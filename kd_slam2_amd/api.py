@@ -678,6 +678,8 @@ def adapter_build_dual(ctx, fixed, moving, X_moving, v_moving, X_fixed, v_fixed,
 
 _PAIR_DT = None
 _RES_DT = None
+_DUAL_DT = None
+_FORM_DT = None
 _I4 = np.eye(4)
 _Z6 = np.zeros(6)
 
@@ -691,6 +693,19 @@ def _stats_dicts(rec):
         _STAT_NAMES = [f for f, _ in L.Stats._fields_ if f != "pad"]
     names = _STAT_NAMES
     return [dict(zip(names, row)) for row in zip(*[rec[f].tolist() for f in names])]
+
+
+def _params_column(items, dt):
+    """The params structs of items as one array: each distinct struct
+    converted once, then gathered."""
+    slot, blobs, idx = {}, [], []
+    for p in items:
+        k = id(p["params"])
+        if k not in slot:
+            slot[k] = len(blobs)
+            blobs.append(bytes(p["params"]))
+        idx.append(slot[k])
+    return np.frombuffer(b"".join(blobs), dtype=dt)[idx]
 
 
 def _pair_array(pairs):
@@ -707,15 +722,7 @@ def _pair_array(pairs):
     A["X0"] = np.asarray([p["X0"] for p in pairs], dtype=np.float64)[:, :3, :4].reshape(n, 12)
     A["Xf"] = np.asarray([p["Xf"] if "Xf" in p else _I4 for p in pairs], dtype=np.float64)[:, :3, :4].reshape(n, 12)
     A["v0"] = np.asarray([p["v0"] if "v0" in p else _Z6 for p in pairs], dtype=np.float64).reshape(n, 6)
-    # each distinct params struct converted once, then one gathered assignment
-    slot, blobs, idx = {}, [], []
-    for p in pairs:
-        k = id(p["params"])
-        if k not in slot:
-            slot[k] = len(blobs)
-            blobs.append(bytes(p["params"]))
-        idx.append(slot[k])
-    A["params"] = np.frombuffer(b"".join(blobs), dtype=_PAIR_DT["params"])[idx]
+    A["params"] = _params_column(pairs, _PAIR_DT["params"])
     if any(p.get("prior") is not None for p in pairs):
         A["prior"] = [C.addressof(p["prior"]) if p.get("prior") is not None else 0 for p in pairs]
     return arr
@@ -743,28 +750,30 @@ def batch_build_dual(ctx, factors, stats_mode=False):
     X_moving, v_moving, X_fixed, v_fixed, params.  Returns dicts with H
     (12x12), b, H_dual_vel_B (6x6), b_dual_vel_B, H_dual_cross_pose_vel_B
     (6x6, rows: pose), H_dual_cross_vel_AB (6x6, rows: velocities A), stats."""
+    global _DUAL_DT, _FORM_DT
+    if _DUAL_DT is None:
+        _DUAL_DT, _FORM_DT = L.np_dtype(L.DualPair), L.np_dtype(L.DualForm)
     n = len(factors)
+    if n == 0:
+        return []
     arr = (L.DualPair * n)()
-    for i, f in enumerate(factors):
-        d = arr[i]
-        d.fixed = f["fixed"].h.value
-        d.moving = f["moving"].h.value
-        d.X_moving[:] = np.asarray(f["X_moving"], dtype=np.float64)[:3, :4].reshape(12).tolist()
-        d.v_moving[:] = np.asarray(f.get("v_moving", _Z6), dtype=np.float64).reshape(6).tolist()
-        d.X_fixed[:] = np.asarray(f.get("X_fixed", _I4), dtype=np.float64)[:3, :4].reshape(12).tolist()
-        d.v_fixed[:] = np.asarray(f.get("v_fixed", _Z6), dtype=np.float64).reshape(6).tolist()
-        d.params = f["params"]
+    A = np.frombuffer(arr, dtype=_DUAL_DT)
+    A["fixed"] = [f["fixed"].h.value for f in factors]
+    A["moving"] = [f["moving"].h.value for f in factors]
+    A["X_moving"] = np.asarray([f["X_moving"] for f in factors], dtype=np.float64)[:, :3, :4].reshape(n, 12)
+    A["v_moving"] = np.asarray([f.get("v_moving", _Z6) for f in factors], dtype=np.float64).reshape(n, 6)
+    A["X_fixed"] = np.asarray([f.get("X_fixed", _I4) for f in factors], dtype=np.float64)[:, :3, :4].reshape(n, 12)
+    A["v_fixed"] = np.asarray([f.get("v_fixed", _Z6) for f in factors], dtype=np.float64).reshape(n, 6)
+    A["params"] = _params_column(factors, _DUAL_DT["params"])
     out = (L.DualForm * n)()
     ctx.check(L.lib().kdh_batch_build_dual(ctx.h, n, arr, int(stats_mode), out), "kdh_batch_build_dual")
-    res = []
-    for i in range(n):
-        o = out[i]
-        g = lambda name, shape: np.array(getattr(o, name), dtype=np.float64).reshape(shape)  # noqa: E731
-        res.append(dict(H=g("H", (12, 12)), b=g("b", (12,)), H_dual_vel_B=g("H_dual_vel_B", (6, 6)),
-                        b_dual_vel_B=g("b_dual_vel_B", (6,)),
-                        H_dual_cross_pose_vel_B=g("H_dual_cross_pose_vel_B", (6, 6)),
-                        H_dual_cross_vel_AB=g("H_dual_cross_vel_AB", (6, 6)), stats=o.stats.as_dict()))
-    return res
+    R = np.frombuffer(out, dtype=_FORM_DT).copy()
+    H = R["H"].reshape(n, 12, 12)
+    Hv, Hcp, Hca = (R[k].reshape(n, 6, 6) for k in ("H_dual_vel_B", "H_dual_cross_pose_vel_B",
+                                                      "H_dual_cross_vel_AB"))
+    stats = _stats_dicts(R["stats"])
+    return [dict(H=H[i], b=R["b"][i], H_dual_vel_B=Hv[i], b_dual_vel_B=R["b_dual_vel_B"][i],
+                 H_dual_cross_pose_vel_B=Hcp[i], H_dual_cross_vel_AB=Hca[i], stats=stats[i]) for i in range(n)]
 
 
 def batch_align(ctx, pairs):

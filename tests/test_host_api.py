@@ -76,3 +76,63 @@ def test_pair_array_marshals_every_pair_field():
         assert np.array_equal(np.array(dst.v0), src["v0"])
         assert bytes(dst.params) == bytes(src["params"]), i
         assert not dst.prior
+
+
+def test_dual_factor_marshalling_round_trip(monkeypatch):
+    """batch_build_dual fills every kdh_dual_pair field from the factor dicts
+    and returns every kdh_dual_form block and statistic of each factor: the
+    library call is replaced by one that checks the structs and writes a
+    distinct value into each output."""
+    import ctypes as C
+
+    from kd_slam2_amd import _lib as L
+    from kd_slam2_amd import api, default_params
+
+    class _Tree:
+        def __init__(self, h):
+            self.h = C.c_void_p(h)
+
+    class _Ctx:
+        h = C.c_void_p(1)
+
+        def check(self, rc, what):
+            assert rc == 0, what
+
+    rng = np.random.default_rng(5)
+    ps = [default_params() for _ in range(2)]
+    ps[1].max_iterations = 3
+    facs = [dict(fixed=_Tree(100 + i), moving=_Tree(200 + i), X_moving=synth.expmap(rng.normal(size=6)),
+                 v_moving=rng.normal(size=6), X_fixed=synth.expmap(rng.normal(size=6)), v_fixed=rng.normal(size=6),
+                 params=ps[i % 2]) for i in range(5)]
+
+    class _Lib:
+        @staticmethod
+        def kdh_batch_build_dual(h, n, arr, stats_mode, out):
+            pairs = C.cast(arr, C.POINTER(L.DualPair))
+            forms = C.cast(out, C.POINTER(L.DualForm))
+            for i in range(n):
+                d, f = pairs[i], facs[i]
+                assert d.fixed == f["fixed"].h.value and d.moving == f["moving"].h.value
+                for name in ("X_moving", "X_fixed"):
+                    assert np.array_equal(np.array(getattr(d, name)), f[name][:3, :4].reshape(12)), (i, name)
+                for name in ("v_moving", "v_fixed"):
+                    assert np.array_equal(np.array(getattr(d, name)), f[name]), (i, name)
+                assert bytes(d.params) == bytes(f["params"]), i
+                o = forms[i]
+                for k, (name, ct) in enumerate(L.DualForm._fields_[:-1]):
+                    getattr(o, name)[:] = (1000.0 * i + 100.0 * k + np.arange(ct._length_)).tolist()
+                o.stats.num_inliers = 7 + i
+                o.stats.chi2 = 0.5 * i
+            return 0
+
+    monkeypatch.setattr(L, "lib", lambda: _Lib)
+    got = api.batch_build_dual(_Ctx(), facs, stats_mode=True)
+    shapes = dict(H=(12, 12), b=(12,), H_dual_vel_B=(6, 6), b_dual_vel_B=(6,), H_dual_cross_pose_vel_B=(6, 6),
+                  H_dual_cross_vel_AB=(6, 6))
+    for i, g in enumerate(got):
+        for k, (name, ct) in enumerate(L.DualForm._fields_[:-1]):
+            want = (1000.0 * i + 100.0 * k + np.arange(ct._length_)).reshape(shapes[name])
+            assert np.array_equal(g[name], want), (i, name)
+        assert g["stats"]["num_inliers"] == 7 + i and g["stats"]["chi2"] == 0.5 * i
+        assert set(g["stats"]) == {f for f, _ in L.Stats._fields_ if f != "pad"}
+    assert api.batch_build_dual(_Ctx(), []) == []
