@@ -115,27 +115,29 @@ struct Rodrigues {
   bool small;
 };
 
+// Below 1e-9 rad: R = I, S = 0, n = 0, small = true.  Written without a
+// branch (the general form on a safe angle, then a per-element choice of
+// values): a branch that assigns the matrices on two paths leaves them in
+// private scratch memory on the device.
 KDH_HD Rodrigues rodrigues_f(V3f w) {
   Rodrigues o;
   o.angle = sqrtf(sqnorm(w));
-  if (o.angle < 1e-9f) {
-    o.small = true;
-    o.R = m3_identity<float>();
-    o.S = m3_zero<float>();
-    o.n = v3(0.0f, 0.0f, 0.0f);
-    return o;
-  }
-  o.small = false;
-  float inv = 1.0f / o.angle;
-  o.n = w * inv;
-  o.S = skew(o.n);
-  M3f S2 = mul(o.S, o.S);
+  const bool small = o.angle < 1e-9f;
+  o.small = small;
+  const float inv = 1.0f / (small ? 1.0f : o.angle);
+  const V3f n = w * inv;
+  const M3f S = skew(n);
+  const M3f S2 = mul(S, S);
   float s, c;
   sincos_shared(o.angle, &s, &c);
-  float omc = 1.0f - c;
-  M3f I = m3_identity<float>();
+  const float omc = 1.0f - c;
+  const M3f I = m3_identity<float>();
 #pragma unroll
-  for (int i = 0; i < 9; ++i) o.R.m[i] = (I.m[i] + s * o.S.m[i]) + omc * S2.m[i];
+  for (int i = 0; i < 9; ++i) {
+    o.R.m[i] = small ? I.m[i] : (I.m[i] + s * S.m[i]) + omc * S2.m[i];
+    o.S.m[i] = small ? 0.0f : S.m[i];
+  }
+  o.n = small ? v3(0.0f, 0.0f, 0.0f) : n;
   return o;
 }
 
