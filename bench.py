@@ -537,6 +537,7 @@ def run_tree_build(ctx, w, sweep5, DeviceTree, fixed_tree=None, params=None):
             tr = g.makeTreeOnDevice(ctx, pts, st, download=False)
             dev.append(time.perf_counter() - t)
             tr.close()
+        devin = _build_from_hbm(ctx, g, pts, st, reps)
         host1, host16 = [], []
         for _ in range(host_reps):
             t = time.perf_counter()
@@ -548,12 +549,54 @@ def run_tree_build(ctx, w, sweep5, DeviceTree, fixed_tree=None, params=None):
         gc.enable()
         out[name] = dict(points=len(pts), device_ms_p50=round(1e3 * float(np.median(dev)), 3),
                          device_ms_p99=round(1e3 * float(np.percentile(dev, 99)), 3),
+                         device_from_hbm_ms_p50=round(1e3 * float(np.median(devin)), 3),
                          host_1thread_ms=round(1e3 * float(np.median(host1)), 2),
                          host_16thread_ms=round(1e3 * float(np.median(host16)), 2))
-        log(f"tree build {name}: device {out[name]['device_ms_p50']:.2f} ms, host 1 thread "
+        log(f"tree build {name}: device {out[name]['device_ms_p50']:.2f} ms ({out[name]['device_from_hbm_ms_p50']:.2f} "
+            f"ms from points in HBM), host 1 thread "
             f"{out[name]['host_1thread_ms']:.1f} ms, 16 threads {out[name]['host_16thread_ms']:.1f} ms")
     if fixed_tree is not None:
         out["odometry_from_points"] = run_points_to_pose(ctx, w, fixed_tree, params, g, g16, DeviceTree)
+    return out
+
+
+def _build_from_hbm(ctx, g, pts, st, reps):
+    """Wall times of kdh_tree_build over points (and stamps) already in HBM
+    (inputs_on_device = 1): the device build without the host-to-device copy
+    of the cloud."""
+    import ctypes as C
+
+    from kd_slam2_amd import _lib as L
+
+    hip = C.CDLL("libamdhip64.so.7")
+    hip.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+    hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    hip.hipFree.argtypes = [C.c_void_p]
+    x = np.ascontiguousarray(pts, dtype=np.float32)
+    t = None if st is None else np.ascontiguousarray(st, dtype=np.float64)
+    dx, dt = C.c_void_p(), C.c_void_p()
+    out = []
+    try:
+        if hip.hipMalloc(C.byref(dx), x.nbytes) != 0 or hip.hipMemcpy(dx, x.ctypes.data, x.nbytes, 1) != 0:
+            raise RuntimeError("device copy of the points failed")
+        if t is not None and (hip.hipMalloc(C.byref(dt), t.nbytes) != 0 or
+                              hip.hipMemcpy(dt, t.ctypes.data, t.nbytes, 1) != 0):
+            raise RuntimeError("device copy of the stamps failed")
+        tp = None if t is None else C.cast(dt, L.F64P)
+        for k in range(reps + 3):
+            h = C.c_void_p()
+            t0 = time.perf_counter()
+            ctx.check(L.lib().kdh_tree_build(ctx.h, C.cast(dx, L.F32P), tp, len(x), C.byref(g._p), 1, C.byref(h)),
+                      "kdh_tree_build")
+            if k >= 3:
+                out.append(time.perf_counter() - t0)
+            if h.value:
+                L.lib().kdh_tree_free(ctx.h, h)
+    finally:
+        if dx.value:
+            hip.hipFree(dx)
+        if dt.value:
+            hip.hipFree(dt)
     return out
 
 
