@@ -383,6 +383,22 @@ int kdh_kdtree3f_decode(const uint8_t* buf, size_t len, kdh_kdtree3f_info* info,
   if ((uint64_t)n * 52 > (uint64_t)(len - in.pos) + 4) return KDH_ERR_INVALID;
   *n_nodes = n;
   const bool fill = nodes && node_cap >= (int64_t)n;
+  if (!fill) {
+    // Asked for the count only.  A node is 40 bytes of 4-byte fields and two
+    // doubles at 8-byte alignment: the first may carry 4 bytes of padding,
+    // every later one starts 8-aligned and takes 56.  The end of the records
+    // is therefore known without reading them, and a shorter buffer is the
+    // only way they can be malformed.
+    size_t end = in.pos;
+    if (n > 0) {
+      end += 40;
+      end += (8 - (end - 4) % 8) % 8;
+      end += 16 + (size_t)56 * (n - 1);
+    }
+    if (end > len) return KDH_ERR_INVALID;
+    *info = out;
+    return KDH_OK;
+  }
   for (uint32_t i = 0; i < n; ++i) {
     kdh_node3f k;
     memset(&k, 0, sizeof(k));

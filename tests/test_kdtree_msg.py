@@ -201,3 +201,30 @@ def test_mcap_writer_layout(tmp_path):
     assert (n_msg, n_schema, n_chan, n_att, n_meta, n_chunk, t0, t1) == (3, 1, 2, 0, 0, 0, 5, 9)
     counts = dict(struct.unpack_from("<HQ", st, 46 + 10 * k) for k in range(mlen // 10))
     assert counts == {0: 2, 1: 1}
+
+
+def test_count_only_decode_agrees_with_the_full_parse_on_every_cut():
+    """kdh_kdtree3f_decode asked for the count only (nodes NULL) finds the end
+    of the node records from their fixed layout; it must accept and reject
+    exactly the buffers the node-by-node parse does.  Frame ids of several
+    lengths put the first node at both 8-byte phases."""
+    import ctypes as C
+
+    from kd_slam2_amd import _lib as L
+
+    nodes = np.concatenate([_node([k, 2, 3], [0, 0, 1], k, k + 4, -1, -1, 0.5 * k, 0.25) for k in range(5)])
+    for fid in ("", "m", "map", "odom", "map_frame_x"):
+        data = KM.encode(KM.KDTree3fMessage(nodes, frame_id=fid, src_topic="/t"))
+        for cut in range(4, len(data) + 1):
+            raw = (C.c_uint8 * cut).from_buffer_copy(data[:cut])
+            out = []
+            for fill in (False, True):
+                info, nn = L.KDTree3fInfo(), C.c_int64()
+                fb, tb = C.create_string_buffer(64), C.create_string_buffer(64)
+                buf = np.zeros(len(nodes), dtype=NODE_DTYPE)
+                rc = L.lib().kdh_kdtree3f_decode(raw, cut, C.byref(info), fb, 64, tb, 64,
+                                                 buf.ctypes.data_as(C.c_void_p) if fill else None,
+                                                 len(buf) if fill else 0, C.byref(nn))
+                out.append((rc, nn.value if rc == L.KDH_OK else None))
+            assert out[0] == out[1], (fid, cut, out)
+            assert (out[1][0] == L.KDH_OK) == (cut == len(data)), (fid, cut)
