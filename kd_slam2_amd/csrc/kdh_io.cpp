@@ -383,38 +383,35 @@ int kdh_kdtree3f_decode(const uint8_t* buf, size_t len, kdh_kdtree3f_info* info,
   if ((uint64_t)n * 52 > (uint64_t)(len - in.pos) + 4) return KDH_ERR_INVALID;
   *n_nodes = n;
   const bool fill = nodes && node_cap >= (int64_t)n;
-  if (!fill) {
-    // Asked for the count only.  A node is 40 bytes of 4-byte fields and two
-    // doubles at 8-byte alignment: the first may carry 4 bytes of padding,
-    // every later one starts 8-aligned and takes 56.  The end of the records
-    // is therefore known without reading them, and a shorter buffer is the
-    // only way they can be malformed.
-    size_t end = in.pos;
-    if (n > 0) {
-      end += 40;
-      end += (8 - (end - 4) % 8) % 8;
-      end += 16 + (size_t)56 * (n - 1);
+  // A node is 40 bytes of 4-byte fields (mean, direction, idx_start, idx_end,
+  // idx_left, idx_right) and two doubles at 8-byte alignment: the first may
+  // carry 4 bytes of padding, every later one starts 8-aligned and takes 56.
+  // The end of the records is therefore known before reading them, and a
+  // shorter buffer is the only way they can be malformed.
+  const size_t first = in.pos;
+  const size_t pad = (8 - (first + 40 - 4) % 8) % 8;
+  const size_t end = n > 0 ? first + 40 + pad + 16 + (size_t)56 * (n - 1) : first;
+  if (end > len) return KDH_ERR_INVALID;
+  if (fill) {
+    const uint8_t* p = buf + first;
+    for (uint32_t i = 0; i < n; ++i) {
+      kdh_node3f k;
+      memset(&k, 0, sizeof(k));
+      uint32_t a = 0, b = 0;
+      memcpy(k.mean, p, 12);
+      memcpy(k.direction, p + 12, 12);
+      memcpy(&a, p + 24, 4);
+      memcpy(&b, p + 28, 4);
+      memcpy(&k.idx_left, p + 32, 4);
+      memcpy(&k.idx_right, p + 36, 4);
+      const uint8_t* q = p + 40 + (i == 0 ? pad : 0);
+      memcpy(&k.dts_mean, q, 8);
+      memcpy(&k.dts_covariance, q + 8, 8);
+      k.idx_start = a;
+      k.idx_end = b;
+      nodes[i] = k;
+      p = q + 16;
     }
-    if (end > len) return KDH_ERR_INVALID;
-    *info = out;
-    return KDH_OK;
-  }
-  for (uint32_t i = 0; i < n; ++i) {
-    kdh_node3f k;
-    memset(&k, 0, sizeof(k));
-    uint32_t a = 0, b = 0;
-    for (int d = 0; d < 3; ++d) in.get(k.mean[d]);
-    for (int d = 0; d < 3; ++d) in.get(k.direction[d]);
-    in.get(a);
-    in.get(b);
-    in.get(k.idx_left);
-    in.get(k.idx_right);
-    in.get(k.dts_mean);
-    in.get(k.dts_covariance);
-    if (!in.ok) return KDH_ERR_INVALID;
-    k.idx_start = a;
-    k.idx_end = b;
-    if (fill) nodes[i] = k;
   }
   *info = out;
   return KDH_OK;

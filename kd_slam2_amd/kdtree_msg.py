@@ -62,7 +62,9 @@ def encode(msg):
 
 def decode(data):
     """Parse a serialized KDTree3f: a KDTree3fMessage."""
-    raw = (C.c_uint8 * len(data)).from_buffer_copy(data)
+    # bytes are read in place (the library only reads the buffer); other
+    # buffer types are copied once
+    raw = C.c_char_p(data) if isinstance(data, bytes) else (C.c_uint8 * len(data)).from_buffer_copy(data)
     info = L.KDTree3fInfo()
     nn = C.c_int64()
     cap = len(data) + 1
