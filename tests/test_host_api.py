@@ -136,3 +136,17 @@ def test_dual_factor_marshalling_round_trip(monkeypatch):
         assert g["stats"]["num_inliers"] == 7 + i and g["stats"]["chi2"] == 0.5 * i
         assert set(g["stats"]) == {f for f, _ in L.Stats._fields_ if f != "pad"}
     assert api.batch_build_dual(_Ctx(), []) == []
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py measures one GPU: --gpus above 1 exits with status 2 and says
+    why, before touching a device."""
+    import os
+    import subprocess
+    import sys
+
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, os.path.join(root, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120, cwd=root)
+    assert r.returncode == 2
+    assert "one GPU only" in r.stderr and r.stdout == ""
