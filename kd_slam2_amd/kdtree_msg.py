@@ -154,7 +154,8 @@ def write_mcap(path, records):
                                                             C.byref(cid)), what="kdh_mcap_writer_add_channel")
                 channels[topic] = cid.value
             data = msg if isinstance(msg, (bytes, bytearray)) else encode(msg)
-            buf = (C.c_uint8 * len(data)).from_buffer_copy(data)
+            # bytes are passed in place (the writer only reads them)
+            buf = C.c_char_p(data) if isinstance(data, bytes) else (C.c_uint8 * len(data)).from_buffer_copy(data)
             L.check(L.lib().kdh_mcap_writer_write(w, channels[topic], int(t), int(t), buf, len(data)),
                     what="kdh_mcap_writer_write")
     finally:
