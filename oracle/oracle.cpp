@@ -668,10 +668,15 @@ static void dlogSO3_f(const Mat3<float>& R, float J[27]) {
   }
 }
 
-static void prior_form(const Iso3<float>& ZXi, const Iso3<float>& X, const float* info, float* H6, float* b6) {
+// poseToPoseError and dPoseToPoseError (geometry_3d_impl_.h:117-140):
+// e = logmap(ZXi * X); J = dlogmap(P) * B with B the 12x6 derivative of
+// P = ZXi * X under the left perturbation X <- exp(dx) X.
+static void pose_error_f(const Iso3<float>& ZXi, const Iso3<float>& X, float e[6], float J[36]) {
   Iso3<float> P = compose(ZXi, X);
   Vec3<float> w = logSO3(P.R);
-  float e[6] = {P.t[0], P.t[1], P.t[2], w[0], w[1], w[2]};
+  e[0] = P.t[0]; e[1] = P.t[1]; e[2] = P.t[2];
+  e[3] = w[0]; e[4] = w[1]; e[5] = w[2];
+  if (!J) return;
   float B[72];
   for (int i = 0; i < 72; ++i) B[i] = 0;
   Mat3<float> Rst = matmul(ZXi.R, skew(X.t));
@@ -687,7 +692,6 @@ static void prior_form(const Iso3<float>& ZXi, const Iso3<float>& X, const float
   }
   float Dl[27];
   dlogSO3_f(P.R, Dl);
-  float J[36];
   for (int r = 0; r < 6; ++r)
     for (int c = 0; c < 6; ++c) {
       float s = 0;
@@ -696,6 +700,11 @@ static void prior_form(const Iso3<float>& ZXi, const Iso3<float>& X, const float
         for (int k = 0; k < 9; ++k) s += Dl[(r - 3) * 9 + k] * B[(3 + k) * 6 + c];
       J[r * 6 + c] = s;
     }
+}
+
+static void prior_form(const Iso3<float>& ZXi, const Iso3<float>& X, const float* info, float* H6, float* b6) {
+  float e[6], J[36];
+  pose_error_f(ZXi, X, e, J);
   for (int r = 0; r < 6; ++r) {
     float sb = 0;
     for (int k = 0; k < 6; ++k) {
@@ -1424,6 +1433,12 @@ float orc_stable_sum_f(const float* x, int64_t n) {
 // Pose prior H/b (FP32) for unit checks against finite differences.
 void orc_prior_form(const float* ZXi34, const float* X34, const float* info, float* H6, float* b6) {
   prior_form(from34<float>(ZXi34), from34<float>(X34), info, H6, b6);
+}
+
+// poseToPoseError (e, 6) and dPoseToPoseError (J, 6x6 row-major, may be NULL)
+// in FP32 (geometry_3d_impl_.h:117-140).
+void orc_pose_to_pose_error(const float* ZXi34, const float* X34, float* e6, float* J36) {
+  pose_error_f(from34<float>(ZXi34), from34<float>(X34), e6, J36);
 }
 
 }  // extern "C"

@@ -123,6 +123,12 @@ def lib(fast=False):
         L.orc_stable_sum_f.restype = C.c_float
         L.orc_stable_sum_f.argtypes = [f32, C.c_int64]
         L.orc_prior_form.argtypes = [f32, f32, f32, f32, f32]
+        L.orc_pose_to_pose_error.argtypes = [f32, f32, f32, f32]
+        L.orc_generate_scene.restype = C.c_int64
+        L.orc_generate_scene.argtypes = [C.c_float, C.c_float, C.c_int, f32, C.c_int64]
+        L.orc_make_scan.restype = C.c_int64
+        L.orc_make_scan.argtypes = [f32, C.c_int64, f32, f32, C.c_double, f32, f64, C.c_int64]
+        L.orc_scene_expmap.argtypes = [f32, f32]
         _libs[name] = L
     return _libs[name]
 
@@ -349,3 +355,49 @@ def tree_build(points, stamps=None, **params):
         return None
     return dict(nodes=nodes[:nn.value].copy(), leaves=leaves[:nl.value].copy(), points=xyz[:n].copy(),
                 stamps=sto[:n].copy())
+
+
+# ---- the reference tests' scenes (oracle/scenes.cpp) -------------------------
+
+def generate_scene(range_=20.0, resolution=0.05, num_planes=10):
+    """generateScene (d3/typedefs.cpp:51-64) from a fresh process's random
+    state: (n, 3) float32."""
+    L = lib()
+    n = L.orc_generate_scene(range_, resolution, num_planes, None, 0)
+    out = np.zeros((n, 3), dtype=np.float32)
+    L.orc_generate_scene(range_, resolution, num_planes, p32(out), n)
+    return out
+
+
+def make_scan(scene, start_pose=None, velocities=None, t_start=0.0):
+    """Scanner::makeScan (d3/scanner.cpp:5-83), default ScannerParams:
+    (points (m, 3) float32 in the sensor frame, stamps (m,) float64)."""
+    L = lib()
+    sc = _f32(scene)
+    X = _f32(np.eye(4)[:3, :4].reshape(12) if start_pose is None else np.asarray(start_pose)[:3, :4].reshape(12))
+    v = _f32(np.zeros(6) if velocities is None else velocities)
+    cap = 1024 * 128
+    xyz = np.zeros((cap, 3), dtype=np.float32)
+    st = np.zeros(cap)
+    m = L.orc_make_scan(p32(sc), len(sc), p32(X), p32(v), t_start, p32(xyz), p64(st), cap)
+    return xyz[:m].copy(), st[:m].copy()
+
+
+def scene_expmap(v):
+    """GeometryTraits_<3>::expmap in FP32 (geometry_3d_impl_.h:179-184): 4x4."""
+    out = np.zeros(12, dtype=np.float32)
+    lib().orc_scene_expmap(p32(_f32(v)), p32(out))
+    X = np.eye(4)
+    X[:3, :4] = out.reshape(3, 4)
+    return X
+
+
+def pose_to_pose_error(ZXi, X, jacobian=True):
+    """poseToPoseError / dPoseToPoseError in FP32 (geometry_3d_impl_.h:117-140):
+    (e (6,), J (6, 6) or None)."""
+    z = _f32(np.asarray(ZXi)[:3, :4].reshape(12))
+    x = _f32(np.asarray(X)[:3, :4].reshape(12))
+    e = np.zeros(6, dtype=np.float32)
+    J = np.zeros(36, dtype=np.float32) if jacobian else None
+    lib().orc_pose_to_pose_error(p32(z), p32(x), p32(e), None if J is None else p32(J))
+    return e, (None if J is None else J.reshape(6, 6))
