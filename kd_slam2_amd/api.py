@@ -474,21 +474,26 @@ class _AlignerBase:
         self.stats_log = []
         self.saveStartState()
         max_iter = 1 if stats_mode else self.params.max_iterations
-        prev_score = 0.0
+        prev_score = np.float32(0.0)
         for i in range(max_iter):
             ok = self.oneRound(stats_mode)
             self.stats_log.append(dict(self.stats))
             if not ok:
                 return -i
             if self.params.use_term_crit:
+                # ICPTerminationCriteriaScore_::hasToStop (icp_stats_.h:85-102):
+                # the score in FP32 as ICPStats_::score(), the relative change
+                # divided in double (the reference's 1e-6 is a double literal)
+                f = np.float32
                 s = self.stats
                 nl = s["num_inliers"] + s["num_outliers"] + s["num_bad"] + s["num_err_normal"] + s["num_err_fn"]
-                ratio = s["num_inliers"] / nl if nl else 0.0
-                score = ratio / (s["chi2_kernelized"] / max(s["num_inliers"], 1e-30) + 1e-6)
+                ratio = f(s["num_inliers"]) / f(nl) if nl else f(0.0)
+                with np.errstate(divide="ignore", invalid="ignore"):
+                    score = f(ratio / (f(s["chi2_kernelized"]) / f(s["num_inliers"]) + f(1e-6)))
                 if i > 0:
-                    d = score - prev_score
-                    if s["pert_pose_norm"] < self.params.term_d_pert and d > 0 and \
-                            d / (prev_score + 1e-6) < self.params.term_d_score:
+                    d = f(score - prev_score)
+                    if f(s["pert_pose_norm"]) < f(self.params.term_d_pert) and d > 0 and \
+                            float(d) / (float(prev_score) + 1e-6) < float(f(self.params.term_d_score)):
                         return i
                 prev_score = score
         return self.params.max_iterations

@@ -559,11 +559,16 @@ __device__ void finish_pair(PairState& ps, const PairState& S, const float* __re
   st.pert_vel_norm = sqrt(vn);
   ps.stats_log[it] = st;
   ps.iter = it + 1;
-  // termination: stats mode runs one round; the score criterion
-  // (icp_stats_.h:85-102) may stop early; otherwise max_iterations.
-  bool stop = stats_mode != 0 || it + 1 >= sp.max_iterations;
+  // termination (compute(), tree_aligner_base_.h:100-115): the score
+  // criterion (icp_stats_.h:85-102) is asked after every round, the last one
+  // included, and a stop it reports wins over the end of the loop (compute()
+  // then returns the round index, not max_iterations).  Stats mode runs one
+  // round.  The score is FP32 as ICPStats_::score(); the relative change is
+  // divided by prev_score + 1e-6 in double, as the reference's double literal
+  // promotes it.
+  const bool last = stats_mode != 0 || it + 1 >= sp.max_iterations;
   int status = 0;
-  if (!stop && sp.use_term_crit) {
+  if (sp.use_term_crit) {
     const float ni = (float)st.num_inliers;
     const int nl = st.num_inliers + st.num_outliers + st.num_bad + st.num_err_normal + st.num_err_fn;
     const float ratio = nl ? ni / (float)nl : 0.0f;
@@ -571,13 +576,12 @@ __device__ void finish_pair(PairState& ps, const PairState& S, const float* __re
     if (it > 0) {
       const bool pert_ok = (float)st.pert_pose_norm < sp.d_pert;
       const float ds = score - prev_score;
-      const bool score_ok = (ds > 0.0f) && ds / (prev_score + 1e-6f) < sp.d_score;
-      if (pert_ok && score_ok) stop = true;
+      const bool score_ok = (ds > 0.0f) && (double)ds / ((double)prev_score + 1e-6) < (double)sp.d_score;
+      if (pert_ok && score_ok) status = 3;
     }
     ps.prev_score = score;
-    if (stop) status = 3;
   }
-  if (stop && status == 0) status = 1;
+  if (status == 0 && last) status = 1;
   ps.status = status;
   if (status_out) status_out[0] = status;
   for (int m = 1; m < n_entries; ++m) {

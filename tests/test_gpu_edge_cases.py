@@ -92,18 +92,54 @@ def test_stats_mode_one_round_no_update(ctx):
     assert abs(s["pert_pose_norm"] - so["pert_pose_norm"]) <= 1e-3 * max(1e-3, so["pert_pose_norm"])
 
 
-def test_termination_criterion_stops_like_oracle(ctx):
+def _criterion_margins(log, d_pert, d_score):
+    """Per round i > 0 of an oracle log: (pert_pose_norm / d_pert, relative
+    score change / d_score) as ICPTerminationCriteriaScore_ evaluates them
+    (icp_stats_.h:85-102), to tell a decision near its threshold."""
+    f = np.float32
+    out, prev = [], f(0.0)
+    for i, s in enumerate(log):
+        nl = s["num_inliers"] + s["num_outliers"] + s["num_bad"] + s["num_err_normal"] + s["num_err_fn"]
+        score = f(f(f(s["num_inliers"]) / f(nl)) / (f(s["chi2_kernelized"]) / f(s["num_inliers"]) + f(1e-6)))
+        if i > 0:
+            d = f(score - prev)
+            out.append((s["pert_pose_norm"] / d_pert, float(d) / (float(prev) + 1e-6) / d_score))
+        prev = score
+    return out
+
+
+@pytest.mark.parametrize("stop_on_last", [False, True])
+def test_termination_criterion_stops_like_oracle(ctx, stop_on_last):
+    """The score criterion ends the device loop at the oracle's round, with
+    compute()'s return value (the round index), also when the stop falls on
+    the last allowed round (tree_aligner_base_.h:100-115 asks the criterion
+    before the loop ends)."""
     w = cfg1()
     F, M = DeviceTree(ctx, w["fixed"]), DeviceTree(ctx, w["moving"])
     p = icp_params(max_iterations=30, use_term_crit=1, term_d_pert=1e-3, term_d_score=1e-3)
+    Xo, _, log_o, res_o = O.compute(w["fixed"], w["moving"], w["X0"], w["Xf"], O.from_kdh_params(p))
+    assert 0 < res_o < 29
+    # the oracle's decisions are clear of their thresholds in every round
+    # up to the stop, so the FP64 device loop must take the same ones
+    for pm, sm in _criterion_margins(log_o, 1e-3, 1e-3):
+        assert abs(pm - 1.0) > 0.02 and (sm <= 0 or abs(sm - 1.0) > 0.02)
+    if stop_on_last:
+        p = icp_params(max_iterations=res_o + 1, use_term_crit=1, term_d_pert=1e-3, term_d_score=1e-3)
+        Xo, _, log_o, res_o2 = O.compute(w["fixed"], w["moving"], w["X0"], w["Xf"], O.from_kdh_params(p))
+        assert res_o2 == res_o == p.max_iterations - 1
     al = _aligner(ctx, ICPHIP3D, F, M, w, p)
     res = al.computeOnDevice()
-    Xo, _, log_o, res_o = O.compute(w["fixed"], w["moving"], w["X0"], w["Xf"], O.from_kdh_params(p))
     print("criterion stop", res, res_o)
-    assert 0 < res < 30 and abs(res - res_o) <= 1
+    assert res == res_o
     assert len(al.stats_log) == res + 1
+    for a, b in zip(al.stats_log, log_o):
+        for k in ("num_inliers", "num_outliers", "num_bad", "num_err_normal", "num_err_fn"):
+            assert a[k] == b[k]
     dt, dr = synth.pose_error(al.getX(), Xo)
-    assert dt < 2e-3 and dr < 2e-3
+    assert dt < 1e-3 and dr < 1e-3
+    # the host per-call loop (the drop-in) returns the same code
+    al2 = _aligner(ctx, ICPHIP3D, F, M, w, p)
+    assert al2.compute() == res_o
 
 
 def test_pose_prior_device_host_and_oracle(ctx):
