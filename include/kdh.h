@@ -334,6 +334,9 @@ typedef struct kdh_server_info {
 } kdh_server_info;
 
 int kdh_server_create(int device, int64_t budget_bytes, int32_t max_inflight, kdh_server** out);
+/* Stops the engine, fails every unfinished request with KDH_ERR_CANCELLED,
+ * wakes the threads blocked in kdh_server_wait and returns once they have
+ * left; no other call on the server may start after destroy has begun. */
 int kdh_server_destroy(kdh_server* server);
 const char* kdh_server_last_error(kdh_server* server);
 /* Counts a resident map against the budget (KDH_ERR_BUDGET if it cannot fit). */

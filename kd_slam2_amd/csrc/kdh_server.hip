@@ -81,8 +81,9 @@ struct kdh_server {
   int max_inflight = 0;
   std::thread th;
   std::mutex mu;
-  std::condition_variable cv_work, cv_done;
+  std::condition_variable cv_work, cv_done, cv_idle;
   bool stop = false;
+  int waiters = 0;  // threads inside kdh_server_wait / kdh_server_poll
   std::string err;
   std::deque<std::shared_ptr<Req>> queue;
   std::map<int64_t, std::shared_ptr<Req>> reqs;  // queued, active and finished-not-collected
@@ -169,7 +170,19 @@ static void engine(kdh_server* S) {
             slot = i;
             break;
           }
-        if (slot < 0 || S->bytes_resident + S->bytes_inflight + r->bytes > S->budget) break;
+        if (slot < 0) break;
+        if (S->bytes_resident + S->bytes_inflight + r->bytes > S->budget) {
+          // Nothing in flight will free memory: resident maps added after
+          // this request was queued leave it no room, now or later.
+          if (S->bytes_inflight == 0) {
+            S->queue.pop_front();
+            S->err = "request no longer fits beside the resident maps";
+            finish_req(S, r, KDH_ERR_BUDGET);
+            S->cv_done.notify_all();
+            continue;
+          }
+          break;
+        }
         S->queue.pop_front();
         r->slot = slot;
         r->state = 1;
@@ -401,7 +414,13 @@ int kdh_server_destroy(kdh_server* S) {
   if (S->d_tile_pair) hipFree(S->d_tile_pair);
   if (S->h_status) hipHostFree(S->h_status);
   kdh_ctx_destroy(S->ctx);
-  S->cv_done.notify_all();
+  {
+    // wake every kdh_server_wait (it returns KDH_ERR_CANCELLED) and free the
+    // server only once the last waiter has left its lock
+    std::unique_lock<std::mutex> lk(S->mu);
+    S->cv_done.notify_all();
+    S->cv_idle.wait(lk, [&] { return S->waiters == 0; });
+  }
   delete S;
   return KDH_OK;
 }
@@ -471,9 +490,21 @@ int kdh_server_submit(kdh_server* S, const kdh_request* rq, int64_t* id) {
   return KDH_OK;
 }
 
+// Counts a caller inside kdh_server_wait / kdh_server_poll (constructed and
+// destroyed with S->mu held), so that kdh_server_destroy frees the server
+// only after every such caller has left.
+struct WaiterGuard {
+  kdh_server* S;
+  explicit WaiterGuard(kdh_server* s) : S(s) { ++S->waiters; }
+  ~WaiterGuard() {
+    if (--S->waiters == 0) S->cv_idle.notify_all();
+  }
+};
+
 int kdh_server_poll(kdh_server* S, int64_t id, int* done, kdh_request_result* out) {
   if (!S || !done) return KDH_ERR_INVALID;
   std::lock_guard<std::mutex> lk(S->mu);
+  WaiterGuard guard(S);
   auto it = S->reqs.find(id);
   if (it == S->reqs.end()) return KDH_ERR_NOT_FOUND;
   *done = it->second->state == 2;
@@ -487,6 +518,7 @@ int kdh_server_poll(kdh_server* S, int64_t id, int* done, kdh_request_result* ou
 int kdh_server_wait(kdh_server* S, int64_t id, double timeout_s, kdh_request_result* out) {
   if (!S) return KDH_ERR_INVALID;
   std::unique_lock<std::mutex> lk(S->mu);
+  WaiterGuard guard(S);
   auto it = S->reqs.find(id);
   if (it == S->reqs.end()) return KDH_ERR_NOT_FOUND;
   auto r = it->second;

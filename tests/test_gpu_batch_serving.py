@@ -123,3 +123,70 @@ def test_server_budget_and_cancel(ctx, small_pairs):
         assert r1["queue_us"] >= 0.0
     finally:
         srv.close()
+
+
+def _request_bytes_lower_bound(tree):
+    """Device bytes of a moving tree's records, below what admission counts."""
+    return len(tree.nodes) * 36 + len(tree.leaves) * 32
+
+
+def test_server_resident_growth_fails_a_queued_request(ctx, small_pairs):
+    """A resident map added after a request was queued can leave it no room
+    even with nothing in flight: admission fails it with KDH_ERR_BUDGET
+    instead of holding the queue forever."""
+    w = small_pairs[6]
+    F = DeviceTree(ctx, w["fixed"])
+    G = DeviceTree(ctx, w["fixed"])  # a second resident map of the same size
+    est = _request_bytes_lower_bound(w["moving"])
+    srv = Server(0, budget_bytes=F.device_bytes() + G.device_bytes() + est // 2, max_inflight=1)
+    try:
+        srv.add_resident(F)
+        p = icp_params(max_iterations=64, velocity_damping=10.0)
+        b = srv.submit(F, w["moving"], w["X0"], w["Xf"], p, ct=1, v0=w["v0"])
+        c = srv.submit(F, w["moving"], w["X0"], w["Xf"], p, ct=1, v0=w["v0"])
+        srv.add_resident(G)
+        info = srv.info()
+        # the single slot is still busy with b, so c is still queued
+        assert info["queued"] == 1, info
+        rb = srv.wait(b, 60.0)
+        rc = srv.wait(c, 60.0)
+        assert rb["status"] == 0
+        assert rc["status"] == -4  # KDH_ERR_BUDGET
+        info = srv.info()
+        assert info["inflight"] == 0 and info["queued"] == 0 and info["bytes_inflight"] == 0
+    finally:
+        srv.close()
+
+
+def test_server_destroy_while_a_thread_waits(ctx, small_pairs):
+    """kdh_server_destroy wakes a blocked kdh_server_wait and frees the
+    server only after the waiter has left (no use after free)."""
+    import ctypes as C
+    import threading
+    import time
+
+    from kd_slam2_amd import _lib as L
+
+    w = small_pairs[6]
+    F = DeviceTree(ctx, w["fixed"])
+    for _ in range(3):
+        srv = Server(0, budget_bytes=1 << 30, max_inflight=1)
+        srv.add_resident(F)
+        p = icp_params(max_iterations=64, velocity_damping=10.0)
+        ids = [srv.submit(F, w["moving"], w["X0"], w["Xf"], p, ct=1, v0=w["v0"]) for _ in range(2)]
+        h = srv.h
+        codes = []
+
+        def waiter(rid):
+            rr = L.RequestResult()
+            codes.append(L.lib().kdh_server_wait(h, C.c_int64(rid), C.c_double(0.0), C.byref(rr)))
+
+        ths = [threading.Thread(target=waiter, args=(r,)) for r in ids]
+        for t in ths:
+            t.start()
+        time.sleep(0.0005)
+        srv.close()
+        for t in ths:
+            t.join(30.0)
+            assert not t.is_alive()
+        assert len(codes) == 2 and all(c in (0, -6) for c in codes), codes
