@@ -144,6 +144,18 @@ int kdh_tree_extract_leaves(kdh_ctx* ctx, const kdh_tree* tree, const float* v, 
  * (out: 6 * n_nodes floats).  No reference counterpart. */
 int kdh_tree_read_nodes(kdh_ctx* ctx, const kdh_tree* tree, float* out);
 
+/* TreeCUDA_::clone (Tree_CUDAfromCUDA_, tree/tree_cuda_impl_.h:176-195;
+ * tree/tree_cuda_.h:57): a device-to-device deep copy of a resident tree
+ * (every record, the stamps and, for device-built trees, the full records,
+ * leaf list and points).  Free it with kdh_tree_free. */
+int kdh_tree_clone(kdh_ctx* ctx, const kdh_tree* src, kdh_tree** out);
+/* TreeCUDA_::copyNodes (tree/tree_cuda_impl_.h:60-69; tree/tree_cuda_.h:43):
+ * overwrites dst's node and leaf records (and full records) with src's, on
+ * the device; KDH_ERR_INVALID on a size mismatch, as the reference's
+ * "size mismatch".  The restore of the bundle-adjustment pre-deskew
+ * (optimizer/optimizer_proc_solver_action_.h:39-58, map/map_.h:88-102). */
+int kdh_tree_copy_nodes(kdh_ctx* ctx, const kdh_tree* src, kdh_tree* dst);
+
 /* ---- one quadratic form (the plain per-call path) -------------------------
  * Replaces ICP_CUDA_::_buildQuadraticForm (icp/icp_cuda_impl_.cuh:95-132):
  * for every leaf of `moving`, descend `fixed`, apply the gates, and reduce

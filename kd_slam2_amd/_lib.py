@@ -239,6 +239,8 @@ _SIGS = {
     "kdh_tree_apply_isometry": (C.c_int, [P, P, F32P]),
     "kdh_tree_extract_leaves": (C.c_int, [P, P, F32P, F32P]),
     "kdh_tree_read_nodes": (C.c_int, [P, P, F32P]),
+    "kdh_tree_clone": (C.c_int, [P, P, C.POINTER(P)]),
+    "kdh_tree_copy_nodes": (C.c_int, [P, P, P]),
     "kdh_icp_build": (C.c_int, [P, P, P, F32P, F32P, F32P, C.POINTER(Params), C.c_int, F64P, F64P, C.POINTER(Stats)]),
     "kdh_build_at": (
         C.c_int,

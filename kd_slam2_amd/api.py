@@ -223,6 +223,32 @@ class DeviceTree:
             self.ctx.check(L.lib().kdh_tree_read_nodes(self.ctx.h, self.h, L.f32p(out)), "kdh_tree_read_nodes")
         return out
 
+    def clone(self):
+        """TreeCUDA_::clone (tree/tree_cuda_impl_.h:176-195): a deep copy on
+        the device (the Frame::save backup, map/map_.h:88-94)."""
+        h = C.c_void_p()
+        self.ctx.check(L.lib().kdh_tree_clone(self.ctx.h, self.h, C.byref(h)), "kdh_tree_clone")
+        out = self.__class__.__new__(self.__class__)
+        out.ctx = self.ctx
+        out.h = h
+        out.host = self.host
+        if hasattr(self, "stamps"):
+            out.stamps = self.stamps
+        return out
+
+    def copyNodes(self, other):
+        """TreeCUDA_::copyNodes (tree/tree_cuda_impl_.h:60-69): overwrite
+        other's nodes with this tree's (Frame::restore, map/map_.h:96-102)."""
+        self.ctx.check(L.lib().kdh_tree_copy_nodes(self.ctx.h, self.h, other.h), "kdh_tree_copy_nodes")
+
+    def downloadNodes(self):
+        """A device-built tree's full Node_ records as they are now."""
+        nn, _ = self._counts()
+        nodes = np.zeros(nn, dtype=L.NODE_DTYPE)
+        self.ctx.check(L.lib().kdh_tree_download(self.ctx.h, self.h, nodes.ctypes.data_as(C.c_void_p), None, None,
+                                                 None), "kdh_tree_download")
+        return nodes
+
     def close(self):
         if self.h and self.ctx.h:
             L.lib().kdh_tree_free(self.ctx.h, self.h)

@@ -45,9 +45,24 @@ __device__ __forceinline__ void warp(V3f& mean, V3f& dir, float dts, const Vel6&
 
 // One thread per node record, then per leaf record.  A node is a leaf when
 // both links are -1 (Node_::isLeaf, tree/node_.h:54-63).
+// A tree built on the device also holds the full Node_ records
+// (kdh_tree_download); `full` (or null) gets the same mean and direction as
+// the compact record, so both stay identical after an in-place op.
+__device__ __forceinline__ void store_full(kdh_node3f* full, int i, V3f mean, V3f dir) {
+  if (!full) return;
+  kdh_node3f& n = full[i];
+  n.mean[0] = mean.x;
+  n.mean[1] = mean.y;
+  n.mean[2] = mean.z;
+  n.direction[0] = dir.x;
+  n.direction[1] = dir.y;
+  n.direction[2] = dir.z;
+}
+
 __global__ __launch_bounds__(kOpThreads) void apply_velocities_kernel(float4* nodes, float4* leaves,
-                                                                     const float* node_dts, int n_nodes,
-                                                                     int n_leaves, Vel6 V, int skip_leaves) {
+                                                                     const float* node_dts, kdh_node3f* full,
+                                                                     int n_nodes, int n_leaves, Vel6 V,
+                                                                     int skip_leaves) {
   const int i = blockIdx.x * kOpThreads + threadIdx.x;
   if (i < n_nodes) {
     float4 a = nodes[2 * i], b = nodes[2 * i + 1];
@@ -57,6 +72,7 @@ __global__ __launch_bounds__(kOpThreads) void apply_velocities_kernel(float4* no
       warp(mean, dir, node_dts[i], V);
       nodes[2 * i] = make_float4(mean.x, mean.y, mean.z, dir.x);
       nodes[2 * i + 1] = make_float4(dir.y, dir.z, b.z, b.w);
+      store_full(full, i, mean, dir);
     }
   }
   if (i < n_leaves && !skip_leaves) {
@@ -68,8 +84,9 @@ __global__ __launch_bounds__(kOpThreads) void apply_velocities_kernel(float4* no
   }
 }
 
-__global__ __launch_bounds__(kOpThreads) void apply_isometry_kernel(float4* nodes, float4* leaves, int n_nodes,
-                                                                   int n_leaves, Iso12 T) {
+__global__ __launch_bounds__(kOpThreads) void apply_isometry_kernel(float4* nodes, float4* leaves,
+                                                                   kdh_node3f* full, int n_nodes, int n_leaves,
+                                                                   Iso12 T) {
   const int i = blockIdx.x * kOpThreads + threadIdx.x;
   const IsoF X = iso_from_34<float>(T.m);
   if (i < n_nodes) {
@@ -77,6 +94,7 @@ __global__ __launch_bounds__(kOpThreads) void apply_isometry_kernel(float4* node
     const V3f mean = apply_iso(X, v3(a.x, a.y, a.z)), dir = apply_rot(X, v3(a.w, b.x, b.y));
     nodes[2 * i] = make_float4(mean.x, mean.y, mean.z, dir.x);
     nodes[2 * i + 1] = make_float4(dir.y, dir.z, b.z, b.w);
+    store_full(full, i, mean, dir);
   }
   if (i < n_leaves) {
     float4 a = leaves[2 * i], b = leaves[2 * i + 1];
@@ -144,7 +162,7 @@ int kdh_tree_apply_velocities(kdh_ctx* ctx, kdh_tree* tree, const float v[6], in
   for (int i = 0; i < 6; ++i) V.v[i] = v[i];
   hipLaunchKernelGGL(apply_velocities_kernel, dim3(blocks_for(n)), dim3(kOpThreads), 0, ctx->stream,
                      (float4*)tree->dt.nodes, (float4*)tree->dt.leaves, (const float*)tree->node_dts,
-                     tree->dt.n_nodes, tree->dt.n_leaves, V, skip_leaves ? 1 : 0);
+                     tree->full_nodes, tree->dt.n_nodes, tree->dt.n_leaves, V, skip_leaves ? 1 : 0);
   e = hipGetLastError();
   if (e != hipSuccess) return kdh_hip_fail(ctx, e, "apply_velocities_kernel");
   return refresh_root(ctx, tree);
@@ -162,7 +180,8 @@ int kdh_tree_apply_isometry(kdh_ctx* ctx, kdh_tree* tree, const float iso[12]) {
   Iso12 T;
   for (int i = 0; i < 12; ++i) T.m[i] = iso[i];
   hipLaunchKernelGGL(apply_isometry_kernel, dim3(blocks_for(n)), dim3(kOpThreads), 0, ctx->stream,
-                     (float4*)tree->dt.nodes, (float4*)tree->dt.leaves, tree->dt.n_nodes, tree->dt.n_leaves, T);
+                     (float4*)tree->dt.nodes, (float4*)tree->dt.leaves, tree->full_nodes, tree->dt.n_nodes,
+                     tree->dt.n_leaves, T);
   e = hipGetLastError();
   if (e != hipSuccess) return kdh_hip_fail(ctx, e, "apply_isometry_kernel");
   return refresh_root(ctx, tree);
@@ -221,5 +240,79 @@ int kdh_tree_read_nodes(kdh_ctx* ctx, const kdh_tree* tree, float* out) {
     e = hipMemcpyAsync(out, ctx->d_extract, (size_t)n * 6 * sizeof(float), hipMemcpyDeviceToHost, ctx->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
   if (e != hipSuccess) return kdh_hip_fail(ctx, e, "kdh_tree_read_nodes");
+  return KDH_OK;
+}
+
+// ---- backup and restore of a resident tree ----------------------------------
+// TreeCUDA_::clone / Tree_CUDAfromCUDA_ (tree/tree_cuda_impl_.h:176-195,
+// tree_cuda_.h:57) and copyNodes (tree_cuda_impl_.h:60-69, tree_cuda_.h:43):
+// the bundle-adjustment pre-deskew keeps a clone of every keyframe tree and
+// restores the nodes from it before each re-deskew
+// (optimizer/optimizer_proc_solver_action_.h:39-58, map/map_.h:88-102).
+// A tree is one device allocation here, so a clone is one device-to-device
+// copy of it with every section pointer moved by the same offset.
+
+static kdh_tree* rebased_copy(const kdh_tree* src, void* d) {
+  kdh_tree* t = new kdh_tree(*src);
+  const char* o = (const char*)src->dmem;
+  char* n = (char*)d;
+  auto mv = [&](const void* p) -> char* { return p ? n + ((const char*)p - o) : nullptr; };
+  t->dmem = d;
+  t->dt.nodes = (const float4*)mv(src->dt.nodes);
+  t->dt.leaves = (const float4*)mv(src->dt.leaves);
+  t->node_dts = (float*)mv(src->node_dts);
+  t->dt.node_dts = t->node_dts;
+  t->full_nodes = (kdh_node3f*)mv(src->full_nodes);
+  t->full_leaves = (uint64_t*)mv(src->full_leaves);
+  t->full_xyz = (float*)mv(src->full_xyz);
+  t->full_stamps = (double*)mv(src->full_stamps);
+  return t;
+}
+
+int kdh_tree_clone(kdh_ctx* ctx, const kdh_tree* src, kdh_tree** out) {
+  if (!ctx) return KDH_ERR_INVALID;
+  if (!src || !out) return kdh_fail(ctx, KDH_ERR_INVALID, "kdh_tree_clone: null argument");
+  if (src->device != ctx->device) return kdh_fail(ctx, KDH_ERR_INVALID, "tree lives on another device");
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  hipError_t e = hipSetDevice(ctx->device);
+  if (e != hipSuccess) return kdh_hip_fail(ctx, e, "hipSetDevice");
+  void* d = nullptr;
+  e = hipMalloc(&d, src->bytes ? src->bytes : 1);
+  if (e != hipSuccess) return kdh_fail(ctx, KDH_ERR_NOMEM, "kdh_tree_clone: hipMalloc");
+  if (src->bytes) e = hipMemcpyAsync(d, src->dmem, src->bytes, hipMemcpyDeviceToDevice, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  if (e != hipSuccess) {
+    hipFree(d);
+    return kdh_hip_fail(ctx, e, "kdh_tree_clone: copy");
+  }
+  *out = rebased_copy(src, d);
+  return KDH_OK;
+}
+
+int kdh_tree_copy_nodes(kdh_ctx* ctx, const kdh_tree* src, kdh_tree* dst) {
+  if (!ctx) return KDH_ERR_INVALID;
+  if (!src || !dst) return kdh_fail(ctx, KDH_ERR_INVALID, "kdh_tree_copy_nodes: null argument");
+  if (src->device != ctx->device || dst->device != ctx->device)
+    return kdh_fail(ctx, KDH_ERR_INVALID, "tree lives on another device");
+  if (src->dt.n_nodes != dst->dt.n_nodes || src->dt.n_leaves != dst->dt.n_leaves)
+    return kdh_fail(ctx, KDH_ERR_INVALID, "kdh_tree_copy_nodes: size mismatch");
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  hipError_t e = hipSetDevice(ctx->device);
+  if (e != hipSuccess) return kdh_hip_fail(ctx, e, "hipSetDevice");
+  const size_t nb = (size_t)src->dt.n_nodes * 32, lb = (size_t)src->dt.n_leaves * 32;
+  // node and leaf records (the leaf records repeat the leaf nodes' mean and
+  // direction), then the full records when both trees carry them
+  if (nb) e = hipMemcpyAsync((void*)dst->dt.nodes, src->dt.nodes, nb, hipMemcpyDeviceToDevice, ctx->stream);
+  if (e == hipSuccess && lb)
+    e = hipMemcpyAsync((void*)dst->dt.leaves, src->dt.leaves, lb, hipMemcpyDeviceToDevice, ctx->stream);
+  if (e == hipSuccess && nb && src->node_dts && dst->node_dts)
+    e = hipMemcpyAsync(dst->node_dts, src->node_dts, (size_t)src->dt.n_nodes * sizeof(float),
+                       hipMemcpyDeviceToDevice, ctx->stream);
+  if (e == hipSuccess && nb && src->full_nodes && dst->full_nodes)
+    e = hipMemcpyAsync(dst->full_nodes, src->full_nodes, (size_t)src->dt.n_nodes * sizeof(kdh_node3f),
+                       hipMemcpyDeviceToDevice, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  if (e != hipSuccess) return kdh_hip_fail(ctx, e, "kdh_tree_copy_nodes");
+  dst->root_mean = src->root_mean;
   return KDH_OK;
 }

@@ -90,3 +90,82 @@ def test_moved_moving_tree_aligns_to_the_same_world_pose(ctx):
         out.append(al.getX() @ T)
     t, r = synth.pose_error(out[0], out[1])
     assert t < 1e-3 and r < 1e-2
+
+
+def test_clone_is_a_deep_copy(ctx):
+    """TreeCUDA_::clone: the same records, and independent of the source."""
+    t = cfg2()["moving"]
+    D = DeviceTree(ctx, t)
+    B = D.clone()
+    assert B.device_bytes() == D.device_bytes() and B.num_leaves == D.num_leaves
+    assert np.array_equal(B.readNodes(), D.readNodes())
+    assert np.array_equal(B.extractCompressedLeaves(), D.extractCompressedLeaves())
+    D.applyVelocitiesInPlace(V)
+    assert np.array_equal(B.readNodes(), _nodes6(t.nodes))
+    assert not np.array_equal(D.readNodes(), B.readNodes())
+    B.close()
+    D.close()
+
+
+def test_bundle_adjustment_backup_deskew_restore_cycle(ctx):
+    """optimizer_proc_solver_action_.h:39-58 with map/map_.h:88-102: save()
+    clones the keyframe tree; every solver iteration restores the nodes from
+    the backup (copyNodes) and re-deskews them with the current velocities,
+    skipping the leaves.  Each deskew is bit-identical to the oracle's on the
+    original nodes, each restore gives the original back, and an alignment on
+    the restored tree is bit-identical to one on a fresh upload."""
+    w = cfg2()
+    t = w["moving"]
+    D = DeviceTree(ctx, t)
+    backup = D.clone()
+    rng = np.random.default_rng(5)
+    for k in range(3):
+        backup.copyNodes(D)
+        assert np.array_equal(D.readNodes(), _nodes6(t.nodes))
+        v = (V * rng.uniform(0.5, 1.5, 6)).astype(np.float32)
+        D.applyVelocitiesInPlace(v, skip_leaves=True)
+        ref = O.tree_apply_velocities(t.nodes, v, True)
+        assert np.array_equal(D.readNodes(), _nodes6(ref)), k
+    backup.copyNodes(D)
+    F = DeviceTree(ctx, w["fixed"])
+    fresh = DeviceTree(ctx, t)
+    p = icp_params(max_iterations=5, velocity_damping=10.0)
+    out = []
+    for M in (D, fresh):
+        from kd_slam2_amd import CTICPHIP3D
+
+        al = CTICPHIP3D(ctx)
+        al.params = p
+        al.setFixed(F)
+        al.setMoving(M)
+        al.setX(w["X0"])
+        al.setFixedX(w["Xf"])
+        al.setVelocities(w["v0"])
+        al.computeOnDevice()
+        out.append((al.getX(), al.velocities()))
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+    # copyNodes between trees of different shape is refused ("size mismatch")
+    other = DeviceTree(ctx, cfg1()["moving"])
+    with pytest.raises(RuntimeError):
+        backup.copyNodes(other)
+    for T in (D, backup, F, fresh, other):
+        T.close()
+
+
+def test_device_built_full_records_follow_in_place_ops(ctx):
+    """A device-built tree keeps its full Node_ records in step with the
+    compact records through an in-place deskew and transform, and a clone
+    carries them."""
+    from kd_slam2_amd import TreeGenerator3D
+
+    w = cfg2()
+    g = TreeGenerator3D()
+    D = g.makeTreeOnDevice(ctx, w["scan"], w["stamps"])
+    D.applyVelocitiesInPlace(V)
+    D.applyIsometryInPlace(synth.expmap(np.array([3.0, -1.0, 0.5, 0.02, -0.01, 0.3])))
+    full = D.downloadNodes()
+    assert np.array_equal(_nodes6(full), D.readNodes())
+    C2 = D.clone()
+    assert np.array_equal(C2.downloadNodes(), full)
+    C2.close()
+    D.close()
