@@ -354,7 +354,7 @@ def adapter():
         A.kdha_registered_types.argtypes = [C.c_char_p, C.c_int]
         A.kdha_align.restype = C.c_int
         A.kdha_align.argtypes = [C.c_char_p, P, P, P, F64P, F64P, F64P, C.POINTER(Params), C.c_int, C.c_int, F64P,
-                                 F64P, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(Stats)]
+                                 F64P, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(Stats), C.POINTER(Stats)]
         A.kdha_build_dual.restype = C.c_int
         A.kdha_build_dual.argtypes = [P, P, P, F64P, F64P, F64P, F64P, C.POINTER(Params), C.c_int,
                                       C.POINTER(DualForm)]

@@ -522,7 +522,7 @@ namespace {
 template <typename A>
 int run_aligner(A* al, kdh_ctx* ctx, const kdh_tree* fixed, const kdh_tree* moving, const double* X0,
                 const double* v0, const double* Xf, const kdh_params* p, int on_device, int stats_mode,
-                double* X_out, double* v_out, int* result, int* n_iter, kdh_stats* last) {
+                double* X_out, double* v_out, int* result, int* n_iter, kdh_stats* last, kdh_stats* log) {
   al->setContext(ctx);
   al->param_kernel_threshold.setValue(p->kernel_threshold);
   al->param_damping.setValue(p->damping);
@@ -547,19 +547,22 @@ int run_aligner(A* al, kdh_ctx* ctx, const kdh_tree* fixed, const kdh_tree* movi
   memcpy(X_out, al->X, sizeof(al->X));
   if (v_out) memcpy(v_out, al->v, sizeof(al->v));
   *n_iter = (int)al->stats_log.size();
-  if (last) {
-    const ICPStats& s = al->stats;
-    last->num_inliers = s.num_inliers;
-    last->num_outliers = s.num_outliers;
-    last->num_bad = s.num_bad;
-    last->num_err_normal = s.num_err_normal;
-    last->num_err_fn = s.num_err_fn;
-    last->pad = 0;
-    last->chi2 = s.chi2;
-    last->chi2_kernelized = s.chi2_kernelized;
-    last->pert_pose_norm = s.pert_pose_norm;
-    last->pert_vel_norm = s.pert_vel_norm;
-  }
+  auto put = [](kdh_stats* o, const ICPStats& s) {
+    o->num_inliers = s.num_inliers;
+    o->num_outliers = s.num_outliers;
+    o->num_bad = s.num_bad;
+    o->num_err_normal = s.num_err_normal;
+    o->num_err_fn = s.num_err_fn;
+    o->pad = 0;
+    o->chi2 = s.chi2;
+    o->chi2_kernelized = s.chi2_kernelized;
+    o->pert_pose_norm = s.pert_pose_norm;
+    o->pert_vel_norm = s.pert_vel_norm;
+  };
+  if (last) put(last, al->stats);
+  if (log)
+    for (size_t i = 0; i < al->stats_log.size() && (int)i < std::max(p->max_iterations, 1); ++i)
+      put(log + i, al->stats_log[i]);
   return KDH_OK;
 }
 
@@ -580,20 +583,22 @@ int kdha_registered_types(char* buf, int len) {
 
 // Creates `type_name` through the registry, configures it through its PARAM
 // properties and runs compute() (on_device = 0: the plain per-call path) or
-// computeOnDevice() on trees uploaded with kdh_tree_upload.
+// computeOnDevice() on trees uploaded with kdh_tree_upload.  stats_log (may be
+// NULL) receives stats_log, max(max_iterations, 1) entries at most.
 int kdha_align(const char* type_name, kdh_ctx* ctx, const kdh_tree* fixed, const kdh_tree* moving,
                const double X0[12], const double v0[6], const double Xf[12], const kdh_params* p, int on_device,
-               int stats_mode, double X_out[12], double v_out[6], int* result, int* n_iter, kdh_stats* last) {
+               int stats_mode, double X_out[12], double v_out[6], int* result, int* n_iter, kdh_stats* last,
+               kdh_stats* stats_log) {
   if (!type_name || !ctx || !fixed || !moving || !X0 || !Xf || !p || !X_out || !result || !n_iter)
     return KDH_ERR_INVALID;
   try {
     auto obj = srrg2_core::TypeRegistry::instance().create(type_name);
     if (auto* a = dynamic_cast<ICPHIP3D*>(obj.get()))
       return run_aligner(a, ctx, fixed, moving, X0, v0, Xf, p, on_device, stats_mode, X_out, v_out, result, n_iter,
-                         last);
+                         last, stats_log);
     if (auto* c = dynamic_cast<CTICPHIP3D*>(obj.get()))
       return run_aligner(c, ctx, fixed, moving, X0, v0, Xf, p, on_device, stats_mode, X_out, v_out, result, n_iter,
-                         last);
+                         last, stats_log);
     return KDH_ERR_INVALID;
   } catch (const std::exception&) {
     return KDH_ERR_INVALID;

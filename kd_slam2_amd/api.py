@@ -669,10 +669,12 @@ class CTICPHIP3D(_AlignerBase):
             setattr(self, k, f[k])
 
 
-def adapter_align(type_name, ctx, fixed, moving, X0, Xf, params, v0=None, on_device=False, stats_mode=False):
+def adapter_align(type_name, ctx, fixed, moving, X0, Xf, params, v0=None, on_device=False, stats_mode=False,
+                  log=None):
     """Create `type_name` ("ICPHIP3D" / "CTICPHIP3D") through the C++ plugin
     registry and run compute() (plain per-call path) or computeOnDevice().
-    Returns (X 4x4, v, result, iterations, last stats)."""
+    Returns (X 4x4, v, result, iterations, last stats); with a list as `log`
+    it also receives the stats_log entries as dicts."""
     A = L.adapter()
     X0a = np.ascontiguousarray(np.asarray(X0, dtype=np.float64)[:3, :4].reshape(12))
     Xfa = np.ascontiguousarray(np.asarray(Xf, dtype=np.float64)[:3, :4].reshape(12))
@@ -681,11 +683,15 @@ def adapter_align(type_name, ctx, fixed, moving, X0, Xf, params, v0=None, on_dev
     vo = np.zeros(6)
     res, nit = C.c_int(), C.c_int()
     st = L.Stats()
+    cap = max(int(params.max_iterations), 1)
+    lg = (L.Stats * cap)() if log is not None else None
     rc = A.kdha_align(type_name.encode(), ctx.h, fixed.h, moving.h, L.f64p(X0a), L.f64p(v0a), L.f64p(Xfa),
                       C.byref(params), int(on_device), int(stats_mode), L.f64p(Xo), L.f64p(vo), C.byref(res),
-                      C.byref(nit), C.byref(st))
+                      C.byref(nit), C.byref(st), lg)
     if rc != L.KDH_OK:
         raise RuntimeError(f"kdha_align({type_name}) failed: {L.ERRORS.get(rc, rc)}")
+    if log is not None:
+        log.extend(lg[i].as_dict() for i in range(min(nit.value, cap)))
     X = np.eye(4)
     X[:3, :4] = Xo.reshape(3, 4)
     return X, vo, res.value, nit.value, st.as_dict()
