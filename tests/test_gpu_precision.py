@@ -1,7 +1,8 @@
-"""Precision far from the origin (the precision-stress workload at reduced
-size: an outdoor scene whose world frame sits 10 km away, a 2M-point map and
-a ~360k-point 128-beam scan with ranges up to 200 m).  At one FP64 state the
-test compares
+"""Precision far from the origin (the precision-stress workload at full
+size: an outdoor scene whose world frame sits 10 km away, an 8M-point map and
+a ~1.9M-point 128-beam scan with ranges up to 200 m).  Iteration-0
+correspondences and counters are bit-exact against the oracle at the
+reference's FP32 cache.  At one FP64 state the test compares
 
 - naive FP32 (recenter=0: the reference's world-coordinate e and J),
 - re-centred FP32 (recenter=1),
@@ -23,8 +24,8 @@ from .workloads import cfg1, icp_params, rel_err
 
 pytestmark = pytest.mark.gpu
 
-N_COLUMNS = 4096
-N_MAP = 2_000_000
+N_COLUMNS = 22000
+N_MAP = 8_000_000
 
 
 def _scan_block(b):
@@ -37,12 +38,39 @@ def _map_block(b):
 
 @functools.lru_cache(maxsize=None)
 def far_workload():
-    with mp.get_context("fork").Pool(8) as pool:
+    with mp.get_context("fork").Pool(16) as pool:
         scan = np.concatenate(pool.map(_scan_block, range(8)))
         M = np.concatenate(pool.map(_map_block, range(8)))
-    g = TreeGenerator3D(parallel_level=2)
+    g = TreeGenerator3D(parallel_level=3)
     X_gt, X0, Xf = synth.config4_poses()
     return dict(fixed=g.makeTree(M), moving=g.makeTree(scan), X_gt=X_gt, X0=X0, Xf=Xf)
+
+
+def test_correspondences_far_from_origin_bit_exact(ctx):
+    """At the reference's FP32 cache (updateCache, icp_impl_.h:16-21) every
+    moving leaf's fixed leaf, descent depth and outcome counter equals the
+    oracle's, 2M-point scan against the 8M-point map at 10 km."""
+    w = far_workload()
+    assert len(w["moving"].leaves) > 15_000 and len(w["fixed"].nodes) > 700_000
+    F, M = DeviceTree(ctx, w["fixed"]), DeviceTree(ctx, w["moving"])
+    p = icp_params()
+    Xm, Xf, Xmf = update_cache_f32(w["X0"], w["Xf"])
+    ref = O.build_form(w["fixed"], w["moving"], Xm, Xf, Xmf, O.from_kdh_params(p, 8))
+    for rc in (0, 1):
+        al = ICPHIP3D(ctx)
+        al.params = icp_params(recenter=rc)
+        al.setFixed(F)
+        al.setMoving(M)
+        al.setX(w["X0"])
+        al.setFixedX(w["Xf"])
+        leaf, depth = al.correspondences(Xm, Xf, Xmf)
+        assert np.array_equal(leaf, ref["corr"]) and np.array_equal(depth, ref["depth"])
+        al.buildQuadraticForm()
+        for k in ("num_inliers", "num_outliers", "num_bad", "num_err_normal", "num_err_fn"):
+            assert al.stats[k] == ref["stats"][k], (rc, k)
+    print("leaves", len(leaf), "mean depth", depth.mean(), ref["stats"])
+    F.close()
+    M.close()
 
 
 def gn_step(H, b, damping):
@@ -111,6 +139,10 @@ def test_alignment_far_from_origin_against_fp64_loop(ctx):
         print(f"{name:15s} pose vs FP64 loop: {t:.2e} m {r:.2e} deg")
     print(f"FP64 loop vs ground truth: {synth.pose_error(X64, w['X_gt'])}")
     assert d["recentred"][0] < 1e-5 and d["recentred"][1] < 1e-5
+    # and within the pose bar of the FP32 oracle
+    assert d["reference FP32"][0] < 1e-3 and d["reference FP32"][1] < 1e-3
+    t32, r32 = synth.pose_error(out[1], X32)
+    assert t32 < 1e-3 and r32 < 1e-3
 
 
 def test_reference_mode_matches_fp32_oracle_near_origin(ctx):
