@@ -10,11 +10,14 @@
 //      of the tree staged in LDS,
 //   2. evaluates the point-to-plane term (kdh_term.h) into a compact 16-value
 //      record per lane,
-//   3. expands the 34 (ICP) or 97 (CT) accumulator slots from the record in
-//      chunks of 64 and reduces each chunk across the 64 lanes of the wave
-//      with a butterfly reduce-scatter (63 shuffles per chunk instead of
-//      64 x 6), then across the 4 waves through LDS,
-//   4. writes one FP32 partial row per tile to a fixed slot.
+//   3. forms the tile's normal equations on the matrix cores: every
+//      accumulator slot is an entry of C = sum_leaves w w^T over the leaf
+//      vectors w = [sqrt(g) J, sqrt(g) Jv, sqrt(g) e, e], one 16x16 C per
+//      wave from v_mfma_f32_16x16x4f32 with the 64 leaves as the K dimension
+//      (operands transposed through LDS), the outcome counts from ballots,
+//      then the 4 waves' C tiles summed through LDS in a fixed order,
+//   4. writes one FP32 partial row (34 ICP / 97 CT slots) per tile to a
+//      fixed slot.
 // A second kernel, `finish_kernel`, runs one workgroup per alignment: it sums
 // the alignment's partial rows in FP64 in tile order -- the result does not
 // depend on which workgroup ran which tile, nor on what else shares the
