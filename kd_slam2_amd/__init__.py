@@ -14,6 +14,8 @@ from .api import (  # noqa: F401
     ICPHIP3D,
     Server,
     TreeGenerator3D,
+    BatchResult,
+    PairBatch,
     adapter_align,
     batch_align,
     batch_build,

@@ -813,6 +813,66 @@ def batch_build_dual(ctx, factors, stats_mode=False):
                  H_dual_cross_pose_vel_B=Hcp[i], H_dual_cross_vel_AB=Hca[i], stats=stats[i]) for i in range(n)]
 
 
+class BatchResult:
+    """Results of one kdh_batch_align call as arrays: X (n, 4, 4), v (n, 6),
+    iterations (n,), result (n,) (compute()'s return value), last / entry
+    (structured arrays of kdh_stats: the last round's statistics and, for
+    fixed-forest entries, the entry's share).  Indexing gives pair i as a
+    dict (X, v, iterations, result, last, entry)."""
+
+    def __init__(self, R):
+        n = len(R)
+        self.X = np.tile(np.eye(4), (n, 1, 1))
+        self.X[:, :3, :4] = R["X"].reshape(n, 3, 4)
+        self.v = R["v"].copy()
+        self.iterations = R["iterations"].copy()
+        self.result = R["result"].copy()
+        self.last = R["last"].copy()
+        self.entry = R["entry"].copy()
+
+    def __len__(self):
+        return len(self.X)
+
+    def __getitem__(self, i):
+        return dict(X=self.X[i], v=self.v[i], iterations=int(self.iterations[i]), result=int(self.result[i]),
+                    last=_stats_dicts(self.last[i:i + 1])[0], entry=_stats_dicts(self.entry[i:i + 1])[0])
+
+    def __iter__(self):
+        last, entry = _stats_dicts(self.last), _stats_dicts(self.entry)
+        its, rs = self.iterations.tolist(), self.result.tolist()
+        for i in range(len(self.X)):
+            yield dict(X=self.X[i], v=self.v[i], iterations=its[i], result=rs[i], last=last[i], entry=entry[i])
+
+
+class PairBatch:
+    """A batch of alignments marshalled once into the kdh_pair array: trees,
+    fixed poses, parameters, priors and forests (the `batch_align` pair
+    dicts).  set_initial() replaces the initial poses / velocities of every
+    pair (one array assignment), run() calls kdh_batch_align and returns a
+    BatchResult.  The serving loop's shape: a resident set of alignments
+    whose initial states change from call to call."""
+
+    def __init__(self, pairs):
+        global _PAIR_DT, _RES_DT
+        self.n = len(pairs)
+        self._arr = _pair_array(pairs)
+        self._A = np.frombuffer(self._arr, dtype=_PAIR_DT)
+        self._res = (L.PairResult * max(self.n, 1))()
+        self._R = np.frombuffer(self._res, dtype=_RES_DT)[:self.n]
+        self._keep = list(pairs)  # the trees and priors stay alive with the batch
+
+    def set_initial(self, X0=None, v0=None):
+        """X0: (n, 4, 4) or (n, 3, 4) poses; v0: (n, 6) velocities."""
+        if X0 is not None:
+            self._A["X0"] = np.asarray(X0, dtype=np.float64)[:, :3, :4].reshape(self.n, 12)
+        if v0 is not None:
+            self._A["v0"] = np.asarray(v0, dtype=np.float64).reshape(self.n, 6)
+
+    def run(self, ctx):
+        ctx.check(L.lib().kdh_batch_align(ctx.h, self.n, self._arr, self._res), "kdh_batch_align")
+        return BatchResult(self._R)
+
+
 def batch_align(ctx, pairs):
     """Run many alignments in one batched launch (kdh_batch_align).
 
@@ -820,23 +880,10 @@ def batch_align(ctx, pairs):
     params (L.Params), prior, forest (k > 0: this pair leads a fixed forest
     with the next k pairs).  Returns a list of dicts (X, v, iterations,
     result, last, entry).  The C structs are filled and read through numpy
-    views, column by column."""
-    n = len(pairs)
-    arr = _pair_array(pairs)
-    res = (L.PairResult * n)()
-    ctx.check(L.lib().kdh_batch_align(ctx.h, n, arr, res), "kdh_batch_align")
-    R = np.frombuffer(res, dtype=_RES_DT)
-    Xs = np.tile(np.eye(4), (n, 1, 1))
-    Xs[:, :3, :4] = R["X"].reshape(n, 3, 4)
-    vs = R["v"].copy()
-    its, rs = R["iterations"].tolist(), R["result"].tolist()
-    last = _stats_dicts(R["last"])
-    # a pair outside a forest has entry == last (kdh_pair_result.entry)
-    if R["entry"].tobytes() == R["last"].tobytes():
-        entry = [dict(d) for d in last]
-    else:
-        entry = _stats_dicts(R["entry"])
-    return [dict(X=Xs[i], v=vs[i], iterations=its[i], result=rs[i], last=last[i], entry=entry[i]) for i in range(n)]
+    views, column by column (PairBatch keeps them between calls)."""
+    if len(pairs) == 0:
+        return []
+    return list(PairBatch(pairs).run(ctx))
 
 
 # ---------------------------------------------------------------------------

@@ -123,8 +123,11 @@ static int batch_setup(kdh_ctx* ctx, int n_pairs, const kdh_pair* pairs, int mod
   }
   rc = kdh_upload_tile_pair(ctx, tp);
   if (rc) return rc;
-  KDH_HIP_TRY(ctx, hipMemcpyAsync(ctx->d_pairs, ctx->h_pairs, sizeof(PairState) * n_pairs, hipMemcpyHostToDevice,
-                                  ctx->stream));
+  // Only the inputs and the evolving state go up (offsetof(stats_log) of each
+  // ~6 KB record): the statistics log, H, b and the dual blocks are outputs
+  // the device writes before anything reads them.
+  KDH_HIP_TRY(ctx, hipMemcpy2DAsync(ctx->d_pairs, sizeof(PairState), ctx->h_pairs, sizeof(PairState),
+                                    kStateBytes, (size_t)n_pairs, hipMemcpyHostToDevice, ctx->stream));
   return KDH_OK;
 }
 

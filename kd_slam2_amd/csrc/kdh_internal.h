@@ -15,6 +15,8 @@
 // _idx_start/_idx_end and _dts_covariance, which the alignment never reads.
 #pragma once
 
+#include <stddef.h>
+
 #include "kdh_math.h"
 #include "kdh_term.h"
 
@@ -92,6 +94,10 @@ struct PairState {
   // H_cross_pose_vel_B 36, H_cross_vel_AB 36
   double Hd[114];
 };
+
+// Bytes of a PairState the host fills (inputs and evolving state, up to the
+// statistics log); the rest is written by the device before it is read.
+constexpr size_t kStateBytes = (offsetof(PairState, stats_log) + 15) & ~(size_t)15;
 
 // Re-centring anchor of a state (the TermCache fields mroot and cf, and the
 // FP64 anchor the solver maps the form back with): c = Xm * mroot in FP64,

@@ -627,24 +627,23 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(DUAL ?
 
   // Tile schedule.  Workgroups are dispatched round-robin over the 8 XCDs of
   // the chip, each with its own L2; with xcd_remap the XCD of workgroup b
-  // (b % 8) walks a contiguous eighth of the tile list, so consecutive tiles of
-  // one alignment -- which read the same fixed tree -- share an L2.
+  // (b % 8) takes a contiguous eighth of the tile list, so consecutive tiles
+  // of one alignment -- which read the same fixed tree -- share an L2.
+  // Within its range (the whole list without remap) every workgroup takes
+  // one contiguous chunk of tiles: a persistent grid smaller than the tile
+  // count then walks consecutive tiles of one alignment and stages the top
+  // of that alignment's tree in LDS once, not once per tile.
   const int G = gridDim.x;
   const bool remap = xcd_remap && (G % 8) == 0 && G >= 8;
-  const int per_xcd = remap ? (n_tiles_total + 7) / 8 : 0;
-  const int xcd = blockIdx.x & 7;
-  const int j0 = remap ? (int)(blockIdx.x >> 3) : (int)blockIdx.x;
-  const int stride = remap ? G / 8 : G;
-  for (int k = j0;; k += stride) {
-    int t;
-    if (remap) {
-      if (k >= per_xcd) break;
-      t = xcd * per_xcd + k;
-      if (t >= n_tiles_total) break;
-    } else {
-      if (k >= n_tiles_total) break;
-      t = k;
-    }
+  const int lo_all = remap ? (int)(blockIdx.x & 7) * ((n_tiles_total + 7) / 8) : 0;
+  const int hi_all = remap ? min(lo_all + (n_tiles_total + 7) / 8, n_tiles_total) : n_tiles_total;
+  const int n_wg = remap ? G / 8 : G;
+  const int j = remap ? (int)(blockIdx.x >> 3) : (int)blockIdx.x;
+  const int span = hi_all - lo_all;
+  // chunk j of n_wg over `span` tiles: sizes differ by at most one
+  const int c_lo = lo_all + (int)(((long long)span * j) / n_wg);
+  const int c_hi = lo_all + (int)(((long long)span * (j + 1)) / n_wg);
+  for (int t = c_lo; t < c_hi; ++t) {
     // Three memory round trips before the descent: the tile's pair, the pair
     // fields, then the leaf records together with the top-of-tree staging.
     const int p = tile_pair ? tile_pair[t] : 0;

@@ -75,12 +75,13 @@ struct TermCache {
 
 // Greedy descent (icp_impl_.h:116-121, node_.h:54-63).  Node record layout:
 //   a = (mean.x, mean.y, mean.z, dir.x), b = (dir.y, dir.z, left, right)
-// with the child links stored as int bits.  Returns the leaf index or -1.
+// with the child links stored as int bits.  Returns the leaf index or -1;
+// (a, b) end as the record of the last node fetched, which is the leaf's
+// when one is reached (no second fetch of it).
 template <typename NodeFetch>
-KDH_HD int descend(const NodeFetch& fetch, V3f q, int* depth_out) {
+KDH_HD int descend(const NodeFetch& fetch, V3f q, int* depth_out, float4& a, float4& b) {
   int idx = 0;
   int depth = 0;
-  float4 a, b;
   fetch(0, a, b);
   for (;;) {
 #if defined(__HIP_DEVICE_COMPILE__)
@@ -253,13 +254,12 @@ template <typename NodeFetch>
 KDH_HD int icp_term(const NodeFetch& fetch, const TermCache& C, const TermParams& P, const LeafIn& L, TermRec& R,
                     int* depth_out) {
   const V3f p_mif = apply_iso(C.Xmf, L.mean);
-  const int leaf = descend(fetch, p_mif, depth_out);
+  float4 a, b;
+  const int leaf = descend(fetch, p_mif, depth_out, a, b);
   if (leaf < 0) {
     rec_reject(R, kBad);
     return -1;
   }
-  float4 a, b;
-  fetch(leaf, a, b);
   const V3f pf = v3(a.x, a.y, a.z);
   const V3f nf = v3(a.w, b.x, b.y);
   const V3f p_m = apply_iso(C.Xm, L.mean);
@@ -309,13 +309,12 @@ KDH_HD int ct_term(const NodeFetch& fetch, const TermCache& C, const TermParams&
   const V3f wmean = mul(rd.R, L.mean) + t_delta;
   const V3f wdir = mul(rd.R, L.dir);
   const V3f p_mif = apply_iso(C.Xmf, wmean);
-  const int leaf = descend(fetch, p_mif, depth_out);
+  float4 a, b;
+  const int leaf = descend(fetch, p_mif, depth_out, a, b);
   if (leaf < 0) {
     rec_reject(R, kBad);
     return -1;
   }
-  float4 a, b;
-  fetch(leaf, a, b);
   const V3f pf = v3(a.x, a.y, a.z);
   const V3f nf = v3(a.w, b.x, b.y);
   const V3f p_m = apply_iso(C.Xm, wmean);
