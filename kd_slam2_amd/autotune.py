@@ -1,36 +1,46 @@
 """Project autotuner for the tile kernel's launch shape.
 
-Sweeps the knobs of kdh_set_launch_config -- the number of top-of-tree nodes
-staged in LDS (0/256/512/1024, plus 2048 for rigid ICP) and the workgroup
-count (one per tile, or a persistent grid of k workgroups per CU) -- on a
-representative batch, timing each candidate with HIP events around the tile
-kernel (profiling mode), and keeps the fastest.  The tile size is fixed at
-256 moving leaves (one workgroup pass); the leaf size is a property of the
-trees, chosen when they are built.
+Sweeps the knobs of the tile kernel on a representative batch and keeps the
+fastest, timing each candidate with HIP events around the tile kernel
+(profiling mode):
+
+* leaves per lane -- 1 (256-thread workgroups, one descent per lane) or 2
+  (128-thread workgroups, two descents in flight per lane): the latency /
+  occupancy trade of the greedy descent (kdh_set_leaves_per_lane);
+* the number of top-of-tree nodes staged in LDS per workgroup
+  (0/256/512/1024, plus 2048 for rigid ICP);
+* the workgroup count: one per tile, or a persistent grid of k workgroups
+  per CU walking contiguous tile chunks.
+
+A tile is 256 moving leaves in every shape; the leaf size is a property of
+the trees, chosen when they are built (TreeGenerator3D leaf_max_points).
 """
 import numpy as np
 
 N_CU = 256
 LDS_CHOICES = (0, 256, 512, 1024, 2048)
+LPL_CHOICES = (1, 2)
 
 
-def tune(ctx, pairs, batch_align, ct, repeats=2, grids=(0, N_CU * 2, N_CU * 4, N_CU * 8)):
+def tune(ctx, pairs, batch_align, ct, repeats=2, grids=(0, N_CU * 2, N_CU * 4, N_CU * 8), lds=LDS_CHOICES,
+         leaves_per_lane=LPL_CHOICES):
     """Returns (best config dict, all results).  Leaves the best config set."""
     results = []
     ctx.set_profiling(True)
     try:
-        for lds in LDS_CHOICES:
-            if ct and lds > 1024:
-                continue
-            for grid in grids:
-                ctx.set_launch_config(grid=grid, lds_nodes=lds)
-                t = []
-                for _ in range(repeats):
-                    batch_align(ctx, pairs)
-                    t.append(ctx.kernel_breakdown()[0])
-                results.append(dict(lds_nodes=lds, grid=grid, tile_us=float(np.median(t))))
+        for lpl in leaves_per_lane:
+            for nodes in lds:
+                if ct and nodes > 1024:
+                    continue
+                for grid in grids:
+                    ctx.set_launch_config(grid=grid, lds_nodes=nodes, leaves_per_lane=lpl)
+                    t = []
+                    for _ in range(repeats):
+                        batch_align(ctx, pairs)
+                        t.append(ctx.kernel_breakdown()[0])
+                    results.append(dict(leaves_per_lane=lpl, lds_nodes=nodes, grid=grid, tile_us=float(np.median(t))))
     finally:
         ctx.set_profiling(False)
     best = min(results, key=lambda r: r["tile_us"])
-    ctx.set_launch_config(grid=best["grid"], lds_nodes=best["lds_nodes"])
+    ctx.set_launch_config(grid=best["grid"], lds_nodes=best["lds_nodes"], leaves_per_lane=best["leaves_per_lane"])
     return best, results
