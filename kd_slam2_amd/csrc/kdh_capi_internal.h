@@ -61,7 +61,8 @@ struct kdh_ctx {
   GraphSlot graph_align, graph_batch;
   // launch configuration (autotuner)
   int grid_override = 0;
-  int lds_nodes = 1024;
+  int lds_nodes = 256;  // measured: deeper staging costs more per tile than its LDS hits save
+  int leaves_per_lane = 1;  // moving leaves per lane of the tile kernel: 1 or 2
   int xcd_remap = 1;
   // timing of the last device loop
   static constexpr int kEvents = 2 * kdh::kMaxIter + 2;
