@@ -1,33 +1,44 @@
 """Benchmark of the MI355X scan-alignment path (one JSON line on stdout).
 
-Headline workload (`value`): CT-ICP with deskew, scan to map -- a 128-beam,
-0.1 s sweep of 131,072 points (per-point stamps, 10 m/s forward, 30 deg/s
-yaw) against a 2,000,000-point outdoor map, 12-DoF pose + velocity, 15
-Gauss-Newton iterations, leaf policy defaults.
+Headline workload (`value`): batched loop-closure verification -- 256
+independent rigid point-to-plane ICP pairs, each scan and map from the
+room-scene generator with its own seed (3000 + i), scans log-uniform in
+4k-120k points (8.5M in all) and maps 3x their scan, every map with its own
+KD-tree (leaf size 16), initial errors up to 5 deg / 1 m, 10 Gauss-Newton
+iterations, max_error 0.5.  Trees are resident in HBM before timing starts.
 
-A step is one served batch: `--batch` concurrent alignments of that workload
-(the map resident in HBM, each alignment with its own initial pose and
-velocity) run to completion by kdh_batch_align -- one tile launch and one
-finish launch per Gauss-Newton iteration, all iterations in one HIP graph.
-value = scan points of completed alignments per second over the timed steps
-(all iterations included; inputs resident in HBM before timing starts).
+A step is one kdh_batch_align call over the 256 pairs (a PairBatch: the pair
+array is marshalled once; each step sets a new set of initial poses): one
+tile launch and one finish launch per iteration, all iterations in one HIP
+graph, results back on the host.  value = scan points of completed
+alignments per second over the timed steps.  The launch shape (leaves per
+lane, LDS staging depth, grid) is picked first by the project autotuner on
+the same batch.
 
 Beside it the line reports
-  * the plain one-alignment-per-call path (CTICPHIP3D::compute from the C++
-    plugin: host Gauss-Newton loop, one kernel build + host FP64 solve per
-    iteration) and the device-resident single-alignment loop;
+  * the plain one-alignment-per-call path on the same 256 pairs
+    (ICPHIP3D::compute from the C++ plugin, one kernel build + host FP64
+    solve per iteration) and the batched form's speedup over it;
+  * the batched poses against the CPU oracle (max over the 256 pairs) and the
+    H/b relative error against the oracle's FP64 twin on a sample of pairs;
+  * the tile kernel's roofline: HIP events around it on the context stream,
+    and its traffic to the memory fabric from two rocprofv3 --pmc passes
+    (FETCH_SIZE, WRITE_SIZE) scaled by the same counters on a device copy of
+    known size; beside it the cache-inclusive demand bytes (every node visit
+    at 32 B);
+  * CT-ICP served batches (a 131k-point 0.1 s sweep against a 2M-point map,
+    15 iterations), the single-alignment device loop and the plain path;
   * the serving mix: 4 odometry sequences at 10 Hz (64k-point CT sweeps
     against their own resident 4M-point maps) and loop-closure requests at
-    20/s (rigid, 8k-200k points, map 3x), Poisson arrivals into the
-    continuous-batching server with a 64 GB budget, p50/p99 latency from
-    arrival to host-visible result;
-  * the tile kernel's roofline (HIP events on the context stream) and its
-    HBM traffic from two rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE)
-    scaled by the same counters on a device copy of known size;
-  * the precision stress: a 2M-point scan against an 8M-point map with the
-    world frame 10 km away -- naive FP32, re-centred FP32 and the reference's
-    FP32 form against the oracle's FP64 twin and FP64 loop;
-  * the CPU oracle (restated ICP_CPU_ / CTICP_CPU_) on 8 host threads.
+    20/s (8k-200k points), Poisson arrivals into the continuous-batching
+    server with a 64 GB budget, p50/p99 latency from arrival to host-visible
+    result, and the highest arrival-rate multiple it sustains with odometry
+    p99 under 10 ms;
+  * the precision stress (2M-point scan, 8M-point map, 10 km from the
+    origin), device tree generation, tree maintenance, batched factor forms
+    and the KDTree3f codec;
+  * the CPU oracle (restated ICP_CPU_) on the headline workload, 8 threads
+    (the reference's cap) and 1 thread, with the host CPU model.
 """
 import argparse
 import csv
@@ -59,11 +70,26 @@ def log(*a):
 
 
 def algorithmic_bytes_per_iter(depth, ct, n_tiles):
-    """Per Gauss-Newton iteration: per query 24 B moving mean+normal, 4 B dts
-    (CT), 4 B leaf index, 32 B per internal node visited, 24 B fixed leaf;
-    plus one partial row (27 or 90 floats) per tile."""
+    """Cache-inclusive demand bytes of one Gauss-Newton iteration: per query
+    24 B moving mean+normal, 4 B dts (CT), 4 B leaf index, 32 B per internal
+    node visited, 24 B fixed leaf; plus one partial row (27 or 90 floats) per
+    tile.  Every visit is counted at 32 B whether LDS, L2 or HBM serves it."""
     per_q = 24 + (4 if ct else 0) + 4 + 24
     return float(per_q * len(depth) + 32 * int(np.sum(depth)) + n_tiles * 4 * (90 if ct else 27))
+
+
+def host_cpu():
+    """(model name, logical CPUs) of this host."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return model, os.cpu_count() or 0
 
 
 # ---------------------------------------------------------------------------
@@ -104,7 +130,9 @@ def _gen_seq(s):
 
 
 def _gen_pair3(i_n):
-    """Config 3 pair i: config-1 generator, seed 3000+i, scan n, map 3n."""
+    """Loop-closure pair i of the headline batch: room-scene generator, seed
+    3000+i, scan n, map 3n; plus three more initial poses (seeds 7000+i) for
+    the timed steps."""
     from kd_slam2_amd import TreeGenerator3D, synth
 
     i, n = i_n
@@ -113,6 +141,8 @@ def _gen_pair3(i_n):
     w["fixed"] = g.makeTree(w["map"])
     w["moving"] = g.makeTree(w["scan"])
     w["n_pts"] = len(w["scan"])
+    rng = np.random.default_rng(np.random.PCG64(7000 + i))
+    w["X0_steps"] = [w["X0"]] + [synth.perturb(w["X_gt"], rng, 5.0, 1.0, uniform=True) for _ in range(3)]
     del w["map"], w["scan"], w["stamps"]
     return w
 
@@ -138,6 +168,20 @@ def _gen_cfg4_block(kb):
     return synth.config4_scan_block(b) if kind == 0 else synth.config4_map_block(b)
 
 
+def _gen_pairs3(n_pairs, pool):
+    from kd_slam2_amd import synth
+
+    sizes = synth.config3_sizes(256)[:n_pairs]
+    return pool.map_async(_gen_pair3, list(enumerate(sizes)))
+
+
+def _p3_params(default_params):
+    p = default_params()
+    p.max_error = 0.5
+    p.max_iterations = 10
+    return p
+
+
 # ---------------------------------------------------------------------------
 # PMC traffic: rocprofv3 --pmc passes over a child run of this script
 
@@ -151,18 +195,19 @@ def _hip():
 
 
 def pmc_child(args):
-    """One served batch of the headline workload plus a device copy of
-    COPY_BYTES bytes, for the counter passes."""
-    from kd_slam2_amd import Context, DeviceTree, batch_align, default_params, synth
+    """One batch of the headline workload at the tuned launch shape plus a
+    device copy of COPY_BYTES bytes, for the counter passes."""
+    from kd_slam2_amd import Context, DeviceTree, PairBatch, default_params
 
-    w = _gen_cfg2(0)
+    with mp.get_context("fork").Pool(min(16, os.cpu_count() or 4)) as pool:
+        pairs3 = _gen_pairs3(args.pairs3, pool).get()
     ctx = Context(0)
-    F, M = DeviceTree(ctx, w["fixed"]), DeviceTree(ctx, w["moving"])
-    p = _cfg2_params(default_params)
-    rng = np.random.default_rng(np.random.PCG64(99))
-    pairs = [dict(fixed=F, moving=M, ct=1, X0=synth.perturb(w["X_gt"], rng, 1.0, 0.3), Xf=w["Xf"], v0=w["v0"],
-                  params=p) for _ in range(args.batch)]
-    batch_align(ctx, pairs)
+    ctx.set_launch_config(grid=args.pmc_grid, lds_nodes=args.pmc_lds, leaves_per_lane=args.pmc_lpl)
+    p = _p3_params(default_params)
+    trees = [(DeviceTree(ctx, q["fixed"]), DeviceTree(ctx, q["moving"])) for q in pairs3]
+    B = PairBatch([dict(fixed=F, moving=M, ct=0, X0=q["X0"], Xf=q["Xf"], params=p)
+                   for q, (F, M) in zip(pairs3, trees)])
+    B.run(ctx)
     hip = _hip()
     a, b = ctypes.c_void_p(), ctypes.c_void_p()
     hip.hipMalloc(ctypes.byref(a), COPY_BYTES)
@@ -173,15 +218,18 @@ def pmc_child(args):
     print("pmc child done", flush=True)
 
 
-def pmc_pass(counter, batch, tiles_per_alignment):
-    """Per-dispatch counter (KiB) for the served tile kernel and for the copy
-    kernel, from one rocprofv3 --pmc run.  Returns (tile, copy) or None."""
+def pmc_pass(counter, args, tuned):
+    """Per-dispatch counter (KiB) of the headline tile kernel (median over its
+    dispatches) and of the copy kernel, from one rocprofv3 --pmc run.
+    Returns (tile, copy) or None."""
     exe = shutil.which("rocprofv3")
     if not exe:
         return None
     out = tempfile.mkdtemp(prefix="kdh_pmc_")
     cmd = ["timeout", "-s", "KILL", "150", exe, "--pmc", counter, "-d", out, "-o", "pmc", "--output-format", "csv",
-           "--", sys.executable, os.path.abspath(__file__), "--pmc-child", "--batch", str(batch)]
+           "--", sys.executable, os.path.abspath(__file__), "--pmc-child", "--pairs3", str(args.pairs3),
+           "--pmc-grid", str(tuned["grid"]), "--pmc-lds", str(tuned["lds_nodes"]),
+           "--pmc-lpl", str(tuned["leaves_per_lane"])]
     try:
         r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, timeout=170, cwd=ROOT,
                            env=dict(os.environ, TMPDIR="/tmp"))
@@ -197,7 +245,7 @@ def pmc_pass(counter, batch, tiles_per_alignment):
                 name = row.get("Kernel_Name", "")
                 grid = int(row.get("Grid_Size", row.get("Grid_Size_X", "0")) or 0)
                 v = float(row["Counter_Value"])
-                if "tiles_kernel" in name and grid == 256 * batch * tiles_per_alignment:
+                if "tiles_kernel" in name:
                     tile.append(v)
                 elif "copyBuffer" in name and grid > 1 << 16:
                     copy.append(v)
@@ -228,20 +276,24 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--batch", type=int, default=32, help="concurrent alignments per served step")
+    ap.add_argument("--pairs3", type=int, default=256, help="loop-closure pairs of the headline batch")
+    ap.add_argument("--batch", type=int, default=32, help="concurrent CT alignments per served CT step")
     ap.add_argument("--plain-runs", type=int, default=5)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--serve-seconds", type=float, default=3.0, help="length of the serving-mix arrival window")
     ap.add_argument("--no-serve", action="store_true")
-    ap.add_argument("--pairs3", type=int, default=256, help="loop-closure pairs of the batched config (0: skip)")
     ap.add_argument("--no-pmc", action="store_true")
     ap.add_argument("--no-precision", action="store_true", help="skip the 10 km precision-stress workload")
     ap.add_argument("--no-tree-build", action="store_true", help="skip the tree-generation timings")
     ap.add_argument("--no-next-rows", action="store_true", help="skip tree maintenance, factor forms, KDTree3f")
+    ap.add_argument("--headline-only", action="store_true", help="only the headline batch and its roofline")
     ap.add_argument("--tree-build-only", action="store_true",
                     help="only the tree-generation timings (for kernel traces of the builder)")
     ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--pmc-grid", type=int, default=0, help=argparse.SUPPRESS)
+    ap.add_argument("--pmc-lds", type=int, default=256, help=argparse.SUPPRESS)
+    ap.add_argument("--pmc-lpl", type=int, default=1, help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.pmc_child:
         pmc_child(args)
@@ -250,6 +302,8 @@ def main():
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); alignments shard by pair with no "
               "communication across GPUs: run one process per GPU, each on its own share of the pairs", file=sys.stderr)
         sys.exit(2)
+    if args.headline_only:
+        args.no_serve = args.no_precision = args.no_tree_build = args.no_next_rows = True
 
     if args.tree_build_only:
         from kd_slam2_amd import Context, DeviceTree, TreeGenerator3D, default_params, synth
@@ -262,26 +316,20 @@ def main():
         return
 
     t0 = time.time()
-    n_loop_pool = 16
     with mp.get_context("fork").Pool(min(16, os.cpu_count() or 4)) as pool:
-        j_cfg2 = pool.apply_async(_gen_cfg2, (0,))
+        j_p3 = _gen_pairs3(args.pairs3, pool)
+        j_cfg2 = None if args.headline_only else pool.apply_async(_gen_cfg2, (0,))
         j_sw5 = None if args.no_tree_build else pool.apply_async(_gen_sweep5, (0,))
         j_seq = None if args.no_serve else pool.map_async(_gen_seq, range(4))
-        j_loop = None if args.no_serve else pool.map_async(_gen_loop, range(n_loop_pool))
-        j_p3 = None
-        if args.pairs3:
-            from kd_slam2_amd import synth as _s
-
-            sizes = _s.config3_sizes(args.pairs3)
-            j_p3 = pool.map_async(_gen_pair3, list(enumerate(sizes)))
+        j_loop = None if args.no_serve else pool.map_async(_gen_loop, range(16))
         j_c4 = None if args.no_precision else pool.map_async(_gen_cfg4_block, [(k, b) for k in (0, 1)
                                                                                 for b in range(8)])
-        w = j_cfg2.get()
+        pairs3 = j_p3.get()
+        w = j_cfg2.get() if j_cfg2 else None
         sweep5 = j_sw5.get() if j_sw5 else None
         c4_blocks = j_c4.get() if j_c4 else None
         seqs = j_seq.get() if j_seq else []
         loops = j_loop.get() if j_loop else []
-        pairs3 = j_p3.get() if j_p3 else []
     w4 = None
     if c4_blocks is not None:
         from kd_slam2_amd import TreeGenerator3D, synth as _s
@@ -292,18 +340,264 @@ def main():
         w4 = dict(n_pts=len(scan4), moving=g4.makeTree(scan4), fixed=g4.makeTree(np.concatenate(c4_blocks[8:])),
                   X_gt=X_gt4, X0=X04, Xf=Xf4)
         del scan4, c4_blocks
-    fixed, moving = w["fixed"], w["moving"]
-    n_pts = 131_072
-    log(f"workloads ready in {time.time() - t0:.1f}s: fixed {fixed.num_nodes} nodes, moving {moving.num_leaves} "
-        f"leaves")
+    log(f"workloads ready in {time.time() - t0:.1f}s")
 
     from kd_slam2_amd import (CTICPHIP3D, Context, DeviceTree, Server, adapter_align, batch_align,
                               default_params, synth)
 
     ctx = Context(0)
-    F = DeviceTree(ctx, fixed)
-    M = DeviceTree(ctx, moving)
+
+    # ---- headline: 256-pair batched loop-closure verification --------------
+    head = run_headline(ctx, pairs3, args)
+    ctx.set_launch_config()  # the defaults again for the other rows
+
+    # ---- CT-ICP: served batches, device loop, plain path ---------------------
+    ct = None
+    F = M = None
+    if w is not None:
+        F = DeviceTree(ctx, w["fixed"])
+        M = DeviceTree(ctx, w["moving"])
+        ct = run_ct(ctx, w, F, M, args, CTICPHIP3D, adapter_align, batch_align, default_params, synth)
+
+    # ---- serving mix -------------------------------------------------------
+    serving = None
+    if not args.no_serve:
+        serving = run_serving(ctx, seqs, loops, args.serve_seconds, DeviceTree, Server, default_params)
+        serving["capacity"] = run_capacity(ctx, seqs, loops, DeviceTree, Server, default_params)
+
+    # ---- precision stress (config 4) ---------------------------------------
+    precision = None
+    if w4 is not None:
+        precision = run_precision(ctx, w4, DeviceTree, default_params, synth)
+
+    # ---- tree generation on the device (makeTree + upload) ------------------
+    tree_build = None
+    if not args.no_tree_build and w is not None:
+        tree_build = run_tree_build(ctx, w, sweep5, DeviceTree, F, _cfg2_params(default_params))
+    if w is not None:
+        w.pop("map", None)
+
+    # ---- the other rows around the path: tree maintenance, batched factor
+    # forms, the KDTree3f wire and bag format -----------------------------
+    next_rows = None
+    if not args.no_next_rows and w is not None:
+        next_rows = run_next_rows(ctx, w, DeviceTree, F, M, _cfg2_params(default_params))
+
+    # ---- CPU baseline: the oracle on the headline workload --------------------
+    cpu = None
+    if not args.no_cpu:
+        cpu = run_cpu_baseline(pairs3, args, default_params)
+
+    # ---- fabric traffic of the headline tile kernel ---------------------------
+    traffic = None
+    pmc = {}
+    if not args.no_pmc:
+        f = pmc_pass("FETCH_SIZE", args, head["tuned"])
+        wr = pmc_pass("WRITE_SIZE", args, head["tuned"])
+        if f and wr:
+            kib = COPY_BYTES / 1024.0
+            f_scale = kib / f[1]
+            w_scale = kib / wr[1]
+            traffic = round((f[0] * f_scale + wr[0] * w_scale) * 1024.0, 1)
+            pmc = dict(fetch_kib=round(f[0], 1), write_kib=round(wr[0], 1), copy_fetch_kib=round(f[1], 1),
+                       copy_write_kib=round(wr[1], 1), fetch_scale=round(f_scale, 4), write_scale=round(w_scale, 4))
+            log(f"pmc: tile fetch {f[0]:.0f} KiB, write {wr[0]:.0f} KiB per launch (copy {f[1]:.0f}/{wr[1]:.0f} "
+                f"KiB for {kib:.0f} KiB)")
+
+    tile_us = head["tile_us"]
+    demand = head["demand_bytes_per_launch"]
+    achieved = traffic / (tile_us * 1e-6) / 1e9 if traffic else None
+    roofline = dict(bound="hbm", achieved=round(achieved, 1) if achieved else None, peak=HBM_PEAK_GBS, unit="GB/s",
+                    frac=round(achieved / HBM_PEAK_GBS, 4) if achieved else None, traffic=traffic,
+                    kernel="tiles_kernel (rigid, 256 pairs)", kernel_us=round(tile_us, 2),
+                    cache_inclusive=dict(demand_bytes=demand,
+                                         achieved=round(demand / (tile_us * 1e-6) / 1e9, 1),
+                                         frac=round(demand / (tile_us * 1e-6) / 1e9 / HBM_PEAK_GBS, 4)))
+    if achieved is not None:
+        assert roofline["frac"] <= 1.0, roofline
+    line = dict(
+        metric="scan-pts aligned/s + p99 align latency, served vs plain HIP, <=1mm/1e-3deg",
+        value=head["value"],
+        unit="scan-pts/s",
+        n_gpus=1,
+        steps=args.steps,
+        warmup=args.warmup,
+        ms_per_step=head["ms_per_step"],
+        higher_is_better=True,
+        scaling="weak",
+        vs_baseline=None,
+        dtype="f32",
+        data="synthetic",
+        config=dict(workload=f"batched loop-closure verification: {head['pairs']} rigid ICP pairs, scans "
+                             f"log-uniform 4k-120k pts ({head['scan_points']} in all), maps 3x, 10 iterations",
+                    pairs=head["pairs"], scan_points=head["scan_points"], moving_leaves=head["moving_leaves"],
+                    fixed_nodes=head["fixed_nodes"], mean_depth=head["mean_depth"], tuned=head["tuned"]),
+        latency_ms=head["latency_ms"],
+        served_speedup_vs_plain=head["speedup_vs_plain"],
+        plain_pts_per_s=head["plain_pts_per_s"],
+        accuracy=head["accuracy"],
+        kernel_us_per_iter=head["kernel_us_per_iter"],
+        pmc=pmc or None,
+        ct_icp=ct,
+        serving_mix=serving,
+        precision_stress=precision,
+        tree_build=tree_build,
+        next_rows=next_rows,
+        roofline=roofline,
+        cpu_baseline=cpu,
+    )
+    print(json.dumps(line), flush=True)
+
+
+def run_headline(ctx, pairs3, args):
+    """The 256-pair batch: autotuned launch shape, W + K PairBatch steps,
+    the plain per-call path on the same pairs, accuracy against the oracle,
+    tile / finish kernel times and the cache-inclusive demand bytes."""
+    from kd_slam2_amd import DeviceTree, ICPHIP3D, PairBatch, adapter_align, autotune, batch_align, default_params
+    from kd_slam2_amd import synth
+    from oracle import pyoracle as O
+
+    p = _p3_params(default_params)
+    trees = [(DeviceTree(ctx, q["fixed"]), DeviceTree(ctx, q["moving"])) for q in pairs3]
+    pairs = [dict(fixed=F, moving=M, ct=0, X0=q["X0"], Xf=q["Xf"], params=p) for q, (F, M) in zip(pairs3, trees)]
+    n_pts = int(sum(q["n_pts"] for q in pairs3))
+    tuned, _ = autotune.tune(ctx, pairs, batch_align, ct=0)
+    log(f"headline: {len(pairs)} pairs, {n_pts} scan points, tuned {tuned}")
+    B = PairBatch(pairs)
+    X0_sets = [np.stack([q["X0_steps"][k] for q in pairs3]) for k in range(len(pairs3[0]["X0_steps"]))]
+    for k in range(args.warmup):
+        B.set_initial(X0_sets[k % len(X0_sets)])
+        B.run(ctx)
+    gc.collect()
+    gc.disable()
+    step_t = []
+    kern = []
+    for k in range(args.steps):
+        t = time.perf_counter()
+        B.set_initial(X0_sets[k % len(X0_sets)])
+        B.run(ctx)
+        step_t.append(time.perf_counter() - t)
+        kern.append(ctx.last_kernel_time()[0])
+    gc.enable()
+    total = float(np.sum(step_t))
+    value = n_pts * args.steps / total
+    # the first pose set once more, for the accuracy check
+    B.set_initial(X0_sets[0])
+    out = B.run(ctx)
+    log(f"headline: {1e3 * total / args.steps:.3f} ms/step, {value / 1e9:.2f} G pts/s")
+    # kernel times (profiled: plain launches with events, tile kernel repeated)
+    ctx.set_profiling(True)
+    prof = []
+    for _ in range(3):
+        B.run(ctx)
+        prof.append(ctx.kernel_breakdown())
+    ctx.set_profiling(False)
+    tile_us = float(np.median([x[0] for x in prof]))
+    fin_us = float(np.median([x[1] for x in prof]))
+    # plain per-call path on the same pairs (C++ plugin, host loop)
+    t = time.perf_counter()
+    for q, (F, M) in zip(pairs3, trees):
+        adapter_align("ICPHIP3D", ctx, F, M, q["X0"], q["Xf"], p)
+    t_plain = time.perf_counter() - t
+    # demand bytes per launch: descent depth of every query at the initial cache
+    depth_sum, n_q, n_tiles = 0, 0, 0
+    for q, (F, M) in zip(pairs3, trees):
+        al = ICPHIP3D(ctx)
+        al.params = p
+        al.setFixed(F)
+        al.setMoving(M)
+        al.setX(q["X0"])
+        al.setFixedX(q["Xf"])
+        _, d = al.correspondences()
+        depth_sum += int(np.sum(d))
+        n_q += len(d)
+        n_tiles += (len(d) + 255) // 256
+    demand = float((24 + 4 + 24) * n_q + 32 * depth_sum + n_tiles * 4 * 27)
+    # accuracy: every pair against the oracle; H/b against the FP64 twin on a sample
+    op = O.from_kdh_params(p, 8)
+    worst_t = worst_r = 0.0
+    same_code = 0
+    for i, q in enumerate(pairs3):
+        Xo, _, _, res_o = O.compute(q["fixed"], q["moving"], q["X0"], q["Xf"], op, fast=False)
+        dt, dr = synth.pose_error(out.X[i], Xo)
+        worst_t, worst_r = max(worst_t, dt), max(worst_r, dr)
+        same_code += int(out.result[i] == res_o)
+    hb = []
+    for i in range(0, len(pairs3), 32):
+        q = pairs3[i]
+        F, M = trees[i]
+        al = ICPHIP3D(ctx)
+        al.params = p
+        al.setFixed(F)
+        al.setMoving(M)
+        al.setX(q["X0"])
+        al.setFixedX(q["Xf"])
+        al.buildQuadraticFormAt()
+        ref = O.build_form_at(q["fixed"], q["moving"], q["X0"], q["Xf"], op, fast=True)
+        hb.append((float(np.abs(al.H - ref["H64"]).max() / np.abs(ref["H64"]).max()),
+                   float(np.abs(al.b - ref["b64"]).max() / np.abs(ref["b64"]).max())))
+    n_gt = sum(1 for i, q in enumerate(pairs3) if synth.pose_error(out.X[i], q["X_gt"])[0] < 0.05)
+    res = dict(
+        value=round(value, 1), ms_per_step=round(1e3 * total / args.steps, 3), pairs=len(pairs3),
+        scan_points=n_pts, moving_leaves=int(n_q), fixed_nodes=int(sum(len(q["fixed"].nodes) for q in pairs3)),
+        mean_depth=round(depth_sum / max(n_q, 1), 2), tuned=tuned,
+        latency_ms=dict(step_p50=round(1e3 * float(np.percentile(step_t, 50)), 3),
+                        step_p99=round(1e3 * float(np.percentile(step_t, 99)), 3),
+                        plain_per_alignment_mean=round(1e3 * t_plain / len(pairs3), 3)),
+        plain_pts_per_s=round(n_pts / t_plain, 1), speedup_vs_plain=round(value / (n_pts / t_plain), 2),
+        accuracy=dict(pose_vs_oracle_max_m_deg=[worst_t, worst_r], result_code_equal=same_code,
+                      hb_rel_err_vs_fp64_max=[max(x[0] for x in hb), max(x[1] for x in hb)],
+                      hb_sample_pairs=len(hb), within_5cm_of_ground_truth=n_gt),
+        kernel_us_per_iter=dict(iteration=round(float(np.median(kern)), 2), tile_kernel=round(tile_us, 2),
+                                finish_kernel=round(fin_us, 2)),
+        tile_us=tile_us, demand_bytes_per_launch=demand)
+    log(f"headline: tile {tile_us:.1f} us, finish {fin_us:.1f} us, plain {1e3 * t_plain:.1f} ms, accuracy "
+        f"{res['accuracy']}")
+    for F, M in trees:
+        F.close()
+        M.close()
+    return res
+
+
+def run_cpu_baseline(pairs3, args, default_params):
+    """The oracle (restated ICP_CPU_, timing build) on pairs of the headline
+    batch, in pair order, for about --cpu-seconds on 8 threads (the
+    reference's cap) and a third of that on 1 thread."""
+    from oracle import pyoracle as O
+
+    p = _p3_params(default_params)
+    model, ncpu = host_cpu()
+    out = {}
+    for threads, secs in ((8, args.cpu_seconds), (1, args.cpu_seconds / 3)):
+        op = O.from_kdh_params(p, num_threads=threads)
+        t = time.perf_counter()
+        n_done, pts = 0, 0
+        while time.perf_counter() - t < secs or n_done == 0:
+            q = pairs3[n_done % len(pairs3)]
+            O.compute(q["fixed"], q["moving"], q["X0"], q["Xf"], op, fast=True)
+            n_done += 1
+            pts += q["n_pts"]
+        el = time.perf_counter() - t
+        out[threads] = (pts / el, n_done, el)
+    v8, n8, e8 = out[8]
+    v1, n1, e1 = out[1]
+    cpu = dict(value=round(v8, 1), unit="scan-pts/s", cores=8, kind="port",
+               sample=f"{n8} whole rigid alignments (10 iterations) of the headline pairs in pair order on 8 threads "
+                      f"(oracle/liboracle_fast.so, std::async chunks as ICP_CPU_), {e8:.1f} s",
+               one_thread=dict(value=round(v1, 1), alignments=n1, seconds=round(e1, 1)),
+               host=dict(cpu_model=model, logical_cpus=ncpu))
+    log(f"cpu oracle: {v8 / 1e6:.2f} Mpts/s on 8 threads, {v1 / 1e6:.2f} on 1 ({model}, {ncpu} CPUs)")
+    return cpu
+
+
+def run_ct(ctx, w, F, M, args, CTICPHIP3D, adapter_align, batch_align, default_params, synth):
+    """CT-ICP with deskew (131k-point sweep vs 2M-point map, 15 iterations):
+    plain per-call path, device-resident single-alignment loop, served
+    batches of --batch alignments (the map shared, own initial states)."""
+    from kd_slam2_amd import PairBatch
+
     p = _cfg2_params(default_params)
+    n_pts = len(w["scan"]) if "scan" in w else 131_072
 
     def aligner():
         al = CTICPHIP3D(ctx)
@@ -315,201 +609,53 @@ def main():
         al.setVelocities(w["v0"])
         return al
 
-    # descent depths at the initial cache (for algorithmic bytes)
-    _, depth = aligner().correspondences()
-    n_tiles = (moving.num_leaves + 255) // 256
-    b_iter = algorithmic_bytes_per_iter(depth, 1, n_tiles)
-
-    # ---- plain per-call path: CTICPHIP3D::compute() of the C++ plugin -----
-    plain_lat = []
-    gc.collect()
+    plain = []
     gc.disable()
     for i in range(args.plain_runs + 1):
         t = time.perf_counter()
         adapter_align("CTICPHIP3D", ctx, F, M, w["X0"], w["Xf"], p, v0=w["v0"])
-        dt = time.perf_counter() - t
-        if i > 0:
-            plain_lat.append(dt)
-    gc.enable()
-    plain_pps = n_pts / np.median(plain_lat)
-    log(f"plain per-call: median {1e3 * np.median(plain_lat):.2f} ms/alignment, {plain_pps / 1e6:.1f} Mpts/s")
-
-    # ---- device-resident loop, one alignment ------------------------------
-    dev_lat, kern_us = [], []
-    gc.disable()
+        if i:
+            plain.append(time.perf_counter() - t)
+    dev, kern = [], []
     for i in range(args.plain_runs + 1):
         al = aligner()
         t = time.perf_counter()
         al.computeOnDevice()
-        dt = time.perf_counter() - t
-        if i > 0:
-            dev_lat.append(dt)
-            kern_us.append(ctx.last_kernel_time()[0])
+        if i:
+            dev.append(time.perf_counter() - t)
+            kern.append(ctx.last_kernel_time()[0])
     gc.enable()
-    dev_pps = n_pts / np.median(dev_lat)
-    log(f"device loop: median {1e3 * np.median(dev_lat):.2f} ms/alignment, {np.median(kern_us):.1f} us/iter")
-
-    # ---- served batches (headline) ----------------------------------------
     rng = np.random.default_rng(np.random.PCG64(99))
     pairs = [dict(fixed=F, moving=M, ct=1, X0=synth.perturb(w["X_gt"], rng, 1.0, 0.3), Xf=w["Xf"], v0=w["v0"],
                   params=p) for _ in range(args.batch)]
-    for _ in range(args.warmup):
-        batch_align(ctx, pairs)
-    # The Python collector stays off inside timed regions: a full collection
-    # of the workload's large object graph would be timed as kernel work.
-    gc.collect()
+    B = PairBatch(pairs)
+    for _ in range(5):
+        B.run(ctx)
     gc.disable()
-    step_t, batch_kern = [], []
-    for _ in range(args.steps):
+    st = []
+    for _ in range(50):
         t = time.perf_counter()
-        out = batch_align(ctx, pairs)
-        step_t.append(time.perf_counter() - t)
-        batch_kern.append(ctx.last_kernel_time()[0])
+        B.run(ctx)
+        st.append(time.perf_counter() - t)
     gc.enable()
-    total = float(np.sum(step_t))
-    value = args.batch * n_pts * args.steps / total
-    errs = [synth.pose_error(o["X"], w["X_gt"]) for o in out]
-    slow = [f"{i}:{1e3 * x:.2f}" for i, x in enumerate(step_t) if x > 3 * np.median(step_t)]
-    log(f"served steps: median {1e3 * np.median(step_t):.3f} ms, slow (>3x median) {slow}")
-    log(f"served: {1e3 * np.mean(step_t):.2f} ms/step of {args.batch}, {value / 1e6:.1f} Mpts/s, "
-        f"{np.median(batch_kern):.1f} us/iter, max err {max(e[0] for e in errs):.4f} m")
-
-    # the same served step at other batch sizes (not the headline)
-    sweep = {}
-    for nb in (1, 8, 16, 64, 128):
-        pb = [dict(pairs[i % len(pairs)]) for i in range(nb)]
-        batch_align(ctx, pb)
-        gc.disable()
-        ts = []
-        for _ in range(5):
-            t = time.perf_counter()
-            batch_align(ctx, pb)
-            ts.append(time.perf_counter() - t)
-        gc.enable()
-        sweep[str(nb)] = dict(ms_per_step=round(1e3 * float(np.median(ts)), 3),
-                              pts_per_s=round(nb * n_pts / float(np.median(ts)), 1))
-    sweep[str(args.batch)] = dict(ms_per_step=round(1e3 * float(np.median(step_t)), 3),
-                                  pts_per_s=round(args.batch * n_pts / float(np.median(step_t)), 1))
-    log("batch sweep: " + ", ".join(f"{k}: {v['pts_per_s'] / 1e6:.0f} Mpts/s" for k, v in sweep.items()))
-
-    # roofline of the tile kernel in the served launch (profiled batches: no
-    # graph, HIP events around 4 back-to-back tile launches and the finish)
     ctx.set_profiling(True)
-    prof_tile, prof_fin = [], []
-    for _ in range(3):
-        batch_align(ctx, pairs)
-        tu, fu = ctx.kernel_breakdown()
-        prof_tile.append(tu)
-        prof_fin.append(fu)
+    B.run(ctx)
+    tu, fu = ctx.kernel_breakdown()
     ctx.set_profiling(False)
-    tile_us = float(np.median(prof_tile))
-    achieved = args.batch * b_iter / (tile_us * 1e-6) / 1e9
-    log(f"tile kernel {tile_us:.1f} us, finish kernel {np.median(prof_fin):.1f} us per iteration "
-        f"-> {achieved:.0f} GB/s algorithmic")
-
-    # ---- serving mix --------------------------------------------------------
-    serving = None
-    if not args.no_serve:
-        serving = run_serving(ctx, seqs, loops, args.serve_seconds, DeviceTree, Server, default_params)
-
-    # ---- batched loop-closure verification (config 3) ----------------------
-    batch3 = None
-    if pairs3:
-        batch3 = run_batch3(ctx, pairs3, DeviceTree, batch_align, adapter_align, default_params, synth)
-
-    # ---- precision stress (config 4) ---------------------------------------
-    precision = None
-    if w4 is not None:
-        precision = run_precision(ctx, w4, DeviceTree, default_params, synth)
-
-    # ---- tree generation on the device (makeTree + upload) ------------------
-    tree_build = None
-    if not args.no_tree_build:
-        tree_build = run_tree_build(ctx, w, sweep5, DeviceTree, F, p)
-    w.pop("map", None)
-
-    # ---- the other rows around the path: tree maintenance, batched factor
-    # forms, the KDTree3f wire and bag format -----------------------------
-    next_rows = None
-    if not args.no_next_rows:
-        next_rows = run_next_rows(ctx, w, DeviceTree, F, M, p)
-
-    # ---- CPU baseline -------------------------------------------------------
-    cpu = None
-    if not args.no_cpu:
-        from oracle import pyoracle as O
-
-        op = O.from_kdh_params(p, num_threads=8)
-        t = time.perf_counter()
-        n_done = 0
-        while time.perf_counter() - t < args.cpu_seconds or n_done == 0:
-            O.compute(fixed, moving, w["X0"], w["Xf"], op, ct=1, v0=w["v0"], fast=True)
-            n_done += 1
-        el = time.perf_counter() - t
-        cpu = dict(value=round(n_done * n_pts / el, 1), unit="scan-pts/s", cores=8, kind="port",
-                   sample=f"{n_done} full CT-ICP alignments (15 iterations) of the headline workload on 8 threads "
-                          f"(oracle/liboracle_fast.so, std::async chunks as ICP_CPU_), {el:.1f} s")
-        log(f"cpu oracle: {cpu['value'] / 1e6:.2f} Mpts/s")
-
-    # ---- HBM traffic of the tile kernel -------------------------------------
-    traffic = None
-    pmc = {}
-    if not args.no_pmc:
-        f = pmc_pass("FETCH_SIZE", args.batch, n_tiles)
-        wr = pmc_pass("WRITE_SIZE", args.batch, n_tiles)
-        if f and wr:
-            kib = COPY_BYTES / 1024.0
-            f_scale = kib / f[1]
-            w_scale = kib / wr[1]
-            traffic = round((f[0] * f_scale + wr[0] * w_scale) * 1024.0, 1)
-            pmc = dict(fetch_kib=round(f[0], 1), write_kib=round(wr[0], 1), copy_fetch_kib=round(f[1], 1),
-                       copy_write_kib=round(wr[1], 1), fetch_scale=round(f_scale, 4), write_scale=round(w_scale, 4))
-            log(f"pmc: tile fetch {f[0]:.0f} KiB, write {wr[0]:.0f} KiB per launch (copy {f[1]:.0f}/{wr[1]:.0f} "
-                f"KiB for {kib:.0f} KiB)")
-
-    roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic)
-    line = dict(
-        metric="scan-pts aligned/s + p99 align latency, served vs plain HIP, <=1mm/1e-3deg",
-        value=round(value, 1),
-        unit="scan-pts/s",
-        n_gpus=1,
-        steps=args.steps,
-        warmup=args.warmup,
-        ms_per_step=round(1e3 * total / args.steps, 3),
-        higher_is_better=True,
-        scaling="weak",
-        vs_baseline=None,
-        dtype="f32",
-        data="synthetic",
-        config=dict(workload="CT-ICP deskew scan-to-map, 131072-pt 128-beam sweep vs 2M-pt map, 15 iterations",
-                    served_batch=args.batch, moving_leaves=int(moving.num_leaves),
-                    fixed_nodes=int(fixed.num_nodes), mean_depth=round(float(np.mean(depth)), 2)),
-        served_speedup_vs_plain=round(value / plain_pps, 2),
-        served_batch_sweep=sweep,
-        plain_pts_per_s=round(plain_pps, 1),
-        device_loop_pts_per_s=round(dev_pps, 1),
-        latency_ms=dict(plain_p50=round(1e3 * float(np.percentile(plain_lat, 50)), 3),
-                        plain_p99=round(1e3 * float(np.percentile(plain_lat, 99)), 3),
-                        device_loop_p50=round(1e3 * float(np.percentile(dev_lat, 50)), 3),
-                        device_loop_p99=round(1e3 * float(np.percentile(dev_lat, 99)), 3),
-                        served_batch_p50=round(1e3 * float(np.percentile(step_t, 50)), 3),
-                        served_batch_p99=round(1e3 * float(np.percentile(step_t, 99)), 3)),
-        kernel_us_per_iter=dict(device_loop=round(float(np.median(kern_us)), 2),
-                                served_iteration=round(float(np.median(batch_kern)), 2),
-                                served_tile_kernel=round(tile_us, 2),
-                                served_finish_kernel=round(float(np.median(prof_fin)), 2)),
-        algorithmic_bytes_per_iter=b_iter,
-        pmc=pmc or None,
-        serving_mix=serving,
-        loop_closure_batch=batch3,
-        precision_stress=precision,
-        tree_build=tree_build,
-        next_rows=next_rows,
-        roofline=roofline,
-        cpu_baseline=cpu,
-    )
-    print(json.dumps(line), flush=True)
+    med = float(np.median(st))
+    out = dict(workload="CT-ICP deskew scan-to-map, 131072-pt 128-beam sweep vs 2M-pt map, 15 iterations",
+               plain_ms_p50=round(1e3 * float(np.median(plain)), 3),
+               device_loop_ms_p50=round(1e3 * float(np.median(dev)), 3),
+               device_loop_ms_p99=round(1e3 * float(np.percentile(dev, 99)), 3),
+               device_loop_us_per_iter=round(float(np.median(kern)), 2),
+               served_batch=args.batch, served_ms_p50=round(1e3 * med, 3),
+               served_ms_p99=round(1e3 * float(np.percentile(st, 99)), 3),
+               served_pts_per_s=round(args.batch * n_pts / med, 1),
+               plain_pts_per_s=round(n_pts / float(np.median(plain)), 1),
+               served_speedup_vs_plain=round(args.batch * float(np.median(plain)) / med, 2),
+               served_tile_us=round(tu, 2), served_finish_us=round(fu, 2))
+    log(f"ct: {out}")
+    return out
 
 
 def run_tree_build(ctx, w, sweep5, DeviceTree, fixed_tree=None, params=None):
@@ -807,59 +953,13 @@ def run_precision(ctx, w4, DeviceTree, default_params, synth):
     return out
 
 
-def run_batch3(ctx, pairs3, DeviceTree, batch_align, adapter_align, default_params, synth):
-    """Config 3: 256 independent rigid pairs (scan 4k-120k points, map 3x) in
-    one batched call vs the plain per-call path; the launch shape is picked
-    by the autotuner on this batch first."""
-    from kd_slam2_amd import autotune
 
-    p = default_params()
-    p.max_error = 0.5
-    p.max_iterations = 10
-    trees = [(DeviceTree(ctx, q["fixed"]), DeviceTree(ctx, q["moving"])) for q in pairs3]
-    pairs = [dict(fixed=F, moving=M, ct=0, X0=q["X0"], Xf=q["Xf"], params=p) for q, (F, M) in zip(pairs3, trees)]
-    n_pts = sum(q["n_pts"] for q in pairs3)
-    best, _ = autotune.tune(ctx, pairs, batch_align, ct=0)
-    batch_align(ctx, pairs)  # warm (graph capture)
-    tb = []
-    for _ in range(3):
-        t = time.perf_counter()
-        out = batch_align(ctx, pairs)
-        tb.append(time.perf_counter() - t)
-    t_batch = float(np.median(tb))
-    t = time.perf_counter()
-    for q, (F, M) in zip(pairs3, trees):
-        adapter_align("ICPHIP3D", ctx, F, M, q["X0"], q["Xf"], p)
-    t_plain = time.perf_counter() - t
-    ok = sum(1 for o, q in zip(out, pairs3) if synth.pose_error(o["X"], q["X_gt"])[0] < 0.05)
-    ctx.set_launch_config()  # back to the defaults
-    for F, M in trees:
-        F.close()
-        M.close()
-    res = dict(pairs=len(pairs3), scan_points=int(n_pts), batch_ms=round(1e3 * t_batch, 3),
-               batch_pts_per_s=round(n_pts / t_batch, 1), plain_ms=round(1e3 * t_plain, 3),
-               plain_pts_per_s=round(n_pts / t_plain, 1), speedup=round(t_plain / t_batch, 2),
-               autotuned=best, converged_within_5cm=ok)
-    log(f"config 3: {len(pairs3)} pairs, batch {res['batch_ms']} ms vs plain {res['plain_ms']} ms "
-        f"({res['speedup']}x), tuned {best}")
-    return res
+def _serve_window(srv, seqs, loops, seq_maps, loop_maps, seconds, rate_mult, seed, default_params):
+    """Poisson arrivals for `seconds` at rate_mult x (10 Hz per odometry
+    sequence, 20/s loop closure); returns [(kind, n_pts, result)] and the
+    wall time."""
+    from kd_slam2_amd import synth
 
-
-def run_serving(ctx, seqs, loops, seconds, DeviceTree, Server, default_params):
-    """Config 5: Poisson arrivals into the continuous-batching server."""
-    srv = Server(0, budget_bytes=64 << 30, max_inflight=64)
-    keep = []
-    seq_maps = []
-    for q in seqs:
-        T = DeviceTree(ctx, q["fixed"])
-        srv.add_resident(T)
-        seq_maps.append(T)
-    loop_maps = []
-    for lp in loops:
-        T = DeviceTree(ctx, lp["fixed"])
-        srv.add_resident(T)
-        loop_maps.append(T)
-    keep += seq_maps + loop_maps
     p_ct = default_params()
     p_ct.max_error = 0.5
     p_ct.max_iterations = 15
@@ -875,18 +975,12 @@ def run_serving(ctx, seqs, loops, seconds, DeviceTree, Server, default_params):
         lp = loops[idx]
         return srv.submit(loop_maps[idx], lp["moving"], lp["X0"], lp["Xf"], p_icp, ct=0), lp["n_pts"]
 
-    # warm the engine with one request of each kind (not timed)
-    for kind in ("odom", "loop"):
-        rid, _ = submit(kind, 0, 0)
-        srv.wait(rid, 60.0)
-    rng = np.random.default_rng(np.random.PCG64(5999))
-    from kd_slam2_amd import synth
-
+    rng = np.random.default_rng(np.random.PCG64(seed))
     events = []
     for s in range(len(seqs)):
-        for k, t in enumerate(synth.poisson_arrivals(10.0, seconds, rng)):
+        for k, t in enumerate(synth.poisson_arrivals(10.0 * rate_mult, seconds, rng)):
             events.append((t, "odom", s, k % len(seqs[s]["frames"])))
-    for t in synth.poisson_arrivals(20.0, seconds, rng):
+    for t in synth.poisson_arrivals(20.0 * rate_mult, seconds, rng):
         events.append((t, "loop", int(rng.integers(len(loops))), 0))
     events.sort()
     submitted = []
@@ -924,27 +1018,72 @@ def run_serving(ctx, seqs, loops, seconds, DeviceTree, Server, default_params):
     th.join()
     wall = time.perf_counter() - t_start
     gc.enable()
-    srv_info = srv.info()
+    return results, wall
+
+
+def _pct(results, kind=None):
+    lat = [r["latency_us"] / 1e3 for kk, _, r in results if kind is None or kk == kind]
+    if not lat:
+        return None
+    return dict(n=len(lat), p50_ms=round(float(np.percentile(lat, 50)), 3),
+                p99_ms=round(float(np.percentile(lat, 99)), 3), max_ms=round(float(np.max(lat)), 3))
+
+
+def _serving_setup(ctx, seqs, loops, DeviceTree, Server):
+    srv = Server(0, budget_bytes=64 << 30, max_inflight=64)
+    seq_maps = [DeviceTree(ctx, q["fixed"]) for q in seqs]
+    loop_maps = [DeviceTree(ctx, lp["fixed"]) for lp in loops]
+    for T in seq_maps + loop_maps:
+        srv.add_resident(T)
+    return srv, seq_maps, loop_maps
+
+
+def run_serving(ctx, seqs, loops, seconds, DeviceTree, Server, default_params):
+    """Config 5: Poisson arrivals into the continuous-batching server."""
+    srv, seq_maps, loop_maps = _serving_setup(ctx, seqs, loops, DeviceTree, Server)
+    # warm the engine (not timed)
+    _serve_window(srv, seqs, loops, seq_maps, loop_maps, 0.3, 1.0, 5998, default_params)
+    results, wall = _serve_window(srv, seqs, loops, seq_maps, loop_maps, seconds, 1.0, 5999, default_params)
+    info = srv.info()
     srv.close()
-    for T in keep:
+    for T in seq_maps + loop_maps:
         T.close()
-
-    def pct(kind=None):
-        lat = [r["latency_us"] / 1e3 for kk, _, r in results if kind is None or kk == kind]
-        if not lat:
-            return None
-        return dict(n=len(lat), p50_ms=round(float(np.percentile(lat, 50)), 3),
-                    p99_ms=round(float(np.percentile(lat, 99)), 3), max_ms=round(float(np.max(lat)), 3))
-
     ok = all(r["status"] == 0 for _, _, r in results)
     pts = sum(n for _, n, _ in results)
-    out = dict(window_s=seconds, requests=len(results), all_ok=ok, odometry=pct("odom"), loop_closure=pct("loop"),
-               all=pct(), pts_per_s=round(pts / wall, 1), engine_steps=srv_info["steps"],
-               budget_gb=64, resident_maps=len(keep))
-    log(f"serving: {len(results)} requests in {wall:.2f}s, odometry p50/p99 {out['odometry']['p50_ms']}/"
-        f"{out['odometry']['p99_ms']} ms, loop p50/p99 {out['loop_closure']['p50_ms']}/"
-        f"{out['loop_closure']['p99_ms']} ms")
+    out = dict(window_s=seconds, requests=len(results), all_ok=ok, odometry=_pct(results, "odom"),
+               loop_closure=_pct(results, "loop"), all=_pct(results), pts_per_s=round(pts / wall, 1),
+               engine_steps=info["steps"], budget_gb=64, resident_maps=len(seq_maps) + len(loop_maps))
+    log(f"serving: {len(results)} requests in {wall:.2f}s, odometry {out['odometry']}, loop {out['loop_closure']}")
     return out
+
+
+def run_capacity(ctx, seqs, loops, DeviceTree, Server, default_params, seconds=1.5,
+                 mults=(4, 16, 64, 128, 256, 512)):
+    """Highest arrival-rate multiple of the serving mix (x 4 sequences at
+    10 Hz and loop closure at 20/s) at which every request completes and the
+    odometry p99 latency stays under 10 ms; points/s served at that rate."""
+    srv, seq_maps, loop_maps = _serving_setup(ctx, seqs, loops, DeviceTree, Server)
+    runs = []
+    best = None
+    for m in mults:
+        results, wall = _serve_window(srv, seqs, loops, seq_maps, loop_maps, seconds, float(m), 6000 + m,
+                                      default_params)
+        odo = _pct(results, "odom")
+        ok = all(r["status"] == 0 for _, _, r in results)
+        pts = sum(n for _, n, _ in results)
+        run = dict(rate_mult=m, requests=len(results), requests_per_s=round(len(results) / wall, 1),
+                   pts_per_s=round(pts / wall, 1), odometry_p99_ms=odo["p99_ms"] if odo else None,
+                   loop_p99_ms=(_pct(results, "loop") or {}).get("p99_ms"), all_ok=ok)
+        runs.append(run)
+        log(f"capacity x{m}: {run}")
+        if ok and odo and odo["p99_ms"] < 10.0:
+            best = run
+        else:
+            break
+    srv.close()
+    for T in seq_maps + loop_maps:
+        T.close()
+    return dict(window_s=seconds, sustained=best, runs=runs)
 
 
 if __name__ == "__main__":
