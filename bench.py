@@ -218,6 +218,41 @@ def pmc_child(args):
     print("pmc child done", flush=True)
 
 
+def pmc_child_precision(args):
+    """The precision-stress alignment (2M-point scan, 8M-point map, 10 km
+    from the origin, re-centred FP32) run three times as device loops, plus a
+    device copy of COPY_BYTES bytes: the workload of the traces and counter
+    passes of the 8M-point map."""
+    from kd_slam2_amd import Context, DeviceTree, ICPHIP3D, TreeGenerator3D, default_params, synth
+
+    with mp.get_context("fork").Pool(min(16, os.cpu_count() or 4)) as pool:
+        blocks = pool.map(_gen_cfg4_block, [(k, b) for k in (0, 1) for b in range(8)])
+    g4 = TreeGenerator3D(parallel_level=3)
+    X_gt4, X04, Xf4 = synth.config4_poses()
+    moving, fixed = g4.makeTree(np.concatenate(blocks[:8])), g4.makeTree(np.concatenate(blocks[8:]))
+    ctx = Context(0)
+    F, M = DeviceTree(ctx, fixed), DeviceTree(ctx, moving)
+    p = _p3_params(default_params)
+    for _ in range(3):
+        al = ICPHIP3D(ctx)
+        al.params = p
+        al.setFixed(F)
+        al.setMoving(M)
+        al.setX(X04)
+        al.setFixedX(Xf4)
+        al.computeOnDevice()
+    print(f"precision device loop: {ctx.last_kernel_time()[0]:.2f} us/iter, {M.num_leaves} moving leaves, "
+          f"{len(fixed.nodes)} fixed nodes", flush=True)
+    hip = _hip()
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    hip.hipMalloc(ctypes.byref(a), COPY_BYTES)
+    hip.hipMalloc(ctypes.byref(b), COPY_BYTES)
+    hip.hipMemcpy(b, a, COPY_BYTES, 3)  # device to device
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print("pmc child done", flush=True)
+
+
 def pmc_pass(counter, args, tuned):
     """Per-dispatch counter (KiB) of the headline tile kernel (median over its
     dispatches) and of the copy kernel, from one rocprofv3 --pmc run.
@@ -291,12 +326,17 @@ def main():
     ap.add_argument("--tree-build-only", action="store_true",
                     help="only the tree-generation timings (for kernel traces of the builder)")
     ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--pmc-child-precision", action="store_true",
+                    help="only the precision-stress device loop (for traces and counter passes of the 8M map)")
     ap.add_argument("--pmc-grid", type=int, default=0, help=argparse.SUPPRESS)
     ap.add_argument("--pmc-lds", type=int, default=256, help=argparse.SUPPRESS)
     ap.add_argument("--pmc-lpl", type=int, default=1, help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.pmc_child:
         pmc_child(args)
+        return
+    if args.pmc_child_precision:
+        pmc_child_precision(args)
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); alignments shard by pair with no "
