@@ -215,6 +215,21 @@ int kdh_align(kdh_ctx* ctx, const kdh_tree* fixed, const kdh_tree* moving, int c
               double X_out[12], double v_out[6], kdh_stats* stats_log, int* n_log,
               int* result, float* cache_log);
 
+/* kdh_align without any allocation.  No counterpart in the reference: for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  The same loop,
+ * kernels and results as kdh_align (no cache log); its pair state and partial
+ * rows live in a fixed device array of the library (1.6 MB per GPU, held
+ * until the process ends), so a moving tree of more than
+ * kdh_align_noalloc_max_leaves() leaves (1,048,576) is refused with
+ * KDH_ERR_INVALID.  Calls on one GPU share that array: the library serialises
+ * them per device (they would race otherwise); plain launches, no graph. */
+int kdh_align_noalloc(kdh_ctx* ctx, const kdh_tree* fixed, const kdh_tree* moving, int ct,
+                      const double X0[12], const double v0[6], const double Xf[12],
+                      const kdh_params* params, const kdh_prior* prior, int stats_mode,
+                      double X_out[12], double v_out[6], kdh_stats* stats_log, int* n_log, int* result);
+int kdh_align_noalloc_max_leaves(void);
+
 /* ---- many alignments in one launch -------------------------------------- */
 typedef struct kdh_pair {
   const kdh_tree* fixed;
@@ -380,6 +395,12 @@ int kdh_last_kernel_breakdown(kdh_ctx* ctx, double* tile_us, double* finish_us);
 /* Tuning knobs (project autotuner): workgroups per launch (0 = one per tile)
  * and LDS-staged top-of-tree nodes (0/256/512/1024/2048). */
 int kdh_set_launch_config(kdh_ctx* ctx, int grid, int lds_nodes);
+/* Tuning knob (project autotuner): moving leaves per lane of the tile kernel,
+ * 1 (256-thread workgroups) or 2 (128 threads, two descents in flight per
+ * lane).  A tile is 256 leaves either way; results are the same bits in any
+ * batch for one setting (the per-tile FP32 sums differ in order between
+ * settings).  The dual form always runs 1. */
+int kdh_set_leaves_per_lane(kdh_ctx* ctx, int leaves_per_lane);
 
 #ifdef __cplusplus
 }

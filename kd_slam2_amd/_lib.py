@@ -256,11 +256,18 @@ _SIGS = {
         [P, P, P, C.c_int, F64P, F64P, F64P, C.POINTER(Params), C.POINTER(Prior), C.c_int, F64P, F64P,
          C.POINTER(Stats), C.POINTER(C.c_int), C.POINTER(C.c_int), F32P],
     ),
+    "kdh_align_noalloc": (
+        C.c_int,
+        [P, P, P, C.c_int, F64P, F64P, F64P, C.POINTER(Params), C.POINTER(Prior), C.c_int, F64P, F64P,
+         C.POINTER(Stats), C.POINTER(C.c_int), C.POINTER(C.c_int)],
+    ),
+    "kdh_align_noalloc_max_leaves": (C.c_int, []),
     "kdh_batch_align": (C.c_int, [P, C.c_int, C.POINTER(Pair), C.POINTER(PairResult)]),
     "kdh_batch_build": (C.c_int, [P, C.c_int, C.POINTER(Pair), C.c_int, F64P, F64P, C.POINTER(Stats)]),
     "kdh_batch_build_dual": (C.c_int, [P, C.c_int, C.POINTER(DualPair), C.c_int, C.POINTER(DualForm)]),
     "kdh_last_kernel_time": (C.c_int, [P, C.POINTER(C.c_double), C.POINTER(C.c_int)]),
     "kdh_set_launch_config": (C.c_int, [P, C.c_int, C.c_int]),
+    "kdh_set_leaves_per_lane": (C.c_int, [P, C.c_int]),
     "kdh_set_profiling": (C.c_int, [P, C.c_int]),
     "kdh_last_kernel_breakdown": (C.c_int, [P, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "kdh_server_create": (C.c_int, [C.c_int, C.c_int64, C.c_int32, C.POINTER(P)]),
