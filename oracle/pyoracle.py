@@ -77,9 +77,14 @@ def build():
 
 
 def lib(fast=False):
+    """The parity build (liboracle.so), or the timing build when fast; the
+    environment variable KDH_ORACLE_LIB names another parity build to load
+    (the sanitizer build of tests/test_sanitizers.py)."""
     name = "liboracle_fast.so" if fast else "liboracle.so"
     if name not in _libs:
         path = os.path.join(_HERE, name)
+        if not fast and os.environ.get("KDH_ORACLE_LIB"):
+            path = os.environ["KDH_ORACLE_LIB"]
         if not os.path.exists(path):
             build()
         L = C.CDLL(path)
