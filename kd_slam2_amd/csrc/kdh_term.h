@@ -106,6 +106,62 @@ KDH_HD int descend(const NodeFetch& fetch, V3f q, int* depth_out, float4& a, flo
   return idx;
 }
 
+// Two greedy descents in lockstep (each exactly `descend`): every step
+// issues the next node fetch of both queries before either is needed, so a
+// lane keeps two independent gathers in flight.  A query whose `live` is
+// false takes no step and returns -1 with depth 0.
+template <typename NodeFetch>
+KDH_HD void descend2(const NodeFetch& fetch, V3f q0, V3f q1, bool live0, bool live1, int& idx0, int& idx1,
+                     int& depth0, int& depth1, float4& a0, float4& b0, float4& a1, float4& b1) {
+  idx0 = live0 ? 0 : -1;
+  idx1 = live1 ? 0 : -1;
+  depth0 = depth1 = 0;
+  bool go0 = live0, go1 = live1;
+  fetch(0, a0, b0);
+  a1 = a0;
+  b1 = b0;
+  while (go0 || go1) {
+    if (go0) {
+#if defined(__HIP_DEVICE_COMPILE__)
+      const int l = __float_as_int(b0.z), r = __float_as_int(b0.w);
+#else
+      int l, r;
+      memcpy(&l, &b0.z, 4);
+      memcpy(&r, &b0.w, 4);
+#endif
+      if (l < 0 && r < 0) {
+        go0 = false;
+      } else {
+        const float dx = q0.x - a0.x, dy = q0.y - a0.y, dz = q0.z - a0.z;
+        const float sd = (a0.w * dx + b0.x * dy) + b0.y * dz;
+        idx0 = sd > 0.0f ? l : r;
+        ++depth0;
+        go0 = idx0 >= 0;
+      }
+    }
+    if (go1) {
+#if defined(__HIP_DEVICE_COMPILE__)
+      const int l = __float_as_int(b1.z), r = __float_as_int(b1.w);
+#else
+      int l, r;
+      memcpy(&l, &b1.z, 4);
+      memcpy(&r, &b1.w, 4);
+#endif
+      if (l < 0 && r < 0) {
+        go1 = false;
+      } else {
+        const float dx = q1.x - a1.x, dy = q1.y - a1.y, dz = q1.z - a1.z;
+        const float sd = (a1.w * dx + b1.x * dy) + b1.y * dz;
+        idx1 = sd > 0.0f ? l : r;
+        ++depth1;
+        go1 = idx1 >= 0;
+      }
+    }
+    if (go0) fetch(idx0, a0, b0);
+    if (go1) fetch(idx1, a1, b1);
+  }
+}
+
 // rodrigues(w) (geometry_3d_impl_.h:22-46): identity below 1e-9, otherwise
 // R = I + sin(a) S + (1 - cos(a)) S^2 with S = skew(w / |w|).
 struct Rodrigues {
@@ -248,17 +304,16 @@ KDH_HD void classify(TermRec& R, float e_ref, const TermParams& P) {
   }
 }
 
-// Rigid ICP term (icp_impl_.h:103-152).  Returns the fixed leaf index reached
-// by the descent (-1 for num_bad).
-template <typename NodeFetch>
-KDH_HD int icp_term(const NodeFetch& fetch, const TermCache& C, const TermParams& P, const LeafIn& L, TermRec& R,
-                    int* depth_out) {
-  const V3f p_mif = apply_iso(C.Xmf, L.mean);
-  float4 a, b;
-  const int leaf = descend(fetch, p_mif, depth_out, a, b);
+// Rigid ICP term (icp_impl_.h:103-152), in three steps: the query point
+// (icp_query), the descent, and the term at the fixed leaf the descent
+// reached (icp_eval: leaf index and its record (a, b), or -1).
+KDH_HD V3f icp_query(const TermCache& C, const LeafIn& L) { return apply_iso(C.Xmf, L.mean); }
+
+KDH_HD void icp_eval(const TermCache& C, const TermParams& P, const LeafIn& L, V3f p_mif, int leaf, float4 a,
+                     float4 b, TermRec& R) {
   if (leaf < 0) {
     rec_reject(R, kBad);
-    return -1;
+    return;
   }
   const V3f pf = v3(a.x, a.y, a.z);
   const V3f nf = v3(a.w, b.x, b.y);
@@ -269,13 +324,13 @@ KDH_HD int icp_term(const NodeFetch& fetch, const TermCache& C, const TermParams
   // errorAndJacobian (d3/icp_.h:41-60)
   if (dot(n_f, n_m) < P.min_normal_cos) {
     rec_reject(R, kErrNormal);
-    return leaf;
+    return;
   }
   const V3f n = n_m + n_f;
   const float e = dot(p_m - p_f, n);
   if (fabsf(e) > P.max_error) {
     rec_reject(R, kErrFn);
-    return leaf;
+    return;
   }
   // Reference mode has mroot = cf = 0 and starts from the full translations,
   // which makes jr exactly p_m x n_f + p_f x n_m.
@@ -288,6 +343,17 @@ KDH_HD int icp_term(const NodeFetch& fetch, const TermCache& C, const TermParams
   R.J[3] = jr.x; R.J[4] = jr.y; R.J[5] = jr.z;
   for (int i = 0; i < 6; ++i) R.Jv[i] = 0.0f;
   classify<false>(R, e, P);
+}
+
+// The whole rigid term for one leaf.  Returns the fixed leaf index reached by
+// the descent (-1 for num_bad).
+template <typename NodeFetch>
+KDH_HD int icp_term(const NodeFetch& fetch, const TermCache& C, const TermParams& P, const LeafIn& L, TermRec& R,
+                    int* depth_out) {
+  const V3f p_mif = icp_query(C, L);
+  float4 a, b;
+  const int leaf = descend(fetch, p_mif, depth_out, a, b);
+  icp_eval(C, P, L, p_mif, leaf, a, b, R);
   return leaf;
 }
 
@@ -297,24 +363,38 @@ KDH_HD int icp_term(const NodeFetch& fetch, const TermCache& C, const TermParams
 // (dts_of(leaf)), errorAndJacobianVelocityB (d3/cticp_impl_.h:43-73):
 //   J_t = -dt n^T Rb ; J_w = dt (-n^T Rb dR_pb + dp^T Rb dR_nb)
 // with the fixed leaf warped by vf; re-centred, Rb^T n = Rmf wdir + nb_moved
-// and Rb^T dp = p_mif - pb_moved, in the fixed tree's frame.
-template <bool DUAL = false, typename NodeFetch, typename DtsFetch = int>
-KDH_HD int ct_term(const NodeFetch& fetch, const TermCache& C, const TermParams& P, const LeafIn& L, TermRec& R,
-                   int* depth_out, float* Jb = nullptr, const DtsFetch& dts_of = 0) {
-  // applyVelocities on the leaf (d3/cticp_impl_.h:76-85)
+// and Rb^T dp = p_mif - pb_moved, in the fixed tree's frame.  In three steps
+// like the rigid term: query point, descent, term at the fixed leaf.
+
+// The CT query point: the leaf warped by applyVelocities
+// (d3/cticp_impl_.h:76-85), then moved into the fixed tree's frame.
+KDH_HD void ct_warp(const TermCache& C, const LeafIn& L, Rodrigues& rd, V3f& wmean, V3f& wdir) {
   const float dt = L.dts;
   const V3f t_delta = v3(C.v[0] * dt, C.v[1] * dt, C.v[2] * dt);
   const V3f r_delta = v3(C.v[3] * dt, C.v[4] * dt, C.v[5] * dt);
-  const Rodrigues rd = rodrigues_f(r_delta);
-  const V3f wmean = mul(rd.R, L.mean) + t_delta;
-  const V3f wdir = mul(rd.R, L.dir);
-  const V3f p_mif = apply_iso(C.Xmf, wmean);
-  float4 a, b;
-  const int leaf = descend(fetch, p_mif, depth_out, a, b);
+  rd = rodrigues_f(r_delta);
+  wmean = mul(rd.R, L.mean) + t_delta;
+  wdir = mul(rd.R, L.dir);
+}
+KDH_HD V3f ct_query(const TermCache& C, const LeafIn& L) {
+  Rodrigues rd;
+  V3f wmean, wdir;
+  ct_warp(C, L, rd, wmean, wdir);
+  return apply_iso(C.Xmf, wmean);
+}
+
+// The CT term at the fixed leaf the descent reached (leaf index and its
+// record (a, b), or -1), given the leaf's warp (rd, wmean, wdir) and query
+// point p_mif.
+template <bool DUAL = false, typename DtsFetch = int>
+KDH_HD void ct_eval_warped(const TermCache& C, const TermParams& P, const LeafIn& L, const Rodrigues& rd,
+                           V3f wmean, V3f wdir, V3f p_mif, int leaf, float4 a, float4 b, TermRec& R,
+                           float* Jb = nullptr, const DtsFetch& dts_of = 0) {
   if (leaf < 0) {
     rec_reject(R, kBad);
-    return -1;
+    return;
   }
+  const float dt = L.dts;
   const V3f pf = v3(a.x, a.y, a.z);
   const V3f nf = v3(a.w, b.x, b.y);
   const V3f p_m = apply_iso(C.Xm, wmean);
@@ -323,14 +403,14 @@ KDH_HD int ct_term(const NodeFetch& fetch, const TermCache& C, const TermParams&
   const V3f n_f = apply_rot(C.Xf, nf);
   if (dot(n_f, n_m) < P.min_normal_cos) {
     rec_reject(R, kErrNormal);
-    return leaf;
+    return;
   }
   const V3f n = n_m + n_f;
   const V3f dp = p_m - p_f;
   const float e = dot(dp, n);
   if (fabsf(e) > P.max_error) {
     rec_reject(R, kErrFn);
-    return leaf;
+    return;
   }
   // As in icp_term; re-centred, Ra^T dp = Rmf^T (p_mif - pf) as well.
   const M3f& Ra = C.Xm.R;
@@ -375,6 +455,32 @@ KDH_HD int ct_term(const NodeFetch& fetch, const TermCache& C, const TermParams&
     Jb[3] = jwb.x * dtb; Jb[4] = jwb.y * dtb; Jb[5] = jwb.z * dtb;
   }
   classify<DUAL>(R, e, P);
+}
+
+// The same, forming the warp again from the leaf (the same operations on the
+// same values, so the same bits as the query's): the two-leaves-per-lane
+// path keeps no warp across its descents.
+template <bool DUAL = false, typename DtsFetch = int>
+KDH_HD void ct_eval(const TermCache& C, const TermParams& P, const LeafIn& L, int leaf, float4 a, float4 b,
+                    TermRec& R, float* Jb = nullptr, const DtsFetch& dts_of = 0) {
+  Rodrigues rd;
+  V3f wmean, wdir;
+  ct_warp(C, L, rd, wmean, wdir);
+  ct_eval_warped<DUAL>(C, P, L, rd, wmean, wdir, apply_iso(C.Xmf, wmean), leaf, a, b, R, Jb, dts_of);
+}
+
+// The whole CT term for one leaf (the warp formed once).  Returns the fixed
+// leaf index reached by the descent (-1 for num_bad).
+template <bool DUAL = false, typename NodeFetch, typename DtsFetch = int>
+KDH_HD int ct_term(const NodeFetch& fetch, const TermCache& C, const TermParams& P, const LeafIn& L, TermRec& R,
+                   int* depth_out, float* Jb = nullptr, const DtsFetch& dts_of = 0) {
+  Rodrigues rd;
+  V3f wmean, wdir;
+  ct_warp(C, L, rd, wmean, wdir);
+  const V3f p_mif = apply_iso(C.Xmf, wmean);
+  float4 a, b;
+  const int leaf = descend(fetch, p_mif, depth_out, a, b);
+  ct_eval_warped<DUAL>(C, P, L, rd, wmean, wdir, p_mif, leaf, a, b, R, Jb, dts_of);
   return leaf;
 }
 
