@@ -401,6 +401,12 @@ int kdh_set_launch_config(kdh_ctx* ctx, int grid, int lds_nodes);
  * batch for one setting (the per-tile FP32 sums differ in order between
  * settings).  The dual form always runs 1. */
 int kdh_set_leaves_per_lane(kdh_ctx* ctx, int leaves_per_lane);
+/* Tuning knob (project autotuner): kdh_batch_align replays its iterations on
+ * 1 stream, or on 2 -- the pairs split into two halves of about equal tile
+ * count, the second half's chain started one tile launch after the first's,
+ * so that one half's finish stage runs beside the other half's tile launch.
+ * Results are the same bits either way. */
+int kdh_set_batch_streams(kdh_ctx* ctx, int streams);
 
 #ifdef __cplusplus
 }

@@ -117,10 +117,12 @@ class Context:
     def check(self, rc, what):
         L.check(rc, self.h, what)
 
-    def set_launch_config(self, grid=0, lds_nodes=256, leaves_per_lane=None):
+    def set_launch_config(self, grid=0, lds_nodes=256, leaves_per_lane=None, batch_streams=None):
         self.check(L.lib().kdh_set_launch_config(self.h, grid, lds_nodes), "kdh_set_launch_config")
         if leaves_per_lane is not None:
             self.check(L.lib().kdh_set_leaves_per_lane(self.h, int(leaves_per_lane)), "kdh_set_leaves_per_lane")
+        if batch_streams is not None:
+            self.check(L.lib().kdh_set_batch_streams(self.h, int(batch_streams)), "kdh_set_batch_streams")
 
     def set_profiling(self, on=True):
         self.check(L.lib().kdh_set_profiling(self.h, int(bool(on))), "kdh_set_profiling")

@@ -10,7 +10,10 @@ fastest, timing each candidate with HIP events around the tile kernel
 * the number of top-of-tree nodes staged in LDS per workgroup
   (0/256/512/1024, plus 2048 for rigid ICP);
 * the workgroup count: one per tile, or a persistent grid of k workgroups
-  per CU walking contiguous tile chunks.
+  per CU walking contiguous tile chunks;
+* then, at the best shape, whether kdh_batch_align replays on one stream or
+  on two (the pairs split in halves whose finish stages run beside the other
+  half's tile launches), timed by whole batches (kdh_set_batch_streams).
 
 A tile is 256 moving leaves in every shape; the leaf size is a property of
 the trees, chosen when they are built (TreeGenerator3D leaf_max_points).
@@ -41,6 +44,17 @@ def tune(ctx, pairs, batch_align, ct, repeats=2, grids=(0, N_CU * 2, N_CU * 4, N
                     results.append(dict(leaves_per_lane=lpl, lds_nodes=nodes, grid=grid, tile_us=float(np.median(t))))
     finally:
         ctx.set_profiling(False)
-    best = min(results, key=lambda r: r["tile_us"])
+    best = dict(min(results, key=lambda r: r["tile_us"]))
     ctx.set_launch_config(grid=best["grid"], lds_nodes=best["lds_nodes"], leaves_per_lane=best["leaves_per_lane"])
+    per_iter = {}
+    for streams in (1, 2):
+        ctx.set_launch_config(grid=best["grid"], lds_nodes=best["lds_nodes"], batch_streams=streams)
+        t = []
+        for _ in range(repeats + 2):
+            batch_align(ctx, pairs)
+            t.append(ctx.last_kernel_time()[0])
+        per_iter[streams] = float(np.median(t[2:]))
+    best["batch_streams"] = min(per_iter, key=per_iter.get)
+    best["iteration_us"] = per_iter[best["batch_streams"]]
+    ctx.set_launch_config(grid=best["grid"], lds_nodes=best["lds_nodes"], batch_streams=best["batch_streams"])
     return best, results

@@ -33,11 +33,16 @@ using namespace kdh;
 
 namespace {
 
+// The tiles of one aligner kind within one pair range: group ct + 2 * half
+// holds the pairs of kind ct in [p_lo, p_hi); a batch run on two streams
+// splits its pairs into two halves (half 0, half 1) at a forest boundary.
+constexpr int kGroups = 4;
 struct Group {
   int ct = 0;
   int n_tiles = 0;
   size_t partial_off = 0;  // floats
   size_t tile_off = 0;     // ints
+  int p_lo = 0, p_hi = 0;  // pair range (finish launch)
 };
 
 }  // namespace
@@ -46,14 +51,17 @@ struct Group {
 // mode 0: align (solve + update each iteration); mode 1: build the form once.
 // dual: the CT pairs build the dual form, with the fixed velocities vf
 // (n_pairs x 6).  Called with ctx->mu held and the device set.
+// halves = 2: the pairs are split into two ranges of about equal tile count
+// (group ct + 2 * half), for the two-stream replay of kdh_batch_align.
 static int batch_setup(kdh_ctx* ctx, int n_pairs, const kdh_pair* pairs, int mode, int stats_mode, Group* g,
-                       std::vector<int>& leader, int& max_iter, int dual = 0, const double* vf = nullptr) {
+                       std::vector<int>& leader, int& max_iter, int dual = 0, const double* vf = nullptr,
+                       int halves = 1) {
   int rc = kdh_ensure_pairs(ctx, (size_t)n_pairs);
   if (rc) return rc;
-  g[0] = Group();
-  g[1] = Group();
-  g[0].ct = 0;
-  g[1].ct = 1;
+  for (int k = 0; k < kGroups; ++k) {
+    g[k] = Group();
+    g[k].ct = k & 1;
+  }
   max_iter = 0;
   // Forests: pair i with forest = k leads pairs i+1 .. i+k.
   leader.assign(n_pairs, 0);
@@ -84,9 +92,6 @@ static int batch_setup(kdh_ctx* ctx, int n_pairs, const kdh_pair* pairs, int mod
     kdh_cache_from_state(ps);
     if (vf)
       for (int k = 0; k < 6; ++k) ps.cache.vf[k] = (float)vf[6 * i + k];
-    Group& G = g[ps.ct];
-    ps.tile_first = G.n_tiles;
-    G.n_tiles += ps.n_tiles;
     if (leader[i] != i) {
       ps.group_n = 0;
       PairState& L = ctx->h_pairs[leader[i]];
@@ -104,13 +109,37 @@ static int batch_setup(kdh_ctx* ctx, int n_pairs, const kdh_pair* pairs, int mod
       ps.status = L.sp.min_good_measurements > 0 ? 2 : 1;
     }
   }
-  g[0].partial_off = 0;
-  g[1].partial_off = (size_t)g[0].n_tiles * slots(0);
-  const int ct_slots = slots(1, dual);
-  g[0].tile_off = 0;
-  g[1].tile_off = (size_t)g[0].n_tiles;
-  const size_t n_partials = g[1].partial_off + (size_t)g[1].n_tiles * ct_slots;
-  const size_t n_tiles_all = (size_t)g[0].n_tiles + (size_t)g[1].n_tiles;
+  // The split: the first forest boundary where the tiles before it reach
+  // half of all tiles (no split with one half, or when nothing would follow).
+  int split = n_pairs;
+  if (halves == 2) {
+    long long total = 0, acc = 0;
+    for (int i = 0; i < n_pairs; ++i) total += ctx->h_pairs[i].n_tiles;
+    for (int i = 0; i < n_pairs; ++i) {
+      if (leader[i] == i && i > 0 && 2 * acc >= total) {
+        split = i;
+        break;
+      }
+      acc += ctx->h_pairs[i].n_tiles;
+    }
+  }
+  for (int k = 0; k < kGroups; ++k) {
+    g[k].p_lo = k < 2 ? 0 : split;
+    g[k].p_hi = k < 2 ? split : n_pairs;
+  }
+  for (int i = 0; i < n_pairs; ++i) {
+    PairState& ps = ctx->h_pairs[i];
+    Group& G = g[ps.ct + (i >= split ? 2 : 0)];
+    ps.tile_first = G.n_tiles;
+    G.n_tiles += ps.n_tiles;
+  }
+  size_t n_partials = 0, n_tiles_all = 0;
+  for (int k = 0; k < kGroups; ++k) {
+    g[k].partial_off = n_partials;
+    g[k].tile_off = n_tiles_all;
+    n_partials += (size_t)g[k].n_tiles * slots(g[k].ct, g[k].ct ? dual : 0);
+    n_tiles_all += (size_t)g[k].n_tiles;
+  }
   rc = kdh_ensure_partials(ctx, std::max<size_t>(n_partials, 1));
   if (rc) return rc;
   rc = kdh_ensure_tile_pair(ctx, std::max<size_t>(n_tiles_all, 1));
@@ -118,7 +147,7 @@ static int batch_setup(kdh_ctx* ctx, int n_pairs, const kdh_pair* pairs, int mod
   std::vector<int> tp(std::max<size_t>(n_tiles_all, 1), 0);
   for (int i = 0; i < n_pairs; ++i) {
     const PairState& ps = ctx->h_pairs[i];
-    const Group& G = g[ps.ct];
+    const Group& G = g[ps.ct + (i >= split ? 2 : 0)];
     for (int t = 0; t < ps.n_tiles; ++t) tp[G.tile_off + ps.tile_first + t] = i;
   }
   rc = kdh_upload_tile_pair(ctx, tp);
@@ -137,21 +166,36 @@ int kdh_batch_align(kdh_ctx* ctx, int n_pairs, const kdh_pair* pairs, kdh_pair_r
   if (n_pairs == 0) return KDH_OK;
   std::lock_guard<std::mutex> lk(ctx->mu);
   KDH_HIP_TRY(ctx, hipSetDevice(ctx->device));
-  Group g[2];
+  Group g[kGroups];
   std::vector<int> leader;
   int max_iter = 0;
-  int rc = batch_setup(ctx, n_pairs, pairs, 0, 0, g, leader, max_iter);
+  // profiling times each phase on one stream
+  const int halves = ctx->profile ? 1 : ctx->batch_streams;
+  int rc = batch_setup(ctx, n_pairs, pairs, 0, 0, g, leader, max_iter, 0, nullptr, halves);
   if (rc) return rc;
+  const bool two = halves == 2 && (g[2].n_tiles + g[3].n_tiles) > 0 && (g[0].n_tiles + g[1].n_tiles) > 0;
+  if (two && !ctx->stream2) {
+    KDH_HIP_TRY(ctx, hipStreamCreateWithFlags(&ctx->stream2, hipStreamNonBlocking));
+    KDH_HIP_TRY(ctx, hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming));
+    KDH_HIP_TRY(ctx, hipEventCreateWithFlags(&ctx->ev_stag, hipEventDisableTiming));
+    KDH_HIP_TRY(ctx, hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming));
+  }
 
   const int iters = std::min(max_iter, kMaxIter);
-  auto launch_phase = [&](int phases) -> int {
-    for (int k = 0; k < 2; ++k) {
+  // the groups of half h (kinds 0 and 1) on stream s
+  auto launch_half = [&](int h, int phases, hipStream_t st) -> int {
+    for (int k = 2 * h; k < 2 * h + 2; ++k) {
       if (g[k].n_tiles == 0) continue;
       int lrc = kdh_launch(ctx, ctx->d_pairs, ctx->d_tile_pair + g[k].tile_off, g[k].n_tiles, g[k].ct, nullptr,
-                           nullptr, ctx->d_partials + g[k].partial_off, n_pairs, phases);
+                           nullptr, ctx->d_partials + g[k].partial_off, g[k].p_hi - g[k].p_lo, phases, 0, g[k].p_lo,
+                           st);
       if (lrc != KDH_OK) return lrc;
     }
     return KDH_OK;
+  };
+  auto launch_phase = [&](int phases) -> int {
+    int lrc = launch_half(0, phases, ctx->stream);
+    return lrc != KDH_OK ? lrc : launch_half(1, phases, ctx->stream);
   };
   hipError_t le = hipSuccess;
   if (ctx->profile) {
@@ -171,20 +215,40 @@ int kdh_batch_align(kdh_ctx* ctx, int n_pairs, const kdh_pair* pairs, kdh_pair_r
     // Capture every iteration (tile + finish launch per non-empty group) in
     // one graph and launch it once: the pair states live in device memory,
     // so the same launches replay each Gauss-Newton step, and pairs that have
-    // finished skip their tiles.
-    const long long key[12] = {n_pairs, g[0].n_tiles, g[1].n_tiles, iters, ctx->lds_nodes, ctx->grid_override,
+    // finished skip their tiles.  With two streams each half of the pairs is
+    // its own chain of launches; the second half starts after the first
+    // half's first tile launch, so from then on one half's finish stage runs
+    // beside the other half's tile launch.
+    const long long key[16] = {n_pairs, g[0].n_tiles, g[1].n_tiles, iters, ctx->lds_nodes, ctx->grid_override,
                                ctx->xcd_remap, (long long)(intptr_t)ctx->d_pairs,
-                               (long long)(intptr_t)ctx->d_partials, (long long)(intptr_t)ctx->d_tile_pair, 2, 0};
+                               (long long)(intptr_t)ctx->d_partials, (long long)(intptr_t)ctx->d_tile_pair, 2,
+                               ctx->leaves_per_lane, two ? 2 : 1, g[2].n_tiles, g[3].n_tiles,
+                               (long long)(intptr_t)ctx->stream2};
     hipGraphExec_t exec = kdh_graph_lookup(ctx->graph_batch, key);
     if (!exec) {
       hipGraph_t graph = nullptr;
       KDH_HIP_TRY(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
       int lrc = KDH_OK;
-      for (int it = 0; it < iters && lrc == KDH_OK; ++it) lrc = launch_phase(3);
+      hipError_t ee = hipSuccess;
+      if (!two) {
+        for (int it = 0; it < iters && lrc == KDH_OK; ++it) lrc = launch_phase(3);
+      } else {
+        ee = hipEventRecord(ctx->ev_fork, ctx->stream);
+        if (ee == hipSuccess) ee = hipStreamWaitEvent(ctx->stream2, ctx->ev_fork, 0);
+        if (ee == hipSuccess) lrc = launch_half(0, 1, ctx->stream);
+        if (ee == hipSuccess && lrc == KDH_OK) ee = hipEventRecord(ctx->ev_stag, ctx->stream);
+        if (ee == hipSuccess && lrc == KDH_OK) lrc = launch_half(0, 2, ctx->stream);
+        if (ee == hipSuccess) ee = hipStreamWaitEvent(ctx->stream2, ctx->ev_stag, 0);
+        for (int it = 1; it < iters && lrc == KDH_OK && ee == hipSuccess; ++it) lrc = launch_half(0, 3, ctx->stream);
+        for (int it = 0; it < iters && lrc == KDH_OK && ee == hipSuccess; ++it)
+          lrc = launch_half(1, 3, ctx->stream2);
+        if (ee == hipSuccess) ee = hipEventRecord(ctx->ev_join, ctx->stream2);
+        if (ee == hipSuccess) ee = hipStreamWaitEvent(ctx->stream, ctx->ev_join, 0);
+      }
       hipError_t ce = hipStreamEndCapture(ctx->stream, &graph);
-      if (lrc != KDH_OK) {
+      if (lrc != KDH_OK || ee != hipSuccess) {
         if (graph) hipGraphDestroy(graph);
-        return lrc;
+        return lrc != KDH_OK ? lrc : kdh_hip_fail(ctx, ee, "two-stream capture");
       }
       if (ce != hipSuccess) return kdh_hip_fail(ctx, ce, "hipStreamEndCapture");
       hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
@@ -228,7 +292,7 @@ int kdh_batch_align(kdh_ctx* ctx, int n_pairs, const kdh_pair* pairs, kdh_pair_r
     if (leader[i] != i || L.group_n < 2 || L.iter == 0 || L.group_tiles == 0) continue;
     const int N = slots(L.ct), CS = L.ct ? 90 : 27;
     rows.resize((size_t)L.group_tiles * N);
-    const float* src = ctx->d_partials + g[L.ct].partial_off + (size_t)L.tile_first * N;
+    const float* src = ctx->d_partials + g[L.ct + (i >= g[2].p_lo ? 2 : 0)].partial_off + (size_t)L.tile_first * N;
     KDH_HIP_TRY(ctx, hipMemcpy(rows.data(), src, rows.size() * sizeof(float), hipMemcpyDeviceToHost));
     for (int e = 0; e < L.group_n; ++e) {
       const PairState& ps = ctx->h_pairs[i + e];
@@ -272,7 +336,7 @@ int kdh_batch_build(kdh_ctx* ctx, int n_pairs, const kdh_pair* pairs, int stats_
   if (n_pairs == 0) return KDH_OK;
   std::lock_guard<std::mutex> lk(ctx->mu);
   KDH_HIP_TRY(ctx, hipSetDevice(ctx->device));
-  Group g[2];
+  Group g[kGroups];
   std::vector<int> leader;
   int max_iter = 0;
   int rc = batch_setup(ctx, n_pairs, pairs, 1, stats_mode, g, leader, max_iter);
@@ -329,7 +393,7 @@ int kdh_batch_build_dual(kdh_ctx* ctx, int n_factors, const kdh_dual_pair* facto
   }
   std::lock_guard<std::mutex> lk(ctx->mu);
   KDH_HIP_TRY(ctx, hipSetDevice(ctx->device));
-  Group g[2];
+  Group g[kGroups];
   std::vector<int> leader;
   int max_iter = 0;
   int rc = batch_setup(ctx, n_factors, pairs.data(), 1, stats_mode, g, leader, max_iter, 1, vf.data());

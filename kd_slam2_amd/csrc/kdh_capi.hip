@@ -106,6 +106,12 @@ int kdh_ctx_destroy(kdh_ctx* ctx) {
   if (ctx->h_tile_pair) hipHostFree(ctx->h_tile_pair);
   for (int i = 0; i < kdh_ctx::kEvents; ++i)
     if (ctx->ev[i]) hipEventDestroy(ctx->ev[i]);
+  for (hipEvent_t e : {ctx->ev_fork, ctx->ev_stag, ctx->ev_join})
+    if (e) hipEventDestroy(e);
+  if (ctx->stream2) {
+    hipStreamSynchronize(ctx->stream2);
+    hipStreamDestroy(ctx->stream2);
+  }
   hipStreamDestroy(ctx->stream);
   delete ctx;
   return KDH_OK;
@@ -124,6 +130,23 @@ int kdh_set_launch_config(kdh_ctx* ctx, int grid, int lds_nodes) {
   std::lock_guard<std::mutex> lk(ctx->mu);
   ctx->grid_override = grid;
   ctx->lds_nodes = lds_nodes;
+  return KDH_OK;
+}
+
+int kdh_set_batch_streams(kdh_ctx* ctx, int streams) {
+  if (!ctx) return KDH_ERR_INVALID;
+  if (streams != 1 && streams != 2) return kdh_fail(ctx, KDH_ERR_INVALID, "batch streams must be 1 or 2");
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  ctx->batch_streams = streams;
+  return KDH_OK;
+}
+
+int kdh_set_leaves_per_lane(kdh_ctx* ctx, int leaves_per_lane) {
+  if (!ctx) return KDH_ERR_INVALID;
+  if (leaves_per_lane != 1 && leaves_per_lane != 2)
+    return kdh_fail(ctx, KDH_ERR_INVALID, "leaves_per_lane must be 1 or 2");
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  ctx->leaves_per_lane = leaves_per_lane;
   return KDH_OK;
 }
 
@@ -415,10 +438,10 @@ int kdh_fill_pair(kdh_ctx* ctx, PairState& ps, const kdh_tree* fixed, const kdh_
 }
 
 int kdh_launch(kdh_ctx* ctx, PairState* d_pairs, const int* d_tile_pair, int n_tiles, int ct, int* corr,
-               int* depth, float* partials, int n_pairs, int phases, int dual) {
+               int* depth, float* partials, int n_pairs, int phases, int dual, int pair_first, hipStream_t stream) {
   LaunchArgs a;
   a.dual = dual;
-  a.pair_first = 0;
+  a.pair_first = pair_first;
   a.n_pairs = n_pairs;
   a.phases = phases;
   a.pair_tile_first = nullptr;
@@ -436,8 +459,9 @@ int kdh_launch(kdh_ctx* ctx, PairState* d_pairs, const int* d_tile_pair, int n_t
   if (ct && lds > 1024) lds = 1024;
   if (dual && lds > 512) lds = 512;
   a.lds_nodes = lds;
+  a.leaves_per_lane = dual ? 1 : ctx->leaves_per_lane;
   if (n_tiles <= 0) return KDH_OK;
-  hipError_t e = launch_tiles(a, ctx->stream);
+  hipError_t e = launch_tiles(a, stream ? stream : ctx->stream);
   if (e != hipSuccess) return kdh_hip_fail(ctx, e, "tile kernel launch");
   return KDH_OK;
 }
@@ -625,8 +649,9 @@ int kdh_align(kdh_ctx* ctx, const kdh_tree* fixed, const kdh_tree* moving, int c
     if (!prof && !cache_log) {
       // Production path: every iteration captured in one graph, one launch;
       // the executable graph is cached while the launch shape is unchanged.
-      const long long key[12] = {nt, ct, iters, ctx->lds_nodes, ctx->grid_override, ctx->xcd_remap,
-                                 (long long)(intptr_t)ctx->d_pairs, (long long)(intptr_t)ctx->d_partials, 1, 0, 0, 0};
+      const long long key[16] = {nt, ct, iters, ctx->lds_nodes, ctx->grid_override, ctx->xcd_remap,
+                                 (long long)(intptr_t)ctx->d_pairs, (long long)(intptr_t)ctx->d_partials, 1,
+                                 ctx->leaves_per_lane, 0, 0, 0, 0, 0, 0};
       hipGraphExec_t exec = kdh_graph_lookup(ctx->graph_align, key);
       if (!exec) {
         hipGraph_t graph = nullptr;

@@ -56,13 +56,18 @@ struct kdh_ctx {
   // Instantiated graphs, reused while the launch shape and buffers match.
   struct GraphSlot {
     hipGraphExec_t exec = nullptr;
-    long long key[12] = {};
+    long long key[16] = {};
   };
   GraphSlot graph_align, graph_batch;
   // launch configuration (autotuner)
   int grid_override = 0;
   int lds_nodes = 256;  // measured: deeper staging costs more per tile than its LDS hits save
   int leaves_per_lane = 1;  // moving leaves per lane of the tile kernel: 1 or 2
+  int batch_streams = 1;    // kdh_batch_align replay on 1 or 2 streams
+  // second stream (default priority) and fork / stagger / join events of the
+  // two-stream batch replay, made on first use
+  hipStream_t stream2 = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_stag = nullptr, ev_join = nullptr;
   int xcd_remap = 1;
   // timing of the last device loop
   static constexpr int kEvents = 2 * kdh::kMaxIter + 2;
@@ -95,7 +100,8 @@ int kdh_fill_pair(kdh_ctx* ctx, kdh::PairState& ps, const kdh_tree* fixed, const
 void kdh_cache_from_state(kdh::PairState& ps);
 int kdh_setup_prior(kdh::PairState& ps, const kdh_prior* prior);
 int kdh_launch(kdh_ctx* ctx, kdh::PairState* d_pairs, const int* d_tile_pair, int n_tiles, int ct, int* corr,
-               int* depth, float* partials, int n_pairs, int phases = 3, int dual = 0);
+               int* depth, float* partials, int n_pairs, int phases = 3, int dual = 0, int pair_first = 0,
+               hipStream_t stream = nullptr);
 int kdh_result_code(const kdh::PairState& ps, int max_iterations_param);
 void kdh_collect_times(kdh_ctx* ctx, int iters, bool prof);
 // Returns the cached executable graph for `key`, or nullptr (and drops the

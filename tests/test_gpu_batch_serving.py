@@ -190,3 +190,23 @@ def test_server_destroy_while_a_thread_waits(ctx, small_pairs):
             t.join(30.0)
             assert not t.is_alive()
         assert len(codes) == 2 and all(c in (0, -6) for c in codes), codes
+
+
+def test_two_stream_batch_is_bit_identical(ctx, small_pairs):
+    """kdh_set_batch_streams(2): the pairs split in two halves replayed on two
+    streams give the same bits as one stream, rigid and CT pairs mixed."""
+    trees = [(DeviceTree(ctx, w["fixed"]), DeviceTree(ctx, w["moving"])) for w in small_pairs]
+    pairs = _pairs(ctx, small_pairs, trees)
+    try:
+        one = batch_align(ctx, pairs)
+        ctx.set_launch_config(batch_streams=2)
+        two = batch_align(ctx, pairs)
+        two_again = batch_align(ctx, pairs)
+    finally:
+        ctx.set_launch_config(batch_streams=1)
+    for a, b, c in zip(one, two, two_again):
+        assert np.array_equal(a["X"], b["X"]) and np.array_equal(a["v"], b["v"]) and a["last"] == b["last"]
+        assert a["result"] == b["result"] and np.array_equal(b["X"], c["X"])
+    for F, M in trees:
+        F.close()
+        M.close()
