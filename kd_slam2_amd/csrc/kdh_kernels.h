@@ -24,6 +24,7 @@ struct LaunchArgs {
   const int* pair_tile_first;  // optional: first tile of each pair in this launch (else PairState::tile_first)
   int* status_out;        // optional: per-pair status mirror written by the finish stage
   int dual = 0;           // 1: CT dual form of a bundle-adjustment factor (build mode, ct = 1)
+  int leaves_per_lane = 1;  // 1, or 2: two descents in flight per lane (not the dual form)
 };
 
 hipError_t launch_tiles(const LaunchArgs& a, hipStream_t s);

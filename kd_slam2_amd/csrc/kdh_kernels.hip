@@ -40,9 +40,6 @@
 
 namespace kdh {
 
-constexpr int kThreads = 256;
-constexpr int kWaves = kThreads / 64;
-
 template <bool CT> struct Slots;
 template <> struct Slots<false> { static constexpr int N = kIcpSlots; static constexpr int P = 64; };
 template <> struct Slots<true> { static constexpr int N = kCtSlots; static constexpr int P = 128; };
@@ -595,10 +592,15 @@ __device__ void finish_pair(PairState& ps, const PairState& S, const float* __re
 
 // ---- the tile kernel --------------------------------------------------------
 
+// Workgroup of the tile kernel: 256 / QPL threads, each lane carrying QPL of
+// the tile's 256 moving leaves (QPL = 2: two descents in flight per lane).
+template <int QPL> constexpr int tile_threads() { return kTile / QPL; }
+
 // DUAL (CT only): the dual form of quadraticTermDual, built for bundle
 // adjustment; three MFMA products per wave instead of one.
-template <bool CT, int LDS_NODES, bool DUAL>
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(DUAL ? 3 : 4))) void tiles_kernel(const PairState* __restrict__ pairs,
+// QPL: moving leaves per lane (1, or 2 for the rigid and CT forms).
+template <bool CT, int LDS_NODES, bool DUAL, int QPL>
+__global__ __launch_bounds__(kTile / QPL) __attribute__((amdgpu_waves_per_eu((DUAL || (CT && QPL == 2)) ? 3 : 4))) void tiles_kernel(const PairState* __restrict__ pairs,
                                                          const int* __restrict__ tile_pair,
                                                          int n_tiles_total,
                                                          float* __restrict__ partials,
@@ -607,6 +609,9 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(DUAL ?
                                                          int xcd_remap,
                                                          const int* __restrict__ pair_tile_first) {
   static_assert(CT || !DUAL, "the dual form is a CT form");
+  static_assert(QPL == 1 || (QPL == 2 && !DUAL), "two leaves per lane: rigid and CT forms");
+  constexpr int NT = kTile / QPL;      // threads
+  constexpr int NWV = NT / 64;         // waves
   constexpr int N = n_slots<CT, DUAL>();
   constexpr int E = CT ? 12 : 6;       // component of sqrt(g) e in w
   constexpr int NW = E + 2;            // components of w
@@ -616,8 +621,8 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(DUAL ?
   constexpr int WBUF = DUAL ? 3 * 256 : HW * WS;  // per wave: transpose, then the C tile(s)
   static_assert(HW * WS >= 256 && WBUF >= HW * WS, "the per-wave C tiles reuse the transpose buffer");
   __shared__ float4 s_nodes[LDS_NODES > 0 ? 2 * LDS_NODES : 2];
-  __shared__ __align__(16) float s_w[kWaves][WBUF];
-  __shared__ int s_cnt[kWaves][5];
+  __shared__ __align__(16) float s_w[NWV][WBUF];
+  __shared__ int s_cnt[NWV][5];
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
@@ -653,11 +658,18 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(DUAL ?
     const DevTree moving = ps.moving;
     const int tfirst = pair_tile_first ? pair_tile_first[p] : ps.tile_first;
     if (status != 0) continue;
-    const int q = (t - tfirst) * kTile + tid;
-    const bool live = q < moving.n_leaves && fixed.n_nodes > 0;
-    const int qc = live ? q : 0;  // a tile exists, so the pair has >= 1 leaf
-    const float4 la = moving.leaves[2 * qc];
-    const float4 lb = moving.leaves[2 * qc + 1];
+    // lane `tid` carries leaves q[k] = tile base + k * NT + tid
+    int q[QPL];
+    bool live[QPL];
+    float4 la[QPL], lb[QPL];
+#pragma unroll
+    for (int k = 0; k < QPL; ++k) {
+      q[k] = (t - tfirst) * kTile + k * NT + tid;
+      live[k] = q[k] < moving.n_leaves && fixed.n_nodes > 0;
+      const int qc = live[k] ? q[k] : 0;  // a tile exists, so the pair has >= 1 leaf
+      la[k] = moving.leaves[2 * qc];
+      lb[k] = moving.leaves[2 * qc + 1];
+    }
     // the cache and term parameters in the same round trip as the leaves
     // and the staging, ahead of the barrier
     const TermCache C = ps.cache;
@@ -665,90 +677,127 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(DUAL ?
     if (LDS_NODES > 0 && p != cached_pair) {
       __syncthreads();
       n_lds = fixed.n_nodes < LDS_NODES ? fixed.n_nodes : LDS_NODES;
-      for (int i = tid; i < 2 * n_lds; i += kThreads) s_nodes[i] = fixed.nodes[i];
+      for (int i = tid; i < 2 * n_lds; i += NT) s_nodes[i] = fixed.nodes[i];
       cached_pair = p;
       __syncthreads();
     }
 
-    TermRec R;
-    rec_reject(R, -1);  // padding lanes contribute nothing
+    TermRec R[QPL];
     float Jb[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
-    if (live) {
-      LeafIn L;
-      L.mean = v3(la.x, la.y, la.z);
-      L.dir = v3(la.w, lb.x, lb.y);
-      L.dts = lb.z;
-      int leaf, depth = 0;
-      const float* fdts = fixed.node_dts;
-      auto dts_of = [fdts](int i) { return fdts[i]; };
-      if (LDS_NODES > 0) {
-        LdsFetch f{fixed.nodes, (lds_f4*)s_nodes, n_lds};
-        leaf = CT ? ct_term<DUAL>(f, C, TP, L, R, &depth, Jb, dts_of) : icp_term(f, C, TP, L, R, &depth);
+    LeafIn L[QPL];
+#pragma unroll
+    for (int k = 0; k < QPL; ++k) {
+      rec_reject(R[k], -1);  // padding lanes contribute nothing
+      L[k].mean = v3(la[k].x, la[k].y, la[k].z);
+      L[k].dir = v3(la[k].w, lb[k].x, lb[k].y);
+      L[k].dts = lb[k].z;
+    }
+    const float* fdts = fixed.node_dts;
+    auto dts_of = [fdts](int i) { return fdts[i]; };
+    auto leaf_terms = [&](const auto& f) {
+      if constexpr (QPL == 1) {
+        if (live[0]) {
+          int depth = 0;
+          const int leaf = CT ? ct_term<DUAL>(f, C, TP, L[0], R[0], &depth, Jb, dts_of)
+                              : icp_term(f, C, TP, L[0], R[0], &depth);
+          if (DUAL && R[0].outcome != kInlier && R[0].outcome != kOutlier)
+            for (int i = 0; i < 6; ++i) Jb[i] = 0.0f;
+          if (corr_out) corr_out[q[0]] = leaf;
+          if (depth_out) depth_out[q[0]] = depth;
+        }
       } else {
-        GlobalFetch f{fixed.nodes};
-        leaf = CT ? ct_term<DUAL>(f, C, TP, L, R, &depth, Jb, dts_of) : icp_term(f, C, TP, L, R, &depth);
+        // both query points, both descents in lockstep, then both terms
+        const V3f zero = v3(0.0f, 0.0f, 0.0f);
+        const V3f q0 = live[0] ? (CT ? ct_query(C, L[0]) : icp_query(C, L[0])) : zero;
+        const V3f q1 = live[1] ? (CT ? ct_query(C, L[1]) : icp_query(C, L[1])) : zero;
+        int leaf0, leaf1, d0, d1;
+        float4 a0, b0, a1, b1;
+        descend2(f, q0, q1, live[0], live[1], leaf0, leaf1, d0, d1, a0, b0, a1, b1);
+        if (live[0]) {
+          if constexpr (CT) ct_eval<false>(C, TP, L[0], leaf0, a0, b0, R[0]);
+          else icp_eval(C, TP, L[0], q0, leaf0, a0, b0, R[0]);
+          if (corr_out) corr_out[q[0]] = leaf0;
+          if (depth_out) depth_out[q[0]] = d0;
+        }
+        if (live[1]) {
+          if constexpr (CT) ct_eval<false>(C, TP, L[1], leaf1, a1, b1, R[1]);
+          else icp_eval(C, TP, L[1], q1, leaf1, a1, b1, R[1]);
+          if (corr_out) corr_out[q[1]] = leaf1;
+          if (depth_out) depth_out[q[1]] = d1;
+        }
       }
-      if (DUAL && R.outcome != kInlier && R.outcome != kOutlier)
-        for (int i = 0; i < 6; ++i) Jb[i] = 0.0f;
-      if (corr_out) corr_out[q] = leaf;
-      if (depth_out) depth_out[q] = depth;
+    };
+    if (LDS_NODES > 0) {
+      leaf_terms(LdsFetch{fixed.nodes, (lds_f4*)s_nodes, n_lds});
+    } else {
+      leaf_terms(GlobalFetch{fixed.nodes});
     }
 
     // Normal equations of the tile on the matrix cores.  With
     // w = [sqrt(g) J, sqrt(g) Jv (CT), sqrt(g) e, e] per leaf, every
     // accumulator slot is an entry of C = sum_leaves w w^T: the 6x6 (and CT
     // 12x12) blocks of H, b, sum g e^2 (chi2 kernelised) and sum e^2 (chi2).
-    // Per wave C is 16 v_mfma_f32_16x16x4f32 with the 64 leaves as the
-    // K dimension; the leaf vectors are transposed through LDS into the
-    // operand layout (lane i + 16k holds component i of leaf 16k + m for
+    // Per wave C is 16 v_mfma_f32_16x16x4f32 per 64 leaves with the leaves
+    // as the K dimension; the leaf vectors are transposed through LDS into
+    // the operand layout (lane i + 16k holds component i of leaf 16k + m for
     // step m).  The outcome counts come from ballots.
     {
-      float w[NW];
-      const float sg = sqrtf(R.gamma);
-#pragma unroll
-      for (int i = 0; i < 6; ++i) w[i] = sg * R.J[i];
-      if (CT) {
-#pragma unroll
-        for (int i = 0; i < 6; ++i) w[6 + i] = sg * R.Jv[i];
-      }
-      w[E] = sg * R.e;
-      w[E + 1] = R.e;
-      // The transpose goes through LDS in two halves of HW components, so
-      // the buffer stays small enough for four workgroups per CU.
       float* sw = s_w[wave];
       const int ci = lane & 15, cg = lane >> 4;
-      float a[16];
-#pragma unroll
-      for (int m = 0; m < 16; ++m) a[m] = 0.0f;
-#pragma unroll
-      for (int half = 0; half < 2; ++half) {
-#pragma unroll
-        for (int i = 0; i < HW; ++i)
-          if (half * HW + i < NW) sw[i * WS + lane] = w[half * HW + i];
-        wave_lds_sync();
-        if (ci >= half * HW && ci < half * HW + HW && ci < NW) {
-          const float4* src = reinterpret_cast<const float4*>(sw + (ci - half * HW) * WS + cg * 16);
-#pragma unroll
-          for (int m4 = 0; m4 < 4; ++m4) {
-            const float4 v = src[m4];
-            a[4 * m4 + 0] = v.x;
-            a[4 * m4 + 1] = v.y;
-            a[4 * m4 + 2] = v.z;
-            a[4 * m4 + 3] = v.w;
-          }
-        }
-        wave_lds_sync();
-      }
       floatx4 c0 = {0.0f, 0.0f, 0.0f, 0.0f}, c1 = {0.0f, 0.0f, 0.0f, 0.0f};
+      float a[16];
+      int cnt[5] = {0, 0, 0, 0, 0};
 #pragma unroll
-      for (int m = 0; m < 16; m += 2) {
-        c0 = __builtin_amdgcn_mfma_f32_16x16x4f32(a[m], a[m], c0, 0, 0, 0);
-        c1 = __builtin_amdgcn_mfma_f32_16x16x4f32(a[m + 1], a[m + 1], c1, 0, 0, 0);
+      for (int k = 0; k < QPL; ++k) {
+        float w[NW];
+        const float sg = sqrtf(R[k].gamma);
+#pragma unroll
+        for (int i = 0; i < 6; ++i) w[i] = sg * R[k].J[i];
+        if (CT) {
+#pragma unroll
+          for (int i = 0; i < 6; ++i) w[6 + i] = sg * R[k].Jv[i];
+        }
+        w[E] = sg * R[k].e;
+        w[E + 1] = R[k].e;
+        // The transpose goes through LDS in two halves of HW components, so
+        // the buffer stays small enough for four workgroups per CU.
+#pragma unroll
+        for (int m = 0; m < 16; ++m) a[m] = 0.0f;
+#pragma unroll
+        for (int half = 0; half < 2; ++half) {
+#pragma unroll
+          for (int i = 0; i < HW; ++i)
+            if (half * HW + i < NW) sw[i * WS + lane] = w[half * HW + i];
+          wave_lds_sync();
+          if (ci >= half * HW && ci < half * HW + HW && ci < NW) {
+            const float4* src = reinterpret_cast<const float4*>(sw + (ci - half * HW) * WS + cg * 16);
+#pragma unroll
+            for (int m4 = 0; m4 < 4; ++m4) {
+              const float4 v = src[m4];
+              a[4 * m4 + 0] = v.x;
+              a[4 * m4 + 1] = v.y;
+              a[4 * m4 + 2] = v.z;
+              a[4 * m4 + 3] = v.w;
+            }
+          }
+          wave_lds_sync();
+        }
+#pragma unroll
+        for (int m = 0; m < 16; m += 2) {
+          c0 = __builtin_amdgcn_mfma_f32_16x16x4f32(a[m], a[m], c0, 0, 0, 0);
+          c1 = __builtin_amdgcn_mfma_f32_16x16x4f32(a[m + 1], a[m + 1], c1, 0, 0, 0);
+        }
+        cnt[kInlier] += __popcll(__ballot(R[k].outcome == kInlier));
+        cnt[kOutlier] += __popcll(__ballot(R[k].outcome == kOutlier));
+        cnt[kBad] += __popcll(__ballot(R[k].outcome == kBad));
+        cnt[kErrNormal] += __popcll(__ballot(R[k].outcome == kErrNormal));
+        cnt[kErrFn] += __popcll(__ballot(R[k].outcome == kErrFn));
       }
       const floatx4 c = c0 + c1;
       if constexpr (DUAL) {
         // z = sqrt(g) J_vel_B: a third transpose, then C_uz = sum w z^T and
-        // C_zz = sum z z^T
+        // C_zz = sum z z^T (QPL = 1: a[] still holds this lane's w operands)
+        const float sg = sqrtf(R[0].gamma);
         float az[16];
 #pragma unroll
         for (int m = 0; m < 16; ++m) az[m] = 0.0f;
@@ -779,30 +828,24 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(DUAL ?
       // lane (ci, cg) holds C[4 cg + r][ci], stored at ci * 16 + 4 cg + r,
       // in this wave's part of the transpose buffer
       *reinterpret_cast<float4*>(&sw[ci * 16 + 4 * cg]) = make_float4(c[0], c[1], c[2], c[3]);
-      const unsigned long long m_in = __ballot(R.outcome == kInlier);
-      const unsigned long long m_out = __ballot(R.outcome == kOutlier);
-      const unsigned long long m_bad = __ballot(R.outcome == kBad);
-      const unsigned long long m_nrm = __ballot(R.outcome == kErrNormal);
-      const unsigned long long m_fn = __ballot(R.outcome == kErrFn);
       if (lane == 0) {
-        s_cnt[wave][kInlier] = __popcll(m_in);
-        s_cnt[wave][kOutlier] = __popcll(m_out);
-        s_cnt[wave][kBad] = __popcll(m_bad);
-        s_cnt[wave][kErrNormal] = __popcll(m_nrm);
-        s_cnt[wave][kErrFn] = __popcll(m_fn);
+#pragma unroll
+        for (int k = 0; k < 5; ++k) s_cnt[wave][k] = cnt[k];
       }
     }
     __syncthreads();
-    if (tid < N) {
-      float s;
-      if (tid < CS + 2 || (DUAL && tid >= kCtSlots)) {
-        const int cell = (DUAL && tid >= kCtSlots) ? dual_slot_cell(tid) : slot_cell<CT>(tid);
-        s = (s_w[0][cell] + s_w[1][cell]) + (s_w[2][cell] + s_w[3][cell]);
+    for (int sl = tid; sl < N; sl += NT) {
+      float sv;
+      if (sl < CS + 2 || (DUAL && sl >= kCtSlots)) {
+        const int cell = (DUAL && sl >= kCtSlots) ? dual_slot_cell(sl) : slot_cell<CT>(sl);
+        if constexpr (NWV == 4) sv = (s_w[0][cell] + s_w[1][cell]) + (s_w[2][cell] + s_w[3][cell]);
+        else sv = s_w[0][cell] + s_w[1][cell];
       } else {
-        const int k = tid - CS - 2;
-        s = (float)((s_cnt[0][k] + s_cnt[1][k]) + (s_cnt[2][k] + s_cnt[3][k]));
+        const int k = sl - CS - 2;
+        if constexpr (NWV == 4) sv = (float)((s_cnt[0][k] + s_cnt[1][k]) + (s_cnt[2][k] + s_cnt[3][k]));
+        else sv = (float)(s_cnt[0][k] + s_cnt[1][k]);
       }
-      partials[(size_t)t * N + tid] = s;
+      partials[(size_t)t * N + sl] = sv;
     }
     __syncthreads();
   }
@@ -834,11 +877,12 @@ __global__ __launch_bounds__(64) void finish_kernel(PairState* __restrict__ pair
 
 // ---- launch wrappers --------------------------------------------------------
 
-template <bool CT, int LDS, bool DUAL>
+template <bool CT, int LDS, bool DUAL, int QPL>
 static hipError_t launch_t(const LaunchArgs& a, hipStream_t s) {
   if (a.phases & 1) {
-    hipLaunchKernelGGL((tiles_kernel<CT, LDS, DUAL>), dim3(a.grid), dim3(kThreads), 0, s, a.pairs, a.tile_pair,
-                       a.n_tiles_total, a.partials, a.corr_out, a.depth_out, a.xcd_remap, a.pair_tile_first);
+    hipLaunchKernelGGL((tiles_kernel<CT, LDS, DUAL, QPL>), dim3(a.grid), dim3(kTile / QPL), 0, s, a.pairs,
+                       a.tile_pair, a.n_tiles_total, a.partials, a.corr_out, a.depth_out, a.xcd_remap,
+                       a.pair_tile_first);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
   }
@@ -850,30 +894,35 @@ static hipError_t launch_t(const LaunchArgs& a, hipStream_t s) {
   return hipSuccess;
 }
 
+template <bool CT, int LDS>
+static hipError_t launch_q(const LaunchArgs& a, hipStream_t s) {
+  return a.leaves_per_lane == 2 ? launch_t<CT, LDS, false, 2>(a, s) : launch_t<CT, LDS, false, 1>(a, s);
+}
+
 hipError_t launch_tiles(const LaunchArgs& a, hipStream_t s) {
   if (a.dual) {
     // the dual form caches at most 512 tree nodes in LDS: its third C tile
     // takes the room of the larger caches
     switch (a.lds_nodes) {
-      case 0: return launch_t<true, 0, true>(a, s);
-      case 256: return launch_t<true, 256, true>(a, s);
-      default: return launch_t<true, 512, true>(a, s);
+      case 0: return launch_t<true, 0, true, 1>(a, s);
+      case 256: return launch_t<true, 256, true, 1>(a, s);
+      default: return launch_t<true, 512, true, 1>(a, s);
     }
   }
   if (a.ct) {
     switch (a.lds_nodes) {
-      case 0: return launch_t<true, 0, false>(a, s);
-      case 256: return launch_t<true, 256, false>(a, s);
-      case 512: return launch_t<true, 512, false>(a, s);
-      default: return launch_t<true, 1024, false>(a, s);
+      case 0: return launch_q<true, 0>(a, s);
+      case 256: return launch_q<true, 256>(a, s);
+      case 512: return launch_q<true, 512>(a, s);
+      default: return launch_q<true, 1024>(a, s);
     }
   }
   switch (a.lds_nodes) {
-    case 0: return launch_t<false, 0, false>(a, s);
-    case 256: return launch_t<false, 256, false>(a, s);
-    case 512: return launch_t<false, 512, false>(a, s);
-    case 1024: return launch_t<false, 1024, false>(a, s);
-    default: return launch_t<false, 2048, false>(a, s);
+    case 0: return launch_q<false, 0>(a, s);
+    case 256: return launch_q<false, 256>(a, s);
+    case 512: return launch_q<false, 512>(a, s);
+    case 1024: return launch_q<false, 1024>(a, s);
+    default: return launch_q<false, 2048>(a, s);
   }
 }
 
