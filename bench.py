@@ -475,6 +475,7 @@ def main():
         latency_ms=head["latency_ms"],
         served_speedup_vs_plain=head["speedup_vs_plain"],
         plain_pts_per_s=head["plain_pts_per_s"],
+        pcie_inclusive=head["pcie_inclusive"],
         accuracy=head["accuracy"],
         kernel_us_per_iter=head["kernel_us_per_iter"],
         pmc=pmc or None,
@@ -577,6 +578,15 @@ def run_headline(ctx, pairs3, args):
         hb.append((float(np.abs(al.H - ref["H64"]).max() / np.abs(ref["H64"]).max()),
                    float(np.abs(al.b - ref["b64"]).max() / np.abs(ref["b64"]).max())))
     n_gt = sum(1 for i, q in enumerate(pairs3) if synth.pose_error(out.X[i], q["X_gt"])[0] < 0.05)
+    # host-buffer boundary: every tree of the batch uploaded from host memory
+    # (record staging, allocation, PCIe copy) before the same batch step
+    t = time.perf_counter()
+    fresh = [(DeviceTree(ctx, q["fixed"]), DeviceTree(ctx, q["moving"])) for q in pairs3]
+    t_upload = time.perf_counter() - t
+    tree_bytes = int(sum(32 * (len(T.nodes) + len(T.leaves)) for q in pairs3 for T in (q["fixed"], q["moving"])))
+    for F, M in fresh:
+        F.close()
+        M.close()
     res = dict(
         value=round(value, 1), ms_per_step=round(1e3 * total / args.steps, 3), pairs=len(pairs3),
         scan_points=n_pts, moving_leaves=int(n_q), fixed_nodes=int(sum(len(q["fixed"].nodes) for q in pairs3)),
@@ -590,6 +600,9 @@ def run_headline(ctx, pairs3, args):
                       hb_sample_pairs=len(hb), within_5cm_of_ground_truth=n_gt),
         kernel_us_per_iter=dict(iteration=round(float(np.median(kern)), 2), tile_kernel=round(tile_us, 2),
                                 finish_kernel=round(fin_us, 2)),
+        pcie_inclusive=dict(upload_ms=round(1e3 * t_upload, 2), tree_MB=round(tree_bytes / 1e6, 1),
+                            upload_GBps=round(tree_bytes / t_upload / 1e9, 2),
+                            pts_per_s=round(n_pts / (t_upload + total / args.steps), 1)),
         tile_us=tile_us, demand_bytes_per_launch=demand)
     log(f"headline: tile {tile_us:.1f} us, finish {fin_us:.1f} us, plain {1e3 * t_plain:.1f} ms, accuracy "
         f"{res['accuracy']}")
