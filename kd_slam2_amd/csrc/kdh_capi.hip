@@ -453,7 +453,7 @@ int kdh_launch(kdh_ctx* ctx, PairState* d_pairs, const int* d_tile_pair, int n_t
   a.xcd_remap = ctx->xcd_remap;
   a.corr_out = corr;
   a.depth_out = depth;
-  a.grid = ctx->grid_override > 0 ? std::min(ctx->grid_override, n_tiles) : n_tiles;
+  a.grid = kdh_tile_grid(ctx, n_tiles);
   a.ct = ct;
   int lds = ctx->lds_nodes;
   if (ct && lds > 1024) lds = 1024;

@@ -103,6 +103,15 @@ int kdh_launch(kdh_ctx* ctx, kdh::PairState* d_pairs, const int* d_tile_pair, in
                int* depth, float* partials, int n_pairs, int phases = 3, int dual = 0, int pair_first = 0,
                hipStream_t stream = nullptr);
 int kdh_result_code(const kdh::PairState& ps, int max_iterations_param);
+// Workgroups of a tile launch over n_tiles tiles: the context's persistent
+// grid, or one per tile; with the XCD remap on, rounded up to a multiple of 8
+// so that every XCD takes a contiguous range of tiles (the extra workgroups of
+// the last range find no tile).
+inline int kdh_tile_grid(const kdh_ctx* ctx, int n_tiles) {
+  int g = ctx->grid_override > 0 ? (ctx->grid_override < n_tiles ? ctx->grid_override : n_tiles) : n_tiles;
+  if (ctx->xcd_remap && g >= 8) g = (g + 7) & ~7;
+  return g;
+}
 void kdh_collect_times(kdh_ctx* ctx, int iters, bool prof);
 // Returns the cached executable graph for `key`, or nullptr (and drops the
 // stale one) when the key differs.

@@ -640,7 +640,7 @@ __global__ __launch_bounds__(kTile / QPL) __attribute__((amdgpu_waves_per_eu((DU
   // of that alignment's tree in LDS once, not once per tile.
   const int G = gridDim.x;
   const bool remap = xcd_remap && (G % 8) == 0 && G >= 8;
-  const int lo_all = remap ? (int)(blockIdx.x & 7) * ((n_tiles_total + 7) / 8) : 0;
+  const int lo_all = remap ? min((int)(blockIdx.x & 7) * ((n_tiles_total + 7) / 8), n_tiles_total) : 0;
   const int hi_all = remap ? min(lo_all + (n_tiles_total + 7) / 8, n_tiles_total) : n_tiles_total;
   const int n_wg = remap ? G / 8 : G;
   const int j = remap ? (int)(blockIdx.x >> 3) : (int)blockIdx.x;

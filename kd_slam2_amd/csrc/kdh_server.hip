@@ -296,7 +296,7 @@ static void engine(kdh_server* S) {
         a.tile_pair = S->d_tile_pair + toff;
         a.n_tiles_total = n_t[k];
         a.partials = S->d_partials + poff;
-        a.grid = ctx->grid_override > 0 ? std::min(ctx->grid_override, n_t[k]) : n_t[k];
+        a.grid = kdh_tile_grid(ctx, n_t[k]);
         a.ct = k;
         a.lds_nodes = (k && ctx->lds_nodes > 1024) ? 1024 : ctx->lds_nodes;
         a.xcd_remap = ctx->xcd_remap;
