@@ -267,6 +267,26 @@ typedef struct kdh_pair_result {
  * kdh_batch_align; forest entries' results repeat their leader's pose. */
 int kdh_batch_align(kdh_ctx* ctx, int n_pairs, const kdh_pair* pairs, kdh_pair_result* results);
 
+/* A batch prepared once and run many times with new initial states: the
+ * loop-closure verifier's shape, the same candidate set re-verified from new
+ * guesses (the batched form of the sequential candidate loop,
+ * slam/slam_proc_loops_impl_.h:95-115).  kdh_batch_prepare validates and
+ * uploads the pairs as kdh_batch_align does and keeps them on the device with
+ * their own buffers; the trees must outlive the batch.  kdh_batch_run replays
+ * one graph holding the whole run -- the states initialised on the device
+ * from X0 (n_pairs x 12, row-major 3x4; NULL keeps the previous run's, first
+ * the pairs' own) and v0 (n_pairs x 6, CT pairs; NULL likewise), every
+ * iteration, the results written back to pinned host memory -- so a run is one
+ * graph launch and one synchronisation.  Results are bit-identical to
+ * kdh_batch_align on the same pairs and initial states (forest entries take
+ * their leader's X0 / v0).  The graph follows the context's launch
+ * configuration at run time.  Runs of one batch, and calls on its context,
+ * are serialised by the context. */
+typedef struct kdh_batch kdh_batch;
+int kdh_batch_prepare(kdh_ctx* ctx, int n_pairs, const kdh_pair* pairs, kdh_batch** out);
+int kdh_batch_run(kdh_batch* batch, const double* X0, const double* v0, kdh_pair_result* results);
+int kdh_batch_release(kdh_batch* batch);
+
 /* One quadratic form per pair (or forest) at its X0 / v0 / Xf, all in one
  * tile launch and one finish launch: the per-factor builds of a bundle
  * adjustment batched (MultiViewICPFactor_::compute calls buildQuadraticForm

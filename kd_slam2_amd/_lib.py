@@ -263,6 +263,9 @@ _SIGS = {
     ),
     "kdh_align_noalloc_max_leaves": (C.c_int, []),
     "kdh_batch_align": (C.c_int, [P, C.c_int, C.POINTER(Pair), C.POINTER(PairResult)]),
+    "kdh_batch_prepare": (C.c_int, [P, C.c_int, C.POINTER(Pair), C.POINTER(P)]),
+    "kdh_batch_run": (C.c_int, [P, F64P, F64P, C.POINTER(PairResult)]),
+    "kdh_batch_release": (C.c_int, [P]),
     "kdh_batch_build": (C.c_int, [P, C.c_int, C.POINTER(Pair), C.c_int, F64P, F64P, C.POINTER(Stats)]),
     "kdh_batch_build_dual": (C.c_int, [P, C.c_int, C.POINTER(DualPair), C.c_int, C.POINTER(DualForm)]),
     "kdh_last_kernel_time": (C.c_int, [P, C.POINTER(C.c_double), C.POINTER(C.c_int)]),

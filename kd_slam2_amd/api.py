@@ -8,6 +8,7 @@ icp/icp_.h:53-161; CTICP_, cticp/cticp_.h:20-261), exposed here as
 GPU; this module only marshals arrays.
 """
 import ctypes as C
+import weakref
 import math
 
 import numpy as np
@@ -102,9 +103,12 @@ class Context:
             raise RuntimeError(f"kdh_ctx_create({device}) failed: {L.ERRORS.get(rc, rc)}")
         self.h = h
         self.device = device
+        self._batches = weakref.WeakSet()  # PairBatch objects prepared on this context
 
     def close(self):
         if self.h:
+            for b in list(self._batches):
+                b._release(self)
             L.lib().kdh_ctx_destroy(self.h)
             self.h = None
 
@@ -825,17 +829,43 @@ class BatchResult:
     dict (X, v, iterations, result, last, entry)."""
 
     def __init__(self, R):
-        n = len(R)
-        self.X = np.tile(np.eye(4), (n, 1, 1))
-        self.X[:, :3, :4] = R["X"].reshape(n, 3, 4)
-        self.v = R["v"].copy()
-        self.iterations = R["iterations"].copy()
-        self.result = R["result"].copy()
-        self.last = R["last"].copy()
-        self.entry = R["entry"].copy()
+        # one byte copy of the result records (a field-wise structured copy
+        # costs ten times more); the arrays are views of it
+        self._R = np.frombuffer(R.tobytes(), dtype=R.dtype)
+        self._X = None
+
+    @property
+    def X(self):
+        if self._X is None:
+            n = len(self._R)
+            X = np.zeros((n, 4, 4))
+            X[:, :3, :4] = self._R["X"].reshape(n, 3, 4)
+            X[:, 3, 3] = 1.0
+            self._X = X
+        return self._X
+
+    @property
+    def v(self):
+        return self._R["v"]
+
+    @property
+    def iterations(self):
+        return self._R["iterations"]
+
+    @property
+    def result(self):
+        return self._R["result"]
+
+    @property
+    def last(self):
+        return self._R["last"]
+
+    @property
+    def entry(self):
+        return self._R["entry"]
 
     def __len__(self):
-        return len(self.X)
+        return len(self._R)
 
     def __getitem__(self, i):
         return dict(X=self.X[i], v=self.v[i], iterations=int(self.iterations[i]), result=int(self.result[i]),
@@ -851,10 +881,13 @@ class BatchResult:
 class PairBatch:
     """A batch of alignments marshalled once into the kdh_pair array: trees,
     fixed poses, parameters, priors and forests (the `batch_align` pair
-    dicts).  set_initial() replaces the initial poses / velocities of every
-    pair (one array assignment), run() calls kdh_batch_align and returns a
-    BatchResult.  The serving loop's shape: a resident set of alignments
-    whose initial states change from call to call."""
+    dicts).  The first run() on a context prepares it there
+    (kdh_batch_prepare: validated, uploaded, one graph for the whole run);
+    set_initial() replaces the initial poses / velocities of every pair, and
+    each run() is one kdh_batch_run returning a BatchResult.  The serving
+    loop's shape: a resident set of alignments whose initial states change
+    from call to call.  Results equal kdh_batch_align's bit for bit.  The
+    trees must stay open while the batch runs."""
 
     def __init__(self, pairs):
         global _PAIR_DT, _RES_DT
@@ -864,17 +897,47 @@ class PairBatch:
         self._res = (L.PairResult * max(self.n, 1))()
         self._R = np.frombuffer(self._res, dtype=_RES_DT)[:self.n]
         self._keep = list(pairs)  # the trees and priors stay alive with the batch
+        self._X0 = np.array(self._A["X0"], dtype=np.float64)  # (n, 12), the run inputs
+        self._v0 = np.array(self._A["v0"], dtype=np.float64)  # (n, 6)
+        self._prep = {}  # id(ctx) -> (ctx, kdh_batch handle)
 
     def set_initial(self, X0=None, v0=None):
         """X0: (n, 4, 4) or (n, 3, 4) poses; v0: (n, 6) velocities."""
         if X0 is not None:
-            self._A["X0"] = np.asarray(X0, dtype=np.float64)[:, :3, :4].reshape(self.n, 12)
+            self._X0[:] = np.asarray(X0, dtype=np.float64)[:, :3, :4].reshape(self.n, 12)
+            self._A["X0"] = self._X0
         if v0 is not None:
-            self._A["v0"] = np.asarray(v0, dtype=np.float64).reshape(self.n, 6)
+            self._v0[:] = np.asarray(v0, dtype=np.float64).reshape(self.n, 6)
+            self._A["v0"] = self._v0
 
     def run(self, ctx):
-        ctx.check(L.lib().kdh_batch_align(ctx.h, self.n, self._arr, self._res), "kdh_batch_align")
+        if self.n == 0:
+            return BatchResult(self._R)
+        e = self._prep.get(id(ctx))
+        if e is None or e[0] is not ctx:
+            h = C.c_void_p()
+            ctx.check(L.lib().kdh_batch_prepare(ctx.h, self.n, self._arr, C.byref(h)), "kdh_batch_prepare")
+            e = (ctx, h)
+            self._prep[id(ctx)] = e
+            ctx._batches.add(self)
+        rc = L.lib().kdh_batch_run(e[1], self._X0.ctypes.data_as(L.F64P), self._v0.ctypes.data_as(L.F64P), self._res)
+        ctx.check(rc, "kdh_batch_run")
         return BatchResult(self._R)
+
+    def _release(self, ctx):
+        e = self._prep.pop(id(ctx), None)
+        if e is not None and e[0].h:
+            L.lib().kdh_batch_release(e[1])
+
+    def close(self):
+        for ctx, _ in list(self._prep.values()):
+            self._release(ctx)
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def batch_align(ctx, pairs):
@@ -885,9 +948,13 @@ def batch_align(ctx, pairs):
     with the next k pairs).  Returns a list of dicts (X, v, iterations,
     result, last, entry).  The C structs are filled and read through numpy
     views, column by column (PairBatch keeps them between calls)."""
-    if len(pairs) == 0:
+    n = len(pairs)
+    if n == 0:
         return []
-    return list(PairBatch(pairs).run(ctx))
+    arr = _pair_array(pairs)
+    res = (L.PairResult * n)()
+    ctx.check(L.lib().kdh_batch_align(ctx.h, n, arr, res), "kdh_batch_align")
+    return list(BatchResult(np.frombuffer(res, dtype=_RES_DT)[:n]))
 
 
 # ---------------------------------------------------------------------------

@@ -47,17 +47,17 @@ struct Group {
 
 }  // namespace
 
-// Pair states, forests and the per-kind tile lists of a batch, uploaded.
-// mode 0: align (solve + update each iteration); mode 1: build the form once.
-// dual: the CT pairs build the dual form, with the fixed velocities vf
-// (n_pairs x 6).  Called with ctx->mu held and the device set.
-// halves = 2: the pairs are split into two ranges of about equal tile count
-// (group ct + 2 * half), for the two-stream replay of kdh_batch_align.
-static int batch_setup(kdh_ctx* ctx, int n_pairs, const kdh_pair* pairs, int mode, int stats_mode, Group* g,
-                       std::vector<int>& leader, int& max_iter, int dual = 0, const double* vf = nullptr,
-                       int halves = 1) {
-  int rc = kdh_ensure_pairs(ctx, (size_t)n_pairs);
-  if (rc) return rc;
+// Pair states (into hs[0 .. n_pairs)), forests and the per-kind tile lists
+// of a batch, on the host.  mode 0: align (solve + update each iteration);
+// mode 1: build the form once.  dual: the CT pairs build the dual form, with
+// the fixed velocities vf (n_pairs x 6).  halves = 2: the pairs are split
+// into two ranges of about equal tile count (group ct + 2 * half), for the
+// two-stream replay.  tp: the tile list (pair of each tile, groups one after
+// another); n_partials: floats of the partial rows.
+static int batch_plan(kdh_ctx* ctx, int n_pairs, const kdh_pair* pairs, int mode, int stats_mode, Group* g,
+                      std::vector<int>& leader, int& max_iter, int dual, const double* vf, int halves,
+                      PairState* hs, std::vector<int>& tp, size_t& n_partials) {
+  int rc = KDH_OK;
   for (int k = 0; k < kGroups; ++k) {
     g[k] = Group();
     g[k].ct = k & 1;
@@ -82,7 +82,7 @@ static int batch_setup(kdh_ctx* ctx, int n_pairs, const kdh_pair* pairs, int mod
   for (int i = 0; i < n_pairs; ++i) {
     const kdh_pair& lp = pairs[leader[i]];  // pose, velocities, params, prior
     const kdh_pair& pr = pairs[i];           // fixed tree and pose
-    PairState& ps = ctx->h_pairs[i];
+    PairState& ps = hs[i];
     rc = kdh_fill_pair(ctx, ps, pr.fixed, lp.moving, lp.ct, lp.params, stats_mode, mode);
     if (rc) return kdh_fail(ctx, rc, "pair " + std::to_string(i) + ": " + ctx->err);
     memcpy(ps.X, lp.X0, sizeof(ps.X));
@@ -94,17 +94,17 @@ static int batch_setup(kdh_ctx* ctx, int n_pairs, const kdh_pair* pairs, int mod
       for (int k = 0; k < 6; ++k) ps.cache.vf[k] = (float)vf[6 * i + k];
     if (leader[i] != i) {
       ps.group_n = 0;
-      PairState& L = ctx->h_pairs[leader[i]];
+      PairState& L = hs[leader[i]];
       L.group_n += 1;
       L.group_tiles += ps.n_tiles;
     }
     max_iter = std::max(max_iter, ps.sp.max_iterations);
   }
   for (int i = 0; i < n_pairs; ++i) {
-    const PairState& L = ctx->h_pairs[leader[i]];
+    const PairState& L = hs[leader[i]];
     if (L.group_tiles == 0) {
       // no moving leaves: the first round fails the min_good gate
-      PairState& ps = ctx->h_pairs[i];
+      PairState& ps = hs[i];
       ps.iter = 1;
       ps.status = L.sp.min_good_measurements > 0 ? 2 : 1;
     }
@@ -114,13 +114,13 @@ static int batch_setup(kdh_ctx* ctx, int n_pairs, const kdh_pair* pairs, int mod
   int split = n_pairs;
   if (halves == 2) {
     long long total = 0, acc = 0;
-    for (int i = 0; i < n_pairs; ++i) total += ctx->h_pairs[i].n_tiles;
+    for (int i = 0; i < n_pairs; ++i) total += hs[i].n_tiles;
     for (int i = 0; i < n_pairs; ++i) {
       if (leader[i] == i && i > 0 && 2 * acc >= total) {
         split = i;
         break;
       }
-      acc += ctx->h_pairs[i].n_tiles;
+      acc += hs[i].n_tiles;
     }
   }
   for (int k = 0; k < kGroups; ++k) {
@@ -128,28 +128,45 @@ static int batch_setup(kdh_ctx* ctx, int n_pairs, const kdh_pair* pairs, int mod
     g[k].p_hi = k < 2 ? split : n_pairs;
   }
   for (int i = 0; i < n_pairs; ++i) {
-    PairState& ps = ctx->h_pairs[i];
+    PairState& ps = hs[i];
     Group& G = g[ps.ct + (i >= split ? 2 : 0)];
     ps.tile_first = G.n_tiles;
     G.n_tiles += ps.n_tiles;
   }
-  size_t n_partials = 0, n_tiles_all = 0;
+  size_t n_tiles_all = 0;
+  n_partials = 0;
   for (int k = 0; k < kGroups; ++k) {
     g[k].partial_off = n_partials;
     g[k].tile_off = n_tiles_all;
     n_partials += (size_t)g[k].n_tiles * slots(g[k].ct, g[k].ct ? dual : 0);
     n_tiles_all += (size_t)g[k].n_tiles;
   }
-  rc = kdh_ensure_partials(ctx, std::max<size_t>(n_partials, 1));
-  if (rc) return rc;
-  rc = kdh_ensure_tile_pair(ctx, std::max<size_t>(n_tiles_all, 1));
-  if (rc) return rc;
-  std::vector<int> tp(std::max<size_t>(n_tiles_all, 1), 0);
+  tp.assign(std::max<size_t>(n_tiles_all, 1), 0);
   for (int i = 0; i < n_pairs; ++i) {
-    const PairState& ps = ctx->h_pairs[i];
+    const PairState& ps = hs[i];
     const Group& G = g[ps.ct + (i >= split ? 2 : 0)];
     for (int t = 0; t < ps.n_tiles; ++t) tp[G.tile_off + ps.tile_first + t] = i;
   }
+  return KDH_OK;
+}
+
+// batch_plan into the context's pinned states, then the tile list and the
+// states' input sections uploaded into the context's buffers.  Called with
+// ctx->mu held and the device set.
+static int batch_setup(kdh_ctx* ctx, int n_pairs, const kdh_pair* pairs, int mode, int stats_mode, Group* g,
+                       std::vector<int>& leader, int& max_iter, int dual = 0, const double* vf = nullptr,
+                       int halves = 1) {
+  int rc = kdh_ensure_pairs(ctx, (size_t)n_pairs);
+  if (rc) return rc;
+  std::vector<int> tp;
+  size_t n_partials = 0;
+  rc = batch_plan(ctx, n_pairs, pairs, mode, stats_mode, g, leader, max_iter, dual, vf, halves, ctx->h_pairs, tp,
+                  n_partials);
+  if (rc) return rc;
+  rc = kdh_ensure_partials(ctx, std::max<size_t>(n_partials, 1));
+  if (rc) return rc;
+  rc = kdh_ensure_tile_pair(ctx, tp.size());
+  if (rc) return rc;
   rc = kdh_upload_tile_pair(ctx, tp);
   if (rc) return rc;
   // Only the inputs and the evolving state go up (offsetof(stats_log) of each
@@ -157,6 +174,111 @@ static int batch_setup(kdh_ctx* ctx, int n_pairs, const kdh_pair* pairs, int mod
   // the device writes before anything reads them.
   KDH_HIP_TRY(ctx, hipMemcpy2DAsync(ctx->d_pairs, sizeof(PairState), ctx->h_pairs, sizeof(PairState),
                                     kStateBytes, (size_t)n_pairs, hipMemcpyHostToDevice, ctx->stream));
+  return KDH_OK;
+}
+
+// The device buffers of a batch: pair states, tile list, partial rows.
+struct BatchBufs {
+  PairState* pairs;
+  const int* tile_pair;
+  float* partials;
+};
+
+// Launches of the groups of half h (kinds 0 and 1) on stream st; phases as
+// kdh_launch (1 tile, 2 finish, 3 both).
+static int launch_half(kdh_ctx* ctx, const BatchBufs& B, const Group* g, int h, int phases, hipStream_t st) {
+  for (int k = 2 * h; k < 2 * h + 2; ++k) {
+    if (g[k].n_tiles == 0) continue;
+    int rc = kdh_launch(ctx, B.pairs, B.tile_pair + g[k].tile_off, g[k].n_tiles, g[k].ct, nullptr, nullptr,
+                        B.partials + g[k].partial_off, g[k].p_hi - g[k].p_lo, phases, 0, g[k].p_lo, st);
+    if (rc != KDH_OK) return rc;
+  }
+  return KDH_OK;
+}
+
+static int launch_phase(kdh_ctx* ctx, const BatchBufs& B, const Group* g, int phases) {
+  int rc = launch_half(ctx, B, g, 0, phases, ctx->stream);
+  return rc != KDH_OK ? rc : launch_half(ctx, B, g, 1, phases, ctx->stream);
+}
+
+// Profiling: plain launches on ctx->stream with events around each kernel
+// phase of every iteration (the tile phase repeated kProfileReps times).
+static int enqueue_profiled(kdh_ctx* ctx, const BatchBufs& B, const Group* g, int iters) {
+  KDH_HIP_TRY(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
+  for (int it = 0; it < iters; ++it) {
+    for (int r = 0; r < kdh_ctx::kProfileReps; ++r) {
+      int rc = launch_phase(ctx, B, g, 1);
+      if (rc != KDH_OK) return rc;
+    }
+    KDH_HIP_TRY(ctx, hipEventRecord(ctx->ev[2 * it + 1], ctx->stream));
+    int rc = launch_phase(ctx, B, g, 2);
+    if (rc != KDH_OK) return rc;
+    KDH_HIP_TRY(ctx, hipEventRecord(ctx->ev[2 * it + 2], ctx->stream));
+  }
+  return KDH_OK;
+}
+
+// Every iteration (tile + finish launch per non-empty group), enqueued on
+// ctx->stream while it is being captured.  The pair states live in device
+// memory, so the same launches replay each Gauss-Newton step, and pairs that
+// have finished skip their tiles.  With two streams each half of the pairs
+// is its own chain of launches; the second half starts after the first
+// half's first tile launch, so from then on one half's finish stage runs
+// beside the other half's tile launch.
+static int enqueue_iterations(kdh_ctx* ctx, const BatchBufs& B, const Group* g, int iters, bool two) {
+  if (!two) {
+    for (int it = 0; it < iters; ++it) {
+      int rc = launch_phase(ctx, B, g, 3);
+      if (rc != KDH_OK) return rc;
+    }
+    return KDH_OK;
+  }
+  KDH_HIP_TRY(ctx, hipEventRecord(ctx->ev_fork, ctx->stream));
+  KDH_HIP_TRY(ctx, hipStreamWaitEvent(ctx->stream2, ctx->ev_fork, 0));
+  int rc = launch_half(ctx, B, g, 0, 1, ctx->stream);
+  if (rc != KDH_OK) return rc;
+  KDH_HIP_TRY(ctx, hipEventRecord(ctx->ev_stag, ctx->stream));
+  rc = launch_half(ctx, B, g, 0, 2, ctx->stream);
+  if (rc != KDH_OK) return rc;
+  KDH_HIP_TRY(ctx, hipStreamWaitEvent(ctx->stream2, ctx->ev_stag, 0));
+  for (int it = 1; it < iters && rc == KDH_OK; ++it) rc = launch_half(ctx, B, g, 0, 3, ctx->stream);
+  for (int it = 0; it < iters && rc == KDH_OK; ++it) rc = launch_half(ctx, B, g, 1, 3, ctx->stream2);
+  if (rc != KDH_OK) return rc;
+  KDH_HIP_TRY(ctx, hipEventRecord(ctx->ev_join, ctx->stream2));
+  KDH_HIP_TRY(ctx, hipStreamWaitEvent(ctx->stream, ctx->ev_join, 0));
+  return KDH_OK;
+}
+
+// The second stream and its fork / stagger / join events, made on first use.
+static int ensure_stream2(kdh_ctx* ctx) {
+  if (ctx->stream2) return KDH_OK;
+  KDH_HIP_TRY(ctx, hipStreamCreateWithFlags(&ctx->stream2, hipStreamNonBlocking));
+  KDH_HIP_TRY(ctx, hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming));
+  KDH_HIP_TRY(ctx, hipEventCreateWithFlags(&ctx->ev_stag, hipEventDisableTiming));
+  KDH_HIP_TRY(ctx, hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming));
+  return KDH_OK;
+}
+
+// Ends the capture on ctx->stream begun by the caller and instantiates it;
+// on an earlier failure (rc) the partial graph is dropped.
+static int end_capture(kdh_ctx* ctx, int rc, hipGraphExec_t* exec) {
+  hipGraph_t graph = nullptr;
+  hipError_t ce = hipStreamEndCapture(ctx->stream, &graph);
+  if (rc != KDH_OK) {
+    if (graph) hipGraphDestroy(graph);
+    return rc;
+  }
+  if (ce != hipSuccess) return kdh_hip_fail(ctx, ce, "hipStreamEndCapture");
+  hipError_t ie = hipGraphInstantiate(exec, graph, nullptr, nullptr, 0);
+  hipGraphDestroy(graph);
+  if (ie != hipSuccess) return kdh_hip_fail(ctx, ie, "hipGraphInstantiate");
+  // the launch resources go to the device now, not at the first launch
+  ie = hipGraphUpload(*exec, ctx->stream);
+  if (ie != hipSuccess) {
+    hipGraphExecDestroy(*exec);
+    *exec = nullptr;
+    return kdh_hip_fail(ctx, ie, "hipGraphUpload");
+  }
   return KDH_OK;
 }
 
@@ -174,51 +296,18 @@ int kdh_batch_align(kdh_ctx* ctx, int n_pairs, const kdh_pair* pairs, kdh_pair_r
   int rc = batch_setup(ctx, n_pairs, pairs, 0, 0, g, leader, max_iter, 0, nullptr, halves);
   if (rc) return rc;
   const bool two = halves == 2 && (g[2].n_tiles + g[3].n_tiles) > 0 && (g[0].n_tiles + g[1].n_tiles) > 0;
-  if (two && !ctx->stream2) {
-    KDH_HIP_TRY(ctx, hipStreamCreateWithFlags(&ctx->stream2, hipStreamNonBlocking));
-    KDH_HIP_TRY(ctx, hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming));
-    KDH_HIP_TRY(ctx, hipEventCreateWithFlags(&ctx->ev_stag, hipEventDisableTiming));
-    KDH_HIP_TRY(ctx, hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming));
+  if (two) {
+    rc = ensure_stream2(ctx);
+    if (rc) return rc;
   }
-
   const int iters = std::min(max_iter, kMaxIter);
-  // the groups of half h (kinds 0 and 1) on stream s
-  auto launch_half = [&](int h, int phases, hipStream_t st) -> int {
-    for (int k = 2 * h; k < 2 * h + 2; ++k) {
-      if (g[k].n_tiles == 0) continue;
-      int lrc = kdh_launch(ctx, ctx->d_pairs, ctx->d_tile_pair + g[k].tile_off, g[k].n_tiles, g[k].ct, nullptr,
-                           nullptr, ctx->d_partials + g[k].partial_off, g[k].p_hi - g[k].p_lo, phases, 0, g[k].p_lo,
-                           st);
-      if (lrc != KDH_OK) return lrc;
-    }
-    return KDH_OK;
-  };
-  auto launch_phase = [&](int phases) -> int {
-    int lrc = launch_half(0, phases, ctx->stream);
-    return lrc != KDH_OK ? lrc : launch_half(1, phases, ctx->stream);
-  };
+  const BatchBufs B{ctx->d_pairs, ctx->d_tile_pair, ctx->d_partials};
   hipError_t le = hipSuccess;
   if (ctx->profile) {
-    // Profiling: plain launches with events around each kernel phase.
-    le = hipEventRecord(ctx->ev[0], ctx->stream);
-    for (int it = 0; it < iters && le == hipSuccess; ++it) {
-      int lrc = KDH_OK;
-      for (int r = 0; r < kdh_ctx::kProfileReps && lrc == KDH_OK; ++r) lrc = launch_phase(1);
-      if (lrc != KDH_OK) return lrc;
-      le = hipEventRecord(ctx->ev[2 * it + 1], ctx->stream);
-      if (le != hipSuccess) break;
-      lrc = launch_phase(2);
-      if (lrc != KDH_OK) return lrc;
-      le = hipEventRecord(ctx->ev[2 * it + 2], ctx->stream);
-    }
+    rc = enqueue_profiled(ctx, B, g, iters);
+    if (rc) return rc;
   } else {
-    // Capture every iteration (tile + finish launch per non-empty group) in
-    // one graph and launch it once: the pair states live in device memory,
-    // so the same launches replay each Gauss-Newton step, and pairs that have
-    // finished skip their tiles.  With two streams each half of the pairs is
-    // its own chain of launches; the second half starts after the first
-    // half's first tile launch, so from then on one half's finish stage runs
-    // beside the other half's tile launch.
+    // all iterations in one graph, launched once
     const long long key[16] = {n_pairs, g[0].n_tiles, g[1].n_tiles, iters, ctx->lds_nodes, ctx->grid_override,
                                ctx->xcd_remap, (long long)(intptr_t)ctx->d_pairs,
                                (long long)(intptr_t)ctx->d_partials, (long long)(intptr_t)ctx->d_tile_pair, 2,
@@ -226,34 +315,9 @@ int kdh_batch_align(kdh_ctx* ctx, int n_pairs, const kdh_pair* pairs, kdh_pair_r
                                (long long)(intptr_t)ctx->stream2};
     hipGraphExec_t exec = kdh_graph_lookup(ctx->graph_batch, key);
     if (!exec) {
-      hipGraph_t graph = nullptr;
       KDH_HIP_TRY(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-      int lrc = KDH_OK;
-      hipError_t ee = hipSuccess;
-      if (!two) {
-        for (int it = 0; it < iters && lrc == KDH_OK; ++it) lrc = launch_phase(3);
-      } else {
-        ee = hipEventRecord(ctx->ev_fork, ctx->stream);
-        if (ee == hipSuccess) ee = hipStreamWaitEvent(ctx->stream2, ctx->ev_fork, 0);
-        if (ee == hipSuccess) lrc = launch_half(0, 1, ctx->stream);
-        if (ee == hipSuccess && lrc == KDH_OK) ee = hipEventRecord(ctx->ev_stag, ctx->stream);
-        if (ee == hipSuccess && lrc == KDH_OK) lrc = launch_half(0, 2, ctx->stream);
-        if (ee == hipSuccess) ee = hipStreamWaitEvent(ctx->stream2, ctx->ev_stag, 0);
-        for (int it = 1; it < iters && lrc == KDH_OK && ee == hipSuccess; ++it) lrc = launch_half(0, 3, ctx->stream);
-        for (int it = 0; it < iters && lrc == KDH_OK && ee == hipSuccess; ++it)
-          lrc = launch_half(1, 3, ctx->stream2);
-        if (ee == hipSuccess) ee = hipEventRecord(ctx->ev_join, ctx->stream2);
-        if (ee == hipSuccess) ee = hipStreamWaitEvent(ctx->stream, ctx->ev_join, 0);
-      }
-      hipError_t ce = hipStreamEndCapture(ctx->stream, &graph);
-      if (lrc != KDH_OK || ee != hipSuccess) {
-        if (graph) hipGraphDestroy(graph);
-        return lrc != KDH_OK ? lrc : kdh_hip_fail(ctx, ee, "two-stream capture");
-      }
-      if (ce != hipSuccess) return kdh_hip_fail(ctx, ce, "hipStreamEndCapture");
-      hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-      hipGraphDestroy(graph);
-      if (ie != hipSuccess) return kdh_hip_fail(ctx, ie, "hipGraphInstantiate");
+      rc = end_capture(ctx, enqueue_iterations(ctx, B, g, iters, two), &exec);
+      if (rc) return rc;
       kdh_graph_store(ctx->graph_batch, key, exec);
     }
     le = hipEventRecord(ctx->ev[0], ctx->stream);
@@ -419,5 +483,318 @@ int kdh_batch_build_dual(kdh_ctx* ctx, int n_factors, const kdh_dual_pair* facto
     memcpy(o.H_dual_cross_vel_AB, ps.Hd + 78, sizeof(o.H_dual_cross_vel_AB));
     memcpy(&o.stats, &ps.stats_log[0], sizeof(kdh_stats));
   }
+  return KDH_OK;
+}
+
+// ---- prepared batches (kdh_batch_prepare / kdh_batch_run) ------------------
+//
+// The batch of kdh_batch_align validated, planned and uploaded once, with one
+// graph that holds the whole run: the state initialisation from the caller's
+// initial poses (read by the device from pinned host memory), every
+// iteration, and the results written back into pinned host memory.  A run is
+// then one graph launch and one synchronisation.
+
+namespace {
+
+// Per pair: its forest leader, its state at the start of a run (iter,
+// status: a forest without moving leaves starts finished, as batch_plan sets
+// it), and the first float of its group's partial rows.
+struct BatchMeta {
+  int leader;
+  int init_iter;
+  int init_status;
+  int max_iterations;  // the leader's params.max_iterations (result code)
+  long long partial_base;
+};
+
+// What a run returns per pair, written by the device into pinned memory.
+struct BatchOut {
+  double X[12];
+  double v[6];
+  DevStats last;
+  DevStats entry;
+  int iter;
+  int status;
+  int pad[2];
+};
+static_assert(sizeof(BatchOut) % 16 == 0, "BatchOut rows stay 16-byte aligned");
+constexpr int kBatchIn = 18;  // doubles per pair of the run inputs: X0 (3x4), v0
+
+}  // namespace
+
+struct kdh_batch {
+  kdh_ctx* ctx = nullptr;
+  int n = 0;
+  int iters = 0;
+  Group g[kGroups];
+  bool two = false;
+  PairState* d_pairs = nullptr;
+  float* d_partials = nullptr;
+  int* d_tile_pair = nullptr;
+  BatchMeta* d_meta = nullptr;
+  std::vector<BatchMeta> meta;
+  double* h_in = nullptr;   // pinned, mapped: kBatchIn doubles per pair
+  double* h_in_dev = nullptr;
+  BatchOut* h_out = nullptr;  // pinned, mapped
+  BatchOut* h_out_dev = nullptr;
+  hipGraphExec_t exec = nullptr;
+  long long key[8] = {};  // the launch configuration the graph was built with
+};
+
+// Workgroup of 64 pairs: their leaders' inputs staged through LDS (16-byte
+// loads from pinned host memory, consecutive lanes on consecutive bytes),
+// then lane l sets pair 64 b + l: its pose and velocities from its leader's
+// inputs, its counters reset, the FP32 cache of that state
+// (cache_from_state, the expressions of the host set-up).  Everything else in
+// the state is input that a run does not change.  Pairs and their leaders
+// may sit in different workgroups, so each lane reads its leader's row
+// directly when it lies outside the workgroup's range.
+constexpr int kInitPairs = 64;
+__global__ __launch_bounds__(kInitPairs) void k_batch_init(PairState* __restrict__ d,
+                                                           const BatchMeta* __restrict__ meta,
+                                                           const double* __restrict__ in, int n) {
+  __shared__ __align__(16) double2 s_in[kInitPairs * kBatchIn / 2];
+  const int base = blockIdx.x * kInitPairs;
+  const int rows = min(kInitPairs, n - base);
+  const double2* src = reinterpret_cast<const double2*>(in + (size_t)kBatchIn * base);
+  // every load in flight before the first LDS store: one PCIe round trip, not nine
+  constexpr int kPer = kBatchIn / 2;  // 16-byte words per lane (kInitPairs rows of kBatchIn doubles)
+  double2 w[kPer];
+#pragma unroll
+  for (int j = 0; j < kPer; ++j) {
+    const int k = threadIdx.x + j * kInitPairs;
+    if (k < rows * kPer) w[j] = src[k];
+  }
+#pragma unroll
+  for (int j = 0; j < kPer; ++j) {
+    const int k = threadIdx.x + j * kInitPairs;
+    if (k < rows * kPer) s_in[k] = w[j];
+  }
+  __syncthreads();
+  const int i = base + threadIdx.x;
+  if (i >= n) return;
+  const BatchMeta m = meta[i];
+  const int r = m.leader - base;
+  const double* x = (r >= 0 && r < rows) ? reinterpret_cast<const double*>(s_in) + kBatchIn * r
+                                         : in + (size_t)kBatchIn * m.leader;
+  PairState& ps = d[i];
+  for (int k = 0; k < 12; ++k) ps.X[k] = x[k];
+  const int ct = ps.ct;
+  for (int k = 0; k < 6; ++k) ps.v[k] = ct ? x[12 + k] : 0.0;
+  ps.iter = m.init_iter;
+  ps.status = m.init_status;
+  ps.prev_score = 0.0f;
+  cache_from_state(ps);
+}
+
+// Workgroup i (64 lanes): pair i's results -- its leader's pose, velocities,
+// counters and last statistics, and its own share of the last round when it
+// is an entry of a forest (the FixedEntry::stats of icp_impl_.h:53-71: its
+// tiles' statistics slots summed in FP64 in tile order, as kdh_batch_align
+// sums them on the host) -- assembled in LDS and written to pinned host
+// memory as one contiguous row of 16-byte stores.
+__global__ __launch_bounds__(64) void k_batch_collect(const PairState* __restrict__ d,
+                                                      const BatchMeta* __restrict__ meta,
+                                                      const float* __restrict__ partials, BatchOut* out, int n) {
+  __shared__ __align__(16) BatchOut s;
+  const int i = blockIdx.x, t = threadIdx.x;
+  if (i >= n) return;
+  const BatchMeta m = meta[i];
+  const PairState& L = d[m.leader];
+  if (t < 12) s.X[t] = L.X[t];
+  if (t < 6) s.v[t] = L.v[t];
+  if (t == 0) {
+    const int it = L.iter;
+    s.iter = it;
+    s.status = L.status;
+    s.pad[0] = s.pad[1] = 0;
+    DevStats last{};
+    if (it > 0) last = L.stats_log[it - 1];
+    s.last = last;
+    DevStats e = last;
+    if (L.group_n >= 2) {
+      e.pert_pose_norm = e.pert_vel_norm = 0.0;
+      if (it > 0 && L.group_tiles > 0) {
+        const PairState& ps = d[i];
+        const int N = L.ct ? kCtSlots : kIcpSlots, CS = L.ct ? 90 : 27;
+        double acc[7] = {0, 0, 0, 0, 0, 0, 0};
+        for (int tt = 0; tt < ps.n_tiles; ++tt) {
+          const float* row = partials + m.partial_base + (size_t)(ps.tile_first + tt) * N;
+          for (int k = 0; k < 7; ++k) acc[k] += (double)row[CS + k];
+        }
+        e.chi2 = acc[0];
+        e.chi2_kernelized = acc[1];
+        e.num_inliers = (int)acc[2 + kInlier];
+        e.num_outliers = (int)acc[2 + kOutlier];
+        e.num_bad = (int)acc[2 + kBad];
+        e.num_err_normal = (int)acc[2 + kErrNormal];
+        e.num_err_fn = (int)acc[2 + kErrFn];
+      }
+    }
+    s.entry = e;
+  }
+  __syncthreads();
+  constexpr int kWords = (int)(sizeof(BatchOut) / 16);
+  static_assert(kWords <= 64, "one 16-byte store per lane");
+  if (t < kWords) reinterpret_cast<float4*>(out + i)[t] = reinterpret_cast<const float4*>(&s)[t];
+  __threadfence_system();
+}
+
+static void batch_free(kdh_batch* b) {
+  if (b->exec) hipGraphExecDestroy(b->exec);
+  if (b->d_pairs) hipFree(b->d_pairs);
+  if (b->d_partials) hipFree(b->d_partials);
+  if (b->d_tile_pair) hipFree(b->d_tile_pair);
+  if (b->d_meta) hipFree(b->d_meta);
+  if (b->h_in) hipHostFree(b->h_in);
+  if (b->h_out) hipHostFree(b->h_out);
+  delete b;
+}
+
+int kdh_batch_prepare(kdh_ctx* ctx, int n_pairs, const kdh_pair* pairs, kdh_batch** out) {
+  if (!ctx) return KDH_ERR_INVALID;
+  if (!out || n_pairs <= 0 || !pairs) return kdh_fail(ctx, KDH_ERR_INVALID, "bad arguments");
+  *out = nullptr;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  KDH_HIP_TRY(ctx, hipSetDevice(ctx->device));
+  kdh_batch* b = new kdh_batch();
+  b->ctx = ctx;
+  b->n = n_pairs;
+  std::vector<PairState> hs((size_t)n_pairs);
+  std::vector<int> leader, tp;
+  size_t n_partials = 0;
+  int max_iter = 0;
+  int rc = batch_plan(ctx, n_pairs, pairs, 0, 0, b->g, leader, max_iter, 0, nullptr, ctx->batch_streams, hs.data(),
+                      tp, n_partials);
+  if (rc) {
+    delete b;
+    return rc;
+  }
+  b->iters = std::min(max_iter, kMaxIter);
+  b->two = ctx->batch_streams == 2 && (b->g[2].n_tiles + b->g[3].n_tiles) > 0 &&
+           (b->g[0].n_tiles + b->g[1].n_tiles) > 0;
+  b->meta.resize(n_pairs);
+  for (int i = 0; i < n_pairs; ++i) {
+    BatchMeta& m = b->meta[i];
+    const PairState& ps = hs[i];
+    m.leader = leader[i];
+    m.init_iter = ps.iter;
+    m.init_status = ps.status;
+    m.max_iterations = pairs[leader[i]].params.max_iterations;
+    m.partial_base = (long long)b->g[ps.ct + (i >= b->g[2].p_lo ? 2 : 0)].partial_off;  // p_lo of half 1: the split
+  }
+  hipError_t e = hipMalloc((void**)&b->d_pairs, sizeof(PairState) * (size_t)n_pairs);
+  if (e == hipSuccess) e = hipMalloc((void**)&b->d_partials, sizeof(float) * std::max<size_t>(n_partials, 1));
+  if (e == hipSuccess) e = hipMalloc((void**)&b->d_tile_pair, sizeof(int) * tp.size());
+  if (e == hipSuccess) e = hipMalloc((void**)&b->d_meta, sizeof(BatchMeta) * (size_t)n_pairs);
+  // coherent: the device reads the inputs and writes the results in place
+  if (e == hipSuccess)
+    e = hipHostMalloc((void**)&b->h_in, sizeof(double) * kBatchIn * (size_t)n_pairs, hipHostMallocCoherent);
+  if (e == hipSuccess) e = hipHostGetDevicePointer((void**)&b->h_in_dev, b->h_in, 0);
+  if (e == hipSuccess) e = hipHostMalloc((void**)&b->h_out, sizeof(BatchOut) * (size_t)n_pairs, hipHostMallocCoherent);
+  if (e == hipSuccess) e = hipHostGetDevicePointer((void**)&b->h_out_dev, b->h_out, 0);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(b->d_pairs, hs.data(), sizeof(PairState) * (size_t)n_pairs, hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(b->d_tile_pair, tp.data(), sizeof(int) * tp.size(), hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(b->d_meta, b->meta.data(), sizeof(BatchMeta) * (size_t)n_pairs, hipMemcpyHostToDevice,
+                       ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  if (e != hipSuccess) {
+    batch_free(b);
+    return kdh_hip_fail(ctx, e, "kdh_batch_prepare");
+  }
+  for (int i = 0; i < n_pairs; ++i) {
+    memcpy(b->h_in + (size_t)kBatchIn * i, pairs[i].X0, sizeof(double) * 12);
+    memcpy(b->h_in + (size_t)kBatchIn * i + 12, pairs[i].v0, sizeof(double) * 6);
+  }
+  *out = b;
+  return KDH_OK;
+}
+
+int kdh_batch_run(kdh_batch* b, const double* X0, const double* v0, kdh_pair_result* results) {
+  if (!b) return KDH_ERR_INVALID;
+  kdh_ctx* ctx = b->ctx;
+  if (!results) return kdh_fail(ctx, KDH_ERR_INVALID, "bad arguments");
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  KDH_HIP_TRY(ctx, hipSetDevice(ctx->device));
+  const int n = b->n;
+  if (X0)
+    for (int i = 0; i < n; ++i) memcpy(b->h_in + (size_t)kBatchIn * i, X0 + (size_t)12 * i, sizeof(double) * 12);
+  if (v0)
+    for (int i = 0; i < n; ++i) memcpy(b->h_in + (size_t)kBatchIn * i + 12, v0 + (size_t)6 * i, sizeof(double) * 6);
+  const BatchBufs B{b->d_pairs, b->d_tile_pair, b->d_partials};
+  const dim3 init_grid((n + kInitPairs - 1) / kInitPairs);
+  int rc = KDH_OK;
+  if (ctx->profile) {
+    // plain launches with events around each kernel phase, on one stream
+    hipLaunchKernelGGL(k_batch_init, init_grid, dim3(kInitPairs), 0, ctx->stream, b->d_pairs, b->d_meta, b->h_in_dev, n);
+    KDH_HIP_TRY(ctx, hipGetLastError());
+    rc = enqueue_profiled(ctx, B, b->g, b->iters);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_batch_collect, dim3(n), dim3(64), 0, ctx->stream, b->d_pairs, b->d_meta, b->d_partials,
+                       b->h_out_dev, n);
+    KDH_HIP_TRY(ctx, hipGetLastError());
+    KDH_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    kdh_collect_times(ctx, b->iters, true);
+  } else {
+    const bool two = b->two && ctx->batch_streams == 2;
+    const long long key[8] = {ctx->lds_nodes, ctx->grid_override, ctx->xcd_remap, ctx->leaves_per_lane,
+                              two ? 2 : 1, (long long)(intptr_t)ctx->stream, 0, 0};
+    if (b->exec && memcmp(key, b->key, sizeof(key)) != 0) {
+      hipGraphExecDestroy(b->exec);
+      b->exec = nullptr;
+    }
+    if (!b->exec) {
+      if (two) {
+        rc = ensure_stream2(ctx);
+        if (rc) return rc;
+      }
+      KDH_HIP_TRY(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+      hipLaunchKernelGGL(k_batch_init, init_grid, dim3(kInitPairs), 0, ctx->stream, b->d_pairs, b->d_meta, b->h_in_dev, n);
+      rc = hipGetLastError() == hipSuccess ? KDH_OK : kdh_fail(ctx, KDH_ERR_HIP, "k_batch_init launch");
+      if (rc == KDH_OK) rc = enqueue_iterations(ctx, B, b->g, b->iters, two);
+      if (rc == KDH_OK) {
+        hipLaunchKernelGGL(k_batch_collect, dim3(n), dim3(64), 0, ctx->stream, b->d_pairs, b->d_meta, b->d_partials,
+                           b->h_out_dev, n);
+        if (hipGetLastError() != hipSuccess) rc = kdh_fail(ctx, KDH_ERR_HIP, "k_batch_collect launch");
+      }
+      rc = end_capture(ctx, rc, &b->exec);
+      if (rc) return rc;
+      memcpy(b->key, key, sizeof(key));
+    }
+    hipError_t le = hipEventRecord(ctx->ev[0], ctx->stream);
+    if (le == hipSuccess) le = hipGraphLaunch(b->exec, ctx->stream);
+    if (le == hipSuccess) le = hipEventRecord(ctx->ev[1], ctx->stream);
+    if (le == hipSuccess) le = hipStreamSynchronize(ctx->stream);
+    if (le != hipSuccess) return kdh_hip_fail(ctx, le, "kdh_batch_run");
+    float ms = 0.0f;
+    hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]);
+    ctx->last_kernel_us = b->iters ? 1000.0 * ms / b->iters : 0.0;
+    ctx->last_tile_us = ctx->last_finish_us = 0.0;
+    ctx->last_launches = b->iters;
+  }
+  static_assert(sizeof(kdh_stats) == sizeof(DevStats), "kdh_stats mirrors DevStats");
+  for (int i = 0; i < n; ++i) {
+    const BatchOut& o = b->h_out[i];
+    kdh_pair_result& r = results[i];
+    memcpy(r.X, o.X, sizeof(r.X));
+    memcpy(r.v, o.v, sizeof(r.v));
+    memcpy(&r.last, &o.last, sizeof(r.last));
+    memcpy(&r.entry, &o.entry, sizeof(r.entry));
+    r.iterations = o.iter;
+    r.result = kdh_result_code_of(o.status, o.iter, b->meta[i].max_iterations);
+  }
+  return KDH_OK;
+}
+
+int kdh_batch_release(kdh_batch* b) {
+  if (!b) return KDH_ERR_INVALID;
+  kdh_ctx* ctx = b->ctx;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  hipSetDevice(ctx->device);
+  hipStreamSynchronize(ctx->stream);
+  batch_free(b);
   return KDH_OK;
 }

@@ -403,15 +403,7 @@ static int n_tiles_of(int n_leaves) { return (n_leaves + kTile - 1) / kTile; }
 
 // Initial FP32 cache from FP64 state: the same formula the device applies
 // after every update (kdh_kernels.hip, finish_pair).
-void kdh_cache_from_state(PairState& ps) {
-  IsoD Xc = iso_from_34<double>(ps.X);
-  IsoD Xf = iso_from_34<double>(ps.Xf);
-  ps.cache.Xm = iso_cast<float>(Xc);
-  ps.cache.Xf = iso_cast<float>(Xf);
-  ps.cache.Xmf = iso_cast<float>(compose(inverse(Xf), Xc));
-  for (int i = 0; i < 6; ++i) ps.cache.v[i] = (float)ps.v[i];
-  set_anchor(ps.cache, ps.c64, make_anchor(Xc, Xf, ps.moving_root_mean, ps.sp.recenter));
-}
+void kdh_cache_from_state(PairState& ps) { cache_from_state(ps); }
 
 int kdh_fill_pair(kdh_ctx* ctx, PairState& ps, const kdh_tree* fixed, const kdh_tree* moving, int ct,
                   const kdh_params& p, int stats_mode, int mode) {
@@ -594,10 +586,14 @@ int kdh_correspondences(kdh_ctx* ctx, const kdh_tree* fixed, const kdh_tree* mov
 // compute()'s return value (tree_aligner_base_.h:100-115): -i when round i
 // failed the min_good gate, i when the criterion stopped after round i,
 // otherwise the configured max_iterations (also in stats mode).
-int kdh_result_code(const PairState& ps, int max_iterations_param) {
-  if (ps.status == 2) return -(ps.iter - 1);
-  if (ps.status == 3) return ps.iter - 1;
+int kdh_result_code_of(int status, int iter, int max_iterations_param) {
+  if (status == 2) return -(iter - 1);
+  if (status == 3) return iter - 1;
   return max_iterations_param;
+}
+
+int kdh_result_code(const PairState& ps, int max_iterations_param) {
+  return kdh_result_code_of(ps.status, ps.iter, max_iterations_param);
 }
 
 int kdh_setup_prior(PairState& ps, const kdh_prior* prior) {
@@ -668,6 +664,11 @@ int kdh_align(kdh_ctx* ctx, const kdh_tree* fixed, const kdh_tree* moving, int c
         hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
         hipGraphDestroy(graph);
         if (ie != hipSuccess) return kdh_hip_fail(ctx, ie, "hipGraphInstantiate");
+        ie = hipGraphUpload(exec, ctx->stream);
+        if (ie != hipSuccess) {
+          hipGraphExecDestroy(exec);
+          return kdh_hip_fail(ctx, ie, "hipGraphUpload");
+        }
         kdh_graph_store(ctx->graph_align, key, exec);
       }
       hipError_t le = hipEventRecord(ctx->ev[0], ctx->stream);

@@ -102,7 +102,11 @@ int kdh_setup_prior(kdh::PairState& ps, const kdh_prior* prior);
 int kdh_launch(kdh_ctx* ctx, kdh::PairState* d_pairs, const int* d_tile_pair, int n_tiles, int ct, int* corr,
                int* depth, float* partials, int n_pairs, int phases = 3, int dual = 0, int pair_first = 0,
                hipStream_t stream = nullptr);
+// compute()'s return value from a finished state: -i after a failed
+// min_good gate in round i, i after a stop by the criterion, else
+// max_iterations.
 int kdh_result_code(const kdh::PairState& ps, int max_iterations_param);
+int kdh_result_code_of(int status, int iter, int max_iterations_param);
 // Workgroups of a tile launch over n_tiles tiles: the context's persistent
 // grid, or one per tile; with the XCD remap on, rounded up to a multiple of 8
 // so that every XCD takes a contiguous range of tiles (the extra workgroups of

@@ -128,4 +128,17 @@ KDH_HD void set_anchor(TermCache& C, double* c64, const Anchor& a) {
   c64[2] = a.c.z;
 }
 
+// The FP32 cache of a state's FP64 pose, fixed pose and velocities, with its
+// re-centring anchor (the host set-up and the device's batch initialisation
+// evaluate the same expressions, so the cache bits agree).
+KDH_HD void cache_from_state(PairState& ps) {
+  const IsoD Xc = iso_from_34<double>(ps.X);
+  const IsoD Xf = iso_from_34<double>(ps.Xf);
+  ps.cache.Xm = iso_cast<float>(Xc);
+  ps.cache.Xf = iso_cast<float>(Xf);
+  ps.cache.Xmf = iso_cast<float>(compose(inverse(Xf), Xc));
+  for (int i = 0; i < 6; ++i) ps.cache.v[i] = (float)ps.v[i];
+  set_anchor(ps.cache, ps.c64, make_anchor(Xc, Xf, ps.moving_root_mean, ps.sp.recenter));
+}
+
 }  // namespace kdh

@@ -210,3 +210,76 @@ def test_two_stream_batch_is_bit_identical(ctx, small_pairs):
     for F, M in trees:
         F.close()
         M.close()
+
+
+def _same(a, b):
+    return (np.array_equal(a["X"], b["X"]) and np.array_equal(a["v"], b["v"]) and a["last"] == b["last"]
+            and a["entry"] == b["entry"] and a["result"] == b["result"] and a["iterations"] == b["iterations"])
+
+
+def test_prepared_batch_equals_batch_align(ctx, small_pairs):
+    """kdh_batch_prepare / kdh_batch_run (PairBatch): the same bits as
+    kdh_batch_align on a mix of rigid pairs, CT pairs, a fixed forest of
+    three entries and a pair whose scan has no leaves, on the first run and
+    on runs from new initial states (set_initial), on two streams and in
+    profiling mode."""
+    from kd_slam2_amd import HostTree, PairBatch
+
+    from .test_tree_and_descent import _node
+    from .workloads import forest_cfg1
+
+    trees = [(DeviceTree(ctx, w["fixed"]), DeviceTree(ctx, w["moving"])) for w in small_pairs]
+    pairs = _pairs(ctx, small_pairs, trees)
+    f = forest_cfg1()
+    fw = f["w"]
+    fdev = [DeviceTree(ctx, t) for t in f["trees"]]
+    FM = DeviceTree(ctx, fw["moving"])
+    p = icp_params()
+    group = [dict(fixed=T, moving=FM, ct=0, X0=fw["X0"], Xf=Xf, params=p, forest=len(fdev) - 1 if i == 0 else 0)
+             for i, (T, Xf) in enumerate(zip(fdev, f["poses"]))]
+    empty = DeviceTree(ctx, HostTree(np.concatenate([_node([0, 0, 1], [0, 0, -1])]), np.zeros(0, dtype=np.uint64)))
+    w0 = small_pairs[0]
+    lone = dict(fixed=trees[0][0], moving=empty, ct=0, X0=w0["X0"], Xf=w0["Xf"], params=icp_params())
+    allp = pairs[:3] + group + [lone] + pairs[3:]
+    B = PairBatch(allp)
+    try:
+        ref = batch_align(ctx, allp)
+        got = list(B.run(ctx))
+        assert all(_same(a, b) for a, b in zip(ref, got))
+        assert got[3]["entry"] != got[4]["entry"]  # forest entries carry their own share
+        # new initial states: every pair perturbed again (forest entries follow their leader)
+        rng = np.random.default_rng(5)
+        X0s = np.stack([synth.perturb(q["X0"], rng, 1.0, 0.2) for q in allp])
+        v0s = np.stack([np.asarray(q.get("v0", np.zeros(6))) + rng.normal(0, 0.05, 6) for q in allp])
+        B.set_initial(X0s, v0s)
+        again = list(B.run(ctx))
+        moved = [dict(q, X0=X0s[i], v0=v0s[i]) for i, q in enumerate(allp)]
+        for k in range(1, len(group)):  # entries take the leader's state
+            moved[3 + k]["X0"], moved[3 + k]["v0"] = X0s[3], v0s[3]
+        ref2 = batch_align(ctx, moved)
+        assert all(_same(a, b) for a, b in zip(ref2, again))
+        assert not np.array_equal(again[0]["X"], got[0]["X"])
+        # two streams, then profiling mode: same bits
+        ctx.set_launch_config(batch_streams=2)
+        B2 = PairBatch(allp)
+        two = list(B2.run(ctx))
+        two_again = list(B2.run(ctx))
+        ctx.set_launch_config(batch_streams=1)
+        assert all(_same(a, b) for a, b in zip(ref, two)) and all(_same(a, b) for a, b in zip(ref, two_again))
+        ctx.set_profiling(True)
+        prof = list(PairBatch(allp).run(ctx))
+        ctx.set_profiling(False)
+        assert all(_same(a, b) for a, b in zip(ref, prof))
+        tile_us, finish_us = ctx.kernel_breakdown()
+        assert tile_us > 0 and finish_us > 0
+    finally:
+        ctx.set_launch_config(batch_streams=1)
+        ctx.set_profiling(False)
+        B.close()
+        for F, M in trees:
+            F.close()
+            M.close()
+        for T in fdev:
+            T.close()
+        FM.close()
+        empty.close()
