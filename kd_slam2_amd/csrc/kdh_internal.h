@@ -131,14 +131,18 @@ KDH_HD void set_anchor(TermCache& C, double* c64, const Anchor& a) {
 // The FP32 cache of a state's FP64 pose, fixed pose and velocities, with its
 // re-centring anchor (the host set-up and the device's batch initialisation
 // evaluate the same expressions, so the cache bits agree).
+KDH_HD void cache_from_pose(TermCache& C, double c64[3], const double* X, const double* Xf34, const double* v,
+                            V3f mroot, int recenter) {
+  const IsoD Xc = iso_from_34<double>(X);
+  const IsoD Xf = iso_from_34<double>(Xf34);
+  C.Xm = iso_cast<float>(Xc);
+  C.Xf = iso_cast<float>(Xf);
+  C.Xmf = iso_cast<float>(compose(inverse(Xf), Xc));
+  for (int i = 0; i < 6; ++i) C.v[i] = (float)v[i];
+  set_anchor(C, c64, make_anchor(Xc, Xf, mroot, recenter));
+}
 KDH_HD void cache_from_state(PairState& ps) {
-  const IsoD Xc = iso_from_34<double>(ps.X);
-  const IsoD Xf = iso_from_34<double>(ps.Xf);
-  ps.cache.Xm = iso_cast<float>(Xc);
-  ps.cache.Xf = iso_cast<float>(Xf);
-  ps.cache.Xmf = iso_cast<float>(compose(inverse(Xf), Xc));
-  for (int i = 0; i < 6; ++i) ps.cache.v[i] = (float)ps.v[i];
-  set_anchor(ps.cache, ps.c64, make_anchor(Xc, Xf, ps.moving_root_mean, ps.sp.recenter));
+  cache_from_pose(ps.cache, ps.c64, ps.X, ps.Xf, ps.v, ps.moving_root_mean, ps.sp.recenter);
 }
 
 }  // namespace kdh
