@@ -31,6 +31,12 @@ typedef struct kdh_tree_gen_params {
   float leaf_eigenvalue_threshold;   /* 1e-3 (unused by the reference policy too) */
   float leaf_normal_eigenratio;      /* 0.1 */
   int32_t parallel_level;            /* 0: single thread; k: split the top k levels over threads */
+  int32_t label_mode;                /* 0: the canonical build, FP64 moments in a fixed order, Jacobi
+                                        eigen-solver, order-keeping split (host = device, bit for bit);
+                                        1: the reference's own arithmetic (label_policy_base_impl_.h:41-95,
+                                        node_impl_.h:12-26): FP32 StableAdder_ moments, E[xx^T] - mu mu^T,
+                                        SelfAdjointEigenSolver::computeDirect, the two-ended splitRange.
+                                        Host builder only. */
 } kdh_tree_gen_params;
 
 typedef struct kdh_host_tree kdh_host_tree;

@@ -217,6 +217,7 @@ class TreeGenParams(C.Structure):
         ("leaf_eigenvalue_threshold", C.c_float),
         ("leaf_normal_eigenratio", C.c_float),
         ("parallel_level", C.c_int32),
+        ("label_mode", C.c_int32),
     ]
 
 

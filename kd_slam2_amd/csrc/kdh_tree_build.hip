@@ -1388,6 +1388,10 @@ int kdh_tree_build(kdh_ctx* ctx, const float* xyz, const double* stamps, int64_t
   if (n_points > (int64_t)(1 << 28))
     return kdh_fail(ctx, KDH_ERR_INVALID, "kdh_tree_build: more than 2^28 points");
   const kdh_tree_gen_params P = *params;
+  if (P.label_mode != 0)
+    return kdh_fail(ctx, KDH_ERR_INVALID,
+                    "kdh_tree_build: label_mode 1 (the reference's own arithmetic) is built on the host "
+                    "(kdh_host_tree_build)");
   if (n_points < (int64_t)P.leaf_min_points || n_points == 0) return KDH_OK;  // makeTree returns nullptr
   std::lock_guard<std::mutex> lk(ctx->mu);
   KDH_HIP_TRY(ctx, hipSetDevice(ctx->device));

@@ -11,6 +11,9 @@
 // The moments, the eigen-solver, the labelling and the split rule are shared
 // with the device builder (kdh_treegen.h, which lists where this build fixes
 // what the reference leaves open); both give the same tree bit for bit.
+// label_mode 1 instead evaluates them with the reference's own arithmetic
+// (FP32 StableAdder_ moments, computeDirect, the two-ended splitRange); the
+// device builder has no such mode.
 #include <math.h>
 #include <string.h>
 
@@ -77,6 +80,23 @@ struct Builder {
     for (int k = 0; k < 6; ++k) c2[k] = v6[0][k];
   }
 
+  // splitRange as the reference writes it (tree/node_impl_.h:12-26): a point
+  // on the right side is swapped with the last unexamined one.  Signed
+  // indices: an all-right range starting at point 0 ends the scan at s = 0
+  // (the reference's unsigned end index would wrap there).
+  size_t split_ref(const kdh_node3f& node) {
+    long s = (long)node.idx_start, e = (long)node.idx_end - 1;
+    while (s <= e) {
+      if (kdh::tg::goes_left(node.direction, node.mean, pts[s].x, pts[s].y, pts[s].z)) {
+        ++s;
+      } else {
+        std::swap(pts[s], pts[e]);
+        --e;
+      }
+    }
+    return (size_t)s;
+  }
+
   // splitRange with the order of each side kept
   size_t split(const kdh_node3f& node) {
     const size_t s = node.idx_start, e = node.idx_end;
@@ -101,12 +121,28 @@ struct Builder {
     node.idx_end = end;
     node.idx_left = -1;
     node.idx_right = -1;
-    double s1[5], c2[6];
-    moments(start, end, s1, c2);
     static const float zero3[3] = {0.0f, 0.0f, 0.0f};
     Label L;
-    kdh::tg::label_node(P, (long)(end - start), s1, c2, parent == nullptr, parent ? parent->prop : zero3,
-                        parent ? parent->has_prop : 0, L);
+    if (P.label_mode == 1) {
+      kdh::tg::RefMoments M;
+      const Pt* base = pts + start;
+      kdh::tg::ref_moments(
+          (long)(end - start),
+          [base](long i, float x[3], double& st) {
+            x[0] = base[i].x;
+            x[1] = base[i].y;
+            x[2] = base[i].z;
+            st = base[i].stamp;
+          },
+          ts_start, M);
+      kdh::tg::label_node_ref(P, (long)(end - start), M, parent == nullptr, parent ? parent->prop : zero3,
+                              parent ? parent->has_prop : 0, L);
+    } else {
+      double s1[5], c2[6];
+      moments(start, end, s1, c2);
+      kdh::tg::label_node(P, (long)(end - start), s1, c2, parent == nullptr, parent ? parent->prop : zero3,
+                          parent ? parent->has_prop : 0, L);
+    }
     node.dts_mean = L.dts_mean;
     node.dts_covariance = L.dts_cov;
     memcpy(node.mean, L.mean, sizeof(node.mean));
@@ -115,7 +151,7 @@ struct Builder {
       nodes[me] = node;
       return me;
     }
-    const size_t mid = split(node);
+    const size_t mid = P.label_mode == 1 ? split_ref(node) : split(node);
     if (mid == start || mid == end) {
       // one-sided split: keep the node as a leaf with its normal
       memcpy(node.direction, L.evec0, sizeof(node.direction));
@@ -194,11 +230,13 @@ void kdh_tree_gen_params_default(kdh_tree_gen_params* p) {
   p->leaf_eigenvalue_threshold = 1e-3f;
   p->leaf_normal_eigenratio = 0.1f;
   p->parallel_level = 0;
+  p->label_mode = 0;
 }
 
 int kdh_host_tree_build(const float* xyz, const double* stamps, int64_t n_points, const kdh_tree_gen_params* params,
                         kdh_host_tree** out) {
   if (!out || !params || n_points < 0 || (n_points > 0 && !xyz)) return KDH_ERR_INVALID;
+  if (params->label_mode != 0 && params->label_mode != 1) return KDH_ERR_INVALID;
   *out = nullptr;
   const kdh_tree_gen_params& P = *params;
   // makeTree returns nullptr below leaf_min_points; an empty cloud gives no tree

@@ -1473,7 +1473,100 @@ struct TreeGenParams {  // kdh_tree_gen_params layout
   int32_t propagate_normal, tree_min_good_leaves, leaf_min_points, leaf_max_points;
   float leaf_density, leaf_min_normal_incidence, leaf_eigenvalue_threshold, leaf_normal_eigenratio;
   int32_t parallel_level;
+  int32_t label_mode;  // 1: the reference's FP32 arithmetic (below)
 };
+
+// SelfAdjointEigenSolver<Matrix3f>::computeDirect (Eigen 3.4, which the
+// reference takes from the system: kd_slam2/CMakeLists.txt:26), from its
+// published closed form.  Matrix shifted by trace / 3 and divided by its
+// largest |entry|; eigenvalues from the trigonometric solution of the
+// characteristic cubic; eigenvectors from cross products of the columns of
+// the rank-2 matrices A - lambda I.  Parity with Eigen's own bits unpinned:
+// Eigen is absent here.
+struct EigDirect {
+  float w[3];          // ascending
+  Vec3<float> v[3];    // v[k]: eigenvector of w[k]
+};
+
+static Vec3<float> scaled(Vec3<float> a, float f) { return mk(a[0] * f, a[1] * f, a[2] * f); }
+static Vec3<float> divided(Vec3<float> a, float d) { return mk(a[0] / d, a[1] / d, a[2] / d); }
+
+// extract_kernel: the column of largest |diagonal| (first on ties) crossed
+// with each following column; the product of larger squared norm, unit.
+static void kernel_of(const Mat3<float>& M, Vec3<float>& res, Vec3<float>& representative) {
+  int i0 = 0;
+  for (int i = 1; i < 3; ++i)
+    if (fabsf(M.m[i][i]) > fabsf(M.m[i0][i0])) i0 = i;
+  auto col = [&](int c) { return mk(M.m[0][c], M.m[1][c], M.m[2][c]); };
+  representative = col(i0);
+  const Vec3<float> a = cross(representative, col((i0 + 1) % 3));
+  const Vec3<float> b = cross(representative, col((i0 + 2) % 3));
+  const float na = dot(a, a), nb = dot(b, b);
+  res = na > nb ? divided(a, sqrtf(na)) : divided(b, sqrtf(nb));
+}
+
+static EigDirect eig_direct(const Mat3<float>& A) {
+  EigDirect out;
+  const float third = 1.0f / 3.0f;
+  const float shift = ((A.m[0][0] + A.m[1][1]) + A.m[2][2]) / 3.0f;
+  Mat3<float> M;
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) M.m[r][c] = A.m[r < c ? c : r][r < c ? r : c];
+  for (int i = 0; i < 3; ++i) M.m[i][i] -= shift;
+  float scale = 0.0f;
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) scale = std::max(scale, fabsf(M.m[r][c]));
+  if (scale > 0.0f)
+    for (int r = 0; r < 3; ++r)
+      for (int c = 0; c < 3; ++c) M.m[r][c] = M.m[r][c] / scale;
+  const float a00 = M.m[0][0], a11 = M.m[1][1], a22 = M.m[2][2];
+  const float a10 = M.m[1][0], a20 = M.m[2][0], a21 = M.m[2][1];
+  // det, the sum of the principal 2x2 minors and the trace
+  const float det = a00 * a11 * a22 + 2.0f * a10 * a20 * a21 - a00 * a21 * a21 - a11 * a20 * a20 - a22 * a10 * a10;
+  const float minors = a00 * a11 - a10 * a10 + a00 * a22 - a20 * a20 + a11 * a22 - a21 * a21;
+  const float tr = a00 + a11 + a22;
+  const float m = tr * third;
+  const float p = std::max((tr * m - minors) * third, 0.0f);
+  const float hb = 0.5f * (det + m * (2.0f * m * m - minors));
+  const float disc = std::max(p * p * p - hb * hb, 0.0f);
+  const float r = sqrtf(p);
+  const float phi = atan2f(sqrtf(disc), hb) * third;
+  const float cph = cosf(phi), sph = sinf(phi), s3 = sqrtf(3.0f);
+  out.w[0] = m - r * (cph + s3 * sph);
+  out.w[1] = m - r * (cph - s3 * sph);
+  out.w[2] = m + 2.0f * r * cph;
+  const float eps = std::numeric_limits<float>::epsilon();
+  if (out.w[2] - out.w[0] <= eps) {
+    out.v[0] = mk(1.0f, 0.0f, 0.0f);
+    out.v[1] = mk(0.0f, 1.0f, 0.0f);
+    out.v[2] = mk(0.0f, 0.0f, 1.0f);
+  } else {
+    const float gap_hi = out.w[2] - out.w[1], gap_lo = out.w[1] - out.w[0];
+    const bool top_first = gap_hi > gap_lo;  // the most separated eigenvalue is the largest
+    const int k = top_first ? 2 : 0, l = top_first ? 0 : 2;
+    const float dk = top_first ? gap_lo : gap_hi;  // the smaller gap
+    const float dl = gap_lo;
+    Mat3<float> T = M;
+    for (int i = 0; i < 3; ++i) T.m[i][i] -= out.w[k];
+    kernel_of(T, out.v[k], out.v[l]);
+    if (dk <= 2.0f * eps * dl) {
+      const float pr = dot(out.v[k], out.v[l]);
+      out.v[l] = sub(out.v[l], scaled(out.v[l], pr));
+      const float z = dot(out.v[l], out.v[l]);
+      if (z > 0.0f) out.v[l] = divided(out.v[l], sqrtf(z));
+    } else {
+      T = M;
+      for (int i = 0; i < 3; ++i) T.m[i][i] -= out.w[l];
+      Vec3<float> ignored;
+      kernel_of(T, out.v[l], ignored);
+    }
+    const Vec3<float> mid = cross(out.v[2], out.v[0]);
+    const float z = dot(mid, mid);
+    out.v[1] = z > 0.0f ? divided(mid, sqrtf(z)) : mid;
+  }
+  for (int k = 0; k < 3; ++k) out.w[k] = out.w[k] * scale + shift;
+  return out;
+}
 
 struct TPoint {
   float c[3];
@@ -1571,6 +1664,57 @@ struct TreeGen {
     for (int i = 0; i < 3; ++i) n.direction[i] = d[i];
   }
 
+  // LabelPolicyBase_::fillMetadata with the reference's arithmetic
+  // (label_policy_base_impl_.h:41-95): FP32 raw moments through StableAdder_,
+  // E[x x^T] - mu mu^T, stamp moments through a double StableAdder_ scaled by
+  // the FP32 1/n, computeDirect.
+  void fill_base_ref(TMeta& m, Node& node) {
+    const float isize = 1.0f / (float)m.n_points;
+    StableAdder<Packed<double, 2>> ts_adder;
+    StableAdder<Packed<float, 9>> acc;
+    for (size_t i = node.idx_start; i < node.idx_end; ++i) {
+      const double d = pts[i].t - m.ts_start;
+      Packed<double, 2> tv;
+      tv.a[0] = d;
+      tv.a[1] = d * d;
+      ts_adder.add(tv);
+      Packed<float, 9> v;
+      const float* c = pts[i].c;
+      int k = 0;
+      for (int r = 0; r < 3; ++r)
+        for (int q = r; q < 3; ++q) v.a[k++] = c[r] * c[q];
+      for (int r = 0; r < 3; ++r) v.a[k++] = c[r];
+      acc.add(v);
+    }
+    const Packed<double, 2> ts2 = ts_adder.sum();
+    const double t0 = ts2.a[0] * (double)isize, t1 = ts2.a[1] * (double)isize;
+    node.dts_mean = t0;
+    node.dts_covariance = t1 - t0 * t0;
+    Packed<float, 9> sum = acc.sum();
+    for (int k = 0; k < 9; ++k) sum.a[k] = sum.a[k] * isize;
+    Mat3<float> cov;
+    int k = 0;
+    for (int r = 0; r < 3; ++r)
+      for (int q = r; q < 3; ++q, ++k) cov.m[r][q] = cov.m[q][r] = sum.a[k];
+    m.mean = mk(sum.a[6], sum.a[7], sum.a[8]);
+    for (int r = 0; r < 3; ++r) node.mean[r] = m.mean[r];
+    for (int r = 0; r < 3; ++r)
+      for (int q = 0; q < 3; ++q) cov.m[r][q] = cov.m[r][q] - m.mean[r] * m.mean[q];
+    if (m.n_points < 3) {
+      set_dir(node, mk(0.0f, 0.0f, 0.0f));
+      m.is_bad = true;
+      return;
+    }
+    const EigDirect e = eig_direct(cov);
+    for (int q = 0; q < 3; ++q) {
+      m.lam[q] = (double)e.w[q];
+      m.evec[q] = e.v[q];
+    }
+    if (dot(m.mean, m.evec[0]) > 0) m.evec[0] = mk(-m.evec[0][0], -m.evec[0][1], -m.evec[0][2]);
+    if (dot(m.mean, m.evec[2]) > 0) m.evec[2] = mk(-m.evec[2][0], -m.evec[2][1], -m.evec[2][2]);
+    m.is_bad = false;
+  }
+
   void fill_base(TMeta& m, Node& node, const TMeta* parent) {
     m.n_points = node.idx_end - node.idx_start;
     if (!m.n_points) {
@@ -1585,6 +1729,10 @@ struct TreeGen {
       m.ts_start = parent->ts_start;
     }
     m.level = parent ? parent->level + 1 : 0;
+    if (P.label_mode == 1) {
+      fill_base_ref(m, node);
+      return;
+    }
     const double isize = 1.0 / (double)m.n_points;
     const Acc<5> s1 = canonical_sum<5>(node.idx_start, node.idx_end, [&](size_t i) {
       const double d = pts[i].t - m.ts_start;
@@ -1682,15 +1830,32 @@ struct TreeGen {
     TMeta m;
     fill(m, node, parent);
     if (m.is_node) {
-      // splitRange, each side kept in order
-      std::vector<TPoint> left, right;
-      for (size_t i = s; i < e; ++i) {
-        const Vec3<float> d = mk(pts[i].c[0] - node.mean[0], pts[i].c[1] - node.mean[1], pts[i].c[2] - node.mean[2]);
-        (dot(dir_of(node), d) > 0 ? left : right).push_back(pts[i]);
+      auto left_side = [&](const TPoint& q) {
+        const Vec3<float> d = mk(q.c[0] - node.mean[0], q.c[1] - node.mean[1], q.c[2] - node.mean[2]);
+        return dot(dir_of(node), d) > 0;
+      };
+      size_t mid;
+      if (P.label_mode == 1) {
+        // splitRange (node_impl_.h:12-26): right-side points swapped to the
+        // back; signed bounds (the reference's unsigned end wraps below 0)
+        long lo = (long)s, hi = (long)e - 1;
+        while (lo <= hi) {
+          if (left_side(pts[lo])) {
+            ++lo;
+          } else {
+            std::swap(pts[lo], pts[hi]);
+            --hi;
+          }
+        }
+        mid = (size_t)lo;
+      } else {
+        // splitRange, each side kept in order
+        std::vector<TPoint> left, right;
+        for (size_t i = s; i < e; ++i) (left_side(pts[i]) ? left : right).push_back(pts[i]);
+        mid = s + left.size();
+        std::copy(left.begin(), left.end(), pts + s);
+        std::copy(right.begin(), right.end(), pts + mid);
       }
-      const size_t mid = s + left.size();
-      std::copy(left.begin(), left.end(), pts + s);
-      std::copy(right.begin(), right.end(), pts + mid);
       if (mid == s || mid == e) {
         set_dir(node, m.evec[0]);
       } else {
@@ -1726,6 +1891,20 @@ struct TreeGen {
 }  // namespace orc
 
 extern "C" {
+
+// computeDirect of a symmetric 3x3 (row-major A, lower triangle read):
+// eigenvalues ascending in w, eigenvectors as the columns of V (row-major).
+void orc_eig_direct(const float* A9, float* w, float* V9) {
+  using namespace orc;
+  Mat3<float> A;
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) A.m[r][c] = A9[3 * r + c];
+  const EigDirect e = eig_direct(A);
+  for (int k = 0; k < 3; ++k) {
+    w[k] = e.w[k];
+    for (int r = 0; r < 3; ++r) V9[3 * r + k] = e.v[k][r];
+  }
+}
 
 // makeTree on n points (xyz 3 floats each, stamps or NULL).  Returns 1 and
 // fills the BFS-ordered nodes (capacity 2n), the leaf list (capacity n) and

@@ -59,6 +59,7 @@ class OrcTreeGenParams(C.Structure):
         ("leaf_eigenvalue_threshold", C.c_float),
         ("leaf_normal_eigenratio", C.c_float),
         ("parallel_level", C.c_int32),
+        ("label_mode", C.c_int32),
     ]
 
 
@@ -66,7 +67,7 @@ def tree_gen_params(**kw):
     """The reference defaults (tree_generator_.h:18-25), overridden by kw."""
     p = OrcTreeGenParams(propagate_normal=0, tree_min_good_leaves=10, leaf_min_points=5, leaf_max_points=30,
                          leaf_density=100.0, leaf_min_normal_incidence=0.1, leaf_eigenvalue_threshold=1e-3,
-                         leaf_normal_eigenratio=0.1, parallel_level=0)
+                         leaf_normal_eigenratio=0.1, parallel_level=0, label_mode=0)
     for k, v in kw.items():
         setattr(p, k, v)
     return p
@@ -134,6 +135,7 @@ def lib(fast=False):
         L.orc_make_scan.restype = C.c_int64
         L.orc_make_scan.argtypes = [f32, C.c_int64, f32, f32, C.c_double, f32, f64, C.c_int64]
         L.orc_scene_expmap.argtypes = [f32, f32]
+        L.orc_eig_direct.argtypes = [f32, f32, f32]
         _libs[name] = L
     return _libs[name]
 
@@ -360,6 +362,16 @@ def tree_build(points, stamps=None, **params):
         return None
     return dict(nodes=nodes[:nn.value].copy(), leaves=leaves[:nl.value].copy(), points=xyz[:n].copy(),
                 stamps=sto[:n].copy())
+
+
+def eig_direct(A):
+    """SelfAdjointEigenSolver<Matrix3f>::computeDirect of a symmetric 3x3:
+    (ascending eigenvalues (3,), eigenvectors as columns (3, 3)), FP32."""
+    A = np.ascontiguousarray(A, dtype=np.float32).reshape(3, 3)
+    w = np.zeros(3, dtype=np.float32)
+    V = np.zeros((3, 3), dtype=np.float32)
+    lib().orc_eig_direct(p32(A), p32(w), p32(V))
+    return w, V
 
 
 # ---- the reference tests' scenes (oracle/scenes.cpp) -------------------------

@@ -62,7 +62,7 @@ def gen_params(**kw):
     p = L.TreeGenParams()
     base = dict(propagate_normal=0, tree_min_good_leaves=10, leaf_min_points=5, leaf_max_points=30,
                 leaf_density=100.0, leaf_min_normal_incidence=0.1, leaf_eigenvalue_threshold=1e-3,
-                leaf_normal_eigenratio=0.1, parallel_level=0)
+                leaf_normal_eigenratio=0.1, parallel_level=0, label_mode=0)
     base.update(kw)
     for k, v in base.items():
         setattr(p, k, v)
@@ -93,14 +93,17 @@ def trees(H):
     out = []
     for name, pts, st, kw in tree_clouds():
         for level in (0, 2):
-            t = build_tree(H, pts, st, parallel_level=level, **kw)
-            if t is not None:
-                out.append(t)
+            for mode in (0, 1):
+                t = build_tree(H, pts, st, parallel_level=level, label_mode=mode, **kw)
+                if t is not None:
+                    out.append(t)
     rng = np.random.default_rng(7)
-    for n in (0, 1, 4, 5, 31, 200, 5000):
-        build_tree(H, rng.normal(size=(n, 3)), rng.uniform(0, 0.1, n) if n else None, tree_min_good_leaves=0)
-    build_tree(H, np.full((100, 3), np.nan, dtype=np.float32), None, tree_min_good_leaves=0)
-    build_tree(H, np.full((100, 3), np.inf, dtype=np.float32), None, tree_min_good_leaves=0)
+    for mode in (0, 1):
+        for n in (0, 1, 4, 5, 31, 200, 5000):
+            build_tree(H, rng.normal(size=(n, 3)), rng.uniform(0, 0.1, n) if n else None, tree_min_good_leaves=0,
+                       label_mode=mode)
+        build_tree(H, np.full((100, 3), np.nan, dtype=np.float32), None, tree_min_good_leaves=0, label_mode=mode)
+        build_tree(H, np.full((100, 3), np.inf, dtype=np.float32), None, tree_min_good_leaves=0, label_mode=mode)
     print(f"tree builder: {len(out)} trees", flush=True)
     return out
 
@@ -229,6 +232,7 @@ def oracle(H, path):
     O.compute(F, M, w["X0"], w["Xf"], p, ct=1, v0=np.zeros(6))
     O.build_form_dual(F, M, w["X0"], np.zeros(6), w["Xf"], np.zeros(6), p)
     O.tree_build(w["scan"], None, leaf_max_points=16)
+    O.tree_build(w["scan"], None, leaf_max_points=16, label_mode=1)
     O.compute64(F, M, w["X0"], w["Xf"], p)
     print("oracle: compute, CT compute, dual form, tree build, FP64 loop", flush=True)
 

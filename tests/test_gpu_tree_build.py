@@ -146,3 +146,34 @@ def test_tree_from_a_kdtree3f_message_aligns_identically(ctx):
         al.computeOnDevice()
         out.append((al.getX(), al.velocities()))
     assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+
+
+def test_reference_arithmetic_trees_on_the_device(ctx):
+    """Trees built with the reference's own arithmetic (label_mode 1, host
+    builder) align on the device like any other: a rigid alignment against
+    the FP32 oracle on the same trees, pose within 1 mm / 1e-3 deg and the
+    same result code and per-round counters.  The device builder refuses the
+    mode with an error naming the host builder."""
+    from kd_slam2_amd import ICPHIP3D
+
+    w = synth.config1(seed=4, n_map=120_000, n_scan=12_000)
+    g = TreeGenerator3D(leaf_max_points=16, label_mode=1)
+    Fh, Mh = g.makeTree(w["map"]), g.makeTree(w["scan"])
+    p = icp_params(max_iterations=10)
+    F, M = DeviceTree(ctx, Fh), DeviceTree(ctx, Mh)
+    al = ICPHIP3D(ctx)
+    al.params = p
+    al.setFixed(F)
+    al.setMoving(M)
+    al.setX(w["X0"])
+    al.setFixedX(w["Xf"])
+    res = al.computeOnDevice()
+    Xo, _, log, res_o = O.compute(Fh, Mh, w["X0"], w["Xf"], O.from_kdh_params(p, 8))
+    dt, dr = synth.pose_error(al.getX(), Xo)
+    assert dt < 1e-3 and dr < 1e-3 and res == res_o
+    for a, b in zip(al.stats_log, log):
+        assert all(a[k] == b[k] for k in ("num_inliers", "num_outliers", "num_bad", "num_err_normal", "num_err_fn"))
+    with pytest.raises(RuntimeError, match="host"):
+        g.makeTreeOnDevice(ctx, w["scan"])
+    F.close()
+    M.close()
