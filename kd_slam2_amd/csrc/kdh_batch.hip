@@ -345,10 +345,16 @@ int kdh_batch_align(kdh_ctx* ctx, int n_pairs, const kdh_pair* pairs, kdh_pair_r
   // are not rewritten).
   std::vector<kdh_stats> entry(n_pairs);
   std::vector<float> rows;
+  // A forest without moving leaves never ran a round on the device: its one
+  // failed round has all-zero statistics (the state's log entry was not
+  // written and holds whatever an earlier batch left there).
+  auto last_of = [](const PairState& L, kdh_stats& st) {
+    memset(&st, 0, sizeof(kdh_stats));
+    if (L.iter > 0 && L.group_tiles > 0) memcpy(&st, &L.stats_log[L.iter - 1], sizeof(kdh_stats));
+  };
   for (int i = 0; i < n_pairs; ++i) {
     const PairState& L = ctx->h_pairs[leader[i]];
-    memset(&entry[i], 0, sizeof(kdh_stats));
-    if (L.iter > 0) memcpy(&entry[i], &L.stats_log[L.iter - 1], sizeof(kdh_stats));
+    last_of(L, entry[i]);
     entry[i].pert_pose_norm = entry[i].pert_vel_norm = 0.0;
   }
   for (int i = 0; i < n_pairs; ++i) {
@@ -381,10 +387,7 @@ int kdh_batch_align(kdh_ctx* ctx, int n_pairs, const kdh_pair* pairs, kdh_pair_r
     memcpy(r.X, ps.X, sizeof(r.X));
     memcpy(r.v, ps.v, sizeof(r.v));
     r.iterations = ps.iter;
-    if (ps.iter > 0)
-      memcpy(&r.last, &ps.stats_log[ps.iter - 1], sizeof(kdh_stats));
-    else
-      memset(&r.last, 0, sizeof(kdh_stats));
+    last_of(ps, r.last);
     r.entry = entry[i];
     if (ps.group_n < 2) r.entry = r.last;
     r.result = kdh_result_code(ps, pairs[leader[i]].params.max_iterations);
@@ -628,8 +631,8 @@ __global__ __launch_bounds__(64) void k_batch_collect(const PairState* __restric
     s.iter = it;
     s.status = L.status;
     s.pad[0] = s.pad[1] = 0;
-    DevStats last{};
-    if (it > 0) last = L.stats_log[it - 1];
+    DevStats last{};  // all zero for a forest without moving leaves (no round ran on the device)
+    if (it > 0 && L.group_tiles > 0) last = L.stats_log[it - 1];
     s.last = last;
     DevStats e = last;
     if (L.group_n >= 2) {

@@ -222,7 +222,8 @@ def test_prepared_batch_equals_batch_align(ctx, small_pairs):
     kdh_batch_align on a mix of rigid pairs, CT pairs, a fixed forest of
     three entries and a pair whose scan has no leaves, on the first run and
     on runs from new initial states (set_initial), on two streams and in
-    profiling mode."""
+    profiling mode.  The leafless pair's one failed round has all-zero
+    statistics in both forms, whatever the device state held before."""
     from kd_slam2_amd import HostTree, PairBatch
 
     from .test_tree_and_descent import _node
@@ -242,8 +243,14 @@ def test_prepared_batch_equals_batch_align(ctx, small_pairs):
     lone = dict(fixed=trees[0][0], moving=empty, ct=0, X0=w0["X0"], Xf=w0["Xf"], params=icp_params())
     allp = pairs[:3] + group + [lone] + pairs[3:]
     B = PairBatch(allp)
+    lone_i = 3 + len(group)
     try:
+        # the context's device states first hold a finished batch, so a pair
+        # slot that no round rewrites would show that batch's statistics
+        batch_align(ctx, pairs)
         ref = batch_align(ctx, allp)
+        assert ref[lone_i]["iterations"] == 1 and ref[lone_i]["result"] == 0  # compute() returns -0
+        assert all(v == 0 for v in ref[lone_i]["last"].values()), ref[lone_i]["last"]
         got = list(B.run(ctx))
         assert all(_same(a, b) for a, b in zip(ref, got))
         assert got[3]["entry"] != got[4]["entry"]  # forest entries carry their own share
