@@ -7,10 +7,10 @@ room-scene generator with its own seed (3000 + i), scans log-uniform in
 KD-tree (leaf size 16), initial errors up to 5 deg / 1 m, 10 Gauss-Newton
 iterations, max_error 0.5.  Trees are resident in HBM before timing starts.
 
-A step is one kdh_batch_align call over the 256 pairs (a PairBatch: the pair
-array is marshalled once; each step sets a new set of initial poses): one
-tile launch and one finish launch per iteration, all iterations in one HIP
-graph, results back on the host.  value = scan points of completed
+A step is one run of the 256 pairs prepared as a batch (PairBatch:
+kdh_batch_prepare once, then kdh_batch_run with a new set of initial poses
+per step): one graph holding the state initialisation, one tile launch and
+one finish launch per iteration, and the result write-back to the host.  value = scan points of completed
 alignments per second over the timed steps.  The launch shape (leaves per
 lane, LDS staging depth, grid) is picked first by the project autotuner on
 the same batch.
@@ -197,7 +197,7 @@ def _hip():
 def pmc_child(args):
     """One batch of the headline workload at the tuned launch shape plus a
     device copy of COPY_BYTES bytes, for the counter passes."""
-    from kd_slam2_amd import Context, DeviceTree, PairBatch, default_params
+    from kd_slam2_amd import Context, DeviceTree, batch_align, default_params
 
     with mp.get_context("fork").Pool(min(16, os.cpu_count() or 4)) as pool:
         pairs3 = _gen_pairs3(args.pairs3, pool).get()
@@ -205,9 +205,9 @@ def pmc_child(args):
     ctx.set_launch_config(grid=args.pmc_grid, lds_nodes=args.pmc_lds, leaves_per_lane=args.pmc_lpl)
     p = _p3_params(default_params)
     trees = [(DeviceTree(ctx, q["fixed"]), DeviceTree(ctx, q["moving"])) for q in pairs3]
-    B = PairBatch([dict(fixed=F, moving=M, ct=0, X0=q["X0"], Xf=q["Xf"], params=p)
-                   for q, (F, M) in zip(pairs3, trees)])
-    B.run(ctx)
+    batch_align(ctx, [dict(fixed=F, moving=M, ct=0, X0=q["X0"], Xf=q["Xf"], params=p)
+                      for q, (F, M) in zip(pairs3, trees)])
+    print("pmc child: batch done", flush=True)
     hip = _hip()
     a, b = ctypes.c_void_p(), ctypes.c_void_p()
     hip.hipMalloc(ctypes.byref(a), COPY_BYTES)
@@ -270,7 +270,8 @@ def pmc_pass(counter, args, tuned):
                            env=dict(os.environ, TMPDIR="/tmp"))
         files = glob.glob(os.path.join(out, "**", "*counter_collection.csv"), recursive=True)
         if not files:
-            log(f"pmc {counter}: no counter file (rc {r.returncode})")
+            tail = r.stdout.decode(errors="replace").strip().splitlines()[-3:] if r.stdout else []
+            log(f"pmc {counter}: no counter file (rc {r.returncode}); child output ends: {tail}")
             return None
         tile, copy = [], []
         with open(files[0]) as fh:

@@ -272,13 +272,6 @@ static int end_capture(kdh_ctx* ctx, int rc, hipGraphExec_t* exec) {
   hipError_t ie = hipGraphInstantiate(exec, graph, nullptr, nullptr, 0);
   hipGraphDestroy(graph);
   if (ie != hipSuccess) return kdh_hip_fail(ctx, ie, "hipGraphInstantiate");
-  // the launch resources go to the device now, not at the first launch
-  ie = hipGraphUpload(*exec, ctx->stream);
-  if (ie != hipSuccess) {
-    hipGraphExecDestroy(*exec);
-    *exec = nullptr;
-    return kdh_hip_fail(ctx, ie, "hipGraphUpload");
-  }
   return KDH_OK;
 }
 

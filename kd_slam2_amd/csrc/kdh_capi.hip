@@ -664,11 +664,6 @@ int kdh_align(kdh_ctx* ctx, const kdh_tree* fixed, const kdh_tree* moving, int c
         hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
         hipGraphDestroy(graph);
         if (ie != hipSuccess) return kdh_hip_fail(ctx, ie, "hipGraphInstantiate");
-        ie = hipGraphUpload(exec, ctx->stream);
-        if (ie != hipSuccess) {
-          hipGraphExecDestroy(exec);
-          return kdh_hip_fail(ctx, ie, "hipGraphUpload");
-        }
         kdh_graph_store(ctx->graph_align, key, exec);
       }
       hipError_t le = hipEventRecord(ctx->ev[0], ctx->stream);
