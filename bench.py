@@ -208,6 +208,12 @@ def pmc_child(args):
     batch_align(ctx, [dict(fixed=F, moving=M, ct=0, X0=q["X0"], Xf=q["Xf"], params=p)
                       for q, (F, M) in zip(pairs3, trees)])
     print("pmc child: batch done", flush=True)
+    _copy_probe()
+
+
+def _copy_probe():
+    """One device-to-device copy of COPY_BYTES bytes: the known traffic that
+    scales the counters."""
     hip = _hip()
     a, b = ctypes.c_void_p(), ctypes.c_void_p()
     hip.hipMalloc(ctypes.byref(a), COPY_BYTES)
@@ -216,6 +222,24 @@ def pmc_child(args):
     hip.hipFree(a)
     hip.hipFree(b)
     print("pmc child done", flush=True)
+
+
+def pmc_child_ct(args):
+    """One CT served batch (--batch alignments of the 131k-point sweep against
+    the 2M-point map, 15 iterations, the initial states of the CT row) through
+    kdh_batch_align, then the copy probe: the workload of the CT counter
+    passes."""
+    from kd_slam2_amd import Context, DeviceTree, batch_align, default_params, synth
+
+    w = _gen_cfg2(0)
+    ctx = Context(0)
+    F, M = DeviceTree(ctx, w["fixed"]), DeviceTree(ctx, w["moving"])
+    p = _cfg2_params(default_params)
+    rng = np.random.default_rng(np.random.PCG64(99))
+    batch_align(ctx, [dict(fixed=F, moving=M, ct=1, X0=synth.perturb(w["X_gt"], rng, 1.0, 0.3), Xf=w["Xf"],
+                           v0=w["v0"], params=p) for _ in range(args.batch)])
+    print("pmc child: CT batch done", flush=True)
+    _copy_probe()
 
 
 def pmc_child_precision(args):
@@ -243,14 +267,7 @@ def pmc_child_precision(args):
         al.computeOnDevice()
     print(f"precision device loop: {ctx.last_kernel_time()[0]:.2f} us/iter, {M.num_leaves} moving leaves, "
           f"{len(fixed.nodes)} fixed nodes", flush=True)
-    hip = _hip()
-    a, b = ctypes.c_void_p(), ctypes.c_void_p()
-    hip.hipMalloc(ctypes.byref(a), COPY_BYTES)
-    hip.hipMalloc(ctypes.byref(b), COPY_BYTES)
-    hip.hipMemcpy(b, a, COPY_BYTES, 3)  # device to device
-    hip.hipFree(a)
-    hip.hipFree(b)
-    print("pmc child done", flush=True)
+    _copy_probe()
 
 
 def pmc_pass(counter, args, tuned):
@@ -327,6 +344,8 @@ def main():
     ap.add_argument("--tree-build-only", action="store_true",
                     help="only the tree-generation timings (for kernel traces of the builder)")
     ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--pmc-child-ct", action="store_true",
+                    help="only one CT served batch and the copy probe (for the CT counter passes)")
     ap.add_argument("--pmc-child-precision", action="store_true",
                     help="only the precision-stress device loop (for traces and counter passes of the 8M map)")
     ap.add_argument("--pmc-grid", type=int, default=0, help=argparse.SUPPRESS)
@@ -338,6 +357,9 @@ def main():
         return
     if args.pmc_child_precision:
         pmc_child_precision(args)
+        return
+    if args.pmc_child_ct:
+        pmc_child_ct(args)
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); alignments shard by pair with no "
