@@ -158,16 +158,21 @@ __device__ __forceinline__ double rcp_nr(double d) {
 // of the symmetric matrix in a[] and b_i in bi; right-looking LDL^T without
 // pivoting (the damped normal matrix is positive definite), then forward and
 // backward substitution with lane broadcasts.  Lanes >= D carry zeros.  The
-// caller's workgroup is this one wave.
+// caller's workgroup is this one wave.  The result is the same on the wave
+// alone or in any batch; it differs from the host solve of kdh_solve.h in
+// the last bits (fused multiply-adds here).
 template <int D>
 __device__ void wave_ldlt_solve(double (&a)[D], double bi, double damp_pose, double damp_vel, int lane,
                                 double* sx, double* sL) {
+  // FP64 only, and no decision rests on its last bits: multiply-adds may fuse
+  // here (one dependent operation per update instead of two), unlike the
+  // FP32 code whose rounding the decisions follow.
+#pragma clang fp contract(fast)
   const bool row = lane < D;
-  if (row) {
+  // the damping on the diagonal, by selects (a branch per column would split
+  // the wave lane by lane); off the diagonal the added zero changes nothing
 #pragma unroll
-    for (int j = 0; j < D; ++j)
-      if (j == lane) a[j] += (j < 6) ? damp_pose : damp_vel;
-  }
+  for (int j = 0; j < D; ++j) a[j] = a[j] + ((row && j == lane) ? ((j < 6) ? damp_pose : damp_vel) : 0.0);
   double r = row ? -bi : 0.0;
   double dd = 0.0;  // this lane's pivot D_i
 #pragma unroll
@@ -227,19 +232,27 @@ struct FinishLds {
   static constexpr int kBytes = kL + 8 * (D * D > 40 ? D * D : 40);
 };
 
-// Entry (r, c) of the re-centred H_c from the reduced slots (upper-triangular
-// ICPWsItemDiag_ blocks and the ICPWsItemCross_ block).
+// Row r (< D) of H_c from the reduced slots, for a lane-dependent r and every
+// column: the slot of (r, c) is a per-lane base plus a column offset, so the
+// loop over c is one add and one LDS read per entry.  Entries (r, c) of a
+// diagonal block with c < r come from the upper triangle (c, r), whose slot
+// is a compile-time base plus r.
 template <bool CT>
-__device__ __forceinline__ double hc_entry(const double* s_red, int r, int c) {
-  if (r < 6 && c < 6) {
-    const int rr = r < c ? r : c, c2 = r < c ? c : r;
-    return s_red[rr * 6 - (rr * (rr - 1)) / 2 + (c2 - rr)];
+__device__ __forceinline__ void hc_row(const double* s_red, int r, double (&h)[CT ? 12 : 6]) {
+  constexpr int D = CT ? 12 : 6;
+  const bool pose = r < 6;
+  const int rl = pose ? r : r - 6;                       // row within its diagonal block
+  const int tri_r = rl * 6 - (rl * (rl - 1)) / 2 - rl;   // slot of (rl, c) is tri_r + c, c >= rl
+  const int blk = pose ? 0 : 27;
+#pragma unroll
+  for (int c = 0; c < D; ++c) {
+    const int cl = c < 6 ? c : c - 6;
+    const int tri_c = cl * 6 - (cl * (cl - 1)) / 2 - cl;  // compile time: slot of (cl, rl) is tri_c + rl
+    const bool same = pose == (c < 6);
+    const int diag = blk + (cl >= rl ? tri_r + cl : tri_c + rl);
+    const int cross = pose ? 54 + r * 6 + (c - 6) : 54 + c * 6 + rl;
+    h[c] = s_red[same ? diag : cross];
   }
-  if (r >= 6 && c >= 6) {
-    const int rr = (r < c ? r : c) - 6, c2 = (r < c ? c : r) - 6;
-    return s_red[27 + rr * 6 - (rr * (rr - 1)) / 2 + (c2 - rr)];
-  }
-  return r < 6 ? s_red[54 + r * 6 + (c - 6)] : s_red[54 + c * 6 + (r - 6)];
 }
 
 // Column j of [c]x (skew), chosen by selects: indexing a private matrix with
@@ -324,15 +337,14 @@ __device__ void finish_pair(PairState& ps, const PairState& S, const float* __re
   const bool row = lane < D;
   const int r = row ? lane : 0;
   double h[D];
-#pragma unroll
-  for (int c = 0; c < D; ++c) h[c] = hc_entry<CT>(s_red, r, c);
+  hc_row<CT>(s_red, r, h);
   {
+    // h[3..5] -= h[0..2] . column (c - 3) of [cc]x; the zero of each column
+    // is left out (adding a product with zero changes no finite value)
     const double h0 = h[0], h1 = h[1], h2 = h[2];
-#pragma unroll
-    for (int c = 3; c < 6; ++c) {
-      const V3d k = skew_col(cc, c - 3);
-      h[c] -= (h0 * k.x + h1 * k.y) + h2 * k.z;
-    }
+    h[3] -= h1 * cc.z + h2 * (-cc.y);
+    h[4] -= h0 * (-cc.z) + h2 * cc.x;
+    h[5] -= h0 * cc.y + h1 * (-cc.x);
   }
   if (lane < 3) {
 #pragma unroll
@@ -342,10 +354,15 @@ __device__ void finish_pair(PairState& ps, const PairState& S, const float* __re
   double bv = bc(r);
   __syncthreads();
   if (lane >= 3 && lane < 6) {
-    const V3d k = skew_col(cc, lane - 3);
+    // row (lane - 3) of [cc]x^T is column (lane - 3) of [cc]x: two non-zero
+    // entries, on rows i1 < i2 of G
+    const int j = lane - 3;
+    const int i1 = j == 0 ? 1 : 0, i2 = j == 2 ? 1 : 2;
+    const double k1 = j == 0 ? cc.z : j == 1 ? -cc.z : cc.y;
+    const double k2 = j == 0 ? -cc.y : j == 1 ? cc.x : -cc.x;
 #pragma unroll
-    for (int c = 0; c < D; ++c) h[c] -= (k.x * sG[0 * D + c] + k.y * sG[1 * D + c]) + k.z * sG[2 * D + c];
-    bv = bv - ((k.x * bc(0) + k.y * bc(1)) + k.z * bc(2));
+    for (int c = 0; c < D; ++c) h[c] -= k1 * sG[i1 * D + c] + k2 * sG[i2 * D + c];
+    bv = bv - (k1 * bc(i1) + k2 * bc(i2));
   }
 
   DevStats st;

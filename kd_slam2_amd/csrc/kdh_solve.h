@@ -18,9 +18,37 @@ namespace kdh {
 
 // ---- SE(3) in FP64 ----------------------------------------------------------
 
+// Rotation by the axis-angle vector w (Rodrigues).  Below 0.25 rad -- every
+// Gauss-Newton step in practice -- R = I + A [w]x + B [w]x^2 with
+// A = sin(t)/t and B = (1 - cos(t))/t^2 from their Taylor series in t^2
+// (seven terms each; the first omitted ones are below 3e-21) and
+// [w]x^2 = w w^T - t^2 I: no square root, division or trigonometric call on
+// the solver's serial chain.  Larger angles take the closed form.
 KDH_HD M3d rodrigues_d(V3d w) {
-  double angle = sqrt(sqnorm(w));
-  if (angle < 1e-9) return m3_identity<double>();
+  const double t2 = (w.x * w.x + w.y * w.y) + w.z * w.z;
+  if (t2 < 0.0625) {
+    const double A =
+        1.0 + t2 * (-1.0 / 6.0 + t2 * (1.0 / 120.0 + t2 * (-1.0 / 5040.0 + t2 * (1.0 / 362880.0 +
+        t2 * (-1.0 / 39916800.0 + t2 * (1.0 / 6227020800.0))))));
+    const double B =
+        0.5 + t2 * (-1.0 / 24.0 + t2 * (1.0 / 720.0 + t2 * (-1.0 / 40320.0 + t2 * (1.0 / 3628800.0 +
+        t2 * (-1.0 / 479001600.0 + t2 * (1.0 / 87178291200.0))))));
+    const double d = 1.0 - B * t2;
+    const double ax = A * w.x, ay = A * w.y, az = A * w.z;
+    const double bx = B * w.x, by = B * w.y;
+    M3d R;
+    R.m[0] = d + bx * w.x;
+    R.m[1] = bx * w.y - az;
+    R.m[2] = bx * w.z + ay;
+    R.m[3] = bx * w.y + az;
+    R.m[4] = d + by * w.y;
+    R.m[5] = by * w.z - ax;
+    R.m[6] = bx * w.z - ay;
+    R.m[7] = by * w.z + ax;
+    R.m[8] = d + (B * w.z) * w.z;
+    return R;
+  }
+  double angle = sqrt(t2);
   V3d n = w * (1.0 / angle);
   M3d S = skew(n);
   M3d S2 = mul(S, S);
