@@ -101,6 +101,7 @@ int kdh_ctx_destroy(kdh_ctx* ctx) {
   if (ctx->d_corr) hipFree(ctx->d_corr);
   if (ctx->d_depth) hipFree(ctx->d_depth);
   if (ctx->d_extract) hipFree(ctx->d_extract);
+  if (ctx->h_root) hipHostFree(ctx->h_root);
   kdh_tree_build_ws_free(ctx);
   if (ctx->h_pairs) hipHostFree(ctx->h_pairs);
   if (ctx->h_tile_pair) hipHostFree(ctx->h_tile_pair);

@@ -51,6 +51,8 @@ struct kdh_ctx {
   int* d_depth = nullptr;
   size_t depth_cap = 0;
   float* d_extract = nullptr;  // staging of tree read-backs
+  float* h_root = nullptr;      // pinned, mapped: the root mean an in-place tree op writes back
+  float* h_root_dev = nullptr;
   size_t extract_cap = 0;
   void* tree_build_ws = nullptr;  // device tree builder workspace (kdh_tree_build.hip)
   // Instantiated graphs, reused while the launch shape and buffers match.

@@ -59,21 +59,31 @@ __device__ __forceinline__ void store_full(kdh_node3f* full, int i, V3f mean, V3
   n.direction[2] = dir.z;
 }
 
+// root_out (pinned host memory mapped for the device): the root record's new
+// mean, the moving tree's re-centring anchor, read by the host after the
+// stream synchronisation with no copy of its own.
+__device__ __forceinline__ void store_root(float* root_out, V3f mean) {
+  root_out[0] = mean.x;
+  root_out[1] = mean.y;
+  root_out[2] = mean.z;
+}
+
 __global__ __launch_bounds__(kOpThreads) void apply_velocities_kernel(float4* nodes, float4* leaves,
                                                                      const float* node_dts, kdh_node3f* full,
                                                                      int n_nodes, int n_leaves, Vel6 V,
-                                                                     int skip_leaves) {
+                                                                     int skip_leaves, float* root_out) {
   const int i = blockIdx.x * kOpThreads + threadIdx.x;
   if (i < n_nodes) {
     float4 a = nodes[2 * i], b = nodes[2 * i + 1];
     const bool leaf = __float_as_int(b.z) < 0 && __float_as_int(b.w) < 0;
+    V3f mean = v3(a.x, a.y, a.z), dir = v3(a.w, b.x, b.y);
     if (!(leaf && skip_leaves)) {
-      V3f mean = v3(a.x, a.y, a.z), dir = v3(a.w, b.x, b.y);
       warp(mean, dir, node_dts[i], V);
       nodes[2 * i] = make_float4(mean.x, mean.y, mean.z, dir.x);
       nodes[2 * i + 1] = make_float4(dir.y, dir.z, b.z, b.w);
       store_full(full, i, mean, dir);
     }
+    if (i == 0) store_root(root_out, mean);
   }
   if (i < n_leaves && !skip_leaves) {
     float4 a = leaves[2 * i], b = leaves[2 * i + 1];
@@ -86,7 +96,7 @@ __global__ __launch_bounds__(kOpThreads) void apply_velocities_kernel(float4* no
 
 __global__ __launch_bounds__(kOpThreads) void apply_isometry_kernel(float4* nodes, float4* leaves,
                                                                    kdh_node3f* full, int n_nodes, int n_leaves,
-                                                                   Iso12 T) {
+                                                                   Iso12 T, float* root_out) {
   const int i = blockIdx.x * kOpThreads + threadIdx.x;
   const IsoF X = iso_from_34<float>(T.m);
   if (i < n_nodes) {
@@ -95,6 +105,7 @@ __global__ __launch_bounds__(kOpThreads) void apply_isometry_kernel(float4* node
     nodes[2 * i] = make_float4(mean.x, mean.y, mean.z, dir.x);
     nodes[2 * i + 1] = make_float4(dir.y, dir.z, b.z, b.w);
     store_full(full, i, mean, dir);
+    if (i == 0) store_root(root_out, mean);
   }
   if (i < n_leaves) {
     float4 a = leaves[2 * i], b = leaves[2 * i + 1];
@@ -136,14 +147,26 @@ __global__ __launch_bounds__(kOpThreads) void read_nodes_kernel(float* out, cons
 
 int blocks_for(int n) { return (n + kOpThreads - 1) / kOpThreads; }
 
-// the root record's mean is the re-centring anchor of a moving tree
-int refresh_root(kdh_ctx* ctx, kdh_tree* t) {
-  if (t->dt.n_nodes <= 0) return KDH_OK;
-  float4 a;
-  hipError_t e = hipMemcpyAsync(&a, t->dt.nodes, sizeof(float4), hipMemcpyDeviceToHost, ctx->stream);
-  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-  if (e != hipSuccess) return kdh_hip_fail(ctx, e, "tree root read-back");
-  t->root_mean = v3(a.x, a.y, a.z);
+// The pinned root buffer of the context, made on first use.
+int ensure_root_buffer(kdh_ctx* ctx) {
+  if (ctx->h_root) return KDH_OK;
+  hipError_t e = hipHostMalloc((void**)&ctx->h_root, 4 * sizeof(float), hipHostMallocCoherent);
+  if (e == hipSuccess) e = hipHostGetDevicePointer((void**)&ctx->h_root_dev, ctx->h_root, 0);
+  if (e != hipSuccess) {
+    if (ctx->h_root) hipHostFree(ctx->h_root);
+    ctx->h_root = ctx->h_root_dev = nullptr;
+    return kdh_hip_fail(ctx, e, "pinned root buffer");
+  }
+  return KDH_OK;
+}
+
+// The call returns once the op has run: the root record's mean, which the
+// kernel wrote into the pinned buffer, is the re-centring anchor of a moving
+// tree.
+int finish_op(kdh_ctx* ctx, kdh_tree* t) {
+  hipError_t e = hipStreamSynchronize(ctx->stream);
+  if (e != hipSuccess) return kdh_hip_fail(ctx, e, "tree op");
+  if (t->dt.n_nodes > 0) t->root_mean = v3(ctx->h_root[0], ctx->h_root[1], ctx->h_root[2]);
   return KDH_OK;
 }
 
@@ -158,14 +181,17 @@ int kdh_tree_apply_velocities(kdh_ctx* ctx, kdh_tree* tree, const float v[6], in
   if (e != hipSuccess) return kdh_hip_fail(ctx, e, "hipSetDevice");
   const int n = std::max(tree->dt.n_nodes, tree->dt.n_leaves);
   if (n == 0) return KDH_OK;
+  int rc = ensure_root_buffer(ctx);
+  if (rc) return rc;
   Vel6 V;
   for (int i = 0; i < 6; ++i) V.v[i] = v[i];
   hipLaunchKernelGGL(apply_velocities_kernel, dim3(blocks_for(n)), dim3(kOpThreads), 0, ctx->stream,
                      (float4*)tree->dt.nodes, (float4*)tree->dt.leaves, (const float*)tree->node_dts,
-                     tree->full_nodes, tree->dt.n_nodes, tree->dt.n_leaves, V, skip_leaves ? 1 : 0);
+                     tree->full_nodes, tree->dt.n_nodes, tree->dt.n_leaves, V, skip_leaves ? 1 : 0,
+                     ctx->h_root_dev);
   e = hipGetLastError();
   if (e != hipSuccess) return kdh_hip_fail(ctx, e, "apply_velocities_kernel");
-  return refresh_root(ctx, tree);
+  return finish_op(ctx, tree);
 }
 
 int kdh_tree_apply_isometry(kdh_ctx* ctx, kdh_tree* tree, const float iso[12]) {
@@ -177,14 +203,16 @@ int kdh_tree_apply_isometry(kdh_ctx* ctx, kdh_tree* tree, const float iso[12]) {
   if (e != hipSuccess) return kdh_hip_fail(ctx, e, "hipSetDevice");
   const int n = std::max(tree->dt.n_nodes, tree->dt.n_leaves);
   if (n == 0) return KDH_OK;
+  int rc = ensure_root_buffer(ctx);
+  if (rc) return rc;
   Iso12 T;
   for (int i = 0; i < 12; ++i) T.m[i] = iso[i];
   hipLaunchKernelGGL(apply_isometry_kernel, dim3(blocks_for(n)), dim3(kOpThreads), 0, ctx->stream,
                      (float4*)tree->dt.nodes, (float4*)tree->dt.leaves, tree->full_nodes, tree->dt.n_nodes,
-                     tree->dt.n_leaves, T);
+                     tree->dt.n_leaves, T, ctx->h_root_dev);
   e = hipGetLastError();
   if (e != hipSuccess) return kdh_hip_fail(ctx, e, "apply_isometry_kernel");
-  return refresh_root(ctx, tree);
+  return finish_op(ctx, tree);
 }
 
 static int ensure_staging(kdh_ctx* ctx, size_t n_floats, float** dbuf, size_t* cap, const char* what) {
