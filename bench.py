@@ -202,7 +202,8 @@ def pmc_child(args):
     with mp.get_context("fork").Pool(min(16, os.cpu_count() or 4)) as pool:
         pairs3 = _gen_pairs3(args.pairs3, pool).get()
     ctx = Context(0)
-    ctx.set_launch_config(grid=args.pmc_grid, lds_nodes=args.pmc_lds, leaves_per_lane=args.pmc_lpl)
+    ctx.set_launch_config(grid=args.pmc_grid, lds_nodes=args.pmc_lds, leaves_per_lane=args.pmc_lpl,
+                          tile_leaves=args.pmc_tile)
     p = _p3_params(default_params)
     trees = [(DeviceTree(ctx, q["fixed"]), DeviceTree(ctx, q["moving"])) for q in pairs3]
     batch_align(ctx, [dict(fixed=F, moving=M, ct=0, X0=q["X0"], Xf=q["Xf"], params=p)
@@ -281,7 +282,7 @@ def pmc_pass(counter, args, tuned):
     cmd = ["timeout", "-s", "KILL", "150", exe, "--pmc", counter, "-d", out, "-o", "pmc", "--output-format", "csv",
            "--", sys.executable, os.path.abspath(__file__), "--pmc-child", "--pairs3", str(args.pairs3),
            "--pmc-grid", str(tuned["grid"]), "--pmc-lds", str(tuned["lds_nodes"]),
-           "--pmc-lpl", str(tuned["leaves_per_lane"])]
+           "--pmc-lpl", str(tuned["leaves_per_lane"]), "--pmc-tile", str(tuned.get("tile_leaves", 256))]
     try:
         r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, timeout=170, cwd=ROOT,
                            env=dict(os.environ, TMPDIR="/tmp"))
@@ -351,6 +352,7 @@ def main():
     ap.add_argument("--pmc-grid", type=int, default=0, help=argparse.SUPPRESS)
     ap.add_argument("--pmc-lds", type=int, default=256, help=argparse.SUPPRESS)
     ap.add_argument("--pmc-lpl", type=int, default=1, help=argparse.SUPPRESS)
+    ap.add_argument("--pmc-tile", type=int, default=256, help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.pmc_child:
         pmc_child(args)
@@ -412,7 +414,7 @@ def main():
 
     # ---- headline: 256-pair batched loop-closure verification --------------
     head = run_headline(ctx, pairs3, args)
-    ctx.set_launch_config()  # the defaults again for the other rows
+    ctx.set_launch_config(leaves_per_lane=1, batch_streams=1, tile_leaves=256)  # the defaults for the other rows
 
     # ---- CT-ICP: served batches, device loop, plain path ---------------------
     ct = None
@@ -575,7 +577,7 @@ def run_headline(ctx, pairs3, args):
         _, d = al.correspondences()
         depth_sum += int(np.sum(d))
         n_q += len(d)
-        n_tiles += (len(d) + 255) // 256
+        n_tiles += (len(d) + tuned["tile_leaves"] - 1) // tuned["tile_leaves"]
     demand = float((24 + 4 + 24) * n_q + 32 * depth_sum + n_tiles * 4 * 27)
     # accuracy: every pair against the oracle; H/b against the FP64 twin on a sample
     op = O.from_kdh_params(p, 8)

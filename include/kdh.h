@@ -228,7 +228,7 @@ int kdh_align_noalloc(kdh_ctx* ctx, const kdh_tree* fixed, const kdh_tree* movin
                       const double X0[12], const double v0[6], const double Xf[12],
                       const kdh_params* params, const kdh_prior* prior, int stats_mode,
                       double X_out[12], double v_out[6], kdh_stats* stats_log, int* n_log, int* result);
-int kdh_align_noalloc_max_leaves(void);
+int kdh_align_noalloc_max_leaves(void);  /* at the default 256-leaf tiles (half at 128) */
 
 /* ---- many alignments in one launch -------------------------------------- */
 typedef struct kdh_pair {
@@ -421,6 +421,15 @@ int kdh_set_launch_config(kdh_ctx* ctx, int grid, int lds_nodes);
  * batch for one setting (the per-tile FP32 sums differ in order between
  * settings).  The dual form always runs 1. */
 int kdh_set_leaves_per_lane(kdh_ctx* ctx, int leaves_per_lane);
+/* Tuning knob (project autotuner): moving leaves per tile, 256 (default) or
+ * 128 (128-thread workgroups, one leaf per lane; twice the workgroups and
+ * partial rows).  No reference counterpart (the reference's CUDA reduction
+ * has its own fixed block size, cuda/reduce_impl_.h:10-78).  Decisions are
+ * the same bits at either size; H, b and the statistics differ in the FP32
+ * summation order within a tile (the per-tile sums), each size the same bits
+ * alone or in any batch.  A batch prepared at one size is refused by
+ * kdh_batch_run at the other.  128-leaf tiles take one leaf per lane. */
+int kdh_set_tile_leaves(kdh_ctx* ctx, int tile_leaves);
 /* Tuning knob (project autotuner): kdh_batch_align replays its iterations on
  * 1 stream, or on 2 -- the pairs split into two halves of about equal tile
  * count, the second half's chain started one tile launch after the first's,

@@ -272,6 +272,7 @@ _SIGS = {
     "kdh_last_kernel_time": (C.c_int, [P, C.POINTER(C.c_double), C.POINTER(C.c_int)]),
     "kdh_set_launch_config": (C.c_int, [P, C.c_int, C.c_int]),
     "kdh_set_leaves_per_lane": (C.c_int, [P, C.c_int]),
+    "kdh_set_tile_leaves": (C.c_int, [P, C.c_int]),
     "kdh_set_batch_streams": (C.c_int, [P, C.c_int]),
     "kdh_set_profiling": (C.c_int, [P, C.c_int]),
     "kdh_last_kernel_breakdown": (C.c_int, [P, C.POINTER(C.c_double), C.POINTER(C.c_double)]),

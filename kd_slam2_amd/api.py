@@ -121,10 +121,14 @@ class Context:
     def check(self, rc, what):
         L.check(rc, self.h, what)
 
-    def set_launch_config(self, grid=0, lds_nodes=256, leaves_per_lane=None, batch_streams=None):
+    def set_launch_config(self, grid=0, lds_nodes=256, leaves_per_lane=None, batch_streams=None, tile_leaves=None):
         self.check(L.lib().kdh_set_launch_config(self.h, grid, lds_nodes), "kdh_set_launch_config")
+        if tile_leaves is not None and int(tile_leaves) == 256:
+            self.check(L.lib().kdh_set_tile_leaves(self.h, 256), "kdh_set_tile_leaves")
         if leaves_per_lane is not None:
             self.check(L.lib().kdh_set_leaves_per_lane(self.h, int(leaves_per_lane)), "kdh_set_leaves_per_lane")
+        if tile_leaves is not None and int(tile_leaves) != 256:
+            self.check(L.lib().kdh_set_tile_leaves(self.h, int(tile_leaves)), "kdh_set_tile_leaves")
         if batch_streams is not None:
             self.check(L.lib().kdh_set_batch_streams(self.h, int(batch_streams)), "kdh_set_batch_streams")
 

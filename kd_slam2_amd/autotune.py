@@ -4,9 +4,11 @@ Sweeps the knobs of the tile kernel on a representative batch and keeps the
 fastest, timing each candidate with HIP events around the tile kernel
 (profiling mode):
 
-* leaves per lane -- 1 (256-thread workgroups, one descent per lane) or 2
-  (128-thread workgroups, two descents in flight per lane): the latency /
-  occupancy trade of the greedy descent (kdh_set_leaves_per_lane);
+* the tile shape -- 256 leaves with one per lane (256-thread workgroups),
+  256 leaves with two per lane (128 threads, two descents in flight per
+  lane) or 128 leaves with one per lane (128 threads, twice the workgroups):
+  the latency / occupancy trade of the greedy descent
+  (kdh_set_leaves_per_lane, kdh_set_tile_leaves);
 * the number of top-of-tree nodes staged in LDS per workgroup
   (0/256/512/1024, plus 2048 for rigid ICP);
 * the workgroup count: one per tile, or a persistent grid of k workgroups
@@ -15,37 +17,45 @@ fastest, timing each candidate with HIP events around the tile kernel
   on two (the pairs split in halves whose finish stages run beside the other
   half's tile launches), timed by whole batches (kdh_set_batch_streams).
 
-A tile is 256 moving leaves in every shape; the leaf size is a property of
-the trees, chosen when they are built (TreeGenerator3D leaf_max_points).
+The leaf size is a property of the trees, chosen when they are built
+(TreeGenerator3D leaf_max_points).
 """
 import numpy as np
 
 N_CU = 256
 LDS_CHOICES = (0, 256, 512, 1024, 2048)
-LPL_CHOICES = (1, 2)
+SHAPES = ((1, 256), (2, 256), (1, 128))  # (leaves per lane, leaves per tile)
 
 
 def tune(ctx, pairs, batch_align, ct, repeats=2, grids=(0, N_CU * 2, N_CU * 4, N_CU * 8), lds=LDS_CHOICES,
-         leaves_per_lane=LPL_CHOICES):
-    """Returns (best config dict, all results).  Leaves the best config set."""
+         shapes=SHAPES):
+    """Returns (best config dict, all results).  Leaves the best config set.
+    Each candidate is timed by one tile launch (all tiles of the batch) plus
+    one finish launch: 128-leaf tiles cover the same leaves in twice the
+    tiles, and the finish stage then sums twice the partial rows."""
     results = []
     ctx.set_profiling(True)
     try:
-        for lpl in leaves_per_lane:
+        for lpl, tile in shapes:
             for nodes in lds:
                 if ct and nodes > 1024:
                     continue
                 for grid in grids:
-                    ctx.set_launch_config(grid=grid, lds_nodes=nodes, leaves_per_lane=lpl)
-                    t = []
+                    ctx.set_launch_config(grid=grid, lds_nodes=nodes, leaves_per_lane=lpl, tile_leaves=tile)
+                    t, f = [], []
                     for _ in range(repeats):
                         batch_align(ctx, pairs)
-                        t.append(ctx.kernel_breakdown()[0])
-                    results.append(dict(leaves_per_lane=lpl, lds_nodes=nodes, grid=grid, tile_us=float(np.median(t))))
+                        tu, fu = ctx.kernel_breakdown()
+                        t.append(tu)
+                        f.append(fu)
+                    results.append(dict(leaves_per_lane=lpl, tile_leaves=tile, lds_nodes=nodes, grid=grid,
+                                        tile_us=float(np.median(t)), finish_us=float(np.median(f))))
     finally:
         ctx.set_profiling(False)
-    best = dict(min(results, key=lambda r: r["tile_us"]))
-    ctx.set_launch_config(grid=best["grid"], lds_nodes=best["lds_nodes"], leaves_per_lane=best["leaves_per_lane"])
+    # the tile shape also sets the finish stage's work (partial rows per pair)
+    best = dict(min(results, key=lambda r: r["tile_us"] + r["finish_us"]))
+    ctx.set_launch_config(grid=best["grid"], lds_nodes=best["lds_nodes"], leaves_per_lane=best["leaves_per_lane"],
+                          tile_leaves=best["tile_leaves"])
     per_iter = {}
     for streams in (1, 2):
         ctx.set_launch_config(grid=best["grid"], lds_nodes=best["lds_nodes"], batch_streams=streams)

@@ -304,7 +304,8 @@ int kdh_batch_align(kdh_ctx* ctx, int n_pairs, const kdh_pair* pairs, kdh_pair_r
     const long long key[16] = {n_pairs, g[0].n_tiles, g[1].n_tiles, iters, ctx->lds_nodes, ctx->grid_override,
                                ctx->xcd_remap, (long long)(intptr_t)ctx->d_pairs,
                                (long long)(intptr_t)ctx->d_partials, (long long)(intptr_t)ctx->d_tile_pair, 2,
-                               ctx->leaves_per_lane, two ? 2 : 1, g[2].n_tiles, g[3].n_tiles,
+                               ctx->leaves_per_lane | (ctx->tile_leaves << 8), two ? 2 : 1, g[2].n_tiles,
+                               g[3].n_tiles,
                                (long long)(intptr_t)ctx->stream2};
     hipGraphExec_t exec = kdh_graph_lookup(ctx->graph_batch, key);
     if (!exec) {
@@ -522,6 +523,7 @@ struct kdh_batch {
   kdh_ctx* ctx = nullptr;
   int n = 0;
   int iters = 0;
+  int tile_leaves = 0;  // the context's tile size the batch was planned with
   Group g[kGroups];
   bool two = false;
   PairState* d_pairs = nullptr;
@@ -677,6 +679,7 @@ int kdh_batch_prepare(kdh_ctx* ctx, int n_pairs, const kdh_pair* pairs, kdh_batc
   kdh_batch* b = new kdh_batch();
   b->ctx = ctx;
   b->n = n_pairs;
+  b->tile_leaves = ctx->tile_leaves;
   std::vector<PairState> hs((size_t)n_pairs);
   std::vector<int> leader, tp;
   size_t n_partials = 0;
@@ -737,6 +740,9 @@ int kdh_batch_run(kdh_batch* b, const double* X0, const double* v0, kdh_pair_res
   if (!results) return kdh_fail(ctx, KDH_ERR_INVALID, "bad arguments");
   std::lock_guard<std::mutex> lk(ctx->mu);
   KDH_HIP_TRY(ctx, hipSetDevice(ctx->device));
+  if (ctx->tile_leaves != b->tile_leaves)
+    return kdh_fail(ctx, KDH_ERR_INVALID,
+                    "kdh_batch_run: the context's tile size changed since kdh_batch_prepare; prepare the batch again");
   const int n = b->n;
   if (X0)
     for (int i = 0; i < n; ++i) memcpy(b->h_in + (size_t)kBatchIn * i, X0 + (size_t)12 * i, sizeof(double) * 12);
@@ -763,7 +769,7 @@ int kdh_batch_run(kdh_batch* b, const double* X0, const double* v0, kdh_pair_res
   } else {
     const bool two = b->two && ctx->batch_streams == 2;
     const long long key[8] = {ctx->lds_nodes, ctx->grid_override, ctx->xcd_remap, ctx->leaves_per_lane,
-                              two ? 2 : 1, (long long)(intptr_t)ctx->stream, 0, 0};
+                              two ? 2 : 1, (long long)(intptr_t)ctx->stream, ctx->tile_leaves, 0};
     if (b->exec && memcmp(key, b->key, sizeof(key)) != 0) {
       hipGraphExecDestroy(b->exec);
       b->exec = nullptr;

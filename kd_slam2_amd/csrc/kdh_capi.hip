@@ -147,7 +147,20 @@ int kdh_set_leaves_per_lane(kdh_ctx* ctx, int leaves_per_lane) {
   if (leaves_per_lane != 1 && leaves_per_lane != 2)
     return kdh_fail(ctx, KDH_ERR_INVALID, "leaves_per_lane must be 1 or 2");
   std::lock_guard<std::mutex> lk(ctx->mu);
+  if (leaves_per_lane == 2 && ctx->tile_leaves != 256)
+    return kdh_fail(ctx, KDH_ERR_INVALID, "two leaves per lane take 256-leaf tiles");
   ctx->leaves_per_lane = leaves_per_lane;
+  return KDH_OK;
+}
+
+int kdh_set_tile_leaves(kdh_ctx* ctx, int tile_leaves) {
+  if (!ctx) return KDH_ERR_INVALID;
+  if (tile_leaves != 256 && tile_leaves != 128)
+    return kdh_fail(ctx, KDH_ERR_INVALID, "tile_leaves must be 256 or 128");
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  if (tile_leaves == 128 && ctx->leaves_per_lane != 1)
+    return kdh_fail(ctx, KDH_ERR_INVALID, "128-leaf tiles take one leaf per lane");
+  ctx->tile_leaves = tile_leaves;
   return KDH_OK;
 }
 
@@ -400,7 +413,7 @@ static SolveParams solve_params(const kdh_params& p) {
   return s;
 }
 
-static int n_tiles_of(int n_leaves) { return (n_leaves + kTile - 1) / kTile; }
+static int n_tiles_of(int n_leaves, int tile_leaves) { return (n_leaves + tile_leaves - 1) / tile_leaves; }
 
 // Initial FP32 cache from FP64 state: the same formula the device applies
 // after every update (kdh_kernels.hip, finish_pair).
@@ -421,7 +434,7 @@ int kdh_fill_pair(kdh_ctx* ctx, PairState& ps, const kdh_tree* fixed, const kdh_
   if (stats_mode) ps.sp.max_iterations = 1;
   ps.ct = ct ? 1 : 0;
   ps.mode = mode;
-  ps.n_tiles = n_tiles_of(moving->dt.n_leaves);
+  ps.n_tiles = n_tiles_of(moving->dt.n_leaves, ctx->tile_leaves);
   ps.group_n = 1;
   ps.group_tiles = ps.n_tiles;
   ps.moving_root_mean = moving->root_mean;
@@ -453,6 +466,7 @@ int kdh_launch(kdh_ctx* ctx, PairState* d_pairs, const int* d_tile_pair, int n_t
   if (dual && lds > 512) lds = 512;
   a.lds_nodes = lds;
   a.leaves_per_lane = dual ? 1 : ctx->leaves_per_lane;
+  a.tile_leaves = ctx->tile_leaves;
   if (n_tiles <= 0) return KDH_OK;
   hipError_t e = launch_tiles(a, stream ? stream : ctx->stream);
   if (e != hipSuccess) return kdh_hip_fail(ctx, e, "tile kernel launch");
@@ -648,7 +662,7 @@ int kdh_align(kdh_ctx* ctx, const kdh_tree* fixed, const kdh_tree* moving, int c
       // the executable graph is cached while the launch shape is unchanged.
       const long long key[16] = {nt, ct, iters, ctx->lds_nodes, ctx->grid_override, ctx->xcd_remap,
                                  (long long)(intptr_t)ctx->d_pairs, (long long)(intptr_t)ctx->d_partials, 1,
-                                 ctx->leaves_per_lane, 0, 0, 0, 0, 0, 0};
+                                 ctx->leaves_per_lane, ctx->tile_leaves, 0, 0, 0, 0, 0};
       hipGraphExec_t exec = kdh_graph_lookup(ctx->graph_align, key);
       if (!exec) {
         hipGraph_t graph = nullptr;

@@ -65,6 +65,7 @@ struct kdh_ctx {
   int grid_override = 0;
   int lds_nodes = 256;  // measured: deeper staging costs more per tile than its LDS hits save
   int leaves_per_lane = 1;  // moving leaves per lane of the tile kernel: 1 or 2
+  int tile_leaves = kdh::kTile;  // moving leaves per tile: 256, or 128 (one leaf per lane)
   int batch_streams = 1;    // kdh_batch_align replay on 1 or 2 streams
   // second stream (default priority) and fork / stagger / join events of the
   // two-stream batch replay, made on first use

@@ -24,7 +24,8 @@ struct LaunchArgs {
   const int* pair_tile_first;  // optional: first tile of each pair in this launch (else PairState::tile_first)
   int* status_out;        // optional: per-pair status mirror written by the finish stage
   int dual = 0;           // 1: CT dual form of a bundle-adjustment factor (build mode, ct = 1)
-  int leaves_per_lane = 1;  // 1, or 2: two descents in flight per lane (not the dual form)
+  int leaves_per_lane = 1;  // 1, or 2: two descents in flight per lane (256-leaf tiles, not the dual form)
+  int tile_leaves = 256;    // moving leaves per tile: 256, or 128 (one leaf per lane)
 };
 
 hipError_t launch_tiles(const LaunchArgs& a, hipStream_t s);

@@ -609,15 +609,14 @@ __device__ void finish_pair(PairState& ps, const PairState& S, const float* __re
 
 // ---- the tile kernel --------------------------------------------------------
 
-// Workgroup of the tile kernel: 256 / QPL threads, each lane carrying QPL of
-// the tile's 256 moving leaves (QPL = 2: two descents in flight per lane).
-template <int QPL> constexpr int tile_threads() { return kTile / QPL; }
-
+// Workgroup of the tile kernel: TL / QPL threads, each lane carrying QPL of
+// the tile's TL moving leaves (QPL = 2: two descents in flight per lane).
 // DUAL (CT only): the dual form of quadraticTermDual, built for bundle
 // adjustment; three MFMA products per wave instead of one.
 // QPL: moving leaves per lane (1, or 2 for the rigid and CT forms).
-template <bool CT, int LDS_NODES, bool DUAL, int QPL>
-__global__ __launch_bounds__(kTile / QPL) __attribute__((amdgpu_waves_per_eu((DUAL || (CT && QPL == 2)) ? 3 : 4))) void tiles_kernel(const PairState* __restrict__ pairs,
+// TL: moving leaves per tile, 256 or 128 (one leaf per lane).
+template <bool CT, int LDS_NODES, bool DUAL, int QPL, int TL>
+__global__ __launch_bounds__(TL / QPL) __attribute__((amdgpu_waves_per_eu((DUAL || (CT && QPL == 2)) ? 3 : 4))) void tiles_kernel(const PairState* __restrict__ pairs,
                                                          const int* __restrict__ tile_pair,
                                                          int n_tiles_total,
                                                          float* __restrict__ partials,
@@ -626,8 +625,9 @@ __global__ __launch_bounds__(kTile / QPL) __attribute__((amdgpu_waves_per_eu((DU
                                                          int xcd_remap,
                                                          const int* __restrict__ pair_tile_first) {
   static_assert(CT || !DUAL, "the dual form is a CT form");
-  static_assert(QPL == 1 || (QPL == 2 && !DUAL), "two leaves per lane: rigid and CT forms");
-  constexpr int NT = kTile / QPL;      // threads
+  static_assert(QPL == 1 || (QPL == 2 && !DUAL && TL == 256), "two leaves per lane: 256-leaf rigid and CT tiles");
+  static_assert(TL == 256 || TL == 128, "tiles of 256 or 128 moving leaves");
+  constexpr int NT = TL / QPL;         // threads
   constexpr int NWV = NT / 64;         // waves
   constexpr int N = n_slots<CT, DUAL>();
   constexpr int E = CT ? 12 : 6;       // component of sqrt(g) e in w
@@ -681,7 +681,7 @@ __global__ __launch_bounds__(kTile / QPL) __attribute__((amdgpu_waves_per_eu((DU
     float4 la[QPL], lb[QPL];
 #pragma unroll
     for (int k = 0; k < QPL; ++k) {
-      q[k] = (t - tfirst) * kTile + k * NT + tid;
+      q[k] = (t - tfirst) * TL + k * NT + tid;
       live[k] = q[k] < moving.n_leaves && fixed.n_nodes > 0;
       const int qc = live[k] ? q[k] : 0;  // a tile exists, so the pair has >= 1 leaf
       la[k] = moving.leaves[2 * qc];
@@ -856,11 +856,13 @@ __global__ __launch_bounds__(kTile / QPL) __attribute__((amdgpu_waves_per_eu((DU
       if (sl < CS + 2 || (DUAL && sl >= kCtSlots)) {
         const int cell = (DUAL && sl >= kCtSlots) ? dual_slot_cell(sl) : slot_cell<CT>(sl);
         if constexpr (NWV == 4) sv = (s_w[0][cell] + s_w[1][cell]) + (s_w[2][cell] + s_w[3][cell]);
-        else sv = s_w[0][cell] + s_w[1][cell];
+        else if constexpr (NWV == 2) sv = s_w[0][cell] + s_w[1][cell];
+        else sv = s_w[0][cell];
       } else {
         const int k = sl - CS - 2;
         if constexpr (NWV == 4) sv = (float)((s_cnt[0][k] + s_cnt[1][k]) + (s_cnt[2][k] + s_cnt[3][k]));
-        else sv = (float)(s_cnt[0][k] + s_cnt[1][k]);
+        else if constexpr (NWV == 2) sv = (float)(s_cnt[0][k] + s_cnt[1][k]);
+        else sv = (float)s_cnt[0][k];
       }
       partials[(size_t)t * N + sl] = sv;
     }
@@ -894,10 +896,10 @@ __global__ __launch_bounds__(64) void finish_kernel(PairState* __restrict__ pair
 
 // ---- launch wrappers --------------------------------------------------------
 
-template <bool CT, int LDS, bool DUAL, int QPL>
+template <bool CT, int LDS, bool DUAL, int QPL, int TL = 256>
 static hipError_t launch_t(const LaunchArgs& a, hipStream_t s) {
   if (a.phases & 1) {
-    hipLaunchKernelGGL((tiles_kernel<CT, LDS, DUAL, QPL>), dim3(a.grid), dim3(kTile / QPL), 0, s, a.pairs,
+    hipLaunchKernelGGL((tiles_kernel<CT, LDS, DUAL, QPL, TL>), dim3(a.grid), dim3(TL / QPL), 0, s, a.pairs,
                        a.tile_pair, a.n_tiles_total, a.partials, a.corr_out, a.depth_out, a.xcd_remap,
                        a.pair_tile_first);
     hipError_t e = hipGetLastError();
@@ -913,7 +915,13 @@ static hipError_t launch_t(const LaunchArgs& a, hipStream_t s) {
 
 template <bool CT, int LDS>
 static hipError_t launch_q(const LaunchArgs& a, hipStream_t s) {
+  if (a.tile_leaves == 128) return launch_t<CT, LDS, false, 1, 128>(a, s);
   return a.leaves_per_lane == 2 ? launch_t<CT, LDS, false, 2>(a, s) : launch_t<CT, LDS, false, 1>(a, s);
+}
+
+template <int LDS>
+static hipError_t launch_dual(const LaunchArgs& a, hipStream_t s) {
+  return a.tile_leaves == 128 ? launch_t<true, LDS, true, 1, 128>(a, s) : launch_t<true, LDS, true, 1>(a, s);
 }
 
 hipError_t launch_tiles(const LaunchArgs& a, hipStream_t s) {
@@ -921,9 +929,9 @@ hipError_t launch_tiles(const LaunchArgs& a, hipStream_t s) {
     // the dual form caches at most 512 tree nodes in LDS: its third C tile
     // takes the room of the larger caches
     switch (a.lds_nodes) {
-      case 0: return launch_t<true, 0, true, 1>(a, s);
-      case 256: return launch_t<true, 256, true, 1>(a, s);
-      default: return launch_t<true, 512, true, 1>(a, s);
+      case 0: return launch_dual<0>(a, s);
+      case 256: return launch_dual<256>(a, s);
+      default: return launch_dual<512>(a, s);
     }
   }
   if (a.ct) {

@@ -85,7 +85,8 @@ int kdh_align_noalloc(kdh_ctx* ctx, const kdh_tree* fixed, const kdh_tree* movin
   if (nt > kNoAllocMaxTiles)
     return kdh_fail(ctx, KDH_ERR_INVALID,
                     "kdh_align_noalloc: " + std::to_string(moving->dt.n_leaves) + " moving leaves exceed the "
-                    "scratch maximum of " + std::to_string(kNoAllocMaxTiles * kTile) + " (use kdh_align)");
+                    "scratch maximum of " + std::to_string(kNoAllocMaxTiles * ctx->tile_leaves) +
+                    " at this tile size (use kdh_align)");
   memcpy(ps.X, X0, sizeof(ps.X));
   memcpy(ps.Xf, Xf, sizeof(ps.Xf));
   for (int i = 0; i < 6; ++i) ps.v[i] = (ct && v0) ? v0[i] : 0.0;

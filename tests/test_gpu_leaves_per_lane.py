@@ -1,7 +1,9 @@
-"""The tile kernel with two moving leaves per lane (two descents in flight
-per lane, kdh_set_leaves_per_lane(ctx, 2)): correspondences bit-exact
-against the oracle, whole alignments within the pose bar, and batched
-results the same alone and in a mix.  GPU."""
+"""The tile kernel's other shapes: two moving leaves per lane (two descents
+in flight per lane, kdh_set_leaves_per_lane(ctx, 2)) and 128-leaf tiles
+(kdh_set_tile_leaves(ctx, 128)): correspondences bit-exact against the
+oracle, whole alignments within the pose bar, batched results the same alone
+and in a mix, prepared batches equal to one-shot ones; the setters' limits.
+GPU."""
 import numpy as np
 import pytest
 
@@ -13,10 +15,11 @@ from .workloads import cfg1, cfg2, icp_params
 pytestmark = pytest.mark.gpu
 
 
-@pytest.fixture(scope="module")
-def ctx2():
+@pytest.fixture(scope="module", params=[(2, 256), (1, 128)], ids=["2_leaves_per_lane", "128_leaf_tiles"])
+def ctx2(request):
+    lpl, tile = request.param
     c = Context(0)
-    c.set_launch_config(grid=0, lds_nodes=256, leaves_per_lane=2)
+    c.set_launch_config(grid=0, lds_nodes=256, leaves_per_lane=lpl, tile_leaves=tile)
     yield c
     c.close()
 
@@ -64,6 +67,12 @@ def test_batch_alone_equals_mixed_two_leaves_per_lane(ctx2):
     p = icp_params()
     pairs = [dict(fixed=F, moving=M, ct=0, X0=w["X0"], Xf=w["Xf"], params=p) for w, (F, M) in zip(ws, trees)]
     mixed = batch_align(ctx2, pairs)
+    from kd_slam2_amd import PairBatch
+
+    B = PairBatch(pairs)
+    prepared = list(B.run(ctx2))
+    for a, b in zip(mixed, prepared):
+        assert np.array_equal(a["X"], b["X"]) and a["last"] == b["last"] and a["result"] == b["result"]
     for i in range(len(pairs)):
         alone = batch_align(ctx2, [pairs[i]])[0]
         assert np.array_equal(alone["X"], mixed[i]["X"]) and alone["last"] == mixed[i]["last"]
@@ -73,3 +82,34 @@ def test_batch_alone_equals_mixed_two_leaves_per_lane(ctx2):
     for F, M in trees:
         F.close()
         M.close()
+
+
+def test_tile_shape_limits():
+    """Two leaves per lane take 256-leaf tiles and 128-leaf tiles one leaf per
+    lane; other sizes are refused; a batch prepared at one tile size is
+    refused at the other."""
+    from kd_slam2_amd import PairBatch, TreeGenerator3D
+
+    c = Context(0)
+    try:
+        with pytest.raises(RuntimeError):
+            c.set_launch_config(tile_leaves=64)
+        c.set_launch_config(leaves_per_lane=2)
+        with pytest.raises(RuntimeError):
+            c.set_launch_config(tile_leaves=128)
+        c.set_launch_config(leaves_per_lane=1, tile_leaves=128)
+        with pytest.raises(RuntimeError):
+            c.set_launch_config(leaves_per_lane=2)
+        w = synth.config1(seed=3200, n_map=12000, n_scan=4000, rot_deg=5.0, trans_m=1.0, uniform_init=True)
+        g = TreeGenerator3D(leaf_max_points=16)
+        F, M = DeviceTree(c, g.makeTree(w["map"])), DeviceTree(c, g.makeTree(w["scan"]))
+        B = PairBatch([dict(fixed=F, moving=M, ct=0, X0=w["X0"], Xf=w["Xf"], params=icp_params())])
+        B.run(c)
+        c.set_launch_config(tile_leaves=256)
+        with pytest.raises(RuntimeError, match="tile size"):
+            B.run(c)
+        B.close()
+        F.close()
+        M.close()
+    finally:
+        c.close()
