@@ -101,24 +101,43 @@ __device__ __forceinline__ int slot_cell(int s) {
   return j * 16 + i;
 }
 
+// Tree records are read through explicit global (address space 1) pointers
+// and the LDS copy through explicit LDS (address space 3) pointers.  Pointers
+// loaded from the pair state are generic: through them every load would be a
+// flat load, which the backend must route between the apertures and which
+// counts against the LDS wait counter too, so each wait for a tree record
+// would also wait for every LDS access in flight.
+// Staged records travel as clang's own 4-float vector: arrays of the HIP
+// float4 class that are written under a branch stay in scratch memory.
+typedef float v4f __attribute__((ext_vector_type(4)));
+#if defined(__HIP_DEVICE_COMPILE__)
+typedef __attribute__((address_space(3))) const float4 lds_f4;
+typedef __attribute__((address_space(1))) const float4 glb_f4;
+typedef __attribute__((address_space(1))) const float glb_f;
+typedef __attribute__((address_space(1))) const v4f glb_v4;
+typedef __attribute__((address_space(3))) v4f lds_v4;
+#else
+typedef const float4 lds_f4;
+typedef const float4 glb_f4;
+typedef const float glb_f;
+typedef const v4f glb_v4;
+typedef v4f lds_v4;
+#endif
+__device__ __forceinline__ glb_f4* global_f4(const float4* p) { return (glb_f4*)p; }
+// A valid address for loads whose value is never used (a tree without nodes).
+__device__ const float4 kNoRecord[1] = {{0.0f, 0.0f, 0.0f, 0.0f}};
+__device__ __forceinline__ glb_f* global_f(const float* p) { return (glb_f*)p; }
+
 struct GlobalFetch {
-  const float4* __restrict__ nodes;
+  glb_f4* __restrict__ nodes;
   __device__ __forceinline__ void operator()(int idx, float4& a, float4& b) const {
     a = nodes[2 * idx];
     b = nodes[2 * idx + 1];
   }
 };
 
-// The LDS copy is addressed through an explicit LDS (address space 3)
-// pointer: with a generic one the backend must select between the two
-// apertures per load.
-#if defined(__HIP_DEVICE_COMPILE__)
-typedef __attribute__((address_space(3))) const float4 lds_f4;
-#else
-typedef const float4 lds_f4;
-#endif
 struct LdsFetch {
-  const float4* __restrict__ nodes;
+  glb_f4* __restrict__ nodes;
   lds_f4* lds;
   int n_lds;
   __device__ __forceinline__ void operator()(int idx, float4& a, float4& b) const {
@@ -155,15 +174,18 @@ __device__ __forceinline__ double rcp_nr(double d) {
 }
 
 // Damped solve (A + diag(damp)) x = -b on one wave: lane i (< D) holds row i
-// of the symmetric matrix in a[] and b_i in bi; right-looking LDL^T without
-// pivoting (the damped normal matrix is positive definite), then forward and
-// backward substitution with lane broadcasts.  Lanes >= D carry zeros.  The
-// caller's workgroup is this one wave.  The result is the same on the wave
-// alone or in any batch; it differs from the host solve of kdh_solve.h in
-// the last bits (fused multiply-adds here).
+// of the symmetric matrix in a[] and b_i in bi; right-looking elimination
+// without pivoting (the damped normal matrix is positive definite) with the
+// forward substitution carried along, then backward substitution, with lane
+// broadcasts.  Lanes >= D carry zeros.  After the elimination lane i holds
+// row i of U = D L^T in a[j], j > i; scaled by its own pivot reciprocal that
+// is row i of L^T, so the backward steps need no transpose.  The caller's
+// workgroup is this one wave.  The result is the same on the wave alone or
+// in any batch; it differs from the host solve of kdh_solve.h in the last
+// bits (fused multiply-adds, order of the back-substitution products).
 template <int D>
 __device__ void wave_ldlt_solve(double (&a)[D], double bi, double damp_pose, double damp_vel, int lane,
-                                double* sx, double* sL) {
+                                double* sx) {
   // FP64 only, and no decision rests on its last bits: multiply-adds may fuse
   // here (one dependent operation per update instead of two), unlike the
   // FP32 code whose rounding the decisions follow.
@@ -174,7 +196,7 @@ __device__ void wave_ldlt_solve(double (&a)[D], double bi, double damp_pose, dou
 #pragma unroll
   for (int j = 0; j < D; ++j) a[j] = a[j] + ((row && j == lane) ? ((j < 6) ? damp_pose : damp_vel) : 0.0);
   double r = row ? -bi : 0.0;
-  double dd = 0.0;  // this lane's pivot D_i
+  double my_inv = 0.0;  // this lane's pivot reciprocal 1 / D_i
 #pragma unroll
   for (int k = 0; k < D; ++k) {
     double akk = 0.0;
@@ -182,38 +204,28 @@ __device__ void wave_ldlt_solve(double (&a)[D], double bi, double damp_pose, dou
     for (int j = 0; j < D; ++j)
       if (j == k) akk = a[j];
     const double dk = bcast(akk, k);
-    if (lane == k) dd = dk;
+    const double rk = bcast(r, k);  // y_k: every update of row k is in
     const double inv = (dk != 0.0) ? rcp_nr(dk) : 0.0;
+    if (lane == k) my_inv = inv;
     const double lik = (lane > k && row) ? a[k] * inv : 0.0;
-    // A_ij -= L_ik * A_kj for j > k (row k broadcast from lane k)
+    // A_ij -= L_ik * A_kj for j > k (row k broadcast from lane k), and the
+    // same for the right-hand side
 #pragma unroll
     for (int j = k + 1; j < D; ++j) {
       const double akj = bcast(a[j], k);
       a[j] -= lik * akj;
     }
-    if (lane > k && row) a[k] = lik;  // store L_ik in the strictly lower part
+    r -= lik * rk;
   }
-  // forward: L y = r
+  // z = D^-1 y, and row i of L^T = row i of U / D_i (lanes >= D: zeros)
+  r *= my_inv;
 #pragma unroll
-  for (int k = 0; k < D; ++k) {
-    const double yk = bcast(r, k);
-    if (lane > k && row) r -= a[k] * yk;
-  }
-  r = (row && dd != 0.0) ? r / dd : 0.0;
-  // backward: L^T x = z.  Lane i needs column i of L (L_ki = a[i] of lane k,
-  // k > i): one transpose through LDS, then one broadcast per step.
-  if (row) {
-#pragma unroll
-    for (int j = 0; j < D; ++j) sL[lane * D + j] = a[j];
-  }
-  __syncthreads();
-  double lt[D];
-#pragma unroll
-  for (int k = 0; k < D; ++k) lt[k] = (row && k > lane) ? sL[k * D + lane] : 0.0;
+  for (int j = 0; j < D; ++j) a[j] = j > lane ? a[j] * my_inv : 0.0;
+  // backward: L^T x = z
 #pragma unroll
   for (int k = D - 1; k >= 0; --k) {
     const double xk = bcast(r, k);
-    if (lane < k) r -= lt[k] * xk;
+    if (lane < k) r -= a[k] * xk;
   }
   if (row) sx[lane] = r;
 }
@@ -228,8 +240,8 @@ struct FinishLds {
   static constexpr int kG = kRed + 8 * ((N + 1) & ~1);    // double[3*D]: rows 0..2 of G
   static constexpr int kP = kG + 8 * 3 * D;               // double[36 + 6]: prior H, b
   static constexpr int kX = kP + 8 * 42;                  // double[D]
-  static constexpr int kL = kX + 8 * D;                   // double[max(D*D, 40)]: L, then update scratch
-  static constexpr int kBytes = kL + 8 * (D * D > 40 ? D * D : 40);
+  static constexpr int kL = kX + 8 * D;                   // double[40]: update scratch
+  static constexpr int kBytes = kL + 8 * 40;
 };
 
 // Row r (< D) of H_c from the reduced slots, for a lane-dependent r and every
@@ -471,7 +483,7 @@ __device__ void finish_pair(PairState& ps, const PairState& S, const float* __re
     for (int c = 0; c < D; ++c) h[c] = 0.0;
     bv = 0.0;
   }
-  wave_ldlt_solve<D>(h, bv, sp.damping, sp.velocity_damping, lane, sx, sL);
+  wave_ldlt_solve<D>(h, bv, sp.damping, sp.velocity_damping, lane, sx);
   __syncthreads();
 
   // 6. Update, termination and the next FP32 cache (updateCache,
@@ -490,7 +502,7 @@ __device__ void finish_pair(PairState& ps, const PairState& S, const float* __re
 #pragma unroll
     for (int i = 6; i < D; ++i) vn += dx[i] * dx[i];
   }
-  double* sE = sL;        // exp(dx), 3x4 (sL is free after the solve)
+  double* sE = sL;        // exp(dx), 3x4
   double* sXc = sL + 12;  // new pose, 3x4
   double* sC = sL + 24;   // anchor, world frame
   if (tid == 0 && !stats_mode) iso_to_34(expmap_d(dx), sE);
@@ -666,15 +678,35 @@ __global__ __launch_bounds__(TL / QPL) __attribute__((amdgpu_waves_per_eu((DUAL 
   const int c_lo = lo_all + (int)(((long long)span * j) / n_wg);
   const int c_hi = lo_all + (int)(((long long)span * (j + 1)) / n_wg);
   for (int t = c_lo; t < c_hi; ++t) {
-    // Three memory round trips before the descent: the tile's pair, the pair
-    // fields, then the leaf records together with the top-of-tree staging.
+    // Three memory round trips before the descent: the tile's pair, the
+    // pair's tree fields, then the leaf records together with the
+    // top-of-tree staging, the status, the cache and the term parameters.
+    // A finished pair is skipped only after these loads are issued: a skip
+    // test right after a status load lets the compiler sink every later load
+    // below it, which costs a round trip of its own.
     const int p = tile_pair ? tile_pair[t] : 0;
     const PairState& ps = pairs[p];
-    const int status = ps.status;
     const DevTree fixed = ps.fixed;
     const DevTree moving = ps.moving;
     const int tfirst = pair_tile_first ? pair_tile_first[p] : ps.tile_first;
-    if (status != 0) continue;
+    glb_f4* nodes = global_f4(fixed.nodes);
+    glb_f4* leaves = global_f4(moving.leaves);
+    // top-of-tree staging loads: all in flight at once, from clamped
+    // addresses (a tree without nodes stages nothing and reads a dummy
+    // record), stored after the barrier below.  They are issued for every
+    // tile (a later tile of the same pair on a persistent grid re-reads
+    // cached lines and stores nothing): loads under a branch would leave the
+    // array in scratch memory.
+    const bool stage = LDS_NODES > 0 && p != cached_pair;
+    constexpr int kStage = LDS_NODES > 0 ? (2 * LDS_NODES + NT - 1) / NT : 1;
+    const int n_st = fixed.n_nodes < LDS_NODES ? fixed.n_nodes : LDS_NODES;
+    glb_v4* st_src = (glb_v4*)(n_st > 0 ? nodes : global_f4(kNoRecord));
+    const int st_last = n_st > 0 ? 2 * n_st - 1 : 0;
+    v4f st[kStage];
+    if constexpr (LDS_NODES > 0) {
+#pragma unroll
+      for (int k = 0; k < kStage; ++k) st[k] = st_src[min(tid + k * NT, st_last)];
+    }
     // lane `tid` carries leaves q[k] = tile base + k * NT + tid
     int q[QPL];
     bool live[QPL];
@@ -684,20 +716,22 @@ __global__ __launch_bounds__(TL / QPL) __attribute__((amdgpu_waves_per_eu((DUAL 
       q[k] = (t - tfirst) * TL + k * NT + tid;
       live[k] = q[k] < moving.n_leaves && fixed.n_nodes > 0;
       const int qc = live[k] ? q[k] : 0;  // a tile exists, so the pair has >= 1 leaf
-      la[k] = moving.leaves[2 * qc];
-      lb[k] = moving.leaves[2 * qc + 1];
+      la[k] = leaves[2 * qc];
+      lb[k] = leaves[2 * qc + 1];
     }
-    // the cache and term parameters in the same round trip as the leaves
-    // and the staging, ahead of the barrier
+    const int status = ps.status;
     const TermCache C = ps.cache;
     const TermParams TP = ps.tp;
-    if (LDS_NODES > 0 && p != cached_pair) {
+    if (stage) {
       __syncthreads();
-      n_lds = fixed.n_nodes < LDS_NODES ? fixed.n_nodes : LDS_NODES;
-      for (int i = tid; i < 2 * n_lds; i += NT) s_nodes[i] = fixed.nodes[i];
+#pragma unroll
+      for (int k = 0; k < kStage; ++k)
+        if (tid + k * NT < 2 * n_st) ((lds_v4*)s_nodes)[tid + k * NT] = st[k];
+      n_lds = n_st;
       cached_pair = p;
       __syncthreads();
     }
+    if (status != 0) continue;
 
     TermRec R[QPL];
     float Jb[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
@@ -709,7 +743,7 @@ __global__ __launch_bounds__(TL / QPL) __attribute__((amdgpu_waves_per_eu((DUAL 
       L[k].dir = v3(la[k].w, lb[k].x, lb[k].y);
       L[k].dts = lb[k].z;
     }
-    const float* fdts = fixed.node_dts;
+    glb_f* fdts = global_f(fixed.node_dts);
     auto dts_of = [fdts](int i) { return fdts[i]; };
     auto leaf_terms = [&](const auto& f) {
       if constexpr (QPL == 1) {
@@ -745,9 +779,9 @@ __global__ __launch_bounds__(TL / QPL) __attribute__((amdgpu_waves_per_eu((DUAL 
       }
     };
     if (LDS_NODES > 0) {
-      leaf_terms(LdsFetch{fixed.nodes, (lds_f4*)s_nodes, n_lds});
+      leaf_terms(LdsFetch{nodes, (lds_f4*)s_nodes, n_lds});
     } else {
-      leaf_terms(GlobalFetch{fixed.nodes});
+      leaf_terms(GlobalFetch{nodes});
     }
 
     // Normal equations of the tile on the matrix cores.  With
@@ -878,15 +912,25 @@ __global__ __launch_bounds__(64) void finish_kernel(PairState* __restrict__ pair
                                                           const int* __restrict__ pair_tile_first,
                                                           int* __restrict__ status_out) {
   __shared__ __align__(16) char s_area[FinishLds<CT>::kBytes];
-  __shared__ __align__(16) float4 s_state[kStateVec4];
+  constexpr int kLoads = (kStateVec4 + 63) / 64;
+  __shared__ __align__(16) float4 s_state[64 * kLoads];
+  // one workgroup per alignment: the grid is n_pairs wide (no range test,
+  // which would split the argument loads into two round trips)
+  (void)n_pairs;
   const int p = pair_first + blockIdx.x;
-  if (blockIdx.x >= n_pairs) return;
   PairState& ps = pairs[p];
-  // One memory round trip for the whole state: 16 bytes per lane into LDS,
-  // issued together with the tile-range load.
+  // One memory round trip for the whole state: every 16-byte load in flight
+  // at once, issued together with the tile-range load.  Addresses are
+  // clamped and the stores unconditional (the LDS copy is padded to whole
+  // waves): a loop, or stores under a condition, would let each load wait
+  // for the one before.
   const int tf = pair_tile_first ? pair_tile_first[p] : -1;
-  const float4* src = reinterpret_cast<const float4*>(&ps);
-  for (int i = threadIdx.x; i < kStateVec4; i += 64) s_state[i] = src[i];
+  const v4f* src = reinterpret_cast<const v4f*>(&ps);
+  v4f sv[kLoads];
+#pragma unroll
+  for (int k = 0; k < kLoads; ++k) sv[k] = src[min((int)threadIdx.x + 64 * k, kStateVec4 - 1)];
+#pragma unroll
+  for (int k = 0; k < kLoads; ++k) reinterpret_cast<v4f*>(s_state)[threadIdx.x + 64 * k] = sv[k];
   __syncthreads();
   const PairState& S = *reinterpret_cast<const PairState*>(s_state);
   // forest entries other than the leader are finished by their leader
