@@ -52,11 +52,12 @@ struct Group {
 // mode 1: build the form once.  dual: the CT pairs build the dual form, with
 // the fixed velocities vf (n_pairs x 6).  halves = 2: the pairs are split
 // into two ranges of about equal tile count (group ct + 2 * half), for the
-// two-stream replay.  tp: the tile list (pair of each tile, groups one after
-// another); n_partials: floats of the partial rows.
+// two-stream replay.  tp: the tile list (one record per tile: its pair, the
+// pair's trees and the tile's first leaf; groups one after another);
+// n_partials: floats of the partial rows.
 static int batch_plan(kdh_ctx* ctx, int n_pairs, const kdh_pair* pairs, int mode, int stats_mode, Group* g,
                       std::vector<int>& leader, int& max_iter, int dual, const double* vf, int halves,
-                      PairState* hs, std::vector<int>& tp, size_t& n_partials) {
+                      PairState* hs, std::vector<TileDesc>& tp, size_t& n_partials) {
   int rc = KDH_OK;
   for (int k = 0; k < kGroups; ++k) {
     g[k] = Group();
@@ -141,11 +142,11 @@ static int batch_plan(kdh_ctx* ctx, int n_pairs, const kdh_pair* pairs, int mode
     n_partials += (size_t)g[k].n_tiles * slots(g[k].ct, g[k].ct ? dual : 0);
     n_tiles_all += (size_t)g[k].n_tiles;
   }
-  tp.assign(std::max<size_t>(n_tiles_all, 1), 0);
+  tp.assign(std::max<size_t>(n_tiles_all, 1), TileDesc{});
   for (int i = 0; i < n_pairs; ++i) {
     const PairState& ps = hs[i];
     const Group& G = g[ps.ct + (i >= split ? 2 : 0)];
-    for (int t = 0; t < ps.n_tiles; ++t) tp[G.tile_off + ps.tile_first + t] = i;
+    for (int t = 0; t < ps.n_tiles; ++t) tp[G.tile_off + ps.tile_first + t] = make_tile_desc(ps, i, t, ctx->tile_leaves);
   }
   return KDH_OK;
 }
@@ -158,16 +159,16 @@ static int batch_setup(kdh_ctx* ctx, int n_pairs, const kdh_pair* pairs, int mod
                        int halves = 1) {
   int rc = kdh_ensure_pairs(ctx, (size_t)n_pairs);
   if (rc) return rc;
-  std::vector<int> tp;
+  std::vector<TileDesc> tp;
   size_t n_partials = 0;
   rc = batch_plan(ctx, n_pairs, pairs, mode, stats_mode, g, leader, max_iter, dual, vf, halves, ctx->h_pairs, tp,
                   n_partials);
   if (rc) return rc;
   rc = kdh_ensure_partials(ctx, std::max<size_t>(n_partials, 1));
   if (rc) return rc;
-  rc = kdh_ensure_tile_pair(ctx, tp.size());
+  rc = kdh_ensure_tiles(ctx, tp.size());
   if (rc) return rc;
-  rc = kdh_upload_tile_pair(ctx, tp);
+  rc = kdh_upload_tiles(ctx, tp);
   if (rc) return rc;
   // Only the inputs and the evolving state go up (offsetof(stats_log) of each
   // ~6 KB record): the statistics log, H, b and the dual blocks are outputs
@@ -180,7 +181,7 @@ static int batch_setup(kdh_ctx* ctx, int n_pairs, const kdh_pair* pairs, int mod
 // The device buffers of a batch: pair states, tile list, partial rows.
 struct BatchBufs {
   PairState* pairs;
-  const int* tile_pair;
+  const TileDesc* tiles;
   float* partials;
 };
 
@@ -189,7 +190,7 @@ struct BatchBufs {
 static int launch_half(kdh_ctx* ctx, const BatchBufs& B, const Group* g, int h, int phases, hipStream_t st) {
   for (int k = 2 * h; k < 2 * h + 2; ++k) {
     if (g[k].n_tiles == 0) continue;
-    int rc = kdh_launch(ctx, B.pairs, B.tile_pair + g[k].tile_off, g[k].n_tiles, g[k].ct, nullptr, nullptr,
+    int rc = kdh_launch(ctx, B.pairs, B.tiles + g[k].tile_off, g[k].n_tiles, g[k].ct, nullptr, nullptr,
                         B.partials + g[k].partial_off, g[k].p_hi - g[k].p_lo, phases, 0, g[k].p_lo, st);
     if (rc != KDH_OK) return rc;
   }
@@ -294,7 +295,7 @@ int kdh_batch_align(kdh_ctx* ctx, int n_pairs, const kdh_pair* pairs, kdh_pair_r
     if (rc) return rc;
   }
   const int iters = std::min(max_iter, kMaxIter);
-  const BatchBufs B{ctx->d_pairs, ctx->d_tile_pair, ctx->d_partials};
+  const BatchBufs B{ctx->d_pairs, ctx->d_tiles, ctx->d_partials};
   hipError_t le = hipSuccess;
   if (ctx->profile) {
     rc = enqueue_profiled(ctx, B, g, iters);
@@ -303,7 +304,7 @@ int kdh_batch_align(kdh_ctx* ctx, int n_pairs, const kdh_pair* pairs, kdh_pair_r
     // all iterations in one graph, launched once
     const long long key[16] = {n_pairs, g[0].n_tiles, g[1].n_tiles, iters, ctx->lds_nodes, ctx->grid_override,
                                ctx->xcd_remap, (long long)(intptr_t)ctx->d_pairs,
-                               (long long)(intptr_t)ctx->d_partials, (long long)(intptr_t)ctx->d_tile_pair, 2,
+                               (long long)(intptr_t)ctx->d_partials, (long long)(intptr_t)ctx->d_tiles, 2,
                                ctx->leaves_per_lane | (ctx->tile_leaves << 8), two ? 2 : 1, g[2].n_tiles,
                                g[3].n_tiles,
                                (long long)(intptr_t)ctx->stream2};
@@ -404,7 +405,7 @@ int kdh_batch_build(kdh_ctx* ctx, int n_pairs, const kdh_pair* pairs, int stats_
   if (rc) return rc;
   for (int k = 0; k < 2; ++k) {
     if (g[k].n_tiles == 0) continue;
-    rc = kdh_launch(ctx, ctx->d_pairs, ctx->d_tile_pair + g[k].tile_off, g[k].n_tiles, g[k].ct, nullptr, nullptr,
+    rc = kdh_launch(ctx, ctx->d_pairs, ctx->d_tiles + g[k].tile_off, g[k].n_tiles, g[k].ct, nullptr, nullptr,
                     ctx->d_partials + g[k].partial_off, n_pairs, 3);
     if (rc) return rc;
   }
@@ -460,7 +461,7 @@ int kdh_batch_build_dual(kdh_ctx* ctx, int n_factors, const kdh_dual_pair* facto
   int rc = batch_setup(ctx, n_factors, pairs.data(), 1, stats_mode, g, leader, max_iter, 1, vf.data());
   if (rc) return rc;
   if (g[1].n_tiles > 0) {
-    rc = kdh_launch(ctx, ctx->d_pairs, ctx->d_tile_pair + g[1].tile_off, g[1].n_tiles, 1, nullptr, nullptr,
+    rc = kdh_launch(ctx, ctx->d_pairs, ctx->d_tiles + g[1].tile_off, g[1].n_tiles, 1, nullptr, nullptr,
                     ctx->d_partials + g[1].partial_off, n_factors, 3, 1);
     if (rc) return rc;
   }
@@ -528,7 +529,7 @@ struct kdh_batch {
   bool two = false;
   PairState* d_pairs = nullptr;
   float* d_partials = nullptr;
-  int* d_tile_pair = nullptr;
+  TileDesc* d_tiles = nullptr;
   BatchMeta* d_meta = nullptr;
   double* d_in = nullptr;   // the run inputs on the device (copied from h_in before each run)
   std::vector<BatchMeta> meta;
@@ -662,7 +663,7 @@ static void batch_free(kdh_batch* b) {
   if (b->exec) hipGraphExecDestroy(b->exec);
   if (b->d_pairs) hipFree(b->d_pairs);
   if (b->d_partials) hipFree(b->d_partials);
-  if (b->d_tile_pair) hipFree(b->d_tile_pair);
+  if (b->d_tiles) hipFree(b->d_tiles);
   if (b->d_meta) hipFree(b->d_meta);
   if (b->d_in) hipFree(b->d_in);
   if (b->h_in) hipHostFree(b->h_in);
@@ -681,7 +682,8 @@ int kdh_batch_prepare(kdh_ctx* ctx, int n_pairs, const kdh_pair* pairs, kdh_batc
   b->n = n_pairs;
   b->tile_leaves = ctx->tile_leaves;
   std::vector<PairState> hs((size_t)n_pairs);
-  std::vector<int> leader, tp;
+  std::vector<int> leader;
+  std::vector<TileDesc> tp;
   size_t n_partials = 0;
   int max_iter = 0;
   int rc = batch_plan(ctx, n_pairs, pairs, 0, 0, b->g, leader, max_iter, 0, nullptr, ctx->batch_streams, hs.data(),
@@ -705,7 +707,7 @@ int kdh_batch_prepare(kdh_ctx* ctx, int n_pairs, const kdh_pair* pairs, kdh_batc
   }
   hipError_t e = hipMalloc((void**)&b->d_pairs, sizeof(PairState) * (size_t)n_pairs);
   if (e == hipSuccess) e = hipMalloc((void**)&b->d_partials, sizeof(float) * std::max<size_t>(n_partials, 1));
-  if (e == hipSuccess) e = hipMalloc((void**)&b->d_tile_pair, sizeof(int) * tp.size());
+  if (e == hipSuccess) e = hipMalloc((void**)&b->d_tiles, sizeof(TileDesc) * tp.size());
   if (e == hipSuccess) e = hipMalloc((void**)&b->d_meta, sizeof(BatchMeta) * (size_t)n_pairs);
   // coherent: the device reads the inputs and writes the results in place
   if (e == hipSuccess)
@@ -717,7 +719,7 @@ int kdh_batch_prepare(kdh_ctx* ctx, int n_pairs, const kdh_pair* pairs, kdh_batc
   if (e == hipSuccess)
     e = hipMemcpyAsync(b->d_pairs, hs.data(), sizeof(PairState) * (size_t)n_pairs, hipMemcpyHostToDevice, ctx->stream);
   if (e == hipSuccess)
-    e = hipMemcpyAsync(b->d_tile_pair, tp.data(), sizeof(int) * tp.size(), hipMemcpyHostToDevice, ctx->stream);
+    e = hipMemcpyAsync(b->d_tiles, tp.data(), sizeof(TileDesc) * tp.size(), hipMemcpyHostToDevice, ctx->stream);
   if (e == hipSuccess)
     e = hipMemcpyAsync(b->d_meta, b->meta.data(), sizeof(BatchMeta) * (size_t)n_pairs, hipMemcpyHostToDevice,
                        ctx->stream);
@@ -752,7 +754,7 @@ int kdh_batch_run(kdh_batch* b, const double* X0, const double* v0, kdh_pair_res
   // pinned memory across PCIe took longer than the copy)
   KDH_HIP_TRY(ctx, hipMemcpyAsync(b->d_in, b->h_in, sizeof(double) * kBatchIn * (size_t)n, hipMemcpyHostToDevice,
                                   ctx->stream));
-  const BatchBufs B{b->d_pairs, b->d_tile_pair, b->d_partials};
+  const BatchBufs B{b->d_pairs, b->d_tiles, b->d_partials};
   const dim3 init_grid((n + kInitPairs - 1) / kInitPairs);
   int rc = KDH_OK;
   if (ctx->profile) {

@@ -97,14 +97,14 @@ int kdh_ctx_destroy(kdh_ctx* ctx) {
   if (ctx->graph_batch.exec) hipGraphExecDestroy(ctx->graph_batch.exec);
   if (ctx->d_pairs) hipFree(ctx->d_pairs);
   if (ctx->d_partials) hipFree(ctx->d_partials);
-  if (ctx->d_tile_pair) hipFree(ctx->d_tile_pair);
+  if (ctx->d_tiles) hipFree(ctx->d_tiles);
   if (ctx->d_corr) hipFree(ctx->d_corr);
   if (ctx->d_depth) hipFree(ctx->d_depth);
   if (ctx->d_extract) hipFree(ctx->d_extract);
   if (ctx->h_root) hipHostFree(ctx->h_root);
   kdh_tree_build_ws_free(ctx);
   if (ctx->h_pairs) hipHostFree(ctx->h_pairs);
-  if (ctx->h_tile_pair) hipHostFree(ctx->h_tile_pair);
+  if (ctx->h_tiles) hipHostFree(ctx->h_tiles);
   for (int i = 0; i < kdh_ctx::kEvents; ++i)
     if (ctx->ev[i]) hipEventDestroy(ctx->ev[i]);
   for (hipEvent_t e : {ctx->ev_fork, ctx->ev_stag, ctx->ev_join})
@@ -362,29 +362,33 @@ int kdh_ensure_partials(kdh_ctx* ctx, size_t n_floats) {
   return KDH_OK;
 }
 
-int kdh_ensure_tile_pair(kdh_ctx* ctx, size_t n) {
-  int* before = ctx->d_tile_pair;
-  KDH_HIP_TRY(ctx, grow(&ctx->d_tile_pair, &ctx->tile_cap, n));
-  if (ctx->d_tile_pair != before) ctx->tile_pair_on_device.clear();
+int kdh_ensure_tiles(kdh_ctx* ctx, size_t n) {
+  TileDesc* before = ctx->d_tiles;
+  KDH_HIP_TRY(ctx, grow(&ctx->d_tiles, &ctx->tile_cap, n));
+  if (ctx->d_tiles != before) ctx->tiles_on_device.clear();
   return KDH_OK;
 }
 
-int kdh_upload_tile_pair(kdh_ctx* ctx, const std::vector<int>& tp) {
-  if (tp == ctx->tile_pair_on_device) return KDH_OK;
-  if (tp.size() > ctx->h_tile_cap) {
-    if (ctx->h_tile_pair) hipHostFree(ctx->h_tile_pair);
-    ctx->h_tile_pair = nullptr;
+bool kdh_same_tiles(const std::vector<TileDesc>& a, const std::vector<TileDesc>& b) {
+  return a.size() == b.size() && (a.empty() || memcmp(a.data(), b.data(), a.size() * sizeof(TileDesc)) == 0);
+}
+
+int kdh_upload_tiles(kdh_ctx* ctx, const std::vector<TileDesc>& tiles) {
+  if (kdh_same_tiles(tiles, ctx->tiles_on_device)) return KDH_OK;
+  if (tiles.size() > ctx->h_tile_cap) {
+    if (ctx->h_tiles) hipHostFree(ctx->h_tiles);
+    ctx->h_tiles = nullptr;
     ctx->h_tile_cap = 0;
-    const size_t nc = std::max(tp.size(), (size_t)4096);
-    KDH_HIP_TRY(ctx, hipHostMalloc((void**)&ctx->h_tile_pair, nc * sizeof(int), hipHostMallocDefault));
+    const size_t nc = std::max(tiles.size(), (size_t)4096);
+    KDH_HIP_TRY(ctx, hipHostMalloc((void**)&ctx->h_tiles, nc * sizeof(TileDesc), hipHostMallocDefault));
     ctx->h_tile_cap = nc;
   }
   // the staging buffer may still feed an earlier copy on the stream
   KDH_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-  memcpy(ctx->h_tile_pair, tp.data(), tp.size() * sizeof(int));
-  KDH_HIP_TRY(ctx, hipMemcpyAsync(ctx->d_tile_pair, ctx->h_tile_pair, tp.size() * sizeof(int), hipMemcpyHostToDevice,
+  memcpy(ctx->h_tiles, tiles.data(), tiles.size() * sizeof(TileDesc));
+  KDH_HIP_TRY(ctx, hipMemcpyAsync(ctx->d_tiles, ctx->h_tiles, tiles.size() * sizeof(TileDesc), hipMemcpyHostToDevice,
                                   ctx->stream));
-  ctx->tile_pair_on_device = tp;
+  ctx->tiles_on_device = tiles;
   return KDH_OK;
 }
 
@@ -443,7 +447,7 @@ int kdh_fill_pair(kdh_ctx* ctx, PairState& ps, const kdh_tree* fixed, const kdh_
   return KDH_OK;
 }
 
-int kdh_launch(kdh_ctx* ctx, PairState* d_pairs, const int* d_tile_pair, int n_tiles, int ct, int* corr,
+int kdh_launch(kdh_ctx* ctx, PairState* d_pairs, const TileDesc* d_tiles, int n_tiles, int ct, int* corr,
                int* depth, float* partials, int n_pairs, int phases, int dual, int pair_first, hipStream_t stream) {
   LaunchArgs a;
   a.dual = dual;
@@ -453,7 +457,7 @@ int kdh_launch(kdh_ctx* ctx, PairState* d_pairs, const int* d_tile_pair, int n_t
   a.pair_tile_first = nullptr;
   a.status_out = nullptr;
   a.pairs = d_pairs;
-  a.tile_pair = d_tile_pair;
+  a.tiles = d_tiles;
   a.n_tiles_total = n_tiles;
   a.partials = partials ? partials : ctx->d_partials;
   a.xcd_remap = ctx->xcd_remap;

@@ -39,13 +39,13 @@ struct kdh_ctx {
   size_t h_pairs_cap = 0;
   float* d_partials = nullptr;
   size_t partials_cap = 0;
-  int* d_tile_pair = nullptr;
+  kdh::TileDesc* d_tiles = nullptr;
   size_t tile_cap = 0;
   // pinned staging of the tile list, and the list the device holds now (an
-  // unchanged batch shape skips the upload)
-  int* h_tile_pair = nullptr;
+  // unchanged batch skips the upload)
+  kdh::TileDesc* h_tiles = nullptr;
   size_t h_tile_cap = 0;
-  std::vector<int> tile_pair_on_device;
+  std::vector<kdh::TileDesc> tiles_on_device;
   int* d_corr = nullptr;
   size_t corr_cap = 0;
   int* d_depth = nullptr;
@@ -94,15 +94,17 @@ int kdh_ensure_partials(kdh_ctx* ctx, size_t n_floats);
 int kdh_pull_results(kdh_ctx* ctx, int n_pairs);
 // H, b (and the dual blocks when dual), stats_log[0] of build mode, the same way.
 int kdh_pull_forms(kdh_ctx* ctx, int n_pairs, int dual);
-int kdh_ensure_tile_pair(kdh_ctx* ctx, size_t n);
+int kdh_ensure_tiles(kdh_ctx* ctx, size_t n);
 // Uploads the tile list (through pinned staging) unless the device already
 // holds exactly this list.
-int kdh_upload_tile_pair(kdh_ctx* ctx, const std::vector<int>& tp);
+int kdh_upload_tiles(kdh_ctx* ctx, const std::vector<kdh::TileDesc>& tiles);
+// Two tile lists hold the same records.
+bool kdh_same_tiles(const std::vector<kdh::TileDesc>& a, const std::vector<kdh::TileDesc>& b);
 int kdh_fill_pair(kdh_ctx* ctx, kdh::PairState& ps, const kdh_tree* fixed, const kdh_tree* moving, int ct,
                   const kdh_params& p, int stats_mode, int mode);
 void kdh_cache_from_state(kdh::PairState& ps);
 int kdh_setup_prior(kdh::PairState& ps, const kdh_prior* prior);
-int kdh_launch(kdh_ctx* ctx, kdh::PairState* d_pairs, const int* d_tile_pair, int n_tiles, int ct, int* corr,
+int kdh_launch(kdh_ctx* ctx, kdh::PairState* d_pairs, const kdh::TileDesc* d_tiles, int n_tiles, int ct, int* corr,
                int* depth, float* partials, int n_pairs, int phases = 3, int dual = 0, int pair_first = 0,
                hipStream_t stream = nullptr);
 // compute()'s return value from a finished state: -i after a failed

@@ -99,6 +99,33 @@ struct PairState {
 // statistics log); the rest is written by the device before it is read.
 constexpr size_t kStateBytes = (offsetof(PairState, stats_log) + 15) & ~(size_t)15;
 
+// One tile of a launch: its alignment and what the tile kernel needs of that
+// alignment before it can load the tile's leaves and stage the top of the
+// fixed tree, in one 32-byte record (one scalar load on the device; from the
+// pair state they would take a second round trip after the tile's pair).
+struct alignas(32) TileDesc {
+  const float4* nodes;   // fixed tree records
+  const float4* leaves;  // moving tree leaves
+  int n_nodes;
+  int n_leaves;
+  int pair;              // index of the pair state
+  int leaf0;             // first moving leaf of the tile
+};
+static_assert(sizeof(TileDesc) == 32, "a tile record is one 32-byte load");
+
+// Tile `tile` (0-based within its alignment) of pair state `pair`, with
+// `tile_leaves` moving leaves per tile.
+KDH_HD TileDesc make_tile_desc(const PairState& ps, int pair, int tile, int tile_leaves) {
+  TileDesc d;
+  d.nodes = ps.fixed.nodes;
+  d.leaves = ps.moving.leaves;
+  d.n_nodes = ps.fixed.n_nodes;
+  d.n_leaves = ps.moving.n_leaves;
+  d.pair = pair;
+  d.leaf0 = tile * tile_leaves;
+  return d;
+}
+
 // Re-centring anchor of a state (the TermCache fields mroot and cf, and the
 // FP64 anchor the solver maps the form back with): c = Xm * mroot in FP64,
 // cf = Xf^-1 * c rounded once.  All zero when re-centring is off.

@@ -9,7 +9,7 @@ namespace kdh {
 
 struct LaunchArgs {
   PairState* pairs;       // device array of alignments
-  const int* tile_pair;   // device array: tile -> pair (nullptr: all tiles belong to pair 0)
+  const TileDesc* tiles;  // device array, one record per tile (nullptr: all tiles belong to pair 0)
   int n_tiles_total;
   float* partials;        // n_tiles_total x slots(ct, dual) floats
   int* corr_out;          // optional, single-pair launches: fixed leaf per moving leaf
@@ -21,7 +21,7 @@ struct LaunchArgs {
   int pair_first;         // finish stage: alignments [pair_first, pair_first + n_pairs)
   int n_pairs;            // (0: no finish launch)
   int phases;             // bit 0: tile kernel, bit 1: finish stage
-  const int* pair_tile_first;  // optional: first tile of each pair in this launch (else PairState::tile_first)
+  const int* pair_tile_first;  // optional, finish stage: first tile of each pair in this launch (else PairState::tile_first)
   int* status_out;        // optional: per-pair status mirror written by the finish stage
   int dual = 0;           // 1: CT dual form of a bundle-adjustment factor (build mode, ct = 1)
   int leaves_per_lane = 1;  // 1, or 2: two descents in flight per lane (256-leaf tiles, not the dual form)

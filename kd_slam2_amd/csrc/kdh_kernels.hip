@@ -128,6 +128,22 @@ __device__ __forceinline__ glb_f4* global_f4(const float4* p) { return (glb_f4*)
 __device__ const float4 kNoRecord[1] = {{0.0f, 0.0f, 0.0f, 0.0f}};
 __device__ __forceinline__ glb_f* global_f(const float* p) { return (glb_f*)p; }
 
+// A tile record in one 32-byte load (scalar for a wave-uniform address):
+// a plain struct copy may be split into field loads, some of which the
+// compiler then sinks below later branches.
+__device__ __forceinline__ TileDesc load_tile_desc(const TileDesc* p) {
+  typedef unsigned int u8v __attribute__((ext_vector_type(8)));
+  const u8v v = *reinterpret_cast<const u8v*>(p);
+  TileDesc d;
+  d.nodes = reinterpret_cast<const float4*>(((unsigned long long)v[1] << 32) | v[0]);
+  d.leaves = reinterpret_cast<const float4*>(((unsigned long long)v[3] << 32) | v[2]);
+  d.n_nodes = (int)v[4];
+  d.n_leaves = (int)v[5];
+  d.pair = (int)v[6];
+  d.leaf0 = (int)v[7];
+  return d;
+}
+
 struct GlobalFetch {
   glb_f4* __restrict__ nodes;
   __device__ __forceinline__ void operator()(int idx, float4& a, float4& b) const {
@@ -629,13 +645,12 @@ __device__ void finish_pair(PairState& ps, const PairState& S, const float* __re
 // TL: moving leaves per tile, 256 or 128 (one leaf per lane).
 template <bool CT, int LDS_NODES, bool DUAL, int QPL, int TL>
 __global__ __launch_bounds__(TL / QPL) __attribute__((amdgpu_waves_per_eu((DUAL || (CT && QPL == 2)) ? 3 : 4))) void tiles_kernel(const PairState* __restrict__ pairs,
-                                                         const int* __restrict__ tile_pair,
+                                                         const TileDesc* __restrict__ tiles,
                                                          int n_tiles_total,
                                                          float* __restrict__ partials,
                                                          int* __restrict__ corr_out,
                                                          int* __restrict__ depth_out,
-                                                         int xcd_remap,
-                                                         const int* __restrict__ pair_tile_first) {
+                                                         int xcd_remap) {
   static_assert(CT || !DUAL, "the dual form is a CT form");
   static_assert(QPL == 1 || (QPL == 2 && !DUAL && TL == 256), "two leaves per lane: 256-leaf rigid and CT tiles");
   static_assert(TL == 256 || TL == 128, "tiles of 256 or 128 moving leaves");
@@ -678,19 +693,18 @@ __global__ __launch_bounds__(TL / QPL) __attribute__((amdgpu_waves_per_eu((DUAL 
   const int c_lo = lo_all + (int)(((long long)span * j) / n_wg);
   const int c_hi = lo_all + (int)(((long long)span * (j + 1)) / n_wg);
   for (int t = c_lo; t < c_hi; ++t) {
-    // Three memory round trips before the descent: the tile's pair, the
-    // pair's tree fields, then the leaf records together with the
-    // top-of-tree staging, the status, the cache and the term parameters.
-    // A finished pair is skipped only after these loads are issued: a skip
-    // test right after a status load lets the compiler sink every later load
-    // below it, which costs a round trip of its own.
-    const int p = tile_pair ? tile_pair[t] : 0;
+    // Two memory round trips before the descent: the tile's record (its
+    // pair, the pair's trees, its first leaf), then the leaf records
+    // together with the top-of-tree staging and the pair's status, cache
+    // and term parameters.  A finished pair is skipped only after these
+    // loads are issued: a skip test right after a status load lets the
+    // compiler sink every later load below it, which costs a round trip of
+    // its own.  Without a tile list every tile belongs to pair 0.
+    const TileDesc d = tiles ? load_tile_desc(tiles + t) : make_tile_desc(pairs[0], 0, t - pairs[0].tile_first, TL);
+    const int p = d.pair;
     const PairState& ps = pairs[p];
-    const DevTree fixed = ps.fixed;
-    const DevTree moving = ps.moving;
-    const int tfirst = pair_tile_first ? pair_tile_first[p] : ps.tile_first;
-    glb_f4* nodes = global_f4(fixed.nodes);
-    glb_f4* leaves = global_f4(moving.leaves);
+    glb_f4* nodes = global_f4(d.nodes);
+    glb_f4* leaves = global_f4(d.leaves);
     // top-of-tree staging loads: all in flight at once, from clamped
     // addresses (a tree without nodes stages nothing and reads a dummy
     // record), stored after the barrier below.  They are issued for every
@@ -699,7 +713,7 @@ __global__ __launch_bounds__(TL / QPL) __attribute__((amdgpu_waves_per_eu((DUAL 
     // array in scratch memory.
     const bool stage = LDS_NODES > 0 && p != cached_pair;
     constexpr int kStage = LDS_NODES > 0 ? (2 * LDS_NODES + NT - 1) / NT : 1;
-    const int n_st = fixed.n_nodes < LDS_NODES ? fixed.n_nodes : LDS_NODES;
+    const int n_st = d.n_nodes < LDS_NODES ? d.n_nodes : LDS_NODES;
     glb_v4* st_src = (glb_v4*)(n_st > 0 ? nodes : global_f4(kNoRecord));
     const int st_last = n_st > 0 ? 2 * n_st - 1 : 0;
     v4f st[kStage];
@@ -713,8 +727,8 @@ __global__ __launch_bounds__(TL / QPL) __attribute__((amdgpu_waves_per_eu((DUAL 
     float4 la[QPL], lb[QPL];
 #pragma unroll
     for (int k = 0; k < QPL; ++k) {
-      q[k] = (t - tfirst) * TL + k * NT + tid;
-      live[k] = q[k] < moving.n_leaves && fixed.n_nodes > 0;
+      q[k] = d.leaf0 + k * NT + tid;
+      live[k] = q[k] < d.n_leaves && d.n_nodes > 0;
       const int qc = live[k] ? q[k] : 0;  // a tile exists, so the pair has >= 1 leaf
       la[k] = leaves[2 * qc];
       lb[k] = leaves[2 * qc + 1];
@@ -743,7 +757,7 @@ __global__ __launch_bounds__(TL / QPL) __attribute__((amdgpu_waves_per_eu((DUAL 
       L[k].dir = v3(la[k].w, lb[k].x, lb[k].y);
       L[k].dts = lb[k].z;
     }
-    glb_f* fdts = global_f(fixed.node_dts);
+    glb_f* fdts = global_f(ps.fixed.node_dts);
     auto dts_of = [fdts](int i) { return fdts[i]; };
     auto leaf_terms = [&](const auto& f) {
       if constexpr (QPL == 1) {
@@ -944,8 +958,7 @@ template <bool CT, int LDS, bool DUAL, int QPL, int TL = 256>
 static hipError_t launch_t(const LaunchArgs& a, hipStream_t s) {
   if (a.phases & 1) {
     hipLaunchKernelGGL((tiles_kernel<CT, LDS, DUAL, QPL, TL>), dim3(a.grid), dim3(TL / QPL), 0, s, a.pairs,
-                       a.tile_pair, a.n_tiles_total, a.partials, a.corr_out, a.depth_out, a.xcd_remap,
-                       a.pair_tile_first);
+                       a.tiles, a.n_tiles_total, a.partials, a.corr_out, a.depth_out, a.xcd_remap);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
   }

@@ -98,12 +98,13 @@ struct kdh_server {
   std::vector<int> h_tile_first;
   float* d_partials = nullptr;
   size_t partials_cap = 0;
-  int* d_tile_pair = nullptr;
+  TileDesc* d_tiles = nullptr;
   size_t tile_cap = 0;
   // the tile list and per-slot first tiles on the device, to skip the upload
   // while the active set is unchanged
-  std::vector<int> dev_tp, dev_tile_first;
-  const int* dev_tp_ptr = nullptr;
+  std::vector<TileDesc> dev_tp;
+  std::vector<int> dev_tile_first;
+  const TileDesc* dev_tp_ptr = nullptr;
 };
 
 static int64_t request_bytes(const kdh_request& r) {
@@ -258,34 +259,34 @@ static void engine(kdh_server* S) {
     }
     if (active.empty()) continue;
     int n_t[2] = {0, 0};
-    std::vector<int> lists[2];
+    std::vector<TileDesc> lists[2];
     for (int slot : active) {
       const PairState& ps = h_pairs[slot];
       if (ps.status != 0) continue;  // completed at admission
       const int k = ps.ct;
       S->h_tile_first[slot] = n_t[k];
-      for (int t = 0; t < ps.n_tiles; ++t) lists[k].push_back(slot);
+      for (int t = 0; t < ps.n_tiles; ++t) lists[k].push_back(make_tile_desc(ps, slot, t, ctx->tile_leaves));
       n_t[k] += ps.n_tiles;
     }
     bool ok = true;
     const size_t tiles_all = (size_t)n_t[0] + n_t[1];
-    ok = ok && grow_dev(&S->d_tile_pair, &S->tile_cap, std::max<size_t>(tiles_all, 1)) == hipSuccess;
+    ok = ok && grow_dev(&S->d_tiles, &S->tile_cap, std::max<size_t>(tiles_all, 1)) == hipSuccess;
     const size_t part_all = (size_t)n_t[0] * slots(0) + (size_t)n_t[1] * slots(1);
     ok = ok && grow_dev(&S->d_partials, &S->partials_cap, std::max<size_t>(part_all, 1)) == hipSuccess;
-    std::vector<int> tp;
+    std::vector<TileDesc> tp;
     tp.reserve(tiles_all);
     tp.insert(tp.end(), lists[0].begin(), lists[0].end());
     tp.insert(tp.end(), lists[1].begin(), lists[1].end());
     if (ok && tiles_all) {
-      if (tp != S->dev_tp || S->h_tile_first != S->dev_tile_first || S->dev_tp_ptr != S->d_tile_pair) {
-        ok = hipMemcpyAsync(S->d_tile_pair, tp.data(), tiles_all * sizeof(int), hipMemcpyHostToDevice,
+      if (!kdh_same_tiles(tp, S->dev_tp) || S->h_tile_first != S->dev_tile_first || S->dev_tp_ptr != S->d_tiles) {
+        ok = hipMemcpyAsync(S->d_tiles, tp.data(), tiles_all * sizeof(TileDesc), hipMemcpyHostToDevice,
                             ctx->stream) == hipSuccess;
         ok = ok && hipMemcpyAsync(S->d_tile_first, S->h_tile_first.data(), M * sizeof(int), hipMemcpyHostToDevice,
                                   ctx->stream) == hipSuccess;
         // what the device now holds
         S->dev_tp = tp;
         S->dev_tile_first = S->h_tile_first;
-        S->dev_tp_ptr = ok ? S->d_tile_pair : nullptr;
+        S->dev_tp_ptr = ok ? S->d_tiles : nullptr;
       }
       for (int it = 0; it < iters && ok; ++it) {
       size_t poff = 0, toff = 0;
@@ -293,10 +294,11 @@ static void engine(kdh_server* S) {
         if (!n_t[k]) continue;
         LaunchArgs a{};
         a.pairs = ctx->d_pairs;
-        a.tile_pair = S->d_tile_pair + toff;
+        a.tiles = S->d_tiles + toff;
         a.n_tiles_total = n_t[k];
         a.partials = S->d_partials + poff;
         a.grid = kdh_tile_grid(ctx, n_t[k]);
+        a.tile_leaves = ctx->tile_leaves;  // the private context's, as kdh_fill_pair counted the tiles
         a.ct = k;
         a.lds_nodes = (k && ctx->lds_nodes > 1024) ? 1024 : ctx->lds_nodes;
         a.xcd_remap = ctx->xcd_remap;
@@ -411,7 +413,7 @@ int kdh_server_destroy(kdh_server* S) {
   if (S->d_status) hipFree(S->d_status);
   if (S->d_tile_first) hipFree(S->d_tile_first);
   if (S->d_partials) hipFree(S->d_partials);
-  if (S->d_tile_pair) hipFree(S->d_tile_pair);
+  if (S->d_tiles) hipFree(S->d_tiles);
   if (S->h_status) hipHostFree(S->h_status);
   kdh_ctx_destroy(S->ctx);
   {
