@@ -401,6 +401,7 @@ static TermParams term_params(const kdh_params& p, int stats_mode) {
   t.min_normal_cos = p.min_normal_cos;
   t.stats_mode = stats_mode ? 1 : 0;
   t.recenter = p.recenter ? 1 : 0;
+  t.kt_quarter = p.kernel_threshold > 0.0f ? (float)sqrt(sqrt((double)p.kernel_threshold)) : 0.0f;
   return t;
 }
 

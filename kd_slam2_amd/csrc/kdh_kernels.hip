@@ -150,8 +150,30 @@ struct GlobalFetch {
     a = nodes[2 * idx];
     b = nodes[2 * idx + 1];
   }
+  __device__ __forceinline__ int descend(V3f q, int* depth_out, float4& a, float4& b) const {
+    return kdh::descend(*this, q, depth_out, a, b);
+  }
 };
 
+// One greedy step at the node whose record is (a, b) (the step of `descend`,
+// kdh_term.h): false at a leaf (idx stays the leaf); otherwise idx becomes the
+// child on q's side, -1 when that child is missing.
+__device__ __forceinline__ bool descent_step(V3f q, const float4& a, const float4& b, int& idx) {
+  const int l = __float_as_int(b.z);
+  const int r = __float_as_int(b.w);
+  if ((l & r) < 0) return false;  // both links negative: a leaf
+  const float dx = q.x - a.x;
+  const float dy = q.y - a.y;
+  const float dz = q.z - a.z;
+  const float s = (a.w * dx + b.x * dy) + b.y * dz;
+  idx = s > 0.0f ? l : r;
+  return true;
+}
+
+// Records 0 .. n_lds-1 (the top of the tree in BFS order) come from the LDS
+// copy, the rest from global memory.  DEPTH: the descent counts its steps
+// into *depth_out (the launch asked for depths).
+template <bool DEPTH>
 struct LdsFetch {
   glb_f4* __restrict__ nodes;
   lds_f4* lds;
@@ -164,6 +186,47 @@ struct LdsFetch {
       a = nodes[2 * idx];
       b = nodes[2 * idx + 1];
     }
+  }
+  // The greedy descent in two loops: the staged levels from LDS until the
+  // path leaves them, then the rest from global memory.  A BFS prefix is
+  // closed under parents, so once a path leaves it, it never comes back, and
+  // each loop tests only what its records need: the decisions and their order
+  // are those of `descend`.
+  __device__ __forceinline__ int descend(V3f q, int* depth_out, float4& a, float4& b) const {
+    int idx = 0;
+    int depth = 0;
+    bool deep = true;  // the descent goes on at idx in global memory
+    if (n_lds > 0) {
+      a = lds[0];
+      b = lds[1];
+      for (;;) {
+        if (!descent_step(q, a, b, idx)) {
+          deep = false;
+          break;
+        }
+        if (DEPTH) ++depth;
+        if (idx < 0) {
+          deep = false;
+          break;
+        }
+        if (idx >= n_lds) break;
+        a = lds[2 * idx];
+        b = lds[2 * idx + 1];
+      }
+    }
+    if (deep) {
+      a = nodes[2 * idx];
+      b = nodes[2 * idx + 1];
+      for (;;) {
+        if (!descent_step(q, a, b, idx)) break;
+        if (DEPTH) ++depth;
+        if (idx < 0) break;
+        a = nodes[2 * idx];
+        b = nodes[2 * idx + 1];
+      }
+    }
+    if (DEPTH) *depth_out = depth;
+    return idx;
   }
 };
 
@@ -637,6 +700,16 @@ __device__ void finish_pair(PairState& ps, const PairState& S, const float* __re
 
 // ---- the tile kernel --------------------------------------------------------
 
+// Resident waves per SIMD the tile kernel is compiled for (its register
+// budget is 512 / waves VGPRs): 8, the most a SIMD holds, for the rigid form
+// with one leaf per lane when its LDS allows 8 (at most 256 staged nodes,
+// 64 VGPRs); 4 for the CT form, for two rigid leaves per lane and for deeper
+// staging, 3 for the dual form and for two CT leaves per lane.
+template <bool CT, bool DUAL, int QPL, int LDS_NODES>
+constexpr int kTileWaves() {
+  return (DUAL || (CT && QPL == 2)) ? 3 : ((CT || QPL == 2 || LDS_NODES > 256) ? 4 : 8);
+}
+
 // Workgroup of the tile kernel: TL / QPL threads, each lane carrying QPL of
 // the tile's TL moving leaves (QPL = 2: two descents in flight per lane).
 // DUAL (CT only): the dual form of quadraticTermDual, built for bundle
@@ -644,7 +717,7 @@ __device__ void finish_pair(PairState& ps, const PairState& S, const float* __re
 // QPL: moving leaves per lane (1, or 2 for the rigid and CT forms).
 // TL: moving leaves per tile, 256 or 128 (one leaf per lane).
 template <bool CT, int LDS_NODES, bool DUAL, int QPL, int TL>
-__global__ __launch_bounds__(TL / QPL) __attribute__((amdgpu_waves_per_eu((DUAL || (CT && QPL == 2)) ? 3 : 4))) void tiles_kernel(const PairState* __restrict__ pairs,
+__global__ __launch_bounds__(TL / QPL) __attribute__((amdgpu_waves_per_eu(kTileWaves<CT, DUAL, QPL, LDS_NODES>()))) void tiles_kernel(const PairState* __restrict__ pairs,
                                                          const TileDesc* __restrict__ tiles,
                                                          int n_tiles_total,
                                                          float* __restrict__ partials,
@@ -689,9 +762,18 @@ __global__ __launch_bounds__(TL / QPL) __attribute__((amdgpu_waves_per_eu((DUAL 
   const int n_wg = remap ? G / 8 : G;
   const int j = remap ? (int)(blockIdx.x >> 3) : (int)blockIdx.x;
   const int span = hi_all - lo_all;
-  // chunk j of n_wg over `span` tiles: sizes differ by at most one
-  const int c_lo = lo_all + (int)(((long long)span * j) / n_wg);
-  const int c_hi = lo_all + (int)(((long long)span * (j + 1)) / n_wg);
+  // With at least as many workgroups as tiles (the usual one-per-tile grid)
+  // workgroup j takes tile j of the range, without a division; a persistent
+  // grid gives workgroup j chunk j of n_wg, sizes differing by at most one
+  // (a 64-bit division, once per workgroup of many tiles).
+  int c_lo, c_hi;
+  if (n_wg >= span) {
+    c_lo = lo_all + min(j, span);
+    c_hi = lo_all + min(j + 1, span);
+  } else {
+    c_lo = lo_all + (int)(((long long)span * j) / n_wg);
+    c_hi = lo_all + (int)(((long long)span * (j + 1)) / n_wg);
+  }
   for (int t = c_lo; t < c_hi; ++t) {
     // Two memory round trips before the descent: the tile's record (its
     // pair, the pair's trees, its first leaf), then the leaf records
@@ -793,7 +875,11 @@ __global__ __launch_bounds__(TL / QPL) __attribute__((amdgpu_waves_per_eu((DUAL 
       }
     };
     if (LDS_NODES > 0) {
-      leaf_terms(LdsFetch{nodes, (lds_f4*)s_nodes, n_lds});
+      // the descent counts its depth only when the launch asks for depths
+      if (depth_out && QPL == 1)
+        leaf_terms(LdsFetch<true>{nodes, (lds_f4*)s_nodes, n_lds});
+      else
+        leaf_terms(LdsFetch<false>{nodes, (lds_f4*)s_nodes, n_lds});
     } else {
       leaf_terms(GlobalFetch{nodes});
     }
@@ -815,7 +901,7 @@ __global__ __launch_bounds__(TL / QPL) __attribute__((amdgpu_waves_per_eu((DUAL 
 #pragma unroll
       for (int k = 0; k < QPL; ++k) {
         float w[NW];
-        const float sg = sqrtf(R[k].gamma);
+        const float sg = R[k].sg;
 #pragma unroll
         for (int i = 0; i < 6; ++i) w[i] = sg * R[k].J[i];
         if (CT) {
@@ -862,7 +948,7 @@ __global__ __launch_bounds__(TL / QPL) __attribute__((amdgpu_waves_per_eu((DUAL 
       if constexpr (DUAL) {
         // z = sqrt(g) J_vel_B: a third transpose, then C_uz = sum w z^T and
         // C_zz = sum z z^T (QPL = 1: a[] still holds this lane's w operands)
-        const float sg = sqrtf(R[0].gamma);
+        const float sg = R[0].sg;
         float az[16];
 #pragma unroll
         for (int m = 0; m < 16; ++m) az[m] = 0.0f;

@@ -48,6 +48,7 @@ struct TermParams {
   float min_normal_cos;
   int stats_mode;
   int recenter;  // 0: the reference's e and J; 1: e and J about the anchor (below)
+  float kt_quarter;  // kernel_threshold^(1/4): the outlier weight's square root is kt_quarter / sqrt|e|
 };
 
 // Per-iteration inputs of the term (QuadraticTermCache, icp_.h:97-102, plus
@@ -73,7 +74,10 @@ struct TermCache {
   float vf[6];  // fixed velocities [t; w] (dual form only)
 };
 
-// Greedy descent (icp_impl_.h:116-121, node_.h:54-63).  Node record layout:
+// Greedy descent (icp_impl_.h:116-121, node_.h:54-63).  A node fetcher
+// (operator()(idx, a, b) loads record idx) also provides descend(q,
+// depth_out, a, b): this loop, or one specialised to where its records live,
+// with the same decisions in the same order.  Node record layout:
 //   a = (mean.x, mean.y, mean.z, dir.x), b = (dir.y, dir.z, left, right)
 // with the child links stored as int bits.  Returns the leaf index or -1;
 // (a, b) end as the record of the last node fetched, which is the leaf's
@@ -236,14 +240,15 @@ struct LeafIn {
 };
 
 // Compact result of one leaf: everything the accumulator slots are formed
-// from.  A rejected leaf has gamma = 0, e = 0, J = 0 and its outcome set, so
-// every product slot expands to 0 and only its one-hot counter is 1.
+// from.  The robust weight gamma travels as its square root sg: every slot is
+// a product of two entries of w = [sg J, sg Jv, sg e, e], so the kernels never
+// need gamma itself.  A rejected leaf has sg = 0, e = 0, J = 0 and its outcome
+// set, so every product slot expands to 0 and only its one-hot counter is 1.
 struct TermRec {
   float J[6];   // pose Jacobian (re-centred rotational part)
   float Jv[6];  // velocity Jacobian (CT)
   float e;
-  float gamma;
-  float chi2;
+  float sg;     // sqrt(gamma)
   int outcome;  // kInlier .. kErrFn
 };
 
@@ -253,16 +258,13 @@ KDH_HD void rec_reject(TermRec& r, int outcome) {
     r.Jv[i] = 0.0f;
   }
   r.e = 0.0f;
-  r.gamma = 0.0f;
-  r.chi2 = 0.0f;
+  r.sg = 0.0f;
   r.outcome = outcome;
 }
 
-// Accumulator slot s of a leaf record, with the packing of
-// ICPWsItemDiag_::pack (upper-triangular J_r J_c g, then J_r e g;
-// icp_ws_item_.h:26-34) and ICPWsItemCross_::pack (J_pose_r J_vel_c g,
-// icp_ws_item_.h:72-75).  s must be a compile-time constant after unrolling.
-KDH_HD int tri_r(int k) {  // row of the k-th upper-triangular entry of a 6x6
+// Packing of the 6x6 upper triangles in ICPWsItemDiag_::pack (row by row;
+// icp_ws_item_.h:26-34): row and column of the k-th entry.
+KDH_HD int tri_r(int k) {
   return k < 6 ? 0 : k < 11 ? 1 : k < 15 ? 2 : k < 18 ? 3 : k < 20 ? 4 : 5;
 }
 KDH_HD int tri_c(int k) {
@@ -271,36 +273,30 @@ KDH_HD int tri_c(int k) {
   return r + (k - start);
 }
 
-template <bool CT>
-KDH_HD float rec_slot(const TermRec& r, int s) {
-  constexpr int CS = CT ? 90 : 27;
-  if (s < 21) return (r.J[tri_r(s)] * r.J[tri_c(s)]) * r.gamma;
-  if (s < 27) return (r.J[s - 21] * r.e) * r.gamma;
-  if (CT) {
-    if (s < 48) return (r.Jv[tri_r(s - 27)] * r.Jv[tri_c(s - 27)]) * r.gamma;
-    if (s < 54) return (r.Jv[s - 48] * r.e) * r.gamma;
-    if (s < 90) return (r.J[(s - 54) / 6] * r.Jv[(s - 54) % 6]) * r.gamma;
-  }
-  if (s == CS) return r.chi2;
-  if (s == CS + 1) return r.chi2 * r.gamma;
-  if (s < CS + 7) return r.outcome == s - CS - 2 ? 1.0f : 0.0f;
-  return 0.0f;
-}
-
 // Robust weight and outcome (icp_impl_.h:138-151): the inlier test on the
-// reference's FP32 chi2 = e_ref^2; the weight and the chi2 statistics from the
-// residual R.e that enters b (equal to e_ref unless re-centred).
+// reference's FP32 chi2 = e_ref^2 decides the outcome bit for bit; the weight
+// gamma = sqrt(kernel_threshold / e^2) of an outlier, with e the residual
+// R.e that enters b (equal to e_ref unless re-centred), is kept as its square
+// root kt_quarter / sqrt|e|, from the hardware reciprocal square root (a few
+// ulp from the reference's correctly rounded division and square root: H, b
+// and chi2 are tolerance results, no decision reads the weight).
 // The dual form weighs differently (quadraticTermDual, cticp_impl_.h:240-248):
 // outliers are always weighted, inliers drop to 0 in stats mode.
+KDH_HD float rsqrt_f(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_rsqf(x);
+#else
+  return 1.0f / sqrtf(x);
+#endif
+}
 template <bool DUAL = false>
 KDH_HD void classify(TermRec& R, float e_ref, const TermParams& P) {
-  R.chi2 = R.e * R.e;
   if (e_ref * e_ref > P.kernel_threshold) {
     R.outcome = kOutlier;
-    R.gamma = (P.stats_mode && !DUAL) ? 0.0f : sqrtf(P.kernel_threshold / R.chi2);
+    R.sg = (P.stats_mode && !DUAL) ? 0.0f : P.kt_quarter * rsqrt_f(fabsf(R.e));
   } else {
     R.outcome = kInlier;
-    R.gamma = (P.stats_mode && DUAL) ? 0.0f : 1.0f;
+    R.sg = (P.stats_mode && DUAL) ? 0.0f : 1.0f;
   }
 }
 
@@ -352,7 +348,7 @@ KDH_HD int icp_term(const NodeFetch& fetch, const TermCache& C, const TermParams
                     int* depth_out) {
   const V3f p_mif = icp_query(C, L);
   float4 a, b;
-  const int leaf = descend(fetch, p_mif, depth_out, a, b);
+  const int leaf = fetch.descend(p_mif, depth_out, a, b);
   icp_eval(C, P, L, p_mif, leaf, a, b, R);
   return leaf;
 }
@@ -479,7 +475,7 @@ KDH_HD int ct_term(const NodeFetch& fetch, const TermCache& C, const TermParams&
   ct_warp(C, L, rd, wmean, wdir);
   const V3f p_mif = apply_iso(C.Xmf, wmean);
   float4 a, b;
-  const int leaf = descend(fetch, p_mif, depth_out, a, b);
+  const int leaf = fetch.descend(p_mif, depth_out, a, b);
   ct_eval_warped<DUAL>(C, P, L, rd, wmean, wdir, p_mif, leaf, a, b, R, Jb, dts_of);
   return leaf;
 }
