@@ -14,8 +14,10 @@
 //      accumulator slot is an entry of C = sum_leaves w w^T over the leaf
 //      vectors w = [sqrt(g) J, sqrt(g) Jv, sqrt(g) e, e], one 16x16 C per
 //      wave from v_mfma_f32_16x16x4f32 with the 64 leaves as the K dimension
-//      (operands transposed through LDS), the outcome counts from ballots,
-//      then the 4 waves' C tiles summed through LDS in a fixed order,
+//      (operands transposed through LDS; the rigid form's 8-component
+//      vectors fill both halves of the 16 rows with two leaf sets, 8 steps
+//      per wave instead of 16), the outcome counts from ballots, then the
+//      4 waves' C tiles summed through LDS in a fixed order,
 //   4. writes one FP32 partial row (34 ICP / 97 CT slots) per tile to a
 //      fixed slot.
 // A second kernel, `finish_kernel`, runs one workgroup per alignment: it sums
@@ -736,6 +738,8 @@ __global__ __launch_bounds__(TL / QPL) __attribute__((amdgpu_waves_per_eu(kTileW
   constexpr int HW = (NW + 1) / 2;     // components per transpose half
   constexpr int WS = 68;               // transpose row stride (floats): 16-B aligned, spreads banks
   constexpr int WBUF = DUAL ? 3 * 256 : HW * WS;  // per wave: transpose, then the C tile(s)
+  constexpr int kSteps = CT ? 16 : 8;             // MFMA steps per 64 leaves (rigid: two leaf sets per step)
+  constexpr int kPacked = CT ? 0 : 8 * 16 + 8;    // rigid: cell offset of the second set's block
   static_assert(HW * WS >= 256 && WBUF >= HW * WS, "the per-wave C tiles reuse the transpose buffer");
   __shared__ float4 s_nodes[LDS_NODES > 0 ? 2 * LDS_NODES : 2];
   __shared__ __align__(16) float s_w[NWV][WBUF];
@@ -888,10 +892,15 @@ __global__ __launch_bounds__(TL / QPL) __attribute__((amdgpu_waves_per_eu(kTileW
     // w = [sqrt(g) J, sqrt(g) Jv (CT), sqrt(g) e, e] per leaf, every
     // accumulator slot is an entry of C = sum_leaves w w^T: the 6x6 (and CT
     // 12x12) blocks of H, b, sum g e^2 (chi2 kernelised) and sum e^2 (chi2).
-    // Per wave C is 16 v_mfma_f32_16x16x4f32 per 64 leaves with the leaves
-    // as the K dimension; the leaf vectors are transposed through LDS into
-    // the operand layout (lane i + 16k holds component i of leaf 16k + m for
-    // step m).  The outcome counts come from ballots.
+    // Per wave C is a sum of v_mfma_f32_16x16x4f32 over 64 leaves with the
+    // leaves as the K dimension; the leaf vectors are transposed through LDS
+    // into the operand layout.  CT (14 components): 16 steps, lane i + 16k
+    // holds component i of leaf 16k + m at step m.  Rigid (8 components):
+    // the operand's rows 8..15 carry a second set of leaves, so a step takes
+    // 8 leaves and a wave 8 steps: lane i + 16k holds component i & 7 of leaf
+    // 16k + 8 (i >> 3) + m at step m, and the tile's C is the sum of the two
+    // diagonal 8x8 blocks (the off-diagonal blocks mix the two sets and are
+    // never read).  The outcome counts come from ballots.
     {
       float* sw = s_w[wave];
       const int ci = lane & 15, cg = lane >> 4;
@@ -920,10 +929,12 @@ __global__ __launch_bounds__(TL / QPL) __attribute__((amdgpu_waves_per_eu(kTileW
           for (int i = 0; i < HW; ++i)
             if (half * HW + i < NW) sw[i * WS + lane] = w[half * HW + i];
           wave_lds_sync();
-          if (ci >= half * HW && ci < half * HW + HW && ci < NW) {
-            const float4* src = reinterpret_cast<const float4*>(sw + (ci - half * HW) * WS + cg * 16);
+          const int comp = CT ? ci : (ci & 7);                   // component this lane feeds
+          const int first = CT ? cg * 16 : cg * 16 + 8 * (ci >> 3);  // its first leaf
+          if (comp >= half * HW && comp < half * HW + HW && comp < NW) {
+            const float4* src = reinterpret_cast<const float4*>(sw + (comp - half * HW) * WS + first);
 #pragma unroll
-            for (int m4 = 0; m4 < 4; ++m4) {
+            for (int m4 = 0; m4 < kSteps / 4; ++m4) {
               const float4 v = src[m4];
               a[4 * m4 + 0] = v.x;
               a[4 * m4 + 1] = v.y;
@@ -934,7 +945,7 @@ __global__ __launch_bounds__(TL / QPL) __attribute__((amdgpu_waves_per_eu(kTileW
           wave_lds_sync();
         }
 #pragma unroll
-        for (int m = 0; m < 16; m += 2) {
+        for (int m = 0; m < kSteps; m += 2) {
           c0 = __builtin_amdgcn_mfma_f32_16x16x4f32(a[m], a[m], c0, 0, 0, 0);
           c1 = __builtin_amdgcn_mfma_f32_16x16x4f32(a[m + 1], a[m + 1], c1, 0, 0, 0);
         }
@@ -989,9 +1000,11 @@ __global__ __launch_bounds__(TL / QPL) __attribute__((amdgpu_waves_per_eu(kTileW
       float sv;
       if (sl < CS + 2 || (DUAL && sl >= kCtSlots)) {
         const int cell = (DUAL && sl >= kCtSlots) ? dual_slot_cell(sl) : slot_cell<CT>(sl);
-        if constexpr (NWV == 4) sv = (s_w[0][cell] + s_w[1][cell]) + (s_w[2][cell] + s_w[3][cell]);
-        else if constexpr (NWV == 2) sv = s_w[0][cell] + s_w[1][cell];
-        else sv = s_w[0][cell];
+        // a wave's entry: its C cell, plus the second leaf set's for the rigid form
+        auto cv = [&](int wv) { return kPacked ? s_w[wv][cell] + s_w[wv][cell + kPacked] : s_w[wv][cell]; };
+        if constexpr (NWV == 4) sv = (cv(0) + cv(1)) + (cv(2) + cv(3));
+        else if constexpr (NWV == 2) sv = cv(0) + cv(1);
+        else sv = cv(0);
       } else {
         const int k = sl - CS - 2;
         if constexpr (NWV == 4) sv = (float)((s_cnt[0][k] + s_cnt[1][k]) + (s_cnt[2][k] + s_cnt[3][k]));
