@@ -41,6 +41,7 @@ Beside it the line reports
     (the reference's cap) and 1 thread, with the host CPU model.
 """
 import argparse
+import contextlib
 import csv
 import ctypes
 import gc
@@ -79,16 +80,23 @@ def algorithmic_bytes_per_iter(depth, ct, n_tiles):
 
 
 def host_cpu():
-    """(model name, logical CPUs) of this host."""
-    model = "unknown"
+    """(model name with its family / model / stepping numbers, logical CPUs)
+    of this host: a virtual machine may report only the vendor's brand name,
+    and the numbers still tell the processor generation."""
+    fields = {}
     try:
         with open("/proc/cpuinfo") as fh:
             for line in fh:
-                if line.startswith("model name"):
-                    model = line.split(":", 1)[1].strip()
-                    break
+                if not line.strip():
+                    break  # the first processor's block is enough
+                k, _, v = line.partition(":")
+                fields.setdefault(k.strip(), v.strip())
     except OSError:
         pass
+    model = fields.get("model name", "unknown")
+    ids = [f"{k} {fields[k]}" for k in ("cpu family", "model", "stepping") if k in fields]
+    if ids:
+        model = f"{model} ({', '.join(ids)})"
     return model, os.cpu_count() or 0
 
 
@@ -168,6 +176,21 @@ def _gen_cfg4_block(kb):
     return synth.config4_scan_block(b) if kind == 0 else synth.config4_map_block(b)
 
 
+@contextlib.contextmanager
+def _worker_pool():
+    """A fork pool of workload generators (at most 16, the CPU share of one
+    GPU) that ends with close() and join(): the workers exit by themselves.
+    Pool's own context exit terminates them with SIGTERM, which a process
+    run under rocprofv3 catches in the profiler's preloaded handler, where it
+    can stall, and the pool then waits on it for ever."""
+    pool = mp.get_context("fork").Pool(min(16, os.cpu_count() or 4))
+    try:
+        yield pool
+    finally:
+        pool.close()
+        pool.join()
+
+
 def _gen_pairs3(n_pairs, pool):
     from kd_slam2_amd import synth
 
@@ -199,7 +222,7 @@ def pmc_child(args):
     device copy of COPY_BYTES bytes, for the counter passes."""
     from kd_slam2_amd import Context, DeviceTree, batch_align, default_params
 
-    with mp.get_context("fork").Pool(min(16, os.cpu_count() or 4)) as pool:
+    with _worker_pool() as pool:
         pairs3 = _gen_pairs3(args.pairs3, pool).get()
     ctx = Context(0)
     ctx.set_launch_config(grid=args.pmc_grid, lds_nodes=args.pmc_lds, leaves_per_lane=args.pmc_lpl,
@@ -250,7 +273,7 @@ def pmc_child_precision(args):
     passes of the 8M-point map."""
     from kd_slam2_amd import Context, DeviceTree, ICPHIP3D, TreeGenerator3D, default_params, synth
 
-    with mp.get_context("fork").Pool(min(16, os.cpu_count() or 4)) as pool:
+    with _worker_pool() as pool:
         blocks = pool.map(_gen_cfg4_block, [(k, b) for k in (0, 1) for b in range(8)])
     g4 = TreeGenerator3D(parallel_level=3)
     X_gt4, X04, Xf4 = synth.config4_poses()
@@ -381,7 +404,7 @@ def main():
         return
 
     t0 = time.time()
-    with mp.get_context("fork").Pool(min(16, os.cpu_count() or 4)) as pool:
+    with _worker_pool() as pool:
         j_p3 = _gen_pairs3(args.pairs3, pool)
         j_cfg2 = None if args.headline_only else pool.apply_async(_gen_cfg2, (0,))
         j_sw5 = None if args.no_tree_build else pool.apply_async(_gen_sweep5, (0,))
@@ -415,6 +438,24 @@ def main():
     # ---- headline: 256-pair batched loop-closure verification --------------
     head = run_headline(ctx, pairs3, args)
     ctx.set_launch_config(leaves_per_lane=1, batch_streams=1, tile_leaves=256)  # the defaults for the other rows
+
+    # ---- fabric traffic of the headline tile kernel: two counter passes over
+    # a child run of this script, made while this process holds only the
+    # headline workload ------------------------------------------------------
+    traffic = None
+    pmc = {}
+    if not args.no_pmc:
+        f = pmc_pass("FETCH_SIZE", args, head["tuned"])
+        wr = pmc_pass("WRITE_SIZE", args, head["tuned"])
+        if f and wr:
+            kib = COPY_BYTES / 1024.0
+            f_scale = kib / f[1]
+            w_scale = kib / wr[1]
+            traffic = round((f[0] * f_scale + wr[0] * w_scale) * 1024.0, 1)
+            pmc = dict(fetch_kib=round(f[0], 1), write_kib=round(wr[0], 1), copy_fetch_kib=round(f[1], 1),
+                       copy_write_kib=round(wr[1], 1), fetch_scale=round(f_scale, 4), write_scale=round(w_scale, 4))
+            log(f"pmc: tile fetch {f[0]:.0f} KiB, write {wr[0]:.0f} KiB per launch (copy {f[1]:.0f}/{wr[1]:.0f} "
+                f"KiB for {kib:.0f} KiB)")
 
     # ---- CT-ICP: served batches, device loop, plain path ---------------------
     ct = None
@@ -452,22 +493,6 @@ def main():
     cpu = None
     if not args.no_cpu:
         cpu = run_cpu_baseline(pairs3, args, default_params)
-
-    # ---- fabric traffic of the headline tile kernel ---------------------------
-    traffic = None
-    pmc = {}
-    if not args.no_pmc:
-        f = pmc_pass("FETCH_SIZE", args, head["tuned"])
-        wr = pmc_pass("WRITE_SIZE", args, head["tuned"])
-        if f and wr:
-            kib = COPY_BYTES / 1024.0
-            f_scale = kib / f[1]
-            w_scale = kib / wr[1]
-            traffic = round((f[0] * f_scale + wr[0] * w_scale) * 1024.0, 1)
-            pmc = dict(fetch_kib=round(f[0], 1), write_kib=round(wr[0], 1), copy_fetch_kib=round(f[1], 1),
-                       copy_write_kib=round(wr[1], 1), fetch_scale=round(f_scale, 4), write_scale=round(w_scale, 4))
-            log(f"pmc: tile fetch {f[0]:.0f} KiB, write {wr[0]:.0f} KiB per launch (copy {f[1]:.0f}/{wr[1]:.0f} "
-                f"KiB for {kib:.0f} KiB)")
 
     tile_us = head["tile_us"]
     demand = head["demand_bytes_per_launch"]
