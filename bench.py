@@ -523,6 +523,7 @@ def main():
                     pairs=head["pairs"], scan_points=head["scan_points"], moving_leaves=head["moving_leaves"],
                     fixed_nodes=head["fixed_nodes"], mean_depth=head["mean_depth"], tuned=head["tuned"]),
         latency_ms=head["latency_ms"],
+        leaf_iterations_per_s=head["leaf_iterations_per_s"],
         served_speedup_vs_plain=head["speedup_vs_plain"],
         plain_pts_per_s=head["plain_pts_per_s"],
         pcie_inclusive=head["pcie_inclusive"],
@@ -563,15 +564,19 @@ def run_headline(ctx, pairs3, args):
     gc.disable()
     step_t = []
     kern = []
+    steps_out = []
     for k in range(args.steps):
         t = time.perf_counter()
         B.set_initial(X0_sets[k % len(X0_sets)])
-        B.run(ctx)
+        steps_out.append(B.run(ctx))
         step_t.append(time.perf_counter() - t)
         kern.append(ctx.last_kernel_time()[0])
     gc.enable()
     total = float(np.sum(step_t))
     value = n_pts * args.steps / total
+    # moving-leaf iterations: every pair's leaves times the rounds it ran in each step
+    leaves_per_pair = np.array([len(q["moving"].leaves) for q in pairs3], dtype=np.float64)
+    leaf_iters = float(sum(np.dot(leaves_per_pair, o.iterations.astype(np.float64)) for o in steps_out))
     # the first pose set once more, for the accuracy check
     B.set_initial(X0_sets[0])
     out = B.run(ctx)
@@ -639,6 +644,7 @@ def run_headline(ctx, pairs3, args):
         M.close()
     res = dict(
         value=round(value, 1), ms_per_step=round(1e3 * total / args.steps, 3), pairs=len(pairs3),
+        leaf_iterations_per_s=round(leaf_iters / total, 1),
         scan_points=n_pts, moving_leaves=int(n_q), fixed_nodes=int(sum(len(q["fixed"].nodes) for q in pairs3)),
         mean_depth=round(depth_sum / max(n_q, 1), 2), tuned=tuned,
         latency_ms=dict(step_p50=round(1e3 * float(np.percentile(step_t, 50)), 3),
