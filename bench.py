@@ -219,7 +219,9 @@ def _hip():
 
 def pmc_child(args):
     """One batch of the headline workload at the tuned launch shape plus a
-    device copy of COPY_BYTES bytes, for the counter passes."""
+    device copy of COPY_BYTES bytes, for the counter passes.  With
+    --pmc-copies k every pair's trees are uploaded k times (separate device
+    copies, so separate traffic) and the batch holds k x the pairs."""
     from kd_slam2_amd import Context, DeviceTree, batch_align, default_params
 
     with _worker_pool() as pool:
@@ -228,10 +230,12 @@ def pmc_child(args):
     ctx.set_launch_config(grid=args.pmc_grid, lds_nodes=args.pmc_lds, leaves_per_lane=args.pmc_lpl,
                           tile_leaves=args.pmc_tile)
     p = _p3_params(default_params)
-    trees = [(DeviceTree(ctx, q["fixed"]), DeviceTree(ctx, q["moving"])) for q in pairs3]
-    batch_align(ctx, [dict(fixed=F, moving=M, ct=0, X0=q["X0"], Xf=q["Xf"], params=p)
-                      for q, (F, M) in zip(pairs3, trees)])
-    print("pmc child: batch done", flush=True)
+    job = []
+    for _ in range(max(args.pmc_copies, 1)):
+        trees = [(DeviceTree(ctx, q["fixed"]), DeviceTree(ctx, q["moving"])) for q in pairs3]
+        job += [dict(fixed=F, moving=M, ct=0, X0=q["X0"], Xf=q["Xf"], params=p) for q, (F, M) in zip(pairs3, trees)]
+    batch_align(ctx, job)
+    print(f"pmc child: batch of {len(job)} pairs done", flush=True)
     _copy_probe()
 
 
@@ -364,6 +368,8 @@ def main():
     ap.add_argument("--no-precision", action="store_true", help="skip the 10 km precision-stress workload")
     ap.add_argument("--no-tree-build", action="store_true", help="skip the tree-generation timings")
     ap.add_argument("--no-next-rows", action="store_true", help="skip tree maintenance, factor forms, KDTree3f")
+    ap.add_argument("--no-scaling", action="store_true", help="skip the batch-size scaling rows")
+    ap.add_argument("--scaling-steps", type=int, default=20, help="timed batches per batch size")
     ap.add_argument("--headline-only", action="store_true", help="only the headline batch and its roofline")
     ap.add_argument("--tree-build-only", action="store_true",
                     help="only the tree-generation timings (for kernel traces of the builder)")
@@ -376,6 +382,7 @@ def main():
     ap.add_argument("--pmc-lds", type=int, default=256, help=argparse.SUPPRESS)
     ap.add_argument("--pmc-lpl", type=int, default=1, help=argparse.SUPPRESS)
     ap.add_argument("--pmc-tile", type=int, default=256, help=argparse.SUPPRESS)
+    ap.add_argument("--pmc-copies", type=int, default=1, help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.pmc_child:
         pmc_child(args)
@@ -391,7 +398,7 @@ def main():
               "communication across GPUs: run one process per GPU, each on its own share of the pairs", file=sys.stderr)
         sys.exit(2)
     if args.headline_only:
-        args.no_serve = args.no_precision = args.no_tree_build = args.no_next_rows = True
+        args.no_serve = args.no_precision = args.no_tree_build = args.no_next_rows = args.no_scaling = True
 
     if args.tree_build_only:
         from kd_slam2_amd import Context, DeviceTree, TreeGenerator3D, default_params, synth
@@ -456,6 +463,9 @@ def main():
                        copy_write_kib=round(wr[1], 1), fetch_scale=round(f_scale, 4), write_scale=round(w_scale, 4))
             log(f"pmc: tile fetch {f[0]:.0f} KiB, write {wr[0]:.0f} KiB per launch (copy {f[1]:.0f}/{wr[1]:.0f} "
                 f"KiB for {kib:.0f} KiB)")
+
+    # ---- the same path at other batch sizes --------------------------------
+    scaling = None if args.no_scaling else run_batch_scaling(ctx, pairs3, args)
 
     # ---- CT-ICP: served batches, device loop, plain path ---------------------
     ct = None
@@ -529,6 +539,7 @@ def main():
         pcie_inclusive=head["pcie_inclusive"],
         accuracy=head["accuracy"],
         kernel_us_per_iter=head["kernel_us_per_iter"],
+        batch_scaling=scaling,
         pmc=pmc or None,
         ct_icp=ct,
         serving_mix=serving,
@@ -697,6 +708,55 @@ def run_cpu_baseline(pairs3, args, default_params):
                host=dict(cpu_model=model, logical_cpus=ncpu))
     log(f"cpu oracle: {v8 / 1e6:.2f} Mpts/s on 8 threads, {v1 / 1e6:.2f} on 1 ({model}, {ncpu} CPUs)")
     return cpu
+
+
+def run_batch_scaling(ctx, pairs3, args):
+    """Throughput of prepared batches of the first n headline pairs, and of
+    all 256 twice (every tree uploaded a second time: separate device
+    copies, so the 512 alignments share no tree), each at its own tuned
+    launch shape: where the GPU fills and the fixed cost of a launch (one
+    dependent gather chain, the launch itself) stops dominating."""
+    from kd_slam2_amd import DeviceTree, PairBatch, autotune, batch_align, default_params
+
+    p = _p3_params(default_params)
+    copies = [[(DeviceTree(ctx, q["fixed"]), DeviceTree(ctx, q["moving"])) for q in pairs3] for _ in range(2)]
+    every = [dict(fixed=F, moving=M, ct=0, X0=q["X0"], Xf=q["Xf"], params=p)
+             for trees in copies for q, (F, M) in zip(pairs3, trees)]
+    pts = [q["n_pts"] for q in pairs3] * 2
+    tiles = [(len(q["moving"].leaves) + 255) // 256 for q in pairs3] * 2
+    out = []
+    for n in (32, 64, 128, 256, 512):
+        if n > len(every):
+            continue
+        job = every[:n]
+        tuned, _ = autotune.tune(ctx, job, batch_align, ct=0)
+        B = PairBatch(job)
+        for _ in range(3):
+            B.run(ctx)
+        ts = []
+        for _ in range(args.scaling_steps):
+            t = time.perf_counter()
+            B.run(ctx)
+            ts.append(time.perf_counter() - t)
+        ctx.set_profiling(True)
+        B.run(ctx)
+        tu, fu = ctx.kernel_breakdown()
+        ctx.set_profiling(False)
+        ms = 1e3 * float(np.median(ts))
+        n_pts = int(sum(pts[:n]))
+        out.append(dict(pairs=n, scan_points=n_pts, tiles=int(sum(tiles[:n])), ms_per_batch=round(ms, 3),
+                        pts_per_s=round(n_pts / (ms * 1e-3), 1), tile_kernel_us=round(tu, 2),
+                        finish_kernel_us=round(fu, 2),
+                        tuned=dict(lds_nodes=tuned["lds_nodes"], leaves_per_lane=tuned["leaves_per_lane"],
+                                   tile_leaves=tuned["tile_leaves"], grid=tuned["grid"],
+                                   batch_streams=tuned["batch_streams"])))
+        log(f"batch scaling: {out[-1]}")
+    for trees in copies:
+        for F, M in trees:
+            F.close()
+            M.close()
+    ctx.set_launch_config(leaves_per_lane=1, batch_streams=1, tile_leaves=256)
+    return out
 
 
 def run_ct(ctx, w, F, M, args, CTICPHIP3D, adapter_align, batch_align, default_params, synth):

@@ -8,8 +8,9 @@
 // one alignment ("pair").  Per tile the workgroup
 //   1. descends the fixed tree once per leaf (one lane per leaf) with the top
 //      of the tree staged in LDS,
-//   2. evaluates the point-to-plane term (kdh_term.h) into a compact 16-value
-//      record per lane,
+//   2. evaluates the point-to-plane term (kdh_term.h) into a compact record
+//      per lane (Jacobians, residual, square root of the robust weight,
+//      outcome),
 //   3. forms the tile's normal equations on the matrix cores: every
 //      accumulator slot is an entry of C = sum_leaves w w^T over the leaf
 //      vectors w = [sqrt(g) J, sqrt(g) Jv, sqrt(g) e, e], one 16x16 C per
