@@ -542,13 +542,15 @@ struct kdh_batch {
 };
 
 // Workgroup of 64 pairs: their leaders' inputs staged through LDS (16-byte
-// loads from pinned host memory, consecutive lanes on consecutive bytes),
-// then lane l sets pair 64 b + l: its pose and velocities from its leader's
-// inputs, its counters reset, the FP32 cache of that state
-// (cache_from_state, the expressions of the host set-up).  Everything else in
-// the state is input that a run does not change.  Pairs and their leaders
-// may sit in different workgroups, so each lane reads its leader's row
-// directly when it lies outside the workgroup's range.
+// loads, consecutive lanes on consecutive bytes), then lane l sets pair
+// 64 b + l: its pose and velocities from its leader's inputs, its counters
+// reset, the FP32 cache of that state (cache_from_state, the expressions of
+// the host set-up).  Everything else in the state is input that a run does
+// not change.  The lane's own meta record and the state fields the cache
+// needs are loaded together with the inputs, so the common case is one
+// memory round trip.  Pairs and their leaders may sit in different
+// workgroups, so a lane reads its leader's row directly when it lies outside
+// the workgroup's range.
 constexpr int kInitPairs = 64;
 __global__ __launch_bounds__(kInitPairs) void k_batch_init(PairState* __restrict__ d,
                                                            const BatchMeta* __restrict__ meta,
@@ -556,22 +558,35 @@ __global__ __launch_bounds__(kInitPairs) void k_batch_init(PairState* __restrict
   __shared__ __align__(16) double2 s_in[kInitPairs * kBatchIn / 2];
   const int base = blockIdx.x * kInitPairs;
   const int rows = min(kInitPairs, n - base);
+  const int i = base + threadIdx.x;
+  const int ic = min(i, n - 1);  // lanes past the end load pair n - 1 and store nothing
   const double2* src = reinterpret_cast<const double2*>(in + (size_t)kBatchIn * base);
-  // every load in flight before the first LDS store: one PCIe round trip, not nine
+  // every load in flight before the first LDS store: the inputs, then this
+  // lane's meta record and the state fields the cache needs
   constexpr int kPer = kBatchIn / 2;  // 16-byte words per lane (kInitPairs rows of kBatchIn doubles)
   const int last = rows * kPer - 1;  // rows >= 1: the grid covers n pairs exactly
   double2 w[kPer];
 #pragma unroll
   for (int j = 0; j < kPer; ++j) w[j] = src[min((int)threadIdx.x + j * kInitPairs, last)];
+  const BatchMeta m = meta[ic];
+  PairState& ps = d[ic];
+  const int ct = ps.ct;
+  double Xf[12];
 #pragma unroll
-  for (int j = 0; j < kPer; ++j) {
-    const int k = threadIdx.x + j * kInitPairs;
-    if (k < rows * kPer) s_in[k] = w[j];
-  }
+  for (int k = 0; k < 12; ++k) Xf[k] = ps.Xf[k];
+  TermCache C = ps.cache;  // vf (dual form) is kept
+  const V3f mroot = ps.moving_root_mean;
+  const int recenter = ps.sp.recenter;
+  // the meta record arrives with the inputs: the compiler would otherwise sink
+  // its load below the barrier, where its value is first used
+  __asm__ volatile("" ::"v"(m.leader));
+  // Unconditional stores (the buffer holds kInitPairs rows; a row past `rows`
+  // receives a clamped copy that no lane reads): a store under a condition
+  // lets the compiler sink its load into the branch, one round trip each.
+#pragma unroll
+  for (int j = 0; j < kPer; ++j) s_in[threadIdx.x + j * kInitPairs] = w[j];
   __syncthreads();
-  const int i = base + threadIdx.x;
   if (i >= n) return;
-  const BatchMeta m = meta[i];
   const int r = m.leader - base;
   double x[kBatchIn];
   if (r >= 0 && r < rows) {
@@ -583,18 +598,12 @@ __global__ __launch_bounds__(kInitPairs) void k_batch_init(PairState* __restrict
 #pragma unroll
     for (int k = 0; k < kBatchIn; ++k) x[k] = gx[k];
   }
-  PairState& ps = d[i];
-  const int ct = ps.ct;
   if (!ct) {
 #pragma unroll
     for (int k = 12; k < kBatchIn; ++k) x[k] = 0.0;
   }
-  double Xf[12];
-#pragma unroll
-  for (int k = 0; k < 12; ++k) Xf[k] = ps.Xf[k];
-  TermCache C = ps.cache;  // vf (dual form) is kept
   double c64[3];
-  cache_from_pose(C, c64, x, Xf, x + 12, ps.moving_root_mean, ps.sp.recenter);
+  cache_from_pose(C, c64, x, Xf, x + 12, mroot, recenter);
 #pragma unroll
   for (int k = 0; k < 12; ++k) ps.X[k] = x[k];
 #pragma unroll
@@ -606,36 +615,55 @@ __global__ __launch_bounds__(kInitPairs) void k_batch_init(PairState* __restrict
   ps.prev_score = 0.0f;
 }
 
-// Workgroup i (64 lanes): pair i's results -- its leader's pose, velocities,
-// counters and last statistics, and its own share of the last round when it
-// is an entry of a forest (the FixedEntry::stats of icp_impl_.h:53-71: its
-// tiles' statistics slots summed in FP64 in tile order, as kdh_batch_align
-// sums them on the host) -- assembled in LDS and written to pinned host
-// memory as one contiguous row of 16-byte stores.
+// One workgroup per pair: the pair's result record (its leader's pose,
+// velocities, iteration count and status, the last round's statistics, the
+// entry's own share for a forest entry) staged in LDS, then written to pinned
+// host memory in 16-byte stores.  The common pair is its own leader and ran
+// every one of the batch's `iters` rounds, so its leader's fields and the
+// statistics of round iters - 1 are loaded together with its meta record (one
+// memory round trip); a forest entry, or a pair that stopped earlier, loads
+// what differs after that.
 __global__ __launch_bounds__(64) void k_batch_collect(const PairState* __restrict__ d,
                                                       const BatchMeta* __restrict__ meta,
-                                                      const float* __restrict__ partials, BatchOut* out, int n) {
+                                                      const float* __restrict__ partials, BatchOut* out, int n,
+                                                      int iters) {
   __shared__ __align__(16) BatchOut s;
   const int i = blockIdx.x, t = threadIdx.x;
   if (i >= n) return;
   const BatchMeta m = meta[i];
-  const PairState& L = d[m.leader];
-  if (t < 12) s.X[t] = L.X[t];
-  if (t < 6) s.v[t] = L.v[t];
+  // the pair as its own leader after all iters rounds: every load in flight at once
+  const int guess = min(max(iters, 1), kMaxIter) - 1;
+  const PairState* L = d + i;
+  double xv = t < 12 ? L->X[t] : (t < 18 ? L->v[t - 12] : 0.0);
+  int it = L->iter;
+  int status = L->status;
+  int group_tiles = L->group_tiles;
+  int group_n = L->group_n;
+  DevStats last = L->stats_log[guess];
+  if (m.leader != i) {  // a forest entry: its leader's fields
+    L = d + m.leader;
+    xv = t < 12 ? L->X[t] : (t < 18 ? L->v[t - 12] : 0.0);
+    it = L->iter;
+    status = L->status;
+    group_tiles = L->group_tiles;
+    group_n = L->group_n;
+    last = L->stats_log[guess];
+  }
+  if (it - 1 != guess && it > 0) last = L->stats_log[it - 1];  // stopped before the last round
+  if (t < 12) s.X[t] = xv;
+  else if (t < 18) s.v[t - 12] = xv;
   if (t == 0) {
-    const int it = L.iter;
     s.iter = it;
-    s.status = L.status;
+    s.status = status;
     s.pad[0] = s.pad[1] = 0;
-    DevStats last{};  // all zero for a forest without moving leaves (no round ran on the device)
-    if (it > 0 && L.group_tiles > 0) last = L.stats_log[it - 1];
+    if (!(it > 0 && group_tiles > 0)) last = DevStats{};  // no round ran on the device
     s.last = last;
     DevStats e = last;
-    if (L.group_n >= 2) {
+    if (group_n >= 2) {
       e.pert_pose_norm = e.pert_vel_norm = 0.0;
-      if (it > 0 && L.group_tiles > 0) {
+      if (it > 0 && group_tiles > 0) {
         const PairState& ps = d[i];
-        const int N = L.ct ? kCtSlots : kIcpSlots, CS = L.ct ? 90 : 27;
+        const int N = L->ct ? kCtSlots : kIcpSlots, CS = L->ct ? 90 : 27;
         double acc[7] = {0, 0, 0, 0, 0, 0, 0};
         for (int tt = 0; tt < ps.n_tiles; ++tt) {
           const float* row = partials + m.partial_base + (size_t)(ps.tile_first + tt) * N;
@@ -655,8 +683,9 @@ __global__ __launch_bounds__(64) void k_batch_collect(const PairState* __restric
   __syncthreads();
   constexpr int kWords = (int)(sizeof(BatchOut) / 16);
   static_assert(kWords <= 64, "one 16-byte store per lane");
+  // the host reads the records after synchronising with the stream; the
+  // kernel's completion makes the pinned-memory writes visible to it
   if (t < kWords) reinterpret_cast<float4*>(out + i)[t] = reinterpret_cast<const float4*>(&s)[t];
-  __threadfence_system();
 }
 
 static void batch_free(kdh_batch* b) {
@@ -764,7 +793,7 @@ int kdh_batch_run(kdh_batch* b, const double* X0, const double* v0, kdh_pair_res
     rc = enqueue_profiled(ctx, B, b->g, b->iters);
     if (rc) return rc;
     hipLaunchKernelGGL(k_batch_collect, dim3(n), dim3(64), 0, ctx->stream, b->d_pairs, b->d_meta, b->d_partials,
-                       b->h_out_dev, n);
+                       b->h_out_dev, n, b->iters);
     KDH_HIP_TRY(ctx, hipGetLastError());
     KDH_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     kdh_collect_times(ctx, b->iters, true);
@@ -787,7 +816,7 @@ int kdh_batch_run(kdh_batch* b, const double* X0, const double* v0, kdh_pair_res
       if (rc == KDH_OK) rc = enqueue_iterations(ctx, B, b->g, b->iters, two);
       if (rc == KDH_OK) {
         hipLaunchKernelGGL(k_batch_collect, dim3(n), dim3(64), 0, ctx->stream, b->d_pairs, b->d_meta, b->d_partials,
-                           b->h_out_dev, n);
+                           b->h_out_dev, n, b->iters);
         if (hipGetLastError() != hipSuccess) rc = kdh_fail(ctx, KDH_ERR_HIP, "k_batch_collect launch");
       }
       rc = end_capture(ctx, rc, &b->exec);

@@ -318,7 +318,8 @@ __global__ void k_pull_results(const PairState* __restrict__ d, PairState* h, in
   const double* src = reinterpret_cast<const double*>(s.stats_log);
   double* dst = reinterpret_cast<double*>(o.stats_log);
   for (int i = t; i < words; i += 64) dst[i] = src[i];
-  __threadfence_system();
+  // no fence of its own: the host reads the mirror after synchronising with
+  // the stream, and the kernel's completion makes these writes visible to it
 }
 
 int kdh_pull_results(kdh_ctx* ctx, int n_pairs) {
@@ -344,7 +345,7 @@ __global__ void k_pull_forms(const PairState* __restrict__ d, PairState* h, int 
   const double* src = reinterpret_cast<const double*>(&s.stats_log[0]);
   double* dst = reinterpret_cast<double*>(&o.stats_log[0]);
   for (int i = t; i < (int)(sizeof(DevStats) / 8); i += 64) dst[i] = src[i];
-  __threadfence_system();
+  // visible to the host after its stream synchronisation, as k_pull_results
 }
 
 int kdh_pull_forms(kdh_ctx* ctx, int n_pairs, int dual) {
