@@ -824,7 +824,9 @@ __global__ __launch_bounds__(TL / QPL) __attribute__((amdgpu_waves_per_eu(kTileW
     const TermCache C = ps.cache;
     const TermParams TP = ps.tp;
     if (stage) {
-      __syncthreads();
+      // the barrier before the stores keeps an earlier tile's readers of the
+      // staged nodes ahead of them; a workgroup's first tile has none
+      if (t != c_lo) __syncthreads();
 #pragma unroll
       for (int k = 0; k < kStage; ++k)
         if (tid + k * NT < 2 * n_st) ((lds_v4*)s_nodes)[tid + k * NT] = st[k];
@@ -1014,7 +1016,9 @@ __global__ __launch_bounds__(TL / QPL) __attribute__((amdgpu_waves_per_eu(kTileW
       }
       partials[(size_t)t * N + sl] = sv;
     }
-    __syncthreads();
+    // the next tile reuses the transpose buffers: its waves wait here for
+    // every reader of this tile's C tiles (no next tile, no wait)
+    if (t + 1 < c_hi) __syncthreads();
   }
 }
 
