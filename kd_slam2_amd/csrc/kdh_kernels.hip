@@ -706,11 +706,11 @@ __device__ void finish_pair(PairState& ps, const PairState& S, const float* __re
 // Resident waves per SIMD the tile kernel is compiled for (its register
 // budget is 512 / waves VGPRs): 8, the most a SIMD holds, for the rigid form
 // with one leaf per lane when its LDS allows 8 (at most 256 staged nodes,
-// 64 VGPRs); 4 for the CT form, for two rigid leaves per lane and for deeper
-// staging, 3 for the dual form and for two CT leaves per lane.
+// 64 VGPRs); 4 (128 VGPRs) for the CT and dual forms, for two leaves per lane
+// and for deeper staging.
 template <bool CT, bool DUAL, int QPL, int LDS_NODES>
 constexpr int kTileWaves() {
-  return (DUAL || (CT && QPL == 2)) ? 3 : ((CT || QPL == 2 || LDS_NODES > 256) ? 4 : 8);
+  return (CT || DUAL || QPL == 2 || LDS_NODES > 256) ? 4 : 8;
 }
 
 // Workgroup of the tile kernel: TL / QPL threads, each lane carrying QPL of

@@ -170,7 +170,6 @@ KDH_HD void descend2(const NodeFetch& fetch, V3f q0, V3f q1, bool live0, bool li
 // R = I + sin(a) S + (1 - cos(a)) S^2 with S = skew(w / |w|).
 struct Rodrigues {
   M3f R;
-  M3f S;
   V3f n;
   float angle;
   bool small;
@@ -196,34 +195,43 @@ KDH_HD Rodrigues rodrigues_f(V3f w) {
 #pragma unroll
   for (int i = 0; i < 9; ++i) {
     o.R.m[i] = small ? I.m[i] : (I.m[i] + s * S.m[i]) + omc * S2.m[i];
-    o.S.m[i] = small ? 0.0f : S.m[i];
   }
   o.n = small ? v3(0.0f, 0.0f, 0.0f) : n;
   return o;
 }
 
-// d(R(w) u)/dw (Gallego & Yezzi; geometry_3d_impl_.h:143-174), for the
-// velocity Jacobian only (tolerance path).
-KDH_HD M3f dRxp_f(const Rodrigues& rd, V3f u) {
-  if (rd.small) return skew(-u);  // -skew(u)
-  // -R * skew(u) * (n n^T + (R^T - I) S / angle)
-  const float nv[3] = {rd.n.x, rd.n.y, rd.n.z};
-  M3f nn;
-#pragma unroll
-  for (int r = 0; r < 3; ++r)
-#pragma unroll
-    for (int c = 0; c < 3; ++c) nn.m[r * 3 + c] = nv[r] * nv[c];
-  M3f RtmI = transpose(rd.R);
-  RtmI.m[0] -= 1.0f; RtmI.m[4] -= 1.0f; RtmI.m[8] -= 1.0f;
-  M3f B = mul(RtmI, rd.S);
-  float ia = 1.0f / rd.angle;
-#pragma unroll
-  for (int i = 0; i < 9; ++i) B.m[i] = nn.m[i] + B.m[i] * ia;
-  M3f A = mul(rd.R, skew(u));
-  M3f out = mul(A, B);
-#pragma unroll
-  for (int i = 0; i < 9; ++i) out.m[i] = -out.m[i];
-  return out;
+// Rotational velocity Jacobian (tolerance path: no decision reads it, so
+// multiply-adds may fuse).  d(R(w) u)/dw = -R [u]x B with
+// B = n n^T + (R^T - I) [n]x / angle (Gallego & Yezzi; the reference's dRxp,
+// geometry_3d_impl_.h:143-174), and B = I below 1e-9 rad, where R = I.  The
+// kernels need it only transposed and applied to a vector,
+// (d(R u)/dw)^T v = B^T (u x R^T v), two such products with one rotation at a
+// time, so this returns
+//   B^T (u1 x R^T v1 + u2 x R^T v2),
+// with B^T s = n (n . s) - n x (R s - s) / angle: no 3x3 product is formed.
+KDH_HD V3f vel_rot_jac(const Rodrigues& rd, V3f u1, V3f v1, V3f u2, V3f v2) {
+#pragma clang fp contract(fast)
+  const float* R = rd.R.m;
+  const float q1x = R[0] * v1.x + R[3] * v1.y + R[6] * v1.z;
+  const float q1y = R[1] * v1.x + R[4] * v1.y + R[7] * v1.z;
+  const float q1z = R[2] * v1.x + R[5] * v1.y + R[8] * v1.z;
+  const float q2x = R[0] * v2.x + R[3] * v2.y + R[6] * v2.z;
+  const float q2y = R[1] * v2.x + R[4] * v2.y + R[7] * v2.z;
+  const float q2z = R[2] * v2.x + R[5] * v2.y + R[8] * v2.z;
+  const float sx = (u1.y * q1z - u1.z * q1y) + (u2.y * q2z - u2.z * q2y);
+  const float sy = (u1.z * q1x - u1.x * q1z) + (u2.z * q2x - u2.x * q2z);
+  const float sz = (u1.x * q1y - u1.y * q1x) + (u2.x * q2y - u2.y * q2x);
+  // d = R s - s
+  const float dx = R[0] * sx + R[1] * sy + R[2] * sz - sx;
+  const float dy = R[3] * sx + R[4] * sy + R[5] * sz - sy;
+  const float dz = R[6] * sx + R[7] * sy + R[8] * sz - sz;
+  const V3f n = rd.n;
+  const float ia = 1.0f / (rd.small ? 1.0f : rd.angle);
+  const float ns = n.x * sx + n.y * sy + n.z * sz;
+  const float bx = n.x * ns - (n.y * dz - n.z * dy) * ia;
+  const float by = n.y * ns - (n.z * dx - n.x * dz) * ia;
+  const float bz = n.z * ns - (n.x * dy - n.y * dx) * ia;
+  return rd.small ? v3(sx, sy, sz) : v3(bx, by, bz);
 }
 
 }  // namespace kdh
@@ -422,9 +430,7 @@ KDH_HD void ct_eval_warped(const TermCache& C, const TermParams& P, const LeafIn
   // errorAndJacobianVelocityA (d3/cticp_impl_.h:11-39) at the unwarped leaf:
   //   J_t = dt n^T Ra ; J_w = dt (n^T Ra dR_p + dp^T Ra dR_n)
   const V3f nR = mul_t(Ra, n);
-  const M3f dRp = dRxp_f(rd, L.mean);
-  const M3f dRn = dRxp_f(rd, L.dir);
-  const V3f jw = mul_t(dRp, nR) + mul_t(dRn, dpR);
+  const V3f jw = vel_rot_jac(rd, L.mean, nR, L.dir, dpR);
   R.Jv[0] = nR.x * dt; R.Jv[1] = nR.y * dt; R.Jv[2] = nR.z * dt;
   R.Jv[3] = jw.x * dt; R.Jv[4] = jw.y * dt; R.Jv[5] = jw.z * dt;
   if constexpr (DUAL) {
@@ -444,9 +450,7 @@ KDH_HD void ct_eval_warped(const TermCache& C, const TermParams& P, const LeafIn
       nbR = mul_t(Rb, nb);
       dpbR = mul_t(Rb, dpb);
     }
-    const M3f dRpb = dRxp_f(rb, pf);
-    const M3f dRnb = dRxp_f(rb, nf);
-    const V3f jwb = mul_t(dRnb, dpbR) - mul_t(dRpb, nbR);
+    const V3f jwb = vel_rot_jac(rb, nf, dpbR, pf, -nbR);
     Jb[0] = -nbR.x * dtb; Jb[1] = -nbR.y * dtb; Jb[2] = -nbR.z * dtb;
     Jb[3] = jwb.x * dtb; Jb[4] = jwb.y * dtb; Jb[5] = jwb.z * dtb;
   }
