@@ -186,18 +186,49 @@ KDH_HD Rodrigues rodrigues_f(V3f w) {
   o.small = small;
   const float inv = 1.0f / (small ? 1.0f : o.angle);
   const V3f n = w * inv;
-  const M3f S = skew(n);
-  const M3f S2 = mul(S, S);
   float s, c;
   sincos_shared(o.angle, &s, &c);
   const float omc = 1.0f - c;
-  const M3f I = m3_identity<float>();
+  // (I + s S) + (1 - c) S^2 entry by entry, without the products with an
+  // exact zero of I or S that the 3x3 forms carry: each entry of S S is then
+  // one product n_i n_j off the diagonal and -(n_j^2 + n_k^2) on it, and
+  // I + s S adds exact zeros.  Leaving out an exact zero changes no value
+  // but the sign of a zero result, which no comparison, and no sum with a
+  // nonzero term, can see: the warp keeps its bits.
+  const float xx = n.x * n.x, yy = n.y * n.y, zz = n.z * n.z;
+  const float xy = n.x * n.y, xz = n.x * n.z, yz = n.y * n.z;
+  const float r[9] = {1.0f + omc * -(zz + yy), omc * xy - s * n.z,      s * n.y + omc * xz,
+                      s * n.z + omc * xy,      1.0f + omc * -(zz + xx), omc * yz - s * n.x,
+                      omc * xz - s * n.y,      s * n.x + omc * yz,      1.0f + omc * -(yy + xx)};
 #pragma unroll
-  for (int i = 0; i < 9; ++i) {
-    o.R.m[i] = small ? I.m[i] : (I.m[i] + s * S.m[i]) + omc * S2.m[i];
-  }
+  for (int i = 0; i < 9; ++i) o.R.m[i] = small ? ((i & 3) == 0 ? 1.0f : 0.0f) : r[i];
   o.n = small ? v3(0.0f, 0.0f, 0.0f) : n;
   return o;
+}
+
+// Tolerance-path forms of t0 + R p, R^T v, a x b and a . b whose
+// multiply-adds fuse: for the Jacobians and the re-centred residual, which no
+// decision reads.  The decision values use the unfused helpers in the
+// reference's order.  Contraction is per source expression (contract(on), not
+// fast), so an expression fuses the same way in every kernel it is inlined
+// into: the CT and dual builds keep equal bits on their shared terms.
+KDH_HD V3f fused_from(const M3f& R, V3f p, V3f t0) {
+#pragma clang fp contract(on)
+  return v3(t0.x + R.m[0] * p.x + R.m[1] * p.y + R.m[2] * p.z, t0.y + R.m[3] * p.x + R.m[4] * p.y + R.m[5] * p.z,
+            t0.z + R.m[6] * p.x + R.m[7] * p.y + R.m[8] * p.z);
+}
+KDH_HD V3f fused_mul_t(const M3f& R, V3f v) {
+#pragma clang fp contract(on)
+  return v3(R.m[0] * v.x + R.m[3] * v.y + R.m[6] * v.z, R.m[1] * v.x + R.m[4] * v.y + R.m[7] * v.z,
+            R.m[2] * v.x + R.m[5] * v.y + R.m[8] * v.z);
+}
+KDH_HD V3f fused_cross(V3f a, V3f b) {
+#pragma clang fp contract(on)
+  return v3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+KDH_HD float fused_dot(V3f a, V3f b) {
+#pragma clang fp contract(on)
+  return a.x * b.x + a.y * b.y + a.z * b.z;
 }
 
 // Rotational velocity Jacobian (tolerance path: no decision reads it, so
@@ -210,7 +241,7 @@ KDH_HD Rodrigues rodrigues_f(V3f w) {
 //   B^T (u1 x R^T v1 + u2 x R^T v2),
 // with B^T s = n (n . s) - n x (R s - s) / angle: no 3x3 product is formed.
 KDH_HD V3f vel_rot_jac(const Rodrigues& rd, V3f u1, V3f v1, V3f u2, V3f v2) {
-#pragma clang fp contract(fast)
+#pragma clang fp contract(on)
   const float* R = rd.R.m;
   const float q1x = R[0] * v1.x + R[3] * v1.y + R[6] * v1.z;
   const float q1y = R[1] * v1.x + R[4] * v1.y + R[7] * v1.z;
@@ -337,11 +368,11 @@ KDH_HD void icp_eval(const TermCache& C, const TermParams& P, const LeafIn& L, V
     return;
   }
   // Reference mode has mroot = cf = 0 and starts from the full translations,
-  // which makes jr exactly p_m x n_f + p_f x n_m.
+  // which makes jr p_m x n_f + p_f x n_m (up to the fused roundings).
   const V3f z = v3(0.0f, 0.0f, 0.0f);
-  const V3f jr = cross(apply_from(C.Xm, L.mean - C.mroot, P.recenter ? z : C.Xm.t), n_f) +
-                 cross(apply_from(C.Xf, pf - C.cf, P.recenter ? z : C.Xf.t), n_m);
-  const float e_loc = dot(p_mif - pf, apply_rot(C.Xmf, L.dir) + nf);
+  const V3f jr = fused_cross(fused_from(C.Xm.R, L.mean - C.mroot, P.recenter ? z : C.Xm.t), n_f) +
+                 fused_cross(fused_from(C.Xf.R, pf - C.cf, P.recenter ? z : C.Xf.t), n_m);
+  const float e_loc = fused_dot(p_mif - pf, fused_from(C.Xmf.R, L.dir, nf));
   R.e = P.recenter ? e_loc : e;
   R.J[0] = n.x; R.J[1] = n.y; R.J[2] = n.z;
   R.J[3] = jr.x; R.J[4] = jr.y; R.J[5] = jr.z;
@@ -419,17 +450,17 @@ KDH_HD void ct_eval_warped(const TermCache& C, const TermParams& P, const LeafIn
   // As in icp_term; re-centred, Ra^T dp = Rmf^T (p_mif - pf) as well.
   const M3f& Ra = C.Xm.R;
   const V3f z = v3(0.0f, 0.0f, 0.0f);
-  const V3f jr = cross(apply_from(C.Xm, wmean - C.mroot, P.recenter ? z : C.Xm.t), n_f) +
-                 cross(apply_from(C.Xf, pf - C.cf, P.recenter ? z : C.Xf.t), n_m);
+  const V3f jr = fused_cross(fused_from(C.Xm.R, wmean - C.mroot, P.recenter ? z : C.Xm.t), n_f) +
+                 fused_cross(fused_from(C.Xf.R, pf - C.cf, P.recenter ? z : C.Xf.t), n_m);
   const V3f dl = p_mif - pf;
-  const float e_loc = dot(dl, apply_rot(C.Xmf, wdir) + nf);
+  const float e_loc = fused_dot(dl, fused_from(C.Xmf.R, wdir, nf));
   R.e = P.recenter ? e_loc : e;
-  const V3f dpR = P.recenter ? mul_t(C.Xmf.R, dl) : mul_t(Ra, dp);
+  const V3f dpR = P.recenter ? fused_mul_t(C.Xmf.R, dl) : fused_mul_t(Ra, dp);
   R.J[0] = n.x; R.J[1] = n.y; R.J[2] = n.z;
   R.J[3] = jr.x; R.J[4] = jr.y; R.J[5] = jr.z;
   // errorAndJacobianVelocityA (d3/cticp_impl_.h:11-39) at the unwarped leaf:
   //   J_t = dt n^T Ra ; J_w = dt (n^T Ra dR_p + dp^T Ra dR_n)
-  const V3f nR = mul_t(Ra, n);
+  const V3f nR = fused_mul_t(Ra, n);
   const V3f jw = vel_rot_jac(rd, L.mean, nR, L.dir, dpR);
   R.Jv[0] = nR.x * dt; R.Jv[1] = nR.y * dt; R.Jv[2] = nR.z * dt;
   R.Jv[3] = jw.x * dt; R.Jv[4] = jw.y * dt; R.Jv[5] = jw.z * dt;
