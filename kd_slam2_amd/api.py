@@ -906,19 +906,24 @@ class PairBatch:
         self._prep = {}  # id(ctx) -> (ctx, kdh_batch handle)
 
     def set_initial(self, X0=None, v0=None):
-        """X0: (n, 4, 4) or (n, 3, 4) poses; v0: (n, 6) velocities."""
+        """X0: (n, 4, 4) or (n, 3, 4) poses; v0: (n, 6) velocities.  Every
+        run passes them to kdh_batch_run; the pair array takes them only when
+        a context prepares the batch."""
         if X0 is not None:
-            self._X0[:] = np.asarray(X0, dtype=np.float64)[:, :3, :4].reshape(self.n, 12)
-            self._A["X0"] = self._X0
+            X0 = np.asarray(X0)
+            if X0.shape != (self.n, 4, 4) and X0.shape != (self.n, 3, 4):
+                raise ValueError(f"X0 must be ({self.n}, 4, 4) or ({self.n}, 3, 4), got {X0.shape}")
+            self._X0.reshape(self.n, 3, 4)[:] = X0[:, :3, :4]  # one strided copy, converted to float64
         if v0 is not None:
             self._v0[:] = np.asarray(v0, dtype=np.float64).reshape(self.n, 6)
-            self._A["v0"] = self._v0
 
     def run(self, ctx):
         if self.n == 0:
             return BatchResult(self._R)
         e = self._prep.get(id(ctx))
         if e is None or e[0] is not ctx:
+            self._A["X0"] = self._X0
+            self._A["v0"] = self._v0
             h = C.c_void_p()
             ctx.check(L.lib().kdh_batch_prepare(ctx.h, self.n, self._arr, C.byref(h)), "kdh_batch_prepare")
             e = (ctx, h)
