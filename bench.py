@@ -533,6 +533,7 @@ def main():
                     pairs=head["pairs"], scan_points=head["scan_points"], moving_leaves=head["moving_leaves"],
                     fixed_nodes=head["fixed_nodes"], mean_depth=head["mean_depth"], tuned=head["tuned"]),
         latency_ms=head["latency_ms"],
+        one_at_a_time=head["one_at_a_time"],
         leaf_iterations_per_s=head["leaf_iterations_per_s"],
         served_speedup_vs_plain=head["speedup_vs_plain"],
         plain_pts_per_s=head["plain_pts_per_s"],
@@ -582,8 +583,23 @@ def run_headline(ctx, pairs3, args):
         steps_out.append(B.run(ctx))
         step_t.append(time.perf_counter() - t)
         kern.append(ctx.last_kernel_time()[0])
+    # the same K steps pipelined: step k + 1 is submitted before step k's
+    # results are collected (kdh_batch_submit / kdh_batch_collect, two runs
+    # in flight), so the device runs the batches back to back while the host
+    # prepares inputs and reads results; `value` is this whole-job rate
+    piped = []
+    t = time.perf_counter()
+    for k in range(args.steps):
+        B.set_initial(X0_sets[k % len(X0_sets)])
+        B.submit(ctx)
+        if k > 0:
+            piped.append(B.collect(ctx))
+    piped.append(B.collect(ctx))
+    total = time.perf_counter() - t
     gc.enable()
-    total = float(np.sum(step_t))
+    same_piped = all(np.array_equal(a.X, b.X) and np.array_equal(a.iterations, b.iterations)
+                     for a, b in zip(steps_out, piped))
+    seq_total = float(np.sum(step_t))
     value = n_pts * args.steps / total
     # moving-leaf iterations: every pair's leaves times the rounds it ran in each step
     leaves_per_pair = np.array([len(q["moving"].leaves) for q in pairs3], dtype=np.float64)
@@ -591,7 +607,8 @@ def run_headline(ctx, pairs3, args):
     # the first pose set once more, for the accuracy check
     B.set_initial(X0_sets[0])
     out = B.run(ctx)
-    log(f"headline: {1e3 * total / args.steps:.3f} ms/step, {value / 1e9:.2f} G pts/s")
+    log(f"headline: {1e3 * total / args.steps:.3f} ms/step pipelined, {value / 1e9:.2f} G pts/s; one at a time "
+        f"{1e3 * seq_total / args.steps:.3f} ms/step; pipelined results equal: {same_piped}")
     # kernel times (profiled: plain launches with events, tile kernel repeated)
     ctx.set_profiling(True)
     prof = []
@@ -661,6 +678,9 @@ def run_headline(ctx, pairs3, args):
         latency_ms=dict(step_p50=round(1e3 * float(np.percentile(step_t, 50)), 3),
                         step_p99=round(1e3 * float(np.percentile(step_t, 99)), 3),
                         plain_per_alignment_mean=round(1e3 * t_plain / len(pairs3), 3)),
+        one_at_a_time=dict(ms_per_step=round(1e3 * seq_total / args.steps, 3),
+                           pts_per_s=round(n_pts * args.steps / seq_total, 1),
+                           pipelined_results_equal=bool(same_piped)),
         plain_pts_per_s=round(n_pts / t_plain, 1), speedup_vs_plain=round(value / (n_pts / t_plain), 2),
         accuracy=dict(pose_vs_oracle_max_m_deg=[worst_t, worst_r], result_code_equal=same_code,
                       hb_rel_err_vs_fp64_max=[max(x[0] for x in hb), max(x[1] for x in hb)],
@@ -738,14 +758,24 @@ def run_batch_scaling(ctx, pairs3, args):
             t = time.perf_counter()
             B.run(ctx)
             ts.append(time.perf_counter() - t)
+        # pipelined, as the headline: the next run submitted before the
+        # previous one is collected
+        t = time.perf_counter()
+        for k in range(args.scaling_steps):
+            B.submit(ctx)
+            if k > 0:
+                B.collect(ctx)
+        B.collect(ctx)
+        piped_ms = 1e3 * (time.perf_counter() - t) / args.scaling_steps
         ctx.set_profiling(True)
         B.run(ctx)
         tu, fu = ctx.kernel_breakdown()
         ctx.set_profiling(False)
         ms = 1e3 * float(np.median(ts))
         n_pts = int(sum(pts[:n]))
-        out.append(dict(pairs=n, scan_points=n_pts, tiles=int(sum(tiles[:n])), ms_per_batch=round(ms, 3),
-                        pts_per_s=round(n_pts / (ms * 1e-3), 1), tile_kernel_us=round(tu, 2),
+        out.append(dict(pairs=n, scan_points=n_pts, tiles=int(sum(tiles[:n])), ms_per_batch=round(piped_ms, 3),
+                        pts_per_s=round(n_pts / (piped_ms * 1e-3), 1), one_at_a_time_ms=round(ms, 3),
+                        tile_kernel_us=round(tu, 2),
                         finish_kernel_us=round(fu, 2),
                         tuned=dict(lds_nodes=tuned["lds_nodes"], leaves_per_lane=tuned["leaves_per_lane"],
                                    tile_leaves=tuned["tile_leaves"], grid=tuned["grid"],

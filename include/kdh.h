@@ -287,6 +287,20 @@ int kdh_batch_prepare(kdh_ctx* ctx, int n_pairs, const kdh_pair* pairs, kdh_batc
 int kdh_batch_run(kdh_batch* batch, const double* X0, const double* v0, kdh_pair_result* results);
 int kdh_batch_release(kdh_batch* batch);
 
+/* kdh_batch_run in two halves, so that a caller keeps the device busy while
+ * it collects one run's results and prepares the next run's inputs: the
+ * reference has no counterpart (its verifier runs one candidate at a time);
+ * it is for serving loops over a prepared batch.  kdh_batch_submit enqueues
+ * one run from X0 / v0 (as kdh_batch_run) and returns at once; at most two
+ * runs are in flight, each with its own pinned inputs and results.
+ * kdh_batch_collect waits for the oldest submitted run and writes its
+ * results.  Runs execute in submission order on the context's stream, each
+ * from its own initial states, so each result equals kdh_batch_run's on the
+ * same inputs bit for bit.  kdh_batch_run is refused while runs are in
+ * flight, and kdh_batch_submit in profiling mode. */
+int kdh_batch_submit(kdh_batch* batch, const double* X0, const double* v0);
+int kdh_batch_collect(kdh_batch* batch, kdh_pair_result* results);
+
 /* One quadratic form per pair (or forest) at its X0 / v0 / Xf, all in one
  * tile launch and one finish launch: the per-factor builds of a bundle
  * adjustment batched (MultiViewICPFactor_::compute calls buildQuadraticForm

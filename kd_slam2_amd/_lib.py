@@ -266,6 +266,8 @@ _SIGS = {
     "kdh_batch_align": (C.c_int, [P, C.c_int, C.POINTER(Pair), C.POINTER(PairResult)]),
     "kdh_batch_prepare": (C.c_int, [P, C.c_int, C.POINTER(Pair), C.POINTER(P)]),
     "kdh_batch_run": (C.c_int, [P, F64P, F64P, C.POINTER(PairResult)]),
+    "kdh_batch_submit": (C.c_int, [P, F64P, F64P]),
+    "kdh_batch_collect": (C.c_int, [P, C.POINTER(PairResult)]),
     "kdh_batch_release": (C.c_int, [P]),
     "kdh_batch_build": (C.c_int, [P, C.c_int, C.POINTER(Pair), C.c_int, F64P, F64P, C.POINTER(Stats)]),
     "kdh_batch_build_dual": (C.c_int, [P, C.c_int, C.POINTER(DualPair), C.c_int, C.POINTER(DualForm)]),

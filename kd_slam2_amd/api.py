@@ -917,9 +917,7 @@ class PairBatch:
         if v0 is not None:
             self._v0[:] = np.asarray(v0, dtype=np.float64).reshape(self.n, 6)
 
-    def run(self, ctx):
-        if self.n == 0:
-            return BatchResult(self._R)
+    def _handle(self, ctx):
         e = self._prep.get(id(ctx))
         if e is None or e[0] is not ctx:
             self._A["X0"] = self._X0
@@ -929,8 +927,32 @@ class PairBatch:
             e = (ctx, h)
             self._prep[id(ctx)] = e
             ctx._batches.add(self)
-        rc = L.lib().kdh_batch_run(e[1], self._X0.ctypes.data_as(L.F64P), self._v0.ctypes.data_as(L.F64P), self._res)
+        return e[1]
+
+    def run(self, ctx):
+        if self.n == 0:
+            return BatchResult(self._R)
+        h = self._handle(ctx)
+        rc = L.lib().kdh_batch_run(h, self._X0.ctypes.data_as(L.F64P), self._v0.ctypes.data_as(L.F64P), self._res)
         ctx.check(rc, "kdh_batch_run")
+        return BatchResult(self._R)
+
+    def submit(self, ctx):
+        """Enqueue one run from the current initial states and return at once
+        (kdh_batch_submit).  At most two runs are in flight; collect() returns
+        their results in submission order, each equal to run()'s bit for bit."""
+        if self.n == 0:
+            return
+        h = self._handle(ctx)
+        rc = L.lib().kdh_batch_submit(h, self._X0.ctypes.data_as(L.F64P), self._v0.ctypes.data_as(L.F64P))
+        ctx.check(rc, "kdh_batch_submit")
+
+    def collect(self, ctx):
+        """Wait for the oldest submitted run and return its BatchResult
+        (kdh_batch_collect)."""
+        if self.n == 0:
+            return BatchResult(self._R)
+        ctx.check(L.lib().kdh_batch_collect(self._handle(ctx), self._res), "kdh_batch_collect")
         return BatchResult(self._R)
 
     def _release(self, ctx):

@@ -531,14 +531,24 @@ struct kdh_batch {
   float* d_partials = nullptr;
   TileDesc* d_tiles = nullptr;
   BatchMeta* d_meta = nullptr;
-  double* d_in = nullptr;   // the run inputs on the device (copied from h_in before each run)
   std::vector<BatchMeta> meta;
-  double* h_in = nullptr;   // pinned, mapped: kBatchIn doubles per pair
-  double* h_in_dev = nullptr;
-  BatchOut* h_out = nullptr;  // pinned, mapped
+  // Two run slots, so that one run can be enqueued while the previous one
+  // executes (kdh_batch_submit): pinned and device inputs (kBatchIn doubles
+  // per pair), pinned, mapped results (one BatchOut per pair) and a graph
+  // per slot (they differ in the input and result addresses).  A slot's
+  // inputs go up on the batch's copy stream, so the copy overlaps the run
+  // before it; the run waits for it through the slot's event.
+  double* d_in = nullptr;
+  double* h_in = nullptr;
+  hipStream_t copy = nullptr;
+  hipEvent_t in_ready[2] = {};
+  BatchOut* h_out = nullptr;
   BatchOut* h_out_dev = nullptr;
-  hipGraphExec_t exec = nullptr;
-  long long key[8] = {};  // the launch configuration the graph was built with
+  hipGraphExec_t exec[2] = {};
+  long long key[2][8] = {};  // the launch configuration each graph was built with
+  hipEvent_t ev[2][2] = {};  // per slot: before and after its run
+  int in_slot = 0;           // the slot holding the latest inputs
+  int first = 0, inflight = 0;  // submitted runs not yet collected: count, and the oldest one's slot
 };
 
 // Workgroup of 64 pairs: their leaders' inputs staged through LDS (16-byte
@@ -689,7 +699,13 @@ __global__ __launch_bounds__(64) void k_batch_collect(const PairState* __restric
 }
 
 static void batch_free(kdh_batch* b) {
-  if (b->exec) hipGraphExecDestroy(b->exec);
+  for (int k = 0; k < 2; ++k) {
+    if (b->exec[k]) hipGraphExecDestroy(b->exec[k]);
+    for (int j = 0; j < 2; ++j)
+      if (b->ev[k][j]) hipEventDestroy(b->ev[k][j]);
+    if (b->in_ready[k]) hipEventDestroy(b->in_ready[k]);
+  }
+  if (b->copy) hipStreamDestroy(b->copy);
   if (b->d_pairs) hipFree(b->d_pairs);
   if (b->d_partials) hipFree(b->d_partials);
   if (b->d_tiles) hipFree(b->d_tiles);
@@ -738,13 +754,19 @@ int kdh_batch_prepare(kdh_ctx* ctx, int n_pairs, const kdh_pair* pairs, kdh_batc
   if (e == hipSuccess) e = hipMalloc((void**)&b->d_partials, sizeof(float) * std::max<size_t>(n_partials, 1));
   if (e == hipSuccess) e = hipMalloc((void**)&b->d_tiles, sizeof(TileDesc) * tp.size());
   if (e == hipSuccess) e = hipMalloc((void**)&b->d_meta, sizeof(BatchMeta) * (size_t)n_pairs);
-  // coherent: the device reads the inputs and writes the results in place
+  // two slots each; coherent: the device writes the results in place
   if (e == hipSuccess)
-    e = hipHostMalloc((void**)&b->h_in, sizeof(double) * kBatchIn * (size_t)n_pairs, hipHostMallocCoherent);
-  if (e == hipSuccess) e = hipHostGetDevicePointer((void**)&b->h_in_dev, b->h_in, 0);
-  if (e == hipSuccess) e = hipMalloc((void**)&b->d_in, sizeof(double) * kBatchIn * (size_t)n_pairs);
-  if (e == hipSuccess) e = hipHostMalloc((void**)&b->h_out, sizeof(BatchOut) * (size_t)n_pairs, hipHostMallocCoherent);
+    e = hipHostMalloc((void**)&b->h_in, 2 * sizeof(double) * kBatchIn * (size_t)n_pairs, hipHostMallocCoherent);
+  if (e == hipSuccess) e = hipMalloc((void**)&b->d_in, 2 * sizeof(double) * kBatchIn * (size_t)n_pairs);
+  if (e == hipSuccess)
+    e = hipHostMalloc((void**)&b->h_out, 2 * sizeof(BatchOut) * (size_t)n_pairs, hipHostMallocCoherent);
   if (e == hipSuccess) e = hipHostGetDevicePointer((void**)&b->h_out_dev, b->h_out, 0);
+  for (int k = 0; k < 2; ++k) {
+    for (int j = 0; j < 2; ++j)
+      if (e == hipSuccess) e = hipEventCreate(&b->ev[k][j]);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&b->in_ready[k], hipEventDisableTiming);
+  }
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->copy, hipStreamNonBlocking);
   if (e == hipSuccess)
     e = hipMemcpyAsync(b->d_pairs, hs.data(), sizeof(PairState) * (size_t)n_pairs, hipMemcpyHostToDevice, ctx->stream);
   if (e == hipSuccess)
@@ -761,7 +783,115 @@ int kdh_batch_prepare(kdh_ctx* ctx, int n_pairs, const kdh_pair* pairs, kdh_batc
     memcpy(b->h_in + (size_t)kBatchIn * i, pairs[i].X0, sizeof(double) * 12);
     memcpy(b->h_in + (size_t)kBatchIn * i + 12, pairs[i].v0, sizeof(double) * 6);
   }
+  b->in_slot = 0;
   *out = b;
+  return KDH_OK;
+}
+
+// Enqueues one run from slot `slot` on the context's stream: the latest
+// inputs, with X0 / v0 applied, into the slot's pinned inputs, up to the
+// device, then the slot's graph (captured on first use and again after a
+// launch configuration change) between the slot's two events.  Called with
+// ctx->mu held and the device set.
+static int batch_enqueue(kdh_batch* b, int slot, const double* X0, const double* v0, bool overlap) {
+  kdh_ctx* ctx = b->ctx;
+  const int n = b->n;
+  const size_t in_doubles = (size_t)kBatchIn * n;
+  double* in = b->h_in + in_doubles * slot;
+  if (slot != b->in_slot) memcpy(in, b->h_in + in_doubles * b->in_slot, sizeof(double) * in_doubles);
+  b->in_slot = slot;
+  if (X0)
+    for (int i = 0; i < n; ++i) memcpy(in + (size_t)kBatchIn * i, X0 + (size_t)12 * i, sizeof(double) * 12);
+  if (v0)
+    for (int i = 0; i < n; ++i) memcpy(in + (size_t)kBatchIn * i + 12, v0 + (size_t)6 * i, sizeof(double) * 6);
+  // the inputs go up by DMA ahead of the graph (a kernel reading them from
+  // pinned memory across PCIe took longer than the copy); with `overlap` on
+  // the copy stream, so that the copy runs beside the run in flight before
+  // this one (with nothing in flight the wait between streams only adds
+  // latency, and the copy goes on the run's stream)
+  double* d_in = b->d_in + in_doubles * slot;
+  if (overlap) {
+    KDH_HIP_TRY(ctx, hipMemcpyAsync(d_in, in, sizeof(double) * in_doubles, hipMemcpyHostToDevice, b->copy));
+    KDH_HIP_TRY(ctx, hipEventRecord(b->in_ready[slot], b->copy));
+    KDH_HIP_TRY(ctx, hipStreamWaitEvent(ctx->stream, b->in_ready[slot], 0));
+  } else {
+    KDH_HIP_TRY(ctx, hipMemcpyAsync(d_in, in, sizeof(double) * in_doubles, hipMemcpyHostToDevice, ctx->stream));
+  }
+  BatchOut* out_dev = b->h_out_dev + (size_t)n * slot;
+  const BatchBufs B{b->d_pairs, b->d_tiles, b->d_partials};
+  const dim3 init_grid((n + kInitPairs - 1) / kInitPairs);
+  const bool two = b->two && ctx->batch_streams == 2;
+  const long long key[8] = {ctx->lds_nodes, ctx->grid_override, ctx->xcd_remap, ctx->leaves_per_lane,
+                            two ? 2 : 1, (long long)(intptr_t)ctx->stream, ctx->tile_leaves, 0};
+  hipGraphExec_t& exec = b->exec[slot];
+  if (exec && memcmp(key, b->key[slot], sizeof(key)) != 0) {
+    hipGraphExecDestroy(exec);
+    exec = nullptr;
+  }
+  if (!exec) {
+    int rc = KDH_OK;
+    if (two) {
+      rc = ensure_stream2(ctx);
+      if (rc) return rc;
+    }
+    KDH_HIP_TRY(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    hipLaunchKernelGGL(k_batch_init, init_grid, dim3(kInitPairs), 0, ctx->stream, b->d_pairs, b->d_meta, d_in, n);
+    rc = hipGetLastError() == hipSuccess ? KDH_OK : kdh_fail(ctx, KDH_ERR_HIP, "k_batch_init launch");
+    if (rc == KDH_OK) rc = enqueue_iterations(ctx, B, b->g, b->iters, two);
+    if (rc == KDH_OK) {
+      hipLaunchKernelGGL(k_batch_collect, dim3(n), dim3(64), 0, ctx->stream, b->d_pairs, b->d_meta, b->d_partials,
+                         out_dev, n, b->iters);
+      if (hipGetLastError() != hipSuccess) rc = kdh_fail(ctx, KDH_ERR_HIP, "k_batch_collect launch");
+    }
+    rc = end_capture(ctx, rc, &exec);
+    if (rc) return rc;
+    memcpy(b->key[slot], key, sizeof(key));
+  }
+  hipError_t le = hipEventRecord(b->ev[slot][0], ctx->stream);
+  if (le == hipSuccess) le = hipGraphLaunch(exec, ctx->stream);
+  if (le == hipSuccess) le = hipEventRecord(b->ev[slot][1], ctx->stream);
+  if (le != hipSuccess) return kdh_hip_fail(ctx, le, "kdh_batch_run");
+  return KDH_OK;
+}
+
+// The results of the run of slot `slot` (complete) into `results`.
+static void copy_results(const kdh_batch* b, int slot, kdh_pair_result* results) {
+  static_assert(sizeof(kdh_stats) == sizeof(DevStats), "kdh_stats mirrors DevStats");
+  const BatchOut* out = b->h_out + (size_t)b->n * slot;
+  for (int i = 0; i < b->n; ++i) {
+    const BatchOut& o = out[i];
+    kdh_pair_result& r = results[i];
+    memcpy(r.X, o.X, sizeof(r.X));
+    memcpy(r.v, o.v, sizeof(r.v));
+    memcpy(&r.last, &o.last, sizeof(r.last));
+    memcpy(&r.entry, &o.entry, sizeof(r.entry));
+    r.iterations = o.iter;
+    r.result = kdh_result_code_of(o.status, o.iter, b->meta[i].max_iterations);
+  }
+}
+
+// Waits for the run of slot `slot`, records its device time per iteration
+// in the context, and copies its results out.  Called with ctx->mu held and
+// the device set.
+static int batch_results(kdh_batch* b, int slot, kdh_pair_result* results) {
+  kdh_ctx* ctx = b->ctx;
+  hipError_t le = hipEventSynchronize(b->ev[slot][1]);
+  if (le != hipSuccess) return kdh_hip_fail(ctx, le, "kdh_batch_run");
+  float ms = 0.0f;
+  hipEventElapsedTime(&ms, b->ev[slot][0], b->ev[slot][1]);
+  ctx->last_kernel_us = b->iters ? 1000.0 * ms / b->iters : 0.0;
+  ctx->last_tile_us = ctx->last_finish_us = 0.0;
+  ctx->last_launches = b->iters;
+  copy_results(b, slot, results);
+  return KDH_OK;
+}
+
+// Checks shared by the run entry points.  Called with ctx->mu held.
+static int batch_check(kdh_batch* b, const char* who) {
+  kdh_ctx* ctx = b->ctx;
+  if (ctx->tile_leaves != b->tile_leaves)
+    return kdh_fail(ctx, KDH_ERR_INVALID,
+                    std::string(who) + ": the context's tile size changed since kdh_batch_prepare; prepare the batch again");
   return KDH_OK;
 }
 
@@ -771,81 +901,70 @@ int kdh_batch_run(kdh_batch* b, const double* X0, const double* v0, kdh_pair_res
   if (!results) return kdh_fail(ctx, KDH_ERR_INVALID, "bad arguments");
   std::lock_guard<std::mutex> lk(ctx->mu);
   KDH_HIP_TRY(ctx, hipSetDevice(ctx->device));
-  if (ctx->tile_leaves != b->tile_leaves)
-    return kdh_fail(ctx, KDH_ERR_INVALID,
-                    "kdh_batch_run: the context's tile size changed since kdh_batch_prepare; prepare the batch again");
+  int rc = batch_check(b, "kdh_batch_run");
+  if (rc) return rc;
+  if (b->inflight)
+    return kdh_fail(ctx, KDH_ERR_INVALID, "kdh_batch_run: collect the submitted runs first (kdh_batch_collect)");
+  if (!ctx->profile) {
+    const int slot = b->first;
+    rc = batch_enqueue(b, slot, X0, v0, false);
+    return rc ? rc : batch_results(b, slot, results);
+  }
+  // profiling: plain launches with events around each kernel phase, on one
+  // stream, from slot 0
   const int n = b->n;
+  const size_t in_doubles = (size_t)kBatchIn * n;
+  double* in = b->h_in;
+  if (b->in_slot != 0) memcpy(in, b->h_in + in_doubles * b->in_slot, sizeof(double) * in_doubles);
+  b->in_slot = 0;
   if (X0)
-    for (int i = 0; i < n; ++i) memcpy(b->h_in + (size_t)kBatchIn * i, X0 + (size_t)12 * i, sizeof(double) * 12);
+    for (int i = 0; i < n; ++i) memcpy(in + (size_t)kBatchIn * i, X0 + (size_t)12 * i, sizeof(double) * 12);
   if (v0)
-    for (int i = 0; i < n; ++i) memcpy(b->h_in + (size_t)kBatchIn * i + 12, v0 + (size_t)6 * i, sizeof(double) * 6);
-  // the inputs go up by DMA ahead of the graph (a kernel reading them from
-  // pinned memory across PCIe took longer than the copy)
-  KDH_HIP_TRY(ctx, hipMemcpyAsync(b->d_in, b->h_in, sizeof(double) * kBatchIn * (size_t)n, hipMemcpyHostToDevice,
-                                  ctx->stream));
+    for (int i = 0; i < n; ++i) memcpy(in + (size_t)kBatchIn * i + 12, v0 + (size_t)6 * i, sizeof(double) * 6);
+  KDH_HIP_TRY(ctx, hipMemcpyAsync(b->d_in, in, sizeof(double) * in_doubles, hipMemcpyHostToDevice, ctx->stream));
   const BatchBufs B{b->d_pairs, b->d_tiles, b->d_partials};
-  const dim3 init_grid((n + kInitPairs - 1) / kInitPairs);
-  int rc = KDH_OK;
-  if (ctx->profile) {
-    // plain launches with events around each kernel phase, on one stream
-    hipLaunchKernelGGL(k_batch_init, init_grid, dim3(kInitPairs), 0, ctx->stream, b->d_pairs, b->d_meta, b->d_in, n);
-    KDH_HIP_TRY(ctx, hipGetLastError());
-    rc = enqueue_profiled(ctx, B, b->g, b->iters);
-    if (rc) return rc;
-    hipLaunchKernelGGL(k_batch_collect, dim3(n), dim3(64), 0, ctx->stream, b->d_pairs, b->d_meta, b->d_partials,
-                       b->h_out_dev, n, b->iters);
-    KDH_HIP_TRY(ctx, hipGetLastError());
-    KDH_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-    kdh_collect_times(ctx, b->iters, true);
-  } else {
-    const bool two = b->two && ctx->batch_streams == 2;
-    const long long key[8] = {ctx->lds_nodes, ctx->grid_override, ctx->xcd_remap, ctx->leaves_per_lane,
-                              two ? 2 : 1, (long long)(intptr_t)ctx->stream, ctx->tile_leaves, 0};
-    if (b->exec && memcmp(key, b->key, sizeof(key)) != 0) {
-      hipGraphExecDestroy(b->exec);
-      b->exec = nullptr;
-    }
-    if (!b->exec) {
-      if (two) {
-        rc = ensure_stream2(ctx);
-        if (rc) return rc;
-      }
-      KDH_HIP_TRY(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-      hipLaunchKernelGGL(k_batch_init, init_grid, dim3(kInitPairs), 0, ctx->stream, b->d_pairs, b->d_meta, b->d_in, n);
-      rc = hipGetLastError() == hipSuccess ? KDH_OK : kdh_fail(ctx, KDH_ERR_HIP, "k_batch_init launch");
-      if (rc == KDH_OK) rc = enqueue_iterations(ctx, B, b->g, b->iters, two);
-      if (rc == KDH_OK) {
-        hipLaunchKernelGGL(k_batch_collect, dim3(n), dim3(64), 0, ctx->stream, b->d_pairs, b->d_meta, b->d_partials,
-                           b->h_out_dev, n, b->iters);
-        if (hipGetLastError() != hipSuccess) rc = kdh_fail(ctx, KDH_ERR_HIP, "k_batch_collect launch");
-      }
-      rc = end_capture(ctx, rc, &b->exec);
-      if (rc) return rc;
-      memcpy(b->key, key, sizeof(key));
-    }
-    hipError_t le = hipEventRecord(ctx->ev[0], ctx->stream);
-    if (le == hipSuccess) le = hipGraphLaunch(b->exec, ctx->stream);
-    if (le == hipSuccess) le = hipEventRecord(ctx->ev[1], ctx->stream);
-    if (le == hipSuccess) le = hipStreamSynchronize(ctx->stream);
-    if (le != hipSuccess) return kdh_hip_fail(ctx, le, "kdh_batch_run");
-    float ms = 0.0f;
-    hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]);
-    ctx->last_kernel_us = b->iters ? 1000.0 * ms / b->iters : 0.0;
-    ctx->last_tile_us = ctx->last_finish_us = 0.0;
-    ctx->last_launches = b->iters;
-  }
-  static_assert(sizeof(kdh_stats) == sizeof(DevStats), "kdh_stats mirrors DevStats");
-  for (int i = 0; i < n; ++i) {
-    const BatchOut& o = b->h_out[i];
-    kdh_pair_result& r = results[i];
-    memcpy(r.X, o.X, sizeof(r.X));
-    memcpy(r.v, o.v, sizeof(r.v));
-    memcpy(&r.last, &o.last, sizeof(r.last));
-    memcpy(&r.entry, &o.entry, sizeof(r.entry));
-    r.iterations = o.iter;
-    r.result = kdh_result_code_of(o.status, o.iter, b->meta[i].max_iterations);
-  }
+  hipLaunchKernelGGL(k_batch_init, dim3((n + kInitPairs - 1) / kInitPairs), dim3(kInitPairs), 0, ctx->stream,
+                     b->d_pairs, b->d_meta, b->d_in, n);
+  KDH_HIP_TRY(ctx, hipGetLastError());
+  rc = enqueue_profiled(ctx, B, b->g, b->iters);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_batch_collect, dim3(n), dim3(64), 0, ctx->stream, b->d_pairs, b->d_meta, b->d_partials,
+                     b->h_out_dev, n, b->iters);
+  KDH_HIP_TRY(ctx, hipGetLastError());
+  KDH_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  kdh_collect_times(ctx, b->iters, true);
+  copy_results(b, 0, results);
   return KDH_OK;
+}
+
+int kdh_batch_submit(kdh_batch* b, const double* X0, const double* v0) {
+  if (!b) return KDH_ERR_INVALID;
+  kdh_ctx* ctx = b->ctx;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  KDH_HIP_TRY(ctx, hipSetDevice(ctx->device));
+  int rc = batch_check(b, "kdh_batch_submit");
+  if (rc) return rc;
+  if (ctx->profile) return kdh_fail(ctx, KDH_ERR_INVALID, "kdh_batch_submit: not in profiling mode (kdh_batch_run)");
+  if (b->inflight == 2)
+    return kdh_fail(ctx, KDH_ERR_INVALID, "kdh_batch_submit: two runs in flight; collect one first (kdh_batch_collect)");
+  const int slot = (b->first + b->inflight) & 1;
+  rc = batch_enqueue(b, slot, X0, v0, b->inflight > 0);
+  if (rc) return rc;
+  b->inflight += 1;
+  return KDH_OK;
+}
+
+int kdh_batch_collect(kdh_batch* b, kdh_pair_result* results) {
+  if (!b) return KDH_ERR_INVALID;
+  kdh_ctx* ctx = b->ctx;
+  if (!results) return kdh_fail(ctx, KDH_ERR_INVALID, "bad arguments");
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  KDH_HIP_TRY(ctx, hipSetDevice(ctx->device));
+  if (b->inflight == 0) return kdh_fail(ctx, KDH_ERR_INVALID, "kdh_batch_collect: no run submitted");
+  const int slot = b->first;
+  b->first ^= 1;
+  b->inflight -= 1;
+  return batch_results(b, slot, results);
 }
 
 int kdh_batch_release(kdh_batch* b) {

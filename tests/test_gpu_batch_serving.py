@@ -217,6 +217,56 @@ def _same(a, b):
             and a["entry"] == b["entry"] and a["result"] == b["result"] and a["iterations"] == b["iterations"])
 
 
+def test_submitted_runs_equal_run(ctx, small_pairs):
+    """kdh_batch_submit / kdh_batch_collect (two runs in flight, each with
+    its own pinned inputs and results) give every run the bits kdh_batch_run
+    gives it: four runs of rigid and CT pairs from different initial states,
+    submitted one ahead.  The entry points refuse a third run in flight, a
+    collect with none in flight, and kdh_batch_run while runs are in flight;
+    a run after them starts from the latest inputs."""
+    from kd_slam2_amd import PairBatch
+
+    trees = [(DeviceTree(ctx, w["fixed"]), DeviceTree(ctx, w["moving"])) for w in small_pairs]
+    pairs = _pairs(ctx, small_pairs, trees)
+    rng = np.random.default_rng(17)
+    sets = [(np.stack([synth.perturb(q["X0"], rng, 1.0, 0.2) for q in pairs]),
+             np.stack([np.asarray(q["v0"]) + rng.normal(0, 0.05, 6) for q in pairs])) for _ in range(4)]
+    B = PairBatch(pairs)
+    try:
+        ref = []
+        for X0, v0 in sets:
+            B.set_initial(X0, v0)
+            ref.append(list(B.run(ctx)))
+        assert not np.array_equal(ref[0][0]["X"], ref[1][0]["X"])
+        got = []
+        B.set_initial(*sets[0])
+        B.submit(ctx)
+        B.set_initial(*sets[1])
+        B.submit(ctx)
+        with pytest.raises(RuntimeError, match="two runs in flight"):
+            B.submit(ctx)
+        with pytest.raises(RuntimeError, match="collect the submitted runs first"):
+            B.run(ctx)
+        got.append(list(B.collect(ctx)))
+        B.set_initial(*sets[2])
+        B.submit(ctx)
+        got.append(list(B.collect(ctx)))
+        B.set_initial(*sets[3])
+        B.submit(ctx)
+        got.append(list(B.collect(ctx)))
+        got.append(list(B.collect(ctx)))
+        with pytest.raises(RuntimeError, match="no run submitted"):
+            B.collect(ctx)
+        for k in range(4):
+            assert all(_same(a, b) for a, b in zip(ref[k], got[k])), k
+        assert all(_same(a, b) for a, b in zip(ref[3], B.run(ctx)))
+    finally:
+        B.close()
+        for F, M in trees:
+            F.close()
+            M.close()
+
+
 def test_prepared_batch_equals_batch_align(ctx, small_pairs):
     """kdh_batch_prepare / kdh_batch_run (PairBatch): the same bits as
     kdh_batch_align on a mix of rigid pairs, CT pairs, a fixed forest of
